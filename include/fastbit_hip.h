@@ -1,0 +1,122 @@
+/*
+ * fastbit_hip.h -- C ABI of libfastbit_hip.so, the MI355X (gfx950) engine for
+ * FastBit's query hot path: WAH bitvector logic, relic/bin range evaluation
+ * and index build.
+ *
+ * Conventions (mirroring src/iapi.h:17-22 of FastBit 2.0.3):
+ *   - every function returns an int status, 0 on success, < 0 on error;
+ *     no C++ exception crosses this boundary;  fbh_last_error() describes
+ *     the last failure of the calling thread;
+ *   - handles are opaque; a handle is not shared between host threads
+ *     without an external lock (as ibis::bitvector);
+ *   - arrays passed in are host memory unless the name says "dev";
+ *   - bitvectors cross the boundary in the reference's serialized form
+ *     (src/bitvector.cpp:2283-2313): the code words, then active.val when
+ *     active.nbits > 0, then active.nbits.
+ */
+#ifndef FASTBIT_HIP_H
+#define FASTBIT_HIP_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FBH_OK 0
+#define FBH_ERR_ARG (-1)
+#define FBH_ERR_DEVICE (-2)
+#define FBH_ERR_NOMEM (-3)
+#define FBH_ERR_INTERNAL (-4)
+#define FBH_ERR_SIZE (-5)
+#define FBH_ERR_NOTFOUND (-6)
+
+/* Logical operators (src/bitvector.h:98-113). */
+#define FBH_AND 0
+#define FBH_OR 1
+#define FBH_XOR 2
+#define FBH_MINUS 3
+
+/* The reference routine a binary operator dispatched to
+ * (src/bitvector.cpp:1219-1272, 1438-1471, 1582-1606, 1710-1750). */
+#define FBH_PATH_C0 1
+#define FBH_PATH_D1 2
+#define FBH_PATH_D1SWAP 3
+#define FBH_PATH_C1 4
+#define FBH_PATH_D2 5
+#define FBH_PATH_C2 6
+#define FBH_PATH_SHORTCUT 7
+
+/* ---------------------------------------------------------------------- */
+/* Library */
+
+/* Select the HIP device and create the library stream.  Optional: every
+ * other call initialises device 0 on first use.  (ibis::init,
+ * src/ibis.h:378-430, plays this role in the reference.) */
+int fbh_init(int device);
+const char* fbh_last_error(void);
+/* Wait for all work queued by the library. */
+int fbh_sync(void);
+
+/* ---------------------------------------------------------------------- */
+/* Bitvectors: ibis::bitvector (src/bitvector.h:62-318) held in HBM. */
+
+typedef struct fbh_bitvector fbh_bitvector;
+
+/* ibis::bitvector(const array_t<word_t>&) -- src/bitvector.cpp:54-105.
+ * Copies the serialized array to the device. */
+int fbh_bv_from_serial(const uint32_t* arr, uint32_t n, fbh_bitvector** out);
+/* Empty bitvector (src/bitvector.cpp:37). */
+int fbh_bv_new(fbh_bitvector** out);
+/* ibis::bitvector::set(val, n) -- src/bitvector.cpp:256-276. */
+int fbh_bv_set(fbh_bitvector* bv, int val, uint32_t n);
+/* Deep copy (src/bitvector.h:680-684). */
+int fbh_bv_copy(const fbh_bitvector* src, fbh_bitvector** out);
+void fbh_bv_free(fbh_bitvector* bv);
+
+/* write(array_t&) -- src/bitvector.cpp:2283-2313.  *n receives the
+ * serialized length; with arr == NULL only the length is returned. */
+int fbh_bv_write(fbh_bitvector* bv, uint32_t* arr, uint32_t cap, uint32_t* n);
+
+/* In-place operators followed by compress():
+ *   a &= b; a.compress();   (src/bitvector.cpp:1194-1284, 378-469)
+ *   a |= b; a.compress();   (1413-1483)
+ *   a ^= b; a.compress();   (1557-1618)
+ *   a -= b; a.compress();   (1685-1762)
+ * The routine chosen by the reference's dispatch decides the result words;
+ * *path (may be NULL) receives it (FBH_PATH_*). */
+int fbh_bv_op(fbh_bitvector* a, const fbh_bitvector* b, int op, int* path);
+/* Out-of-place form: *out = new bitvector equal to (copy of a) op= b,
+ * compressed.  The caller frees *out (as operator& returns a new
+ * bitvector that the caller deletes, src/bitvector.cpp:1286-1291). */
+int fbh_bv_op_new(const fbh_bitvector* a, const fbh_bitvector* b, int op,
+                  fbh_bitvector** out, int* path);
+
+int fbh_bv_compress(fbh_bitvector* bv);   /* src/bitvector.cpp:378-469 */
+int fbh_bv_decompress(fbh_bitvector* bv); /* src/bitvector.cpp:474-512 */
+int fbh_bv_flip(fbh_bitvector* bv);       /* src/bitvector.cpp:1128-1174 */
+/* cnt() and size() -- src/bitvector.h:647-656, do_cnt at
+ * src/bitvector.cpp:583-608 (an all-zero vector of more than one word is
+ * collapsed to one 0-fill, as the reference does). */
+int fbh_bv_cnt(fbh_bitvector* bv, uint32_t* cnt);
+int fbh_bv_size(fbh_bitvector* bv, uint32_t* size);
+/* adjustSize(nv, nt) growth -- src/bitvector.cpp:4086-4113. */
+int fbh_bv_adjust_size(fbh_bitvector* bv, uint32_t nv, uint32_t nt);
+/* Ascending positions of the set bits (the indexSet walk,
+ * src/bitvector.cpp:4524-4590).  *n receives the count; rids may be NULL
+ * to query it; cap bounds what is written. */
+int fbh_bv_rids(fbh_bitvector* bv, uint32_t* rids, uint32_t cap, uint32_t* n);
+/* Number of code words in m_vec (m_vec.size()). */
+int fbh_bv_nwords(const fbh_bitvector* bv, uint32_t* n);
+
+/* Benchmark helper: runs (copy of a) op= b + compress `reps` times on the
+ * device with inputs resident, without host synchronisation between reps,
+ * and returns the mean wall time per op (ms, HIP events) and the result's
+ * word count. */
+int fbh_bench_op(const fbh_bitvector* a, const fbh_bitvector* b, int op, int reps,
+                 float* ms_per_op, uint32_t* out_words);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FASTBIT_HIP_H */

@@ -1,0 +1,10 @@
+"""libfastlib_amd -- MI355X-native FastBit query hot path.
+
+WAH bitvector logic, relic/bin range evaluation and index build as gfx950
+HIP kernels behind a C ABI (``include/fastbit_hip.h``,
+``lib/libfastbit_hip.so``).  This package is the Python host binding of that
+ABI; the C++ host binding is ``include/fastbit_hip.hpp``.
+"""
+from .native import (AND, OR, XOR, MINUS, Bitvector, FastbitError, bench_op, lib)  # noqa: F401
+
+__all__ = ["AND", "OR", "XOR", "MINUS", "Bitvector", "FastbitError", "bench_op", "lib"]

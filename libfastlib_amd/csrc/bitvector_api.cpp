@@ -1,0 +1,772 @@
+// C ABI of the bitvector part of libfastbit_hip.so (include/fastbit_hip.h).
+//
+// The host side mirrors the reference's operator dispatch
+// (src/bitvector.cpp:1194-1762): sizes are equalised with adjustSize, the
+// literal-only tests ca/cb, the all0s/all1s shortcuts and the decompressed-
+// result threshold (|A|w + |B|w) * 31 >= B.nbits choose the routine, and the
+// routine decides how the result words look after compress():
+//   c0 / d1 / d2 / c1   -> canonical WAH of the result bits
+//   c2                  -> the words the *_c2 loop emits, then compress()
+//   shortcuts           -> one operand's words (compressed), active adjusted
+// All word work runs in gfx950 kernels; the host only keeps the metadata the
+// reference keeps in the object (nbits, nset, active word) plus host copies
+// of the first and last code word for the O(1) tests and the tail appends.
+#include "../../include/fastbit_hip.h"
+#include "runtime.h"
+#include "launch.h"
+#include "bitvector_impl.h"
+#include <cstring>
+#include <vector>
+#include <string>
+
+using namespace fbh_rt;
+
+namespace {
+
+constexpr uint32_t MAXBITS = 31;
+constexpr uint32_t ALLONES = 0x7FFFFFFFu;
+constexpr uint32_t MAXCNT = 0x3FFFFFFFu;
+constexpr uint32_t FILLBIT = 0x40000000u;
+constexpr uint32_t HEADER0 = 0x80000000u;
+constexpr uint32_t HEADER1 = 0xC0000000u;
+
+inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+inline uint32_t popc(uint32_t x) { return (uint32_t)__builtin_popcount(x); }
+
+int fail(int code, const std::string& msg) { set_error(msg); return code; }
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// Buffer management of a handle.
+namespace fbh_impl {
+
+int bv_reserve(fbh_bitvector* bv, uint32_t words) {
+    if (words <= bv->cap && bv->d) return FBH_OK;
+    uint32_t* nd = (uint32_t*)dev_alloc(sizeof(uint32_t) * (size_t)(words ? words : 1));
+    if (!nd) return fail(FBH_ERR_NOMEM, "device allocation failed");
+    if (bv->d && bv->nw) {
+        int rc = check(hipMemcpyAsync(nd, bv->d, sizeof(uint32_t) * bv->nw,
+                                      hipMemcpyDeviceToDevice, stream()), "copy");
+        if (rc) { dev_free(nd); return rc; }
+    }
+    dev_free(bv->d);
+    bv->d = nd;
+    bv->cap = words ? words : 1;
+    return FBH_OK;
+}
+
+void bv_release(fbh_bitvector* bv) {
+    dev_free(bv->d); bv->d = nullptr; bv->cap = 0;
+    dev_free(bv->ck); bv->ck = nullptr; bv->ckcap = 0; bv->ck_ok = false;
+}
+
+// Chunk index of the words (start group of every 32nd code word).
+int bv_ensure_ck(fbh_bitvector* bv) {
+    if (bv->ck_ok) return FBH_OK;
+    const uint32_t nc = cdiv(bv->nw, fbk::CHUNK_WORDS);
+    if (nc + 2 > bv->ckcap || !bv->ck) {
+        dev_free(bv->ck);
+        bv->ck = (uint32_t*)dev_alloc(sizeof(uint32_t) * (nc + 2));
+        if (!bv->ck) return fail(FBH_ERR_NOMEM, "device allocation failed");
+        bv->ckcap = nc + 2;
+    }
+    uint32_t* tmp = (uint32_t*)scratch(1).get(sizeof(uint32_t) * (nc + fbk::scan_tmp_words(nc) + 4));
+    if (!tmp) return fail(FBH_ERR_NOMEM, "scratch allocation failed");
+    int rc = check(fbk::launch_chunk_index(bv->d, bv->nw, bv->ck, tmp, stream()), "chunk index");
+    if (rc) return rc;
+    bv->ck_ok = true;
+    return FBH_OK;
+}
+
+// do_cnt (src/bitvector.cpp:583-608): counts groups and ones on the device;
+// an all-zero vector of more than one word collapses to one 0-fill.
+int bv_do_cnt(fbh_bitvector* bv, uint32_t* nb_out) {
+    uint32_t* ctl = (uint32_t*)scratch(2).get(64);
+    if (!ctl) return fail(FBH_ERR_NOMEM, "scratch allocation failed");
+    int rc = check(fbk::launch_cnt(bv->d, bv->nw, ctl, stream()), "cnt");
+    if (rc) return rc;
+    uint32_t h[2];
+    rc = check(hipMemcpyAsync(h, ctl, sizeof(h), hipMemcpyDeviceToHost, stream()), "cnt copy");
+    if (rc) return rc;
+    rc = check(hipStreamSynchronize(stream()), "cnt sync");
+    if (rc) return rc;
+    const uint32_t ones = h[0], groups = h[1];
+    const uint32_t nb = groups * MAXBITS;
+    if (ones == 0 && bv->nw > 1) {
+        const uint32_t w = HEADER0 + groups;
+        rc = check(hipMemcpyAsync(bv->d, &w, 4, hipMemcpyHostToDevice, stream()), "collapse");
+        if (rc) return rc;
+        rc = check(hipStreamSynchronize(stream()), "collapse sync");
+        if (rc) return rc;
+        bv->nw = 1; bv->w0 = bv->wl = w; bv->ck_ok = false;
+    }
+    bv->nset = ones;
+    *nb_out = nb;
+    return FBH_OK;
+}
+
+int bv_size(fbh_bitvector* bv, uint32_t* out) {
+    if (bv->nbits == 0 && bv->nw > 0) {
+        uint32_t nb;
+        int rc = bv_do_cnt(bv, &nb);
+        if (rc) return rc;
+        bv->nbits = nb;
+    }
+    *out = bv->nbits + bv->anb;
+    return FBH_OK;
+}
+
+bool all0s(const fbh_bitvector* bv) {
+    if (bv->nw == 0) return true;
+    if (bv->nw == 1) return bv->w0 == 0 || (bv->w0 >= HEADER0 && bv->w0 < HEADER1);
+    return false;
+}
+
+bool all1s(const fbh_bitvector* bv) {
+    if (bv->nw == 1) return bv->w0 == ALLONES || bv->w0 > HEADER1;
+    return false;
+}
+
+// ---- tail appends (src/bitvector.h:699-831) on the host copy of the last
+// word; new words are collected and uploaded together.
+struct Tail {
+    fbh_bitvector* bv;
+    std::vector<uint32_t> words;  // words[0] replaces the last word if keep
+    bool has_last;
+    void begin() {
+        words.clear();
+        has_last = bv->nw > 0;
+        if (has_last) words.push_back(bv->wl);
+    }
+    uint32_t& back() { return words.back(); }
+    bool empty() const { return words.empty(); }
+    void append_active() {
+        const uint32_t v = bv->aval;
+        if (empty()) words.push_back(v);
+        else if (v == 0) {
+            if (back() == 0) back() = HEADER0 + 2;
+            else if (back() >= HEADER0 && back() < HEADER1) ++back();
+            else words.push_back(v);
+        } else if (v == ALLONES) {
+            if (back() == ALLONES) back() = HEADER1 | 2;
+            else if (back() >= HEADER1) ++back();
+            else words.push_back(v);
+        } else {
+            words.push_back(v);
+        }
+        bv->nbits += MAXBITS;
+        bv->aval = 0; bv->anb = 0;
+        bv->nset = 0;
+    }
+    void append_counter(int val, uint32_t cnt) {
+        const uint32_t head = 2 + val;
+        const uint32_t w = (head << 30) + cnt;
+        bv->nbits += cnt * MAXBITS;
+        if (empty()) words.push_back(w);
+        else if ((back() >> 30) == head) back() += cnt;
+        else if (back() == ALLONES && head == 3) back() = w + 1;
+        else if (back() == 0 && head == 2) back() = w + 1;
+        else words.push_back(w);
+    }
+    void appendFill(int val, uint32_t n) {
+        if (n == 0) return;
+        if (bv->anb > 0) {
+            uint32_t tmp = MAXBITS - bv->anb;
+            if (tmp > n) tmp = n;
+            bv->anb += tmp;
+            bv->aval <<= tmp;
+            n -= tmp;
+            if (val != 0) bv->aval |= (1U << tmp) - 1;
+            if (bv->anb >= MAXBITS) append_active();
+        }
+        if (n >= MAXBITS) {
+            const uint32_t cnt = n / MAXBITS;
+            if (cnt > 1) append_counter(val, cnt);
+            else { bv->aval = val ? ALLONES : 0u; append_active(); }
+            n -= cnt * MAXBITS;
+        }
+        if (n > 0) {
+            bv->anb = n;
+            bv->aval = val * ((1U << n) - 1);
+        }
+    }
+    int commit() {
+        const uint32_t keep = has_last ? bv->nw - 1 : 0;
+        const uint32_t nn = keep + (uint32_t)words.size();
+        int rc = bv_reserve(bv, nn + 1);
+        if (rc) return rc;
+        if (!words.empty()) {
+            rc = check(hipMemcpyAsync(bv->d + keep, words.data(), 4 * words.size(),
+                                      hipMemcpyHostToDevice, stream()), "tail upload");
+            if (rc) return rc;
+            rc = check(hipStreamSynchronize(stream()), "tail sync");
+            if (rc) return rc;
+        }
+        bv->nw = nn;
+        if (nn) { bv->wl = words.back(); if (keep == 0) bv->w0 = words.front(); }
+        bv->ck_ok = false;
+        return FBH_OK;
+    }
+};
+
+int bv_adjust(fbh_bitvector* bv, uint32_t nv, uint32_t nt) {
+    if (bv->nbits == 0 || bv->nbits < bv->nw * MAXBITS) {
+        uint32_t nb;
+        int rc = bv_do_cnt(bv, &nb);
+        if (rc) return rc;
+        bv->nbits = nb;
+    }
+    const uint32_t sz = bv->nbits + bv->anb;
+    if (sz == nt) return FBH_OK;
+    if (nt < sz) return fail(FBH_ERR_SIZE, "adjustSize: truncation is not supported");
+    Tail t{bv, {}, false};
+    t.begin();
+    if (nv > nt) nv = nt;
+    if (nv > sz) {
+        t.appendFill(1, nv - sz);
+        if (nt > nv) t.appendFill(0, nt - nv);
+    } else {
+        t.appendFill(0, nt - sz);
+    }
+    return t.commit();
+}
+
+// Copies `src`'s words (and chunk index) into `dst` (metadata not touched).
+int bv_copy_words(fbh_bitvector* dst, const fbh_bitvector* src) {
+    int rc = bv_reserve(dst, src->nw + 1);
+    if (rc) return rc;
+    if (src->nw) {
+        rc = check(hipMemcpyAsync(dst->d, src->d, 4ull * src->nw, hipMemcpyDeviceToDevice,
+                                  stream()), "copy words");
+        if (rc) return rc;
+    }
+    dst->nw = src->nw; dst->w0 = src->w0; dst->wl = src->wl;
+    dst->ck_ok = false;
+    if (src->ck_ok) {
+        const uint32_t nc = cdiv(src->nw, fbk::CHUNK_WORDS) + 1;
+        if (dst->ckcap < nc + 1 || !dst->ck) {
+            dev_free(dst->ck);
+            dst->ck = (uint32_t*)dev_alloc(4ull * (nc + 1));
+            if (!dst->ck) return fail(FBH_ERR_NOMEM, "device allocation failed");
+            dst->ckcap = nc + 1;
+        }
+        rc = check(hipMemcpyAsync(dst->ck, src->ck, 4ull * nc, hipMemcpyDeviceToDevice, stream()),
+                   "copy ck");
+        if (rc) return rc;
+        dst->ck_ok = true;
+    }
+    return FBH_OK;
+}
+
+// Output of a word kernel: fresh buffers sized for the worst case.
+struct OutBuf {
+    uint32_t* d = nullptr; uint32_t cap = 0;
+    uint32_t* ck = nullptr; uint32_t ckcap = 0;
+    int alloc(uint32_t words) {
+        cap = words + 2;
+        ckcap = cdiv(cap, fbk::CHUNK_WORDS) + 2;
+        d = (uint32_t*)dev_alloc(4ull * cap);
+        ck = (uint32_t*)dev_alloc(4ull * ckcap);
+        if (!d || !ck) { dev_free(d); dev_free(ck); d = ck = nullptr;
+            return fail(FBH_ERR_NOMEM, "device allocation failed"); }
+        return FBH_OK;
+    }
+};
+
+// Reads the control block after a word kernel and installs the output.
+int finish_words(fbh_bitvector* bv, OutBuf& ob, uint32_t* ctl, uint32_t* ones_out) {
+    int rc = check(fbk::launch_edge_words(ob.d, ctl, stream()), "edge words");
+    if (rc) return rc;
+    uint32_t h[8];
+    rc = check(hipMemcpyAsync(h, ctl, sizeof(h), hipMemcpyDeviceToHost, stream()), "ctl copy");
+    if (rc) return rc;
+    rc = check(hipStreamSynchronize(stream()), "op sync");
+    if (rc) return rc;
+    dev_free(bv->d); dev_free(bv->ck);
+    bv->d = ob.d; bv->cap = ob.cap; bv->ck = ob.ck; bv->ckcap = ob.ckcap;
+    bv->nw = h[1]; bv->w0 = h[4]; bv->wl = h[5];
+    bv->ck_ok = true;
+    if (ones_out) *ones_out = h[2];
+    return FBH_OK;
+}
+
+// compress() (src/bitvector.cpp:378-469) of the handle's words.
+int bv_compress(fbh_bitvector* bv) {
+    if (bv->nw < 2) return FBH_OK;
+    OutBuf ob;
+    int rc = ob.alloc(bv->nw);
+    if (rc) return rc;
+    const uint32_t nt = fbk::tok_tiles(bv->nw);
+    uint8_t* sc = (uint8_t*)scratch(0).get(64 + 16ull * nt + 64);
+    if (!sc) return fail(FBH_ERR_NOMEM, "scratch allocation failed");
+    uint32_t* ctl = (uint32_t*)sc;
+    uint64_t* status = (uint64_t*)(sc + 64);
+    rc = check(fbk::launch_tok(bv->d, nullptr, bv->nw, 0, 0, 0, ob.d, ob.ck, status, ctl,
+                               stream()), "compress");
+    if (rc) return rc;
+    return finish_words(bv, ob, ctl, nullptr);
+}
+
+// Canonical WAH (or the exact c2 words) of a OP b into `res` (a, b have
+// equal group counts; `groups` > 0).
+int run_word_op(const fbh_bitvector* a, const fbh_bitvector* b, int op, bool c2,
+                fbh_bitvector* res, uint32_t groups) {
+    const bool la = a->nw == groups;
+    const bool lb = b->nw == groups;
+    OutBuf ob;
+    int rc = ob.alloc(a->nw + b->nw);
+    if (rc) return rc;
+    if (!c2 && la && lb) {
+        // c0: both literal-only, one group per word
+        const uint32_t nt = fbk::tok_tiles(groups);
+        uint8_t* sc = (uint8_t*)scratch(0).get(64 + 16ull * nt + 64);
+        if (!sc) return fail(FBH_ERR_NOMEM, "scratch allocation failed");
+        uint32_t* ctl = (uint32_t*)sc;
+        uint64_t* status = (uint64_t*)(sc + 64);
+        rc = check(fbk::launch_tok(a->d, b->d, groups, groups, op, 1, ob.d, ob.ck, status, ctl,
+                                   stream()), "c0 op");
+        if (rc) return rc;
+        return finish_words(res, ob, ctl, nullptr);
+    }
+    rc = bv_ensure_ck(const_cast<fbh_bitvector*>(a));
+    if (rc) return rc;
+    rc = bv_ensure_ck(const_cast<fbh_bitvector*>(b));
+    if (rc) return rc;
+    const uint32_t nt = fbk::op_tiles(a->nw, b->nw);
+    uint8_t* sc = (uint8_t*)scratch(0).get(64 + 8ull * nt + 64);
+    if (!sc) return fail(FBH_ERR_NOMEM, "scratch allocation failed");
+    uint32_t* ctl = (uint32_t*)sc;
+    uint64_t* status = (uint64_t*)(sc + 64);
+    rc = check(fbk::launch_op(a->d, a->ck, a->nw, b->d, b->ck, b->nw, groups, op, c2 ? 0 : 1,
+                              ob.d, ob.ck, status, ctl, stream()), "op");
+    if (rc) return rc;
+    return finish_words(res, ob, ctl, nullptr);
+}
+
+uint32_t active_op(uint32_t x, uint32_t y, int op) {
+    switch (op) {
+    case FBH_AND: return x & y;
+    case FBH_OR: return x | y;
+    case FBH_XOR: return x ^ y;
+    default: return x & ~y;
+    }
+}
+
+// a OP= b; a.compress()  -- the dispatch of src/bitvector.cpp:1194-1762.
+// When `plan_only` is set the routine is decided (and the shortcut / size
+// work done) but the word kernel is not run; *need_kernel tells the caller.
+int bv_op(fbh_bitvector* a, const fbh_bitvector* b_in, int op, int* path_out) {
+    if (op < 0 || op > 3) return fail(FBH_ERR_ARG, "unknown operator");
+    uint32_t sa, sb;
+    fbh_bitvector* bm = const_cast<fbh_bitvector*>(b_in);
+    int rc = bv_size(a, &sa);
+    if (rc) return rc;
+    rc = bv_size(bm, &sb);
+    if (rc) return rc;
+    fbh_bitvector tmpb;
+    const fbh_bitvector* b = b_in;
+    if (sa > sb) {
+        // the reference copies rhs, extends it and recurses
+        tmpb.nbits = bm->nbits; tmpb.nset = bm->nset; tmpb.aval = bm->aval; tmpb.anb = bm->anb;
+        rc = bv_copy_words(&tmpb, bm);
+        if (rc) { bv_release(&tmpb); return rc; }
+        rc = bv_adjust(&tmpb, 0, sa);
+        if (rc) { bv_release(&tmpb); return rc; }
+        b = &tmpb;
+    } else if (sa < sb) {
+        rc = bv_adjust(a, 0, sb);
+        if (rc) return rc;
+    }
+    const bool ca = (a->nw * MAXBITS == a->nbits && a->nbits > 0);
+    const bool cb = (b->nw * MAXBITS == b->nbits && b->nbits > 0);
+    const uint32_t dsum = (a->nw + b->nw) * MAXBITS;
+    const uint32_t groups = a->nbits / MAXBITS;
+    int path = 0;
+    enum { K_NONE, K_CANON, K_C2, K_COPYB, K_FLIPB } kind = K_NONE;
+
+    auto active_and = [&]() {
+        if (a->anb == b->anb) a->aval &= b->aval;
+        else if (a->anb > b->anb) a->aval &= (b->aval << (a->anb - b->anb));
+        else { a->aval = (a->aval << (b->anb - a->anb)) & b->aval; a->anb = b->anb; }
+    };
+
+    switch (op) {
+    case FBH_AND:
+        if (ca || cb) { kind = K_CANON; path = ca ? (cb ? FBH_PATH_C0 : FBH_PATH_D1) : FBH_PATH_D1SWAP; }
+        else if (all0s(a) || all1s(b)) { active_and(); path = FBH_PATH_SHORTCUT; }
+        else if (all0s(b) || all1s(a)) { kind = K_COPYB; a->nset = b->nset; active_and(); path = FBH_PATH_SHORTCUT; }
+        else if (dsum >= b->nbits) { kind = K_CANON; path = FBH_PATH_D2; }
+        else { kind = K_C2; path = FBH_PATH_C2; }
+        break;
+    case FBH_OR:
+        if (ca || cb) { kind = K_CANON; path = ca ? (cb ? FBH_PATH_C0 : FBH_PATH_D1) : FBH_PATH_D1SWAP; }
+        else if (all1s(a) || all0s(b)) { a->aval |= b->aval; path = FBH_PATH_SHORTCUT; }
+        else if (all0s(a) || all1s(b)) { kind = K_COPYB; a->nset = b->nset; a->aval |= b->aval; path = FBH_PATH_SHORTCUT; }
+        else if (dsum >= b->nbits) { kind = K_CANON; path = FBH_PATH_D2; }
+        else { kind = K_C2; path = FBH_PATH_C2; }
+        break;
+    case FBH_XOR:
+        if (ca) { kind = K_CANON; path = cb ? FBH_PATH_C0 : FBH_PATH_D1; }
+        else if (cb) { kind = K_CANON; path = FBH_PATH_C1; }
+        else if (dsum >= b->nbits) { kind = K_CANON; path = FBH_PATH_D2; }
+        else { kind = K_C2; path = FBH_PATH_C2; }
+        break;
+    default:
+        if (ca) { kind = K_CANON; path = cb ? FBH_PATH_C0 : FBH_PATH_D1; }
+        else if (cb) { kind = K_CANON; path = FBH_PATH_C1; }
+        else if (all0s(a) || all0s(b)) { a->aval &= ~b->aval; path = FBH_PATH_SHORTCUT; }
+        else if (all1s(b)) {
+            // zero out m_vec: one 0-fill of rhs's fill count
+            a->nset = 0; a->nbits = 0; a->nw = 0; a->ck_ok = false;
+            a->aval &= ~b->aval;
+            Tail t{a, {}, false};
+            t.begin();
+            t.append_counter(0, b->w0 & MAXCNT);
+            rc = t.commit();
+            if (rc) { bv_release(&tmpb); return rc; }
+            path = FBH_PATH_SHORTCUT;
+        } else if (all1s(a)) { kind = K_FLIPB; path = FBH_PATH_SHORTCUT; }
+        else if (dsum >= b->nbits) { kind = K_CANON; path = FBH_PATH_D2; }
+        else { kind = K_C2; path = FBH_PATH_C2; }
+        break;
+    }
+
+    if (kind == K_CANON || kind == K_C2) {
+        if (groups == 0) {
+            // no whole words: only the active words combine
+            a->aval = active_op(a->aval, b->aval, op);
+        } else {
+            rc = run_word_op(a, b, op, kind == K_C2, a, groups);
+            if (rc) { bv_release(&tmpb); return rc; }
+            a->aval = active_op(a->aval, b->aval, op);
+            a->nset = 0;
+            a->nbits = groups * MAXBITS;
+        }
+    } else if (kind == K_COPYB) {
+        rc = bv_copy_words(a, b);
+        if (rc) { bv_release(&tmpb); return rc; }
+        rc = bv_compress(a);
+        if (rc) { bv_release(&tmpb); return rc; }
+    } else if (kind == K_FLIPB) {
+        const uint32_t keep = a->aval;
+        rc = bv_copy_words(a, b);
+        if (rc) { bv_release(&tmpb); return rc; }
+        a->nbits = b->nbits; a->nset = b->nset; a->aval = b->aval; a->anb = b->anb;
+        rc = check(fbk::launch_flip(a->d, a->nw, stream()), "flip");
+        if (rc) { bv_release(&tmpb); return rc; }
+        auto fl = [](uint32_t w) { return w > ALLONES ? (w ^ FILLBIT) : (w ^ ALLONES); };
+        a->w0 = fl(a->w0); a->wl = fl(a->wl);
+        if (a->nset > 0) a->nset = a->nbits - a->nset;
+        if (a->anb > 0) a->aval ^= ((1U << a->anb) - 1);
+        a->aval &= keep;
+        rc = bv_compress(a);
+        if (rc) { bv_release(&tmpb); return rc; }
+    } else {
+        rc = bv_compress(a);
+        if (rc) { bv_release(&tmpb); return rc; }
+    }
+    bv_release(&tmpb);
+    if (path_out) *path_out = path;
+    return FBH_OK;
+}
+
+} // namespace fbh_impl
+
+using namespace fbh_impl;
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int fbh_init(int device) { return ensure_init(device); }
+const char* fbh_last_error(void) { return last_error(); }
+int fbh_sync(void) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    return check(hipStreamSynchronize(stream()), "sync");
+}
+
+int fbh_bv_new(fbh_bitvector** out) {
+    if (!out) return fail(FBH_ERR_ARG, "null output");
+    int rc = ensure_init();
+    if (rc) return rc;
+    *out = new fbh_bitvector();
+    return FBH_OK;
+}
+
+int fbh_bv_from_serial(const uint32_t* arr, uint32_t n, fbh_bitvector** out) {
+    if (!out || (n && !arr)) return fail(FBH_ERR_ARG, "null argument");
+    int rc = ensure_init();
+    if (rc) return rc;
+    fbh_bitvector* bv = new fbh_bitvector();
+    // src/bitvector.cpp:54-105
+    uint32_t nw = 0;
+    if (n > 1) {
+        nw = n - 1;
+        const uint32_t last = arr[n - 1];
+        if (last > 0 && last < MAXBITS) { bv->anb = last; bv->aval = arr[n - 2]; nw = n - 2; }
+    }
+    rc = bv_reserve(bv, nw + 1);
+    if (rc) { delete bv; return rc; }
+    if (nw) {
+        rc = check(hipMemcpyAsync(bv->d, arr, 4ull * nw, hipMemcpyHostToDevice, stream()), "upload");
+        if (!rc) rc = check(hipStreamSynchronize(stream()), "upload sync");
+        if (rc) { bv_release(bv); delete bv; return rc; }
+        bv->w0 = arr[0]; bv->wl = arr[nw - 1];
+    }
+    bv->nw = nw;
+    *out = bv;
+    return FBH_OK;
+}
+
+int fbh_bv_set(fbh_bitvector* bv, int val, uint32_t n) {
+    if (!bv) return fail(FBH_ERR_ARG, "null handle");
+    // src/bitvector.cpp:256-276
+    bv->nw = 0; bv->nbits = 0; bv->nset = 0; bv->aval = 0; bv->anb = 0; bv->ck_ok = false;
+    Tail t{bv, {}, false};
+    t.begin();
+    const uint32_t k = n / MAXBITS;
+    if (k > 1) t.append_counter(val ? 1 : 0, k);
+    else if (k == 1) { bv->aval = val ? ALLONES : 0u; t.append_active(); }
+    bv->anb = n - k * MAXBITS;
+    if (val != 0) { bv->aval = (1U << bv->anb) - 1; bv->nset = k * MAXBITS; }
+    return t.commit();
+}
+
+int fbh_bv_copy(const fbh_bitvector* src, fbh_bitvector** out) {
+    if (!src || !out) return fail(FBH_ERR_ARG, "null argument");
+    fbh_bitvector* bv = new fbh_bitvector();
+    bv->nbits = src->nbits; bv->nset = src->nset; bv->aval = src->aval; bv->anb = src->anb;
+    int rc = bv_copy_words(bv, src);
+    if (rc) { bv_release(bv); delete bv; return rc; }
+    *out = bv;
+    return FBH_OK;
+}
+
+void fbh_bv_free(fbh_bitvector* bv) {
+    if (!bv) return;
+    bv_release(bv);
+    delete bv;
+}
+
+int fbh_bv_write(fbh_bitvector* bv, uint32_t* arr, uint32_t cap, uint32_t* n) {
+    if (!bv || !n) return fail(FBH_ERR_ARG, "null argument");
+    const uint32_t need = bv->nw + 1 + (bv->anb > 0 ? 1 : 0);
+    *n = need;
+    if (!arr) return FBH_OK;
+    if (cap < need) return fail(FBH_ERR_SIZE, "output array too small");
+    if (bv->nw) {
+        int rc = check(hipMemcpyAsync(arr, bv->d, 4ull * bv->nw, hipMemcpyDeviceToHost, stream()),
+                       "download");
+        if (!rc) rc = check(hipStreamSynchronize(stream()), "download sync");
+        if (rc) return rc;
+    }
+    uint32_t k = bv->nw;
+    if (bv->anb > 0) arr[k++] = bv->aval;
+    arr[k] = bv->anb;
+    return FBH_OK;
+}
+
+int fbh_bv_op(fbh_bitvector* a, const fbh_bitvector* b, int op, int* path) {
+    if (!a || !b) return fail(FBH_ERR_ARG, "null handle");
+    try { return bv_op(a, b, op, path); }
+    catch (...) { return fail(FBH_ERR_INTERNAL, "exception in fbh_bv_op"); }
+}
+
+int fbh_bv_op_new(const fbh_bitvector* a, const fbh_bitvector* b, int op, fbh_bitvector** out,
+                  int* path) {
+    if (!a || !b || !out) return fail(FBH_ERR_ARG, "null argument");
+    fbh_bitvector* r = nullptr;
+    int rc = fbh_bv_copy(a, &r);
+    if (rc) return rc;
+    rc = fbh_bv_op(r, b, op, path);
+    if (rc) { fbh_bv_free(r); return rc; }
+    *out = r;
+    return FBH_OK;
+}
+
+int fbh_bv_compress(fbh_bitvector* bv) {
+    if (!bv) return fail(FBH_ERR_ARG, "null handle");
+    return bv_compress(bv);
+}
+
+int fbh_bv_decompress(fbh_bitvector* bv) {
+    if (!bv) return fail(FBH_ERR_ARG, "null handle");
+    // src/bitvector.cpp:474-512
+    if (bv->nbits == 0 && bv->nw > 0) {
+        uint32_t nb;
+        int rc = bv_do_cnt(bv, &nb);
+        if (rc) return rc;
+        bv->nbits = nb;
+    }
+    if (bv->nw * MAXBITS == bv->nbits) return FBH_OK;
+    const uint32_t groups = bv->nbits / MAXBITS;
+    int rc = bv_ensure_ck(bv);
+    if (rc) return rc;
+    uint32_t* nd = (uint32_t*)dev_alloc(4ull * (groups + 1));
+    if (!nd) return fail(FBH_ERR_NOMEM, "device allocation failed");
+    rc = check(fbk::launch_decompress(bv->d, bv->ck, bv->nw, groups, nd, 0, stream()), "decompress");
+    if (rc) { dev_free(nd); return rc; }
+    rc = check(hipStreamSynchronize(stream()), "decompress sync");
+    if (rc) { dev_free(nd); return rc; }
+    uint32_t edge[2] = {0, 0};
+    if (groups) {
+        (void)hipMemcpy(&edge[0], nd, 4, hipMemcpyDeviceToHost);
+        (void)hipMemcpy(&edge[1], nd + groups - 1, 4, hipMemcpyDeviceToHost);
+    }
+    dev_free(bv->d);
+    bv->d = nd; bv->cap = groups + 1; bv->nw = groups;
+    bv->w0 = edge[0]; bv->wl = edge[1];
+    bv->ck_ok = false;
+    return FBH_OK;
+}
+
+int fbh_bv_flip(fbh_bitvector* bv) {
+    if (!bv) return fail(FBH_ERR_ARG, "null handle");
+    // src/bitvector.cpp:1128-1174
+    int rc = check(fbk::launch_flip(bv->d, bv->nw, stream()), "flip");
+    if (rc) return rc;
+    auto fl = [](uint32_t w) { return w > ALLONES ? (w ^ FILLBIT) : (w ^ ALLONES); };
+    if (bv->nw) { bv->w0 = fl(bv->w0); bv->wl = fl(bv->wl); }
+    if (bv->nbits == 0 && bv->nw > 0) {
+        uint32_t* ctl = (uint32_t*)scratch(2).get(64);
+        rc = check(fbk::launch_cnt(bv->d, bv->nw, ctl, stream()), "flip count");
+        if (rc) return rc;
+        uint32_t h[2];
+        rc = check(hipMemcpyAsync(h, ctl, 8, hipMemcpyDeviceToHost, stream()), "flip copy");
+        if (!rc) rc = check(hipStreamSynchronize(stream()), "flip sync");
+        if (rc) return rc;
+        bv->nbits = h[1] * MAXBITS;
+    }
+    if (bv->nset > 0) bv->nset = bv->nbits - bv->nset;
+    if (bv->anb > 0) bv->aval ^= ((1U << bv->anb) - 1);
+    return FBH_OK;
+}
+
+int fbh_bv_cnt(fbh_bitvector* bv, uint32_t* cnt) {
+    if (!bv || !cnt) return fail(FBH_ERR_ARG, "null argument");
+    // src/bitvector.h:652-656
+    if (bv->nset == 0 && bv->nw > 0) {
+        uint32_t nb;
+        int rc = bv_do_cnt(bv, &nb);
+        if (rc) return rc;
+        bv->nbits = nb;
+    }
+    *cnt = bv->nset + popc(bv->aval);
+    return FBH_OK;
+}
+
+int fbh_bv_size(fbh_bitvector* bv, uint32_t* size) {
+    if (!bv || !size) return fail(FBH_ERR_ARG, "null argument");
+    return bv_size(bv, size);
+}
+
+int fbh_bv_adjust_size(fbh_bitvector* bv, uint32_t nv, uint32_t nt) {
+    if (!bv) return fail(FBH_ERR_ARG, "null handle");
+    return bv_adjust(bv, nv, nt);
+}
+
+int fbh_bv_nwords(const fbh_bitvector* bv, uint32_t* n) {
+    if (!bv || !n) return fail(FBH_ERR_ARG, "null argument");
+    *n = bv->nw;
+    return FBH_OK;
+}
+
+int fbh_bv_rids(fbh_bitvector* bv, uint32_t* rids, uint32_t cap, uint32_t* n) {
+    if (!bv || !n) return fail(FBH_ERR_ARG, "null argument");
+    uint32_t c;
+    int rc = fbh_bv_cnt(bv, &c);
+    if (rc) return rc;
+    *n = c;
+    if (!rids) return FBH_OK;
+    const uint32_t groups = bv->nbits / MAXBITS;
+    const uint32_t nword_rids = c - popc(bv->aval);
+    if (groups && nword_rids) {
+        rc = bv_ensure_ck(bv);
+        if (rc) return rc;
+        const bool literal = bv->nw == groups;
+        uint32_t* lit = bv->d;
+        uint32_t* litbuf = nullptr;
+        if (!literal) {
+            litbuf = (uint32_t*)dev_alloc(4ull * groups);
+            if (!litbuf) return fail(FBH_ERR_NOMEM, "device allocation failed");
+            rc = check(fbk::launch_decompress(bv->d, bv->ck, bv->nw, groups, litbuf, 0, stream()),
+                       "rids decompress");
+            if (rc) { dev_free(litbuf); return rc; }
+            lit = litbuf;
+        }
+        uint32_t* out = (uint32_t*)dev_alloc(4ull * nword_rids);
+        uint32_t* tmp = (uint32_t*)scratch(1).get(4ull * (2 * groups + fbk::scan_tmp_words(groups) + 4));
+        if (!out || !tmp) { dev_free(litbuf); dev_free(out); return fail(FBH_ERR_NOMEM, "alloc"); }
+        rc = check(fbk::launch_rids_lit(lit, groups, out, nword_rids, tmp, stream()), "rids");
+        const uint32_t ncopy = nword_rids < cap ? nword_rids : cap;
+        if (!rc) rc = check(hipMemcpyAsync(rids, out, 4ull * ncopy, hipMemcpyDeviceToHost, stream()),
+                            "rids copy");
+        if (!rc) rc = check(hipStreamSynchronize(stream()), "rids sync");
+        dev_free(litbuf); dev_free(out);
+        if (rc) return rc;
+    }
+    // the active word's bits follow (first bit = bit anb-1)
+    uint32_t k = nword_rids;
+    const uint32_t pos = bv->nbits;
+    for (uint32_t i = 0; i < bv->anb; ++i)
+        if ((bv->aval >> (bv->anb - 1 - i)) & 1U) { if (k < cap) rids[k] = pos + i; ++k; }
+    return FBH_OK;
+}
+
+int fbh_bench_op(const fbh_bitvector* a, const fbh_bitvector* b, int op, int reps, float* ms,
+                 uint32_t* out_words) {
+    if (!a || !b || reps <= 0) return fail(FBH_ERR_ARG, "bad argument");
+    // decide the routine on a throw-away copy (metadata + shortcut work)
+    fbh_bitvector* ta = nullptr;
+    int rc = fbh_bv_copy(a, &ta);
+    if (rc) return rc;
+    int path = 0;
+    rc = fbh_bv_op(ta, b, op, &path);
+    const uint32_t words = ta ? ta->nw : 0;
+    fbh_bv_free(ta);
+    if (rc) return rc;
+    if (out_words) *out_words = words;
+    if (path != FBH_PATH_C0 && path != FBH_PATH_D1 && path != FBH_PATH_D1SWAP &&
+        path != FBH_PATH_C1 && path != FBH_PATH_D2 && path != FBH_PATH_C2) {
+        if (ms) *ms = 0.f;
+        return FBH_OK;
+    }
+    fbh_bitvector* ma = const_cast<fbh_bitvector*>(a);
+    fbh_bitvector* mb = const_cast<fbh_bitvector*>(b);
+    const uint32_t groups = ma->nbits / MAXBITS;
+    const bool c2 = path == FBH_PATH_C2;
+    const bool lit2 = !c2 && ma->nw == groups && mb->nw == groups;
+    if (!lit2) {
+        rc = bv_ensure_ck(ma); if (rc) return rc;
+        rc = bv_ensure_ck(mb); if (rc) return rc;
+    }
+    OutBuf ob;
+    rc = ob.alloc(ma->nw + mb->nw);
+    if (rc) return rc;
+    const uint32_t nt = lit2 ? fbk::tok_tiles(groups) : fbk::op_tiles(ma->nw, mb->nw);
+    uint8_t* sc = (uint8_t*)scratch(0).get(64 + 16ull * nt + 64);
+    uint32_t* ctl = (uint32_t*)sc;
+    uint64_t* status = (uint64_t*)(sc + 64);
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0); (void)hipEventCreate(&e1);
+    auto one = [&]() {
+        if (lit2)
+            return fbk::launch_tok(ma->d, mb->d, groups, groups, op, 1, ob.d, ob.ck, status, ctl, stream());
+        return fbk::launch_op(ma->d, ma->ck, ma->nw, mb->d, mb->ck, mb->nw, groups, op, c2 ? 0 : 1,
+                              ob.d, ob.ck, status, ctl, stream());
+    };
+    rc = check(one(), "bench warmup");
+    if (!rc) rc = check(hipEventRecord(e0, stream()), "event");
+    for (int i = 0; i < reps && !rc; ++i) rc = check(one(), "bench op");
+    if (!rc) rc = check(hipEventRecord(e1, stream()), "event");
+    if (!rc) rc = check(hipEventSynchronize(e1), "event sync");
+    float t = 0.f;
+    if (!rc) (void)hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+    dev_free(ob.d); dev_free(ob.ck);
+    if (ms) *ms = t / reps;
+    return rc;
+}
+
+} // extern "C"

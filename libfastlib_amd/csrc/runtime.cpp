@@ -1,0 +1,105 @@
+#include "runtime.h"
+#include "../../include/fastbit_hip.h"
+#include <map>
+#include <mutex>
+#include <vector>
+
+namespace fbh_rt {
+
+static int g_device = -1;
+static hipStream_t g_stream = nullptr;
+static std::mutex g_mu;
+static std::multimap<size_t, void*> g_free;        // rounded size -> buffer
+static std::map<void*, size_t> g_size;             // live buffer -> rounded size
+static thread_local std::string t_err;
+
+void set_error(const std::string& s) { t_err = s; }
+const char* last_error() { return t_err.c_str(); }
+
+int check(hipError_t e, const char* what) {
+    if (e == hipSuccess) return FBH_OK;
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return FBH_ERR_DEVICE;
+}
+
+int ensure_init(int device) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_stream && (device < 0 || device == g_device)) return FBH_OK;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) {
+        set_error("no HIP device available");
+        return FBH_ERR_DEVICE;
+    }
+    if (device < 0) device = 0;
+    if (device >= n) { set_error("device index out of range"); return FBH_ERR_ARG; }
+    if ((e = hipSetDevice(device)) != hipSuccess) return check(e, "hipSetDevice");
+    if (g_stream) { (void)hipStreamDestroy(g_stream); g_stream = nullptr; }
+    if ((e = hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking)) != hipSuccess)
+        return check(e, "hipStreamCreate");
+    g_device = device;
+    return FBH_OK;
+}
+
+hipStream_t stream() { return g_stream; }
+
+static size_t round_size(size_t b) {
+    if (b < 256) return 256;
+    size_t r = 256;
+    while (r < b) r <<= 1;
+    return r;
+}
+
+void* dev_alloc(size_t bytes) {
+    const size_t r = round_size(bytes);
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = g_free.find(r);
+        if (it != g_free.end()) {
+            void* p = it->second;
+            g_free.erase(it);
+            g_size[p] = r;
+            return p;
+        }
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, r);
+    if (e != hipSuccess) {
+        // release cached buffers and retry once
+        std::lock_guard<std::mutex> lk(g_mu);
+        (void)hipStreamSynchronize(g_stream);
+        for (auto& kv : g_free) (void)hipFree(kv.second);
+        g_free.clear();
+        e = hipMalloc(&p, r);
+        if (e != hipSuccess) { set_error("hipMalloc failed"); return nullptr; }
+    }
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_size[p] = r;
+    return p;
+}
+
+void dev_free(void* p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_size.find(p);
+    if (it == g_size.end()) return;
+    g_free.emplace(it->second, p);
+    g_size.erase(it);
+}
+
+void* Scratch::get(size_t need) {
+    if (need <= bytes && p) return p;
+    if (p) dev_free(p);
+    size_t b = need < 4096 ? 4096 : need;
+    b += b / 4;
+    p = dev_alloc(b);
+    bytes = p ? b : 0;
+    return p;
+}
+
+Scratch& scratch(int slot) {
+    static Scratch s[8];
+    return s[slot & 7];
+}
+
+} // namespace fbh_rt

@@ -1,0 +1,35 @@
+// Library runtime: device selection, the library stream, a stream-ordered
+// caching allocator for HBM buffers, and the thread-local error string.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include <string>
+
+namespace fbh_rt {
+
+// Select the device and create the stream once; returns 0 or an FBH error.
+int ensure_init(int device = -1);
+hipStream_t stream();
+
+// Caching allocator.  Buffers are handed back to a free list keyed by the
+// rounded size and reused by later requests; all library work runs on one
+// stream, so reuse is ordered after the work that used the buffer.
+void* dev_alloc(size_t bytes);
+void dev_free(void* p);
+
+void set_error(const std::string& s);
+const char* last_error();
+
+// Checks a HIP status; on failure records it and returns FBH_ERR_DEVICE.
+int check(hipError_t e, const char* what);
+
+// A growable device scratch region owned by the library (per process).
+struct Scratch {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void* get(size_t need);
+};
+Scratch& scratch(int slot);
+
+} // namespace fbh_rt

@@ -1,0 +1,694 @@
+// gfx950 kernels for FastBit WAH bitvectors (32-bit words, 31-bit groups).
+//
+//  k_chunk_len      group count of every 32-word chunk (input of the chunk
+//                   position index "ck": start group of code word 32*j)
+//  k_op_tiles       a OP b over two compressed streams: merge-path over the
+//                   chunk index, resynchronisation at common boundaries,
+//                   the *_c2 step walk per thread, run-summary block scan and
+//                   single-pass look-back, then the word write
+//  k_tok_tiles      compressed output from a literal-domain token source:
+//                   compress() of an arbitrary word array and the c0 path
+//                   (both operands literal-only)
+//  k_cnt            popcount + 31*(1-fill counts)
+//  k_decompress     code words -> one literal word per group
+//  k_flip           complement in place
+//  k_rids           ascending positions of the set bits
+#include "dev_common.h"
+
+namespace fbk {
+using namespace wah;
+
+constexpr int TPB = 256;
+constexpr int CHUNK = 32;           // code words per chunk-index entry
+constexpr int KTILE = 64;           // chunk-index items per op tile
+constexpr int WIN = CHUNK * (KTILE + 4);
+
+// Code word accessor: LDS window [base, base+cnt) with global fallback.
+struct WinReader {
+    const word_t* lds;
+    const word_t* g;
+    uint32_t base;
+    uint32_t cnt;
+    __device__ __forceinline__ word_t at(uint32_t i) const {
+        const uint32_t k = i - base;
+        return k < cnt ? lds[k] : g[i];
+    }
+};
+
+struct GlobReader {
+    const word_t* g;
+    __device__ __forceinline__ word_t at(uint32_t i) const { return g[i]; }
+};
+
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(TPB) void k_chunk_len(const word_t* __restrict__ w, uint32_t n,
+                                                   uint32_t* __restrict__ len, uint32_t nchunks) {
+    // one wave per chunk group of 64 chunks: lane j sums chunk (base + j)
+    const uint32_t j = blockIdx.x * TPB + threadIdx.x;
+    if (j >= nchunks) return;
+    const uint32_t b = j * CHUNK;
+    const uint32_t e = min(b + CHUNK, n);
+    uint32_t s = 0;
+    for (uint32_t i = b; i < e; ++i) s += word_len(w[i]);
+    len[j] = s;
+}
+
+// Exclusive scan helpers over uint32 arrays (three-kernel form).
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = TPB * SCAN_ITEMS;
+
+__global__ __launch_bounds__(TPB) void k_scan_tile(const uint32_t* __restrict__ in, uint32_t* out,
+                                                   uint32_t n, uint32_t* __restrict__ tile_sums) {
+    __shared__ uint32_t sh[TPB / WAVE + 1];
+    const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    uint32_t s = 0;
+    #pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        const uint32_t i = base + k;
+        v[k] = i < n ? in[i] : 0u;
+        s += v[k];
+    }
+    uint32_t total;
+    uint32_t ex = block_excl_sum<TPB>(s, sh, &total);
+    #pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        const uint32_t i = base + k;
+        if (i < n) out[i] = ex;
+        ex += v[k];
+    }
+    if (threadIdx.x == 0 && tile_sums) tile_sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(TPB) void k_scan_add(uint32_t* out, uint32_t n,
+                                                  const uint32_t* __restrict__ tile_off) {
+    const uint32_t i = blockIdx.x * TPB + threadIdx.x;
+    if (i < n) out[i] += tile_off[i / SCAN_TILE];
+}
+
+// Writes ck[nchunks] = total (the sentinel) once the scan is done.
+__global__ void k_ck_sentinel(uint32_t* ck, uint32_t nchunks, const uint32_t* len) {
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        ck[nchunks] = nchunks ? ck[nchunks - 1] + len[nchunks - 1] : 0u;
+}
+
+// ---------------------------------------------------------------------------
+struct OpArgs {
+    const word_t* a; const uint32_t* cka; uint32_t na; uint32_t nca;
+    const word_t* b; const uint32_t* ckb; uint32_t nb; uint32_t ncb;
+    uint32_t groups;
+    int op;
+    uint32_t canon;
+    word_t* out;
+    uint32_t* outck;
+    uint64_t* status;
+    uint32_t* ticket;
+    uint32_t* total;      // [0] output words, [1] ones in the output words
+};
+
+// Position of merged chunk-index item d (A items before B items on ties).
+__device__ uint32_t diag_pos(const OpArgs& p, uint32_t d) {
+    if (d >= p.nca + p.ncb) return p.groups;
+    uint32_t lo = d > p.ncb ? d - p.ncb : 0u;
+    uint32_t hi = d < p.nca ? d : p.nca;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (p.cka[mid] <= p.ckb[d - mid - 1]) lo = mid + 1; else hi = mid;
+    }
+    const uint32_t ja = lo, jb = d - lo;
+    const uint32_t pa = ja < p.nca ? p.cka[ja] : 0xFFFFFFFFu;
+    const uint32_t pb = jb < p.ncb ? p.ckb[jb] : 0xFFFFFFFFu;
+    return pa < pb ? pa : pb;
+}
+
+// Emitter for the write pass: result words and the result chunk index.
+struct GlobWriter {
+    word_t* out;
+    uint32_t* outck;
+    uint32_t ones;
+    __device__ __forceinline__ void operator()(uint32_t slot, word_t w, uint32_t head) {
+        out[slot] = w;
+        if ((slot & (CHUNK - 1)) == 0) outck[slot / CHUNK] = head;
+        ones += (w >= HEADER1) ? (w & MAXCNT) * 31u : (w < HEADER0 ? (uint32_t)__popc(w) : 0u);
+    }
+};
+
+// First code word starting at or after position P: looked up in the LDS
+// window (start positions pw[0..cnt], pw[cnt] = end of the window) or, past
+// the window, found by walking the global words.
+__device__ __forceinline__ void first_start_ge(const uint32_t* pw, uint32_t cnt, uint32_t base,
+                                               const word_t* g, uint32_t n, uint32_t P,
+                                               uint32_t& i, uint32_t& pos) {
+    if (P <= pw[cnt]) {
+        uint32_t j = last_le(pw, cnt + 1, P);
+        if (pw[j] < P) ++j;
+        i = base + j; pos = pw[j];
+    } else {
+        i = base + cnt; pos = pw[cnt];
+        while (pos < P && i < n) { pos += word_len(g[i]); ++i; }
+    }
+}
+
+__global__ __launch_bounds__(TPB) void k_op_tiles(OpArgs p) {
+    __shared__ word_t sA[WIN];
+    __shared__ word_t sB[WIN];
+    __shared__ uint32_t posA[WIN + 1];
+    __shared__ uint32_t posB[WIN + 1];
+    __shared__ uint32_t tc[TPB + 1], tia[TPB + 1], tib[TPB + 1];
+    __shared__ Summary sbuf[2 * TPB];
+    __shared__ uint32_t info[16];
+    __shared__ uint32_t shs[TPB / WAVE + 1];
+    __shared__ Summary s_pre;
+
+    const int tid = threadIdx.x;
+    if (tid == 0) info[0] = atomicAdd(p.ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = info[0];
+
+    if (tid == 0) {
+        const uint32_t P0 = diag_pos(p, tile * KTILE);
+        const uint32_t P1 = diag_pos(p, (tile + 1) * KTILE);
+        // chunk windows: from the chunk holding P0 through one chunk past
+        // the chunk holding P1
+        uint32_t ca0 = last_le(p.cka, p.nca, P0);
+        uint32_t ca1 = last_le(p.cka, p.nca, P1) + 2;
+        if (ca1 > p.nca) ca1 = p.nca;
+        uint32_t cb0 = last_le(p.ckb, p.ncb, P0);
+        uint32_t cb1 = last_le(p.ckb, p.ncb, P1) + 2;
+        if (cb1 > p.ncb) cb1 = p.ncb;
+        uint32_t baseA = ca0 * CHUNK, cntA = min(ca1 * CHUNK, p.na) - baseA;
+        uint32_t baseB = cb0 * CHUNK, cntB = min(cb1 * CHUNK, p.nb) - baseB;
+        if (cntA > WIN) cntA = WIN;
+        if (cntB > WIN) cntB = WIN;
+        info[1] = P0; info[2] = P1;
+        info[3] = baseA; info[4] = cntA; info[5] = baseB; info[6] = cntB;
+        info[7] = p.cka[ca0]; info[8] = p.ckb[cb0];
+    }
+    __syncthreads();
+    const uint32_t P0 = info[1], P1 = info[2];
+    const uint32_t baseA = info[3], cntA = info[4], baseB = info[5], cntB = info[6];
+
+    // load windows (coalesced) and their start positions
+    for (uint32_t i = tid; i < cntA; i += TPB) sA[i] = p.a[baseA + i];
+    for (uint32_t i = tid; i < cntB; i += TPB) sB[i] = p.b[baseB + i];
+    __syncthreads();
+    {
+        constexpr int IT = (WIN + 1 + TPB - 1) / TPB;
+        uint32_t la[IT];
+        uint32_t s = 0;
+        #pragma unroll
+        for (int k = 0; k < IT; ++k) {
+            const uint32_t i = tid * IT + k;
+            la[k] = i < cntA ? word_len(sA[i]) : 0u;
+            s += la[k];
+        }
+        uint32_t tot;
+        uint32_t ex = block_excl_sum<TPB>(s, shs, &tot) + info[7];
+        #pragma unroll
+        for (int k = 0; k < IT; ++k) {
+            const uint32_t i = tid * IT + k;
+            if (i <= cntA) posA[i] = ex;
+            ex += la[k];
+        }
+        s = 0;
+        #pragma unroll
+        for (int k = 0; k < IT; ++k) {
+            const uint32_t i = tid * IT + k;
+            la[k] = i < cntB ? word_len(sB[i]) : 0u;
+            s += la[k];
+        }
+        ex = block_excl_sum<TPB>(s, shs, &tot) + info[8];
+        #pragma unroll
+        for (int k = 0; k < IT; ++k) {
+            const uint32_t i = tid * IT + k;
+            if (i <= cntB) posB[i] = ex;
+            ex += la[k];
+        }
+    }
+    __syncthreads();
+
+    WinReader ra{sA, p.a, baseA, cntA};
+    WinReader rb{sB, p.b, baseB, cntB};
+
+    // tile synchronisation points S0 = c(P0), S1 = c(P1)
+    if (tid == 0) {
+        uint32_t S[2], IA[2], IB[2];
+        const uint32_t Pq[2] = {P0, P1};
+        for (int q = 0; q < 2; ++q) {
+            if (Pq[q] >= p.groups) { S[q] = p.groups; IA[q] = p.na; IB[q] = p.nb; continue; }
+            uint32_t ia, pa, ib, pb;
+            first_start_ge(posA, cntA, baseA, p.a, p.na, Pq[q], ia, pa);
+            first_start_ge(posB, cntB, baseB, p.b, p.nb, Pq[q], ib, pb);
+            S[q] = find_common(&ra, &rb, ia, pa, ib, pb, p.groups);
+            IA[q] = ia; IB[q] = ib;
+        }
+        info[9] = S[0]; info[10] = IA[0]; info[11] = IB[0];
+        info[12] = S[1]; info[13] = IA[1]; info[14] = IB[1];
+    }
+    __syncthreads();
+    const uint32_t S0 = info[9], S1 = info[12];
+
+    // per-thread split: merge path over the window's code words in [S0,S1)
+    {
+        const uint32_t ia0 = info[10] - baseA, ib0 = info[11] - baseB;
+        uint32_t iaE = info[13] - baseA, ibE = info[14] - baseB;
+        if (iaE > cntA) iaE = cntA;
+        if (ibE > cntB) ibE = cntB;
+        const uint32_t nA = iaE > ia0 ? iaE - ia0 : 0u;
+        const uint32_t nB = ibE > ib0 ? ibE - ib0 : 0u;
+        const uint32_t tot = nA + nB;
+        const uint32_t E = (tot + TPB - 1) / TPB;
+        const uint32_t d = min((uint32_t)tid * E, tot);
+        uint32_t c, ia, ib;
+        if (d == 0 || S0 >= S1) {
+            c = S0; ia = info[10]; ib = info[11];
+        } else {
+            uint32_t lo = d > nB ? d - nB : 0u, hi = d < nA ? d : nA;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (posA[ia0 + mid] <= posB[ib0 + d - mid - 1]) lo = mid + 1; else hi = mid;
+            }
+            ia = ia0 + lo; ib = ib0 + (d - lo);
+            uint32_t pa = posA[ia], pb = posB[ib];
+            ia += baseA; ib += baseB;
+            c = find_common(&ra, &rb, ia, pa, ib, pb, S1);
+            if (c > S1) c = S1;
+        }
+        tc[tid] = c; tia[tid] = ia; tib[tid] = ib;
+        if (tid == 0) { tc[TPB] = S1; tia[TPB] = info[13]; tib[TPB] = info[14]; }
+    }
+    __syncthreads();
+    const uint32_t c0 = tc[tid], c1 = tc[tid + 1];
+
+    // count pass
+    CountSink cs;
+    cs.init();
+    if (c0 < c1) walk(cs, &ra, &rb, tia[tid], tib[tid], c0, c1, p.op, p.canon);
+    Summary agg;
+    const Summary texcl = block_excl_summary<TPB>(cs.s, sbuf, &agg);
+    if (tid == 0) s_pre = lookback(p.status, tile, agg);
+    __syncthreads();
+    const Summary pre = combine(s_pre, texcl);
+
+    // write pass
+    GlobWriter gw{p.out, p.outck, 0u};
+    if (c0 < c1) {
+        WriteSink<GlobWriter> ws;
+        ws.init(&gw, pre, c0);
+        walk(ws, &ra, &rb, tia[tid], tib[tid], c0, c1, p.op, p.canon);
+        if (c1 == p.groups) {
+            ws.finish();
+            const uint32_t total = combine(pre, cs.s).count;
+            p.total[0] = total;
+            p.outck[(total + CHUNK - 1) / CHUNK] = p.groups;
+        }
+    }
+    // ones in the written words (for cnt caching)
+    uint32_t ones = gw.ones;
+    ones = wave_sum(ones);
+    if (lane_id() == 0 && ones) atomicAdd(&p.total[1], ones);
+}
+
+// ---------------------------------------------------------------------------
+// Literal-domain token source kernels.  Tile = TPB*TOK_ITEMS source items.
+constexpr int TOK_ITEMS = 16;
+constexpr int TOK_TILE = TPB * TOK_ITEMS;
+
+struct TokArgs {
+    const word_t* a;      // compress: the words; c0: operand a (literal)
+    const word_t* b;      // c0: operand b (literal); null for compress
+    uint32_t n;           // source items
+    uint32_t groups;      // total groups represented
+    int op;
+    int mode;             // 0 compress words (fills kept as F), 1 literal op, 2 literal copy
+    word_t* out;
+    uint32_t* outck;
+    uint64_t* status;
+    uint32_t* ticket;
+    uint32_t* total;
+};
+
+template <class S>
+__device__ __forceinline__ void tok_item(S& sink, word_t w, int mode) {
+    if (mode == 0 && w > ALLONES) emit_fill(sink, w >= HEADER1 ? 1 : 0, w & MAXCNT, 1u);
+    else emit_lit(sink, w, 1u);
+}
+
+__global__ __launch_bounds__(TPB) void k_tok_tiles(TokArgs p) {
+    __shared__ word_t sw[TOK_TILE];
+    __shared__ uint32_t sp[TOK_TILE + 1];
+    __shared__ Summary sbuf[2 * TPB];
+    __shared__ uint32_t shs[TPB / WAVE + 1];
+    __shared__ uint32_t info[2];
+    __shared__ Summary s_pre;
+    __shared__ uint32_t s_base_pos;
+    const int tid = threadIdx.x;
+    if (tid == 0) info[0] = atomicAdd(p.ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = info[0];
+    const uint32_t base = tile * TOK_TILE;
+    const uint32_t cnt = base < p.n ? min((uint32_t)TOK_TILE, p.n - base) : 0u;
+    // coalesced load of the tile's words (op applied for the c0 mode)
+    for (uint32_t i = tid; i < cnt; i += TPB) {
+        word_t w = p.a[base + i];
+        if (p.mode == 1) w = apply_op(w, p.b[base + i], p.op);
+        sw[i] = w;
+    }
+    __syncthreads();
+    // positions: group count of each item (mode 0 only; else one group each)
+    const uint32_t i0 = tid * TOK_ITEMS;
+    uint32_t mylen = 0;
+    #pragma unroll
+    for (int k = 0; k < TOK_ITEMS; ++k) {
+        const uint32_t i = i0 + k;
+        if (i < cnt) mylen += (p.mode == 0) ? word_len(sw[i]) : 1u;
+    }
+    CountSink cs;
+    cs.init();
+    #pragma unroll 4
+    for (int k = 0; k < TOK_ITEMS; ++k) {
+        const uint32_t i = i0 + k;
+        if (i < cnt) tok_item(cs, sw[i], p.mode);
+    }
+    uint32_t ltot;
+    const uint32_t lpos = block_excl_sum<TPB>(mylen, shs, &ltot);
+    Summary agg;
+    const Summary texcl = block_excl_summary<TPB>(cs.s, sbuf, &agg);
+    // positions need a prefix over tiles as well: fold the group count into
+    // the look-back by a second status array slot (status[ntiles + tile]).
+    if (tid == 0) {
+        s_pre = lookback(p.status, tile, agg);
+        // group offset of this tile: mode 1/2 -> base; mode 0 -> serial chain
+        if (p.mode == 0) {
+            uint64_t* gs = p.status + gridDim.x;
+            uint32_t off = 0;
+            if (tile > 0) {
+                uint64_t s;
+                do {
+                    s = __hip_atomic_load(&gs[tile - 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                    if ((s >> 63) == 0) __builtin_amdgcn_s_sleep(1);
+                } while ((s >> 63) == 0);
+                off = (uint32_t)s;
+            }
+            __hip_atomic_store(&gs[tile], (1ull << 63) | (uint64_t)(off + ltot), __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            s_base_pos = off;
+        } else {
+            s_base_pos = base;
+        }
+    }
+    __syncthreads();
+    const Summary pre = combine(s_pre, texcl);
+    GlobWriter gw{p.out, p.outck, 0u};
+    WriteSink<GlobWriter> ws;
+    ws.init(&gw, pre, s_base_pos + lpos);
+    #pragma unroll 4
+    for (int k = 0; k < TOK_ITEMS; ++k) {
+        const uint32_t i = i0 + k;
+        if (i < cnt) tok_item(ws, sw[i], p.mode);
+    }
+    const bool last = (cnt > 0) && (base + cnt == p.n) && (i0 < cnt) && (i0 + TOK_ITEMS >= cnt);
+    if (last) {
+        ws.finish();
+        const uint32_t total = combine(pre, cs.s).count;
+        p.total[0] = total;
+        p.total[3] = ws.pos;                 // groups represented
+        p.outck[(total + CHUNK - 1) / CHUNK] = ws.pos;
+    }
+    uint32_t ones = wave_sum(gw.ones);
+    if (lane_id() == 0 && ones) atomicAdd(&p.total[1], ones);
+    (void)sp;
+}
+
+// ---------------------------------------------------------------------------
+// cnt: ones and groups of a word array.  out[0] += ones, out[1] += groups.
+__global__ __launch_bounds__(TPB) void k_cnt(const word_t* __restrict__ w, uint32_t n,
+                                             uint32_t* out) {
+    uint32_t ones = 0, groups = 0;
+    for (uint32_t i = blockIdx.x * TPB + threadIdx.x; i < n; i += gridDim.x * TPB) {
+        const word_t x = w[i];
+        if (x < HEADER0) { ones += __popc(x); groups += 1; }
+        else { const uint32_t c = x & MAXCNT; groups += c; ones += (x >= HEADER1) ? c * 31u : 0u; }
+    }
+    ones = wave_sum(ones);
+    groups = wave_sum(groups);
+    if (lane_id() == 0) {
+        if (ones) atomicAdd(&out[0], ones);
+        if (groups) atomicAdd(&out[1], groups);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// decompress: literal word per group.  One block per DEC_TILE output groups;
+// the block finds the code words covering its groups through the chunk
+// index, stages them and their start positions in LDS, and every thread
+// resolves its groups by binary search.  neg != 0 writes the complement
+// (copy_comp, src/bitvector.cpp:544-567).
+constexpr int DEC_TILE = 2048;
+
+__global__ __launch_bounds__(TPB) void k_decompress(const word_t* __restrict__ w, const uint32_t* ck,
+                                                    uint32_t n, uint32_t nchunks, uint32_t groups,
+                                                    word_t* __restrict__ out, int neg) {
+    __shared__ word_t sw[DEC_TILE + 1];
+    __shared__ uint32_t sp[DEC_TILE + 2];
+    __shared__ uint32_t shs[TPB / WAVE + 1];
+    __shared__ uint32_t info[3];
+    const uint32_t g0 = blockIdx.x * DEC_TILE;
+    if (g0 >= groups) return;
+    const uint32_t g1 = min(g0 + (uint32_t)DEC_TILE, groups);
+    if (threadIdx.x == 0) {
+        const uint32_t c = last_le(ck, nchunks, g0);
+        uint32_t i = c * CHUNK, pos = ck[c];
+        while (i < n) {
+            const uint32_t l = word_len(w[i]);
+            if (pos + l > g0) break;
+            pos += l; ++i;
+        }
+        info[0] = i; info[1] = pos;
+    }
+    __syncthreads();
+    const uint32_t i0 = info[0], p0 = info[1];
+    // at most DEC_TILE+1 code words can overlap DEC_TILE groups
+    const uint32_t cnt = min((uint32_t)DEC_TILE + 1u, n - i0);
+    for (uint32_t k = threadIdx.x; k < cnt; k += TPB) sw[k] = w[i0 + k];
+    __syncthreads();
+    constexpr int IT = (DEC_TILE + 1 + TPB - 1) / TPB;
+    uint32_t l[IT];
+    uint32_t s = 0;
+    #pragma unroll
+    for (int k = 0; k < IT; ++k) {
+        const uint32_t j = threadIdx.x * IT + k;
+        l[k] = j < cnt ? word_len(sw[j]) : 0u;
+        s += l[k];
+    }
+    uint32_t tot;
+    uint32_t ex = block_excl_sum<TPB>(s, shs, &tot) + p0;
+    #pragma unroll
+    for (int k = 0; k < IT; ++k) {
+        const uint32_t j = threadIdx.x * IT + k;
+        if (j <= cnt) sp[j] = ex;
+        ex += l[k];
+    }
+    __syncthreads();
+    for (uint32_t g = g0 + threadIdx.x; g < g1; g += TPB) {
+        const uint32_t j = last_le(sp, cnt, g);
+        const word_t x = sw[j];
+        word_t v = group_value(x);
+        out[g] = neg ? (v ^ ALLONES) : v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(TPB) void k_flip(word_t* w, uint32_t n) {
+    for (uint32_t i = blockIdx.x * TPB + threadIdx.x; i < n; i += gridDim.x * TPB) {
+        const word_t x = w[i];
+        w[i] = x > ALLONES ? (x ^ FILLBIT) : (x ^ ALLONES);
+    }
+}
+
+// Zero fill for scratch (graph-capture friendly replacement of memset).
+__global__ __launch_bounds__(TPB) void k_zero_u32(uint32_t* p, uint32_t n) {
+    for (uint32_t i = blockIdx.x * TPB + threadIdx.x; i < n; i += gridDim.x * TPB) p[i] = 0u;
+}
+
+__global__ __launch_bounds__(TPB) void k_zero_u64(uint64_t* p, uint32_t n) {
+    for (uint32_t i = blockIdx.x * TPB + threadIdx.x; i < n; i += gridDim.x * TPB) p[i] = 0ull;
+}
+
+} // namespace fbk
+
+// ---------------------------------------------------------------------------
+// Set-bit positions from a literal word array: one thread per word writes
+// the positions of its set bits at its exclusive popcount offset.
+namespace fbk {
+__global__ __launch_bounds__(TPB) void k_popc(const word_t* __restrict__ lit, uint32_t n,
+                                              uint32_t* __restrict__ cnt) {
+    const uint32_t i = blockIdx.x * TPB + threadIdx.x;
+    if (i < n) cnt[i] = __popc(lit[i]);
+}
+
+__global__ __launch_bounds__(TPB) void k_rids_lit(const word_t* __restrict__ lit, uint32_t n,
+                                                  const uint32_t* __restrict__ off,
+                                                  uint32_t* __restrict__ out, uint32_t cap) {
+    const uint32_t i = blockIdx.x * TPB + threadIdx.x;
+    if (i >= n) return;
+    word_t w = lit[i];
+    uint32_t o = off[i];
+    const uint32_t pos0 = i * 31u;
+    while (w) {
+        const uint32_t hb = 31u - __clz(w);          // highest set bit
+        if (o < cap) out[o] = pos0 + (30u - hb);
+        ++o;
+        w &= ~(1u << hb);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers.
+static inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_u32, dim3(min(cdiv(n, TPB), 4096u)), dim3(TPB), 0, s, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_u64(uint64_t* p, uint32_t n, hipStream_t s) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_u64, dim3(min(cdiv(n, TPB), 4096u)), dim3(TPB), 0, s, p, n);
+    return hipGetLastError();
+}
+
+// Exclusive scan; tmp needs scan_tmp_words(n) words.
+uint32_t scan_tmp_words(uint32_t n) {
+    uint32_t t = 0;
+    while (n > (uint32_t)SCAN_TILE) { n = cdiv(n, SCAN_TILE); t += 2 * n + 1; }
+    return t + 1;
+}
+
+hipError_t launch_scan_excl(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp,
+                            hipStream_t s) {
+    if (!n) return hipSuccess;
+    const uint32_t nt = cdiv(n, SCAN_TILE);
+    if (nt == 1) {
+        hipLaunchKernelGGL(k_scan_tile, dim3(1), dim3(TPB), 0, s, in, out, n, (uint32_t*)nullptr);
+        return hipGetLastError();
+    }
+    uint32_t* sums = tmp;
+    uint32_t* offs = tmp + nt;
+    hipLaunchKernelGGL(k_scan_tile, dim3(nt), dim3(TPB), 0, s, in, out, n, sums);
+    hipError_t e = launch_scan_excl(sums, offs, nt, tmp + 2 * nt + 1, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_scan_add, dim3(cdiv(n, TPB)), dim3(TPB), 0, s, out, n, offs);
+    return hipGetLastError();
+}
+
+// Chunk index ck[0..nchunks] of a word array (nchunks = ceil(n/32)).
+// tmp needs nchunks + scan_tmp_words(nchunks) words.
+hipError_t launch_chunk_index(const word_t* w, uint32_t n, uint32_t* ck, uint32_t* tmp,
+                              hipStream_t s) {
+    const uint32_t nc = cdiv(n, CHUNK);
+    if (nc == 0) {
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(TPB), 0, s, ck, 1u);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL(k_chunk_len, dim3(cdiv(nc, TPB)), dim3(TPB), 0, s, w, n, tmp, nc);
+    hipError_t e = launch_scan_excl(tmp, ck, nc, tmp + nc, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_ck_sentinel, dim3(1), dim3(1), 0, s, ck, nc, (const uint32_t*)tmp);
+    return hipGetLastError();
+}
+
+uint32_t op_tiles(uint32_t na, uint32_t nb) {
+    const uint32_t items = cdiv(na, CHUNK) + cdiv(nb, CHUNK);
+    return items ? cdiv(items, KTILE) : 1u;
+}
+
+hipError_t launch_op(const word_t* a, const uint32_t* cka, uint32_t na,
+                     const word_t* b, const uint32_t* ckb, uint32_t nb,
+                     uint32_t groups, int op, int canon, word_t* out, uint32_t* outck,
+                     uint64_t* status, uint32_t* ctl, hipStream_t s) {
+    OpArgs p;
+    p.a = a; p.cka = cka; p.na = na; p.nca = cdiv(na, CHUNK);
+    p.b = b; p.ckb = ckb; p.nb = nb; p.ncb = cdiv(nb, CHUNK);
+    p.groups = groups; p.op = op; p.canon = canon ? 1u : 0u;
+    p.out = out; p.outck = outck; p.status = status;
+    p.ticket = ctl; p.total = ctl + 1;
+    const uint32_t nt = op_tiles(na, nb);
+    hipLaunchKernelGGL(k_zero_u64, dim3(min(cdiv(nt, TPB), 4096u)), dim3(TPB), 0, s, status, nt);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(TPB), 0, s, ctl, 8u);
+    hipLaunchKernelGGL(k_op_tiles, dim3(nt), dim3(TPB), 0, s, p);
+    return hipGetLastError();
+}
+
+// ctl[4] = out[0], ctl[5] = out[ctl[1]-1] (first and last result word).
+__global__ void k_edge_words(const word_t* out, uint32_t* ctl) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const uint32_t n = ctl[1];
+        ctl[4] = n ? out[0] : 0u;
+        ctl[5] = n ? out[n - 1] : 0u;
+    }
+}
+
+hipError_t launch_edge_words(const word_t* out, uint32_t* ctl, hipStream_t s) {
+    hipLaunchKernelGGL(k_edge_words, dim3(1), dim3(64), 0, s, out, ctl);
+    return hipGetLastError();
+}
+
+uint32_t tok_tiles(uint32_t n) { return n ? cdiv(n, TOK_TILE) : 1u; }
+
+// mode 0: compress `a` (n code words, `groups` groups); mode 1: canonical
+// WAH of apply_op(a[i], b[i]) over n literal words; mode 2: canonical WAH of
+// the literal words a[0..n).  status needs 2*tok_tiles(n) entries.
+hipError_t launch_tok(const word_t* a, const word_t* b, uint32_t n, uint32_t groups, int op,
+                      int mode, word_t* out, uint32_t* outck, uint64_t* status, uint32_t* ctl,
+                      hipStream_t s) {
+    TokArgs p;
+    p.a = a; p.b = b; p.n = n; p.groups = groups; p.op = op; p.mode = mode;
+    p.out = out; p.outck = outck; p.status = status; p.ticket = ctl; p.total = ctl + 1;
+    const uint32_t nt = tok_tiles(n);
+    hipLaunchKernelGGL(k_zero_u64, dim3(min(cdiv(2 * nt, TPB), 4096u)), dim3(TPB), 0, s, status,
+                       2 * nt);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(TPB), 0, s, ctl, 8u);
+    hipLaunchKernelGGL(k_tok_tiles, dim3(nt), dim3(TPB), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_cnt(const word_t* w, uint32_t n, uint32_t* out2, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(TPB), 0, s, out2, 2u);
+    if (n) {
+        const uint32_t g = min(cdiv(n, TPB * 8), 2048u);
+        hipLaunchKernelGGL(k_cnt, dim3(g), dim3(TPB), 0, s, w, n, out2);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_decompress(const word_t* w, const uint32_t* ck, uint32_t n, uint32_t groups,
+                             word_t* out, int neg, hipStream_t s) {
+    if (!groups) return hipSuccess;
+    hipLaunchKernelGGL(k_decompress, dim3(cdiv(groups, DEC_TILE)), dim3(TPB), 0, s, w, ck, n,
+                       cdiv(n, CHUNK), groups, out, neg);
+    return hipGetLastError();
+}
+
+hipError_t launch_flip(word_t* w, uint32_t n, hipStream_t s) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_flip, dim3(min(cdiv(n, TPB), 4096u)), dim3(TPB), 0, s, w, n);
+    return hipGetLastError();
+}
+
+// rids of a literal array; tmp needs 2*n + scan_tmp_words(n) words.
+hipError_t launch_rids_lit(const word_t* lit, uint32_t n, uint32_t* out, uint32_t cap,
+                           uint32_t* tmp, hipStream_t s) {
+    if (!n) return hipSuccess;
+    uint32_t* cnt = tmp;
+    uint32_t* off = tmp + n;
+    hipLaunchKernelGGL(k_popc, dim3(cdiv(n, TPB)), dim3(TPB), 0, s, lit, n, cnt);
+    hipError_t e = launch_scan_excl(cnt, off, n, tmp + 2 * n, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_rids_lit, dim3(cdiv(n, TPB)), dim3(TPB), 0, s, lit, n,
+                       (const uint32_t*)off, out, cap);
+    return hipGetLastError();
+}
+
+} // namespace fbk

@@ -1,0 +1,192 @@
+"""ctypes binding of libfastbit_hip.so (include/fastbit_hip.h).
+
+The product path: every call goes to the HIP library; if the library is not
+built or no GPU is present, calls raise.  There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libfastbit_hip.so")
+
+AND, OR, XOR, MINUS = 0, 1, 2, 3
+PATHS = {1: "c0", 2: "d1", 3: "d1swap", 4: "c1", 5: "d2", 6: "c2", 7: "shortcut"}
+
+_u32p = C.POINTER(C.c_uint32)
+_lib = None
+
+
+class FastbitError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load the HIP library (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FastbitError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.fbh_last_error.restype = C.c_char_p
+        L.fbh_bv_free.restype = None
+        L.fbh_bv_free.argtypes = [vp]
+        for name, args in {
+            "fbh_init": [C.c_int],
+            "fbh_sync": [],
+            "fbh_bv_new": [C.POINTER(vp)],
+            "fbh_bv_from_serial": [_u32p, C.c_uint32, C.POINTER(vp)],
+            "fbh_bv_set": [vp, C.c_int, C.c_uint32],
+            "fbh_bv_copy": [vp, C.POINTER(vp)],
+            "fbh_bv_write": [vp, _u32p, C.c_uint32, _u32p],
+            "fbh_bv_op": [vp, vp, C.c_int, C.POINTER(C.c_int)],
+            "fbh_bv_op_new": [vp, vp, C.c_int, C.POINTER(vp), C.POINTER(C.c_int)],
+            "fbh_bv_compress": [vp],
+            "fbh_bv_decompress": [vp],
+            "fbh_bv_flip": [vp],
+            "fbh_bv_cnt": [vp, _u32p],
+            "fbh_bv_size": [vp, _u32p],
+            "fbh_bv_adjust_size": [vp, C.c_uint32, C.c_uint32],
+            "fbh_bv_rids": [vp, _u32p, C.c_uint32, _u32p],
+            "fbh_bv_nwords": [vp, _u32p],
+            "fbh_bench_op": [vp, vp, C.c_int, C.c_int, C.POINTER(C.c_float), _u32p],
+        }.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str = ""):
+    if rc != 0:
+        msg = lib().fbh_last_error()
+        raise FastbitError(f"{what} failed (rc={rc}): {msg.decode() if msg else ''}")
+
+
+def _u32(a):
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    return a, a.ctypes.data_as(_u32p)
+
+
+class Bitvector:
+    """An ibis::bitvector held in HBM (src/bitvector.h:62-318).
+
+    The in-place operators are the reference's ``a op= b`` followed by
+    ``compress()``; ``last_path`` records the routine the dispatch chose.
+    """
+
+    __slots__ = ("_h", "last_path")
+
+    def __init__(self, handle=None):
+        if handle is None:
+            h = C.c_void_p()
+            check(lib().fbh_bv_new(C.byref(h)), "fbh_bv_new")
+            handle = h
+        self._h = handle
+        self.last_path = None
+
+    @classmethod
+    def from_serial(cls, words) -> "Bitvector":
+        a, p = _u32(words)
+        h = C.c_void_p()
+        check(lib().fbh_bv_from_serial(p, a.size, C.byref(h)), "fbh_bv_from_serial")
+        return cls(h)
+
+    def __del__(self):
+        try:
+            if self._h is not None and _lib is not None:
+                _lib.fbh_bv_free(self._h)
+        except Exception:
+            pass
+        self._h = None
+
+    def copy(self) -> "Bitvector":
+        h = C.c_void_p()
+        check(lib().fbh_bv_copy(self._h, C.byref(h)), "fbh_bv_copy")
+        return Bitvector(h)
+
+    def set(self, val: int, n: int) -> None:
+        check(lib().fbh_bv_set(self._h, int(val), int(n)), "fbh_bv_set")
+
+    def write(self) -> np.ndarray:
+        """Serialized words (write(array_t&), src/bitvector.cpp:2283-2313)."""
+        n = C.c_uint32()
+        check(lib().fbh_bv_write(self._h, None, 0, C.byref(n)), "fbh_bv_write")
+        out = np.zeros(n.value, dtype=np.uint32)
+        check(lib().fbh_bv_write(self._h, out.ctypes.data_as(_u32p), n.value, C.byref(n)),
+              "fbh_bv_write")
+        return out
+
+    serial = write
+
+    def op(self, other: "Bitvector", op: int) -> str:
+        path = C.c_int(0)
+        check(lib().fbh_bv_op(self._h, other._h, int(op), C.byref(path)), "fbh_bv_op")
+        self.last_path = PATHS.get(path.value, "?")
+        return self.last_path
+
+    def __iand__(self, o):
+        self.op(o, AND)
+        return self
+
+    def __ior__(self, o):
+        self.op(o, OR)
+        return self
+
+    def __ixor__(self, o):
+        self.op(o, XOR)
+        return self
+
+    def __isub__(self, o):
+        self.op(o, MINUS)
+        return self
+
+    def compress(self):
+        check(lib().fbh_bv_compress(self._h), "fbh_bv_compress")
+
+    def decompress(self):
+        check(lib().fbh_bv_decompress(self._h), "fbh_bv_decompress")
+
+    def flip(self):
+        check(lib().fbh_bv_flip(self._h), "fbh_bv_flip")
+
+    def cnt(self) -> int:
+        v = C.c_uint32()
+        check(lib().fbh_bv_cnt(self._h, C.byref(v)), "fbh_bv_cnt")
+        return v.value
+
+    def size(self) -> int:
+        v = C.c_uint32()
+        check(lib().fbh_bv_size(self._h, C.byref(v)), "fbh_bv_size")
+        return v.value
+
+    def adjustSize(self, nv: int, nt: int):
+        check(lib().fbh_bv_adjust_size(self._h, int(nv), int(nt)), "fbh_bv_adjust_size")
+
+    @property
+    def nwords(self) -> int:
+        v = C.c_uint32()
+        check(lib().fbh_bv_nwords(self._h, C.byref(v)), "fbh_bv_nwords")
+        return v.value
+
+    def rids(self) -> np.ndarray:
+        n = C.c_uint32()
+        check(lib().fbh_bv_rids(self._h, None, 0, C.byref(n)), "fbh_bv_rids")
+        out = np.zeros(max(n.value, 1), dtype=np.uint32)
+        check(lib().fbh_bv_rids(self._h, out.ctypes.data_as(_u32p), n.value, C.byref(n)),
+              "fbh_bv_rids")
+        return out[: n.value]
+
+
+def bench_op(a: Bitvector, b: Bitvector, op: int, reps: int):
+    """(ms per op, result words) for (copy of a) op= b + compress."""
+    ms = C.c_float()
+    nw = C.c_uint32()
+    check(lib().fbh_bench_op(a._h, b._h, int(op), int(reps), C.byref(ms), C.byref(nw)),
+          "fbh_bench_op")
+    return ms.value, nw.value

@@ -1,0 +1,93 @@
+"""The partitioned step algorithm that the gfx950 op kernel runs, exercised
+on the CPU (libfbsim.so compiles the same host/device core): arbitrary cut
+points, resynchronisation at common boundaries, walk, run-summary stitching,
+word write.  Checked against the oracle's *_c2 routines and against the
+canonical compression of the literal op."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import markov_bits, random_pair, split_serial, groups_of
+
+SIM = os.path.join(os.path.dirname(os.path.dirname(__file__)), "libfastlib_amd", "lib", "libfbsim.so")
+u32p = C.POINTER(C.c_uint32)
+
+
+@pytest.fixture(scope="module")
+def sim():
+    if not os.path.exists(SIM):
+        pytest.skip("libfbsim.so not built")
+    L = C.CDLL(SIM)
+    L.fbsim_op.restype = C.c_long
+    return L
+
+
+def run_sim(sim, a, b, op, canon, splits):
+    wa, av, an = split_serial(a)
+    wb, bv, bn = split_serial(b)
+    g = groups_of(wa)
+    cap = len(wa) + len(wb) + 8
+    out = np.zeros(cap, np.uint32)
+    hd = np.zeros(cap, np.uint32)
+    sp = np.asarray(splits, np.uint32)
+    wa = np.ascontiguousarray(wa)
+    wb = np.ascontiguousarray(wb)
+    n = sim.fbsim_op(wa.ctypes.data_as(u32p), len(wa), wb.ctypes.data_as(u32p), len(wb), g, op,
+                     canon, sp.ctypes.data_as(u32p), len(sp), out.ctypes.data_as(u32p),
+                     hd.ctypes.data_as(u32p), cap)
+    assert n >= 0, n
+    act = [av & bv, av | bv, av ^ bv, av & ~bv & 0xFFFFFFFF][op]
+    res = list(out[:n]) + ([act] if an else []) + [an]
+    return np.array(res, np.uint32), hd[:n]
+
+
+def c2_oracle(orc, a, b, op):
+    pa, na = O._arr(a)[1:]
+    pb, nb = O._arr(b)[1:]
+    return orc._call_out("wah_c2", pa, na, pb, nb, C.c_int(op))
+
+
+def test_sim_c2_matches_oracle(sim, orc):
+    rng = np.random.default_rng(21)
+    for t in range(120):
+        a, b, n = random_pair(rng, orc)
+        if t % 3 == 0:
+            a = orc.op(a, orc.from_bits(markov_bits(n, 0.1, 7, rng)), 1, compress=False)[0]
+            a = orc.compress(a) if t % 2 else a
+        g = groups_of(split_serial(a)[0])
+        for op in range(4):
+            splits = rng.integers(0, g + 1, size=int(rng.integers(0, 25)))
+            got, heads = run_sim(sim, a, b, op, 0, splits)
+            assert np.array_equal(got, c2_oracle(orc, a, b, op)), (t, op)
+            # heads are the start groups of the result words
+            words = split_serial(got)[0]
+            lens = np.where(words > 0x7FFFFFFF, words & 0x3FFFFFFF, 1)
+            assert np.array_equal(heads, np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.uint32))
+
+
+def test_sim_canonical_matches_compressed_literal_op(sim, orc):
+    rng = np.random.default_rng(22)
+    for t in range(80):
+        a, b, n = random_pair(rng, orc)
+        g = groups_of(split_serial(a)[0])
+        for op in range(4):
+            lit = orc.op(orc.decompress(a), orc.decompress(b), op, compress=False)[0]
+            want = orc.compress(lit)
+            splits = rng.integers(0, g + 1, size=int(rng.integers(0, 25)))
+            got, _ = run_sim(sim, a, b, op, 1, splits)
+            assert np.array_equal(got, want), (t, op)
+
+
+def test_sim_every_cut_point(sim, orc):
+    """Cut at every single group of a small pair."""
+    rng = np.random.default_rng(23)
+    for _ in range(6):
+        a = orc.from_bits(markov_bits(620, 0.3, 40, rng))
+        b = orc.from_bits(markov_bits(620, 0.05, 3, rng))
+        g = groups_of(split_serial(a)[0])
+        for op in range(4):
+            got, _ = run_sim(sim, a, b, op, 0, np.arange(g + 1))
+            assert np.array_equal(got, c2_oracle(orc, a, b, op))

@@ -1,0 +1,162 @@
+"""WAH bitvector operators on the MI355X, bit-exact against the oracle and
+the reference's golden words."""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from helpers import markov_bits, random_pair, split_serial
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module")
+def L():
+    import libfastlib_amd as L
+    L.lib()
+    return L
+
+
+def gpu_op(L, a, b, op):
+    x = L.Bitvector.from_serial(a)
+    y = L.Bitvector.from_serial(b)
+    path = x.op(y, op)
+    return x.write(), path
+
+
+def test_random_ops_match_oracle(L, orc):
+    rng = np.random.default_rng(31)
+    seen = set()
+    for t in range(120):
+        a, b, n = random_pair(rng, orc)
+        if t % 4 == 0:
+            a = orc.op(a, orc.from_bits(markov_bits(n, 0.1, 7, rng)), 1, compress=False)[0]
+        for op in range(4):
+            want, wpath = orc.op(a, b, op)
+            got, gpath = gpu_op(L, a, b, op)
+            assert gpath == wpath, (t, op)
+            assert np.array_equal(got, want), (t, op, gpath, len(got), len(want))
+            seen.add(gpath)
+    assert {"c2", "d2", "c0"} <= seen
+
+
+def test_literal_operand_paths(L, orc):
+    rng = np.random.default_rng(32)
+    for _ in range(30):
+        n = int(rng.integers(62, 5000))
+        a = orc.decompress(orc.from_bits(markov_bits(n, 0.5, 1.5, rng)))
+        b = orc.from_bits(markov_bits(n, 0.01, 3, rng))
+        for x, y in ((a, b), (b, a), (a, a)):
+            for op in range(4):
+                want, wpath = orc.op(x, y, op)
+                got, gpath = gpu_op(L, x, y, op)
+                assert gpath == wpath
+                assert np.array_equal(got, want), (op, gpath)
+
+
+def test_golden_ops(L):
+    g = np.load(os.path.join(GOLDEN, "wah_ops.npz"))
+    for i in range(int(g["ncase"])):
+        for op in range(4):
+            got, _ = gpu_op(L, g[f"a{i}"], g[f"b{i}"], op)
+            assert np.array_equal(got, g[f"r{i}_{op}"]), (i, op)
+
+
+def test_edge_cases(L, orc):
+    cases = []
+    for n in (1, 30, 31, 32, 61, 62, 63, 93, 124):
+        for bits in (np.zeros(n, np.uint8), np.ones(n, np.uint8),
+                     (np.arange(n) % 2).astype(np.uint8), (np.arange(n) % 37 == 0).astype(np.uint8)):
+            cases.append(orc.from_bits(bits))
+    for i, a in enumerate(cases):
+        for b in cases[i - (i % 4): i - (i % 4) + 4]:
+            for op in range(4):
+                want, wp = orc.op(a, b, op)
+                got, gp = gpu_op(L, a, b, op)
+                assert gp == wp and np.array_equal(got, want), (i, op, wp)
+
+
+def test_unequal_sizes_extend(L, orc):
+    rng = np.random.default_rng(33)
+    for _ in range(10):
+        a = orc.from_bits(markov_bits(int(rng.integers(40, 900)), 0.3, 5, rng))
+        b = orc.from_bits(markov_bits(int(rng.integers(900, 2000)), 0.3, 5, rng))
+        for x, y in ((a, b), (b, a)):
+            for op in range(4):
+                want, _ = orc.op(x, y, op)
+                got, _ = gpu_op(L, x, y, op)
+                assert np.array_equal(got, want)
+
+
+def test_unary_ops(L, orc):
+    rng = np.random.default_rng(34)
+    for t in range(40):
+        a, b, n = random_pair(rng, orc)
+        u = orc.op(a, b, int(rng.integers(0, 4)), compress=False)[0]   # non-canonical words
+        x = L.Bitvector.from_serial(u)
+        c, s, after = orc.cnt(u)
+        assert x.size() == s and x.cnt() == c
+        assert np.array_equal(x.write(), after)
+        x = L.Bitvector.from_serial(u)
+        x.compress()
+        assert np.array_equal(x.write(), orc.compress(u))
+        x = L.Bitvector.from_serial(u)
+        x.decompress()
+        assert np.array_equal(x.write(), orc.decompress(u))
+        x = L.Bitvector.from_serial(u)
+        x.size()          # the oracle's flip entry counts first (do_cnt may collapse)
+        x.flip()
+        assert np.array_equal(x.write(), orc.flip(u))
+        x = L.Bitvector.from_serial(u)
+        assert np.array_equal(x.rids(), orc.rids(u))
+
+
+def test_set_and_adjust(L, orc):
+    for val in (0, 1):
+        for n in (0, 5, 31, 62, 100, 1000):
+            x = L.Bitvector()
+            x.set(val, n)
+            bits = np.full(n, val, np.uint8)
+            assert np.array_equal(x.write(), orc.from_bits(bits))
+            x.adjustSize(n + 40, n + 100)
+            bits2 = np.concatenate([bits, np.ones(40, np.uint8), np.zeros(60, np.uint8)])
+            assert np.array_equal(x.write(), orc.from_bits(bits2))
+
+
+def _digest_rows(maxbits):
+    with open(os.path.join(GOLDEN, "wah_config2_digests.json")) as f:
+        return [r for r in json.load(f)["rows"] if r["nbits"] <= maxbits]
+
+
+def _check_rows(L, rows):
+    from libfastlib_amd import datagen as G
+    cache = {}
+    for r in rows:
+        key = (r["seed_a"], r["seed_b"])
+        if key not in cache:
+            cache.clear()
+            a = G.markov_wah(r["seed_a"], r["nbits"], r["density"], r["mean_run"])
+            b = G.markov_wah(r["seed_b"], r["nbits"], r["density"], r["mean_run"])
+            assert [len(a), len(b)] == r["in_words"]
+            cache[key] = (L.Bitvector.from_serial(a), L.Bitvector.from_serial(b))
+        x, y = cache[key]
+        z = x.copy()
+        path = z.op(y, r["op"])
+        out = z.write()
+        assert path == r["path"], r
+        assert len(out) == r["words"], r
+        assert hashlib.sha256(out.tobytes()).hexdigest() == r["sha256"], r
+        assert z.cnt() == r["cnt"], r
+
+
+def test_config2_digests_2p24(L):
+    _check_rows(L, _digest_rows(1 << 24))
+
+
+def test_config2_digests_2p28(L):
+    """All nine Markov cells at 2^28 bits, four operators each: identical
+    words to the reference after compress()."""
+    _check_rows(L, [r for r in _digest_rows(1 << 28) if r["nbits"] == (1 << 28)])
