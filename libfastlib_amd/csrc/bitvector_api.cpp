@@ -305,42 +305,46 @@ int bv_compress(fbh_bitvector* bv) {
     rc = check(fbk::launch_tok(bv->d, nullptr, bv->nw, 0, 0, 0, ob.d, ob.ck, status, ctl,
                                stream()), "compress");
     if (rc) return rc;
-    return finish_words(bv, ob, ctl, nullptr);
+    rc = finish_words(bv, ob, ctl, nullptr);
+    bv->ck_ok = false;      // the compress kernel does not track group positions
+    return rc;
 }
 
 // Canonical WAH (or the exact c2 words) of a OP b into `res` (a, b have
 // equal group counts; `groups` > 0).
-int run_word_op(const fbh_bitvector* a, const fbh_bitvector* b, int op, bool c2,
-                fbh_bitvector* res, uint32_t groups) {
+int launch_word_op(fbh_bitvector* a, fbh_bitvector* b, int op, bool c2, uint32_t groups,
+                   OutBuf& ob, uint32_t** ctl_out) {
     const bool la = a->nw == groups;
     const bool lb = b->nw == groups;
+    int rc;
+    if (!la || c2) { rc = bv_ensure_ck(a); if (rc) return rc; }
+    if (!lb || c2) { rc = bv_ensure_ck(b); if (rc) return rc; }
+    if (c2) {
+        const uint32_t nt = fbk::op_tiles(a->nw, b->nw);
+        uint8_t* sc = (uint8_t*)scratch(0).get(64 + 8ull * nt + 64);
+        if (!sc) return fail(FBH_ERR_NOMEM, "scratch allocation failed");
+        *ctl_out = (uint32_t*)sc;
+        return check(fbk::launch_op(a->d, a->ck, a->nw, b->d, b->ck, b->nw, groups, op, 0,
+                                    ob.d, ob.ck, (uint64_t*)(sc + 64), *ctl_out, stream()), "op");
+    }
+    const uint32_t nt = fbk::canon_tiles(groups);
+    uint8_t* sc = (uint8_t*)scratch(0).get(64 + 8ull * nt + 64);
+    if (!sc) return fail(FBH_ERR_NOMEM, "scratch allocation failed");
+    *ctl_out = (uint32_t*)sc;
+    return check(fbk::launch_canon(a->d, a->ck, a->nw, la ? 1 : 0, b->d, b->ck, b->nw, lb ? 1 : 0,
+                                   groups, op, ob.d, ob.ck, (uint64_t*)(sc + 64), *ctl_out,
+                                   stream()), "canonical op");
+}
+
+int run_word_op(const fbh_bitvector* a, const fbh_bitvector* b, int op, bool c2,
+                fbh_bitvector* res, uint32_t groups) {
     OutBuf ob;
     int rc = ob.alloc(a->nw + b->nw);
     if (rc) return rc;
-    if (!c2 && la && lb) {
-        // c0: both literal-only, one group per word
-        const uint32_t nt = fbk::tok_tiles(groups);
-        uint8_t* sc = (uint8_t*)scratch(0).get(64 + 16ull * nt + 64);
-        if (!sc) return fail(FBH_ERR_NOMEM, "scratch allocation failed");
-        uint32_t* ctl = (uint32_t*)sc;
-        uint64_t* status = (uint64_t*)(sc + 64);
-        rc = check(fbk::launch_tok(a->d, b->d, groups, groups, op, 1, ob.d, ob.ck, status, ctl,
-                                   stream()), "c0 op");
-        if (rc) return rc;
-        return finish_words(res, ob, ctl, nullptr);
-    }
-    rc = bv_ensure_ck(const_cast<fbh_bitvector*>(a));
-    if (rc) return rc;
-    rc = bv_ensure_ck(const_cast<fbh_bitvector*>(b));
-    if (rc) return rc;
-    const uint32_t nt = fbk::op_tiles(a->nw, b->nw);
-    uint8_t* sc = (uint8_t*)scratch(0).get(64 + 8ull * nt + 64);
-    if (!sc) return fail(FBH_ERR_NOMEM, "scratch allocation failed");
-    uint32_t* ctl = (uint32_t*)sc;
-    uint64_t* status = (uint64_t*)(sc + 64);
-    rc = check(fbk::launch_op(a->d, a->ck, a->nw, b->d, b->ck, b->nw, groups, op, c2 ? 0 : 1,
-                              ob.d, ob.ck, status, ctl, stream()), "op");
-    if (rc) return rc;
+    uint32_t* ctl = nullptr;
+    rc = launch_word_op(const_cast<fbh_bitvector*>(a), const_cast<fbh_bitvector*>(b), op, c2,
+                        groups, ob, &ctl);
+    if (rc) { dev_free(ob.d); dev_free(ob.ck); return rc; }
     return finish_words(res, ob, ctl, nullptr);
 }
 
@@ -736,29 +740,16 @@ int fbh_bench_op(const fbh_bitvector* a, const fbh_bitvector* b, int op, int rep
     fbh_bitvector* mb = const_cast<fbh_bitvector*>(b);
     const uint32_t groups = ma->nbits / MAXBITS;
     const bool c2 = path == FBH_PATH_C2;
-    const bool lit2 = !c2 && ma->nw == groups && mb->nw == groups;
-    if (!lit2) {
-        rc = bv_ensure_ck(ma); if (rc) return rc;
-        rc = bv_ensure_ck(mb); if (rc) return rc;
-    }
     OutBuf ob;
     rc = ob.alloc(ma->nw + mb->nw);
     if (rc) return rc;
-    const uint32_t nt = lit2 ? fbk::tok_tiles(groups) : fbk::op_tiles(ma->nw, mb->nw);
-    uint8_t* sc = (uint8_t*)scratch(0).get(64 + 16ull * nt + 64);
-    uint32_t* ctl = (uint32_t*)sc;
-    uint64_t* status = (uint64_t*)(sc + 64);
+    uint32_t* ctl = nullptr;
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0); (void)hipEventCreate(&e1);
-    auto one = [&]() {
-        if (lit2)
-            return fbk::launch_tok(ma->d, mb->d, groups, groups, op, 1, ob.d, ob.ck, status, ctl, stream());
-        return fbk::launch_op(ma->d, ma->ck, ma->nw, mb->d, mb->ck, mb->nw, groups, op, c2 ? 0 : 1,
-                              ob.d, ob.ck, status, ctl, stream());
-    };
-    rc = check(one(), "bench warmup");
+    auto one = [&]() { return launch_word_op(ma, mb, op, c2, groups, ob, &ctl); };
+    rc = one();
     if (!rc) rc = check(hipEventRecord(e0, stream()), "event");
-    for (int i = 0; i < reps && !rc; ++i) rc = check(one(), "bench op");
+    for (int i = 0; i < reps && !rc; ++i) rc = one();
     if (!rc) rc = check(hipEventRecord(e1, stream()), "event");
     if (!rc) rc = check(hipEventSynchronize(e1), "event sync");
     float t = 0.f;

@@ -126,4 +126,167 @@ __device__ __forceinline__ uint32_t last_le(const T* a, uint32_t n, uint32_t key
     return lo;
 }
 
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+    for (int d = 32; d >= 1; d >>= 1) {
+        const uint32_t o = (uint32_t)__shfl_xor((int)v, d, 64);
+        v = v > o ? v : o;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
+
+// ---------------------------------------------------------------------------
+// Run summaries across a wavefront (shuffles; lane order = position order).
+__device__ __forceinline__ wah::Summary shfl_summary_up(const wah::Summary& x, int d) {
+    wah::Summary r;
+    r.count = __shfl_up(x.count, d, 64);
+    r.tlen = __shfl_up(x.tlen, d, 64);
+    r.first = __shfl_up(x.first, d, 64);
+    r.trail = __shfl_up(x.trail, d, 64);
+    r.uniform = __shfl_up(x.uniform, d, 64);
+    r.trep = __shfl_up(x.trep, d, 64);
+    return r;
+}
+
+__device__ __forceinline__ wah::Summary shfl_summary_down(const wah::Summary& x, int d) {
+    wah::Summary r;
+    r.count = __shfl_down(x.count, d, 64);
+    r.tlen = __shfl_down(x.tlen, d, 64);
+    r.first = __shfl_down(x.first, d, 64);
+    r.trail = __shfl_down(x.trail, d, 64);
+    r.uniform = __shfl_down(x.uniform, d, 64);
+    r.trep = __shfl_down(x.trep, d, 64);
+    return r;
+}
+
+__device__ __forceinline__ wah::Summary shfl_summary_idx(const wah::Summary& x, int src) {
+    wah::Summary r;
+    r.count = __shfl(x.count, src, 64);
+    r.tlen = __shfl(x.tlen, src, 64);
+    r.first = __shfl(x.first, src, 64);
+    r.trail = __shfl(x.trail, src, 64);
+    r.uniform = __shfl(x.uniform, src, 64);
+    r.trep = __shfl(x.trep, src, 64);
+    return r;
+}
+
+// Inclusive scan of summaries over the lanes of a wavefront.
+__device__ __forceinline__ wah::Summary wave_incl_summary(wah::Summary x) {
+    const int lane = (int)lane_id();
+    #pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const wah::Summary o = shfl_summary_up(x, d);
+        if (lane >= d) x = wah::combine(o, x);
+    }
+    return x;
+}
+
+// Exclusive block scan of summaries: wave scans + one LDS round.
+// `wsum` needs NT/64 Summary slots.
+template <int NT>
+__device__ __forceinline__ wah::Summary block_excl_summary2(const wah::Summary& v, wah::Summary* wsum,
+                                                            wah::Summary* agg) {
+    constexpr int NW = NT / WAVE;
+    const int lane = (int)lane_id();
+    const int w = threadIdx.x / WAVE;
+    const wah::Summary incl = wave_incl_summary(v);
+    wah::Summary excl = shfl_summary_up(incl, 1);
+    if (lane == 0) excl = wah::summary_empty();
+    if (lane == WAVE - 1) wsum[w] = incl;
+    __syncthreads();
+    wah::Summary pre = wah::summary_empty();
+    for (int i = 0; i < w; ++i) pre = wah::combine(pre, wsum[i]);
+    wah::Summary all = pre;
+    for (int i = w; i < NW; ++i) all = wah::combine(all, wsum[i]);
+    *agg = all;
+    __syncthreads();
+    return wah::combine(pre, excl);
+}
+
+// Wave-parallel look-back over run summaries.  Called by all 64 lanes of
+// one wavefront; lane k reads the status of tile (base - k).  Returns the
+// exclusive prefix (valid in every lane).
+__device__ __forceinline__ wah::Summary lookback_summary_wave(uint64_t* status, uint32_t tile,
+                                                              const wah::Summary& agg) {
+    const int lane = (int)lane_id();
+    if (tile == 0) {
+        if (lane == 0)
+            __hip_atomic_store(&status[0], wah::pack(agg, 2), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        return wah::summary_empty();
+    }
+    if (lane == 0)
+        __hip_atomic_store(&status[tile], wah::pack(agg, 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    wah::Summary acc = wah::summary_empty();
+    int64_t base = (int64_t)tile - 1;
+    for (;;) {
+        const int64_t j = base - lane;
+        uint64_t s = (j >= 0) ? __hip_atomic_load(&status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)
+                              : wah::pack(wah::summary_empty(), 2);
+        const uint32_t st = wah::status_of(s);
+        const uint64_t zero = ballot64(st == 0);
+        const uint64_t pref = ballot64(st == 2);
+        const uint64_t need = pref ? ((pref & (~pref + 1)) << 1) - 1 : ~0ull;
+        if (zero & need) { __builtin_amdgcn_s_sleep(1); continue; }
+        wah::Summary x = ((need >> lane) & 1ull) ? wah::unpack(s) : wah::summary_empty();
+        // combine far (high lane) to near (lane 0)
+        #pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const wah::Summary o = shfl_summary_down(x, d);
+            if (lane + d < 64) x = wah::combine(o, x);
+        }
+        const wah::Summary win = shfl_summary_idx(x, 0);
+        acc = wah::combine(win, acc);
+        if (pref) break;
+        base -= 64;
+    }
+    if (lane == 0)
+        __hip_atomic_store(&status[tile], wah::pack(wah::combine(acc, agg), 2), __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    return acc;
+}
+
+// (count, last head + 1) pairs for canonical compression: count adds,
+// last-head takes the maximum.  Packed [63:62] status, [61:31] count,
+// [30:0] last head + 1 (0 = none).
+__device__ __forceinline__ uint64_t pack_ch(uint32_t cnt, uint32_t lh1, uint64_t st) {
+    return (st << 62) | ((uint64_t)(cnt & 0x7FFFFFFFu) << 31) | (uint64_t)(lh1 & 0x7FFFFFFFu);
+}
+
+// Wave-parallel look-back for (count, last head): all 64 lanes of a wave.
+__device__ __forceinline__ void lookback_ch_wave(uint64_t* status, uint32_t tile, uint32_t acnt,
+                                                 uint32_t alh1, uint32_t& ecnt, uint32_t& elh1) {
+    const int lane = (int)lane_id();
+    if (tile == 0) {
+        if (lane == 0)
+            __hip_atomic_store(&status[0], pack_ch(acnt, alh1, 2), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        ecnt = 0; elh1 = 0;
+        return;
+    }
+    if (lane == 0)
+        __hip_atomic_store(&status[tile], pack_ch(acnt, alh1, 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t c = 0, h = 0;
+    int64_t base = (int64_t)tile - 1;
+    for (;;) {
+        const int64_t j = base - lane;
+        uint64_t s = (j >= 0) ? __hip_atomic_load(&status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)
+                              : pack_ch(0, 0, 2);
+        const uint32_t st = (uint32_t)(s >> 62);
+        const uint64_t zero = ballot64(st == 0);
+        const uint64_t pref = ballot64(st == 2);
+        const uint64_t need = pref ? ((pref & (~pref + 1)) << 1) - 1 : ~0ull;
+        if (zero & need) { __builtin_amdgcn_s_sleep(1); continue; }
+        const bool use = (need >> lane) & 1ull;
+        c += wave_sum(use ? (uint32_t)((s >> 31) & 0x7FFFFFFFu) : 0u);
+        const uint32_t hm = wave_max(use ? (uint32_t)(s & 0x7FFFFFFFu) : 0u);
+        h = h > hm ? h : hm;
+        if (pref) break;
+        base -= 64;
+    }
+    ecnt = c; elh1 = h;
+    if (lane == 0)
+        __hip_atomic_store(&status[tile], pack_ch(c + acnt, h > alh1 ? h : alh1, 2), __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+
 } // namespace fbk

@@ -31,4 +31,12 @@ hipError_t launch_decompress(const word_t* w, const uint32_t* ck, uint32_t n, ui
 hipError_t launch_flip(word_t* w, uint32_t n, hipStream_t s);
 hipError_t launch_rids_lit(const word_t* lit, uint32_t n, uint32_t* out, uint32_t cap,
                            uint32_t* tmp, hipStream_t s);
+// Canonical WAH of a OP b (b == nullptr: of a alone) group by group; an
+// operand with lit != 0 holds one literal word per group.  status needs
+// canon_tiles(groups) entries; ctl 8 words.
+uint32_t canon_tiles(uint32_t groups);
+hipError_t launch_canon(const word_t* a, const uint32_t* cka, uint32_t na, int alit,
+                        const word_t* b, const uint32_t* ckb, uint32_t nb, int blit,
+                        uint32_t groups, int op, word_t* out, uint32_t* outck, uint64_t* status,
+                        uint32_t* ctl, hipStream_t s);
 } // namespace fbk

@@ -155,7 +155,7 @@ __global__ __launch_bounds__(TPB) void k_op_tiles(OpArgs p) {
     __shared__ uint32_t posA[WIN + 1];
     __shared__ uint32_t posB[WIN + 1];
     __shared__ uint32_t tc[TPB + 1], tia[TPB + 1], tib[TPB + 1];
-    __shared__ Summary sbuf[2 * TPB];
+    __shared__ Summary sbuf[TPB / WAVE];
     __shared__ uint32_t info[16];
     __shared__ uint32_t shs[TPB / WAVE + 1];
     __shared__ Summary s_pre;
@@ -285,8 +285,11 @@ __global__ __launch_bounds__(TPB) void k_op_tiles(OpArgs p) {
     cs.init();
     if (c0 < c1) walk(cs, &ra, &rb, tia[tid], tib[tid], c0, c1, p.op, p.canon);
     Summary agg;
-    const Summary texcl = block_excl_summary<TPB>(cs.s, sbuf, &agg);
-    if (tid == 0) s_pre = lookback(p.status, tile, agg);
+    const Summary texcl = block_excl_summary2<TPB>(cs.s, sbuf, &agg);
+    if (tid < WAVE) {
+        const Summary pr = lookback_summary_wave(p.status, tile, agg);
+        if (tid == 0) s_pre = pr;
+    }
     __syncthreads();
     const Summary pre = combine(s_pre, texcl);
 
@@ -336,8 +339,7 @@ __device__ __forceinline__ void tok_item(S& sink, word_t w, int mode) {
 
 __global__ __launch_bounds__(TPB) void k_tok_tiles(TokArgs p) {
     __shared__ word_t sw[TOK_TILE];
-    __shared__ uint32_t sp[TOK_TILE + 1];
-    __shared__ Summary sbuf[2 * TPB];
+    __shared__ Summary sbuf[TPB / WAVE];
     __shared__ uint32_t shs[TPB / WAVE + 1];
     __shared__ uint32_t info[2];
     __shared__ Summary s_pre;
@@ -373,29 +375,10 @@ __global__ __launch_bounds__(TPB) void k_tok_tiles(TokArgs p) {
     uint32_t ltot;
     const uint32_t lpos = block_excl_sum<TPB>(mylen, shs, &ltot);
     Summary agg;
-    const Summary texcl = block_excl_summary<TPB>(cs.s, sbuf, &agg);
-    // positions need a prefix over tiles as well: fold the group count into
-    // the look-back by a second status array slot (status[ntiles + tile]).
-    if (tid == 0) {
-        s_pre = lookback(p.status, tile, agg);
-        // group offset of this tile: mode 1/2 -> base; mode 0 -> serial chain
-        if (p.mode == 0) {
-            uint64_t* gs = p.status + gridDim.x;
-            uint32_t off = 0;
-            if (tile > 0) {
-                uint64_t s;
-                do {
-                    s = __hip_atomic_load(&gs[tile - 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-                    if ((s >> 63) == 0) __builtin_amdgcn_s_sleep(1);
-                } while ((s >> 63) == 0);
-                off = (uint32_t)s;
-            }
-            __hip_atomic_store(&gs[tile], (1ull << 63) | (uint64_t)(off + ltot), __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_AGENT);
-            s_base_pos = off;
-        } else {
-            s_base_pos = base;
-        }
+    const Summary texcl = block_excl_summary2<TPB>(cs.s, sbuf, &agg);
+    if (tid < WAVE) {
+        const Summary pr = lookback_summary_wave(p.status, tile, agg);
+        if (tid == 0) { s_pre = pr; s_base_pos = (p.mode == 0) ? 0u : base; }
     }
     __syncthreads();
     const Summary pre = combine(s_pre, texcl);
@@ -417,7 +400,6 @@ __global__ __launch_bounds__(TPB) void k_tok_tiles(TokArgs p) {
     }
     uint32_t ones = wave_sum(gw.ones);
     if (lane_id() == 0 && ones) atomicAdd(&p.total[1], ones);
-    (void)sp;
 }
 
 // ---------------------------------------------------------------------------

@@ -160,3 +160,19 @@ def test_config2_digests_2p28(L):
     """All nine Markov cells at 2^28 bits, four operators each: identical
     words to the reference after compress()."""
     _check_rows(L, [r for r in _digest_rows(1 << 28) if r["nbits"] == (1 << 28)])
+
+
+def test_canonical_result_spanning_tiles(L, orc):
+    """A canonical (c0) result that is one run across many kernel tiles:
+    middle tiles own no output word (regression: their store count was
+    computed as 0 - 1)."""
+    n = 31 * 40000 + 7
+    ones = np.ones(n, np.uint8)
+    a = orc.decompress(orc.from_bits(ones))                 # literal-only all ones
+    alt = orc.decompress(orc.from_bits((np.arange(n) % 2).astype(np.uint8)))
+    for x, y in ((a, a), (a, alt), (alt, alt)):
+        for op in range(4):
+            want, wpath = orc.op(x, y, op)
+            got, gpath = gpu_op(L, x, y, op)
+            assert gpath == wpath == "c0"
+            assert np.array_equal(got, want), op
