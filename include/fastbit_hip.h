@@ -116,6 +116,73 @@ int fbh_bv_nwords(const fbh_bitvector* bv, uint32_t* n);
 int fbh_bench_op(const fbh_bitvector* a, const fbh_bitvector* b, int op, int reps,
                  float* ms_per_op, uint32_t* out_words);
 
+/* ---------------------------------------------------------------------- */
+/* Indexes: ibis::relic (equality index, one bitmap per distinct value) and
+ * ibis::bin (binned equality index) with their raw column, held in HBM. */
+
+typedef struct fbh_index fbh_index;
+
+#define FBH_RELIC 1
+#define FBH_BIN 2
+
+/* Column element types (a subset of ibis::TYPE_T, src/table.h). */
+#define FBH_COL_INT32 0
+#define FBH_COL_UINT32 1
+#define FBH_COL_UINT16 2
+#define FBH_COL_FLOAT 3
+#define FBH_COL_DOUBLE 4
+
+/* One side of a range lo OP1 x OP2 hi (ibis::qExpr::COMPARE, src/qExpr.h). */
+#define FBH_OP_NONE 0
+#define FBH_OP_LT 1
+#define FBH_OP_LE 2
+
+/* Evaluation plans.  All plans yield the same bits. */
+#define FBH_PLAN_AUTO 0        /* the ibis heuristic (column::evaluateRange) */
+#define FBH_PLAN_LITERAL 1     /* LDS multiway OR into a literal result      */
+#define FBH_PLAN_COMPRESSED 2  /* pairwise ORs of the compressed bitmaps     */
+#define FBH_PLAN_SCAN 3        /* raw column compare                         */
+
+/* relic over the distinct values of a column: index::mapValues +
+ * relic::construct (src/index.cpp:2552-2611, src/irelic.cpp:878-925). */
+int fbh_index_build_relic(const void* col, int type, uint32_t nrows, fbh_index** out);
+/* relic(col, card, ind) (src/irelic.cpp:160-211): values 0..card-1. */
+int fbh_index_build_relic_card(const uint32_t* ind, uint32_t nrows, uint32_t card,
+                               fbh_index** out);
+/* bin with the given boundaries: bin::binning (src/ibin.cpp:3350-3449),
+ * empty inner bins dropped.  The bounds are the reference's bounds array
+ * (bounds[nobs-1] is DBL_MAX). */
+int fbh_index_build_bin(const void* col, int type, uint32_t nrows, const double* bounds,
+                        uint32_t nbounds, fbh_index** out);
+void fbh_index_free(fbh_index* ix);
+int fbh_index_info(const fbh_index* ix, uint32_t* nobs, uint32_t* nrows, uint64_t* nwords);
+/* fastbit_iapi_deconstruct_index (src/iapi.cpp:1915-1953): keys (relic:
+ * values; bin: minval ++ maxval), offsets in words, serialized bitmaps.
+ * The arrays are allocated by the library; release them with fbh_free. */
+int fbh_index_deconstruct(const fbh_index* ix, double** keys, uint64_t* nkeys,
+                          int64_t** offsets, uint64_t* noffsets, uint32_t** bms,
+                          uint64_t* nbms);
+void fbh_free(void* p);
+/* fastbit_iapi_resolve_range (src/iapi.cpp:1981-2009): relic::locate /
+ * bin::locate of the range. */
+int fbh_index_resolve_range(const fbh_index* ix, double lo, int lop, double hi, int rop,
+                            uint32_t* cand0, uint32_t* hit0, uint32_t* hit1, uint32_t* cand1);
+/* estimateCost (src/ibin.cpp:6531-6569, src/irelic.cpp:1858-1902), the
+ * scan cost of column::evaluateRange (src/column.cpp:6102-6111) and the
+ * plan the ibis heuristic picks. */
+int fbh_index_estimate_cost(const fbh_index* ix, double lo, int lop, double hi, int rop,
+                            double* icost, double* scost, int* plan);
+/* relic::evaluate / bin::evaluate (src/irelic.cpp:1827-1839,
+ * src/ibin.cpp:6614-6713): number of hits and, when result != NULL, the
+ * hits as a new (canonical) bitvector the caller frees. */
+int fbh_index_evaluate(fbh_index* ix, double lo, int lop, double hi, int rop, int plan,
+                       uint64_t* hits, fbh_bitvector** result, int* plan_used);
+/* Hit row ids in ascending order (countQuery::getHitRows,
+ * src/countQuery.cpp:387-418). */
+int fbh_index_evaluate_rids(fbh_index* ix, double lo, int lop, double hi, int rop, int plan,
+                            uint32_t* rids, uint64_t cap, uint64_t* n);
+int fbh_index_column_bytes(const fbh_index* ix, uint64_t* bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,801 @@
+// C ABI of the relic / bin part of libfastbit_hip.so (include/fastbit_hip.h).
+//
+// Build: keys on the GPU (relic: position of the value among the sorted
+// distinct values; bin: bin::locate), a stable radix sort of (key, row), one
+// canonical WAH bitmap per key, concatenated in the reference's in-memory
+// serialized layout; then the tile offset table used by range evaluation.
+//
+// Evaluate (relic::evaluate, src/irelic.cpp:1827-1839; bin::evaluate,
+// src/ibin.cpp:6614-6713): locate on the host, then one of three plans that
+// all yield the same bits -- LITERAL (LDS multiway OR + edge check),
+// COMPRESSED (pairwise WAH ORs of the bitmaps), SCAN (raw column compare).
+#include "../../include/fastbit_hip.h"
+#include "runtime.h"
+#include "launch.h"
+#include "bitvector_impl.h"
+#include "index_dev.h"
+#include "index_host.h"
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace fbh_rt;
+
+namespace {
+
+constexpr uint32_t MAXBITS = 31;
+inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+int fail(int code, const std::string& msg) { set_error(msg); return code; }
+
+size_t col_esize(int type) {
+    switch (type) {
+    case FBH_COL_INT32: case FBH_COL_UINT32: case FBH_COL_FLOAT: return 4;
+    case FBH_COL_UINT16: return 2;
+    case FBH_COL_DOUBLE: return 8;
+    default: return 0;
+    }
+}
+
+int dev_type(int type) {
+    switch (type) {
+    case FBH_COL_INT32: return fbk::COL_I32;
+    case FBH_COL_UINT32: return fbk::COL_U32;
+    case FBH_COL_UINT16: return fbk::COL_U16;
+    case FBH_COL_FLOAT: return fbk::COL_F32;
+    default: return fbk::COL_F64;
+    }
+}
+
+bool is_int_type(int type) {
+    return type == FBH_COL_INT32 || type == FBH_COL_UINT32 || type == FBH_COL_UINT16;
+}
+
+// RAII device buffer from the library allocator.
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    int alloc(size_t cnt) {
+        n = cnt;
+        p = (T*)dev_alloc(sizeof(T) * (cnt ? cnt : 1));
+        return p ? FBH_OK : fail(FBH_ERR_NOMEM, "device allocation failed");
+    }
+    ~DBuf() { dev_free(p); }
+    T* release() { T* q = p; p = nullptr; return q; }
+};
+
+int sync() { return check(hipStreamSynchronize(stream()), "sync"); }
+
+template <class T>
+int to_dev(T* d, const T* h, size_t n) {
+    if (!n) return FBH_OK;
+    return check(hipMemcpyAsync(d, h, sizeof(T) * n, hipMemcpyHostToDevice, stream()), "upload");
+}
+
+template <class T>
+int to_host(T* h, const T* d, size_t n) {
+    if (!n) return FBH_OK;
+    int rc = check(hipMemcpyAsync(h, d, sizeof(T) * n, hipMemcpyDeviceToHost, stream()), "download");
+    return rc ? rc : sync();
+}
+
+} // namespace
+
+struct fbh_index {
+    int kind = 0;                 // FBH_RELIC / FBH_BIN
+    int type = 0;                 // column type
+    uint32_t nrows = 0, groups = 0, anb = 0, nobs = 0;
+    std::vector<double> keys;     // relic: distinct values; bin: bounds
+    std::vector<double> minval, maxval;
+    std::vector<uint64_t> off;    // serialized offsets (words), nobs+1
+    std::vector<uint32_t> nw;     // code words per bitmap (0 and no trailer: null)
+    uint32_t* words = nullptr;    // serialized bitmaps (device)
+    uint64_t nwords = 0;
+    uint64_t* doff = nullptr;
+    uint32_t* dnw = nullptr;
+    uint2* toff = nullptr;
+    uint32_t ntiles = 0;
+    void* col = nullptr;          // raw column (device)
+    double* dbounds = nullptr;    // bin bounds (device)
+    ~fbh_index() {
+        dev_free(words); dev_free(doff); dev_free(dnw); dev_free(toff); dev_free(col);
+        dev_free(dbounds);
+    }
+    bool null_bitmap(uint32_t b) const { return off[b + 1] == off[b]; }
+    // ibis::bitvector::bytes(): code words * 4 + sizeof(bitvector) (40 on x86-64)
+    double bytes(uint32_t b) const { return null_bitmap(b) ? 0.0 : 4.0 * nw[b] + 40.0; }
+};
+
+namespace {
+
+// Stable sort of (key, row) by key; `bits` key bits are sorted.
+int sort_pairs(uint32_t* keys, uint32_t n, uint32_t bits, DBuf<uint32_t>& okeys,
+               DBuf<uint32_t>& orows) {
+    int rc = okeys.alloc(n);
+    if (!rc) rc = orows.alloc(n);
+    if (rc) return rc;
+    DBuf<uint32_t> k2, r2, tmp;
+    rc = k2.alloc(n);
+    if (!rc) rc = r2.alloc(n);
+    const uint32_t nb = fbk::rs_blocks(n);
+    if (!rc) rc = tmp.alloc(512ull * nb + fbk::scan_tmp_words(256u * nb) + 16);
+    if (rc) return rc;
+    const uint32_t passes = bits == 0 ? 1 : cdiv(bits, 8);
+    const uint32_t* ki = keys;
+    const uint32_t* vi = nullptr;
+    // alternate buffers so that the last pass lands in okeys/orows
+    uint32_t* bufk[2] = {okeys.p, k2.p};
+    uint32_t* bufv[2] = {orows.p, r2.p};
+    int cur = (passes % 2 == 1) ? 0 : 1;
+    for (uint32_t p = 0; p < passes; ++p) {
+        rc = check(fbk::launch_rs_pass(ki, vi, n, 8 * p, bufk[cur], bufv[cur], tmp.p, stream()),
+                   "radix pass");
+        if (rc) return rc;
+        ki = bufk[cur]; vi = bufv[cur];
+        cur ^= 1;
+    }
+    return FBH_OK;
+}
+
+uint32_t bits_for(uint32_t maxkey) {
+    uint32_t b = 0;
+    while (b < 32 && (maxkey >> b) != 0) ++b;
+    return b;
+}
+
+// Encode bitmaps from sorted rows; fills idx words/offsets/nw (+min/max).
+int encode_bitmaps(fbh_index* ix, const uint32_t* skeys_d, const uint32_t* srows_d, uint32_t nsorted,
+                   bool with_minmax, bool drop_empty_bins) {
+    const uint32_t nbm = ix->nobs;
+    // bitmap starts in the sorted key order (rows without a bitmap sort last)
+    std::vector<uint32_t> start(nbm + 1);
+    {
+        DBuf<uint32_t> ds;
+        int rc0 = ds.alloc(nbm + 1);
+        if (!rc0) rc0 = check(fbk::launch_starts(skeys_d, nsorted, nbm, ds.p, nullptr, stream()), "starts");
+        if (!rc0) rc0 = to_host(start.data(), ds.p, nbm + 1);
+        if (rc0) return rc0;
+    }
+    int rc;
+    std::vector<uint64_t> tmpoff(nbm + 1);
+    tmpoff[0] = 0;
+    for (uint32_t b = 0; b < nbm; ++b) tmpoff[b + 1] = tmpoff[b] + 2ull * (start[b + 1] - start[b]) + 2;
+    DBuf<uint32_t> dstart, tmp, dnw, daval;
+    DBuf<uint64_t> dtmpoff;
+    DBuf<double> dmin, dmax;
+    if ((rc = dstart.alloc(nbm + 1)) || (rc = tmp.alloc(tmpoff[nbm] + 1)) || (rc = dnw.alloc(nbm)) ||
+        (rc = daval.alloc(nbm)) || (rc = dtmpoff.alloc(nbm + 1)))
+        return rc;
+    if (with_minmax && ((rc = dmin.alloc(nbm)) || (rc = dmax.alloc(nbm)))) return rc;
+    if ((rc = to_dev(dstart.p, start.data(), nbm + 1)) || (rc = to_dev(dtmpoff.p, tmpoff.data(), nbm + 1)))
+        return rc;
+    rc = check(fbk::launch_encode(dev_type(ix->type), srows_d, dstart.p, nbm, ix->groups, ix->anb,
+                                  dtmpoff.p, tmp.p, dnw.p, daval.p, with_minmax ? ix->col : nullptr,
+                                  with_minmax ? dmin.p : nullptr, with_minmax ? dmax.p : nullptr,
+                                  stream()), "encode");
+    if (rc) return rc;
+    std::vector<uint32_t> nw(nbm);
+    if ((rc = to_host(nw.data(), dnw.p, nbm))) return rc;
+    std::vector<double> mn, mx;
+    if (with_minmax) {
+        mn.resize(nbm); mx.resize(nbm);
+        if ((rc = to_host(mn.data(), dmin.p, nbm)) || (rc = to_host(mx.data(), dmax.p, nbm))) return rc;
+    }
+    // bins: drop empty ones except the first and the last
+    // (src/ibin.cpp:3397-3428); an empty kept bin is a null bitmap
+    std::vector<uint32_t> keep;
+    std::vector<bool> nullbm;
+    if (drop_empty_bins) {
+        for (uint32_t b = 0; b < nbm; ++b) {
+            const bool empty = start[b + 1] == start[b];
+            if (!empty || b == 0 || b + 1 == nbm) { keep.push_back(b); nullbm.push_back(empty); }
+        }
+    } else {
+        for (uint32_t b = 0; b < nbm; ++b) { keep.push_back(b); nullbm.push_back(false); }
+    }
+    const uint32_t nk = (uint32_t)keep.size();
+    std::vector<uint64_t> off(nk + 1);
+    std::vector<uint32_t> knw(nk), kav_idx(nk);
+    std::vector<uint64_t> ktmp(nk);
+    off[0] = 0;
+    const uint32_t trailer = ix->anb > 0 ? 2u : 1u;
+    for (uint32_t k = 0; k < nk; ++k) {
+        const uint32_t b = keep[k];
+        knw[k] = nullbm[k] ? 0u : nw[b];
+        ktmp[k] = tmpoff[b];
+        off[k + 1] = off[k] + (nullbm[k] ? 0u : nw[b] + trailer);
+    }
+    // gather per kept bitmap (device arrays for the concat)
+    std::vector<uint32_t> kval_idx(keep.begin(), keep.end());
+    DBuf<uint64_t> doff_k, dtmpoff_k;
+    DBuf<uint32_t> dnw_k, daval_k;
+    std::vector<uint32_t> aval(nbm);
+    if ((rc = to_host(aval.data(), daval.p, nbm))) return rc;
+    std::vector<uint32_t> kav(nk);
+    for (uint32_t k = 0; k < nk; ++k) kav[k] = aval[keep[k]];
+    if ((rc = doff_k.alloc(nk + 1)) || (rc = dtmpoff_k.alloc(nk)) || (rc = dnw_k.alloc(nk)) ||
+        (rc = daval_k.alloc(nk)))
+        return rc;
+    if ((rc = to_dev(doff_k.p, off.data(), nk + 1)) || (rc = to_dev(dtmpoff_k.p, ktmp.data(), nk)) ||
+        (rc = to_dev(dnw_k.p, knw.data(), nk)) || (rc = to_dev(daval_k.p, kav.data(), nk)))
+        return rc;
+    DBuf<uint32_t> words;
+    if ((rc = words.alloc(off[nk] + 1))) return rc;
+    rc = check(fbk::launch_concat(tmp.p, dtmpoff_k.p, dnw_k.p, daval_k.p, doff_k.p, nk, ix->anb,
+                                  words.p, stream()), "concat");
+    if (rc) return rc;
+    if ((rc = sync())) return rc;
+    // install
+    ix->nobs = nk;
+    ix->off = off;
+    ix->nw = knw;
+    ix->nwords = off[nk];
+    ix->words = words.release();
+    if (ix->kind == FBH_BIN) {
+        std::vector<double> nb(nk), nmin(nk), nmax(nk);
+        for (uint32_t k = 0; k < nk; ++k) {
+            nb[k] = ix->keys[keep[k]];
+            nmin[k] = mn[keep[k]];
+            nmax[k] = mx[keep[k]];
+        }
+        ix->keys.swap(nb);
+        ix->minval.swap(nmin);
+        ix->maxval.swap(nmax);
+    } else {
+        std::vector<double> nv(nk);
+        for (uint32_t k = 0; k < nk; ++k) nv[k] = ix->keys[keep[k]];
+        ix->keys.swap(nv);
+    }
+    return FBH_OK;
+}
+
+// Device copies of offsets / word counts and the tile offset table.
+int finish_index(fbh_index* ix) {
+    int rc;
+    ix->doff = (uint64_t*)dev_alloc(8ull * (ix->nobs + 1));
+    ix->dnw = (uint32_t*)dev_alloc(4ull * (ix->nobs + 1));
+    if (!ix->doff || !ix->dnw) return fail(FBH_ERR_NOMEM, "device allocation failed");
+    if ((rc = to_dev(ix->doff, ix->off.data(), ix->nobs + 1)) || (rc = to_dev(ix->dnw, ix->nw.data(), ix->nobs)))
+        return rc;
+    if (ix->kind == FBH_BIN && ix->nobs) {
+        ix->dbounds = (double*)dev_alloc(8ull * ix->nobs);
+        if (!ix->dbounds) return fail(FBH_ERR_NOMEM, "device allocation failed");
+        if ((rc = to_dev(ix->dbounds, ix->keys.data(), ix->nobs))) return rc;
+    }
+    ix->ntiles = ix->groups / (uint32_t)fbk::QT + 1;
+    ix->toff = (uint2*)dev_alloc(sizeof(uint2) * (size_t)ix->ntiles * (ix->nobs ? ix->nobs : 1));
+    if (!ix->toff) return fail(FBH_ERR_NOMEM, "device allocation failed");
+    rc = check(fbk::launch_toff(ix->words, ix->doff, ix->dnw, ix->nobs, ix->ntiles, (uint32_t)fbk::QT,
+                                ix->toff, stream()), "tile offsets");
+    if (rc) return rc;
+    return sync();
+}
+
+int upload_column(fbh_index* ix, const void* col, int type, uint32_t nrows) {
+    const size_t es = col_esize(type);
+    if (!es) return fail(FBH_ERR_ARG, "unknown column type");
+    ix->type = type;
+    ix->nrows = nrows;
+    ix->groups = nrows / MAXBITS;
+    ix->anb = nrows - ix->groups * MAXBITS;
+    ix->col = dev_alloc(es * (nrows ? nrows : 1));
+    if (!ix->col) return fail(FBH_ERR_NOMEM, "device allocation failed");
+    if (nrows) {
+        int rc = check(hipMemcpyAsync(ix->col, col, es * nrows, hipMemcpyHostToDevice, stream()),
+                       "column upload");
+        if (rc) return rc;
+    }
+    return FBH_OK;
+}
+
+// Distinct values of the column (sorted), computed on the GPU: order-
+// preserving 32-bit keys, radix sort, host pass over the sorted keys.
+int distinct_values(fbh_index* ix, std::vector<double>& vals) {
+    const uint32_t n = ix->nrows;
+    if (ix->type == FBH_COL_DOUBLE) return fail(FBH_ERR_ARG, "relic over double columns is not supported");
+    DBuf<uint32_t> k, uq, tmp;
+    int rc;
+    if ((rc = k.alloc(n)) || (rc = uq.alloc(n)) || (rc = tmp.alloc(2ull * n + fbk::scan_tmp_words(n) + 4)))
+        return rc;
+    rc = check(fbk::launch_order_keys(ix->col, dev_type(ix->type), n, k.p, stream()), "order keys");
+    if (rc) return rc;
+    DBuf<uint32_t> sk, sr;
+    if ((rc = sort_pairs(k.p, n, 32, sk, sr))) return rc;
+    rc = check(fbk::launch_unique(sk.p, n, uq.p, tmp.p, stream()), "unique");
+    if (rc) return rc;
+    // number of distinct keys = last exclusive position + last flag
+    uint32_t h[2] = {0, 0};
+    if (n) {
+        if ((rc = to_host(&h[0], tmp.p + n + (n - 1), 1)) || (rc = to_host(&h[1], tmp.p + (n - 1), 1)))
+            return rc;
+    }
+    const uint32_t nd = n ? h[0] + h[1] : 0;
+    std::vector<uint32_t> s(nd);
+    if ((rc = to_host(s.data(), uq.p, nd))) return rc;
+    vals.resize(nd);
+    for (uint32_t i = 0; i < nd; ++i) {
+        const uint32_t u = s[i];
+        switch (ix->type) {
+        case FBH_COL_INT32: vals[i] = (double)(int32_t)(u ^ 0x80000000u); break;
+        case FBH_COL_UINT32: case FBH_COL_UINT16: vals[i] = (double)u; break;
+        default: {
+            const uint32_t b = (u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u;
+            float f; memcpy(&f, &b, 4); vals[i] = (double)f; break;
+        }
+        }
+    }
+    return FBH_OK;
+}
+
+int build_common(fbh_index* ix, const uint32_t* dkeys, uint32_t maxkey, bool minmax, bool drop) {
+    DBuf<uint32_t> sk, sr;
+    int rc = sort_pairs(const_cast<uint32_t*>(dkeys), ix->nrows, bits_for(maxkey), sk, sr);
+    if (rc) return rc;
+    rc = encode_bitmaps(ix, sk.p, sr.p, ix->nrows, minmax, drop);
+    if (rc) return rc;
+    return finish_index(ix);
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// Evaluation.
+namespace {
+
+struct Locate {
+    uint32_t cand0 = 0, hit0 = 0, hit1 = 0, cand1 = 0;
+};
+
+void locate(const fbh_index* ix, double lo, int lop, double hi, int rop, Locate& L) {
+    if (ix->kind == FBH_RELIC) {
+        fbh_ix::relic_locate_range(ix->keys, lo, lop, hi, rop, L.hit0, L.hit1);
+        L.cand0 = L.hit0; L.cand1 = L.hit1;
+    } else {
+        fbh_ix::bin_locate_range(ix->keys, ix->minval, ix->maxval, lo, lop, hi, rop, L.cand0,
+                                 L.cand1, L.hit0, L.hit1);
+    }
+    if (L.hit1 < L.hit0) L.hit1 = L.hit0;
+}
+
+// sumBins' choice between OR of [ib,ie) and the complement of the OR of
+// the outside (src/index.cpp:6450-6491), by bitmap bytes.
+struct SumPlan {
+    int kind;         // 0 none (all zero), 1 all ones, 2 straight, 3 complement
+};
+
+SumPlan sum_plan(const fbh_index* ix, uint32_t ib, uint32_t ie) {
+    SumPlan p{2};
+    if (ie > ix->nobs) ie = ix->nobs;
+    if (ib >= ie) { p.kind = 0; return p; }
+    double all = 0, mid = 0;
+    for (uint32_t b = 0; b < ix->nobs; ++b) {
+        all += ix->bytes(b);
+        if (b >= ib && b < ie) mid += ix->bytes(b);
+    }
+    if (mid == 0) p.kind = 0;
+    else if (all == mid) p.kind = 1;
+    else p.kind = (mid <= std::floor(all / 2)) ? 2 : 3;
+    return p;
+}
+
+void fill_query(const fbh_index* ix, fbk::QueryArgs& q, double lo, int lop, double hi, int rop) {
+    memset(&q, 0, sizeof(q));
+    q.words = ix->words; q.off = ix->doff; q.nw = ix->dnw; q.toff = ix->toff;
+    q.nobs = ix->nobs; q.groups = ix->groups; q.anb = ix->anb;
+    q.cand0 = fbk::NOKEY; q.cand1 = fbk::NOKEY;
+    q.col = ix->col;
+    q.is_int = is_int_type(ix->type) ? 1 : 0;
+    int l = lop, r = rop;
+    int64_t ilo = 0, ihi = 0;
+    if (q.is_int) {
+        if (ix->type == FBH_COL_INT32) fbh_ix::int_bounds<int32_t>(lo, l, hi, r, ilo, ihi);
+        else if (ix->type == FBH_COL_UINT32) fbh_ix::int_bounds<uint32_t>(lo, l, hi, r, ilo, ihi);
+        else fbh_ix::int_bounds<uint16_t>(lo, l, hi, r, ilo, ihi);
+    }
+    q.lop = l; q.rop = r; q.lo = lo; q.hi = hi; q.ilo = ilo; q.ihi = ihi;
+}
+
+} // namespace
+
+namespace fbh_impl {
+
+// Literal result of a range query (groups + 1 words: the last one holds the
+// active bits left-aligned) and its ones count.  plan: FBH_PLAN_LITERAL or
+// FBH_PLAN_SCAN or FBH_PLAN_COMPRESSED.
+int index_eval_literal(fbh_index* ix, double lo, int lop, double hi, int rop, int plan,
+                       uint32_t* out, uint32_t* dcount, hipStream_t s) {
+    fbk::QueryArgs q;
+    fill_query(ix, q, lo, lop, hi, rop);
+    q.out = out;
+    q.count = dcount;
+    int rc = check(fbk::launch_zero_u32(dcount, 1, s), "zero");
+    if (rc) return rc;
+    Locate L;
+    locate(ix, lo, lop, hi, rop, L);
+    if (plan == FBH_PLAN_SCAN) {
+        // the predicate of relic/bin::evaluate row by row: exact compares,
+        // except that a bin index over an integer column applies
+        // part::doScan's cast rules only inside its edge bins
+        if (ix->kind == FBH_RELIC || !is_int_type(ix->type)) {
+            q.is_int = 0;
+            q.lop = lop; q.rop = rop;
+        } else {
+            q.scan_bin = 1;
+            q.bounds = ix->dbounds;
+            q.nbounds = ix->nobs;
+            const SumPlan sp0 = sum_plan(ix, L.hit0, L.hit1);
+            if (sp0.kind == 1) { q.h0 = 0; q.h1 = ix->nobs; }
+            else { q.h0 = L.hit0; q.h1 = L.hit1; }
+            q.e0 = (L.cand0 < L.hit0) ? L.cand0 : fbk::NOKEY;
+            q.e1 = (L.cand1 > L.hit1 && L.hit1 < ix->nobs) ? L.hit1 : fbk::NOKEY;
+        }
+        return check(fbk::launch_query_scan(q, dev_type(ix->type), ix->nrows, s), "scan");
+    }
+    const SumPlan sp = sum_plan(ix, L.hit0, L.hit1);
+    q.r0 = q.r1 = q.r2 = q.r3 = 0;
+    if (sp.kind == 1) { q.r0 = 0; q.r1 = 0; q.flip = 1; }
+    else if (sp.kind == 2) { q.r0 = L.hit0; q.r1 = L.hit1; }
+    else if (sp.kind == 3) { q.r0 = 0; q.r1 = L.hit0; q.r2 = L.hit1; q.r3 = ix->nobs; q.flip = 1; }
+    if (ix->kind == FBH_BIN) {
+        if (L.cand0 < L.hit0 && !ix->null_bitmap(L.cand0)) q.cand0 = L.cand0;
+        if (L.cand1 > L.hit1 && L.hit1 < ix->nobs && !ix->null_bitmap(L.hit1)) q.cand1 = L.hit1;
+    }
+    return check(fbk::launch_query_lit(q, dev_type(ix->type), s), "query");
+}
+
+} // namespace fbh_impl
+
+using namespace fbh_impl;
+
+namespace {
+
+// View of bitmap b of the index as a (borrowed) device bitvector.
+void bitmap_view(const fbh_index* ix, uint32_t b, fbh_bitvector& v) {
+    v.d = ix->words + ix->off[b];
+    v.cap = 0;
+    v.nw = ix->nw[b];
+    v.nbits = ix->groups * MAXBITS;
+    v.nset = 0;
+    v.anb = ix->anb;
+    v.aval = 0;
+}
+
+int view_aval(const fbh_index* ix, uint32_t b, uint32_t* aval, uint32_t* w0, uint32_t* wl) {
+    uint32_t h[2] = {0, 0};
+    int rc;
+    if (ix->nw[b]) {
+        if ((rc = to_host(&h[0], ix->words + ix->off[b], 1))) return rc;
+        if ((rc = to_host(&h[1], ix->words + ix->off[b] + ix->nw[b] - 1, 1))) return rc;
+    }
+    *w0 = h[0]; *wl = h[1];
+    *aval = 0;
+    if (ix->anb && !ix->null_bitmap(b)) return to_host(aval, ix->words + ix->off[b] + ix->nw[b], 1);
+    return FBH_OK;
+}
+
+// Compressed plan: the OR chain through the WAH operator kernels.
+int eval_compressed(fbh_index* ix, double lo, int lop, double hi, int rop, fbh_bitvector* res) {
+    Locate L;
+    locate(ix, lo, lop, hi, rop, L);
+    const SumPlan sp = sum_plan(ix, L.hit0, L.hit1);
+    int rc = fbh_bv_set(res, 0, ix->nrows);
+    if (rc) return rc;
+    auto or_range = [&](uint32_t b0, uint32_t b1) -> int {
+        for (uint32_t b = b0; b < b1; ++b) {
+            if (ix->null_bitmap(b)) continue;
+            fbh_bitvector v;
+            bitmap_view(ix, b, v);
+            int r = view_aval(ix, b, &v.aval, &v.w0, &v.wl);
+            if (r) return r;
+            r = bv_op(res, &v, FBH_OR, nullptr);
+            v.d = nullptr;        // borrowed
+            if (v.ck) { dev_free(v.ck); v.ck = nullptr; }
+            if (r) return r;
+        }
+        return FBH_OK;
+    };
+    if (sp.kind == 1) {
+        rc = fbh_bv_set(res, 1, ix->nrows);
+    } else if (sp.kind == 2) {
+        rc = or_range(L.hit0, L.hit1);
+    } else if (sp.kind == 3) {
+        rc = or_range(0, L.hit0);
+        if (!rc) rc = or_range(L.hit1, ix->nobs);
+        if (!rc) rc = fbh_bv_flip(res);
+    }
+    if (rc) return rc;
+    if (ix->kind == FBH_BIN && (L.cand0 < L.hit0 || (L.cand1 > L.hit1 && L.hit1 < ix->nobs))) {
+        // edge rows through the literal kernel restricted to the candidates
+        fbk::QueryArgs q;
+        fill_query(ix, q, lo, lop, hi, rop);
+        if (L.cand0 < L.hit0 && !ix->null_bitmap(L.cand0)) q.cand0 = L.cand0;
+        if (L.cand1 > L.hit1 && L.hit1 < ix->nobs && !ix->null_bitmap(L.hit1)) q.cand1 = L.hit1;
+        DBuf<uint32_t> lit, cnt;
+        if ((rc = lit.alloc(ix->groups + 1)) || (rc = cnt.alloc(4))) return rc;
+        q.out = lit.p; q.count = cnt.p;
+        if ((rc = check(fbk::launch_zero_u32(cnt.p, 1, stream()), "zero"))) return rc;
+        if ((rc = check(fbk::launch_query_lit(q, dev_type(ix->type), stream()), "edge"))) return rc;
+        fbh_bitvector delta;
+        uint32_t act = 0;
+        if (ix->anb) { if ((rc = to_host(&act, lit.p + ix->groups, 1))) return rc; }
+        // wrap the literal array as a literal-only bitvector (no copy)
+        delta.d = lit.p; delta.nw = ix->groups; delta.nbits = ix->groups * MAXBITS;
+        delta.anb = ix->anb; delta.aval = ix->anb ? (act >> (31 - ix->anb)) : 0;
+        if (ix->groups) {
+            if ((rc = to_host(&delta.w0, lit.p, 1)) || (rc = to_host(&delta.wl, lit.p + ix->groups - 1, 1)))
+                return rc;
+        }
+        rc = bv_op(res, &delta, FBH_OR, nullptr);
+        delta.d = nullptr;
+        if (delta.ck) { dev_free(delta.ck); delta.ck = nullptr; }
+        if (rc) return rc;
+    }
+    return FBH_OK;
+}
+
+int plan_heuristic(const fbh_index* ix, double lo, int lop, double hi, int rop);
+
+} // namespace
+
+extern "C" {
+
+void fbh_free(void* p) { std::free(p); }
+
+int fbh_index_build_relic(const void* col, int type, uint32_t nrows, fbh_index** out) {
+    if (!out || (nrows && !col)) return fail(FBH_ERR_ARG, "null argument");
+    int rc = ensure_init();
+    if (rc) return rc;
+    fbh_index* ix = new fbh_index();
+    ix->kind = FBH_RELIC;
+    if ((rc = upload_column(ix, col, type, nrows))) { delete ix; return rc; }
+    std::vector<double> vals;
+    if ((rc = distinct_values(ix, vals))) { delete ix; return rc; }
+    ix->keys = vals;
+    ix->nobs = (uint32_t)vals.size();
+    DBuf<double> dv;
+    DBuf<uint32_t> keys;
+    if ((rc = dv.alloc(vals.size())) || (rc = keys.alloc(nrows)) || (rc = to_dev(dv.p, vals.data(), vals.size()))) {
+        delete ix; return rc;
+    }
+    rc = check(fbk::launch_keys_relic(ix->col, dev_type(type), nrows, dv.p, ix->nobs, keys.p, stream()),
+               "relic keys");
+    if (!rc) rc = build_common(ix, keys.p, ix->nobs, false, false);
+    if (rc) { delete ix; return rc; }
+    *out = ix;
+    return FBH_OK;
+}
+
+int fbh_index_build_relic_card(const uint32_t* ind, uint32_t nrows, uint32_t card, fbh_index** out) {
+    if (!out || (nrows && !ind)) return fail(FBH_ERR_ARG, "null argument");
+    int rc = ensure_init();
+    if (rc) return rc;
+    fbh_index* ix = new fbh_index();
+    ix->kind = FBH_RELIC;
+    if ((rc = upload_column(ix, ind, FBH_COL_UINT32, nrows))) { delete ix; return rc; }
+    ix->nobs = card;
+    ix->keys.resize(card);
+    for (uint32_t i = 0; i < card; ++i) ix->keys[i] = i;
+    DBuf<uint32_t> keys;
+    if ((rc = keys.alloc(nrows))) { delete ix; return rc; }
+    rc = check(fbk::launch_keys_card((const uint32_t*)ix->col, nrows, card, keys.p, stream()), "card keys");
+    if (!rc) rc = build_common(ix, keys.p, card, false, false);
+    if (rc) { delete ix; return rc; }
+    *out = ix;
+    return FBH_OK;
+}
+
+int fbh_index_build_bin(const void* col, int type, uint32_t nrows, const double* bounds,
+                        uint32_t nbounds, fbh_index** out) {
+    if (!out || (nrows && !col) || !bounds || nbounds == 0) return fail(FBH_ERR_ARG, "bad argument");
+    int rc = ensure_init();
+    if (rc) return rc;
+    fbh_index* ix = new fbh_index();
+    ix->kind = FBH_BIN;
+    if ((rc = upload_column(ix, col, type, nrows))) { delete ix; return rc; }
+    ix->keys.assign(bounds, bounds + nbounds);
+    ix->nobs = nbounds;
+    DBuf<double> db;
+    DBuf<uint32_t> keys;
+    if ((rc = db.alloc(nbounds)) || (rc = keys.alloc(nrows)) || (rc = to_dev(db.p, bounds, nbounds))) {
+        delete ix; return rc;
+    }
+    rc = check(fbk::launch_keys_bin(ix->col, dev_type(type), nrows, db.p, nbounds, keys.p, stream()),
+               "bin keys");
+    if (!rc) rc = build_common(ix, keys.p, nbounds, true, true);
+    if (rc) { delete ix; return rc; }
+    *out = ix;
+    return FBH_OK;
+}
+
+void fbh_index_free(fbh_index* ix) { delete ix; }
+
+int fbh_index_info(const fbh_index* ix, uint32_t* nobs, uint32_t* nrows, uint64_t* nwords) {
+    if (!ix) return fail(FBH_ERR_ARG, "null index");
+    if (nobs) *nobs = ix->nobs;
+    if (nrows) *nrows = ix->nrows;
+    if (nwords) *nwords = ix->nwords;
+    return FBH_OK;
+}
+
+int fbh_index_deconstruct(const fbh_index* ix, double** keys, uint64_t* nkeys, int64_t** offsets,
+                          uint64_t* noffsets, uint32_t** bms, uint64_t* nbms) {
+    if (!ix || !keys || !nkeys || !offsets || !noffsets || !bms || !nbms)
+        return fail(FBH_ERR_ARG, "null argument");
+    // relic: keys = vals; bin: keys = minval ++ maxval
+    // (src/irelic.cpp:644-667, src/ibin.cpp:5836-5860)
+    std::vector<double> k;
+    if (ix->kind == FBH_RELIC) k = ix->keys;
+    else { k = ix->minval; k.insert(k.end(), ix->maxval.begin(), ix->maxval.end()); }
+    *nkeys = k.size();
+    *keys = (double*)std::malloc(sizeof(double) * (k.size() ? k.size() : 1));
+    *noffsets = ix->nobs + 1;
+    *offsets = (int64_t*)std::malloc(sizeof(int64_t) * (ix->nobs + 1));
+    *nbms = ix->nwords;
+    *bms = (uint32_t*)std::malloc(sizeof(uint32_t) * (ix->nwords ? ix->nwords : 1));
+    if (!*keys || !*offsets || !*bms) return fail(FBH_ERR_NOMEM, "host allocation failed");
+    if (!k.empty()) memcpy(*keys, k.data(), sizeof(double) * k.size());
+    for (uint32_t b = 0; b <= ix->nobs; ++b) (*offsets)[b] = (int64_t)ix->off[b];
+    return to_host(*bms, ix->words, ix->nwords);
+}
+
+int fbh_index_resolve_range(const fbh_index* ix, double lo, int lop, double hi, int rop,
+                            uint32_t* cand0, uint32_t* hit0, uint32_t* hit1, uint32_t* cand1) {
+    if (!ix) return fail(FBH_ERR_ARG, "null index");
+    Locate L;
+    locate(ix, lo, lop, hi, rop, L);
+    if (cand0) *cand0 = L.cand0;
+    if (hit0) *hit0 = L.hit0;
+    if (hit1) *hit1 = L.hit1;
+    if (cand1) *cand1 = L.cand1;
+    return FBH_OK;
+}
+
+int fbh_index_estimate_cost(const fbh_index* ix, double lo, int lop, double hi, int rop,
+                            double* icost, double* scost, int* plan) {
+    if (!ix) return fail(FBH_ERR_ARG, "null index");
+    // bin::estimateCost (src/ibin.cpp:6531-6569); relic::estimateCost
+    // (src/irelic.cpp:1858-1902) is the same without the edge term.
+    Locate L;
+    locate(ix, lo, lop, hi, rop, L);
+    double ic = 0;
+    double tot = 0, mid = 0;
+    for (uint32_t b = 0; b < ix->nobs; ++b) {
+        const double by = ix->null_bitmap(b) ? 0.0 : 4.0 * (ix->nw[b] + 1 + (ix->anb > 0));
+        tot += by;
+        if (b >= L.cand0 && b < L.cand1) mid += by;
+    }
+    if (L.cand0 < L.cand1) ic = (tot / 2 >= mid) ? mid : tot - mid;
+    if (ix->kind == FBH_BIN && (L.hit0 > L.cand0 || L.hit1 < L.cand1) && ix->nobs > 0) {
+        const double ccheck = (double)col_esize(ix->type) * ix->nrows / ix->nobs;
+        ic += (L.hit0 > L.cand0 && L.hit1 < L.cand1 && L.hit0 <= L.hit1) ? 2.0 * ccheck : ccheck;
+    }
+    // column::evaluateRange (src/column.cpp:6102-6111): page count of a
+    // full mask (src/part.cpp:2616-2678) at 4096-byte pages
+    const double es = (double)col_esize(ix->type);
+    const double wpp = std::floor(4096.0 / es);
+    double pages = std::floor((double)ix->nrows / wpp);
+    if (pages < 1) pages = 1;
+    const double sc = 4096.0 * pages + 8.0 * ix->nrows / 4096.0;
+    if (icost) *icost = ic;
+    if (scost) *scost = sc;
+    if (plan) *plan = plan_heuristic(ix, lo, lop, hi, rop);
+    return FBH_OK;
+}
+
+int fbh_index_evaluate(fbh_index* ix, double lo, int lop, double hi, int rop, int plan,
+                       uint64_t* hits, fbh_bitvector** result, int* plan_used) {
+    if (!ix) return fail(FBH_ERR_ARG, "null index");
+    if (lop < 0 || lop > 2 || rop < 0 || rop > 2) return fail(FBH_ERR_ARG, "bad operator");
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (plan == FBH_PLAN_AUTO) plan = plan_heuristic(ix, lo, lop, hi, rop);
+    if (plan_used) *plan_used = plan;
+    if (plan == FBH_PLAN_COMPRESSED) {
+        fbh_bitvector* res = nullptr;
+        if ((rc = fbh_bv_new(&res))) return rc;
+        rc = eval_compressed(ix, lo, lop, hi, rop, res);
+        uint32_t c = 0;
+        if (!rc) rc = fbh_bv_cnt(res, &c);
+        if (rc) { fbh_bv_free(res); return rc; }
+        if (hits) *hits = c;
+        if (result) *result = res; else fbh_bv_free(res);
+        return FBH_OK;
+    }
+    DBuf<uint32_t> lit, cnt;
+    if ((rc = lit.alloc(ix->groups + 1)) || (rc = cnt.alloc(4))) return rc;
+    rc = index_eval_literal(ix, lo, lop, hi, rop, plan, lit.p, cnt.p, stream());
+    if (rc) return rc;
+    uint32_t c = 0;
+    if ((rc = to_host(&c, cnt.p, 1))) return rc;
+    if (hits) *hits = c;
+    if (result) {
+        // canonical WAH of the literal result
+        fbh_bitvector* res = nullptr;
+        if ((rc = fbh_bv_new(&res))) return rc;
+        uint32_t act = 0;
+        if (ix->anb && (rc = to_host(&act, lit.p + ix->groups, 1))) { fbh_bv_free(res); return rc; }
+        fbh_bitvector tmp;
+        tmp.d = lit.p; tmp.nw = ix->groups; tmp.nbits = ix->groups * MAXBITS;
+        tmp.anb = ix->anb; tmp.aval = ix->anb ? (act >> (31 - ix->anb)) : 0;
+        if (ix->groups) {
+            if ((rc = to_host(&tmp.w0, lit.p, 1)) || (rc = to_host(&tmp.wl, lit.p + ix->groups - 1, 1))) {
+                fbh_bv_free(res); return rc;
+            }
+        }
+        // res = empty | tmp  (the canonical kernel runs through the c0/d1 path)
+        rc = fbh_bv_set(res, 0, ix->nrows);
+        if (!rc) rc = bv_op(res, &tmp, FBH_OR, nullptr);
+        tmp.d = nullptr;
+        if (tmp.ck) { dev_free(tmp.ck); tmp.ck = nullptr; }
+        if (rc) { fbh_bv_free(res); return rc; }
+        *result = res;
+    }
+    return FBH_OK;
+}
+
+int fbh_index_evaluate_rids(fbh_index* ix, double lo, int lop, double hi, int rop, int plan,
+                            uint32_t* rids, uint64_t cap, uint64_t* n) {
+    if (!ix || !n) return fail(FBH_ERR_ARG, "null argument");
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (plan == FBH_PLAN_AUTO || plan == FBH_PLAN_COMPRESSED) plan = FBH_PLAN_LITERAL;
+    DBuf<uint32_t> lit, cnt;
+    if ((rc = lit.alloc(ix->groups + 1)) || (rc = cnt.alloc(4))) return rc;
+    rc = index_eval_literal(ix, lo, lop, hi, rop, plan, lit.p, cnt.p, stream());
+    if (rc) return rc;
+    uint32_t c = 0;
+    if ((rc = to_host(&c, cnt.p, 1))) return rc;
+    *n = c;
+    if (!rids || c == 0) return FBH_OK;
+    // the active slot is one more literal group (rows groups*31 ..)
+    const uint32_t ng = ix->groups + (ix->anb ? 1 : 0);
+    DBuf<uint32_t> out, tmp;
+    if ((rc = out.alloc(c)) || (rc = tmp.alloc(2ull * ng + fbk::scan_tmp_words(ng) + 4))) return rc;
+    rc = check(fbk::launch_rids_lit(lit.p, ng, out.p, c, tmp.p, stream()), "rids");
+    if (rc) return rc;
+    const uint64_t k = c < cap ? c : cap;
+    return to_host(rids, out.p, k);
+}
+
+int fbh_index_column_bytes(const fbh_index* ix, uint64_t* bytes) {
+    if (!ix || !bytes) return fail(FBH_ERR_ARG, "null argument");
+    *bytes = (uint64_t)col_esize(ix->type) * ix->nrows;
+    return FBH_OK;
+}
+
+} // extern "C"
+
+namespace {
+
+// The ibis heuristic mapped onto the three plans: the index is used when
+// its estimated cost is below the scan cost (src/column.cpp:6102-6111);
+// then sumBins' priority-queue branch (src/index.cpp:6673) is a compressed
+// merge, its decompressed-result branches are the literal plan.
+int plan_heuristic(const fbh_index* ix, double lo, int lop, double hi, int rop) {
+    double ic = 0, sc = 0;
+    fbh_index_estimate_cost(ix, lo, lop, hi, rop, &ic, &sc, nullptr);
+    if (!(ic < sc)) return FBH_PLAN_SCAN;
+    Locate L;
+    locate(ix, lo, lop, hi, rop, L);
+    const SumPlan sp = sum_plan(ix, L.hit0, L.hit1);
+    if (sp.kind < 2) return FBH_PLAN_LITERAL;
+    uint32_t ib = sp.kind == 2 ? L.hit0 : 0, ie = sp.kind == 2 ? L.hit1 : L.hit0;
+    double bytes = 0;
+    uint32_t na = 0;
+    if (sp.kind == 2) {
+        for (uint32_t b = ib; b < ie; ++b) { bytes += ix->bytes(b); ++na; }
+    } else {
+        for (uint32_t b = 0; b < L.hit0; ++b) { bytes += ix->bytes(b); ++na; }
+        for (uint32_t b = L.hit1; b < ix->nobs; ++b) { bytes += ix->bytes(b); ++na; }
+    }
+    const double uncomp = (double)(ix->nrows * 4u / 31u);
+    const double sum2 = (na >= 2) ? ix->bytes(ib) + ix->bytes(ib + 1 < ix->nobs ? ib + 1 : ib) : bytes;
+    if (na > 2 && sum2 < uncomp && bytes * (double)na * na <= std::log(2.0) * uncomp)
+        return FBH_PLAN_COMPRESSED;
+    return FBH_PLAN_LITERAL;
+}
+
+} // namespace

@@ -1,0 +1,71 @@
+// Shared definitions of the index kernels and their host launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fbk {
+
+constexpr uint32_t NOKEY = 0xFFFFFFFFu;
+constexpr double DBL_MAX_D = 1.7976931348623157e308;
+constexpr int QT = 8192;                 // groups per query tile (LDS words)
+
+// column types
+enum : int { COL_I32 = 0, COL_U32 = 1, COL_U16 = 2, COL_F32 = 3, COL_F64 = 4 };
+// comparison operators of one side of a range
+enum : int { OPC_NONE = 0, OPC_LT = 1, OPC_LE = 2 };
+
+struct QueryArgs {
+    const uint32_t* words;     // serialized bitmaps
+    const uint64_t* off;       // bitmap b at words[off[b]]
+    const uint32_t* nw;        // code words of bitmap b
+    const uint2* toff;         // tile offset table [tile][bitmap]
+    uint32_t nobs;
+    uint32_t groups;           // whole 31-bit groups per bitmap
+    uint32_t anb;              // active bits per bitmap
+    uint32_t r0, r1, r2, r3;   // OR bitmaps [r0,r1) and [r2,r3)
+    uint32_t flip;             // complement the OR (complement strategy)
+    uint32_t cand0, cand1;     // edge bitmaps to check against the column
+    const void* col;           // raw column
+    int is_int;
+    int lop, rop;
+    double lo, hi;
+    int64_t ilo, ihi;
+    uint32_t* out;             // literal result, groups + 1 words
+    uint32_t* count;           // ones (atomic)
+    // scan plan over a bin index on an integer column: the row's bin
+    // decides (hit bins [h0,h1) hit, edge bins e0/e1 take the raw test)
+    uint32_t scan_bin;
+    const double* bounds;
+    uint32_t nbounds;
+    uint32_t h0, h1, e0, e1;
+};
+
+hipError_t launch_keys_relic(const void* col, int type, uint32_t n, const double* vals, uint32_t nv,
+                             uint32_t* keys, hipStream_t s);
+hipError_t launch_keys_card(const uint32_t* ind, uint32_t n, uint32_t card, uint32_t* keys,
+                            hipStream_t s);
+hipError_t launch_keys_bin(const void* col, int type, uint32_t n, const double* bounds, uint32_t nobs,
+                           uint32_t* keys, hipStream_t s);
+hipError_t launch_order_keys(const void* col, int type, uint32_t n, uint32_t* keys, hipStream_t s);
+hipError_t launch_unique(const uint32_t* k, uint32_t n, uint32_t* out, uint32_t* tmp, hipStream_t s);
+hipError_t launch_starts(const uint32_t* k, uint32_t n, uint32_t nbm, uint32_t* start, uint32_t* room,
+                         hipStream_t s);
+uint32_t rs_blocks(uint32_t n);
+hipError_t launch_rs_pass(const uint32_t* ki, const uint32_t* vi, uint32_t n, uint32_t shift,
+                          uint32_t* ko, uint32_t* vo, uint32_t* tmp, hipStream_t s);
+hipError_t launch_encode(int type, const uint32_t* rows, const uint32_t* start, uint32_t nbm,
+                         uint32_t groups, uint32_t anb, const uint64_t* tmpoff, uint32_t* tmp,
+                         uint32_t* nw, uint32_t* aval, const void* col, double* vmin, double* vmax,
+                         hipStream_t s);
+hipError_t launch_concat(const uint32_t* tmp, const uint64_t* tmpoff, const uint32_t* nw,
+                         const uint32_t* aval, const uint64_t* off, uint32_t nbm, uint32_t anb,
+                         uint32_t* out, hipStream_t s);
+hipError_t launch_toff(const uint32_t* words, const uint64_t* off, const uint32_t* nw, uint32_t nbm,
+                       uint32_t ntiles, uint32_t tg, uint2* toff, hipStream_t s);
+hipError_t launch_query_lit(const QueryArgs& q, int type, hipStream_t s);
+hipError_t launch_query_scan(const QueryArgs& q, int type, uint32_t nrows, hipStream_t s);
+hipError_t launch_scan_excl(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp,
+                            hipStream_t s);
+uint32_t scan_tmp_words(uint32_t n);
+
+} // namespace fbk

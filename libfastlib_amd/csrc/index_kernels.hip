@@ -1,0 +1,682 @@
+// gfx950 kernels for relic / bin indexes: build (keys, stable radix sort,
+// per-bitmap canonical WAH encoding, concatenation, tile offset table) and
+// range evaluation (LDS multiway OR with the edge-bin check, raw column scan).
+//
+// Index layout in HBM (the reference's in-memory form, src/irelic.cpp:644-667
+// and src/ibin.cpp:5836-5860): bitmaps serialized one after another, bitmap
+// b occupying words [off[b], off[b+1]) = its code words, then active.val when
+// active.nbits > 0, then active.nbits.
+//
+// Tile offset table: for every tile t of TG 31-bit groups and every bitmap
+// b, toff[t*nobs + b] = {index of the first code word of b whose span ends
+// after group t*TG, start group of that word}.  A range query then lets one
+// workgroup own one tile of the result and every lane walk the few words of
+// one bitmap that overlap it.
+#include "dev_common.h"
+#include "index_dev.h"
+
+namespace fbk {
+using namespace wah;
+
+constexpr int IX_TPB = 256;
+
+// ---------------------------------------------------------------------------
+// Keys.
+template <class T>
+__device__ __forceinline__ double as_double(T v) { return (double)v; }
+
+// relic: index of the value in the sorted distinct values (lower bound).
+template <class T>
+__global__ __launch_bounds__(IX_TPB) void k_keys_relic(const T* __restrict__ col, uint32_t n,
+                                                       const double* __restrict__ vals, uint32_t nv,
+                                                       uint32_t* __restrict__ keys) {
+    const uint32_t i = blockIdx.x * IX_TPB + threadIdx.x;
+    if (i >= n) return;
+    const double x = as_double(col[i]);
+    uint32_t lo = 0, hi = nv;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (vals[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    keys[i] = lo;
+}
+
+// relic(col, card, ind) (src/irelic.cpp:160-211): the key is the value when
+// it is below card, otherwise the row belongs to no bitmap.
+__global__ __launch_bounds__(IX_TPB) void k_keys_card(const uint32_t* __restrict__ ind, uint32_t n,
+                                                      uint32_t card, uint32_t* __restrict__ keys) {
+    const uint32_t i = blockIdx.x * IX_TPB + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t v = ind[i];
+    keys[i] = v < card ? v : NOKEY;
+}
+
+// bin::locate(double) (src/ibin.cpp:673-738), restated exactly.
+__device__ __forceinline__ uint32_t bin_locate(const double* bounds, uint32_t nobs, double val) {
+    if (nobs == 0) return 0;
+    if (!(val >= bounds[0])) return 0;
+    if (!(val < bounds[nobs - 1])) return bounds[nobs - 1] < DBL_MAX_D ? nobs : nobs - 1;
+    if (nobs >= 8) {
+        uint32_t i0 = 0, i1 = nobs, it = nobs / 2;
+        while (i0 < it) {
+            if (val < bounds[it]) i1 = it; else i0 = it;
+            it = (i0 + i1) / 2;
+        }
+        return i1;
+    }
+    for (uint32_t i = 1; i < nobs; ++i)
+        if (val < bounds[i]) return i;
+    return nobs;
+}
+
+template <class T>
+__global__ __launch_bounds__(IX_TPB) void k_keys_bin(const T* __restrict__ col, uint32_t n,
+                                                     const double* __restrict__ bounds, uint32_t nobs,
+                                                     uint32_t* __restrict__ keys) {
+    const uint32_t i = blockIdx.x * IX_TPB + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t j = bin_locate(bounds, nobs, as_double(col[i]));
+    keys[i] = j < nobs ? j : NOKEY;
+}
+
+// ---------------------------------------------------------------------------
+// Stable LSD radix sort of (key, row) pairs, 8-bit digits.  One pass:
+//   k_rs_hist     per-block digit histogram (block = RS_TILE elements)
+//   scan          over the digit-major [256][nblocks] table
+//   k_rs_scatter  stable scatter: each wave ranks its 64 elements with
+//                 ballot match masks, waves are ordered by LDS prefix sums
+constexpr int RS_TPB = 256;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = RS_TPB * RS_ITEMS;   // elements per block
+constexpr int RS_WAVES = RS_TPB / WAVE;
+
+__global__ __launch_bounds__(RS_TPB) void k_rs_hist(const uint32_t* __restrict__ keys, uint32_t n,
+                                                    uint32_t shift, uint32_t nblocks,
+                                                    uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[256];
+    for (int i = threadIdx.x; i < 256; i += RS_TPB) h[i] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * RS_TILE;
+    for (uint32_t k = threadIdx.x; k < RS_TILE; k += RS_TPB) {
+        const uint32_t i = base + k;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < 256; d += RS_TPB) hist[(uint32_t)d * nblocks + blockIdx.x] = h[d];
+}
+
+__global__ __launch_bounds__(RS_TPB) void k_rs_scatter(const uint32_t* __restrict__ keys,
+                                                       const uint32_t* __restrict__ vals, uint32_t n,
+                                                       uint32_t shift, uint32_t nblocks,
+                                                       const uint32_t* __restrict__ offs,
+                                                       uint32_t* __restrict__ okeys,
+                                                       uint32_t* __restrict__ ovals) {
+    // wave w owns elements [base + w*ITEMS*64, base + (w+1)*ITEMS*64)
+    __shared__ uint32_t cnt[RS_WAVES][256];
+    __shared__ uint32_t boff[256];
+    const int lane = (int)lane_id();
+    const int w = threadIdx.x / WAVE;
+    for (int i = threadIdx.x; i < RS_WAVES * 256; i += RS_TPB) (&cnt[0][0])[i] = 0;
+    for (int d = threadIdx.x; d < 256; d += RS_TPB) boff[d] = offs[(uint32_t)d * nblocks + blockIdx.x];
+    __syncthreads();
+    const uint32_t wbase = blockIdx.x * RS_TILE + (uint32_t)w * RS_ITEMS * WAVE;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t dig[RS_ITEMS], rank[RS_ITEMS];
+    // pass 1: ranks within the wave, counts per (wave, digit)
+    #pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        const uint32_t i = wbase + (uint32_t)r * WAVE + lane;
+        const bool valid = i < n;
+        const uint32_t d = valid ? ((keys[i] >> shift) & 255u) : 256u;
+        uint64_t m = __ballot(valid);
+        #pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const uint64_t bb = __ballot((d >> b) & 1u);
+            m &= ((d >> b) & 1u) ? bb : ~bb;
+        }
+        dig[r] = d;
+        const uint32_t before = (uint32_t)__popcll(m & lt_mask);
+        const uint32_t total = (uint32_t)__popcll(m);
+        // the lowest lane of each digit group adds the group to the counter
+        uint32_t prev = 0;
+        if (valid && before == 0) { prev = cnt[w][d]; cnt[w][d] = prev + total; }
+        // broadcast prev within the group: leader lane index
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        prev = (uint32_t)__shfl((int)prev, leader < 0 ? 0 : leader, 64);
+        rank[r] = prev + before;
+    }
+    __syncthreads();
+    // exclusive prefix over waves per digit
+    for (int d = threadIdx.x; d < 256; d += RS_TPB) {
+        uint32_t acc = 0;
+        for (int ww = 0; ww < RS_WAVES; ++ww) { const uint32_t t = cnt[ww][d]; cnt[ww][d] = acc; acc += t; }
+    }
+    __syncthreads();
+    #pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        const uint32_t i = wbase + (uint32_t)r * WAVE + lane;
+        if (i < n) {
+            const uint32_t d = dig[r];
+            const uint32_t dst = boff[d] + cnt[w][d] + rank[r];
+            okeys[dst] = keys[i];
+            ovals[dst] = vals ? vals[i] : i;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-bitmap encoding.  Bitmap b owns sorted rows [start[b], start[b+1]);
+// one thread walks them and writes the canonical WAH code words of the rows
+// below groups*31 into tmp[toff_b ..] (room for 2*count+2 words), the active
+// bits (rows >= groups*31) into aval[b], the code word count into nw[b] and,
+// for bin indexes, the min / max of the bin's values.
+template <class T>
+__global__ __launch_bounds__(IX_TPB) void k_encode(const uint32_t* __restrict__ rows,
+                                                   const uint32_t* __restrict__ start, uint32_t nbm,
+                                                   uint32_t groups, uint32_t anb,
+                                                   const uint64_t* __restrict__ tmpoff,
+                                                   uint32_t* __restrict__ tmp,
+                                                   uint32_t* __restrict__ nw,
+                                                   uint32_t* __restrict__ aval,
+                                                   const T* __restrict__ col,
+                                                   double* __restrict__ vmin,
+                                                   double* __restrict__ vmax) {
+    const uint32_t b = blockIdx.x * IX_TPB + threadIdx.x;
+    if (b >= nbm) return;
+    uint32_t* out = tmp + tmpoff[b];
+    uint32_t k = 0;               // words written
+    int64_t last = -1;            // last group emitted (covered)
+    uint32_t run1 = 0;            // length of the open 1-run (consecutive full groups)
+    uint32_t run1slot = 0;
+    uint32_t av = 0;
+    double mn = DBL_MAX_D, mx = -DBL_MAX_D;
+    const uint32_t s = start[b], e = start[b + 1];
+    uint32_t i = s;
+    while (i < e) {
+        const uint32_t r = rows[i];
+        if (col) {
+            const double x = (double)col[r];
+            mn = x < mn ? x : mn;
+            mx = x > mx ? x : mx;
+        }
+        const uint32_t g = r / 31u;
+        if (g >= groups) {
+            av |= 1u << (anb - 1u - (r - groups * 31u));
+            ++i;
+            continue;
+        }
+        // literal of group g: all following rows in the same group
+        word_t v = 1u << (30u - (r - g * 31u));
+        uint32_t j = i + 1;
+        while (j < e) {
+            const uint32_t r2 = rows[j];
+            if (r2 / 31u != g) break;
+            if (col) {
+                const double x = (double)col[r2];
+                mn = x < mn ? x : mn;
+                mx = x > mx ? x : mx;
+            }
+            v |= 1u << (30u - (r2 - g * 31u));
+            ++j;
+        }
+        const int64_t gap = (int64_t)g - last - 1;
+        if (gap > 0) {
+            run1 = 0;
+            out[k++] = gap >= 2 ? (HEADER0 | (uint32_t)gap) : 0u;
+        }
+        if (v == ALLONES) {
+            if (run1 > 0 && gap == 0) {
+                ++run1;
+                out[run1slot] = HEADER1 | run1;
+            } else {
+                run1 = 1;
+                run1slot = k;
+                out[k++] = ALLONES;
+            }
+        } else {
+            run1 = 0;
+            out[k++] = v;
+        }
+        last = g;
+        i = j;
+    }
+    const int64_t gap = (int64_t)groups - last - 1;
+    if (gap > 0) out[k++] = gap >= 2 ? (HEADER0 | (uint32_t)gap) : 0u;
+    nw[b] = k;
+    aval[b] = av;
+    if (vmin) { vmin[b] = mn; vmax[b] = mx; }
+}
+
+// Concatenation: bitmap b's code words + trailer at off[b].
+__global__ __launch_bounds__(IX_TPB) void k_concat(const uint32_t* __restrict__ tmp,
+                                                   const uint64_t* __restrict__ tmpoff,
+                                                   const uint32_t* __restrict__ nw,
+                                                   const uint32_t* __restrict__ aval,
+                                                   const uint64_t* __restrict__ off, uint32_t nbm,
+                                                   uint32_t anb, uint32_t* __restrict__ out) {
+    // one block per bitmap
+    const uint32_t b = blockIdx.x;
+    if (b >= nbm) return;
+    if (off[b + 1] == off[b]) return;         // null bitmap: no words at all
+    const uint32_t* src = tmp + tmpoff[b];
+    uint32_t* dst = out + off[b];
+    const uint32_t n = nw[b];
+    for (uint32_t i = threadIdx.x; i < n; i += IX_TPB) dst[i] = src[i];
+    if (threadIdx.x == 0) {
+        if (anb > 0) { dst[n] = aval[b]; dst[n + 1] = anb; }
+        else dst[n] = 0u;
+    }
+}
+
+// Tile offset table: one thread per bitmap walks its code words.
+__global__ __launch_bounds__(IX_TPB) void k_toff(const uint32_t* __restrict__ words,
+                                                 const uint64_t* __restrict__ off,
+                                                 const uint32_t* __restrict__ nw, uint32_t nbm,
+                                                 uint32_t ntiles, uint32_t tg,
+                                                 uint2* __restrict__ toff) {
+    const uint32_t b = blockIdx.x * IX_TPB + threadIdx.x;
+    if (b >= nbm) return;
+    const uint32_t* w = words + off[b];
+    const uint32_t n = nw[b];
+    uint32_t pos = 0, t = 0;
+    for (uint32_t i = 0; i < n && t < ntiles; ++i) {
+        const uint32_t l = word_len(w[i]);
+        // word i covers [pos, pos+l): it is the first word of every tile
+        // whose start lies in that span
+        while (t < ntiles && (uint64_t)t * tg < (uint64_t)pos + l) {
+            toff[(size_t)t * nbm + b] = make_uint2(i, pos);
+            ++t;
+        }
+        pos += l;
+    }
+    for (; t < ntiles; ++t) toff[(size_t)t * nbm + b] = make_uint2(n, pos);
+}
+
+// ---------------------------------------------------------------------------
+// Range evaluation, LDS plan.  One block per tile of QT groups; the tile
+// result R[g] (plus the active slot at g == groups, left-aligned) is the OR
+// of bitmaps in up to two ranges, optionally complemented, then OR-ed with
+// the rows of the candidate (edge) bitmaps whose raw values satisfy the
+// range.
+template <class T>
+__device__ __forceinline__ bool in_range(const QueryArgs& q, T x) {
+    if (q.is_int) {
+        const int64_t v = (int64_t)x;
+        bool ok = true;
+        if (q.lop == OPC_LT) ok = v > q.ilo; else if (q.lop == OPC_LE) ok = v >= q.ilo;
+        if (q.rop == OPC_LT) ok = ok && v < q.ihi; else if (q.rop == OPC_LE) ok = ok && v <= q.ihi;
+        return ok;
+    }
+    const double v = (double)x;
+    bool ok = true;
+    if (q.lop == OPC_LT) ok = q.lo < v; else if (q.lop == OPC_LE) ok = q.lo <= v;
+    if (q.rop == OPC_LT) ok = ok && v < q.hi; else if (q.rop == OPC_LE) ok = ok && v <= q.hi;
+    return ok;
+}
+
+template <class T>
+__device__ void walk_segment(const QueryArgs& q, uint32_t b, uint32_t tile, uint32_t g0, uint32_t g1,
+                             word_t* R, bool check, const T* col, bool has_active) {
+    if (q.off[b + 1] == q.off[b]) return;     // null bitmap
+    const uint2 to = q.toff[(size_t)tile * q.nobs + b];
+    const uint32_t* w = q.words + q.off[b];
+    const uint32_t n = q.nw[b];
+    uint32_t i = to.x, pos = to.y;
+    while (i < n && pos < g1) {
+        const word_t x = w[i];
+        if (x > ALLONES) {
+            const uint32_t l = x & MAXCNT;
+            if (x >= HEADER1) {
+                const uint32_t lo = pos > g0 ? pos : g0;
+                const uint32_t hi = (pos + l) < g1 ? (pos + l) : g1;
+                for (uint32_t g = lo; g < hi; ++g) {
+                    if (!check) R[g - g0] = ALLONES;
+                    else {
+                        word_t m = 0;
+                        for (uint32_t k = 0; k < 31u; ++k)
+                            if (in_range(q, col[g * 31u + k])) m |= 1u << (30u - k);
+                        if (m) atomicOr(&R[g - g0], m);
+                    }
+                }
+            }
+            pos += l;
+        } else {
+            if (x && pos >= g0) {
+                if (!check) atomicOr(&R[pos - g0], x);
+                else {
+                    word_t m = 0, y = x;
+                    while (y) {
+                        const uint32_t hb = 31u - __clz(y);
+                        const uint32_t row = pos * 31u + (30u - hb);
+                        if (in_range(q, col[row])) m |= 1u << hb;
+                        y &= ~(1u << hb);
+                    }
+                    if (m) atomicOr(&R[pos - g0], m);
+                }
+            }
+            pos += 1;
+        }
+        ++i;
+    }
+    // the active word belongs to tile groups / QT (slot groups - g0)
+    if (has_active) {
+        const word_t av = w[n];
+        if (av) {
+            const word_t left = av << (31u - q.anb);
+            if (!check) atomicOr(&R[q.groups - g0], left);
+            else {
+                word_t m = 0, y = left;
+                while (y) {
+                    const uint32_t hb = 31u - __clz(y);
+                    const uint32_t row = q.groups * 31u + (30u - hb);
+                    if (in_range(q, col[row])) m |= 1u << hb;
+                    y &= ~(1u << hb);
+                }
+                if (m) atomicOr(&R[q.groups - g0], m);
+            }
+        }
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(IX_TPB) void k_query_lit(QueryArgs q) {
+    __shared__ word_t R[QT + 1];
+    const uint32_t tile = blockIdx.x;
+    const uint32_t g0 = tile * (uint32_t)QT;
+    // groups of this tile, plus the active slot in tile groups / QT
+    const uint32_t g1 = min(g0 + (uint32_t)QT, q.groups);
+    const bool has_active = (tile == q.groups / (uint32_t)QT) && q.anb > 0;
+    const uint32_t nslot = (g1 > g0 ? g1 - g0 : 0u) + (has_active ? 1u : 0u);
+    for (uint32_t k = threadIdx.x; k < nslot; k += IX_TPB) R[k] = 0u;
+    __syncthreads();
+    const T* col = (const T*)q.col;
+    for (int r = 0; r < 2; ++r) {
+        const uint32_t b0 = r == 0 ? q.r0 : q.r2, b1 = r == 0 ? q.r1 : q.r3;
+        for (uint32_t b = b0 + threadIdx.x; b < b1; b += IX_TPB)
+            walk_segment<T>(q, b, tile, g0, g1, R, false, col, has_active);
+    }
+    __syncthreads();
+    if (q.flip) {
+        for (uint32_t k = threadIdx.x; k < nslot; k += IX_TPB) {
+            const uint32_t g = g0 + k;
+            const word_t mask = g < q.groups ? ALLONES : (((1u << q.anb) - 1u) << (31u - q.anb));
+            R[k] ^= mask;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 2) {
+        const uint32_t cb = threadIdx.x == 0 ? q.cand0 : q.cand1;
+        if (cb != NOKEY) walk_segment<T>(q, cb, tile, g0, g1, R, true, col, has_active);
+    }
+    __syncthreads();
+    uint32_t ones = 0;
+    for (uint32_t k = threadIdx.x; k < nslot; k += IX_TPB) {
+        const word_t v = R[k];
+        q.out[g0 + k] = v;
+        ones += __popc(v);
+    }
+    ones = wave_sum(ones);
+    if (lane_id() == 0 && ones) atomicAdd(q.count, ones);
+}
+
+// Raw scan plan: R[g] = rows 31g..31g+30 that satisfy the range.  A wave
+// covers 64 groups (1984 rows): 31 coalesced loads of 64 values, 31
+// ballots staged in LDS, each lane assembles its group.
+template <class T>
+__global__ __launch_bounds__(IX_TPB) void k_query_scan(QueryArgs q, uint32_t nrows) {
+    __shared__ uint64_t bal[IX_TPB / WAVE][32];
+    const int lane = (int)lane_id();
+    const int w = threadIdx.x / WAVE;
+    const T* col = (const T*)q.col;
+    const uint32_t ngroups_all = q.groups + (q.anb ? 1u : 0u);
+    uint32_t ones = 0;
+    for (uint32_t gb = (blockIdx.x * (IX_TPB / WAVE) + w) * 64u; gb < ngroups_all;
+         gb += gridDim.x * (IX_TPB / WAVE) * 64u) {
+        const uint32_t rbase = gb * 31u;
+        #pragma unroll 4
+        for (int k = 0; k < 31; ++k) {
+            const uint32_t row = rbase + (uint32_t)k * 64u + lane;
+            bool hit = false;
+            if (row < nrows) {
+                const T x = col[row];
+                if (q.scan_bin) {
+                    const uint32_t b = bin_locate(q.bounds, q.nbounds, (double)x);
+                    hit = (b >= q.h0 && b < q.h1) || ((b == q.e0 || b == q.e1) && in_range(q, x));
+                } else {
+                    hit = in_range(q, x);
+                }
+            }
+            const uint64_t m = __ballot(hit);
+            if (lane == 0) bal[w][k] = m;
+        }
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t g = gb + lane;
+        if (g < ngroups_all) {
+            const uint32_t bit0 = (uint32_t)lane * 31u;      // in the 1984-bit stream
+            const uint32_t k0 = bit0 >> 6, s0 = bit0 & 63u;
+            uint64_t lo = bal[w][k0] >> s0;
+            if (s0 > 33u && k0 + 1 < 31u) lo |= bal[w][k0 + 1] << (64u - s0);
+            const uint32_t bits = (uint32_t)(lo & 0x7FFFFFFFull);   // bit j = row 31g+j
+            word_t v = __brev(bits) >> 1;                          // row 31g -> bit 30
+            if (g == q.groups) v &= (((1u << q.anb) - 1u) << (31u - q.anb));
+            q.out[g] = v;
+            ones += __popc(v);
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    ones = wave_sum(ones);
+    if (lane == 0 && ones) atomicAdd(q.count, ones);
+}
+
+// ---------------------------------------------------------------------------
+// Distinct values and bitmap starts.
+// Order-preserving 32-bit key of a value.
+template <class T> __device__ __forceinline__ uint32_t order_key(T v);
+template <> __device__ __forceinline__ uint32_t order_key<int32_t>(int32_t v) { return (uint32_t)v ^ 0x80000000u; }
+template <> __device__ __forceinline__ uint32_t order_key<uint32_t>(uint32_t v) { return v; }
+template <> __device__ __forceinline__ uint32_t order_key<uint16_t>(uint16_t v) { return v; }
+template <> __device__ __forceinline__ uint32_t order_key<float>(float v) {
+    const uint32_t u = __float_as_uint(v);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+template <class T>
+__global__ __launch_bounds__(IX_TPB) void k_order_keys(const T* __restrict__ col, uint32_t n,
+                                                       uint32_t* __restrict__ keys) {
+    const uint32_t i = blockIdx.x * IX_TPB + threadIdx.x;
+    if (i < n) keys[i] = order_key<T>(col[i]);
+}
+
+// flag[i] = 1 when sorted key i differs from key i-1
+__global__ __launch_bounds__(IX_TPB) void k_unique_flags(const uint32_t* __restrict__ k, uint32_t n,
+                                                         uint32_t* __restrict__ flag) {
+    const uint32_t i = blockIdx.x * IX_TPB + threadIdx.x;
+    if (i < n) flag[i] = (i == 0 || k[i] != k[i - 1]) ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(IX_TPB) void k_unique_scatter(const uint32_t* __restrict__ k, uint32_t n,
+                                                           const uint32_t* __restrict__ flag,
+                                                           const uint32_t* __restrict__ pos,
+                                                           uint32_t* __restrict__ out) {
+    const uint32_t i = blockIdx.x * IX_TPB + threadIdx.x;
+    if (i < n && flag[i]) out[pos[i]] = k[i];
+}
+
+// start[b] = first index of key b in the sorted keys (b in [0, nbm])
+__global__ __launch_bounds__(IX_TPB) void k_starts(const uint32_t* __restrict__ k, uint32_t n,
+                                                   uint32_t nbm, uint32_t* __restrict__ start) {
+    const uint32_t b = blockIdx.x * IX_TPB + threadIdx.x;
+    if (b > nbm) return;
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (k[mid] < b) lo = mid + 1; else hi = mid;
+    }
+    start[b] = lo;
+}
+
+// tmp offsets of the encoder: 2*count+2 words per bitmap (exclusive scan
+// input), computed from the starts
+__global__ __launch_bounds__(IX_TPB) void k_enc_room(const uint32_t* __restrict__ start, uint32_t nbm,
+                                                     uint32_t* __restrict__ room) {
+    const uint32_t b = blockIdx.x * IX_TPB + threadIdx.x;
+    if (b < nbm) room[b] = 2u * (start[b + 1] - start[b]) + 2u;
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers.
+static inline uint32_t cdivu(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+hipError_t launch_keys_relic(const void* col, int type, uint32_t n, const double* vals, uint32_t nv,
+                             uint32_t* keys, hipStream_t s) {
+    dim3 g(cdivu(n, IX_TPB)), b(IX_TPB);
+    switch (type) {
+    case COL_I32: hipLaunchKernelGGL(k_keys_relic<int32_t>, g, b, 0, s, (const int32_t*)col, n, vals, nv, keys); break;
+    case COL_U32: hipLaunchKernelGGL(k_keys_relic<uint32_t>, g, b, 0, s, (const uint32_t*)col, n, vals, nv, keys); break;
+    case COL_U16: hipLaunchKernelGGL(k_keys_relic<uint16_t>, g, b, 0, s, (const uint16_t*)col, n, vals, nv, keys); break;
+    case COL_F32: hipLaunchKernelGGL(k_keys_relic<float>, g, b, 0, s, (const float*)col, n, vals, nv, keys); break;
+    case COL_F64: hipLaunchKernelGGL(k_keys_relic<double>, g, b, 0, s, (const double*)col, n, vals, nv, keys); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_keys_card(const uint32_t* ind, uint32_t n, uint32_t card, uint32_t* keys,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(k_keys_card, dim3(cdivu(n, IX_TPB)), dim3(IX_TPB), 0, s, ind, n, card, keys);
+    return hipGetLastError();
+}
+
+hipError_t launch_keys_bin(const void* col, int type, uint32_t n, const double* bounds, uint32_t nobs,
+                           uint32_t* keys, hipStream_t s) {
+    dim3 g(cdivu(n, IX_TPB)), b(IX_TPB);
+    switch (type) {
+    case COL_I32: hipLaunchKernelGGL(k_keys_bin<int32_t>, g, b, 0, s, (const int32_t*)col, n, bounds, nobs, keys); break;
+    case COL_U32: hipLaunchKernelGGL(k_keys_bin<uint32_t>, g, b, 0, s, (const uint32_t*)col, n, bounds, nobs, keys); break;
+    case COL_U16: hipLaunchKernelGGL(k_keys_bin<uint16_t>, g, b, 0, s, (const uint16_t*)col, n, bounds, nobs, keys); break;
+    case COL_F32: hipLaunchKernelGGL(k_keys_bin<float>, g, b, 0, s, (const float*)col, n, bounds, nobs, keys); break;
+    case COL_F64: hipLaunchKernelGGL(k_keys_bin<double>, g, b, 0, s, (const double*)col, n, bounds, nobs, keys); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_order_keys(const void* col, int type, uint32_t n, uint32_t* keys, hipStream_t s) {
+    dim3 g(cdivu(n, IX_TPB)), b(IX_TPB);
+    switch (type) {
+    case COL_I32: hipLaunchKernelGGL(k_order_keys<int32_t>, g, b, 0, s, (const int32_t*)col, n, keys); break;
+    case COL_U32: hipLaunchKernelGGL(k_order_keys<uint32_t>, g, b, 0, s, (const uint32_t*)col, n, keys); break;
+    case COL_U16: hipLaunchKernelGGL(k_order_keys<uint16_t>, g, b, 0, s, (const uint16_t*)col, n, keys); break;
+    case COL_F32: hipLaunchKernelGGL(k_order_keys<float>, g, b, 0, s, (const float*)col, n, keys); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// Distinct sorted keys: out receives them, *ctl the count (device).
+// tmp needs 2*n + scan_tmp_words(n) words.
+hipError_t launch_unique(const uint32_t* k, uint32_t n, uint32_t* out, uint32_t* tmp, hipStream_t s) {
+    if (!n) return hipSuccess;
+    uint32_t* flag = tmp;
+    uint32_t* pos = tmp + n;
+    hipLaunchKernelGGL(k_unique_flags, dim3(cdivu(n, IX_TPB)), dim3(IX_TPB), 0, s, k, n, flag);
+    hipError_t e = launch_scan_excl(flag, pos, n, tmp + 2 * n, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_unique_scatter, dim3(cdivu(n, IX_TPB)), dim3(IX_TPB), 0, s, k, n,
+                       (const uint32_t*)flag, (const uint32_t*)pos, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_starts(const uint32_t* k, uint32_t n, uint32_t nbm, uint32_t* start, uint32_t* room,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(k_starts, dim3(cdivu(nbm + 1, IX_TPB)), dim3(IX_TPB), 0, s, k, n, nbm, start);
+    if (room && nbm)
+        hipLaunchKernelGGL(k_enc_room, dim3(cdivu(nbm, IX_TPB)), dim3(IX_TPB), 0, s,
+                           (const uint32_t*)start, nbm, room);
+    return hipGetLastError();
+}
+
+uint32_t rs_blocks(uint32_t n) { return n ? cdivu(n, RS_TILE) : 1u; }
+
+// One radix pass: (ki, vi) -> (ko, vo) by digit at `shift`.  tmp needs
+// 2*256*rs_blocks(n) + scan_tmp_words(256*rs_blocks(n)) words.
+hipError_t launch_rs_pass(const uint32_t* ki, const uint32_t* vi, uint32_t n, uint32_t shift,
+                          uint32_t* ko, uint32_t* vo, uint32_t* tmp, hipStream_t s) {
+    const uint32_t nb = rs_blocks(n);
+    uint32_t* hist = tmp;
+    uint32_t* offs = tmp + 256u * nb;
+    hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(RS_TPB), 0, s, ki, n, shift, nb, hist);
+    hipError_t e = launch_scan_excl(hist, offs, 256u * nb, tmp + 512u * nb, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(RS_TPB), 0, s, ki, vi, n, shift, nb,
+                       (const uint32_t*)offs, ko, vo);
+    return hipGetLastError();
+}
+
+hipError_t launch_encode(int type, const uint32_t* rows, const uint32_t* start, uint32_t nbm,
+                         uint32_t groups, uint32_t anb, const uint64_t* tmpoff, uint32_t* tmp,
+                         uint32_t* nw, uint32_t* aval, const void* col, double* vmin, double* vmax,
+                         hipStream_t s) {
+    dim3 g(cdivu(nbm, IX_TPB)), b(IX_TPB);
+    if (!col) {
+        hipLaunchKernelGGL(k_encode<float>, g, b, 0, s, rows, start, nbm, groups, anb, tmpoff, tmp, nw,
+                           aval, (const float*)nullptr, (double*)nullptr, (double*)nullptr);
+        return hipGetLastError();
+    }
+    switch (type) {
+    case COL_I32: hipLaunchKernelGGL(k_encode<int32_t>, g, b, 0, s, rows, start, nbm, groups, anb, tmpoff, tmp, nw, aval, (const int32_t*)col, vmin, vmax); break;
+    case COL_U32: hipLaunchKernelGGL(k_encode<uint32_t>, g, b, 0, s, rows, start, nbm, groups, anb, tmpoff, tmp, nw, aval, (const uint32_t*)col, vmin, vmax); break;
+    case COL_U16: hipLaunchKernelGGL(k_encode<uint16_t>, g, b, 0, s, rows, start, nbm, groups, anb, tmpoff, tmp, nw, aval, (const uint16_t*)col, vmin, vmax); break;
+    case COL_F32: hipLaunchKernelGGL(k_encode<float>, g, b, 0, s, rows, start, nbm, groups, anb, tmpoff, tmp, nw, aval, (const float*)col, vmin, vmax); break;
+    case COL_F64: hipLaunchKernelGGL(k_encode<double>, g, b, 0, s, rows, start, nbm, groups, anb, tmpoff, tmp, nw, aval, (const double*)col, vmin, vmax); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_concat(const uint32_t* tmp, const uint64_t* tmpoff, const uint32_t* nw,
+                         const uint32_t* aval, const uint64_t* off, uint32_t nbm, uint32_t anb,
+                         uint32_t* out, hipStream_t s) {
+    if (!nbm) return hipSuccess;
+    hipLaunchKernelGGL(k_concat, dim3(nbm), dim3(IX_TPB), 0, s, tmp, tmpoff, nw, aval, off, nbm, anb, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_toff(const uint32_t* words, const uint64_t* off, const uint32_t* nw, uint32_t nbm,
+                       uint32_t ntiles, uint32_t tg, uint2* toff, hipStream_t s) {
+    if (!nbm) return hipSuccess;
+    hipLaunchKernelGGL(k_toff, dim3(cdivu(nbm, IX_TPB)), dim3(IX_TPB), 0, s, words, off, nw, nbm, ntiles, tg, toff);
+    return hipGetLastError();
+}
+
+hipError_t launch_query_lit(const QueryArgs& q, int type, hipStream_t s) {
+    const uint32_t nt = q.groups / QT + 1;   // includes the active slot's tile
+    dim3 g(nt), b(IX_TPB);
+    switch (type) {
+    case COL_I32: hipLaunchKernelGGL(k_query_lit<int32_t>, g, b, 0, s, q); break;
+    case COL_U32: hipLaunchKernelGGL(k_query_lit<uint32_t>, g, b, 0, s, q); break;
+    case COL_U16: hipLaunchKernelGGL(k_query_lit<uint16_t>, g, b, 0, s, q); break;
+    case COL_F32: hipLaunchKernelGGL(k_query_lit<float>, g, b, 0, s, q); break;
+    case COL_F64: hipLaunchKernelGGL(k_query_lit<double>, g, b, 0, s, q); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_query_scan(const QueryArgs& q, int type, uint32_t nrows, hipStream_t s) {
+    const uint32_t ng = q.groups + (q.anb ? 1u : 0u);
+    const uint32_t waves = cdivu(ng, 64u);
+    const uint32_t blocks = min(cdivu(waves, IX_TPB / WAVE), 8192u);
+    dim3 g(blocks ? blocks : 1u), b(IX_TPB);
+    switch (type) {
+    case COL_I32: hipLaunchKernelGGL(k_query_scan<int32_t>, g, b, 0, s, q, nrows); break;
+    case COL_U32: hipLaunchKernelGGL(k_query_scan<uint32_t>, g, b, 0, s, q, nrows); break;
+    case COL_U16: hipLaunchKernelGGL(k_query_scan<uint16_t>, g, b, 0, s, q, nrows); break;
+    case COL_F32: hipLaunchKernelGGL(k_query_scan<float>, g, b, 0, s, q, nrows); break;
+    case COL_F64: hipLaunchKernelGGL(k_query_scan<double>, g, b, 0, s, q, nrows); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+} // namespace fbk

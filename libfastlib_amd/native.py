@@ -190,3 +190,148 @@ def bench_op(a: Bitvector, b: Bitvector, op: int, reps: int):
     check(lib().fbh_bench_op(a._h, b._h, int(op), int(reps), C.byref(ms), C.byref(nw)),
           "fbh_bench_op")
     return ms.value, nw.value
+
+
+# ---------------------------------------------------------------------------
+# Indexes
+RELIC, BIN = 1, 2
+COL_INT32, COL_UINT32, COL_UINT16, COL_FLOAT, COL_DOUBLE = 0, 1, 2, 3, 4
+OP_NONE, OP_LT, OP_LE = 0, 1, 2
+PLAN_AUTO, PLAN_LITERAL, PLAN_COMPRESSED, PLAN_SCAN = 0, 1, 2, 3
+PLAN_NAMES = {0: "auto", 1: "literal", 2: "compressed", 3: "scan"}
+_COLT = {np.dtype(np.int32): COL_INT32, np.dtype(np.uint32): COL_UINT32,
+         np.dtype(np.uint16): COL_UINT16, np.dtype(np.float32): COL_FLOAT,
+         np.dtype(np.float64): COL_DOUBLE}
+_dp = C.POINTER(C.c_double)
+_i64p = C.POINTER(C.c_int64)
+_u64p = C.POINTER(C.c_uint64)
+
+
+def _index_sigs(L):
+    vp = C.c_void_p
+    sig = {
+        "fbh_index_build_relic": [vp, C.c_int, C.c_uint32, C.POINTER(vp)],
+        "fbh_index_build_relic_card": [_u32p, C.c_uint32, C.c_uint32, C.POINTER(vp)],
+        "fbh_index_build_bin": [vp, C.c_int, C.c_uint32, _dp, C.c_uint32, C.POINTER(vp)],
+        "fbh_index_info": [vp, _u32p, _u32p, _u64p],
+        "fbh_index_deconstruct": [vp, C.POINTER(_dp), _u64p, C.POINTER(_i64p), _u64p,
+                                  C.POINTER(_u32p), _u64p],
+        "fbh_index_resolve_range": [vp, C.c_double, C.c_int, C.c_double, C.c_int, _u32p, _u32p,
+                                    _u32p, _u32p],
+        "fbh_index_estimate_cost": [vp, C.c_double, C.c_int, C.c_double, C.c_int, _dp, _dp,
+                                    C.POINTER(C.c_int)],
+        "fbh_index_evaluate": [vp, C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, _u64p,
+                               C.POINTER(vp), C.POINTER(C.c_int)],
+        "fbh_index_evaluate_rids": [vp, C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, _u32p,
+                                    C.c_uint64, _u64p],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    L.fbh_index_free.argtypes = [vp]
+    L.fbh_index_free.restype = None
+    L.fbh_free.argtypes = [vp]
+    L.fbh_free.restype = None
+
+
+class Index:
+    """ibis::relic / ibis::bin with its raw column, in HBM."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @staticmethod
+    def _lib():
+        L = lib()
+        if not getattr(L, "_index_sigs", False):
+            _index_sigs(L)
+            L._index_sigs = True
+        return L
+
+    @classmethod
+    def relic(cls, col):
+        L = cls._lib()
+        a = np.ascontiguousarray(col)
+        h = C.c_void_p()
+        check(L.fbh_index_build_relic(a.ctypes.data_as(C.c_void_p), _COLT[a.dtype], a.size,
+                                      C.byref(h)), "fbh_index_build_relic")
+        return cls(h)
+
+    @classmethod
+    def relic_card(cls, ind, card):
+        L = cls._lib()
+        a = np.ascontiguousarray(ind, np.uint32)
+        h = C.c_void_p()
+        check(L.fbh_index_build_relic_card(a.ctypes.data_as(_u32p), a.size, int(card), C.byref(h)),
+              "fbh_index_build_relic_card")
+        return cls(h)
+
+    @classmethod
+    def bin(cls, col, bounds):
+        L = cls._lib()
+        a = np.ascontiguousarray(col)
+        b = np.ascontiguousarray(bounds, np.float64)
+        h = C.c_void_p()
+        check(L.fbh_index_build_bin(a.ctypes.data_as(C.c_void_p), _COLT[a.dtype], a.size,
+                                    b.ctypes.data_as(_dp), b.size, C.byref(h)), "fbh_index_build_bin")
+        return cls(h)
+
+    def __del__(self):
+        try:
+            if self._h is not None and _lib is not None:
+                _lib.fbh_index_free(self._h)
+        except Exception:
+            pass
+        self._h = None
+
+    def info(self):
+        nobs, nrows, nw = C.c_uint32(), C.c_uint32(), C.c_uint64()
+        check(self._lib().fbh_index_info(self._h, C.byref(nobs), C.byref(nrows), C.byref(nw)),
+              "fbh_index_info")
+        return nobs.value, nrows.value, nw.value
+
+    def deconstruct(self):
+        L = self._lib()
+        k, o, b = _dp(), _i64p(), _u32p()
+        nk, no, nb = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        check(L.fbh_index_deconstruct(self._h, C.byref(k), C.byref(nk), C.byref(o), C.byref(no),
+                                      C.byref(b), C.byref(nb)), "fbh_index_deconstruct")
+        out = (np.ctypeslib.as_array(k, (nk.value,)).copy() if nk.value else np.zeros(0),
+               np.ctypeslib.as_array(o, (no.value,)).copy(),
+               np.ctypeslib.as_array(b, (nb.value,)).copy() if nb.value else np.zeros(0, np.uint32))
+        for p in (k, o, b):
+            L.fbh_free(p)
+        return out
+
+    def resolve_range(self, lo, lop, hi, rop):
+        r = [C.c_uint32() for _ in range(4)]
+        check(self._lib().fbh_index_resolve_range(self._h, lo, lop, hi, rop, *[C.byref(x) for x in r]),
+              "fbh_index_resolve_range")
+        return tuple(x.value for x in r)
+
+    def estimate_cost(self, lo, lop, hi, rop):
+        ic, sc, plan = C.c_double(), C.c_double(), C.c_int()
+        check(self._lib().fbh_index_estimate_cost(self._h, lo, lop, hi, rop, C.byref(ic), C.byref(sc),
+                                                  C.byref(plan)), "fbh_index_estimate_cost")
+        return ic.value, sc.value, plan.value
+
+    def evaluate(self, lo, lop, hi, rop, plan=PLAN_AUTO, want_bitvector=False):
+        """(hits, plan used, result Bitvector or None)."""
+        hits = C.c_uint64()
+        used = C.c_int()
+        bv = C.c_void_p()
+        check(self._lib().fbh_index_evaluate(self._h, lo, lop, hi, rop, plan, C.byref(hits),
+                                             C.byref(bv) if want_bitvector else None, C.byref(used)),
+              "fbh_index_evaluate")
+        return hits.value, used.value, (Bitvector(bv) if want_bitvector else None)
+
+    def rids(self, lo, lop, hi, rop, plan=PLAN_LITERAL):
+        L = self._lib()
+        n = C.c_uint64()
+        check(L.fbh_index_evaluate_rids(self._h, lo, lop, hi, rop, plan, None, 0, C.byref(n)),
+              "fbh_index_evaluate_rids")
+        out = np.zeros(max(n.value, 1), np.uint32)
+        check(L.fbh_index_evaluate_rids(self._h, lo, lop, hi, rop, plan, out.ctypes.data_as(_u32p),
+                                        n.value, C.byref(n)), "fbh_index_evaluate_rids")
+        return out[: n.value]

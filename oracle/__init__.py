@@ -157,3 +157,205 @@ def reference():
     if _ref is None and os.path.exists(REF_SO):
         _ref = Reference(REF_SO)
     return _ref
+
+
+# ---------------------------------------------------------------------------
+# relic / bin oracle (index.cpp) and the reference's index API.
+COL_I32, COL_U32, COL_U16, COL_F32, COL_F64 = 0, 1, 2, 3, 4
+_COL_NP = {COL_I32: np.int32, COL_U32: np.uint32, COL_U16: np.uint16, COL_F32: np.float32,
+           COL_F64: np.float64}
+_dp = C.POINTER(C.c_double)
+_i64p = C.POINTER(C.c_int64)
+
+
+def col_type_of(arr):
+    a = np.asarray(arr)
+    for k, t in _COL_NP.items():
+        if a.dtype == t:
+            return k
+    raise TypeError(a.dtype)
+
+
+class OracleIndex:
+    """ibis::relic / ibis::bin restated on the CPU (oracle/index.cpp)."""
+
+    def __init__(self, handle, col):
+        self._h = handle
+        self.col = np.ascontiguousarray(col)
+        self.type = col_type_of(self.col)
+
+    @classmethod
+    def relic_card(cls, ind, card):
+        L = oracle().lib
+        L.orc_relic_card.restype = C.c_void_p
+        a = np.ascontiguousarray(ind, np.uint32)
+        return cls(L.orc_relic_card(a.ctypes.data_as(_u32p), C.c_uint32(a.size), C.c_uint32(card)), a)
+
+    @classmethod
+    def relic_values(cls, col):
+        L = oracle().lib
+        L.orc_relic_values.restype = C.c_void_p
+        v = np.ascontiguousarray(col, np.float64)
+        return cls(L.orc_relic_values(v.ctypes.data_as(_dp), C.c_uint32(v.size)), col)
+
+    @classmethod
+    def bin_bounds(cls, col, bounds):
+        L = oracle().lib
+        L.orc_bin_bounds.restype = C.c_void_p
+        v = np.ascontiguousarray(col, np.float64)
+        b = np.ascontiguousarray(bounds, np.float64)
+        return cls(L.orc_bin_bounds(v.ctypes.data_as(_dp), C.c_uint32(v.size), b.ctypes.data_as(_dp),
+                                    C.c_uint32(b.size)), col)
+
+    def __del__(self):
+        try:
+            oracle().lib.orc_index_free(C.c_void_p(self._h))
+        except Exception:
+            pass
+
+    def write(self):
+        """(keys, offsets, bitmaps) as relic::write / bin::write give them."""
+        L = oracle().lib
+        k, o, b = _dp(), _i64p(), _u32p()
+        nk, no, nb = C.c_size_t(), C.c_size_t(), C.c_size_t()
+        L.orc_index_write(C.c_void_p(self._h), C.byref(k), C.byref(nk), C.byref(o), C.byref(no),
+                          C.byref(b), C.byref(nb))
+        out = (np.ctypeslib.as_array(k, (nk.value,)).copy() if nk.value else np.zeros(0),
+               np.ctypeslib.as_array(o, (no.value,)).copy() if no.value else np.zeros(0, np.int64),
+               np.ctypeslib.as_array(b, (nb.value,)).copy() if nb.value else np.zeros(0, np.uint32))
+        for p in (k, o, b):
+            L.orc_free(p)
+        return out
+
+    def bounds(self):
+        L = oracle().lib
+        p, n = _dp(), C.c_size_t()
+        L.orc_index_bounds(C.c_void_p(self._h), C.byref(p), C.byref(n))
+        out = np.ctypeslib.as_array(p, (n.value,)).copy()
+        L.orc_free(p)
+        return out
+
+    def evaluate(self, lo, lop, hi, rop):
+        """(hit count, ascending rids)."""
+        L = oracle().lib
+        cnt = C.c_uint64()
+        r, nr = _u32p(), C.c_size_t()
+        rc = L.orc_index_eval(C.c_void_p(self._h), self.col.ctypes.data_as(C.c_void_p), self.type,
+                              C.c_double(lo), C.c_int(lop), C.c_double(hi), C.c_int(rop),
+                              C.byref(cnt), C.byref(r), C.byref(nr))
+        if rc != 0:
+            raise RuntimeError("orc_index_eval failed")
+        rids = np.ctypeslib.as_array(r, (nr.value,)).copy() if nr.value else np.zeros(0, np.uint32)
+        L.orc_free(r)
+        return int(cnt.value), rids
+
+
+# FastBitDataType codes of the reference's iapi (src/iapi.h)
+_FB_TYPE = {COL_I32: 5, COL_U32: 6, COL_U16: 4, COL_F32: 9, COL_F64: 10}
+
+
+def ref_relic_card(ind, card):
+    r = reference()
+    L = r.lib
+    a = np.ascontiguousarray(ind, np.uint32)
+    k, o, b = _dp(), _i64p(), _u32p()
+    nk, no, nb = C.c_size_t(), C.c_size_t(), C.c_size_t()
+    rc = L.ref_relic_card(a.ctypes.data_as(_u32p), C.c_size_t(a.size), C.c_uint32(card), C.byref(k),
+                          C.byref(nk), C.byref(o), C.byref(no), C.byref(b), C.byref(nb))
+    if rc != 0:
+        raise RuntimeError("ref_relic_card failed")
+    out = (np.ctypeslib.as_array(k, (nk.value,)).copy(), np.ctypeslib.as_array(o, (no.value,)).copy(),
+           np.ctypeslib.as_array(b, (nb.value,)).copy())
+    for p in (k, o, b):
+        L.ref_free(p)
+    return out
+
+
+class RefIapi:
+    """One array registered in the reference's in-memory API."""
+
+    def __init__(self, name, col, spec):
+        self.name = name.encode()
+        self.col = np.ascontiguousarray(col)
+        L = reference().lib
+        L.ref_iapi_query.restype = C.c_int64
+        k, o, b = _dp(), _i64p(), _u32p()
+        nk, no, nb = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        rc = L.ref_iapi_index(self.name, _FB_TYPE[col_type_of(self.col)],
+                              self.col.ctypes.data_as(C.c_void_p), C.c_uint64(self.col.size),
+                              spec.encode(), C.byref(k), C.byref(nk), C.byref(o), C.byref(no),
+                              C.byref(b), C.byref(nb))
+        if rc < 0:
+            raise RuntimeError(f"ref_iapi_index failed rc={rc}")
+        self.keys = np.ctypeslib.as_array(k, (nk.value,)).copy()
+        self.offsets = np.ctypeslib.as_array(o, (no.value,)).copy()
+        self.bitmaps = np.ctypeslib.as_array(b, (nb.value,)).copy()
+        for p in (k, o, b):
+            L.ref_free(p)
+
+    def query(self, lo, lop, hi, rop, want_rids=True):
+        L = reference().lib
+        cap = self.col.size if want_rids else 0
+        buf = np.zeros(max(cap, 1), np.uint64)
+        n = L.ref_iapi_query(self.name, C.c_double(lo), C.c_int(lop), C.c_double(hi), C.c_int(rop),
+                             buf.ctypes.data_as(C.POINTER(C.c_uint64)) if want_rids else None,
+                             C.c_uint64(cap))
+        if n < 0:
+            raise RuntimeError(f"ref_iapi_query failed {n}")
+        return int(n), (buf[:n].astype(np.uint32) if want_rids else None)
+
+
+def bounds_from_keys(keys):
+    """bin bounds as bin(..., keys, offsets) rebuilds them
+    (src/ibin.cpp:358-372): compactValue(maxval[j], minval[j+1]), DBL_MAX."""
+    nb = len(keys) // 2
+    mn, mx = keys[:nb], keys[nb:]
+    out = [compact_value(mx[j], mn[j + 1]) for j in range(nb - 1)]
+    out.append(np.finfo(np.float64).max)
+    return np.array(out, np.float64)
+
+
+def compact_value(left, right, start=0.0):
+    """util::compactValue (src/util.cpp:1227-1312)."""
+    import math
+    FLT_EPS = 1.1920928955078125e-07
+    if left == right:
+        return left
+    if not (left != right):
+        return 0.0
+    if left > right:
+        left, right = right, left
+    if left < 0 and right >= 0:
+        return 0.0
+    if left < 1.0 and right >= 1.0:
+        return 1.0
+    if left < -1.0 and right >= -1.0:
+        return -1.0
+    if left == 0.0:
+        diff = math.floor(math.log10(right))
+        sep = 10.0 ** diff
+        if sep > right:
+            sep *= 0.5 if (-3.0 <= diff < 3.0) else 0.1
+    elif right < 0.0 and right * 10 > left:
+        diff = math.ceil(math.log10(-right))
+        sep = -(10.0 ** diff)
+        if sep > right:
+            sep = sep + sep if (-3.0 <= diff <= 3.0) else sep * 10
+    elif left > 0.0 and right > 10 * left:
+        diff = math.ceil(math.log10(left))
+        sep = 10.0 ** diff
+        if sep <= left:
+            sep = sep + sep if (sep >= -3.0 and diff <= 3.0) else sep * 10.0
+    else:
+        diff = 10.0 ** math.ceil(FLT_EPS + math.log10(right - left))
+        if not (start > left and start <= right):
+            start = 0.5 * (right + left)
+        sep = math.floor(0.5 + start / diff) * diff
+        for div in (2, 5, 2, 2):
+            if sep > left and sep <= right:
+                break
+            diff /= div
+            sep = math.floor(0.5 + start / diff) * diff
+    if not (sep > left and sep <= right):
+        sep = right
+    return sep

@@ -1,1 +1,60 @@
+// C entry points of the index oracle (ctypes).  TEST INFRASTRUCTURE ONLY.
 #include "index.hpp"
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+using namespace orc;
+
+template <class T>
+static T* dup(const std::vector<T>& v) {
+    T* p = (T*)std::malloc(sizeof(T) * (v.size() ? v.size() : 1));
+    if (p && !v.empty()) std::memcpy(p, v.data(), sizeof(T) * v.size());
+    return p;
+}
+
+extern "C" {
+
+void* orc_relic_card(const uint32_t* ind, uint32_t n, uint32_t card) { return relic_card(ind, n, card); }
+void* orc_relic_values(const double* v, uint32_t n) { return relic_values(v, n); }
+void* orc_bin_bounds(const double* v, uint32_t n, const double* b, uint32_t nb) {
+    return bin_bounds(v, n, b, nb);
+}
+void orc_index_free(void* ix) { delete (Index*)ix; }
+
+int orc_index_write(void* ix, double** keys, size_t* nk, int64_t** offs, size_t* no, uint32_t** bms,
+                    size_t* nb) {
+    std::vector<double> k;
+    std::vector<int64_t> o;
+    std::vector<word_t> b;
+    write_index(*(Index*)ix, k, o, b);
+    *keys = dup(k); *nk = k.size();
+    *offs = dup(o); *no = o.size();
+    *bms = dup(b); *nb = b.size();
+    return 0;
+}
+
+int orc_index_bounds(void* ix, double** bounds, size_t* n) {
+    const Index* p = (Index*)ix;
+    *bounds = dup(p->vals);
+    *n = p->vals.size();
+    return 0;
+}
+
+// hits: count and ascending rids
+int orc_index_eval(void* ix, const void* col, int type, double lo, int lop, double hi, int rop,
+                   uint64_t* count, uint32_t** rids, size_t* nr) {
+    try {
+        BV r = evaluate(*(Index*)ix, col, type, lo, lop, hi, rop);
+        std::vector<word_t> v;
+        r.rids(v);
+        *count = r.cnt();
+        *rids = dup(v);
+        *nr = v.size();
+        return 0;
+    } catch (...) {
+        return -2;
+    }
+}
+
+} // extern "C"

@@ -130,3 +130,75 @@ int ref_wah_rids(const uint32_t* a, size_t na, uint32_t** out, size_t* nout) {
 }
 
 } // extern "C"
+
+// ---------------------------------------------------------------------------
+// Indexes through the reference's own API.
+template <class T>
+static T* dupv(const T* p, size_t n) {
+    T* q = (T*)std::malloc(sizeof(T) * (n ? n : 1));
+    if (q && n) std::memcpy(q, p, sizeof(T) * n);
+    return q;
+}
+
+extern "C" {
+
+// relic(col, card, ind) (src/irelic.cpp:160-211), then relic::write
+int ref_relic_card(const uint32_t* ind, size_t n, uint32_t card, double** keys, size_t* nk,
+                   int64_t** offs, size_t* no, uint32_t** bms, size_t* nb) {
+    ensure_init();
+    try {
+        ibis::array_t<uint32_t> arr(n);
+        if (n) std::memcpy(arr.begin(), ind, 4 * n);
+        ibis::relic rl(0, card, arr);
+        ibis::array_t<double> k;
+        ibis::array_t<int64_t> o;
+        ibis::array_t<uint32_t> b;
+        rl.write(k, o, b);
+        *keys = dupv(k.begin(), k.size()); *nk = k.size();
+        *offs = dupv(o.begin(), o.size()); *no = o.size();
+        *bms = dupv(b.begin(), b.size()); *nb = b.size();
+        return 0;
+    } catch (...) {
+        return -2;
+    }
+}
+
+// fastbit_iapi_register_array + fastbit_iapi_build_index +
+// fastbit_iapi_deconstruct_index (src/iapi.cpp:1893-1953).  dtype is a
+// FastBitDataType.  The array stays registered under `name` for queries.
+int ref_iapi_index(const char* name, int dtype, void* data, uint64_t n, const char* spec,
+                   double** keys, uint64_t* nk, int64_t** offs, uint64_t* no, uint32_t** bms,
+                   uint64_t* nb) {
+    ensure_init();
+    int ierr = fastbit_iapi_register_array(name, (FastBitDataType)dtype, data, n);
+    if (ierr < 0) return -10 + ierr;
+    ierr = fastbit_iapi_build_index(name, spec);
+    if (ierr < 0) return -20 + ierr;
+    return fastbit_iapi_deconstruct_index(name, keys, nk, offs, no, bms, nb);
+}
+
+// lo OP1 x OP2 hi through selections (src/iapi.cpp:1265-1562, 2062-2090);
+// ops: 0 none, 1 <, 2 <=.  Returns the hit count; rids (ascending) into buf.
+int64_t ref_iapi_query(const char* name, double lo, int lop, double hi, int rop, uint64_t* buf,
+                       uint64_t cap) {
+    ensure_init();
+    FastBitSelectionHandle a = 0, b = 0, h = 0;
+    if (lop == 1) a = fastbit_selection_osr(name, FastBitCompareGreater, lo);
+    else if (lop == 2) a = fastbit_selection_osr(name, FastBitCompareGreaterEqual, lo);
+    if (rop == 1) b = fastbit_selection_osr(name, FastBitCompareLess, hi);
+    else if (rop == 2) b = fastbit_selection_osr(name, FastBitCompareLessEqual, hi);
+    if (a && b) h = fastbit_selection_combine(a, FastBitCombineAnd, b);
+    else h = a ? a : b;
+    if (!h) return -1;
+    int64_t cnt = fastbit_selection_evaluate(h);
+    if (cnt > 0 && buf) {
+        int64_t got = fastbit_selection_get_coordinates(h, buf, cap, 0);
+        if (got < 0) cnt = got;
+    }
+    fastbit_selection_free(h);
+    return cnt;
+}
+
+void ref_iapi_free_all() { fastbit_iapi_free_all(); }
+
+} // extern "C"
