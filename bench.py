@@ -1,0 +1,210 @@
+#!/usr/bin/env python3
+"""Benchmark of the FastBit query hot path on one MI355X.
+
+Workload (the headline metric, range queries/s at 2^27 rows): a float32
+column of 2^27 rows drawn from a pool of 10^5 distinct values (70% N(0,1)
+draws, 30% Zipf(1.2) support; seed 3), an ibis::bin index with the
+reference's 10,000 equal-weight bins (boundaries from
+tests/golden/config3_2p27_bounds.npz, built here on the GPU), and 1024 range
+queries lo <= x < hi whose endpoints are the column's empirical quantiles at
+uniform draws.  One step evaluates the 1024 queries (hit counts) with the
+plan the ibis heuristic picks per query; inputs are resident in HBM.
+
+The same JSON line reports
+  - per-plan throughput (literal / scan) and the algorithmic HBM bytes they
+    move, the roofline of the dominant kernel;
+  - WAH AND / OR at 2^28 bits on the bandwidth-bound Markov cells
+    (density 0.5 / mean run 1 and 64, density 0.01 / run 1), in GB/s of
+    algorithmic bytes 4(|A|w + |B|w + |C|w), operands rotated over several
+    copies so that they stream from HBM;
+  - the CPU baseline: the reference FastBit library (oracle/_ref) answering
+    a sample of the same queries on the host with the same index attached.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def config3(log2n, nq, seed=3):
+    from libfastlib_amd import datagen as G
+    n = 1 << log2n
+    t = time.time()
+    col = G.float_mix(seed, n, pool=100000, wnorm=0.7, s=1.2)
+    log(f"column: {n} rows in {time.time() - t:.1f}s")
+    path = os.path.join(ROOT, "tests", "golden", f"config3_2p{log2n}_bounds.npz")
+    if os.path.exists(path):
+        bounds = np.load(path)["bounds"]
+        src = "reference equal-weight bins"
+    else:
+        qs = np.quantile(col.astype(np.float64), np.linspace(0, 1, 10001)[1:-1])
+        bounds = np.concatenate([np.unique(qs), [np.finfo(np.float64).max]])
+        src = "quantile bins"
+    t = time.time()
+    srt = np.sort(col)
+    u = G.uniform_f64(seed + 1000, 2 * nq)
+    a = srt[np.minimum((u[:nq] * n).astype(np.int64), n - 1)].astype(np.float64)
+    b = srt[np.minimum((u[nq:] * n).astype(np.int64), n - 1)].astype(np.float64)
+    lo, hi = np.minimum(a, b), np.maximum(a, b)
+    log(f"queries: {nq} in {time.time() - t:.1f}s")
+    del srt
+    return col, bounds, src, lo, hi
+
+
+def query_bytes(ix, lo, hi, plan):
+    return np.array([ix.query_bytes(float(lo[i]), 2, float(hi[i]), 1, plan) for i in range(len(lo))])
+
+
+def wah_cells(L, reps=40, copies=4):
+    from libfastlib_amd import datagen as G
+    out = []
+    for d, run in ((0.5, 1), (0.5, 64), (0.01, 1)):
+        wa = G.markov_wah(100, 1 << 28, d, run)
+        wb = G.markov_wah(101, 1 << 28, d, run)
+        A = [L.Bitvector.from_serial(wa) for _ in range(copies)]
+        B = [L.Bitvector.from_serial(wb) for _ in range(copies)]
+        for op, name in ((0, "AND"), (1, "OR")):
+            z = A[0].copy()
+            path = z.op(B[0], op)
+            outw = z.nwords + 1 + (1 if z.write()[-1] else 0)
+            ms, _ = L.bench_op_rot(A, B, op, reps)
+            byts = 4.0 * (len(wa) + len(wb) + outw)
+            gbs = byts / (ms * 1e-3) / 1e9 if ms > 0 else None
+            out.append(dict(cell=f"d={d},L={run}", op=name, path=path, us=round(ms * 1e3, 2),
+                            in_words=[int(len(wa)), int(len(wb))], out_words=int(outw),
+                            GBs=None if gbs is None else round(gbs, 1),
+                            hbm_frac=None if gbs is None else round(gbs / HBM_PEAK_GBS, 3)))
+            log("wah", out[-1])
+    return out
+
+
+def cpu_baseline(L, ix, col, lo, hi, hits_gpu, sample):
+    import oracle
+    if oracle.reference() is None:
+        return None
+    keys, offs, bms = ix.deconstruct()
+    R = oracle.RefAttached("bench_c3", col, keys, offs, bms)
+    R.count(float(lo[0]), 2, float(hi[0]), 1)   # warm the reference's caches
+    t = time.time()
+    for i in range(sample):
+        c = R.count(float(lo[i]), 2, float(hi[i]), 1)
+        if c != int(hits_gpu[i]):
+            raise RuntimeError(f"reference count {c} != GPU count {hits_gpu[i]} (query {i})")
+    dt = time.time() - t
+    return dict(value=round(sample / dt, 4), unit="queries/s", cores=1, kind="reference",
+                sample=f"{sample} of the 1024 queries (lo <= x < hi) on the same 2^27-row column "
+                       f"with the GPU-built index attached through fastbit_iapi_attach_full_index; "
+                       f"counts checked equal to the GPU's; {dt:.1f}s on one host thread")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--log2-rows", type=int, default=27)
+    ap.add_argument("--queries", type=int, default=1024)
+    ap.add_argument("--cpu-sample", type=int, default=16)
+    ap.add_argument("--no-wah", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on one GPU; multi-GPU sharding is a design "
+                                   "note only (see DESIGN.md)", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import libfastlib_amd as L
+    L.lib()
+    col, bounds, bsrc, lo, hi = config3(args.log2_rows, args.queries)
+    t = time.time()
+    ix = L.Index.bin(col, bounds)
+    L.lib().fbh_sync()
+    build_s = time.time() - t
+    nobs, nrows, nwords = ix.info()
+    log(f"index: {nobs} bins, {nwords} words, GPU build {build_s:.2f}s")
+
+    for _ in range(args.warmup):
+        ix.evaluate_batch(lo, 2, hi, 1, 0)
+    walls, gms = [], []
+    hits = used = None
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        hits, used, gm = ix.evaluate_batch(lo, 2, hi, 1, 0)
+        walls.append(time.perf_counter() - t)
+        gms.append(gm)
+    wall = float(np.sum(walls))
+    qps = args.queries * args.steps / wall
+    plan_mix = {L.native.PLAN_NAMES[k]: int(np.sum(used == k)) for k in (1, 2, 3)}
+    log(f"heuristic plan: {qps:.1f} q/s, mix {plan_mix}")
+
+    # per-plan batches (identical counts required)
+    per_plan = {}
+    for plan in (1, 3):
+        h, u, gm = ix.evaluate_batch(lo, 2, hi, 1, plan)
+        if not np.array_equal(h, hits):
+            raise RuntimeError(f"plan {plan} counts differ from the heuristic plan's")
+        qb = query_bytes(ix, lo, hi, plan)
+        gbs = qb.sum() / (gm * 1e-3) / 1e9
+        per_plan[L.native.PLAN_NAMES[plan]] = dict(
+            qps=round(len(lo) / (gm * 1e-3), 1), gpu_ms=round(gm, 3),
+            algo_GB=round(qb.sum() / 1e9, 3), GBs=round(gbs, 1))
+        log("plan", L.native.PLAN_NAMES[plan], per_plan[L.native.PLAN_NAMES[plan]])
+    # best single plan per query (oracle of the plan choice, by the bytes model)
+    bl = query_bytes(ix, lo, hi, 1)
+    bs = query_bytes(ix, lo, hi, 3)
+    dom = max(per_plan, key=lambda k: per_plan[k]["gpu_ms"])
+    dom_kernel = {"literal": "k_query_lit", "scan": "k_query_scan"}[dom]
+    achieved = per_plan[dom]["GBs"]
+    roofline = dict(bound="hbm", kernel=dom_kernel, achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    avg_launch_us=round(per_plan[dom]["gpu_ms"] * 1e3 / len(lo), 2))
+
+    wah = None if args.no_wah else wah_cells(L)
+    cpu = None if args.no_cpu else cpu_baseline(L, ix, col, lo, hi, hits, args.cpu_sample)
+
+    out = {
+        "metric": "range queries/s at 2^27 rows; WAH op GB/s (% HBM peak); learned vs fixed plan",
+        "value": round(qps, 2),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (deterministic generators, seeds in bench.py)",
+        "config": {"workload": f"binned float index: 2^{args.log2_rows} float32 rows, {bsrc} "
+                               f"({nobs} bitmaps, {nwords} words), {args.queries} quantile range "
+                               f"queries lo <= x < hi, plan chosen per query by the ibis heuristic",
+                   "rows": int(nrows), "bins": int(nobs), "queries": args.queries,
+                   "gpu_index_build_s": round(build_s, 2)},
+        "plan_mix": plan_mix,
+        "per_plan": per_plan,
+        "best_plan_bytes_model_GB": round(float(np.minimum(bl, bs).sum()) / 1e9, 3),
+        "roofline": roofline,
+        "wah_ops_2p28": wah,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -115,6 +115,10 @@ int fbh_bv_nwords(const fbh_bitvector* bv, uint32_t* n);
  * word count. */
 int fbh_bench_op(const fbh_bitvector* a, const fbh_bitvector* b, int op, int reps,
                  float* ms_per_op, uint32_t* out_words);
+/* As fbh_bench_op, cycling over npairs operand pairs (a[i], b[i]) so that
+ * the working set exceeds the last-level cache. */
+int fbh_bench_op_rot(fbh_bitvector* const* a, fbh_bitvector* const* b, int npairs, int op,
+                     int reps, float* ms_per_op, uint32_t* out_words);
 
 /* ---------------------------------------------------------------------- */
 /* Indexes: ibis::relic (equality index, one bitmap per distinct value) and
@@ -137,11 +141,15 @@ typedef struct fbh_index fbh_index;
 #define FBH_OP_LT 1
 #define FBH_OP_LE 2
 
-/* Evaluation plans.  All plans yield the same bits. */
+/* Evaluation plans.  They yield the same bits, except where the reference's
+ * own index and scan paths disagree: part::doScan converts the bounds to an
+ * integer column's element type (src/part.cpp:11130-11237), so a
+ * fractional bound on an integer column can select other rows under
+ * FBH_PLAN_SCAN, exactly as it does in the reference. */
 #define FBH_PLAN_AUTO 0        /* the ibis heuristic (column::evaluateRange) */
 #define FBH_PLAN_LITERAL 1     /* LDS multiway OR into a literal result      */
 #define FBH_PLAN_COMPRESSED 2  /* pairwise ORs of the compressed bitmaps     */
-#define FBH_PLAN_SCAN 3        /* raw column compare                         */
+#define FBH_PLAN_SCAN 3        /* raw column compare (part::doScan)          */
 
 /* relic over the distinct values of a column: index::mapValues +
  * relic::construct (src/index.cpp:2552-2611, src/irelic.cpp:878-925). */
@@ -182,6 +190,31 @@ int fbh_index_evaluate(fbh_index* ix, double lo, int lop, double hi, int rop, in
 int fbh_index_evaluate_rids(fbh_index* ix, double lo, int lop, double hi, int rop, int plan,
                             uint32_t* rids, uint64_t cap, uint64_t* n);
 int fbh_index_column_bytes(const fbh_index* ix, uint64_t* bytes);
+/* The cost rule of the plan choice (FBH_PLAN_AUTO):
+ *   FBH_COST_IAPI       arrays registered through the in-memory API: the
+ *                       index when its estimated bytes are below
+ *                       nrows/2 + 999 (bord::column::evaluateRange,
+ *                       src/bord.cpp:5903-5912); the default;
+ *   FBH_COST_PARTITION  data partitions with index files: the index when
+ *                       its estimated bytes are below the scan's page bytes
+ *                       (column::evaluateRange, src/column.cpp:6102-6111). */
+#define FBH_COST_IAPI 0
+#define FBH_COST_PARTITION 1
+int fbh_index_set_cost_model(fbh_index* ix, int model);
+
+/* Many range queries on one index, queued back to back on the device with
+ * one synchronisation at the end.  plans[i] (or `plan` when plans is NULL)
+ * picks the plan per query; hits[i] receives the counts, plans_used[i] the
+ * plan that ran, *gpu_ms the device time from the first to the last query.
+ * (No counterpart in the reference, which evaluates one selection at a
+ * time through fastbit_selection_evaluate, src/iapi.cpp:1383-1418.) */
+int fbh_index_evaluate_batch(fbh_index* ix, uint32_t nq, const double* lo, const int* lop,
+                             const double* hi, const int* rop, const int* plans, int plan,
+                             uint64_t* hits, int* plans_used, float* gpu_ms);
+/* Algorithmic bytes a plan moves for one query (bitmap words read, raw
+ * values read for the edge check or the scan, literal result written). */
+int fbh_index_query_bytes(const fbh_index* ix, double lo, int lop, double hi, int rop, int plan,
+                          double* bytes);
 
 #ifdef __cplusplus
 }

@@ -5,6 +5,7 @@ HIP kernels behind a C ABI (``include/fastbit_hip.h``,
 ``lib/libfastbit_hip.so``).  This package is the Python host binding of that
 ABI; the C++ host binding is ``include/fastbit_hip.hpp``.
 """
-from .native import (AND, OR, XOR, MINUS, Bitvector, FastbitError, Index, bench_op, lib)  # noqa: F401
+from . import native  # noqa: F401
+from .native import (AND, OR, XOR, MINUS, Bitvector, FastbitError, Index, bench_op, bench_op_rot, lib)  # noqa: F401
 
-__all__ = ["AND", "OR", "XOR", "MINUS", "Bitvector", "FastbitError", "Index", "bench_op", "lib"]
+__all__ = ["AND", "OR", "XOR", "MINUS", "Bitvector", "FastbitError", "Index", "bench_op", "bench_op_rot", "lib"]

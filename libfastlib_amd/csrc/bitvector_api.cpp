@@ -339,7 +339,10 @@ int launch_word_op(fbh_bitvector* a, fbh_bitvector* b, int op, bool c2, uint32_t
 int run_word_op(const fbh_bitvector* a, const fbh_bitvector* b, int op, bool c2,
                 fbh_bitvector* res, uint32_t groups) {
     OutBuf ob;
-    int rc = ob.alloc(a->nw + b->nw);
+    // c2 emits at most one word per consumed code word; a canonical result
+    // has at most one word per group
+    const uint32_t inw = a->nw + b->nw;
+    int rc = ob.alloc(c2 ? inw : (groups > inw ? groups : inw));
     if (rc) return rc;
     uint32_t* ctl = nullptr;
     rc = launch_word_op(const_cast<fbh_bitvector*>(a), const_cast<fbh_bitvector*>(b), op, c2,
@@ -756,6 +759,51 @@ int fbh_bench_op(const fbh_bitvector* a, const fbh_bitvector* b, int op, int rep
     if (!rc) (void)hipEventElapsedTime(&t, e0, e1);
     (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
     dev_free(ob.d); dev_free(ob.ck);
+    if (ms) *ms = t / reps;
+    return rc;
+}
+
+// Like fbh_bench_op, cycling over npairs operand pairs of the same shape so
+// that the working set exceeds the last-level cache and every op streams
+// its operands from HBM.
+int fbh_bench_op_rot(fbh_bitvector* const* a, fbh_bitvector* const* b, int npairs, int op, int reps,
+                     float* ms, uint32_t* out_words) {
+    if (!a || !b || npairs <= 0 || reps <= 0) return fail(FBH_ERR_ARG, "bad argument");
+    std::vector<int> paths(npairs);
+    for (int i = 0; i < npairs; ++i) {
+        fbh_bitvector* ta = nullptr;
+        int rc = fbh_bv_copy(a[i], &ta);
+        if (rc) return rc;
+        rc = fbh_bv_op(ta, b[i], op, &paths[i]);
+        if (i == 0 && out_words) *out_words = ta->nw;
+        fbh_bv_free(ta);
+        if (rc) return rc;
+        if (paths[i] == FBH_PATH_SHORTCUT) { if (ms) *ms = 0.f; return FBH_OK; }
+    }
+    std::vector<OutBuf> obs(npairs);
+    for (int i = 0; i < npairs; ++i) {
+        const bool c2 = paths[i] == FBH_PATH_C2;
+        const uint32_t groups = a[i]->nbits / MAXBITS;
+        int rc = obs[i].alloc(c2 ? a[i]->nw + b[i]->nw : (groups > a[i]->nw + b[i]->nw ? groups : a[i]->nw + b[i]->nw));
+        if (rc) return rc;
+    }
+    uint32_t* ctl = nullptr;
+    auto one = [&](int i) {
+        const uint32_t groups = a[i]->nbits / MAXBITS;
+        return launch_word_op(a[i], b[i], op, paths[i] == FBH_PATH_C2, groups, obs[i], &ctl);
+    };
+    int rc = FBH_OK;
+    for (int i = 0; i < npairs && !rc; ++i) rc = one(i);
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0); (void)hipEventCreate(&e1);
+    if (!rc) rc = check(hipEventRecord(e0, stream()), "event");
+    for (int r = 0; r < reps && !rc; ++r) rc = one(r % npairs);
+    if (!rc) rc = check(hipEventRecord(e1, stream()), "event");
+    if (!rc) rc = check(hipEventSynchronize(e1), "event sync");
+    float t = 0.f;
+    if (!rc) (void)hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+    for (auto& ob : obs) { dev_free(ob.d); dev_free(ob.ck); }
     if (ms) *ms = t / reps;
     return rc;
 }

@@ -16,6 +16,7 @@
 #include "index_dev.h"
 #include "index_host.h"
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -100,6 +101,11 @@ struct fbh_index {
     uint32_t ntiles = 0;
     void* col = nullptr;          // raw column (device)
     double* dbounds = nullptr;    // bin bounds (device)
+    std::vector<double> cumbytes; // prefix sums of bytes(b)
+    std::vector<double> cumser;   // prefix sums of serialized bytes
+    bool offsets_built = false;   // the reference constructor fills offset64
+    int cost_model = FBH_COST_IAPI;
+    double cmin = DBL_MAX, cmax = -DBL_MAX;   // column min / max
     ~fbh_index() {
         dev_free(words); dev_free(doff); dev_free(dnw); dev_free(toff); dev_free(col);
         dev_free(dbounds);
@@ -148,7 +154,7 @@ uint32_t bits_for(uint32_t maxkey) {
 
 // Encode bitmaps from sorted rows; fills idx words/offsets/nw (+min/max).
 int encode_bitmaps(fbh_index* ix, const uint32_t* skeys_d, const uint32_t* srows_d, uint32_t nsorted,
-                   bool with_minmax, bool drop_empty_bins) {
+                   bool with_minmax, bool drop_empty_bins, bool unpack_heavy) {
     const uint32_t nbm = ix->nobs;
     // bitmap starts in the sorted key order (rows without a bitmap sort last)
     std::vector<uint32_t> start(nbm + 1);
@@ -198,20 +204,33 @@ int encode_bitmaps(fbh_index* ix, const uint32_t* skeys_d, const uint32_t* srows
     }
     const uint32_t nk = (uint32_t)keep.size();
     std::vector<uint64_t> off(nk + 1);
-    std::vector<uint32_t> knw(nk), kav_idx(nk);
+    std::vector<uint32_t> knw(nk), srcnw(nk);
     std::vector<uint64_t> ktmp(nk);
+    std::vector<uint8_t> unpack(nk, 0);
+    // index::optionalUnpack with no <compressing> directive
+    // (src/index.cpp:8803-8854): a bitmap whose bytes() exceed
+    // max(nrows/9, sizeof(bitvector)+12) is stored decompressed.
+    const uint64_t bar1 = std::max<uint64_t>(ix->nrows / 9u, 40u + 12u);
+    bool any_unpack = false;
     off[0] = 0;
     const uint32_t trailer = ix->anb > 0 ? 2u : 1u;
     for (uint32_t k = 0; k < nk; ++k) {
         const uint32_t b = keep[k];
-        knw[k] = nullbm[k] ? 0u : nw[b];
+        srcnw[k] = nullbm[k] ? 0u : nw[b];
+        knw[k] = srcnw[k];
+        if (unpack_heavy && !nullbm[k] && 4ull * nw[b] + 40u > bar1 && nw[b] != ix->groups) {
+            unpack[k] = 1;
+            knw[k] = ix->groups;
+            any_unpack = true;
+        }
         ktmp[k] = tmpoff[b];
-        off[k + 1] = off[k] + (nullbm[k] ? 0u : nw[b] + trailer);
+        off[k + 1] = off[k] + (nullbm[k] ? 0u : knw[k] + trailer);
     }
     // gather per kept bitmap (device arrays for the concat)
     std::vector<uint32_t> kval_idx(keep.begin(), keep.end());
     DBuf<uint64_t> doff_k, dtmpoff_k;
     DBuf<uint32_t> dnw_k, daval_k;
+    DBuf<uint8_t> dunpack;
     std::vector<uint32_t> aval(nbm);
     if ((rc = to_host(aval.data(), daval.p, nbm))) return rc;
     std::vector<uint32_t> kav(nk);
@@ -220,12 +239,14 @@ int encode_bitmaps(fbh_index* ix, const uint32_t* skeys_d, const uint32_t* srows
         (rc = daval_k.alloc(nk)))
         return rc;
     if ((rc = to_dev(doff_k.p, off.data(), nk + 1)) || (rc = to_dev(dtmpoff_k.p, ktmp.data(), nk)) ||
-        (rc = to_dev(dnw_k.p, knw.data(), nk)) || (rc = to_dev(daval_k.p, kav.data(), nk)))
+        (rc = to_dev(dnw_k.p, srcnw.data(), nk)) || (rc = to_dev(daval_k.p, kav.data(), nk)))
         return rc;
+    if (any_unpack && ((rc = dunpack.alloc(nk)) || (rc = to_dev(dunpack.p, unpack.data(), nk)))) return rc;
     DBuf<uint32_t> words;
     if ((rc = words.alloc(off[nk] + 1))) return rc;
-    rc = check(fbk::launch_concat(tmp.p, dtmpoff_k.p, dnw_k.p, daval_k.p, doff_k.p, nk, ix->anb,
-                                  words.p, stream()), "concat");
+    rc = check(fbk::launch_concat(tmp.p, dtmpoff_k.p, dnw_k.p, daval_k.p, doff_k.p,
+                                  any_unpack ? dunpack.p : nullptr, nk, ix->groups, ix->anb, words.p,
+                                  stream()), "concat");
     if (rc) return rc;
     if ((rc = sync())) return rc;
     // install
@@ -260,6 +281,12 @@ int finish_index(fbh_index* ix) {
     if (!ix->doff || !ix->dnw) return fail(FBH_ERR_NOMEM, "device allocation failed");
     if ((rc = to_dev(ix->doff, ix->off.data(), ix->nobs + 1)) || (rc = to_dev(ix->dnw, ix->nw.data(), ix->nobs)))
         return rc;
+    ix->cumbytes.assign(ix->nobs + 1, 0.0);
+    ix->cumser.assign(ix->nobs + 1, 0.0);
+    for (uint32_t b = 0; b < ix->nobs; ++b) {
+        ix->cumbytes[b + 1] = ix->cumbytes[b] + ix->bytes(b);
+        ix->cumser[b + 1] = ix->cumser[b] + 4.0 * (double)(ix->off[b + 1] - ix->off[b]);
+    }
     if (ix->kind == FBH_BIN && ix->nobs) {
         ix->dbounds = (double*)dev_alloc(8ull * ix->nobs);
         if (!ix->dbounds) return fail(FBH_ERR_NOMEM, "device allocation failed");
@@ -330,11 +357,12 @@ int distinct_values(fbh_index* ix, std::vector<double>& vals) {
     return FBH_OK;
 }
 
-int build_common(fbh_index* ix, const uint32_t* dkeys, uint32_t maxkey, bool minmax, bool drop) {
+int build_common(fbh_index* ix, const uint32_t* dkeys, uint32_t maxkey, bool minmax, bool drop,
+                 bool unpack_heavy) {
     DBuf<uint32_t> sk, sr;
     int rc = sort_pairs(const_cast<uint32_t*>(dkeys), ix->nrows, bits_for(maxkey), sk, sr);
     if (rc) return rc;
-    rc = encode_bitmaps(ix, sk.p, sr.p, ix->nrows, minmax, drop);
+    rc = encode_bitmaps(ix, sk.p, sr.p, ix->nrows, minmax, drop, unpack_heavy);
     if (rc) return rc;
     return finish_index(ix);
 }
@@ -370,11 +398,8 @@ SumPlan sum_plan(const fbh_index* ix, uint32_t ib, uint32_t ie) {
     SumPlan p{2};
     if (ie > ix->nobs) ie = ix->nobs;
     if (ib >= ie) { p.kind = 0; return p; }
-    double all = 0, mid = 0;
-    for (uint32_t b = 0; b < ix->nobs; ++b) {
-        all += ix->bytes(b);
-        if (b >= ib && b < ie) mid += ix->bytes(b);
-    }
+    const double all = ix->cumbytes[ix->nobs];
+    const double mid = ix->cumbytes[ie] - ix->cumbytes[ib];
     if (mid == 0) p.kind = 0;
     else if (all == mid) p.kind = 1;
     else p.kind = (mid <= std::floor(all / 2)) ? 2 : 3;
@@ -416,22 +441,9 @@ int index_eval_literal(fbh_index* ix, double lo, int lop, double hi, int rop, in
     Locate L;
     locate(ix, lo, lop, hi, rop, L);
     if (plan == FBH_PLAN_SCAN) {
-        // the predicate of relic/bin::evaluate row by row: exact compares,
-        // except that a bin index over an integer column applies
-        // part::doScan's cast rules only inside its edge bins
-        if (ix->kind == FBH_RELIC || !is_int_type(ix->type)) {
-            q.is_int = 0;
-            q.lop = lop; q.rop = rop;
-        } else {
-            q.scan_bin = 1;
-            q.bounds = ix->dbounds;
-            q.nbounds = ix->nobs;
-            const SumPlan sp0 = sum_plan(ix, L.hit0, L.hit1);
-            if (sp0.kind == 1) { q.h0 = 0; q.h1 = ix->nobs; }
-            else { q.h0 = L.hit0; q.h1 = L.hit1; }
-            q.e0 = (L.cand0 < L.hit0) ? L.cand0 : fbk::NOKEY;
-            q.e1 = (L.cand1 > L.hit1 && L.hit1 < ix->nobs) ? L.hit1 : fbk::NOKEY;
-        }
+        // part::doScan over every row: integer columns take the bounds
+        // converted to the element type (fill_query), floats compare as
+        // doubles
         return check(fbk::launch_query_scan(q, dev_type(ix->type), ix->nrows, s), "scan");
     }
     const SumPlan sp = sum_plan(ix, L.hit0, L.hit1);
@@ -537,6 +549,8 @@ int eval_compressed(fbh_index* ix, double lo, int lop, double hi, int rop, fbh_b
 }
 
 int plan_heuristic(const fbh_index* ix, double lo, int lop, double hi, int rop);
+double index_cost(const fbh_index* ix, double lo, int lop, double hi, int rop);
+double scan_cost(const fbh_index* ix);
 
 } // namespace
 
@@ -555,6 +569,7 @@ int fbh_index_build_relic(const void* col, int type, uint32_t nrows, fbh_index**
     if ((rc = distinct_values(ix, vals))) { delete ix; return rc; }
     ix->keys = vals;
     ix->nobs = (uint32_t)vals.size();
+    if (!vals.empty()) { ix->cmin = vals.front(); ix->cmax = vals.back(); }
     DBuf<double> dv;
     DBuf<uint32_t> keys;
     if ((rc = dv.alloc(vals.size())) || (rc = keys.alloc(nrows)) || (rc = to_dev(dv.p, vals.data(), vals.size()))) {
@@ -562,7 +577,7 @@ int fbh_index_build_relic(const void* col, int type, uint32_t nrows, fbh_index**
     }
     rc = check(fbk::launch_keys_relic(ix->col, dev_type(type), nrows, dv.p, ix->nobs, keys.p, stream()),
                "relic keys");
-    if (!rc) rc = build_common(ix, keys.p, ix->nobs, false, false);
+    if (!rc) rc = build_common(ix, keys.p, ix->nobs, false, false, true);
     if (rc) { delete ix; return rc; }
     *out = ix;
     return FBH_OK;
@@ -581,8 +596,13 @@ int fbh_index_build_relic_card(const uint32_t* ind, uint32_t nrows, uint32_t car
     DBuf<uint32_t> keys;
     if ((rc = keys.alloc(nrows))) { delete ix; return rc; }
     rc = check(fbk::launch_keys_card((const uint32_t*)ix->col, nrows, card, keys.p, stream()), "card keys");
-    if (!rc) rc = build_common(ix, keys.p, card, false, false);
+    if (!rc) rc = build_common(ix, keys.p, card, false, false, false);
     if (rc) { delete ix; return rc; }
+    ix->offsets_built = true;     // relic(col, card, ind), src/irelic.cpp:186-190
+    for (uint32_t i = 0; i < nrows; ++i) {
+        ix->cmin = std::min(ix->cmin, (double)ind[i]);
+        ix->cmax = std::max(ix->cmax, (double)ind[i]);
+    }
     *out = ix;
     return FBH_OK;
 }
@@ -604,8 +624,13 @@ int fbh_index_build_bin(const void* col, int type, uint32_t nrows, const double*
     }
     rc = check(fbk::launch_keys_bin(ix->col, dev_type(type), nrows, db.p, nbounds, keys.p, stream()),
                "bin keys");
-    if (!rc) rc = build_common(ix, keys.p, nbounds, true, true);
+    if (!rc) rc = build_common(ix, keys.p, nbounds, true, true, true);
     if (rc) { delete ix; return rc; }
+    for (uint32_t b = 0; b < ix->nobs; ++b) {
+        if (ix->null_bitmap(b)) continue;
+        ix->cmin = std::min(ix->cmin, ix->minval[b]);
+        ix->cmax = std::max(ix->cmax, ix->maxval[b]);
+    }
     *out = ix;
     return FBH_OK;
 }
@@ -656,32 +681,18 @@ int fbh_index_resolve_range(const fbh_index* ix, double lo, int lop, double hi, 
 int fbh_index_estimate_cost(const fbh_index* ix, double lo, int lop, double hi, int rop,
                             double* icost, double* scost, int* plan) {
     if (!ix) return fail(FBH_ERR_ARG, "null index");
-    // bin::estimateCost (src/ibin.cpp:6531-6569); relic::estimateCost
-    // (src/irelic.cpp:1858-1902) is the same without the edge term.
-    Locate L;
-    locate(ix, lo, lop, hi, rop, L);
-    double ic = 0;
-    double tot = 0, mid = 0;
-    for (uint32_t b = 0; b < ix->nobs; ++b) {
-        const double by = ix->null_bitmap(b) ? 0.0 : 4.0 * (ix->nw[b] + 1 + (ix->anb > 0));
-        tot += by;
-        if (b >= L.cand0 && b < L.cand1) mid += by;
-    }
-    if (L.cand0 < L.cand1) ic = (tot / 2 >= mid) ? mid : tot - mid;
-    if (ix->kind == FBH_BIN && (L.hit0 > L.cand0 || L.hit1 < L.cand1) && ix->nobs > 0) {
-        const double ccheck = (double)col_esize(ix->type) * ix->nrows / ix->nobs;
-        ic += (L.hit0 > L.cand0 && L.hit1 < L.cand1 && L.hit0 <= L.hit1) ? 2.0 * ccheck : ccheck;
-    }
-    // column::evaluateRange (src/column.cpp:6102-6111): page count of a
-    // full mask (src/part.cpp:2616-2678) at 4096-byte pages
-    const double es = (double)col_esize(ix->type);
-    const double wpp = std::floor(4096.0 / es);
-    double pages = std::floor((double)ix->nrows / wpp);
-    if (pages < 1) pages = 1;
-    const double sc = 4096.0 * pages + 8.0 * ix->nrows / 4096.0;
+    const double ic = index_cost(ix, lo, lop, hi, rop);
+    const double sc = scan_cost(ix);
     if (icost) *icost = ic;
     if (scost) *scost = sc;
     if (plan) *plan = plan_heuristic(ix, lo, lop, hi, rop);
+    return FBH_OK;
+}
+
+int fbh_index_set_cost_model(fbh_index* ix, int model) {
+    if (!ix) return fail(FBH_ERR_ARG, "null index");
+    if (model != FBH_COST_IAPI && model != FBH_COST_PARTITION) return fail(FBH_ERR_ARG, "unknown cost model");
+    ix->cost_model = model;
     return FBH_OK;
 }
 
@@ -760,6 +771,78 @@ int fbh_index_evaluate_rids(fbh_index* ix, double lo, int lop, double hi, int ro
     return to_host(rids, out.p, k);
 }
 
+int fbh_index_evaluate_batch(fbh_index* ix, uint32_t nq, const double* lo, const int* lop,
+                             const double* hi, const int* rop, const int* plans, int plan,
+                             uint64_t* hits, int* plans_used, float* gpu_ms) {
+    if (!ix || !lo || !lop || !hi || !rop || !hits) return fail(FBH_ERR_ARG, "null argument");
+    int rc = ensure_init();
+    if (rc) return rc;
+    DBuf<uint32_t> lit, cnt;
+    if ((rc = lit.alloc(ix->groups + 1)) || (rc = cnt.alloc(nq ? nq : 1))) return rc;
+    if ((rc = check(fbk::launch_zero_u32(cnt.p, nq, stream()), "zero"))) return rc;
+    std::vector<int> used(nq);
+    std::vector<uint64_t> sync_hits(nq, 0);
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0); (void)hipEventCreate(&e1);
+    rc = check(hipEventRecord(e0, stream()), "event");
+    for (uint32_t i = 0; i < nq && !rc; ++i) {
+        int p = plans ? plans[i] : plan;
+        if (p == FBH_PLAN_AUTO) p = plan_heuristic(ix, lo[i], lop[i], hi[i], rop[i]);
+        used[i] = p;
+        if (p == FBH_PLAN_COMPRESSED) {
+            fbh_bitvector* res = nullptr;
+            rc = fbh_bv_new(&res);
+            if (!rc) rc = eval_compressed(ix, lo[i], lop[i], hi[i], rop[i], res);
+            uint32_t c = 0;
+            if (!rc) rc = fbh_bv_cnt(res, &c);
+            sync_hits[i] = c;
+            fbh_bv_free(res);
+        } else {
+            rc = index_eval_literal(ix, lo[i], lop[i], hi[i], rop[i], p, lit.p, cnt.p + i, stream());
+        }
+    }
+    if (!rc) rc = check(hipEventRecord(e1, stream()), "event");
+    if (!rc) rc = check(hipEventSynchronize(e1), "event sync");
+    float ms = 0.f;
+    if (!rc) (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    std::vector<uint32_t> c(nq);
+    if ((rc = to_host(c.data(), cnt.p, nq))) return rc;
+    for (uint32_t i = 0; i < nq; ++i) hits[i] = used[i] == FBH_PLAN_COMPRESSED ? sync_hits[i] : c[i];
+    if (plans_used) for (uint32_t i = 0; i < nq; ++i) plans_used[i] = used[i];
+    if (gpu_ms) *gpu_ms = ms;
+    return FBH_OK;
+}
+
+// Algorithmic bytes of one range query under a plan: the bitmap words the
+// plan reads (OR set, edge bitmaps), the raw values it reads, and the
+// literal result it writes.
+int fbh_index_query_bytes(const fbh_index* ix, double lo, int lop, double hi, int rop, int plan,
+                          double* bytes) {
+    if (!ix || !bytes) return fail(FBH_ERR_ARG, "null argument");
+    const double res = 4.0 * (ix->groups + 1);
+    const double es = (double)col_esize(ix->type);
+    if (plan == FBH_PLAN_SCAN) { *bytes = es * ix->nrows + res; return FBH_OK; }
+    Locate L;
+    locate(ix, lo, lop, hi, rop, L);
+    const SumPlan sp = sum_plan(ix, L.hit0, L.hit1);
+    auto words = [&](uint32_t b0, uint32_t b1) {
+        return b1 > b0 ? ix->cumser[b1] - ix->cumser[b0] : 0.0;
+    };
+    double w = 0;
+    if (sp.kind == 2) w = words(L.hit0, L.hit1);
+    else if (sp.kind == 3) w = words(0, L.hit0) + words(L.hit1, ix->nobs);
+    double edge = 0;
+    if (ix->kind == FBH_BIN) {
+        const double rows_per_bin = ix->nobs ? (double)ix->nrows / ix->nobs : 0.0;
+        if (L.cand0 < L.hit0) edge += words(L.cand0, L.cand0 + 1) + es * rows_per_bin;
+        if (L.cand1 > L.hit1 && L.hit1 < ix->nobs) edge += words(L.hit1, L.hit1 + 1) + es * rows_per_bin;
+    }
+    *bytes = w + edge + res;
+    return FBH_OK;
+}
+
 int fbh_index_column_bytes(const fbh_index* ix, uint64_t* bytes) {
     if (!ix || !bytes) return fail(FBH_ERR_ARG, "null argument");
     *bytes = (uint64_t)col_esize(ix->type) * ix->nrows;
@@ -770,26 +853,121 @@ int fbh_index_column_bytes(const fbh_index* ix, uint64_t* bytes) {
 
 namespace {
 
-// The ibis heuristic mapped onto the three plans: the index is used when
-// its estimated cost is below the scan cost (src/column.cpp:6102-6111);
-// then sumBins' priority-queue branch (src/index.cpp:6673) is a compressed
-// merge, its decompressed-result branches are the literal plan.
+// relic::estimateCost / bin::estimateCost (src/irelic.cpp:1857-1902,
+// src/ibin.cpp:6531-6569).  offset64 (serialized bytes per bitmap) is
+// there when the index was read from a file or built by relic(col, card,
+// ind); the in-memory constructors leave it empty, and then relic sums
+// bitvector::bytes() over the hit bitmaps and bin keeps only the edge term.
+double index_cost(const fbh_index* ix, double lo, int lop, double hi, int rop) {
+    const bool offsets = ix->cost_model == FBH_COST_PARTITION || ix->offsets_built;
+    const uint32_t nobs = ix->nobs;
+    auto ser = [&](uint32_t b0, uint32_t b1) {
+        b1 = std::min(b1, nobs);
+        return b1 > b0 ? ix->cumser[b1] - ix->cumser[b0] : 0.0;
+    };
+    Locate L;
+    locate(ix, lo, lop, hi, rop, L);
+    if (ix->kind == FBH_RELIC) {
+        const uint32_t h0 = L.hit0, h1 = L.hit1;
+        if (h0 >= h1) return 0.0;
+        if (offsets) {
+            if (h1 > h0 + 1) {
+                const double tot = ser(0, nobs), mid = ser(h0, h1);
+                return std::floor(tot / 2) >= mid ? mid : tot - mid;
+            }
+            return 0.5 * ser(h0, h1);
+        }
+        if (h1 > h0 + 1) {
+            const uint32_t e = std::min(h1, nobs);
+            return e > h0 ? ix->cumbytes[e] - ix->cumbytes[h0] : 0.0;
+        }
+        return h0 < nobs ? 0.5 * ix->bytes(h0) : 0.0;
+    }
+    double r = 0;
+    uint32_t cand0 = 0, cand1 = nobs, hit0 = nobs, hit1 = 0;
+    if (offsets) {
+        cand0 = L.cand0; cand1 = L.cand1; hit0 = L.hit0; hit1 = L.hit1;
+        if (cand0 < cand1 && cand1 <= nobs) {
+            const double tot = ser(0, nobs), mid = ser(cand0, cand1);
+            r = std::floor(tot / 2) >= mid ? mid : tot - mid;
+        }
+    }
+    if (hit0 > cand0 || hit1 < cand1) {
+        // column::elementSize() * nrows / nobs in unsigned 32-bit arithmetic
+        const uint32_t esn = (uint32_t)col_esize(ix->type) * ix->nrows;
+        if (nobs > 0) {
+            const double ccheck = (double)(esn / nobs);
+            r += (hit0 > cand0 && hit1 < cand1 && hit0 <= hit1) ? 2.0 * ccheck : ccheck;
+        } else {
+            r += (double)esn;
+        }
+    }
+    return r;
+}
+
+// part::countPages over a mask of n ones (src/part.cpp:2616-2678): the
+// indexSet walk sees [0, 31g) as one range, then the active word's rows.
+uint32_t full_mask_pages(uint32_t n, uint32_t es) {
+    if (n == 0) return 0;
+    const uint32_t wpp = 4096u / es;
+    const uint32_t g = n / MAXBITS, anb = n % MAXBITS;
+    uint32_t res = 0, last = 0;
+    if (g > 0) { res += (MAXBITS * g) / wpp; last = MAXBITS * g; }
+    if (anb > 0) {
+        const uint32_t p0 = (MAXBITS * g) / wpp;
+        res += (last < p0 * wpp) ? 1u : 0u;
+        last = n - 1;
+        res += (last / wpp > p0) ? 1u : 0u;
+    }
+    return res == 0 ? 1u : res;
+}
+
+// The cost the index cost is compared with: bord::column::evaluateRange
+// (arrays in memory, src/bord.cpp:5903-5912) uses the index below
+// nrows/2 + 999 bytes; column::evaluateRange (partitions on disk,
+// src/column.cpp:6102-6111) below the scan's page bytes at 4096-byte pages.
+double scan_cost(const fbh_index* ix) {
+    if (ix->cost_model == FBH_COST_PARTITION)
+        return 4096.0 * full_mask_pages(ix->nrows, (uint32_t)col_esize(ix->type)) +
+               8.0 * ix->nrows / 4096.0;
+    return ix->nrows * 0.5 + 999.0;
+}
+
+// qContinuousRange::overlap (src/qExpr.cpp:1673-1803) of lo OP x OP hi
+// with the column's [min, max]
+bool overlap(const fbh_index* ix, double lo, int lop, double hi, int rop) {
+    if (!(ix->cmin <= ix->cmax)) return true;
+    if (lop == FBH_OP_LT && !(lo < ix->cmax)) return false;
+    if (lop == FBH_OP_LE && !(lo <= ix->cmax)) return false;
+    if (rop == FBH_OP_LT) return ix->cmin < hi;
+    if (rop == FBH_OP_LE) return ix->cmin <= hi;
+    return lop != FBH_OP_NONE;
+}
+
+// The ibis heuristic mapped onto the three plans: the index when its cost
+// is below the scan cost, else the raw scan (src/bord.cpp:5903-5912 or
+// src/column.cpp:6102-6111 by the cost model); a range that misses the
+// column's [min, max] under the partition model is answered without
+// either (src/column.cpp:6063-6066) -- the literal plan returns its empty
+// answer.  With the index, sumBins' priority-queue branch
+// (src/index.cpp:6673) is a compressed merge, its decompressed-result
+// branches are the literal plan.
 int plan_heuristic(const fbh_index* ix, double lo, int lop, double hi, int rop) {
-    double ic = 0, sc = 0;
-    fbh_index_estimate_cost(ix, lo, lop, hi, rop, &ic, &sc, nullptr);
-    if (!(ic < sc)) return FBH_PLAN_SCAN;
+    if (ix->cost_model == FBH_COST_PARTITION && !overlap(ix, lo, lop, hi, rop)) return FBH_PLAN_LITERAL;
+    if (!(index_cost(ix, lo, lop, hi, rop) < scan_cost(ix))) return FBH_PLAN_SCAN;
     Locate L;
     locate(ix, lo, lop, hi, rop, L);
     const SumPlan sp = sum_plan(ix, L.hit0, L.hit1);
     if (sp.kind < 2) return FBH_PLAN_LITERAL;
     uint32_t ib = sp.kind == 2 ? L.hit0 : 0, ie = sp.kind == 2 ? L.hit1 : L.hit0;
-    double bytes = 0;
-    uint32_t na = 0;
+    double bytes;
+    uint32_t na;
     if (sp.kind == 2) {
-        for (uint32_t b = ib; b < ie; ++b) { bytes += ix->bytes(b); ++na; }
+        bytes = ix->cumbytes[ie] - ix->cumbytes[ib];
+        na = ie - ib;
     } else {
-        for (uint32_t b = 0; b < L.hit0; ++b) { bytes += ix->bytes(b); ++na; }
-        for (uint32_t b = L.hit1; b < ix->nobs; ++b) { bytes += ix->bytes(b); ++na; }
+        bytes = ix->cumbytes[L.hit0] + ix->cumbytes[ix->nobs] - ix->cumbytes[L.hit1];
+        na = L.hit0 + (ix->nobs - L.hit1);
     }
     const double uncomp = (double)(ix->nrows * 4u / 31u);
     const double sum2 = (na >= 2) ? ix->bytes(ib) + ix->bytes(ib + 1 < ix->nobs ? ib + 1 : ib) : bytes;

@@ -32,12 +32,6 @@ struct QueryArgs {
     int64_t ilo, ihi;
     uint32_t* out;             // literal result, groups + 1 words
     uint32_t* count;           // ones (atomic)
-    // scan plan over a bin index on an integer column: the row's bin
-    // decides (hit bins [h0,h1) hit, edge bins e0/e1 take the raw test)
-    uint32_t scan_bin;
-    const double* bounds;
-    uint32_t nbounds;
-    uint32_t h0, h1, e0, e1;
 };
 
 hipError_t launch_keys_relic(const void* col, int type, uint32_t n, const double* vals, uint32_t nv,
@@ -57,9 +51,11 @@ hipError_t launch_encode(int type, const uint32_t* rows, const uint32_t* start, 
                          uint32_t groups, uint32_t anb, const uint64_t* tmpoff, uint32_t* tmp,
                          uint32_t* nw, uint32_t* aval, const void* col, double* vmin, double* vmax,
                          hipStream_t s);
+// unpack[b] != 0 (unpack may be null): bitmap b is stored in literal form,
+// `groups` words.
 hipError_t launch_concat(const uint32_t* tmp, const uint64_t* tmpoff, const uint32_t* nw,
-                         const uint32_t* aval, const uint64_t* off, uint32_t nbm, uint32_t anb,
-                         uint32_t* out, hipStream_t s);
+                         const uint32_t* aval, const uint64_t* off, const uint8_t* unpack,
+                         uint32_t nbm, uint32_t groups, uint32_t anb, uint32_t* out, hipStream_t s);
 hipError_t launch_toff(const uint32_t* words, const uint64_t* off, const uint32_t* nw, uint32_t nbm,
                        uint32_t ntiles, uint32_t tg, uint2* toff, hipStream_t s);
 hipError_t launch_query_lit(const QueryArgs& q, int type, hipStream_t s);

@@ -247,13 +247,49 @@ __global__ __launch_bounds__(IX_TPB) void k_encode(const uint32_t* __restrict__ 
     if (vmin) { vmin[b] = mn; vmax[b] = mx; }
 }
 
-// Concatenation: bitmap b's code words + trailer at off[b].
+// One block writes the literal form of a canonical bitmap (the decompress of
+// index::optionalUnpack, src/bitvector.cpp:474-512): code words are taken
+// IX_TPB at a time, their group spans scanned, and every output group is
+// written by a thread that finds its code word in LDS.
+__device__ void block_unpack(const uint32_t* __restrict__ src, uint32_t n, uint32_t* __restrict__ dst) {
+    __shared__ uint32_t sw[IX_TPB];
+    __shared__ uint32_t sp[IX_TPB];
+    __shared__ uint32_t sh[IX_TPB / WAVE + 1];
+    uint64_t pos = 0;
+    for (uint32_t base = 0; base < n; base += IX_TPB) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t w = i < n ? src[i] : 0u;
+        const uint32_t l = i < n ? word_len(w) : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_excl_sum<IX_TPB>(l, sh, &tot);
+        sw[threadIdx.x] = w;
+        sp[threadIdx.x] = i < n ? ex : 0xFFFFFFFFu;
+        __syncthreads();
+        for (uint32_t g = threadIdx.x; g < tot; g += IX_TPB) {
+            // last k with sp[k] <= g (sp[0] == 0, strictly increasing)
+            uint32_t lo = 0, hi = IX_TPB;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (sp[mid] <= g) lo = mid; else hi = mid;
+            }
+            const uint32_t x = sw[lo];
+            dst[pos + g] = x > ALLONES ? ((x & FILLBIT) ? ALLONES : 0u) : x;
+        }
+        pos += tot;
+        __syncthreads();
+    }
+}
+
+// Concatenation: bitmap b's code words (or, with unpack[b], its literal
+// form of `groups` words) + trailer at off[b].
 __global__ __launch_bounds__(IX_TPB) void k_concat(const uint32_t* __restrict__ tmp,
                                                    const uint64_t* __restrict__ tmpoff,
                                                    const uint32_t* __restrict__ nw,
                                                    const uint32_t* __restrict__ aval,
-                                                   const uint64_t* __restrict__ off, uint32_t nbm,
-                                                   uint32_t anb, uint32_t* __restrict__ out) {
+                                                   const uint64_t* __restrict__ off,
+                                                   const uint8_t* __restrict__ unpack, uint32_t nbm,
+                                                   uint32_t groups, uint32_t anb,
+                                                   uint32_t* __restrict__ out) {
     // one block per bitmap
     const uint32_t b = blockIdx.x;
     if (b >= nbm) return;
@@ -261,10 +297,16 @@ __global__ __launch_bounds__(IX_TPB) void k_concat(const uint32_t* __restrict__ 
     const uint32_t* src = tmp + tmpoff[b];
     uint32_t* dst = out + off[b];
     const uint32_t n = nw[b];
-    for (uint32_t i = threadIdx.x; i < n; i += IX_TPB) dst[i] = src[i];
+    uint32_t m = n;
+    if (unpack != nullptr && unpack[b]) {
+        block_unpack(src, n, dst);
+        m = groups;
+    } else {
+        for (uint32_t i = threadIdx.x; i < n; i += IX_TPB) dst[i] = src[i];
+    }
     if (threadIdx.x == 0) {
-        if (anb > 0) { dst[n] = aval[b]; dst[n + 1] = anb; }
-        else dst[n] = 0u;
+        if (anb > 0) { dst[m] = aval[b]; dst[m + 1] = anb; }
+        else dst[m] = 0u;
     }
 }
 
@@ -437,15 +479,7 @@ __global__ __launch_bounds__(IX_TPB) void k_query_scan(QueryArgs q, uint32_t nro
         for (int k = 0; k < 31; ++k) {
             const uint32_t row = rbase + (uint32_t)k * 64u + lane;
             bool hit = false;
-            if (row < nrows) {
-                const T x = col[row];
-                if (q.scan_bin) {
-                    const uint32_t b = bin_locate(q.bounds, q.nbounds, (double)x);
-                    hit = (b >= q.h0 && b < q.h1) || ((b == q.e0 || b == q.e1) && in_range(q, x));
-                } else {
-                    hit = in_range(q, x);
-                }
-            }
+            if (row < nrows) hit = in_range(q, col[row]);
             const uint64_t m = __ballot(hit);
             if (lane == 0) bal[w][k] = m;
         }
@@ -635,10 +669,11 @@ hipError_t launch_encode(int type, const uint32_t* rows, const uint32_t* start, 
 }
 
 hipError_t launch_concat(const uint32_t* tmp, const uint64_t* tmpoff, const uint32_t* nw,
-                         const uint32_t* aval, const uint64_t* off, uint32_t nbm, uint32_t anb,
-                         uint32_t* out, hipStream_t s) {
+                         const uint32_t* aval, const uint64_t* off, const uint8_t* unpack,
+                         uint32_t nbm, uint32_t groups, uint32_t anb, uint32_t* out, hipStream_t s) {
     if (!nbm) return hipSuccess;
-    hipLaunchKernelGGL(k_concat, dim3(nbm), dim3(IX_TPB), 0, s, tmp, tmpoff, nw, aval, off, nbm, anb, out);
+    hipLaunchKernelGGL(k_concat, dim3(nbm), dim3(IX_TPB), 0, s, tmp, tmpoff, nw, aval, off, unpack, nbm,
+                       groups, anb, out);
     return hipGetLastError();
 }
 

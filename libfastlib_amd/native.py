@@ -54,6 +54,8 @@ def lib():
             "fbh_bv_rids": [vp, _u32p, C.c_uint32, _u32p],
             "fbh_bv_nwords": [vp, _u32p],
             "fbh_bench_op": [vp, vp, C.c_int, C.c_int, C.POINTER(C.c_float), _u32p],
+            "fbh_bench_op_rot": [C.POINTER(vp), C.POINTER(vp), C.c_int, C.c_int, C.c_int,
+                                 C.POINTER(C.c_float), _u32p],
         }.items():
             f = getattr(L, name)
             f.argtypes = args
@@ -192,6 +194,21 @@ def bench_op(a: Bitvector, b: Bitvector, op: int, reps: int):
     return ms.value, nw.value
 
 
+def bench_op_rot(a, b, op: int, reps: int):
+    """As bench_op, cycling over the operand pairs zip(a, b) so that the
+    working set streams from HBM instead of the last-level cache."""
+    n = len(a)
+    if n == 0 or len(b) != n:
+        raise ValueError("bench_op_rot needs equal, non-empty operand lists")
+    pa = (C.c_void_p * n)(*[x._h for x in a])
+    pb = (C.c_void_p * n)(*[x._h for x in b])
+    ms = C.c_float()
+    nw = C.c_uint32()
+    check(lib().fbh_bench_op_rot(pa, pb, n, int(op), int(reps), C.byref(ms), C.byref(nw)),
+          "fbh_bench_op_rot")
+    return ms.value, nw.value
+
+
 # ---------------------------------------------------------------------------
 # Indexes
 RELIC, BIN = 1, 2
@@ -199,12 +216,14 @@ COL_INT32, COL_UINT32, COL_UINT16, COL_FLOAT, COL_DOUBLE = 0, 1, 2, 3, 4
 OP_NONE, OP_LT, OP_LE = 0, 1, 2
 PLAN_AUTO, PLAN_LITERAL, PLAN_COMPRESSED, PLAN_SCAN = 0, 1, 2, 3
 PLAN_NAMES = {0: "auto", 1: "literal", 2: "compressed", 3: "scan"}
+COST_IAPI, COST_PARTITION = 0, 1
 _COLT = {np.dtype(np.int32): COL_INT32, np.dtype(np.uint32): COL_UINT32,
          np.dtype(np.uint16): COL_UINT16, np.dtype(np.float32): COL_FLOAT,
          np.dtype(np.float64): COL_DOUBLE}
 _dp = C.POINTER(C.c_double)
 _i64p = C.POINTER(C.c_int64)
 _u64p = C.POINTER(C.c_uint64)
+_ip = C.POINTER(C.c_int)
 
 
 def _index_sigs(L):
@@ -224,6 +243,11 @@ def _index_sigs(L):
                                C.POINTER(vp), C.POINTER(C.c_int)],
         "fbh_index_evaluate_rids": [vp, C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, _u32p,
                                     C.c_uint64, _u64p],
+        "fbh_index_evaluate_batch": [vp, C.c_uint32, _dp, _ip, _dp, _ip, _ip, C.c_int, _u64p, _ip,
+                                     C.POINTER(C.c_float)],
+        "fbh_index_query_bytes": [vp, C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, _dp],
+        "fbh_index_column_bytes": [vp, _u64p],
+        "fbh_index_set_cost_model": [vp, C.c_int],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -335,3 +359,41 @@ class Index:
         check(L.fbh_index_evaluate_rids(self._h, lo, lop, hi, rop, plan, out.ctypes.data_as(_u32p),
                                         n.value, C.byref(n)), "fbh_index_evaluate_rids")
         return out[: n.value]
+
+    def evaluate_batch(self, lo, lop, hi, rop, plan=PLAN_AUTO, plans=None):
+        """Evaluate many ranges back to back on the device.
+
+        lo/hi are arrays, lop/rop arrays or scalars; `plans` (optional) picks
+        a plan per query.  Returns (hits uint64[nq], plans used int32[nq],
+        device milliseconds from the first to the last query)."""
+        lo = np.ascontiguousarray(lo, np.float64)
+        nq = lo.size
+        hi = np.ascontiguousarray(np.broadcast_to(hi, (nq,)), np.float64)
+        lop = np.ascontiguousarray(np.broadcast_to(lop, (nq,)), np.int32)
+        rop = np.ascontiguousarray(np.broadcast_to(rop, (nq,)), np.int32)
+        pp = None if plans is None else np.ascontiguousarray(np.broadcast_to(plans, (nq,)), np.int32)
+        hits = np.zeros(nq, np.uint64)
+        used = np.zeros(nq, np.int32)
+        ms = C.c_float()
+        check(self._lib().fbh_index_evaluate_batch(
+            self._h, nq, lo.ctypes.data_as(_dp), lop.ctypes.data_as(_ip), hi.ctypes.data_as(_dp),
+            rop.ctypes.data_as(_ip), None if pp is None else pp.ctypes.data_as(_ip), int(plan),
+            hits.ctypes.data_as(_u64p), used.ctypes.data_as(_ip), C.byref(ms)),
+            "fbh_index_evaluate_batch")
+        return hits, used, ms.value
+
+    def query_bytes(self, lo, lop, hi, rop, plan):
+        """Algorithmic bytes `plan` moves for the range."""
+        v = C.c_double()
+        check(self._lib().fbh_index_query_bytes(self._h, lo, lop, hi, rop, int(plan), C.byref(v)),
+              "fbh_index_query_bytes")
+        return v.value
+
+    def set_cost_model(self, model):
+        """COST_IAPI (in-memory arrays, the default) or COST_PARTITION."""
+        check(self._lib().fbh_index_set_cost_model(self._h, int(model)), "fbh_index_set_cost_model")
+
+    def column_bytes(self):
+        v = C.c_uint64()
+        check(self._lib().fbh_index_column_bytes(self._h, C.byref(v)), "fbh_index_column_bytes")
+        return v.value

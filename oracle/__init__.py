@@ -176,6 +176,12 @@ def col_type_of(arr):
     raise TypeError(a.dtype)
 
 
+# Cost models of the plan choice: arrays registered through the in-memory
+# API (src/bord.cpp:5825-6058) and partitions with index files
+# (src/column.cpp:6029-6207).
+COST_IAPI, COST_PARTITION = 0, 1
+
+
 class OracleIndex:
     """ibis::relic / ibis::bin restated on the CPU (oracle/index.cpp)."""
 
@@ -249,6 +255,34 @@ class OracleIndex:
         L.orc_free(r)
         return int(cnt.value), rids
 
+    def evaluate_plan(self, lo, lop, hi, rop, plan=0, model=COST_IAPI):
+        """(hit count, ascending rids, plan used) under `plan`: 0 the
+        reference's own choice under the cost `model`, 1 the index, 3 the
+        raw scan."""
+        L = oracle().lib
+        cnt = C.c_uint64()
+        used = C.c_int()
+        r, nr = _u32p(), C.c_size_t()
+        rc = L.orc_index_eval_plan(C.c_void_p(self._h), self.col.ctypes.data_as(C.c_void_p), self.type,
+                                   C.c_double(lo), C.c_int(lop), C.c_double(hi), C.c_int(rop),
+                                   C.c_int(plan), C.c_int(model), C.byref(cnt), C.byref(r),
+                                   C.byref(nr), C.byref(used))
+        if rc != 0:
+            raise RuntimeError("orc_index_eval_plan failed")
+        rids = np.ctypeslib.as_array(r, (nr.value,)).copy() if nr.value else np.zeros(0, np.uint32)
+        L.orc_free(r)
+        return int(cnt.value), rids, int(used.value)
+
+    def cost(self, lo, lop, hi, rop, model=COST_IAPI):
+        """(index cost, scan cost, plan, empty) of the reference's choice."""
+        L = oracle().lib
+        ic, sc = C.c_double(), C.c_double()
+        plan, empty = C.c_int(), C.c_int()
+        L.orc_index_cost(C.c_void_p(self._h), self.col.ctypes.data_as(C.c_void_p), self.type,
+                         C.c_double(lo), C.c_int(lop), C.c_double(hi), C.c_int(rop), C.c_int(model),
+                         C.byref(ic), C.byref(sc), C.byref(plan), C.byref(empty))
+        return ic.value, sc.value, plan.value, bool(empty.value)
+
 
 # FastBitDataType codes of the reference's iapi (src/iapi.h)
 _FB_TYPE = {COL_I32: 5, COL_U32: 6, COL_U16: 4, COL_F32: 9, COL_F64: 10}
@@ -271,12 +305,20 @@ def ref_relic_card(ind, card):
     return out
 
 
+# The reference's array registry (src/iapi.cpp:1644-1665) identifies arrays
+# by name and by buffer address and never forgets them: registered buffers
+# are kept alive for the life of the process so that no later array reuses
+# an address the registry still maps to an older name.
+_REGISTERED = []
+
+
 class RefIapi:
     """One array registered in the reference's in-memory API."""
 
     def __init__(self, name, col, spec):
         self.name = name.encode()
         self.col = np.ascontiguousarray(col)
+        _REGISTERED.append(self.col)
         L = reference().lib
         L.ref_iapi_query.restype = C.c_int64
         k, o, b = _dp(), _i64p(), _u32p()
@@ -293,7 +335,23 @@ class RefIapi:
         for p in (k, o, b):
             L.ref_free(p)
 
+    def query_range(self, lo, lop, hi, rop, want_rids=True):
+        """lo OP1 x OP2 hi as one range term (as a where clause gives it)."""
+        L = reference().lib
+        L.ref_iapi_query_range.restype = C.c_int64
+        cap = self.col.size if want_rids else 0
+        buf = np.zeros(max(cap, 1), np.uint64)
+        n = L.ref_iapi_query_range(self.name, C.c_double(lo), C.c_int(lop), C.c_double(hi),
+                                   C.c_int(rop),
+                                   buf.ctypes.data_as(C.POINTER(C.c_uint64)) if want_rids else None,
+                                   C.c_uint64(cap))
+        if n < 0:
+            raise RuntimeError(f"ref_iapi_query_range failed {n}")
+        return int(n), (buf[:n].astype(np.uint32) if want_rids else None)
+
     def query(self, lo, lop, hi, rop, want_rids=True):
+        """lo OP1 x AND x OP2 hi: two one-sided selections combined with
+        fastbit_selection_combine (each term picks its own plan)."""
         L = reference().lib
         cap = self.col.size if want_rids else 0
         buf = np.zeros(max(cap, 1), np.uint64)
@@ -359,3 +417,32 @@ def compact_value(left, right, start=0.0):
     if not (sep > left and sep <= right):
         sep = right
     return sep
+
+
+class RefAttached:
+    """A column registered in the reference's in-memory API with an index
+    attached from keys/offsets/bitmaps (no rebuild)."""
+
+    def __init__(self, name, col, keys, offsets, bitmaps):
+        self.name = name.encode()
+        self.col = np.ascontiguousarray(col)
+        self.keys = np.ascontiguousarray(keys, np.float64)
+        self.offsets = np.ascontiguousarray(offsets, np.int64)
+        self.bitmaps = np.ascontiguousarray(bitmaps, np.uint32)
+        _REGISTERED.append((self.col, self.keys, self.offsets, self.bitmaps))
+        L = reference().lib
+        L.ref_iapi_query.restype = C.c_int64
+        rc = L.ref_iapi_attach(self.name, _FB_TYPE[col_type_of(self.col)],
+                               self.col.ctypes.data_as(C.c_void_p), C.c_uint64(self.col.size),
+                               self.keys.ctypes.data_as(_dp), C.c_uint64(self.keys.size),
+                               self.offsets.ctypes.data_as(_i64p), C.c_uint64(self.offsets.size),
+                               self.bitmaps.ctypes.data_as(_u32p), C.c_uint64(self.bitmaps.size))
+        if rc < 0:
+            raise RuntimeError(f"ref_iapi_attach failed rc={rc}")
+
+    def count(self, lo, lop, hi, rop):
+        n = reference().lib.ref_iapi_query(self.name, C.c_double(lo), C.c_int(lop), C.c_double(hi),
+                                           C.c_int(rop), None, C.c_uint64(0))
+        if n < 0:
+            raise RuntimeError(f"ref_iapi_query failed {n}")
+        return int(n)

@@ -19,11 +19,22 @@ Index* relic_card(const uint32_t* ind, uint32_t n, uint32_t card) {
     for (uint32_t i = 0; i < n; ++i)
         if (ind[i] < card) ix->bits[ind[i]]->setBit(i, 1);
     for (uint32_t i = 0; i < card; ++i) ix->bits[i]->adjustSize(0, n);
+    ix->offsets_built = true;     // offset64 filled at src/irelic.cpp:186-190
     return ix;
 }
 
+// index::optionalUnpack without a <compressing> directive
+// (src/index.cpp:8803-8854): bitmaps whose bytes() (code words * 4 +
+// sizeof(bitvector), 40) exceed max(nrows/9, 52) are decompressed.
+static void optional_unpack(Index* ix) {
+    const uint64_t bar1 = std::max<uint64_t>(ix->nrows / 9u, 40u + 12u);
+    for (BV* b : ix->bits)
+        if (b != nullptr && 4ull * b->v.size() + 40u > bar1) b->decompress();
+}
+
 // src/index.cpp:2552-2611 (a std::map from value to bitvector, setBit per
-// row), then every bitmap padded to nrows
+// row), then every bitmap padded to nrows; relic::construct ends with
+// optionalUnpack (src/irelic.cpp:910)
 Index* relic_values(const double* v, uint32_t n) {
     std::map<double, BV*> m;
     for (uint32_t i = 0; i < n; ++i) {
@@ -40,6 +51,7 @@ Index* relic_values(const double* v, uint32_t n) {
         ix->vals.push_back(kv.first);
         ix->bits.push_back(kv.second);
     }
+    optional_unpack(ix);
     return ix;
 }
 
@@ -62,7 +74,7 @@ static uint32_t bin_locate(const std::vector<double>& bounds, double val) {
     return nobs;
 }
 
-// src/ibin.cpp:3350-3449
+// src/ibin.cpp:3350-3449, then optionalUnpack
 Index* bin_bounds(const double* v, uint32_t n, const double* bounds, uint32_t nb) {
     Index* ix = new Index();
     ix->kind = 2;
@@ -111,6 +123,7 @@ Index* bin_bounds(const double* v, uint32_t n, const double* bounds, uint32_t nb
             ix->bits.resize(k);
         }
     }
+    optional_unpack(ix);   // bin::construct, src/ibin.cpp:2960
     return ix;
 }
 
@@ -245,25 +258,28 @@ static bool in_range(const void* col, int type, uint32_t i, double lo, int lop, 
     }
 }
 
-BV evaluate(const Index& ix, const void* col, int type, double lo, int lop, double hi, int rop) {
+// relic::locate(expr, hit0, hit1) -- src/irelic.cpp:1416-1809 (the LT/LE
+// cases of the two sides)
+static void relic_hits(const Index& ix, double lo, int lop, double hi, int rop, uint32_t& hit0,
+                       uint32_t& hit1) {
     const uint32_t nobs = (uint32_t)ix.bits.size();
-    if (ix.kind == 1) {
-        uint32_t bin0 = lop != OPN ? relic_locate(ix.vals, lo) : 0;
-        uint32_t bin1 = rop != OPN ? relic_locate(ix.vals, hi) : 0;
-        uint32_t hit0, hit1;
-        if (lop == OPLT) hit0 = bin0;
-        else if (lop == OPLE) hit0 = bin0 > 0 ? bin0 - (lo == ix.vals[bin0 - 1]) : 0;
-        else hit0 = 0;
-        if (rop == OPLT) hit1 = bin1 > 0 ? bin1 - (hi == ix.vals[bin1 - 1]) : 0;
-        else if (rop == OPLE) hit1 = bin1;
-        else hit1 = nobs;
-        if (hit1 < hit0) hit1 = hit0;
-        return sum_bins(ix, hit0, hit1);
-    }
-    // bin::locate(expr, cand0, cand1, hit0, hit1) -- src/ibin.cpp:7596-8460
+    const uint32_t bin0 = lop != OPN ? relic_locate(ix.vals, lo) : 0;
+    const uint32_t bin1 = rop != OPN ? relic_locate(ix.vals, hi) : 0;
+    if (lop == OPLT) hit0 = bin0;
+    else if (lop == OPLE) hit0 = bin0 > 0 ? bin0 - (lo == ix.vals[bin0 - 1]) : 0;
+    else hit0 = 0;
+    if (rop == OPLT) hit1 = bin1 > 0 ? bin1 - (hi == ix.vals[bin1 - 1]) : 0;
+    else if (rop == OPLE) hit1 = bin1;
+    else hit1 = nobs;
+    if (hit1 < hit0) hit1 = hit0;
+}
+
+// bin::locate(expr, cand0, cand1, hit0, hit1) -- src/ibin.cpp:7596-8460
+static void bin_range(const Index& ix, double lo, int lop, double hi, int rop, uint32_t& cand0,
+                      uint32_t& cand1, uint32_t& hit0, uint32_t& hit1) {
+    const uint32_t nobs = (uint32_t)ix.bits.size();
     const uint32_t bin0 = lop != OPN ? bin_locate(ix.vals, lo) : 0;
     const uint32_t bin1 = rop != OPN ? bin_locate(ix.vals, hi) : 0;
-    uint32_t cand0, cand1, hit0, hit1;
     if (lop == OPLE) {
         if (bin0 >= nobs) { hit0 = cand0 = nobs; }
         else if (lo > ix.maxval[bin0]) { hit0 = cand0 = bin0 + 1; }
@@ -287,6 +303,17 @@ BV evaluate(const Index& ix, const void* col, int type, double lo, int lop, doub
         else { hit1 = bin1; cand1 = bin1 + 1; }
     } else { hit1 = cand1 = nobs; }
     if (hit1 < hit0) hit1 = hit0;
+}
+
+BV evaluate(const Index& ix, const void* col, int type, double lo, int lop, double hi, int rop) {
+    const uint32_t nobs = (uint32_t)ix.bits.size();
+    if (ix.kind == 1) {
+        uint32_t hit0, hit1;
+        relic_hits(ix, lo, lop, hi, rop, hit0, hit1);
+        return sum_bins(ix, hit0, hit1);
+    }
+    uint32_t cand0, cand1, hit0, hit1;
+    bin_range(ix, lo, lop, hi, rop, cand0, cand1, hit0, hit1);
     BV lower = sum_bins(ix, hit0, hit1);
     // edge bins through the raw values (checkBin -> doScan fallback,
     // src/ibin.cpp:6651-6704)
@@ -304,6 +331,156 @@ BV evaluate(const Index& ix, const void* col, int type, double lo, int lop, doub
     if (cand0 < hit0) check(cand0);
     if (cand1 > hit1 && hit1 < nobs) check(hit1);
     return lower;
+}
+
+// part::doScan over every row (src/part.cpp:11130-11237 for integer
+// columns, 12094-12219 / 12849- for float / double).
+BV scan(const void* col, int type, uint32_t n, double lo, int lop, double hi, int rop) {
+    BV r;
+    for (uint32_t i = 0; i < n; ++i)
+        if (in_range(col, type, i, lo, lop, hi, rop)) r.setBit(i, 1);
+    r.adjustSize(0, n);
+    return r;
+}
+
+static uint32_t elem_size(int type) {
+    return type == COL_U16 ? 2u : type == COL_F64 ? 8u : 4u;
+}
+
+// bitvector::getSerialSize() in bytes (src/bitvector.h:150-154)
+static double serial_bytes(const Index& ix, uint32_t b) {
+    if (!ix.bits[b]) return 0.0;
+    return 4.0 * (double)(ix.bits[b]->v.size() + 1 + (ix.nrows % 31u ? 1u : 0u));
+}
+
+double estimate_cost(const Index& ix, int type, double lo, int lop, double hi, int rop, int model) {
+    const uint32_t nobs = (uint32_t)ix.bits.size();
+    // offset64 is filled by relic(col, card, ind) and by reading an index
+    // file; the in-memory constructors leave it empty
+    const bool offsets = model == COST_PARTITION || ix.offsets_built;
+    auto span = [&](uint32_t b0, uint32_t b1) {
+        double m = 0;
+        for (uint32_t b = b0; b < b1 && b < nobs; ++b) m += serial_bytes(ix, b);
+        return m;
+    };
+    if (ix.kind == 1) {
+        // relic::estimateCost -- src/irelic.cpp:1857-1902
+        uint32_t h0, h1;
+        relic_hits(ix, lo, lop, hi, rop, h0, h1);
+        if (h0 >= h1) return 0.0;
+        if (offsets) {
+            if (h1 > h0 + 1) {
+                const double tot = span(0, nobs), mid = span(h0, h1);
+                return std::floor(tot / 2) >= mid ? mid : tot - mid;
+            }
+            return 0.5 * span(h0, h1);
+        }
+        if (h1 > h0 + 1) {
+            double r = 0;
+            for (uint32_t b = h0; b < h1 && b < nobs; ++b) r += bm_bytes(ix.bits[b]);
+            return r;
+        }
+        return h0 < nobs ? 0.5 * bm_bytes(ix.bits[h0]) : 0.0;
+    }
+    // bin::estimateCost -- src/ibin.cpp:6531-6569
+    double r = 0;
+    uint32_t cand0 = 0, cand1 = nobs, hit0 = nobs, hit1 = 0;
+    if (offsets) {
+        bin_range(ix, lo, lop, hi, rop, cand0, cand1, hit0, hit1);
+        if (cand0 < cand1 && cand1 <= nobs) {
+            const double tot = span(0, nobs), mid = span(cand0, cand1);
+            r = std::floor(tot / 2) >= mid ? mid : tot - mid;
+        }
+    }
+    if (hit0 > cand0 || hit1 < cand1) {
+        const uint32_t es = elem_size(type);
+        if (nobs > 0) {
+            // elementSize() * nrows / nobs in unsigned 32-bit arithmetic
+            const double ccheck = (double)((uint32_t)(es * ix.nrows) / nobs);
+            r += (hit0 > cand0 && hit1 < cand1 && hit0 <= hit1) ? 2.0 * ccheck : ccheck;
+        } else {
+            r += (double)(uint32_t)(es * ix.nrows);
+        }
+    }
+    return r;
+}
+
+// part::countPages over a mask of n ones (src/part.cpp:2616-2678): the
+// indexSet walk sees [0, 31g) as one range, then the active word's rows.
+static uint32_t full_mask_pages(uint32_t n, uint32_t es) {
+    if (n == 0) return 0;
+    const uint32_t wpp = 4096u / es;
+    const uint32_t g = n / 31u, anb = n % 31u;
+    uint32_t res = 0, last = 0;
+    if (g > 0) { res += (31u * g) / wpp; last = 31u * g; }
+    if (anb > 0) {
+        const uint32_t first = 31u * g;
+        const uint32_t p0 = first / wpp;
+        res += (last < p0 * wpp) ? 1u : 0u;
+        last = n - 1;
+        res += (last / wpp > p0) ? 1u : 0u;
+    }
+    return res == 0 ? 1u : res;
+}
+
+double scan_cost(const Index& ix, int type, int model) {
+    if (model == COST_PARTITION) {
+        // column::evaluateRange -- src/column.cpp:6102-6111, pageSize 4096
+        return 4096.0 * full_mask_pages(ix.nrows, elem_size(type)) + 8.0 * ix.nrows / 4096.0;
+    }
+    // bord::column::evaluateRange -- src/bord.cpp:5903-5912
+    return ix.nrows * 0.5 + 999.0;
+}
+
+// qContinuousRange::overlap (src/qExpr.cpp:1673-1803) for lo OP x OP hi
+// against the column's [cmin, cmax]
+static bool overlap(double lo, int lop, double hi, int rop, double cmin, double cmax) {
+    if (!(cmin <= cmax)) return true;
+    if (lop == OPLT && !(lo < cmax)) return false;
+    if (lop == OPLE && !(lo <= cmax)) return false;
+    if (lop == OPN) {
+        if (rop == OPLT) return cmin < hi;
+        if (rop == OPLE) return cmin <= hi;
+        return false;
+    }
+    if (rop == OPLT) return cmin < hi;
+    if (rop == OPLE) return cmin <= hi;
+    return true;
+}
+
+int auto_plan(const Index& ix, const void* col, int type, double lo, int lop, double hi, int rop,
+              int model, bool* empty) {
+    *empty = false;
+    if (model == COST_PARTITION && ix.nrows > 0) {
+        double cmin = DBL_MAX, cmax = -DBL_MAX;
+        for (uint32_t i = 0; i < ix.nrows; ++i) {
+            const double x = value_at(col, type, i);
+            if (x < cmin) cmin = x;
+            if (x > cmax) cmax = x;
+        }
+        if (!overlap(lo, lop, hi, rop, cmin, cmax)) { *empty = true; return PLAN_INDEX; }
+    }
+    const double ic = estimate_cost(ix, type, lo, lop, hi, rop, model);
+    return ic < scan_cost(ix, type, model) ? PLAN_INDEX : PLAN_SCAN;
+}
+
+BV evaluate_plan(const Index& ix, const void* col, int type, double lo, int lop, double hi, int rop,
+                 int plan, int model, int* used) {
+    bool empty = false;
+    if (plan == 0) plan = auto_plan(ix, col, type, lo, lop, hi, rop, model, &empty);
+    if (used) *used = plan;
+    if (lop == OPN && rop == OPN) {          // no bound: every row
+        BV r;
+        r.set(1, ix.nrows);
+        return r;
+    }
+    if (empty) {
+        BV r;
+        r.set(0, ix.nrows);
+        return r;
+    }
+    if (plan == PLAN_SCAN) return scan(col, type, ix.nrows, lo, lop, hi, rop);
+    return evaluate(ix, col, type, lo, lop, hi, rop);
 }
 
 } // namespace orc

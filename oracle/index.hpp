@@ -8,6 +8,12 @@ namespace orc {
 
 enum { OPN = 0, OPLT = 1, OPLE = 2 };
 enum { COL_I32 = 0, COL_U32 = 1, COL_U16 = 2, COL_F32 = 3, COL_F64 = 4 };
+// Plan choice of the reference: COST_IAPI is bord::column::evaluateRange
+// (arrays registered through the in-memory API, src/bord.cpp:5825-6058),
+// COST_PARTITION is column::evaluateRange (data partitions with index
+// files, src/column.cpp:6029-6207).
+enum { COST_IAPI = 0, COST_PARTITION = 1 };
+enum { PLAN_INDEX = 1, PLAN_SCAN = 3 };
 
 struct Index {
     int kind = 0;                      // 1 relic, 2 bin
@@ -15,6 +21,7 @@ struct Index {
     std::vector<double> vals;          // relic values / bin bounds
     std::vector<double> minval, maxval;
     std::vector<BV*> bits;             // null for empty bins
+    bool offsets_built = false;        // the constructor filled offset64
     ~Index() { for (auto b : bits) delete b; }
 };
 
@@ -35,5 +42,22 @@ void write_index(const Index& ix, std::vector<double>& keys, std::vector<int64_t
 // comparison rules for the column type (src/part.cpp:11130-12219).
 // Returns the hit bitvector.
 BV evaluate(const Index& ix, const void* col, int type, double lo, int lop, double hi, int rop);
+
+// part::doScan over all rows: integer columns convert the bounds to the
+// element type first (src/part.cpp:11130-11237), floats compare as doubles.
+BV scan(const void* col, int type, uint32_t n, double lo, int lop, double hi, int rop);
+
+// relic::estimateCost / bin::estimateCost (src/irelic.cpp:1857-1902,
+// src/ibin.cpp:6531-6569) as the index built under `model` has them.
+double estimate_cost(const Index& ix, int type, double lo, int lop, double hi, int rop, int model);
+// The scan cost the index cost is compared with.
+double scan_cost(const Index& ix, int type, int model);
+// The reference's choice: PLAN_INDEX or PLAN_SCAN; *empty when the range
+// misses the column's [min, max] and nothing is evaluated.
+int auto_plan(const Index& ix, const void* col, int type, double lo, int lop, double hi, int rop,
+              int model, bool* empty);
+// plan 0 (the reference's choice), PLAN_INDEX or PLAN_SCAN.
+BV evaluate_plan(const Index& ix, const void* col, int type, double lo, int lop, double hi, int rop,
+                 int plan, int model, int* used);
 
 } // namespace orc

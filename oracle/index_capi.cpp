@@ -57,4 +57,32 @@ int orc_index_eval(void* ix, const void* col, int type, double lo, int lop, doub
     }
 }
 
+// plan: 0 the reference's choice under `model`, 1 index, 3 scan
+int orc_index_eval_plan(void* ix, const void* col, int type, double lo, int lop, double hi, int rop,
+                        int plan, int model, uint64_t* count, uint32_t** rids, size_t* nr,
+                        int* used) {
+    try {
+        BV r = evaluate_plan(*(Index*)ix, col, type, lo, lop, hi, rop, plan, model, used);
+        std::vector<word_t> v;
+        r.rids(v);
+        *count = r.cnt();
+        *rids = dup(v);
+        *nr = v.size();
+        return 0;
+    } catch (...) {
+        return -2;
+    }
+}
+
+int orc_index_cost(void* ix, const void* col, int type, double lo, int lop, double hi, int rop,
+                   int model, double* icost, double* scost, int* plan, int* empty) {
+    const Index& x = *(Index*)ix;
+    *icost = estimate_cost(x, type, lo, lop, hi, rop, model);
+    *scost = scan_cost(x, type, model);
+    bool e = false;
+    *plan = auto_plan(x, col, type, lo, lop, hi, rop, model, &e);
+    *empty = e ? 1 : 0;
+    return 0;
+}
+
 } // extern "C"

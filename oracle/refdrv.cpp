@@ -202,3 +202,45 @@ int64_t ref_iapi_query(const char* name, double lo, int lop, double hi, int rop,
 void ref_iapi_free_all() { fastbit_iapi_free_all(); }
 
 } // extern "C"
+
+extern "C" {
+
+// fastbit_iapi_register_array + fastbit_iapi_attach_full_index
+// (src/iapi.cpp:2037-2060): the reference queries an index built elsewhere
+// (no copy: the arrays must outlive the registration).
+int ref_iapi_attach(const char* name, int dtype, void* data, uint64_t n, double* keys, uint64_t nk,
+                    int64_t* offs, uint64_t no, uint32_t* bms, uint64_t nb) {
+    ensure_init();
+    int ierr = fastbit_iapi_register_array(name, (FastBitDataType)dtype, data, n);
+    if (ierr < 0) return -10 + ierr;
+    return fastbit_iapi_attach_full_index(name, keys, nk, offs, no, bms, nb);
+}
+
+} // extern "C"
+
+extern "C" {
+
+// One two-sided range lo OP1 x OP2 hi as a single qContinuousRange term
+// (the form a where clause "lo <= x < hi" takes), evaluated through
+// fastbit_selection_evaluate (src/iapi.cpp:1383-1418).  ops: 0 none, 1 <,
+// 2 <=.  Returns the hit count; rids (ascending) into buf.
+int64_t ref_iapi_query_range(const char* name, double lo, int lop, double hi, int rop, uint64_t* buf,
+                             uint64_t cap) {
+    ensure_init();
+    auto op = [](int o) {
+        return o == 1 ? ibis::qExpr::OP_LT : o == 2 ? ibis::qExpr::OP_LE : ibis::qExpr::OP_UNDEFINED;
+    };
+    ibis::qContinuousRange* q = new ibis::qContinuousRange(lo, op(lop), name, op(rop), hi);
+    FastBitSelectionHandle h = static_cast<FastBitSelectionHandle>(q);
+    int64_t cnt = fastbit_selection_evaluate(h);
+    if (cnt > 0 && buf) {
+        int64_t got = fastbit_selection_get_coordinates(h, buf, cap, 0);
+        if (got < 0) cnt = got;
+    }
+    fastbit_selection_free(h);
+    return cnt;
+}
+
+} // extern "C"
+
+extern "C" void ref_set_verbose(int v) { ensure_init(); ibis::gVerbose = v; }

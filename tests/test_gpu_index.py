@@ -94,7 +94,8 @@ def test_bin_float(L, n):
 
 def test_bin_int_edges(L):
     """Integer column under a bin index: the edge check uses part::doScan's
-    cast-to-T bound rules (fractional and negative bounds)."""
+    cast-to-T bound rules (fractional and negative bounds); the scan plan
+    is part::doScan over every row; AUTO answers as the reference does."""
     rng = np.random.default_rng(3)
     n = 31 * 800 + 3
     col = rng.integers(-500, 500, size=n).astype(np.int32)
@@ -106,6 +107,95 @@ def test_bin_int_edges(L):
         for lop in (1, 2):
             for rop in (1, 2):
                 c, r = oi.evaluate(lo, lop, hi, rop)
-                for plan in PLANS:
+                for plan in (1, 2):
                     assert gi.evaluate(lo, lop, hi, rop, plan)[0] == c
                 assert np.array_equal(gi.rids(lo, lop, hi, rop, 1), r)
+                cs, rs, _ = oi.evaluate_plan(lo, lop, hi, rop, plan=3)
+                assert gi.evaluate(lo, lop, hi, rop, 3)[0] == cs
+                assert np.array_equal(gi.rids(lo, lop, hi, rop, 3), rs)
+                ca, _, used = oi.evaluate_plan(lo, lop, hi, rop)
+                h, gused, _ = gi.evaluate(lo, lop, hi, rop, 0)
+                assert h == ca
+                assert (gused == 3) == (used == 3)
+
+
+@pytest.mark.parametrize("model", [0, 1])
+@pytest.mark.parametrize("kind", ["heavy", "wide", "relic_card"])
+def test_auto_plan_matches_reference_rule(L, kind, model):
+    """AUTO picks index or scan by the reference's cost rule (both cost
+    models); on integer columns with fractional bounds the two disagree, so
+    the counts show the choice."""
+    rng = np.random.default_rng(len(kind) + 10 * model)
+    n = 31 * 3000 + 17
+    if kind == "heavy":
+        col = np.where(rng.random(n) < 0.8, 7, rng.integers(-40, 40, size=n)).astype(np.int32)
+        gi, oi = L.Index.relic(col), O.OracleIndex.relic_values(col)
+    elif kind == "wide":
+        col = rng.integers(-3000, 3000, size=n).astype(np.int32)
+        gi, oi = L.Index.relic(col), O.OracleIndex.relic_values(col)
+    else:
+        col = rng.integers(0, 70, size=n).astype(np.uint32)
+        gi, oi = L.Index.relic_card(col, 60), O.OracleIndex.relic_card(col, 60)
+    gi.set_cost_model(model)
+    lo_v, hi_v = float(col.min()), float(col.max())
+    for _ in range(40):
+        a, b = sorted(rng.uniform(lo_v - 3, hi_v + 3, size=2))
+        q = (float(np.round(a) - 0.25), int(rng.integers(0, 3)), float(np.round(b) + 0.5),
+             int(rng.integers(1, 3)))
+        c, r, used = oi.evaluate_plan(*q, model=model)
+        ic, sc, _, _ = oi.cost(*q, model=model)
+        gic, gsc, gplan = gi.estimate_cost(*q)
+        assert (gic, gsc) == (ic, sc), q
+        h, gused, bv = gi.evaluate(*q, 0, want_bitvector=True)
+        assert h == c, (q, used, gused)
+        assert (gused == 3) == (used == 3), q
+        assert np.array_equal(bv.rids(), r)
+
+
+def test_batch_matches_single(L):
+    """evaluate_batch gives the counts of one-at-a-time evaluation, per plan
+    and with the heuristic's per-query choice."""
+    rng = np.random.default_rng(11)
+    n = 31 * 5000 + 9
+    col = rng.normal(size=n).astype(np.float32)
+    qs = np.quantile(col.astype(np.float64), np.linspace(0, 1, 200)[1:-1])
+    gi = L.Index.bin(col, np.concatenate([np.unique(qs), [np.finfo(np.float64).max]]))
+    oi = O.OracleIndex.bin_bounds(col, np.concatenate([np.unique(qs), [np.finfo(np.float64).max]]))
+    qq = _queries(rng, 0, 64, ints=False)
+    lo = np.array([q[0] for q in qq]); lop = np.array([q[1] for q in qq])
+    hi = np.array([q[2] for q in qq]); rop = np.array([q[3] for q in qq])
+    want = np.array([oi.evaluate(*q)[0] for q in qq], np.uint64)
+    for plan in (0, 1, 2, 3):
+        hits, used, ms = gi.evaluate_batch(lo, lop, hi, rop, plan)
+        assert np.array_equal(hits, want), plan
+        assert ms >= 0
+        if plan:
+            assert (used == plan).all()
+        else:
+            for q, u in zip(qq, used):
+                assert gi.estimate_cost(*q)[2] == u
+    mixed = rng.integers(1, 4, size=len(qq))
+    hits, used, _ = gi.evaluate_batch(lo, lop, hi, rop, plans=mixed)
+    assert np.array_equal(hits, want) and np.array_equal(used, mixed)
+
+
+@pytest.mark.slow
+def test_bin_2p27_reference_words(L):
+    """The 2^27-row float index (10,000 equal-weight bins) built on the GPU is
+    word-identical to the reference's (digests in tests/golden)."""
+    import hashlib
+    import json
+    import os
+    from libfastlib_amd import datagen as G
+    here = os.path.join(os.path.dirname(__file__), "golden")
+    meta = json.load(open(os.path.join(here, "config3_2p27.json")))
+    bounds = np.load(os.path.join(here, "config3_2p27_bounds.npz"))["bounds"]
+    col = G.float_mix(meta["seed"], meta["rows"], pool=100000, wnorm=0.7, s=1.2)
+    gi = L.Index.bin(col, bounds)
+    nobs, nrows, nw = gi.info()
+    assert (nobs, nrows) == (meta["nobs"], meta["rows"])
+    k, o, b = gi.deconstruct()
+    assert hashlib.sha256(k.tobytes()).hexdigest() == meta["keys_sha256"]
+    assert hashlib.sha256(o.tobytes()).hexdigest() == meta["offsets_sha256"]
+    assert len(b) == meta["nwords"]
+    assert hashlib.sha256(b.tobytes()).hexdigest() == meta["bitmaps_sha256"]
