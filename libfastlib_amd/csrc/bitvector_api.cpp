@@ -82,7 +82,7 @@ int bv_ensure_ck(fbh_bitvector* bv) {
 // do_cnt (src/bitvector.cpp:583-608): counts groups and ones on the device;
 // an all-zero vector of more than one word collapses to one 0-fill.
 int bv_do_cnt(fbh_bitvector* bv, uint32_t* nb_out) {
-    uint32_t* ctl = (uint32_t*)scratch(2).get(64);
+    uint32_t* ctl = (uint32_t*)scratch(2).get(4ull * fbk::cnt_scratch_words());
     if (!ctl) return fail(FBH_ERR_NOMEM, "scratch allocation failed");
     int rc = check(fbk::launch_cnt(bv->d, bv->nw, ctl, stream()), "cnt");
     if (rc) return rc;
@@ -314,6 +314,8 @@ int bv_compress(fbh_bitvector* bv) {
 // equal group counts; `groups` > 0).
 int launch_word_op(fbh_bitvector* a, fbh_bitvector* b, int op, bool c2, uint32_t groups,
                    OutBuf& ob, uint32_t** ctl_out) {
+    if (groups == 0 && (a->nw > 1 || b->nw > 1))
+        return fail(FBH_ERR_INTERNAL, "operand sizes not counted before the word op");
     const bool la = a->nw == groups;
     const bool lb = b->nw == groups;
     int rc;
@@ -328,12 +330,14 @@ int launch_word_op(fbh_bitvector* a, fbh_bitvector* b, int op, bool c2, uint32_t
                                     ob.d, ob.ck, (uint64_t*)(sc + 64), *ctl_out, stream()), "op");
     }
     const uint32_t nt = fbk::canon_tiles(groups);
-    uint8_t* sc = (uint8_t*)scratch(0).get(64 + 8ull * nt + 64);
+    // [ctl 64 B][status 8 B per tile][tile starts 2 x (nt + 1) x 8 B]
+    const size_t ts_off = (64 + 8ull * nt + 15) & ~size_t(15);
+    uint8_t* sc = (uint8_t*)scratch(0).get(ts_off + 16ull * (nt + 1) + 64);
     if (!sc) return fail(FBH_ERR_NOMEM, "scratch allocation failed");
     *ctl_out = (uint32_t*)sc;
     return check(fbk::launch_canon(a->d, a->ck, a->nw, la ? 1 : 0, b->d, b->ck, b->nw, lb ? 1 : 0,
                                    groups, op, ob.d, ob.ck, (uint64_t*)(sc + 64), *ctl_out,
-                                   stream()), "canonical op");
+                                   (uint2*)(sc + ts_off), stream()), "canonical op");
 }
 
 int run_word_op(const fbh_bitvector* a, const fbh_bitvector* b, int op, bool c2,
@@ -636,7 +640,8 @@ int fbh_bv_flip(fbh_bitvector* bv) {
     auto fl = [](uint32_t w) { return w > ALLONES ? (w ^ FILLBIT) : (w ^ ALLONES); };
     if (bv->nw) { bv->w0 = fl(bv->w0); bv->wl = fl(bv->wl); }
     if (bv->nbits == 0 && bv->nw > 0) {
-        uint32_t* ctl = (uint32_t*)scratch(2).get(64);
+        uint32_t* ctl = (uint32_t*)scratch(2).get(4ull * fbk::cnt_scratch_words());
+        if (!ctl) return fail(FBH_ERR_NOMEM, "scratch allocation failed");
         rc = check(fbk::launch_cnt(bv->d, bv->nw, ctl, stream()), "flip count");
         if (rc) return rc;
         uint32_t h[2];
@@ -741,6 +746,8 @@ int fbh_bench_op(const fbh_bitvector* a, const fbh_bitvector* b, int op, int rep
     }
     fbh_bitvector* ma = const_cast<fbh_bitvector*>(a);
     fbh_bitvector* mb = const_cast<fbh_bitvector*>(b);
+    uint32_t sa = 0, sb = 0;     // nbits is computed lazily (do_cnt)
+    if ((rc = bv_size(ma, &sa)) || (rc = bv_size(mb, &sb))) return rc;
     const uint32_t groups = ma->nbits / MAXBITS;
     const bool c2 = path == FBH_PATH_C2;
     OutBuf ob;
@@ -779,6 +786,12 @@ int fbh_bench_op_rot(fbh_bitvector* const* a, fbh_bitvector* const* b, int npair
         fbh_bv_free(ta);
         if (rc) return rc;
         if (paths[i] == FBH_PATH_SHORTCUT) { if (ms) *ms = 0.f; return FBH_OK; }
+    }
+    for (int i = 0; i < npairs; ++i) {
+        uint32_t sa = 0, sb = 0;     // nbits is computed lazily (do_cnt)
+        int rc = bv_size(a[i], &sa);
+        if (!rc) rc = bv_size(b[i], &sb);
+        if (rc) return rc;
     }
     std::vector<OutBuf> obs(npairs);
     for (int i = 0; i < npairs; ++i) {

@@ -30,6 +30,7 @@ struct Operand {
     const uint32_t* ck;     // chunk index (ignored for literal operands)
     uint32_t n;             // code words
     uint32_t literal;       // one word per group
+    const uint2* ts;        // tile starts: {word containing group t*CT, its start group}
 };
 
 struct CanonArgs {
@@ -39,7 +40,7 @@ struct CanonArgs {
     word_t* out;
     uint32_t* outck;
     uint64_t* status;
-    uint32_t* ctl;          // [0] ticket [1] words [2] ones [3] groups
+    uint32_t* ctl;          // [0] ticket [1] words [3] groups [4],[5] edge words
 };
 
 __global__ void k_zero_u32_c(uint32_t* p, uint32_t n) {
@@ -50,52 +51,88 @@ __global__ void k_zero_u64_c(uint64_t* p, uint32_t n) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0ull;
 }
 
-__device__ __forceinline__ uint32_t cls3(word_t v) { return v == 0 ? 1u : (v == ALLONES ? 2u : 3u); }
-
-// Value of group g of an operand (global lookups; used for tile edges).
-__device__ word_t value_at(const Operand& o, uint32_t g) {
-    if (o.literal) return o.w[g];
-    const uint32_t nc = (o.n + CHUNKW - 1) / CHUNKW;
-    const uint32_t c = last_le(o.ck, nc, g);
-    uint32_t i = c * CHUNKW, pos = o.ck[c];
-    for (;;) {
-        const word_t x = o.w[i];
-        const uint32_t l = word_len(x);
-        if (pos + l > g) return group_value(x);
-        pos += l; ++i;
+// Tile starts of a compressed operand from its chunk index: the thread of
+// chunk j (code words [32j, 32j+32), groups [ck[j], ck[j+1])) resolves the
+// tiles whose first group falls in its span by walking those 32 words.
+// ts[nt] = {n, groups} closes the table.
+__global__ __launch_bounds__(256) void k_tile_starts(const word_t* __restrict__ w, const uint32_t* __restrict__ ck,
+                                                     uint32_t n, uint32_t nt, uint2* __restrict__ ts) {
+    const uint32_t nc = (n + CHUNKW - 1) / CHUNKW;
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > nc) return;
+    if (j == nc) { ts[nt] = make_uint2(n, ck[nc]); return; }
+    const uint32_t c0 = ck[j], c1 = ck[j + 1];
+    uint32_t t = (c0 + CT - 1) / CT;                      // first tile start >= c0
+    if ((uint64_t)t * CT >= c1 || t >= nt) return;
+    uint32_t i = j * CHUNKW, pos = c0;
+    const uint32_t e = min(i + CHUNKW, n);
+    for (; i < e && t < nt; ++i) {
+        const uint32_t l = word_len(w[i]);
+        while (t < nt && (uint64_t)t * CT < (uint64_t)pos + l) {
+            ts[t] = make_uint2(i, pos);
+            ++t;
+        }
+        pos += l;
     }
 }
 
-// Expands operand `o` over groups [g0, g0+cnt) into vals (combining with the
-// existing values by `op` when `combine` is set).
-__device__ void expand(const Operand& o, uint32_t g0, uint32_t cnt, word_t* vals, word_t* sw,
-                       uint32_t* sp, uint32_t* shs, uint32_t* info, bool combine, int op) {
+__device__ __forceinline__ uint32_t cls3(word_t v) { return v == 0 ? 1u : (v == ALLONES ? 2u : 3u); }
+
+// Expands operand `o` over the tile's groups [g0, g0+cnt) into vals,
+// combining with the values there by `op` when `combine` is set, and
+// returns (to thread 0) the operand's values of groups g0-1 and g0+cnt
+// (0xFFFFFFFF where outside the bitvector).
+__device__ void expand(const Operand& o, uint32_t tile, uint32_t g0, uint32_t cnt, uint32_t groups,
+                       word_t* vals, word_t* sw, uint32_t* sp, uint32_t* shs, bool combine, int op,
+                       word_t& vprev, word_t& vnext) {
     const int tid = threadIdx.x;
+    const uint32_t g1 = g0 + cnt;
     if (o.literal) {
-        for (uint32_t k = tid; k < cnt; k += CT_TPB) {
-            const word_t x = o.w[g0 + k];
-            vals[k] = combine ? apply_op(vals[k], x, op) : x;
+        // 16 independent coalesced loads per thread, then LDS
+        word_t x[CT_ITEMS];
+        #pragma unroll
+        for (int r = 0; r < CT_ITEMS; ++r) {
+            const uint32_t k = (uint32_t)r * CT_TPB + tid;
+            x[r] = k < cnt ? o.w[g0 + k] : 0u;
+        }
+        #pragma unroll
+        for (int r = 0; r < CT_ITEMS; ++r) {
+            const uint32_t k = (uint32_t)r * CT_TPB + tid;
+            if (k < cnt) vals[k] = combine ? apply_op(vals[k], x[r], op) : x[r];
+        }
+        if (tid == 0) {
+            vprev = g0 > 0 ? o.w[g0 - 1] : 0xFFFFFFFFu;
+            vnext = g1 < groups ? o.w[g1] : 0xFFFFFFFFu;
         }
         __syncthreads();
         return;
     }
-    if (tid == 0) {
-        const uint32_t nc = (o.n + CHUNKW - 1) / CHUNKW;
-        const uint32_t c = last_le(o.ck, nc, g0);
-        uint32_t i = c * CHUNKW, pos = o.ck[c];
-        for (;;) {
-            const uint32_t l = word_len(o.w[i]);
-            if (pos + l > g0) break;
-            pos += l; ++i;
+    const uint2 st = o.ts[tile];
+    const uint32_t i0 = st.x, p0 = st.y;
+    // words covering the tile: i0 .. the word holding group g1 - 1
+    const uint32_t ilast = min(o.ts[tile + 1].x, o.n - 1u);
+    const uint32_t nw = ilast + 1u - i0;                   // <= CT + 1
+    constexpr int IT = (CT + 1 + CT_TPB - 1) / CT_TPB;
+    {
+        word_t x[IT];
+        #pragma unroll
+        for (int r = 0; r < IT; ++r) {
+            const uint32_t k = (uint32_t)r * CT_TPB + tid;
+            x[r] = k < nw ? o.w[i0 + k] : 0u;
         }
-        info[0] = i; info[1] = pos;
+        #pragma unroll
+        for (int r = 0; r < IT; ++r) {
+            const uint32_t k = (uint32_t)r * CT_TPB + tid;
+            if (k < nw) sw[k] = x[r];
+        }
+    }
+    if (tid == 0) {
+        // group g0 - 1: in word i0 when it started before g0, else the word before
+        vprev = g0 == 0 ? 0xFFFFFFFFu : (p0 < g0 ? 0u : group_value(o.w[i0 - 1]));
+        vnext = g1 < groups ? group_value(o.w[o.ts[tile + 1].x]) : 0xFFFFFFFFu;
     }
     __syncthreads();
-    const uint32_t i0 = info[0], p0 = info[1];
-    const uint32_t nw = min((uint32_t)CT + 1u, o.n - i0);
-    for (uint32_t k = tid; k < nw; k += CT_TPB) sw[k] = o.w[i0 + k];
-    __syncthreads();
-    constexpr int IT = (CT + 1 + CT_TPB - 1) / CT_TPB;
+    if (tid == 0 && g0 > 0 && p0 < g0) vprev = group_value(sw[0]);
     uint32_t len[IT];
     uint32_t s = 0;
     #pragma unroll
@@ -134,7 +171,6 @@ __global__ __launch_bounds__(CT_TPB) void k_canon_tiles(CanonArgs p) {
     __shared__ word_t s_sw[CT + 1];         // staged code words, then output staging
     __shared__ uint32_t s_sp[CT + 2];
     __shared__ uint32_t s_shs[CT_TPB / WAVE + 1];
-    __shared__ uint32_t s_info[8];
     __shared__ uint32_t s_wc[CT_TPB / WAVE], s_wh[CT_TPB / WAVE];
     __shared__ uint32_t s_tile[4];
 
@@ -146,20 +182,17 @@ __global__ __launch_bounds__(CT_TPB) void k_canon_tiles(CanonArgs p) {
     const uint32_t cnt = g0 < p.groups ? min((uint32_t)CT, p.groups - g0) : 0u;
     word_t* vals = s_vals + 1;
 
+    word_t vpa = 0xFFFFFFFFu, vna = 0xFFFFFFFFu, vpb = 0xFFFFFFFFu, vnb = 0xFFFFFFFFu;
     if (cnt) {
-        expand(p.a, g0, cnt, vals, s_sw, s_sp, s_shs, s_info, false, p.op);
-        if (p.b.w) expand(p.b, g0, cnt, vals, s_sw, s_sp, s_shs, s_info, true, p.op);
+        expand(p.a, tile, g0, cnt, p.groups, vals, s_sw, s_sp, s_shs, false, p.op, vpa, vna);
+        if (p.b.w) expand(p.b, tile, g0, cnt, p.groups, vals, s_sw, s_sp, s_shs, true, p.op, vpb, vnb);
     }
     if (tid == 0) {
-        // neighbours of the tile (class 0 = none)
-        word_t vp = 0xFFFFFFFFu, vn = 0xFFFFFFFFu;
-        if (cnt && g0 > 0) {
-            vp = value_at(p.a, g0 - 1);
-            if (p.b.w) vp = apply_op(vp, value_at(p.b, g0 - 1), p.op);
-        }
-        if (cnt && g0 + cnt < p.groups) {
-            vn = value_at(p.a, g0 + cnt);
-            if (p.b.w) vn = apply_op(vn, value_at(p.b, g0 + cnt), p.op);
+        // neighbours of the tile (0xFFFFFFFF = none)
+        word_t vp = vpa, vn = vna;
+        if (p.b.w) {
+            if (vp != 0xFFFFFFFFu) vp = apply_op(vpa, vpb, p.op);
+            if (vn != 0xFFFFFFFFu) vn = apply_op(vna, vnb, p.op);
         }
         s_vals[0] = vp;
         vals[cnt] = vn;          // slot right after the last group of the tile
@@ -215,7 +248,6 @@ __global__ __launch_bounds__(CT_TPB) void k_canon_tiles(CanonArgs p) {
     // write pass: words with slot >= base_cnt go to LDS staging, a run that
     // began in an earlier tile is written straight to global memory
     word_t* stage = s_sw;
-    uint32_t ones = 0;
     #pragma unroll
     for (int q = 0; q < CT_ITEMS; ++q) {
         const uint32_t k = k0 + q;
@@ -246,8 +278,6 @@ __global__ __launch_bounds__(CT_TPB) void k_canon_tiles(CanonArgs p) {
                 if (slot >= base_cnt) stage[slot - base_cnt] = word;
                 else p.out[slot] = word;
                 if ((slot & (CHUNKW - 1)) == 0) p.outck[slot / CHUNKW] = headpos;
-                ones += (word >= HEADER1) ? (word & MAXCNT) * 31u
-                                          : (word < HEADER0 ? (uint32_t)__popc(word) : 0u);
             }
             if (g + 1 == p.groups) {
                 p.ctl[1] = run_cnt;
@@ -268,19 +298,25 @@ __global__ __launch_bounds__(CT_TPB) void k_canon_tiles(CanonArgs p) {
         const uint32_t nstage = (open && tc > 0u) ? tc - 1u : tc;
         for (uint32_t k = tid; k < nstage; k += CT_TPB) p.out[base_cnt + k] = stage[k];
     }
-    ones = wave_sum(ones);
-    if (lane == 0 && ones) atomicAdd(&p.ctl[2], ones);
 }
 
 hipError_t launch_canon(const word_t* a, const uint32_t* cka, uint32_t na, int alit,
                         const word_t* b, const uint32_t* ckb, uint32_t nb, int blit,
                         uint32_t groups, int op, word_t* out, uint32_t* outck, uint64_t* status,
-                        uint32_t* ctl, hipStream_t s) {
+                        uint32_t* ctl, uint2* ts, hipStream_t s) {
     CanonArgs p;
-    p.a = Operand{a, cka, na, (uint32_t)(alit ? 1 : 0)};
-    p.b = Operand{b, ckb, nb, (uint32_t)(blit ? 1 : 0)};
-    p.groups = groups; p.op = op; p.out = out; p.outck = outck; p.status = status; p.ctl = ctl;
     const uint32_t nt = groups ? (groups + CT - 1) / CT : 1u;
+    p.a = Operand{a, cka, na, (uint32_t)(alit ? 1 : 0), ts};
+    p.b = Operand{b, ckb, nb, (uint32_t)(blit ? 1 : 0), ts + (nt + 1)};
+    p.groups = groups; p.op = op; p.out = out; p.outck = outck; p.status = status; p.ctl = ctl;
+    if (!alit && na > 0) {
+        const uint32_t nc = (na + CHUNKW - 1) / CHUNKW;
+        hipLaunchKernelGGL(k_tile_starts, dim3(nc / 256 + 1), dim3(256), 0, s, a, cka, na, nt, ts);
+    }
+    if (b && !blit && nb > 0) {
+        const uint32_t nc = (nb + CHUNKW - 1) / CHUNKW;
+        hipLaunchKernelGGL(k_tile_starts, dim3(nc / 256 + 1), dim3(256), 0, s, b, ckb, nb, nt, ts + (nt + 1));
+    }
     hipLaunchKernelGGL(k_zero_u64_c, dim3(min((nt + 255) / 256, 4096u)), dim3(256), 0, s, status, nt);
     hipLaunchKernelGGL(k_zero_u32_c, dim3(1), dim3(64), 0, s, ctl, 8u);
     hipLaunchKernelGGL(k_canon_tiles, dim3(nt), dim3(CT_TPB), 0, s, p);

@@ -1,5 +1,5 @@
 // Device-side helpers shared by the gfx950 kernels: wave/block scans built
-// on DPP row shifts and broadcasts (64-lane wavefronts), the single-pass
+// on DPP row shifts and broadcasts (64-lane wavefronts), the wave-parallel
 // look-back between tiles, and small search routines.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -60,60 +60,14 @@ __device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t* sh, uin
     return r;
 }
 
-// Block-wide exclusive scan of run summaries (LDS, Hillis-Steele).  `buf`
-// needs 2*NT Summary slots.  Returns the exclusive prefix of the calling
-// thread and writes the block aggregate to *agg.
-template <int NT>
-__device__ __forceinline__ wah::Summary block_excl_summary(const wah::Summary& v, wah::Summary* buf,
-                                                           wah::Summary* agg) {
-    const int t = threadIdx.x;
-    wah::Summary* in = buf;
-    wah::Summary* out = buf + NT;
-    in[t] = v;
-    __syncthreads();
-    for (int d = 1; d < NT; d <<= 1) {
-        wah::Summary x = in[t];
-        if (t >= d) x = wah::combine(in[t - d], x);
-        out[t] = x;
-        __syncthreads();
-        wah::Summary* tmp = in; in = out; out = tmp;
-    }
-    const wah::Summary excl = (t == 0) ? wah::summary_empty() : in[t - 1];
-    *agg = in[NT - 1];
-    __syncthreads();
-    return excl;
-}
-
-// Single-pass look-back between tiles (tile ids handed out in launch order
-// by an atomic ticket, so every predecessor is resident).  Called by one
-// thread.  Publishes this tile's aggregate, walks back over predecessors
-// until an inclusive prefix is found, publishes the inclusive prefix and
-// returns the exclusive one.
-__device__ __forceinline__ wah::Summary lookback(uint64_t* status, uint32_t tile,
-                                                 const wah::Summary& agg) {
-    if (tile == 0) {
-        __hip_atomic_store(&status[0], wah::pack(agg, 2), __ATOMIC_RELEASE,
-                           __HIP_MEMORY_SCOPE_AGENT);
-        return wah::summary_empty();
-    }
-    __hip_atomic_store(&status[tile], wah::pack(agg, 1), __ATOMIC_RELEASE,
-                       __HIP_MEMORY_SCOPE_AGENT);
-    wah::Summary acc = wah::summary_empty();
-    int64_t j = (int64_t)tile - 1;
-    while (j >= 0) {
-        uint64_t s;
-        do {
-            s = __hip_atomic_load(&status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-            if (wah::status_of(s) == 0) __builtin_amdgcn_s_sleep(1);
-        } while (wah::status_of(s) == 0);
-        acc = wah::combine(wah::unpack(s), acc);
-        if (wah::status_of(s) == 2) break;
-        --j;
-    }
-    __hip_atomic_store(&status[tile], wah::pack(wah::combine(acc, agg), 2), __ATOMIC_RELEASE,
-                       __HIP_MEMORY_SCOPE_AGENT);
-    return acc;
-}
+// Look-back between tiles.  Tile ids are handed out in launch order by an
+// atomic ticket, so every predecessor of a tile is resident or done.  A
+// tile publishes its aggregate (status 1), reads its predecessors 64 at a
+// time until an inclusive prefix (status 2) closes the chain, then
+// publishes its own inclusive prefix.  Every value a tile needs from
+// another travels inside the 64-bit status word, so the loads and stores
+// are relaxed agent-scope atomics: release / acquire would write back and
+// invalidate the XCD's whole L2 on every step.
 
 // Largest index i in [0, n) with a[i] <= key (a sorted, a[0] <= key).
 template <class T>
@@ -212,16 +166,16 @@ __device__ __forceinline__ wah::Summary lookback_summary_wave(uint64_t* status, 
     const int lane = (int)lane_id();
     if (tile == 0) {
         if (lane == 0)
-            __hip_atomic_store(&status[0], wah::pack(agg, 2), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&status[0], wah::pack(agg, 2), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return wah::summary_empty();
     }
     if (lane == 0)
-        __hip_atomic_store(&status[tile], wah::pack(agg, 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&status[tile], wah::pack(agg, 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     wah::Summary acc = wah::summary_empty();
     int64_t base = (int64_t)tile - 1;
     for (;;) {
         const int64_t j = base - lane;
-        uint64_t s = (j >= 0) ? __hip_atomic_load(&status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)
+        uint64_t s = (j >= 0) ? __hip_atomic_load(&status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
                               : wah::pack(wah::summary_empty(), 2);
         const uint32_t st = wah::status_of(s);
         const uint64_t zero = ballot64(st == 0);
@@ -241,7 +195,7 @@ __device__ __forceinline__ wah::Summary lookback_summary_wave(uint64_t* status, 
         base -= 64;
     }
     if (lane == 0)
-        __hip_atomic_store(&status[tile], wah::pack(wah::combine(acc, agg), 2), __ATOMIC_RELEASE,
+        __hip_atomic_store(&status[tile], wah::pack(wah::combine(acc, agg), 2), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
     return acc;
 }
@@ -259,17 +213,17 @@ __device__ __forceinline__ void lookback_ch_wave(uint64_t* status, uint32_t tile
     const int lane = (int)lane_id();
     if (tile == 0) {
         if (lane == 0)
-            __hip_atomic_store(&status[0], pack_ch(acnt, alh1, 2), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&status[0], pack_ch(acnt, alh1, 2), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         ecnt = 0; elh1 = 0;
         return;
     }
     if (lane == 0)
-        __hip_atomic_store(&status[tile], pack_ch(acnt, alh1, 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&status[tile], pack_ch(acnt, alh1, 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint32_t c = 0, h = 0;
     int64_t base = (int64_t)tile - 1;
     for (;;) {
         const int64_t j = base - lane;
-        uint64_t s = (j >= 0) ? __hip_atomic_load(&status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)
+        uint64_t s = (j >= 0) ? __hip_atomic_load(&status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
                               : pack_ch(0, 0, 2);
         const uint32_t st = (uint32_t)(s >> 62);
         const uint64_t zero = ballot64(st == 0);
@@ -285,7 +239,7 @@ __device__ __forceinline__ void lookback_ch_wave(uint64_t* status, uint32_t tile
     }
     ecnt = c; elh1 = h;
     if (lane == 0)
-        __hip_atomic_store(&status[tile], pack_ch(c + acnt, h > alh1 ? h : alh1, 2), __ATOMIC_RELEASE,
+        __hip_atomic_store(&status[tile], pack_ch(c + acnt, h > alh1 ? h : alh1, 2), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
 }
 

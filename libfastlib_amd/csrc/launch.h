@@ -25,7 +25,9 @@ uint32_t tok_tiles(uint32_t n);
 hipError_t launch_tok(const word_t* a, const word_t* b, uint32_t n, uint32_t groups, int op,
                       int mode, word_t* out, uint32_t* outck, uint64_t* status, uint32_t* ctl,
                       hipStream_t s);
+// out2[0] = ones, out2[1] = groups; out2 needs cnt_scratch_words() words.
 hipError_t launch_cnt(const word_t* w, uint32_t n, uint32_t* out2, hipStream_t s);
+uint32_t cnt_scratch_words();
 hipError_t launch_decompress(const word_t* w, const uint32_t* ck, uint32_t n, uint32_t groups,
                              word_t* out, int neg, hipStream_t s);
 hipError_t launch_flip(word_t* w, uint32_t n, hipStream_t s);
@@ -33,10 +35,11 @@ hipError_t launch_rids_lit(const word_t* lit, uint32_t n, uint32_t* out, uint32_
                            uint32_t* tmp, hipStream_t s);
 // Canonical WAH of a OP b (b == nullptr: of a alone) group by group; an
 // operand with lit != 0 holds one literal word per group.  status needs
-// canon_tiles(groups) entries; ctl 8 words.
+// canon_tiles(groups) entries; ctl 8 words; ts 2 * (canon_tiles(groups) + 1)
+// entries (tile starts of the compressed operands).
 uint32_t canon_tiles(uint32_t groups);
 hipError_t launch_canon(const word_t* a, const uint32_t* cka, uint32_t na, int alit,
                         const word_t* b, const uint32_t* ckb, uint32_t nb, int blit,
                         uint32_t groups, int op, word_t* out, uint32_t* outck, uint64_t* status,
-                        uint32_t* ctl, hipStream_t s);
+                        uint32_t* ctl, uint2* ts, hipStream_t s);
 } // namespace fbk

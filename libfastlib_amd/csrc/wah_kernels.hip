@@ -103,7 +103,7 @@ struct OpArgs {
     uint32_t* outck;
     uint64_t* status;
     uint32_t* ticket;
-    uint32_t* total;      // [0] output words, [1] ones in the output words
+    uint32_t* total;      // [0] output words, [3] groups represented
 };
 
 // Position of merged chunk-index item d (A items before B items on ties).
@@ -125,11 +125,9 @@ __device__ uint32_t diag_pos(const OpArgs& p, uint32_t d) {
 struct GlobWriter {
     word_t* out;
     uint32_t* outck;
-    uint32_t ones;
     __device__ __forceinline__ void operator()(uint32_t slot, word_t w, uint32_t head) {
         out[slot] = w;
         if ((slot & (CHUNK - 1)) == 0) outck[slot / CHUNK] = head;
-        ones += (w >= HEADER1) ? (w & MAXCNT) * 31u : (w < HEADER0 ? (uint32_t)__popc(w) : 0u);
     }
 };
 
@@ -294,7 +292,7 @@ __global__ __launch_bounds__(TPB) void k_op_tiles(OpArgs p) {
     const Summary pre = combine(s_pre, texcl);
 
     // write pass
-    GlobWriter gw{p.out, p.outck, 0u};
+    GlobWriter gw{p.out, p.outck};
     if (c0 < c1) {
         WriteSink<GlobWriter> ws;
         ws.init(&gw, pre, c0);
@@ -306,10 +304,6 @@ __global__ __launch_bounds__(TPB) void k_op_tiles(OpArgs p) {
             p.outck[(total + CHUNK - 1) / CHUNK] = p.groups;
         }
     }
-    // ones in the written words (for cnt caching)
-    uint32_t ones = gw.ones;
-    ones = wave_sum(ones);
-    if (lane_id() == 0 && ones) atomicAdd(&p.total[1], ones);
 }
 
 // ---------------------------------------------------------------------------
@@ -382,7 +376,7 @@ __global__ __launch_bounds__(TPB) void k_tok_tiles(TokArgs p) {
     }
     __syncthreads();
     const Summary pre = combine(s_pre, texcl);
-    GlobWriter gw{p.out, p.outck, 0u};
+    GlobWriter gw{p.out, p.outck};
     WriteSink<GlobWriter> ws;
     ws.init(&gw, pre, s_base_pos + lpos);
     #pragma unroll 4
@@ -398,25 +392,63 @@ __global__ __launch_bounds__(TPB) void k_tok_tiles(TokArgs p) {
         p.total[3] = ws.pos;                 // groups represented
         p.outck[(total + CHUNK - 1) / CHUNK] = ws.pos;
     }
-    uint32_t ones = wave_sum(gw.ones);
-    if (lane_id() == 0 && ones) atomicAdd(&p.total[1], ones);
 }
 
 // ---------------------------------------------------------------------------
-// cnt: ones and groups of a word array.  out[0] += ones, out[1] += groups.
+// cnt / do_cnt in two passes without atomics: every block reduces a
+// strided share of the words to (ones, groups) partials, one block sums them.
+constexpr uint32_t CNT_BLOCKS = 512;
+
 __global__ __launch_bounds__(TPB) void k_cnt(const word_t* __restrict__ w, uint32_t n,
-                                             uint32_t* out) {
+                                             uint32_t* __restrict__ part) {
+    __shared__ uint32_t so[TPB / WAVE], sg[TPB / WAVE];
     uint32_t ones = 0, groups = 0;
-    for (uint32_t i = blockIdx.x * TPB + threadIdx.x; i < n; i += gridDim.x * TPB) {
+    const uint32_t stride = gridDim.x * TPB;
+    uint32_t i = blockIdx.x * TPB + threadIdx.x;
+    // four independent loads in flight per thread
+    for (; i + 3u * stride < n; i += 4u * stride) {
+        word_t x[4];
+        #pragma unroll
+        for (int r = 0; r < 4; ++r) x[r] = w[i + (uint32_t)r * stride];
+        #pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            if (x[r] < HEADER0) { ones += __popc(x[r]); groups += 1; }
+            else { const uint32_t c = x[r] & MAXCNT; groups += c; ones += (x[r] >= HEADER1) ? c * 31u : 0u; }
+        }
+    }
+    for (; i < n; i += stride) {
         const word_t x = w[i];
         if (x < HEADER0) { ones += __popc(x); groups += 1; }
         else { const uint32_t c = x & MAXCNT; groups += c; ones += (x >= HEADER1) ? c * 31u : 0u; }
     }
     ones = wave_sum(ones);
     groups = wave_sum(groups);
-    if (lane_id() == 0) {
-        if (ones) atomicAdd(&out[0], ones);
-        if (groups) atomicAdd(&out[1], groups);
+    const uint32_t wv = threadIdx.x / WAVE;
+    if (lane_id() == 0) { so[wv] = ones; sg[wv] = groups; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t a = 0, b = 0;
+        for (int k = 0; k < TPB / WAVE; ++k) { a += so[k]; b += sg[k]; }
+        part[2 * blockIdx.x] = a;
+        part[2 * blockIdx.x + 1] = b;
+    }
+}
+
+__global__ __launch_bounds__(TPB) void k_cnt_final(const uint32_t* __restrict__ part, uint32_t nb,
+                                                   uint32_t* __restrict__ out) {
+    __shared__ uint32_t so[TPB / WAVE], sg[TPB / WAVE];
+    uint32_t a = 0, b = 0;
+    for (uint32_t k = threadIdx.x; k < nb; k += TPB) { a += part[2 * k]; b += part[2 * k + 1]; }
+    a = wave_sum(a);
+    b = wave_sum(b);
+    const uint32_t wv = threadIdx.x / WAVE;
+    if (lane_id() == 0) { so[wv] = a; sg[wv] = b; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t x = 0, y = 0;
+        for (int k = 0; k < TPB / WAVE; ++k) { x += so[k]; y += sg[k]; }
+        out[0] = x;
+        out[1] = y;
     }
 }
 
@@ -637,13 +669,14 @@ hipError_t launch_tok(const word_t* a, const word_t* b, uint32_t n, uint32_t gro
 }
 
 hipError_t launch_cnt(const word_t* w, uint32_t n, uint32_t* out2, hipStream_t s) {
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(TPB), 0, s, out2, 2u);
-    if (n) {
-        const uint32_t g = min(cdiv(n, TPB * 8), 2048u);
-        hipLaunchKernelGGL(k_cnt, dim3(g), dim3(TPB), 0, s, w, n, out2);
-    }
+    uint32_t* part = out2 + 2;
+    const uint32_t g = n ? min(cdiv(n, TPB * 4), CNT_BLOCKS) : 0u;
+    if (g) hipLaunchKernelGGL(k_cnt, dim3(g), dim3(TPB), 0, s, w, n, part);
+    hipLaunchKernelGGL(k_cnt_final, dim3(1), dim3(TPB), 0, s, part, g, out2);
     return hipGetLastError();
 }
+
+uint32_t cnt_scratch_words() { return 2u + 2u * CNT_BLOCKS; }
 
 hipError_t launch_decompress(const word_t* w, const uint32_t* ck, uint32_t n, uint32_t groups,
                              word_t* out, int neg, hipStream_t s) {
