@@ -97,8 +97,10 @@ struct fbh_index {
     uint64_t nwords = 0;
     uint64_t* doff = nullptr;
     uint32_t* dnw = nullptr;
-    uint2* toff = nullptr;
     uint32_t ntiles = 0;
+    uint32_t* tw = nullptr;       // tiled copy of the bitmaps (literal plan)
+    uint32_t* TW = nullptr;       // segment offsets, ntiles*nobs + 1
+    uint64_t ntw = 0;
     void* col = nullptr;          // raw column (device)
     double* dbounds = nullptr;    // bin bounds (device)
     std::vector<double> cumbytes; // prefix sums of bytes(b)
@@ -107,7 +109,7 @@ struct fbh_index {
     int cost_model = FBH_COST_IAPI;
     double cmin = DBL_MAX, cmax = -DBL_MAX;   // column min / max
     ~fbh_index() {
-        dev_free(words); dev_free(doff); dev_free(dnw); dev_free(toff); dev_free(col);
+        dev_free(words); dev_free(doff); dev_free(dnw); dev_free(tw); dev_free(TW); dev_free(col);
         dev_free(dbounds);
     }
     bool null_bitmap(uint32_t b) const { return off[b + 1] == off[b]; }
@@ -292,11 +294,32 @@ int finish_index(fbh_index* ix) {
         if (!ix->dbounds) return fail(FBH_ERR_NOMEM, "device allocation failed");
         if ((rc = to_dev(ix->dbounds, ix->keys.data(), ix->nobs))) return rc;
     }
+    // the tile offset table holds absolute 32-bit word indices
+    if (ix->nwords >= 0xFFFFFFFFull) return fail(FBH_ERR_SIZE, "index larger than 2^32 words");
     ix->ntiles = ix->groups / (uint32_t)fbk::QT + 1;
-    ix->toff = (uint2*)dev_alloc(sizeof(uint2) * (size_t)ix->ntiles * (ix->nobs ? ix->nobs : 1));
-    if (!ix->toff) return fail(FBH_ERR_NOMEM, "device allocation failed");
+    // tile offset table, then the tiled copy of the bitmaps built from it
+    const size_t nseg = (size_t)ix->ntiles * ix->nobs;
+    if (nseg + 1 >= 0xFFFFFFFFull) return fail(FBH_ERR_SIZE, "too many tile segments");
+    DBuf<uint2> toff;
+    DBuf<uint32_t> len, tmp;
+    if ((rc = toff.alloc((size_t)(ix->ntiles + 1) * (ix->nobs ? ix->nobs : 1))) ||
+        (rc = len.alloc(nseg + 1)) || (rc = tmp.alloc(fbk::scan_tmp_words((uint32_t)nseg + 1) + 4)))
+        return rc;
+    ix->TW = (uint32_t*)dev_alloc(4ull * (nseg + 1));
+    if (!ix->TW) return fail(FBH_ERR_NOMEM, "device allocation failed");
     rc = check(fbk::launch_toff(ix->words, ix->doff, ix->dnw, ix->nobs, ix->ntiles, (uint32_t)fbk::QT,
-                                ix->toff, stream()), "tile offsets");
+                                ix->groups, toff.p, stream()), "tile offsets");
+    if (!rc) rc = check(fbk::launch_tseg_len(toff.p, ix->nobs, ix->ntiles, ix->groups, ix->anb, len.p,
+                                             stream()), "segment lengths");
+    if (!rc) rc = check(fbk::launch_scan_excl(len.p, ix->TW, (uint32_t)nseg + 1, tmp.p, stream()), "segment scan");
+    uint32_t total = 0;
+    if (!rc) rc = to_host(&total, ix->TW + nseg, 1);
+    if (rc) return rc;
+    ix->ntw = total;
+    ix->tw = (uint32_t*)dev_alloc(4ull * (total ? total : 1));
+    if (!ix->tw) return fail(FBH_ERR_NOMEM, "device allocation failed");
+    rc = check(fbk::launch_tseg_fill(ix->words, toff.p, ix->TW, ix->nobs, ix->ntiles, ix->groups, ix->anb,
+                                     ix->tw, stream()), "tiled copy");
     if (rc) return rc;
     return sync();
 }
@@ -408,7 +431,7 @@ SumPlan sum_plan(const fbh_index* ix, uint32_t ib, uint32_t ie) {
 
 void fill_query(const fbh_index* ix, fbk::QueryArgs& q, double lo, int lop, double hi, int rop) {
     memset(&q, 0, sizeof(q));
-    q.words = ix->words; q.off = ix->doff; q.nw = ix->dnw; q.toff = ix->toff;
+    q.tw = ix->tw; q.TW = ix->TW; q.ntiles = ix->ntiles;
     q.nobs = ix->nobs; q.groups = ix->groups; q.anb = ix->anb;
     q.cand0 = fbk::NOKEY; q.cand1 = fbk::NOKEY;
     q.col = ix->col;
@@ -423,21 +446,45 @@ void fill_query(const fbh_index* ix, fbk::QueryArgs& q, double lo, int lop, doub
     q.lop = l; q.rop = r; q.lo = lo; q.hi = hi; q.ilo = ilo; q.ihi = ihi;
 }
 
+// Per-block partial counts of evaluated queries (one row per query), summed
+// on the device by one block per query.
+struct PartCounts {
+    DBuf<uint32_t> parts, np, counts;
+    uint32_t stride = 0;
+    int alloc(const fbh_index* ix, uint32_t nq) {
+        stride = std::max(ix->ntiles, fbk::scan_blocks(ix->groups, ix->anb));
+        int rc = parts.alloc((size_t)stride * (nq ? nq : 1));
+        if (!rc) rc = np.alloc(nq ? nq : 1);
+        if (!rc) rc = counts.alloc(nq ? nq : 1);
+        return rc;
+    }
+    uint32_t* row(uint32_t i) { return parts.p + (size_t)i * stride; }
+    // queues the sums (host nparts must stay alive until the stream syncs)
+    int launch_sum(const uint32_t* nparts, uint32_t nq) {
+        int rc = to_dev(np.p, nparts, nq);
+        if (!rc) rc = check(fbk::launch_sum_parts(parts.p, stride, np.p, nq, counts.p, stream()), "sum parts");
+        return rc;
+    }
+    int sum(const uint32_t* nparts, uint32_t nq, uint32_t* out) {
+        int rc = launch_sum(nparts, nq);
+        return rc ? rc : to_host(out, counts.p, nq);
+    }
+};
+
 } // namespace
 
 namespace fbh_impl {
 
 // Literal result of a range query (groups + 1 words: the last one holds the
-// active bits left-aligned) and its ones count.  plan: FBH_PLAN_LITERAL or
-// FBH_PLAN_SCAN or FBH_PLAN_COMPRESSED.
+// active bits left-aligned) and its ones count as *nparts per-block partial
+// counts in part[].  plan: FBH_PLAN_LITERAL or FBH_PLAN_SCAN.
 int index_eval_literal(fbh_index* ix, double lo, int lop, double hi, int rop, int plan,
-                       uint32_t* out, uint32_t* dcount, hipStream_t s) {
+                       uint32_t* out, uint32_t* part, uint32_t* nparts, hipStream_t s) {
     fbk::QueryArgs q;
     fill_query(ix, q, lo, lop, hi, rop);
     q.out = out;
-    q.count = dcount;
-    int rc = check(fbk::launch_zero_u32(dcount, 1, s), "zero");
-    if (rc) return rc;
+    q.part = part;
+    *nparts = plan == FBH_PLAN_SCAN ? fbk::scan_blocks(ix->groups, ix->anb) : ix->ntiles;
     Locate L;
     locate(ix, lo, lop, hi, rop, L);
     if (plan == FBH_PLAN_SCAN) {
@@ -525,10 +572,9 @@ int eval_compressed(fbh_index* ix, double lo, int lop, double hi, int rop, fbh_b
         fill_query(ix, q, lo, lop, hi, rop);
         if (L.cand0 < L.hit0 && !ix->null_bitmap(L.cand0)) q.cand0 = L.cand0;
         if (L.cand1 > L.hit1 && L.hit1 < ix->nobs && !ix->null_bitmap(L.hit1)) q.cand1 = L.hit1;
-        DBuf<uint32_t> lit, cnt;
-        if ((rc = lit.alloc(ix->groups + 1)) || (rc = cnt.alloc(4))) return rc;
-        q.out = lit.p; q.count = cnt.p;
-        if ((rc = check(fbk::launch_zero_u32(cnt.p, 1, stream()), "zero"))) return rc;
+        DBuf<uint32_t> lit, part;
+        if ((rc = lit.alloc(ix->groups + 1)) || (rc = part.alloc(ix->ntiles))) return rc;
+        q.out = lit.p; q.part = part.p;
         if ((rc = check(fbk::launch_query_lit(q, dev_type(ix->type), stream()), "edge"))) return rc;
         fbh_bitvector delta;
         uint32_t act = 0;
@@ -715,12 +761,14 @@ int fbh_index_evaluate(fbh_index* ix, double lo, int lop, double hi, int rop, in
         if (result) *result = res; else fbh_bv_free(res);
         return FBH_OK;
     }
-    DBuf<uint32_t> lit, cnt;
-    if ((rc = lit.alloc(ix->groups + 1)) || (rc = cnt.alloc(4))) return rc;
-    rc = index_eval_literal(ix, lo, lop, hi, rop, plan, lit.p, cnt.p, stream());
-    if (rc) return rc;
+    DBuf<uint32_t> lit;
+    PartCounts pc;
+    if ((rc = lit.alloc(ix->groups + 1)) || (rc = pc.alloc(ix, 1))) return rc;
+    uint32_t np = 0;
+    rc = index_eval_literal(ix, lo, lop, hi, rop, plan, lit.p, pc.row(0), &np, stream());
     uint32_t c = 0;
-    if ((rc = to_host(&c, cnt.p, 1))) return rc;
+    if (!rc) rc = pc.sum(&np, 1, &c);
+    if (rc) return rc;
     if (hits) *hits = c;
     if (result) {
         // canonical WAH of the literal result
@@ -753,12 +801,14 @@ int fbh_index_evaluate_rids(fbh_index* ix, double lo, int lop, double hi, int ro
     int rc = ensure_init();
     if (rc) return rc;
     if (plan == FBH_PLAN_AUTO || plan == FBH_PLAN_COMPRESSED) plan = FBH_PLAN_LITERAL;
-    DBuf<uint32_t> lit, cnt;
-    if ((rc = lit.alloc(ix->groups + 1)) || (rc = cnt.alloc(4))) return rc;
-    rc = index_eval_literal(ix, lo, lop, hi, rop, plan, lit.p, cnt.p, stream());
-    if (rc) return rc;
+    DBuf<uint32_t> lit;
+    PartCounts pc;
+    if ((rc = lit.alloc(ix->groups + 1)) || (rc = pc.alloc(ix, 1))) return rc;
+    uint32_t np = 0;
+    rc = index_eval_literal(ix, lo, lop, hi, rop, plan, lit.p, pc.row(0), &np, stream());
     uint32_t c = 0;
-    if ((rc = to_host(&c, cnt.p, 1))) return rc;
+    if (!rc) rc = pc.sum(&np, 1, &c);
+    if (rc) return rc;
     *n = c;
     if (!rids || c == 0) return FBH_OK;
     // the active slot is one more literal group (rows groups*31 ..)
@@ -777,9 +827,10 @@ int fbh_index_evaluate_batch(fbh_index* ix, uint32_t nq, const double* lo, const
     if (!ix || !lo || !lop || !hi || !rop || !hits) return fail(FBH_ERR_ARG, "null argument");
     int rc = ensure_init();
     if (rc) return rc;
-    DBuf<uint32_t> lit, cnt;
-    if ((rc = lit.alloc(ix->groups + 1)) || (rc = cnt.alloc(nq ? nq : 1))) return rc;
-    if ((rc = check(fbk::launch_zero_u32(cnt.p, nq, stream()), "zero"))) return rc;
+    DBuf<uint32_t> lit;
+    PartCounts pc;
+    if ((rc = lit.alloc(ix->groups + 1)) || (rc = pc.alloc(ix, nq))) return rc;
+    std::vector<uint32_t> np(nq, 0);
     std::vector<int> used(nq);
     std::vector<uint64_t> sync_hits(nq, 0);
     hipEvent_t e0, e1;
@@ -798,17 +849,18 @@ int fbh_index_evaluate_batch(fbh_index* ix, uint32_t nq, const double* lo, const
             sync_hits[i] = c;
             fbh_bv_free(res);
         } else {
-            rc = index_eval_literal(ix, lo[i], lop[i], hi[i], rop[i], p, lit.p, cnt.p + i, stream());
+            rc = index_eval_literal(ix, lo[i], lop[i], hi[i], rop[i], p, lit.p, pc.row(i), &np[i], stream());
         }
     }
+    std::vector<uint32_t> c(nq);
+    if (!rc) rc = pc.launch_sum(np.data(), nq);
     if (!rc) rc = check(hipEventRecord(e1, stream()), "event");
     if (!rc) rc = check(hipEventSynchronize(e1), "event sync");
     float ms = 0.f;
     if (!rc) (void)hipEventElapsedTime(&ms, e0, e1);
     (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
     if (rc) return rc;
-    std::vector<uint32_t> c(nq);
-    if ((rc = to_host(c.data(), cnt.p, nq))) return rc;
+    if ((rc = to_host(c.data(), pc.counts.p, nq))) return rc;
     for (uint32_t i = 0; i < nq; ++i) hits[i] = used[i] == FBH_PLAN_COMPRESSED ? sync_hits[i] : c[i];
     if (plans_used) for (uint32_t i = 0; i < nq; ++i) plans_used[i] = used[i];
     if (gpu_ms) *gpu_ms = ms;

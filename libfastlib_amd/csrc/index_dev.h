@@ -7,7 +7,8 @@ namespace fbk {
 
 constexpr uint32_t NOKEY = 0xFFFFFFFFu;
 constexpr double DBL_MAX_D = 1.7976931348623157e308;
-constexpr int QT = 8192;                 // groups per query tile (LDS words)
+constexpr int QT = 2048;                 // groups per query tile (LDS words)
+constexpr int SCAN_MAX_BLOCKS = 2048;    // blocks of the raw scan plan
 
 // column types
 enum : int { COL_I32 = 0, COL_U32 = 1, COL_U16 = 2, COL_F32 = 3, COL_F64 = 4 };
@@ -15,10 +16,9 @@ enum : int { COL_I32 = 0, COL_U32 = 1, COL_U16 = 2, COL_F32 = 3, COL_F64 = 4 };
 enum : int { OPC_NONE = 0, OPC_LT = 1, OPC_LE = 2 };
 
 struct QueryArgs {
-    const uint32_t* words;     // serialized bitmaps
-    const uint64_t* off;       // bitmap b at words[off[b]]
-    const uint32_t* nw;        // code words of bitmap b
-    const uint2* toff;         // tile offset table [tile][bitmap]
+    const uint32_t* tw;        // tiled copy of the bitmaps
+    const uint32_t* TW;        // segment (t, b) at tw[TW[t*nobs + b]], ntiles*nobs + 1 entries
+    uint32_t ntiles;
     uint32_t nobs;
     uint32_t groups;           // whole 31-bit groups per bitmap
     uint32_t anb;              // active bits per bitmap
@@ -31,7 +31,7 @@ struct QueryArgs {
     double lo, hi;
     int64_t ilo, ihi;
     uint32_t* out;             // literal result, groups + 1 words
-    uint32_t* count;           // ones (atomic)
+    uint32_t* part;            // ones per block (ntiles or scan_blocks entries)
 };
 
 hipError_t launch_keys_relic(const void* col, int type, uint32_t n, const double* vals, uint32_t nv,
@@ -56,10 +56,24 @@ hipError_t launch_encode(int type, const uint32_t* rows, const uint32_t* start, 
 hipError_t launch_concat(const uint32_t* tmp, const uint64_t* tmpoff, const uint32_t* nw,
                          const uint32_t* aval, const uint64_t* off, const uint8_t* unpack,
                          uint32_t nbm, uint32_t groups, uint32_t anb, uint32_t* out, hipStream_t s);
+// Tile offset table (ntiles + 1 rows of nbm entries): row t < ntiles holds,
+// per bitmap, {absolute index of the first code word whose span ends after
+// group t*tg, its start group} ({end, groups} past the last word); row
+// ntiles holds {end of the code words, 1 when the bitmap is not null}.
 hipError_t launch_toff(const uint32_t* words, const uint64_t* off, const uint32_t* nw, uint32_t nbm,
-                       uint32_t ntiles, uint32_t tg, uint2* toff, hipStream_t s);
+                       uint32_t ntiles, uint32_t tg, uint32_t groups, uint2* toff, hipStream_t s);
+// Tiled copy: len[t*nobs + b] = words of segment (t, b) (ntiles*nobs + 1
+// entries, the last 0; its exclusive scan is TW), then the copy itself.
+hipError_t launch_tseg_len(const uint2* toff, uint32_t nobs, uint32_t ntiles, uint32_t groups,
+                           uint32_t anb, uint32_t* len, hipStream_t s);
+hipError_t launch_tseg_fill(const uint32_t* words, const uint2* toff, const uint32_t* TW, uint32_t nobs,
+                            uint32_t ntiles, uint32_t groups, uint32_t anb, uint32_t* tw, hipStream_t s);
 hipError_t launch_query_lit(const QueryArgs& q, int type, hipStream_t s);
 hipError_t launch_query_scan(const QueryArgs& q, int type, uint32_t nrows, hipStream_t s);
+uint32_t scan_blocks(uint32_t groups, uint32_t anb);
+// counts[i] = sum of parts[i*stride, i*stride + np[i])
+hipError_t launch_sum_parts(const uint32_t* parts, uint32_t stride, const uint32_t* np, uint32_t nq,
+                            uint32_t* counts, hipStream_t s);
 hipError_t launch_scan_excl(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp,
                             hipStream_t s);
 uint32_t scan_tmp_words(uint32_t n);

@@ -310,28 +310,38 @@ __global__ __launch_bounds__(IX_TPB) void k_concat(const uint32_t* __restrict__ 
     }
 }
 
-// Tile offset table: one thread per bitmap walks its code words.
+// Tile offset table: one wave per bitmap walks its code words 64 at a time
+// (lengths, DPP prefix sums); the lane whose word covers group t*QT writes
+// row t.  Rows past the bitmap's last group hold {end, groups}; the closing
+// row ntiles holds {end, 1 when the bitmap has words and a trailer}.
+// Word indices are absolute (into the serialized array).
 __global__ __launch_bounds__(IX_TPB) void k_toff(const uint32_t* __restrict__ words,
                                                  const uint64_t* __restrict__ off,
                                                  const uint32_t* __restrict__ nw, uint32_t nbm,
-                                                 uint32_t ntiles, uint32_t tg,
+                                                 uint32_t ntiles, uint32_t tg, uint32_t groups,
                                                  uint2* __restrict__ toff) {
-    const uint32_t b = blockIdx.x * IX_TPB + threadIdx.x;
+    const uint32_t b = blockIdx.x * (IX_TPB / WAVE) + threadIdx.x / WAVE;
     if (b >= nbm) return;
-    const uint32_t* w = words + off[b];
+    const int lane = (int)lane_id();
+    const uint32_t base = (uint32_t)off[b];
     const uint32_t n = nw[b];
-    uint32_t pos = 0, t = 0;
-    for (uint32_t i = 0; i < n && t < ntiles; ++i) {
-        const uint32_t l = word_len(w[i]);
-        // word i covers [pos, pos+l): it is the first word of every tile
-        // whose start lies in that span
-        while (t < ntiles && (uint64_t)t * tg < (uint64_t)pos + l) {
-            toff[(size_t)t * nbm + b] = make_uint2(i, pos);
-            ++t;
+    const bool null_bm = off[b + 1] == off[b];
+    if (lane == 0) toff[(size_t)ntiles * nbm + b] = make_uint2(base + n, null_bm ? 0u : 1u);
+    uint32_t carry = 0;
+    for (uint32_t i0 = 0; i0 < n; i0 += WAVE) {
+        const uint32_t i = i0 + (uint32_t)lane;
+        const uint32_t l = i < n ? word_len(words[base + i]) : 0u;
+        const uint32_t incl = wave_incl_sum(l);
+        const uint32_t start = carry + incl - l, end = carry + incl;
+        if (l) {
+            for (uint32_t t = (start + tg - 1) / tg; t < ntiles && (uint64_t)t * tg < end; ++t)
+                toff[(size_t)t * nbm + b] = make_uint2(base + i, start);
         }
-        pos += l;
+        carry += (uint32_t)__shfl((int)incl, WAVE - 1, WAVE);
     }
-    for (; t < ntiles; ++t) toff[(size_t)t * nbm + b] = make_uint2(n, pos);
+    // tiles starting at or after the last group covered by a word
+    for (uint32_t t = (carry + tg - 1) / tg + (uint32_t)lane; t < ntiles; t += WAVE)
+        toff[(size_t)t * nbm + b] = make_uint2(base + n, groups);
 }
 
 // ---------------------------------------------------------------------------
@@ -356,109 +366,239 @@ __device__ __forceinline__ bool in_range(const QueryArgs& q, T x) {
     return ok;
 }
 
-template <class T>
-__device__ void walk_segment(const QueryArgs& q, uint32_t b, uint32_t tile, uint32_t g0, uint32_t g1,
-                             word_t* R, bool check, const T* col, bool has_active) {
-    if (q.off[b + 1] == q.off[b]) return;     // null bitmap
-    const uint2 to = q.toff[(size_t)tile * q.nobs + b];
-    const uint32_t* w = q.words + q.off[b];
-    const uint32_t n = q.nw[b];
-    uint32_t i = to.x, pos = to.y;
-    while (i < n && pos < g1) {
-        const word_t x = w[i];
-        if (x > ALLONES) {
-            const uint32_t l = x & MAXCNT;
-            if (x >= HEADER1) {
+// ---------------------------------------------------------------------------
+// Tiled copy of the bitmaps for the literal plan.  For every tile t of QT
+// groups and bitmap b, segment (t, b) holds the code words of b that overlap
+// the tile with fills clipped to it, so that every segment of a bitmap with
+// words covers exactly the tile's groups (plus, in the last tile, the active
+// word appended left-aligned as one more literal group).  Segments are laid
+// out tile-major: tw[TW[t*nobs + b] .. TW[t*nobs + b + 1]).
+
+// Code words of bitmap b overlapping tile t: [s, e) with s's start group p
+// (from the tile offset table).
+__device__ __forceinline__ void tile_segment(const uint2* toff, uint32_t nobs, uint32_t ntiles,
+                                             uint32_t groups, uint32_t t, uint32_t b, uint32_t& s,
+                                             uint32_t& e, uint32_t& p) {
+    const uint2 F0 = toff[(size_t)t * nobs + b];
+    const uint2 F1 = toff[(size_t)(t + 1) * nobs + b];
+    const uint32_t g1 = min((t + 1) * (uint32_t)QT, groups);
+    s = F0.x; p = F0.y;
+    e = (t + 1 == ntiles) ? F1.x : F1.x + (F1.y < g1 ? 1u : 0u);
+}
+
+__global__ __launch_bounds__(IX_TPB) void k_tseg_len(const uint2* __restrict__ toff, uint32_t nobs,
+                                                     uint32_t ntiles, uint32_t groups, uint32_t anb,
+                                                     uint32_t* __restrict__ len) {
+    const uint32_t idx = blockIdx.x * IX_TPB + threadIdx.x;
+    if (idx > ntiles * nobs) return;
+    if (idx == ntiles * nobs) { len[idx] = 0u; return; }     // total after the scan
+    const uint32_t t = idx / nobs, b = idx - t * nobs;
+    uint32_t s, e, p;
+    tile_segment(toff, nobs, ntiles, groups, t, b, s, e, p);
+    uint32_t L = e - s;
+    if (t + 1 == ntiles && anb > 0 && toff[(size_t)ntiles * nobs + b].y) ++L;   // active word
+    len[idx] = L;
+}
+
+// One wave per segment: copy with the first and last fill clipped.
+__global__ __launch_bounds__(IX_TPB) void k_tseg_fill(const uint32_t* __restrict__ words,
+                                                      const uint2* __restrict__ toff,
+                                                      const uint32_t* __restrict__ TW, uint32_t nobs,
+                                                      uint32_t ntiles, uint32_t groups, uint32_t anb,
+                                                      uint32_t* __restrict__ tw) {
+    const uint32_t idx = blockIdx.x * (IX_TPB / WAVE) + threadIdx.x / WAVE;
+    if (idx >= ntiles * nobs) return;
+    const int lane = (int)lane_id();
+    const uint32_t t = idx / nobs, b = idx - t * nobs;
+    uint32_t s, e, p;
+    tile_segment(toff, nobs, ntiles, groups, t, b, s, e, p);
+    const uint32_t g0 = t * (uint32_t)QT, g1 = min(g0 + (uint32_t)QT, groups);
+    const uint32_t base = TW[idx];
+    const uint32_t L = e - s;
+    uint32_t carry = p;
+    for (uint32_t o = 0; o < L; o += WAVE) {
+        const uint32_t i = o + (uint32_t)lane;
+        const word_t x = i < L ? words[s + i] : 0u;
+        const uint32_t len = i < L ? word_len(x) : 0u;
+        const uint32_t inc = wave_incl_sum(len);
+        const uint32_t pos = carry + inc - len;
+        if (i < L) {
+            word_t y = x;
+            if (x >= HEADER0) {
                 const uint32_t lo = pos > g0 ? pos : g0;
-                const uint32_t hi = (pos + l) < g1 ? (pos + l) : g1;
-                for (uint32_t g = lo; g < hi; ++g) {
-                    if (!check) R[g - g0] = ALLONES;
-                    else {
-                        word_t m = 0;
-                        for (uint32_t k = 0; k < 31u; ++k)
-                            if (in_range(q, col[g * 31u + k])) m |= 1u << (30u - k);
-                        if (m) atomicOr(&R[g - g0], m);
-                    }
-                }
+                const uint32_t hi = pos + len < g1 ? pos + len : g1;
+                y = (x & HEADER1) | (hi - lo);
             }
-            pos += l;
-        } else {
-            if (x && pos >= g0) {
-                if (!check) atomicOr(&R[pos - g0], x);
-                else {
-                    word_t m = 0, y = x;
-                    while (y) {
-                        const uint32_t hb = 31u - __clz(y);
-                        const uint32_t row = pos * 31u + (30u - hb);
-                        if (in_range(q, col[row])) m |= 1u << hb;
-                        y &= ~(1u << hb);
-                    }
-                    if (m) atomicOr(&R[pos - g0], m);
-                }
-            }
-            pos += 1;
+            tw[base + i] = y;
         }
-        ++i;
+        carry += (uint32_t)__builtin_amdgcn_readlane((int)inc, WAVE - 1);
     }
-    // the active word belongs to tile groups / QT (slot groups - g0)
-    if (has_active) {
-        const word_t av = w[n];
-        if (av) {
-            const word_t left = av << (31u - q.anb);
-            if (!check) atomicOr(&R[q.groups - g0], left);
-            else {
-                word_t m = 0, y = left;
-                while (y) {
-                    const uint32_t hb = 31u - __clz(y);
-                    const uint32_t row = q.groups * 31u + (30u - hb);
-                    if (in_range(q, col[row])) m |= 1u << hb;
-                    y &= ~(1u << hb);
+    if (lane == 0 && t + 1 == ntiles && anb > 0) {
+        const uint2 E = toff[(size_t)ntiles * nobs + b];
+        if (E.y) tw[base + L] = words[E.x] << (31u - anb);
+    }
+}
+
+// Literal range plan.  One block of QB_TPB threads owns tile t: ng = QT
+// groups (in the last tile the remaining groups plus the active slot).  The
+// segments of a bitmap range are contiguous in the tiled copy and each
+// covers exactly ng groups, so a word's group within the tile is the
+// running sum of the word lengths from the start of a segment, modulo ng:
+// waves stream chunks of CB bitmaps (taken from an LDS counter) 64 words a
+// round, QB_U rounds of loads in flight, with a DPP prefix sum of the word
+// lengths.  Literal words are OR-ed into the LDS tile R; 1-fills add +1/-1
+// at their ends to the LDS coverage array cov, whose prefix sums mark
+// all-one groups.  Edge bitmaps stream the same way with every row checked
+// against the raw column.
+constexpr int QB_TPB = 256;
+constexpr int QB_WAVES = QB_TPB / WAVE;
+constexpr int QB_U = 16;                     // rounds of loads in flight per lane
+constexpr uint32_t QB_CB = 64;               // bitmaps per chunk
+
+template <class T>
+__device__ __forceinline__ word_t check_group(const QueryArgs& q, const T* col, uint32_t g, word_t bits) {
+    word_t m = 0;
+    while (bits) {
+        const uint32_t hb = 31u - __clz(bits);
+        const uint32_t row = g * 31u + (30u - hb);
+        if (in_range(q, col[row])) m |= 1u << hb;
+        bits &= ~(1u << hb);
+    }
+    return m;
+}
+
+// Streams tiled words [ws, we) (whole segments) into the tile, by one wave.
+template <class T, bool CHECK, bool POW2>
+__device__ void stream_words(const QueryArgs& q, uint32_t ws, uint32_t we, uint32_t g0, uint32_t ng,
+                             word_t* R, uint32_t* cov, const T* col) {
+    // the checked path (edge bitmaps, one segment) keeps one round in flight
+    constexpr int U = CHECK ? 1 : QB_U;
+    const int lane = (int)lane_id();
+    uint32_t carry = 0;                      // groups before the round, modulo ng
+    for (uint32_t base = ws; base < we; base += U * WAVE) {
+        word_t x[U];
+        #pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t i = base + (uint32_t)u * WAVE + (uint32_t)lane;
+            x[u] = i < we ? q.tw[i] : 0u;
+        }
+        #pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t r0 = base + (uint32_t)u * WAVE;
+            const bool act = r0 + (uint32_t)lane < we;
+            const word_t w = x[u];
+            const uint32_t len = act ? word_len(w) : 0u;
+            const uint32_t inc = wave_incl_sum(len);
+            uint32_t rel = carry + inc - len;
+            rel = POW2 ? (rel & (uint32_t)(QT - 1)) : (rel % ng);
+            const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)inc, WAVE - 1);
+            carry = POW2 ? ((carry + tot) & (uint32_t)(QT - 1)) : ((carry + tot) % ng);
+            if (!act) continue;
+            if (w < HEADER0) {
+                if (w) {
+                    const word_t m = CHECK ? check_group<T>(q, col, g0 + rel, w) : w;
+                    if (m) atomicOr(&R[rel], m);
                 }
-                if (m) atomicOr(&R[q.groups - g0], m);
+            } else if (w >= HEADER1) {
+                if (!CHECK) {
+                    atomicAdd(&cov[rel], 1u);
+                    atomicSub(&cov[rel + len], 1u);
+                } else {
+                    for (uint32_t k = 0; k < len; ++k) {
+                        const word_t m = check_group<T>(q, col, g0 + rel + k, ALLONES);
+                        if (m) atomicOr(&R[rel + k], m);
+                    }
+                }
             }
         }
     }
 }
 
+template <class T, bool POW2>
+__device__ void tile_ranges(const QueryArgs& q, uint32_t tile, uint32_t g0, uint32_t ng, word_t* R,
+                            uint32_t* cov, uint32_t* next, const T* col) {
+    const uint32_t n1 = (q.r1 - q.r0 + QB_CB - 1) / QB_CB;
+    const uint32_t n2 = (q.r3 - q.r2 + QB_CB - 1) / QB_CB;
+    const uint32_t* TWt = q.TW + (size_t)tile * q.nobs;
+    for (;;) {
+        uint32_t k = 0;
+        if (lane_id() == 0) k = atomicAdd(next, 1u);
+        k = (uint32_t)__builtin_amdgcn_readfirstlane((int)k);
+        if (k >= n1 + n2) break;
+        uint32_t bs, be;
+        if (k < n1) { bs = q.r0 + k * QB_CB; be = min(bs + QB_CB, q.r1); }
+        else { bs = q.r2 + (k - n1) * QB_CB; be = min(bs + QB_CB, q.r3); }
+        stream_words<T, false, POW2>(q, TWt[bs], TWt[be], g0, ng, R, cov, col);
+    }
+}
+
 template <class T>
-__global__ __launch_bounds__(IX_TPB) void k_query_lit(QueryArgs q) {
+__global__ __launch_bounds__(QB_TPB) void k_query_lit(QueryArgs q) {
     __shared__ word_t R[QT + 1];
+    __shared__ uint32_t cov[QT + 1];
+    __shared__ uint32_t red[QB_WAVES + 1];
+    __shared__ uint32_t next;
     const uint32_t tile = blockIdx.x;
+    const bool last = tile + 1 == q.ntiles;
     const uint32_t g0 = tile * (uint32_t)QT;
-    // groups of this tile, plus the active slot in tile groups / QT
-    const uint32_t g1 = min(g0 + (uint32_t)QT, q.groups);
-    const bool has_active = (tile == q.groups / (uint32_t)QT) && q.anb > 0;
-    const uint32_t nslot = (g1 > g0 ? g1 - g0 : 0u) + (has_active ? 1u : 0u);
-    for (uint32_t k = threadIdx.x; k < nslot; k += IX_TPB) R[k] = 0u;
+    const uint32_t ng = last ? (q.groups - g0) + (q.anb ? 1u : 0u) : (uint32_t)QT;
+    const int w = threadIdx.x / WAVE;
+    for (uint32_t k = threadIdx.x; k <= QT; k += QB_TPB) { R[k] = 0u; cov[k] = 0u; }
+    if (threadIdx.x == 0) next = 0u;
     __syncthreads();
     const T* col = (const T*)q.col;
-    for (int r = 0; r < 2; ++r) {
-        const uint32_t b0 = r == 0 ? q.r0 : q.r2, b1 = r == 0 ? q.r1 : q.r3;
-        for (uint32_t b = b0 + threadIdx.x; b < b1; b += IX_TPB)
-            walk_segment<T>(q, b, tile, g0, g1, R, false, col, has_active);
+    if (ng) {
+        if (last) tile_ranges<T, false>(q, tile, g0, ng, R, cov, &next, col);
+        else tile_ranges<T, true>(q, tile, g0, ng, R, cov, &next, col);
+    }
+    __syncthreads();
+    // coverage prefix sums: groups under some 1-fill are all ones
+    {
+        constexpr int PER = QT / QB_TPB;
+        const uint32_t k0 = threadIdx.x * PER;
+        uint32_t c[PER], sum = 0;
+        #pragma unroll
+        for (int i = 0; i < PER; ++i) { c[i] = cov[k0 + i]; sum += c[i]; }
+        uint32_t tot;
+        uint32_t run = block_excl_sum<QB_TPB>(sum, red, &tot);
+        #pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            run += c[i];
+            if (run != 0u) R[k0 + i] = ALLONES;
+        }
     }
     __syncthreads();
     if (q.flip) {
-        for (uint32_t k = threadIdx.x; k < nslot; k += IX_TPB) {
+        for (uint32_t k = threadIdx.x; k < ng; k += QB_TPB) {
             const uint32_t g = g0 + k;
             const word_t mask = g < q.groups ? ALLONES : (((1u << q.anb) - 1u) << (31u - q.anb));
             R[k] ^= mask;
         }
         __syncthreads();
     }
-    if (threadIdx.x < 2) {
-        const uint32_t cb = threadIdx.x == 0 ? q.cand0 : q.cand1;
-        if (cb != NOKEY) walk_segment<T>(q, cb, tile, g0, g1, R, true, col, has_active);
+    if (w < 2 && ng) {
+        const uint32_t cb = w == 0 ? q.cand0 : q.cand1;
+        if (cb != NOKEY) {
+            const uint32_t* TWt = q.TW + (size_t)tile * q.nobs;
+            if (last) stream_words<T, true, false>(q, TWt[cb], TWt[cb + 1], g0, ng, R, cov, col);
+            else stream_words<T, true, true>(q, TWt[cb], TWt[cb + 1], g0, ng, R, cov, col);
+        }
     }
     __syncthreads();
     uint32_t ones = 0;
-    for (uint32_t k = threadIdx.x; k < nslot; k += IX_TPB) {
+    for (uint32_t k = threadIdx.x; k < ng; k += QB_TPB) {
         const word_t v = R[k];
         q.out[g0 + k] = v;
         ones += __popc(v);
     }
     ones = wave_sum(ones);
-    if (lane_id() == 0 && ones) atomicAdd(q.count, ones);
+    if (lane_id() == 0) red[w] = ones;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int i = 0; i < QB_WAVES; ++i) t += red[i];
+        q.part[tile] = t;
+    }
 }
 
 // Raw scan plan: R[g] = rows 31g..31g+30 that satisfy the range.  A wave
@@ -498,8 +638,35 @@ __global__ __launch_bounds__(IX_TPB) void k_query_scan(QueryArgs q, uint32_t nro
         }
         __builtin_amdgcn_wave_barrier();
     }
+    __shared__ uint32_t red[IX_TPB / WAVE];
     ones = wave_sum(ones);
-    if (lane == 0 && ones) atomicAdd(q.count, ones);
+    if (lane == 0) red[w] = ones;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int i = 0; i < IX_TPB / WAVE; ++i) t += red[i];
+        q.part[blockIdx.x] = t;
+    }
+}
+
+// counts[i] = sum of parts[i * stride .. i * stride + np[i]), one block per
+// query
+__global__ __launch_bounds__(IX_TPB) void k_sum_parts(const uint32_t* __restrict__ parts, uint32_t stride,
+                                                      const uint32_t* __restrict__ np,
+                                                      uint32_t* __restrict__ counts) {
+    __shared__ uint32_t red[IX_TPB / WAVE];
+    const uint32_t qi = blockIdx.x;
+    const uint32_t n = np[qi];
+    uint32_t t = 0;
+    for (uint32_t k = threadIdx.x; k < n; k += IX_TPB) t += parts[(size_t)qi * stride + k];
+    t = wave_sum(t);
+    if (lane_id() == 0) red[threadIdx.x / WAVE] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t x = 0;
+        for (int i = 0; i < IX_TPB / WAVE; ++i) x += red[i];
+        counts[qi] = x;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -678,15 +845,45 @@ hipError_t launch_concat(const uint32_t* tmp, const uint64_t* tmpoff, const uint
 }
 
 hipError_t launch_toff(const uint32_t* words, const uint64_t* off, const uint32_t* nw, uint32_t nbm,
-                       uint32_t ntiles, uint32_t tg, uint2* toff, hipStream_t s) {
+                       uint32_t ntiles, uint32_t tg, uint32_t groups, uint2* toff, hipStream_t s) {
     if (!nbm) return hipSuccess;
-    hipLaunchKernelGGL(k_toff, dim3(cdivu(nbm, IX_TPB)), dim3(IX_TPB), 0, s, words, off, nw, nbm, ntiles, tg, toff);
+    hipLaunchKernelGGL(k_toff, dim3(cdivu(nbm, IX_TPB / WAVE)), dim3(IX_TPB), 0, s, words, off, nw, nbm,
+                       ntiles, tg, groups, toff);
+    return hipGetLastError();
+}
+
+hipError_t launch_tseg_len(const uint2* toff, uint32_t nobs, uint32_t ntiles, uint32_t groups,
+                           uint32_t anb, uint32_t* len, hipStream_t s) {
+    const uint32_t n = ntiles * nobs + 1;
+    hipLaunchKernelGGL(k_tseg_len, dim3(cdivu(n, IX_TPB)), dim3(IX_TPB), 0, s, toff, nobs, ntiles, groups,
+                       anb, len);
+    return hipGetLastError();
+}
+
+hipError_t launch_tseg_fill(const uint32_t* words, const uint2* toff, const uint32_t* TW, uint32_t nobs,
+                            uint32_t ntiles, uint32_t groups, uint32_t anb, uint32_t* tw, hipStream_t s) {
+    const uint32_t n = ntiles * nobs;
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_tseg_fill, dim3(cdivu(n, IX_TPB / WAVE)), dim3(IX_TPB), 0, s, words, toff, TW, nobs,
+                       ntiles, groups, anb, tw);
+    return hipGetLastError();
+}
+
+uint32_t scan_blocks(uint32_t groups, uint32_t anb) {
+    const uint32_t waves = cdivu(groups + (anb ? 1u : 0u), 64u);
+    const uint32_t blocks = min(cdivu(waves, IX_TPB / WAVE), (uint32_t)SCAN_MAX_BLOCKS);
+    return blocks ? blocks : 1u;
+}
+
+hipError_t launch_sum_parts(const uint32_t* parts, uint32_t stride, const uint32_t* np, uint32_t nq,
+                            uint32_t* counts, hipStream_t s) {
+    if (!nq) return hipSuccess;
+    hipLaunchKernelGGL(k_sum_parts, dim3(nq), dim3(IX_TPB), 0, s, parts, stride, np, counts);
     return hipGetLastError();
 }
 
 hipError_t launch_query_lit(const QueryArgs& q, int type, hipStream_t s) {
-    const uint32_t nt = q.groups / QT + 1;   // includes the active slot's tile
-    dim3 g(nt), b(IX_TPB);
+    dim3 g(q.ntiles), b(QB_TPB);             // the last tile holds the active slot
     switch (type) {
     case COL_I32: hipLaunchKernelGGL(k_query_lit<int32_t>, g, b, 0, s, q); break;
     case COL_U32: hipLaunchKernelGGL(k_query_lit<uint32_t>, g, b, 0, s, q); break;
@@ -699,10 +896,7 @@ hipError_t launch_query_lit(const QueryArgs& q, int type, hipStream_t s) {
 }
 
 hipError_t launch_query_scan(const QueryArgs& q, int type, uint32_t nrows, hipStream_t s) {
-    const uint32_t ng = q.groups + (q.anb ? 1u : 0u);
-    const uint32_t waves = cdivu(ng, 64u);
-    const uint32_t blocks = min(cdivu(waves, IX_TPB / WAVE), 8192u);
-    dim3 g(blocks ? blocks : 1u), b(IX_TPB);
+    dim3 g(scan_blocks(q.groups, q.anb)), b(IX_TPB);
     switch (type) {
     case COL_I32: hipLaunchKernelGGL(k_query_scan<int32_t>, g, b, 0, s, q, nrows); break;
     case COL_U32: hipLaunchKernelGGL(k_query_scan<uint32_t>, g, b, 0, s, q, nrows); break;
