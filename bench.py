@@ -67,6 +67,20 @@ def config3(log2n, nq, seed=3):
     return col, bounds, src, lo, hi
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counter-scaled), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_bin2p27_1024q.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if f and f[0] == kernel:
+            kv = dict(x.split("=") for x in f[1:])
+            return float(kv["fetch_bytes"]) + float(kv["write_bytes"])
+    return None
+
+
 def query_bytes(ix, lo, hi, plan):
     return np.array([ix.query_bytes(float(lo[i]), 2, float(hi[i]), 1, plan) for i in range(len(lo))])
 
@@ -123,6 +137,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=16)
     ap.add_argument("--no-wah", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
+                    help="finish with 5 device-to-device copies of MB MiB (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; multi-GPU sharding is a design "
@@ -173,7 +190,7 @@ def main():
     dom_kernel = {"literal": "k_query_lit", "scan": "k_query_scan"}[dom]
     achieved = per_plan[dom]["GBs"]
     roofline = dict(bound="hbm", kernel=dom_kernel, achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(dom_kernel),
                     avg_launch_us=round(per_plan[dom]["gpu_ms"] * 1e3 / len(lo), 2))
 
     wah = None if args.no_wah else wah_cells(L)
@@ -205,6 +222,16 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+    if args.copy_probe:
+        # an elementwise kernel that reads and writes MB MiB each (a plain
+        # copy through the shader, unlike hipMemcpy's DMA path)
+        import torch
+        n = (args.copy_probe << 20) // 4
+        src = torch.zeros(n, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            torch.bitwise_not(src, out=dst)
+        torch.cuda.synchronize()
 
 
 if __name__ == "__main__":
