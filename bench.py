@@ -3,11 +3,11 @@
 
 Workload (the headline metric, range queries/s at 2^27 rows): a float32
 column of 2^27 rows drawn from a pool of 10^5 distinct values (70% N(0,1)
-draws, 30% Zipf(1.2) support; seed 3), an ibis::bin index with the
-reference's 10,000 equal-weight bins (boundaries from
-tests/golden/config3_2p27_bounds.npz, built here on the GPU), and 1024 range
-queries lo <= x < hi whose endpoints are the column's empirical quantiles at
-uniform draws.  One step evaluates the 1024 queries (hit counts) with the
+draws, 30% Zipf(1.2) support; seed 3), an ibis::bin index with 10,000
+equal-weight bins built on the GPU (the boundaries FastBit's
+"<binning nbins=10000 equal-weight/>" chooses), and 1024 range queries
+lo <= x < hi whose endpoints are the column's empirical quantiles at uniform
+draws.  One step evaluates the 1024 queries (hit counts) with the
 plan the ibis heuristic picks per query; inputs are resident in HBM.
 
 The same JSON line reports
@@ -48,14 +48,6 @@ def config3(log2n, nq, seed=3):
     t = time.time()
     col = G.float_mix(seed, n, pool=100000, wnorm=0.7, s=1.2)
     log(f"column: {n} rows in {time.time() - t:.1f}s")
-    path = os.path.join(ROOT, "tests", "golden", f"config3_2p{log2n}_bounds.npz")
-    if os.path.exists(path):
-        bounds = np.load(path)["bounds"]
-        src = "reference equal-weight bins"
-    else:
-        qs = np.quantile(col.astype(np.float64), np.linspace(0, 1, 10001)[1:-1])
-        bounds = np.concatenate([np.unique(qs), [np.finfo(np.float64).max]])
-        src = "quantile bins"
     t = time.time()
     srt = np.sort(col)
     u = G.uniform_f64(seed + 1000, 2 * nq)
@@ -64,7 +56,7 @@ def config3(log2n, nq, seed=3):
     lo, hi = np.minimum(a, b), np.maximum(a, b)
     log(f"queries: {nq} in {time.time() - t:.1f}s")
     del srt
-    return col, bounds, src, lo, hi
+    return col, lo, hi
 
 
 def measured_traffic(kernel):
@@ -148,9 +140,9 @@ def main():
 
     import libfastlib_amd as L
     L.lib()
-    col, bounds, bsrc, lo, hi = config3(args.log2_rows, args.queries)
+    col, lo, hi = config3(args.log2_rows, args.queries)
     t = time.time()
-    ix = L.Index.bin(col, bounds)
+    ix = L.Index.bin_equal_weight(col, 10000)     # <binning nbins=10000 equal-weight/>
     L.lib().fbh_sync()
     build_s = time.time() - t
     nobs, nrows, nwords = ix.info()
@@ -209,8 +201,8 @@ def main():
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic (deterministic generators, seeds in bench.py)",
-        "config": {"workload": f"binned float index: 2^{args.log2_rows} float32 rows, {bsrc} "
-                               f"({nobs} bitmaps, {nwords} words), {args.queries} quantile range "
+        "config": {"workload": f"binned float index: 2^{args.log2_rows} float32 rows, 10000 equal-weight "
+                               f"bins ({nobs} bitmaps, {nwords} words), {args.queries} quantile range "
                                f"queries lo <= x < hi, plan chosen per query by the ibis heuristic",
                    "rows": int(nrows), "bins": int(nobs), "queries": args.queries,
                    "gpu_index_build_s": round(build_s, 2)},

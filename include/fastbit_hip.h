@@ -162,6 +162,21 @@ int fbh_index_build_relic_card(const uint32_t* ind, uint32_t nrows, uint32_t car
  * (bounds[nobs-1] is DBL_MAX). */
 int fbh_index_build_bin(const void* col, int type, uint32_t nrows, const double* bounds,
                         uint32_t nbounds, fbh_index** out);
+/* bin with equal-weight boundaries: the "<binning nbins=N equal-weight/>"
+ * specification of bin::setBoundaries / scanAndPartition
+ * (src/ibin.cpp:2975-3332, 2389-2582): histogram of the distinct values on
+ * the device, index::divideCounts (src/index.cpp:3648-4206) and
+ * util::compactValue (src/util.cpp:1227-1312) on the host. */
+int fbh_index_build_bin_nbins(const void* col, int type, uint32_t nrows, uint32_t nbins,
+                              fbh_index** out);
+/* The bin boundaries (bounds[nobs-1] == DBL_MAX); *n receives the count. */
+int fbh_index_bounds(const fbh_index* ix, double* bounds, uint32_t cap, uint32_t* n);
+/* Host-only: the equal-weight boundaries of a histogram (sorted distinct
+ * values and their row counts); is_float selects the floating-point rules,
+ * col_upper is the column's known upper bound (-DBL_MAX when unknown). */
+int fbh_bin_equal_weight(const double* vals, const uint32_t* counts, uint32_t ncnt, uint32_t nbins,
+                         int is_float, double col_upper, double* bounds, uint32_t cap,
+                         uint32_t* nbounds);
 void fbh_index_free(fbh_index* ix);
 int fbh_index_info(const fbh_index* ix, uint32_t* nobs, uint32_t* nrows, uint64_t* nwords);
 /* fastbit_iapi_deconstruct_index (src/iapi.cpp:1915-1953): keys (relic:

@@ -343,7 +343,9 @@ int upload_column(fbh_index* ix, const void* col, int type, uint32_t nrows) {
 
 // Distinct values of the column (sorted), computed on the GPU: order-
 // preserving 32-bit keys, radix sort, host pass over the sorted keys.
-int distinct_values(fbh_index* ix, std::vector<double>& vals) {
+// With counts != nullptr also the number of rows of every value (the
+// histogram of index::mapValues, src/index.cpp:3411-3479).
+int distinct_values(fbh_index* ix, std::vector<double>& vals, std::vector<uint32_t>* counts = nullptr) {
     const uint32_t n = ix->nrows;
     if (ix->type == FBH_COL_DOUBLE) return fail(FBH_ERR_ARG, "relic over double columns is not supported");
     DBuf<uint32_t> k, uq, tmp;
@@ -365,6 +367,18 @@ int distinct_values(fbh_index* ix, std::vector<double>& vals) {
     const uint32_t nd = n ? h[0] + h[1] : 0;
     std::vector<uint32_t> s(nd);
     if ((rc = to_host(s.data(), uq.p, nd))) return rc;
+    if (counts) {
+        // first sorted position of every distinct key -> run lengths
+        DBuf<uint32_t> first;
+        if ((rc = first.alloc(nd + 1))) return rc;
+        rc = check(fbk::launch_unique_first(tmp.p, tmp.p + n, n, first.p, stream()), "unique first");
+        std::vector<uint32_t> f(nd + 1);
+        if (!rc) rc = to_host(f.data(), first.p, nd);
+        if (rc) return rc;
+        f[nd] = n;
+        counts->resize(nd);
+        for (uint32_t i = 0; i < nd; ++i) (*counts)[i] = f[i + 1] - f[i];
+    }
     vals.resize(nd);
     for (uint32_t i = 0; i < nd; ++i) {
         const uint32_t u = s[i];
@@ -678,6 +692,44 @@ int fbh_index_build_bin(const void* col, int type, uint32_t nrows, const double*
         ix->cmax = std::max(ix->cmax, ix->maxval[b]);
     }
     *out = ix;
+    return FBH_OK;
+}
+
+int fbh_index_build_bin_nbins(const void* col, int type, uint32_t nrows, uint32_t nbins,
+                              fbh_index** out) {
+    if (!out || (nrows && !col) || nbins == 0) return fail(FBH_ERR_ARG, "bad argument");
+    if (type == FBH_COL_DOUBLE) return fail(FBH_ERR_ARG, "equal-weight bins over double columns are not supported");
+    int rc = ensure_init();
+    if (rc) return rc;
+    // histogram on the device, boundaries on the host
+    fbh_index tmp;
+    if ((rc = upload_column(&tmp, col, type, nrows))) return rc;
+    std::vector<double> vals;
+    std::vector<uint32_t> cnt;
+    if ((rc = distinct_values(&tmp, vals, &cnt))) return rc;
+    // -0.0 and +0.0 are one histogram entry (std::map<double, uint32_t>)
+    for (size_t i = 1; i < vals.size(); ++i) {
+        if (vals[i] == vals[i - 1]) {
+            cnt[i - 1] += cnt[i];
+            vals.erase(vals.begin() + i);
+            cnt.erase(cnt.begin() + i);
+            --i;
+        }
+    }
+    const bool is_float = type == FBH_COL_FLOAT;
+    std::vector<double> bounds;
+    // the in-memory column has no computed upper bound (-DBL_MAX)
+    fbh_ix::equal_weight_bounds(vals.data(), cnt.data(), (uint32_t)vals.size(), nbins, is_float, -DBL_MAX,
+                                bounds);
+    if (bounds.empty()) return fail(FBH_ERR_ARG, "empty column");
+    return fbh_index_build_bin(col, type, nrows, bounds.data(), (uint32_t)bounds.size(), out);
+}
+
+int fbh_index_bounds(const fbh_index* ix, double* bounds, uint32_t cap, uint32_t* n) {
+    if (!ix || !n) return fail(FBH_ERR_ARG, "null argument");
+    if (ix->kind != FBH_BIN) return fail(FBH_ERR_ARG, "not a bin index");
+    *n = (uint32_t)ix->keys.size();
+    if (bounds) memcpy(bounds, ix->keys.data(), sizeof(double) * std::min<size_t>(cap, ix->keys.size()));
     return FBH_OK;
 }
 

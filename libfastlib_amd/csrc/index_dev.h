@@ -42,6 +42,10 @@ hipError_t launch_keys_bin(const void* col, int type, uint32_t n, const double* 
                            uint32_t* keys, hipStream_t s);
 hipError_t launch_order_keys(const void* col, int type, uint32_t n, uint32_t* keys, hipStream_t s);
 hipError_t launch_unique(const uint32_t* k, uint32_t n, uint32_t* out, uint32_t* tmp, hipStream_t s);
+// after launch_unique: first[j] = sorted position of distinct key j
+// (flag = tmp, pos = tmp + n of that call)
+hipError_t launch_unique_first(const uint32_t* flag, const uint32_t* pos, uint32_t n, uint32_t* first,
+                               hipStream_t s);
 hipError_t launch_starts(const uint32_t* k, uint32_t n, uint32_t nbm, uint32_t* start, uint32_t* room,
                          hipStream_t s);
 uint32_t rs_blocks(uint32_t n);

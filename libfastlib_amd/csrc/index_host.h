@@ -164,6 +164,11 @@ inline double compact_value(double left, double right, double start = 0.0) {
     return sep;
 }
 
+// Equal-weight bin boundaries from the sorted distinct values and their
+// counts (bin::scanAndPartition + setBoundaries; bin_bounds.cpp).
+void equal_weight_bounds(const double* val, const uint32_t* cnt, uint32_t ncnt, uint32_t nbins,
+                         bool is_float, double col_upper, std::vector<double>& bounds);
+
 // part::doScan's bound adjustment for an integer column of type T
 // (src/part.cpp:11130-11237): bounds are cast to T and the operator moves
 // between < and <= when the cast changed the value.

@@ -703,6 +703,14 @@ __global__ __launch_bounds__(IX_TPB) void k_unique_scatter(const uint32_t* __res
     if (i < n && flag[i]) out[pos[i]] = k[i];
 }
 
+// first[pos[i]] = i for every i that starts a run of equal sorted keys
+__global__ __launch_bounds__(IX_TPB) void k_unique_first(const uint32_t* __restrict__ flag,
+                                                         const uint32_t* __restrict__ pos, uint32_t n,
+                                                         uint32_t* __restrict__ first) {
+    const uint32_t i = blockIdx.x * IX_TPB + threadIdx.x;
+    if (i < n && flag[i]) first[pos[i]] = i;
+}
+
 // start[b] = first index of key b in the sorted keys (b in [0, nbm])
 __global__ __launch_bounds__(IX_TPB) void k_starts(const uint32_t* __restrict__ k, uint32_t n,
                                                    uint32_t nbm, uint32_t* __restrict__ start) {
@@ -785,6 +793,13 @@ hipError_t launch_unique(const uint32_t* k, uint32_t n, uint32_t* out, uint32_t*
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_unique_scatter, dim3(cdivu(n, IX_TPB)), dim3(IX_TPB), 0, s, k, n,
                        (const uint32_t*)flag, (const uint32_t*)pos, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_unique_first(const uint32_t* flag, const uint32_t* pos, uint32_t n, uint32_t* first,
+                               hipStream_t s) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_unique_first, dim3(cdivu(n, IX_TPB)), dim3(IX_TPB), 0, s, flag, pos, n, first);
     return hipGetLastError();
 }
 

@@ -248,6 +248,8 @@ def _index_sigs(L):
         "fbh_index_query_bytes": [vp, C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, _dp],
         "fbh_index_column_bytes": [vp, _u64p],
         "fbh_index_set_cost_model": [vp, C.c_int],
+        "fbh_index_build_bin_nbins": [vp, C.c_int, C.c_uint32, C.c_uint32, C.POINTER(vp)],
+        "fbh_index_bounds": [vp, _dp, C.c_uint32, _u32p],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -300,6 +302,25 @@ class Index:
         check(L.fbh_index_build_bin(a.ctypes.data_as(C.c_void_p), _COLT[a.dtype], a.size,
                                     b.ctypes.data_as(_dp), b.size, C.byref(h)), "fbh_index_build_bin")
         return cls(h)
+
+    @classmethod
+    def bin_equal_weight(cls, col, nbins):
+        """bin with nbins equal-weight bins, boundaries chosen as the
+        reference's "<binning nbins=N equal-weight/>" does."""
+        L = cls._lib()
+        a = np.ascontiguousarray(col)
+        h = C.c_void_p()
+        check(L.fbh_index_build_bin_nbins(a.ctypes.data_as(C.c_void_p), _COLT[a.dtype], a.size, int(nbins),
+                                          C.byref(h)), "fbh_index_build_bin_nbins")
+        return cls(h)
+
+    def bounds(self):
+        L = self._lib()
+        n = C.c_uint32()
+        check(L.fbh_index_bounds(self._h, None, 0, C.byref(n)), "fbh_index_bounds")
+        out = np.zeros(max(n.value, 1))
+        check(L.fbh_index_bounds(self._h, out.ctypes.data_as(_dp), n.value, C.byref(n)), "fbh_index_bounds")
+        return out[: n.value]
 
     def __del__(self):
         try:

@@ -363,6 +363,19 @@ class RefIapi:
         return int(n), (buf[:n].astype(np.uint32) if want_rids else None)
 
 
+def ref_bin_bounds(name):
+    """The boundaries of the bin index the reference built for the
+    registered array `name` (bin::binBoundaries)."""
+    L = reference().lib
+    p, n = _dp(), C.c_uint64()
+    rc = L.ref_iapi_bin_bounds(name.encode() if isinstance(name, str) else name, C.byref(p), C.byref(n))
+    if rc != 0:
+        raise RuntimeError(f"ref_iapi_bin_bounds failed rc={rc}")
+    out = np.ctypeslib.as_array(p, (n.value,)).copy() if n.value else np.zeros(0)
+    L.ref_free(p)
+    return out
+
+
 def bounds_from_keys(keys):
     """bin bounds as bin(..., keys, offsets) rebuilds them
     (src/ibin.cpp:358-372): compactValue(maxval[j], minval[j+1]), DBL_MAX."""

@@ -244,3 +244,22 @@ int64_t ref_iapi_query_range(const char* name, double lo, int lop, double hi, in
 } // extern "C"
 
 extern "C" void ref_set_verbose(int v) { ensure_init(); ibis::gVerbose = v; }
+
+// The boundaries of the bin index built for a registered array
+// (ibis::bin::binBoundaries through the column's index lock).
+#include "bord.h"
+ibis::bord::column* __fastbit_iapi_array_by_name(const char* name);
+
+extern "C" int ref_iapi_bin_bounds(const char* name, double** out, uint64_t* n) {
+    ensure_init();
+    ibis::bord::column* col = __fastbit_iapi_array_by_name(name);
+    if (col == 0) return -1;
+    ibis::column::indexLock lock(col, "ref_iapi_bin_bounds");
+    const ibis::index* ix = lock.getIndex();
+    if (ix == 0) return -2;
+    std::vector<double> b;
+    ix->binBoundaries(b);
+    *n = b.size();
+    *out = dupv(b.data(), b.size());
+    return 0;
+}

@@ -4,7 +4,8 @@ Column: 2^27 float32 values from a pool of 10^5 distinct values (70% N(0,1)
 draws, 30% Zipf(1.2) support), generated by libfastlib_amd.datagen.float_mix
 with seed 3.  The reference builds its index with the spec
 ``<binning nbins=10000 equal-weight/>`` through fastbit_iapi_build_index;
-the fixture keeps the bin boundaries as the reference rebuilds them from the
+the fixture keeps the bin boundaries the reference computed
+(bin::binBoundaries), the boundaries as the reference rebuilds them from the
 deconstructed keys (src/ibin.cpp:358-372), the bin count, and the SHA-256 of
 the serialized bitmaps.
 
@@ -39,8 +40,9 @@ def main(log2n=27):
     R = oracle.RefIapi("c3", col, "<binning nbins=10000 equal-weight/>")
     print("reference build", round(time.time() - t, 1), flush=True)
     bounds = oracle.bounds_from_keys(R.keys)
+    ref_bounds = oracle.ref_bin_bounds("c3")
     tag = f"config3_2p{log2n}"
-    np.savez_compressed(os.path.join(HERE, tag + "_bounds.npz"), bounds=bounds)
+    np.savez_compressed(os.path.join(HERE, tag + "_bounds.npz"), bounds=bounds, ref_bounds=ref_bounds)
     meta = dict(rows=n, seed=SEED, spec="<binning nbins=10000 equal-weight/>",
                 nobs=int(len(R.keys) // 2), nwords=int(len(R.bitmaps)),
                 keys_sha256=hashlib.sha256(R.keys.tobytes()).hexdigest(),

@@ -179,6 +179,49 @@ def test_batch_matches_single(L):
     assert np.array_equal(hits, want) and np.array_equal(used, mixed)
 
 
+@pytest.mark.parametrize("kind", ["normal", "heavy", "zipf_int"])
+def test_bin_equal_weight(L, kind):
+    """Equal-weight bins built on the device (histogram on the GPU,
+    divideCounts on the host) give the oracle's words for those bounds."""
+    rng = np.random.default_rng(len(kind))
+    n = 31 * 6000 + 11
+    if kind == "normal":
+        col = rng.normal(size=n).astype(np.float32)
+    elif kind == "heavy":
+        col = np.where(rng.random(n) < 0.3, np.float32(0.25), rng.normal(size=n)).astype(np.float32)
+    else:
+        col = np.minimum(rng.zipf(1.3, size=n), 100000).astype(np.int32)
+    gi = L.Index.bin_equal_weight(col, 300)
+    b = gi.bounds()
+    assert b[-1] == np.finfo(np.float64).max and np.all(np.diff(b) > 0)
+    oi = O.OracleIndex.bin_bounds(col, b)
+    _same_words(gi, oi)
+    for (lo, lop, hi, rop) in _queries(rng, 0, 10, ints=False):
+        assert gi.evaluate(lo, lop, hi, rop, 1)[0] == oi.evaluate(lo, lop, hi, rop)[0]
+
+
+@pytest.mark.slow
+def test_bin_2p27_equal_weight_build(L):
+    """The 2^27-row float column binned with nbins=10000 on the device:
+    boundaries and words equal the reference's (tests/golden digests)."""
+    import hashlib
+    import json
+    import os
+    from libfastlib_amd import datagen as G
+    here = os.path.join(os.path.dirname(__file__), "golden")
+    meta = json.load(open(os.path.join(here, "config3_2p27.json")))
+    ref_bounds = np.load(os.path.join(here, "config3_2p27_bounds.npz"))["ref_bounds"]
+    col = G.float_mix(meta["seed"], meta["rows"], pool=100000, wnorm=0.7, s=1.2)
+    gi = L.Index.bin_equal_weight(col, 10000)
+    k, o, b = gi.deconstruct()
+    assert hashlib.sha256(k.tobytes()).hexdigest() == meta["keys_sha256"]
+    assert hashlib.sha256(o.tobytes()).hexdigest() == meta["offsets_sha256"]
+    assert hashlib.sha256(b.tobytes()).hexdigest() == meta["bitmaps_sha256"]
+    # the index drops empty inner bins, so compare the bounds it kept
+    kept = gi.bounds()
+    assert np.all(np.isin(kept, ref_bounds))
+
+
 @pytest.mark.slow
 def test_bin_2p27_reference_words(L):
     """The 2^27-row float index (10,000 equal-weight bins) built on the GPU is
