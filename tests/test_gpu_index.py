@@ -242,3 +242,30 @@ def test_bin_2p27_reference_words(L):
     assert hashlib.sha256(o.tobytes()).hexdigest() == meta["offsets_sha256"]
     assert len(b) == meta["nwords"]
     assert hashlib.sha256(b.tobytes()).hexdigest() == meta["bitmaps_sha256"]
+
+
+def test_config1_relic_2p20(L):
+    """2^20 int32 values uniform in [0,1024), relic index, 256 range terms:
+    index words, hit counts and row ids equal the reference's
+    (tests/golden/config1_relic_2p20.json) under every plan."""
+    import hashlib
+    import json
+    import os
+    import sys
+    here = os.path.join(os.path.dirname(__file__), "golden")
+    sys.path.insert(0, here)
+    from make_config1 import workload
+    meta = json.load(open(os.path.join(here, "config1_relic_2p20.json")))
+    col, lo, hi = workload()
+    gi = L.Index.relic(col)
+    k, o, b = gi.deconstruct()
+    assert hashlib.sha256(k.tobytes()).hexdigest() == meta["keys_sha256"]
+    assert hashlib.sha256(o.tobytes()).hexdigest() == meta["offsets_sha256"]
+    assert hashlib.sha256(b.tobytes()).hexdigest() == meta["bitmaps_sha256"]
+    want = np.array(meta["counts"], np.uint64)
+    for plan in (0, 1, 2, 3):
+        hits, _, _ = gi.evaluate_batch(lo, 2, hi, 1, plan)
+        assert np.array_equal(hits, want), plan
+    for i in range(0, 256, 4):
+        r = gi.rids(float(lo[i]), 2, float(hi[i]), 1, 1)
+        assert hashlib.sha256(r.astype(np.uint32).tobytes()).hexdigest() == meta["rids_sha256"][i]

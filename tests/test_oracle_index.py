@@ -208,3 +208,26 @@ def test_conjunction_of_one_sided_terms(ref, kind):
         rc, rr = R.query(*q)
         assert np.array_equal(want, rr), q
         assert np.array_equal(oi.evaluate_plan(*q)[1], rr), q
+
+
+def test_config1_relic_fixture():
+    """The oracle reproduces the reference's relic index and answers on the
+    2^20-row int32 workload (tests/golden/config1_relic_2p20.json)."""
+    import hashlib
+    import json
+    import os
+    import sys
+    here = os.path.join(os.path.dirname(__file__), "golden")
+    sys.path.insert(0, here)
+    from make_config1 import workload
+    meta = json.load(open(os.path.join(here, "config1_relic_2p20.json")))
+    col, lo, hi = workload()
+    oi = O.OracleIndex.relic_values(col)
+    k, o, b = oi.write()
+    assert hashlib.sha256(k.tobytes()).hexdigest() == meta["keys_sha256"]
+    assert hashlib.sha256(o.tobytes()).hexdigest() == meta["offsets_sha256"]
+    assert hashlib.sha256(b.tobytes()).hexdigest() == meta["bitmaps_sha256"]
+    for i in range(0, 256, 8):
+        c, r, _ = oi.evaluate_plan(float(lo[i]), 2, float(hi[i]), 1)
+        assert c == meta["counts"][i]
+        assert hashlib.sha256(r.astype(np.uint32).tobytes()).hexdigest() == meta["rids_sha256"][i]
