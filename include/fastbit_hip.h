@@ -231,6 +231,42 @@ int fbh_index_evaluate_batch(fbh_index* ix, uint32_t nq, const double* lo, const
 int fbh_index_query_bytes(const fbh_index* ix, double lo, int lop, double hi, int rop, int plan,
                           double* bytes);
 
+/* ---------------------------------------------------------------------- */
+/* Conjunctions: countQuery::evaluate of t[0] AND t[1] AND ... over columns of
+ * the same rows (src/countQuery.cpp:292-373, 929-1197).  Each term picks its
+ * plan as a single range does; terms are evaluated cheapest first, every
+ * tile stopping once it is empty. */
+
+typedef struct fbh_term {
+    const fbh_index* ix;   /* index (and column) of the term */
+    double lo;
+    int lop;               /* FBH_OP_* */
+    double hi;
+    int rop;
+} fbh_term;
+
+typedef struct fbh_conj fbh_conj;
+
+/* Prepares a batch: query q ANDs terms[qoff[q] .. qoff[q+1]).  The term
+ * descriptors go to the device once and the launch sequence (one
+ * persistent kernel over (query, tile) work items and one reduction) is
+ * captured in a hipGraph.  (No counterpart in the reference, which
+ * evaluates one query at a time; a batch API for many queries.) */
+int fbh_conj_prepare(const fbh_term* terms, const uint32_t* qoff, uint32_t nq, int plan,
+                     fbh_conj** out);
+/* Replays the batch: hits[q] (may be NULL) receives the counts, *gpu_ms
+ * (may be NULL) the device time of the replay. */
+int fbh_conj_run(fbh_conj* c, uint64_t* hits, float* gpu_ms);
+/* Autotuner: times the batch over block sizes, tiles per work item and
+ * grid sizes, keeps the fastest and returns it. */
+int fbh_conj_tune(fbh_conj* c, int* threads, uint32_t* blocks, uint32_t* tiles_per_item, float* best_ms);
+/* The plan each term runs (FBH_PLAN_LITERAL or FBH_PLAN_SCAN), in the
+ * caller's term order. */
+int fbh_conj_plans(const fbh_conj* c, int* plans);
+void fbh_conj_free(fbh_conj* c);
+/* One conjunction: *hits receives the count. */
+int fbh_conj_evaluate(const fbh_term* terms, uint32_t nterms, int plan, uint64_t* hits);
+
 #ifdef __cplusplus
 }
 #endif

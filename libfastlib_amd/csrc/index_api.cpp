@@ -955,6 +955,217 @@ int fbh_index_column_bytes(const fbh_index* ix, uint64_t* bytes) {
 
 } // extern "C"
 
+// ---------------------------------------------------------------------------
+// Conjunctions.
+struct fbh_conj {
+    uint32_t nq = 0, nterms = 0, ntiles = 0, groups = 0, anb = 0;
+    fbk::QueryArgs* dterms = nullptr;    // device term descriptors
+    uint32_t* dqoff = nullptr;
+    uint32_t* dnp = nullptr;             // ntiles per query (for the sums)
+    uint32_t* parts = nullptr;           // nq * ntiles
+    uint32_t* counts = nullptr;          // nq
+    std::vector<int> plans;              // plan of every term (caller's order)
+    std::vector<uint32_t> order;         // evaluation order: order[k] = caller's index
+    int nt = 256;                        // threads per block
+    uint32_t blocks = 0;                 // persistent grid
+    uint32_t tpi = 1;                    // tiles per work item
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~fbh_conj() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        dev_free(dterms); dev_free(dqoff); dev_free(dnp); dev_free(parts); dev_free(counts);
+    }
+};
+
+namespace {
+
+int cu_count() {
+    static int n = 0;
+    if (!n) {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+    }
+    return n;
+}
+
+int conj_launch(fbh_conj* c, hipStream_t s) {
+    fbk::ConjArgs a;
+    a.terms = c->dterms; a.qoff = c->dqoff; a.nq = c->nq; a.ntiles = c->ntiles;
+    a.groups = c->groups; a.anb = c->anb; a.tpi = c->tpi; a.part = c->parts; a.out = nullptr;
+    int rc = check(fbk::launch_conj(a, c->nt, c->blocks, s), "conjunctions");
+    if (!rc) rc = check(fbk::launch_sum_parts(c->parts, c->ntiles, c->dnp, c->nq, c->counts, s), "sum parts");
+    return rc;
+}
+
+// (Re)captures the launch sequence of the batch in a graph on the library
+// stream; a replay is then one graph launch.
+int conj_capture(fbh_conj* c) {
+    if (c->exec) { (void)hipGraphExecDestroy(c->exec); c->exec = nullptr; }
+    if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+    int rc = check(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal), "begin capture");
+    if (rc) return rc;
+    const int lrc = conj_launch(c, stream());
+    rc = check(hipStreamEndCapture(stream(), &c->graph), "end capture");
+    if (lrc) return lrc;
+    if (rc) return rc;
+    return check(hipGraphInstantiate(&c->exec, c->graph, nullptr, nullptr, 0), "graph instantiate");
+}
+
+int conj_run(fbh_conj* c, float* ms) {
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0); (void)hipEventCreate(&e1);
+    int rc = check(hipEventRecord(e0, stream()), "event");
+    if (!rc) rc = check(hipGraphLaunch(c->exec, stream()), "graph launch");
+    if (!rc) rc = check(hipEventRecord(e1, stream()), "event");
+    if (!rc) rc = check(hipEventSynchronize(e1), "event sync");
+    float t = 0.f;
+    if (!rc) (void)hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+    if (ms) *ms = t;
+    return rc;
+}
+
+} // namespace
+
+extern "C" {
+
+int fbh_conj_prepare(const fbh_term* terms, const uint32_t* qoff, uint32_t nq, int plan, fbh_conj** out) {
+    if (!terms || !qoff || !out || nq == 0) return fail(FBH_ERR_ARG, "bad argument");
+    if (plan < FBH_PLAN_AUTO || plan > FBH_PLAN_SCAN) return fail(FBH_ERR_ARG, "bad plan");
+    int rc = ensure_init();
+    if (rc) return rc;
+    for (uint32_t q = 0; q < nq; ++q)
+        if (qoff[q + 1] <= qoff[q]) return fail(FBH_ERR_ARG, "a query without terms");
+    const uint32_t nterms = qoff[nq];
+    const fbh_index* ix0 = terms[0].ix;
+    if (!ix0) return fail(FBH_ERR_ARG, "null index");
+    for (uint32_t k = 0; k < nterms; ++k) {
+        const fbh_term& t = terms[k];
+        if (!t.ix || t.ix->nrows != ix0->nrows) return fail(FBH_ERR_ARG, "terms over different row counts");
+        if (t.lop < 0 || t.lop > 2 || t.rop < 0 || t.rop > 2) return fail(FBH_ERR_ARG, "bad operator");
+    }
+    fbh_conj* c = new fbh_conj();
+    c->nq = nq; c->nterms = nterms; c->ntiles = ix0->ntiles; c->groups = ix0->groups; c->anb = ix0->anb;
+    // per query: the terms cheapest first (qExpr::reorder by query::weight,
+    // src/qExpr.cpp:1022; here by the bytes each term's plan reads)
+    c->plans.resize(nterms);
+    c->order.resize(nterms);
+    for (uint32_t q = 0; q < nq; ++q) {
+        std::vector<std::pair<double, uint32_t>> w;
+        for (uint32_t k = qoff[q]; k < qoff[q + 1]; ++k) {
+            const fbh_term& t = terms[k];
+            int p = plan == FBH_PLAN_AUTO ? plan_heuristic(t.ix, t.lo, t.lop, t.hi, t.rop) : plan;
+            if (p == FBH_PLAN_COMPRESSED) p = FBH_PLAN_LITERAL;
+            double bytes = 0;
+            fbh_index_query_bytes(t.ix, t.lo, t.lop, t.hi, t.rop, p, &bytes);
+            w.push_back({bytes, k});
+            c->plans[k] = p;
+        }
+        std::stable_sort(w.begin(), w.end());
+        for (uint32_t i = 0; i < (uint32_t)w.size(); ++i) c->order[qoff[q] + i] = w[i].second;
+    }
+    std::vector<fbk::QueryArgs> h(nterms);
+    for (uint32_t k = 0; k < nterms; ++k) {
+        const fbh_term& t = terms[c->order[k]];
+        fbk::QueryArgs& q = h[k];
+        fill_query(t.ix, q, t.lo, t.lop, t.hi, t.rop);
+        const int p = c->plans[c->order[k]];
+        q.type = dev_type(t.ix->type);
+        q.plan = p == FBH_PLAN_SCAN ? fbk::PLAN_SCAN_D : fbk::PLAN_LIT_D;
+        Locate L;
+        locate(t.ix, t.lo, t.lop, t.hi, t.rop, L);
+        const SumPlan sp = sum_plan(t.ix, L.hit0, L.hit1);
+        q.r0 = q.r1 = q.r2 = q.r3 = 0;
+        if (sp.kind == 1) { q.flip = 1; }
+        else if (sp.kind == 2) { q.r0 = L.hit0; q.r1 = L.hit1; }
+        else if (sp.kind == 3) { q.r0 = 0; q.r1 = L.hit0; q.r2 = L.hit1; q.r3 = t.ix->nobs; q.flip = 1; }
+        if (t.ix->kind == FBH_BIN) {
+            if (L.cand0 < L.hit0 && !t.ix->null_bitmap(L.cand0)) q.cand0 = L.cand0;
+            if (L.cand1 > L.hit1 && L.hit1 < t.ix->nobs && !t.ix->null_bitmap(L.hit1)) q.cand1 = L.hit1;
+        }
+    }
+    std::vector<uint32_t> np(nq, c->ntiles);
+    c->dterms = (fbk::QueryArgs*)dev_alloc(sizeof(fbk::QueryArgs) * nterms);
+    c->dqoff = (uint32_t*)dev_alloc(4ull * (nq + 1));
+    c->dnp = (uint32_t*)dev_alloc(4ull * nq);
+    c->parts = (uint32_t*)dev_alloc(4ull * nq * c->ntiles);
+    c->counts = (uint32_t*)dev_alloc(4ull * nq);
+    if (!c->dterms || !c->dqoff || !c->dnp || !c->parts || !c->counts) {
+        delete c;
+        return fail(FBH_ERR_NOMEM, "device allocation failed");
+    }
+    rc = to_dev(c->dterms, h.data(), nterms);
+    if (!rc) rc = to_dev(c->dqoff, qoff, nq + 1);
+    if (!rc) rc = to_dev(c->dnp, np.data(), nq);
+    if (!rc) rc = sync();
+    c->blocks = 4u * (uint32_t)cu_count();
+    if (!rc) rc = conj_capture(c);
+    if (rc) { delete c; return rc; }
+    *out = c;
+    return FBH_OK;
+}
+
+int fbh_conj_run(fbh_conj* c, uint64_t* hits, float* gpu_ms) {
+    if (!c) return fail(FBH_ERR_ARG, "null batch");
+    int rc = conj_run(c, gpu_ms);
+    if (rc || !hits) return rc;
+    std::vector<uint32_t> h(c->nq);
+    if ((rc = to_host(h.data(), c->counts, c->nq))) return rc;
+    for (uint32_t i = 0; i < c->nq; ++i) hits[i] = h[i];
+    return FBH_OK;
+}
+
+int fbh_conj_tune(fbh_conj* c, int* nt, uint32_t* blocks, uint32_t* tpi, float* best_ms) {
+    if (!c) return fail(FBH_ERR_ARG, "null batch");
+    static const int NTS[] = {128, 256, 512};
+    static const uint32_t TPIS[] = {1, 2, 4};
+    static const uint32_t BPC[] = {2, 4, 8};
+    int bnt = c->nt;
+    uint32_t bblocks = c->blocks, btpi = c->tpi;
+    float best = 0.f;
+    int rc = conj_run(c, &best);        // the current configuration, warmed
+    if (!rc) rc = conj_run(c, &best);
+    for (int a = 0; a < 3 && !rc; ++a)
+        for (int b = 0; b < 3 && !rc; ++b)
+            for (int d = 0; d < 3 && !rc; ++d) {
+                c->nt = NTS[a]; c->tpi = TPIS[b]; c->blocks = BPC[d] * (uint32_t)cu_count();
+                rc = conj_capture(c);
+                float ms = 0.f;
+                if (!rc) rc = conj_run(c, &ms);
+                if (!rc && ms < best) { best = ms; bnt = c->nt; bblocks = c->blocks; btpi = c->tpi; }
+            }
+    c->nt = bnt; c->blocks = bblocks; c->tpi = btpi;
+    if (!rc) rc = conj_capture(c);
+    if (nt) *nt = bnt;
+    if (blocks) *blocks = bblocks;
+    if (tpi) *tpi = btpi;
+    if (best_ms) *best_ms = best;
+    return rc;
+}
+
+int fbh_conj_plans(const fbh_conj* c, int* plans) {
+    if (!c || !plans) return fail(FBH_ERR_ARG, "null argument");
+    for (uint32_t k = 0; k < c->nterms; ++k) plans[k] = c->plans[k];
+    return FBH_OK;
+}
+
+void fbh_conj_free(fbh_conj* c) { delete c; }
+
+int fbh_conj_evaluate(const fbh_term* terms, uint32_t nterms, int plan, uint64_t* hits) {
+    if (!terms || nterms == 0 || !hits) return fail(FBH_ERR_ARG, "bad argument");
+    const uint32_t qoff[2] = {0, nterms};
+    fbh_conj* c = nullptr;
+    int rc = fbh_conj_prepare(terms, qoff, 1, plan, &c);
+    if (rc) return rc;
+    rc = fbh_conj_run(c, hits, nullptr);
+    fbh_conj_free(c);
+    return rc;
+}
+
+} // extern "C"
+
 namespace {
 
 // relic::estimateCost / bin::estimateCost (src/irelic.cpp:1857-1902,

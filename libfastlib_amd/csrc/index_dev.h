@@ -14,6 +14,8 @@ constexpr int SCAN_MAX_BLOCKS = 2048;    // blocks of the raw scan plan
 enum : int { COL_I32 = 0, COL_U32 = 1, COL_U16 = 2, COL_F32 = 3, COL_F64 = 4 };
 // comparison operators of one side of a range
 enum : int { OPC_NONE = 0, OPC_LT = 1, OPC_LE = 2 };
+// plans of a term inside a conjunction
+enum : int { PLAN_LIT_D = 1, PLAN_SCAN_D = 3 };
 
 struct QueryArgs {
     const uint32_t* tw;        // tiled copy of the bitmaps
@@ -32,6 +34,22 @@ struct QueryArgs {
     int64_t ilo, ihi;
     uint32_t* out;             // literal result, groups + 1 words
     uint32_t* part;            // ones per block (ntiles or scan_blocks entries)
+    int type;                  // column type (conjunction terms)
+    int plan;                  // PLAN_LIT_D or PLAN_SCAN_D (conjunction terms)
+};
+
+// A batch of conjunctions: query q ANDs terms[qoff[q] .. qoff[q+1]); all
+// terms are over columns of the same rows.  part[q*ntiles + t] receives the
+// ones of tile t; out (may be null) the literal result of a single query.
+struct ConjArgs {
+    const QueryArgs* terms;
+    const uint32_t* qoff;
+    uint32_t nq;
+    uint32_t ntiles;
+    uint32_t groups, anb;
+    uint32_t tpi;              // tiles per work item
+    uint32_t* part;
+    uint32_t* out;
 };
 
 hipError_t launch_keys_relic(const void* col, int type, uint32_t n, const double* vals, uint32_t nv,
@@ -74,6 +92,8 @@ hipError_t launch_tseg_fill(const uint32_t* words, const uint2* toff, const uint
                             uint32_t ntiles, uint32_t groups, uint32_t anb, uint32_t* tw, hipStream_t s);
 hipError_t launch_query_lit(const QueryArgs& q, int type, hipStream_t s);
 hipError_t launch_query_scan(const QueryArgs& q, int type, uint32_t nrows, hipStream_t s);
+// nt: threads per block (128, 256 or 512); blocks: persistent grid size
+hipError_t launch_conj(const ConjArgs& a, int nt, uint32_t blocks, hipStream_t s);
 uint32_t scan_blocks(uint32_t groups, uint32_t anb);
 // counts[i] = sum of parts[i*stride, i*stride + np[i])
 hipError_t launch_sum_parts(const uint32_t* parts, uint32_t stride, const uint32_t* np, uint32_t nq,

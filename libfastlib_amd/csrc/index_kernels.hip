@@ -532,35 +532,28 @@ __device__ void tile_ranges(const QueryArgs& q, uint32_t tile, uint32_t g0, uint
     }
 }
 
-template <class T>
-__global__ __launch_bounds__(QB_TPB) void k_query_lit(QueryArgs q) {
-    __shared__ word_t R[QT + 1];
-    __shared__ uint32_t cov[QT + 1];
-    __shared__ uint32_t red[QB_WAVES + 1];
-    __shared__ uint32_t next;
-    const uint32_t tile = blockIdx.x;
-    const bool last = tile + 1 == q.ntiles;
-    const uint32_t g0 = tile * (uint32_t)QT;
-    const uint32_t ng = last ? (q.groups - g0) + (q.anb ? 1u : 0u) : (uint32_t)QT;
-    const int w = threadIdx.x / WAVE;
-    for (uint32_t k = threadIdx.x; k <= QT; k += QB_TPB) { R[k] = 0u; cov[k] = 0u; }
-    if (threadIdx.x == 0) next = 0u;
-    __syncthreads();
+// Literal plan of one term over tile t into R[0, ng), by a block of NT
+// threads.  R, cov and *next are zero on entry; the call ends with a
+// barrier.  red needs NT/64 + 1 words.
+template <class T, int NT>
+__device__ void lit_tile(const QueryArgs& q, uint32_t tile, uint32_t g0, uint32_t ng, bool last, word_t* R,
+                         uint32_t* cov, uint32_t* red, uint32_t* next) {
     const T* col = (const T*)q.col;
+    const int w = threadIdx.x / WAVE;
     if (ng) {
-        if (last) tile_ranges<T, false>(q, tile, g0, ng, R, cov, &next, col);
-        else tile_ranges<T, true>(q, tile, g0, ng, R, cov, &next, col);
+        if (last) tile_ranges<T, false>(q, tile, g0, ng, R, cov, next, col);
+        else tile_ranges<T, true>(q, tile, g0, ng, R, cov, next, col);
     }
     __syncthreads();
     // coverage prefix sums: groups under some 1-fill are all ones
     {
-        constexpr int PER = QT / QB_TPB;
+        constexpr int PER = QT / NT;
         const uint32_t k0 = threadIdx.x * PER;
         uint32_t c[PER], sum = 0;
         #pragma unroll
         for (int i = 0; i < PER; ++i) { c[i] = cov[k0 + i]; sum += c[i]; }
         uint32_t tot;
-        uint32_t run = block_excl_sum<QB_TPB>(sum, red, &tot);
+        uint32_t run = block_excl_sum<NT>(sum, red, &tot);
         #pragma unroll
         for (int i = 0; i < PER; ++i) {
             run += c[i];
@@ -569,7 +562,7 @@ __global__ __launch_bounds__(QB_TPB) void k_query_lit(QueryArgs q) {
     }
     __syncthreads();
     if (q.flip) {
-        for (uint32_t k = threadIdx.x; k < ng; k += QB_TPB) {
+        for (uint32_t k = threadIdx.x; k < ng; k += NT) {
             const uint32_t g = g0 + k;
             const word_t mask = g < q.groups ? ALLONES : (((1u << q.anb) - 1u) << (31u - q.anb));
             R[k] ^= mask;
@@ -585,6 +578,58 @@ __global__ __launch_bounds__(QB_TPB) void k_query_lit(QueryArgs q) {
         }
     }
     __syncthreads();
+}
+
+// Scan plan of one term over tile t into R[0, ng): every wave takes 64
+// groups at a time (31 coalesced loads of 64 values, ballots staged in
+// LDS, each lane assembles its group).  bal needs NT/64 x 32 words of 64
+// bits.  Ends with a barrier.
+template <class T, int NT>
+__device__ void scan_tile(const QueryArgs& q, uint32_t g0, uint32_t ng, word_t* R, uint64_t (*bal)[32]) {
+    const T* col = (const T*)q.col;
+    const int lane = (int)lane_id();
+    const int w = threadIdx.x / WAVE;
+    const uint32_t nrows = q.groups * 31u + q.anb;
+    for (uint32_t k0 = (uint32_t)w * 64u; k0 < ng; k0 += (NT / WAVE) * 64u) {
+        const uint32_t rbase = (g0 + k0) * 31u;
+        #pragma unroll 4
+        for (int k = 0; k < 31; ++k) {
+            const uint32_t row = rbase + (uint32_t)k * 64u + lane;
+            const bool hit = row < nrows && in_range(q, col[row]);
+            const uint64_t m = __ballot(hit);
+            if (lane == 0) bal[w][k] = m;
+        }
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t k = k0 + (uint32_t)lane;
+        if (k < ng) {
+            const uint32_t bit0 = (uint32_t)lane * 31u;
+            const uint32_t kk = bit0 >> 6, s0 = bit0 & 63u;
+            uint64_t lo = bal[w][kk] >> s0;
+            if (s0 > 33u && kk + 1 < 31u) lo |= bal[w][kk + 1] << (64u - s0);
+            word_t v = __brev((uint32_t)(lo & 0x7FFFFFFFull)) >> 1;
+            if (g0 + k == q.groups) v &= (((1u << q.anb) - 1u) << (31u - q.anb));
+            R[k] = v;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+}
+
+template <class T>
+__global__ __launch_bounds__(QB_TPB) void k_query_lit(QueryArgs q) {
+    __shared__ word_t R[QT + 1];
+    __shared__ uint32_t cov[QT + 1];
+    __shared__ uint32_t red[QB_WAVES + 1];
+    __shared__ uint32_t next;
+    const uint32_t tile = blockIdx.x;
+    const bool last = tile + 1 == q.ntiles;
+    const uint32_t g0 = tile * (uint32_t)QT;
+    const uint32_t ng = last ? (q.groups - g0) + (q.anb ? 1u : 0u) : (uint32_t)QT;
+    const int w = threadIdx.x / WAVE;
+    for (uint32_t k = threadIdx.x; k <= QT; k += QB_TPB) { R[k] = 0u; cov[k] = 0u; }
+    if (threadIdx.x == 0) next = 0u;
+    __syncthreads();
+    lit_tile<T, QB_TPB>(q, tile, g0, ng, last, R, cov, red, &next);
     uint32_t ones = 0;
     for (uint32_t k = threadIdx.x; k < ng; k += QB_TPB) {
         const word_t v = R[k];
@@ -598,6 +643,94 @@ __global__ __launch_bounds__(QB_TPB) void k_query_lit(QueryArgs q) {
         uint32_t t = 0;
         for (int i = 0; i < QB_WAVES; ++i) t += red[i];
         q.part[tile] = t;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Conjunctions (countQuery::doEvaluate, LOGICAL_AND, src/countQuery.cpp:
+// 953-963): the result of a query is the AND of its terms' results.  One
+// persistent kernel takes work items (query, TPI consecutive tiles) in
+// grid-stride order; per tile the block evaluates the first term into R,
+// every further term into R2 and R &= R2, and stops early once the tile is
+// empty.  Each term runs the plan chosen for it on the host (literal
+// through the tiled copy of its index, or the raw scan of its column).
+template <int NT>
+__device__ void term_tile(const QueryArgs& q, uint32_t tile, uint32_t g0, uint32_t ng, bool last, word_t* R,
+                          uint32_t* cov, uint32_t* red, uint32_t* next, uint64_t (*bal)[32]) {
+    if (q.plan == PLAN_SCAN_D) {
+        switch (q.type) {
+        case COL_I32: scan_tile<int32_t, NT>(q, g0, ng, R, bal); break;
+        case COL_U32: scan_tile<uint32_t, NT>(q, g0, ng, R, bal); break;
+        case COL_U16: scan_tile<uint16_t, NT>(q, g0, ng, R, bal); break;
+        case COL_F32: scan_tile<float, NT>(q, g0, ng, R, bal); break;
+        default: scan_tile<double, NT>(q, g0, ng, R, bal); break;
+        }
+        return;
+    }
+    for (uint32_t k = threadIdx.x; k <= QT; k += NT) { R[k] = 0u; cov[k] = 0u; }
+    if (threadIdx.x == 0) *next = 0u;
+    __syncthreads();
+    switch (q.type) {
+    case COL_I32: lit_tile<int32_t, NT>(q, tile, g0, ng, last, R, cov, red, next); break;
+    case COL_U32: lit_tile<uint32_t, NT>(q, tile, g0, ng, last, R, cov, red, next); break;
+    case COL_U16: lit_tile<uint16_t, NT>(q, tile, g0, ng, last, R, cov, red, next); break;
+    case COL_F32: lit_tile<float, NT>(q, tile, g0, ng, last, R, cov, red, next); break;
+    default: lit_tile<double, NT>(q, tile, g0, ng, last, R, cov, red, next); break;
+    }
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void k_conj(ConjArgs a) {
+    __shared__ word_t R[QT + 1], R2[QT + 1];
+    __shared__ uint32_t cov[QT + 1];
+    __shared__ uint32_t red[NT / WAVE + 1];
+    __shared__ uint64_t bal[NT / WAVE][32];
+    __shared__ uint32_t next;
+    const uint32_t nchunk = (a.ntiles + a.tpi - 1) / a.tpi;
+    const uint32_t nitems = a.nq * nchunk;
+    const int w = threadIdx.x / WAVE;
+    for (uint32_t item = blockIdx.x; item < nitems; item += gridDim.x) {
+        const uint32_t qi = item / nchunk;
+        const uint32_t c = item - qi * nchunk;
+        const uint32_t k0 = a.qoff[qi], k1 = a.qoff[qi + 1];
+        for (uint32_t tile = c * a.tpi; tile < min((c + 1) * a.tpi, a.ntiles); ++tile) {
+            const bool last = tile + 1 == a.ntiles;
+            const uint32_t g0 = tile * (uint32_t)QT;
+            const uint32_t ng = last ? (a.groups - g0) + (a.anb ? 1u : 0u) : (uint32_t)QT;
+            for (uint32_t k = k0; k < k1; ++k) {
+                term_tile<NT>(a.terms[k], tile, g0, ng, last, k == k0 ? R : R2, cov, red, &next, bal);
+                int any = 0;
+                if (k != k0) {
+                    for (uint32_t i = threadIdx.x; i < ng; i += NT) {
+                        const word_t v = R[i] & R2[i];
+                        R[i] = v;
+                        any |= v != 0u;
+                    }
+                } else {
+                    for (uint32_t i = threadIdx.x; i < ng; i += NT) any |= R[i] != 0u;
+                }
+                if (!__syncthreads_or(any)) {            // the tile is empty: skip the rest
+                    for (uint32_t i = threadIdx.x; i < ng; i += NT) R[i] = 0u;
+                    __syncthreads();
+                    break;
+                }
+            }
+            uint32_t ones = 0;
+            for (uint32_t i = threadIdx.x; i < ng; i += NT) {
+                const word_t v = R[i];
+                if (a.out) a.out[g0 + i] = v;
+                ones += __popc(v);
+            }
+            ones = wave_sum(ones);
+            if (lane_id() == 0) red[w] = ones;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                uint32_t t = 0;
+                for (int i = 0; i < NT / WAVE; ++i) t += red[i];
+                a.part[(size_t)qi * a.ntiles + tile] = t;
+            }
+            __syncthreads();
+        }
     }
 }
 
@@ -905,6 +1038,17 @@ hipError_t launch_query_lit(const QueryArgs& q, int type, hipStream_t s) {
     case COL_U16: hipLaunchKernelGGL(k_query_lit<uint16_t>, g, b, 0, s, q); break;
     case COL_F32: hipLaunchKernelGGL(k_query_lit<float>, g, b, 0, s, q); break;
     case COL_F64: hipLaunchKernelGGL(k_query_lit<double>, g, b, 0, s, q); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_conj(const ConjArgs& a, int nt, uint32_t blocks, hipStream_t s) {
+    if (!a.nq) return hipSuccess;
+    switch (nt) {
+    case 128: hipLaunchKernelGGL(k_conj<128>, dim3(blocks), dim3(128), 0, s, a); break;
+    case 256: hipLaunchKernelGGL(k_conj<256>, dim3(blocks), dim3(256), 0, s, a); break;
+    case 512: hipLaunchKernelGGL(k_conj<512>, dim3(blocks), dim3(512), 0, s, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -90,3 +90,39 @@ def uniform_incl_i64(seed, n, lo, hi):
     out = np.empty(n, np.int64)
     _lib().fbg_uniform_incl_i64(seed, n, lo, hi, out.ctypes.data)
     return out
+
+
+def table4(seed, n):
+    """The 4-column table of the conjunctive workload: uint16 uniform in
+    [0, 65536), int32 Zipf(1.1) over 10^6 values, float32 N(0,1), int32
+    sorted runs of geometric(1/512) length."""
+    return [uniform_i32(seed, n, 0, 65536).astype(np.uint16),
+            zipf_i32(seed + 1, n, 1.1, 1 << 20),
+            normal_f32(seed + 2, n),
+            sorted_runs_i32(seed + 3, n, 1.0 / 512)]
+
+
+def conj_queries(seed, cols, nq, min_terms=2, max_terms=4, smin=1e-6, smax=0.3):
+    """nq conjunctions of 2..4 range terms lo <= x < hi on distinct columns,
+    each term's selectivity drawn log-uniform in [smin, smax]; endpoints are
+    the column's empirical quantiles.  Returns [[(col_index, lo, hi), ...]]."""
+    srt = [np.sort(c) for c in cols]
+    n = len(cols[0])
+    u = uniform_f64(seed, nq * 16)
+    k = 0
+    out = []
+    for _ in range(nq):
+        nt = min_terms + int(u[k] * (max_terms - min_terms + 1)); k += 1
+        nt = min(nt, max_terms)
+        order = np.argsort(u[k:k + len(cols)]); k += len(cols)
+        q = []
+        for ci in order[:nt]:
+            sel = np.exp(np.log(smin) + u[k] * (np.log(smax) - np.log(smin))); k += 1
+            start = u[k] * (1.0 - sel); k += 1
+            a = srt[ci][min(int(start * n), n - 1)]
+            b = srt[ci][min(int((start + sel) * n), n - 1)]
+            if b <= a:
+                b = a + (1 if srt[ci].dtype.kind in "iu" else np.spacing(np.float32(abs(a) + 1)))
+            q.append((int(ci), float(a), float(b)))
+        out.append(q)
+    return out

@@ -418,3 +418,79 @@ class Index:
         v = C.c_uint64()
         check(self._lib().fbh_index_column_bytes(self._h, C.byref(v)), "fbh_index_column_bytes")
         return v.value
+
+
+# ---------------------------------------------------------------------------
+# Conjunctions
+class _Term(C.Structure):
+    _fields_ = [("ix", C.c_void_p), ("lo", C.c_double), ("lop", C.c_int), ("hi", C.c_double),
+                ("rop", C.c_int)]
+
+
+def _conj_sigs(L):
+    vp = C.c_void_p
+    for name, args in {
+        "fbh_conj_prepare": [C.POINTER(_Term), _u32p, C.c_uint32, C.c_int, C.POINTER(vp)],
+        "fbh_conj_run": [vp, _u64p, C.POINTER(C.c_float)],
+        "fbh_conj_tune": [vp, C.POINTER(C.c_int), _u32p, _u32p, C.POINTER(C.c_float)],
+        "fbh_conj_plans": [vp, C.POINTER(C.c_int)],
+        "fbh_conj_evaluate": [C.POINTER(_Term), C.c_uint32, C.c_int, _u64p],
+    }.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    L.fbh_conj_free.argtypes = [vp]
+    L.fbh_conj_free.restype = None
+
+
+class Conjunctions:
+    """A prepared batch of conjunctive queries (fbh_conj_prepare): queries
+    is a list of lists of (Index, lo, lop, hi, rop) terms."""
+
+    def __init__(self, queries, plan=PLAN_AUTO):
+        L = Index._lib()
+        if not getattr(L, "_conj_sigs", False):
+            _conj_sigs(L)
+            L._conj_sigs = True
+        self._L = L
+        self._keep = []
+        terms, qoff = [], [0]
+        for q in queries:
+            for (ix, lo, lop, hi, rop) in q:
+                terms.append(_Term(ix._h, float(lo), int(lop), float(hi), int(rop)))
+                self._keep.append(ix)
+            qoff.append(len(terms))
+        self.nq = len(queries)
+        self.nterms = len(terms)
+        arr = (_Term * len(terms))(*terms)
+        off = np.ascontiguousarray(qoff, np.uint32)
+        h = C.c_void_p()
+        check(L.fbh_conj_prepare(arr, off.ctypes.data_as(_u32p), self.nq, int(plan), C.byref(h)),
+              "fbh_conj_prepare")
+        self._h = h
+
+    def run(self):
+        """(hits uint64[nq], device milliseconds of the replay)."""
+        hits = np.zeros(self.nq, np.uint64)
+        ms = C.c_float()
+        check(self._L.fbh_conj_run(self._h, hits.ctypes.data_as(_u64p), C.byref(ms)), "fbh_conj_run")
+        return hits, ms.value
+
+    def tune(self):
+        nt, blocks, tpi, ms = C.c_int(), C.c_uint32(), C.c_uint32(), C.c_float()
+        check(self._L.fbh_conj_tune(self._h, C.byref(nt), C.byref(blocks), C.byref(tpi), C.byref(ms)),
+              "fbh_conj_tune")
+        return dict(threads=nt.value, blocks=blocks.value, tiles_per_item=tpi.value, ms=ms.value)
+
+    def plans(self):
+        out = np.zeros(self.nterms, np.int32)
+        check(self._L.fbh_conj_plans(self._h, out.ctypes.data_as(C.POINTER(C.c_int))), "fbh_conj_plans")
+        return out
+
+    def __del__(self):
+        try:
+            if self._h is not None:
+                self._L.fbh_conj_free(self._h)
+        except Exception:
+            pass
+        self._h = None

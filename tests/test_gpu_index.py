@@ -269,3 +269,48 @@ def test_config1_relic_2p20(L):
     for i in range(0, 256, 4):
         r = gi.rids(float(lo[i]), 2, float(hi[i]), 1, 1)
         assert hashlib.sha256(r.astype(np.uint32).tobytes()).hexdigest() == meta["rids_sha256"][i]
+
+
+def _table_indexes(L, cols):
+    return [L.Index.relic(cols[0]), L.Index.bin_equal_weight(cols[1], 500),
+            L.Index.bin_equal_weight(cols[2], 500), L.Index.relic(cols[3])]
+
+
+def _oracle_indexes(gis, cols):
+    out = []
+    for gi, c in zip(gis, cols):
+        k, o, b = gi.deconstruct()
+        if len(k) == len(o) - 1:
+            out.append(O.OracleIndex.relic_values(c))
+        else:
+            out.append(O.OracleIndex.bin_bounds(c, gi.bounds()))
+    return out
+
+
+@pytest.mark.parametrize("plan", [0, 1, 3])
+def test_conjunctions(L, plan):
+    """AND of 2-4 range terms over four columns (relic and bin indexes):
+    counts equal the intersection of the oracle's per-term answers under
+    the plan each term ran."""
+    from libfastlib_amd import datagen as G
+    n = 31 * 3000 + 17
+    cols = G.table4(21, n)
+    gis = _table_indexes(L, cols)
+    ois = _oracle_indexes(gis, cols)
+    qs = G.conj_queries(22, cols, 48)
+    batch = L.Conjunctions([[(gis[ci], lo, 2, hi, 1) for (ci, lo, hi) in q] for q in qs], plan)
+    hits, _ = batch.run()
+    plans = batch.plans()
+    k = 0
+    for qi, q in enumerate(qs):
+        r = None
+        for (ci, lo, hi) in q:
+            p = int(plans[k]); k += 1
+            assert p in (1, 3)
+            rr = ois[ci].evaluate_plan(lo, 2, hi, 1, plan=p)[1]
+            r = rr if r is None else np.intersect1d(r, rr)
+        assert hits[qi] == len(r), (qi, q)
+    cfg = batch.tune()
+    assert cfg["threads"] in (128, 256, 512)
+    hits2, _ = batch.run()
+    assert np.array_equal(hits, hits2)
