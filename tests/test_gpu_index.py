@@ -314,3 +314,22 @@ def test_conjunctions(L, plan):
     assert cfg["threads"] in (128, 256, 512)
     hits2, _ = batch.run()
     assert np.array_equal(hits, hits2)
+
+
+def test_query_count_kat(L):
+    """The reference's tests/scripts/query-count.list conjunctions of range
+    terms (tests/golden/kat_query_count.json) through the batched
+    conjunction engine, relic and bin indexes."""
+    import json
+    import os
+    d = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "kat_query_count.json")))
+    for kind in ("relic", "bin"):
+        idx = {}
+        for t, cs in d["tables"].items():
+            for c, v in cs.items():
+                a = np.array(v["values"], np.float32 if v["type"] == "float32" else np.int32)
+                idx[(t, c)] = L.Index.relic(a) if kind == "relic" else L.Index.bin_equal_weight(a, 7)
+        qs = [[(idx[(q["table"], c)], lo, lop, hi, rop) for (c, lo, lop, hi, rop) in q["terms"]]
+              for q in d["queries"]]
+        hits, _ = L.Conjunctions(qs).run()
+        assert [int(h) for h in hits] == [q["count"] for q in d["queries"]], kind

@@ -231,3 +231,28 @@ def test_config1_relic_fixture():
         c, r, _ = oi.evaluate_plan(float(lo[i]), 2, float(hi[i]), 1)
         assert c == meta["counts"][i]
         assert hashlib.sha256(r.astype(np.uint32).tobytes()).hexdigest() == meta["rids_sha256"][i]
+
+
+def _kat():
+    import json
+    import os
+    d = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "kat_query_count.json")))
+    cols = {}
+    for t, cs in d["tables"].items():
+        for c, v in cs.items():
+            cols[(t, c)] = np.array(v["values"], np.float32 if v["type"] == "float32" else np.int32)
+    return cols, d["queries"]
+
+
+def test_query_count_kat():
+    """The reference's tests/scripts/query-count.list entries that are
+    conjunctions of range terms: the oracle's relic indexes give the listed
+    counts (each term under the reference's own plan choice)."""
+    cols, queries = _kat()
+    idx = {k: O.OracleIndex.relic_values(v) for k, v in cols.items()}
+    for q in queries:
+        r = None
+        for (c, lo, lop, hi, rop) in q["terms"]:
+            rr = idx[(q["table"], c)].evaluate_plan(lo, lop, hi, rop)[1]
+            r = rr if r is None else np.intersect1d(r, rr)
+        assert len(r) == q["count"], q["text"]
