@@ -100,6 +100,39 @@ def wah_cells(L, reps=40, copies=4):
     return out
 
 
+def conj_section(L, log2n, nq, steps):
+    """4 columns of 2^log2n rows (uint16 uniform -> relic; int32 Zipf(1.1),
+    float32 N(0,1), int32 sorted runs -> 10000 equal-weight bins), nq
+    queries ANDing 2-4 range terms of selectivity log-uniform in
+    [1e-6, 0.3]; one prepared, autotuned, graph-captured batch."""
+    from libfastlib_amd import datagen as G
+    n = 1 << log2n
+    t = time.time()
+    cols = G.table4(41, n)
+    qs = G.conj_queries(42, cols, nq)
+    log(f"conj table: {time.time() - t:.1f}s")
+    t = time.time()
+    ixs = [L.Index.relic(cols[0]), L.Index.bin_equal_weight(cols[1], 10000),
+           L.Index.bin_equal_weight(cols[2], 10000), L.Index.bin_equal_weight(cols[3], 10000)]
+    L.lib().fbh_sync()
+    build_s = time.time() - t
+    batch = L.Conjunctions([[(ixs[ci], lo, 2, hi, 1) for (ci, lo, hi) in q] for q in qs])
+    cfg = batch.tune()
+    batch.run()
+    ms = []
+    for _ in range(steps):
+        hits, m = batch.run()
+        ms.append(m)
+    plans = batch.plans()
+    out = dict(rows=n, queries=nq, terms=int(batch.nterms), index_build_s=round(build_s, 2),
+               qps=round(nq / (np.median(ms) * 1e-3), 1), ms_per_batch=round(float(np.median(ms)), 3),
+               tuned=cfg, term_plans={"literal": int(np.sum(plans == 1)), "scan": int(np.sum(plans == 3))},
+               hits_total=int(hits.sum()))
+    log("conj", out)
+    del ixs
+    return out
+
+
 def cpu_baseline(L, ix, col, lo, hi, hits_gpu, sample):
     import oracle
     if oracle.reference() is None:
@@ -129,6 +162,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=16)
     ap.add_argument("--no-wah", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-conj", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="finish with 5 device-to-device copies of MB MiB (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
@@ -186,6 +220,7 @@ def main():
                     avg_launch_us=round(per_plan[dom]["gpu_ms"] * 1e3 / len(lo), 2))
 
     wah = None if args.no_wah else wah_cells(L)
+    conj = None if args.no_conj else conj_section(L, 26, 4096, 5)
     cpu = None if args.no_cpu else cpu_baseline(L, ix, col, lo, hi, hits, args.cpu_sample)
 
     out = {
@@ -211,6 +246,7 @@ def main():
         "best_plan_bytes_model_GB": round(float(np.minimum(bl, bs).sum()) / 1e9, 3),
         "roofline": roofline,
         "wah_ops_2p28": wah,
+        "conjunctions_4x2p26": conj,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

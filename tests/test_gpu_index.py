@@ -329,7 +329,8 @@ def test_query_count_kat(L):
             for c, v in cs.items():
                 a = np.array(v["values"], np.float32 if v["type"] == "float32" else np.int32)
                 idx[(t, c)] = L.Index.relic(a) if kind == "relic" else L.Index.bin_equal_weight(a, 7)
-        qs = [[(idx[(q["table"], c)], lo, lop, hi, rop) for (c, lo, lop, hi, rop) in q["terms"]]
-              for q in d["queries"]]
-        hits, _ = L.Conjunctions(qs).run()
-        assert [int(h) for h in hits] == [q["count"] for q in d["queries"]], kind
+        for table in d["tables"]:      # a batch covers the rows of one table
+            sel = [q for q in d["queries"] if q["table"] == table]
+            qs = [[(idx[(table, c)], lo, lop, hi, rop) for (c, lo, lop, hi, rop) in q["terms"]] for q in sel]
+            hits, _ = L.Conjunctions(qs).run()
+            assert [int(h) for h in hits] == [q["count"] for q in sel], (kind, table)
