@@ -42,7 +42,9 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def config3(log2n, nq, seed=3):
+def config3(log2n, nq, seed=3, extra=()):
+    """The column, its nq quantile queries (lo, hi), and one more (lo, hi)
+    query set per (seed, count) in `extra`."""
     from libfastlib_amd import datagen as G
     n = 1 << log2n
     t = time.time()
@@ -50,13 +52,17 @@ def config3(log2n, nq, seed=3):
     log(f"column: {n} rows in {time.time() - t:.1f}s")
     t = time.time()
     srt = np.sort(col)
-    u = G.uniform_f64(seed + 1000, 2 * nq)
-    a = srt[np.minimum((u[:nq] * n).astype(np.int64), n - 1)].astype(np.float64)
-    b = srt[np.minimum((u[nq:] * n).astype(np.int64), n - 1)].astype(np.float64)
-    lo, hi = np.minimum(a, b), np.maximum(a, b)
-    log(f"queries: {nq} in {time.time() - t:.1f}s")
+
+    def quantile_ranges(qseed, k):
+        u = G.uniform_f64(qseed, 2 * k)
+        a = srt[np.minimum((u[:k] * n).astype(np.int64), n - 1)].astype(np.float64)
+        b = srt[np.minimum((u[k:] * n).astype(np.int64), n - 1)].astype(np.float64)
+        return np.minimum(a, b), np.maximum(a, b)
+    lo, hi = quantile_ranges(seed + 1000, nq)
+    more = [quantile_ranges(s, k) for s, k in extra]
+    log(f"queries: {nq} + {sum(k for _, k in extra)} in {time.time() - t:.1f}s")
     del srt
-    return col, lo, hi
+    return col, lo, hi, more
 
 
 def measured_traffic(kernel):
@@ -133,6 +139,86 @@ def conj_section(L, log2n, nq, steps):
     return out
 
 
+def learned_section(L, ix, train, test, epochs):
+    """The learned plan choice on the same index: every query of a training
+    set and of a test set timed under the three plans by the engine, the
+    MLP trained on the device, then regret (label times) and end-to-end
+    batch throughput of the learned plans against the ibis heuristic, each
+    fixed plan and the per-query best."""
+    N = L.native
+    (lo_tr, hi_tr), (lo_te, hi_te) = train, test
+    t = time.time()
+    ms_tr, _ = ix.time_plans(lo_tr, 2, hi_tr, 1, cap=2.0)
+    label_s = time.time() - t
+    log(f"learned: timed {len(lo_tr)} training queries x 3 plans in {label_s:.1f}s")
+    x_tr = ix.plan_features(lo_tr, 2, hi_tr, 1)
+    y, m = N.plan_labels(ms_tr)
+    P = N.Planner(seed=2024)
+    t = time.time()
+    hist = P.train(x_tr, y, m, epochs=epochs, batch=256, lr=2e-3, seed=7)
+    train_s = time.time() - t
+    log(f"learned: trained {epochs} epochs in {train_s:.2f}s, loss {hist[0]:.3f} -> {hist[-1]:.3f}")
+
+    ms, hits = ix.time_plans(lo_te, 2, hi_te, 1, cap=2.0)
+    nq = len(lo_te)
+    done = ms[:, 1] > 0
+    identical = bool(np.array_equal(hits[:, 0], hits[:, 2]) and np.array_equal(hits[done, 1], hits[done, 0]))
+    if not identical:
+        raise RuntimeError("plans disagree on the test queries' counts")
+    t = time.perf_counter()
+    learned, nfall = ix.choose_plans(P, lo_te, 2, hi_te, 1)
+    choose_ms = (time.perf_counter() - t) * 1e3
+    heur = np.array([ix.estimate_cost(float(a), 2, float(b), 1)[2] for a, b in zip(lo_te, hi_te)], np.int32)
+    T = np.abs(ms.astype(np.float64))          # censored: the lower bound
+    best = np.argmin(T, axis=1).astype(np.int32) + 1
+    rows = np.arange(nq)
+
+    def label_total(plans):
+        return float(T[rows, plans - 1].sum())
+    tb = label_total(best)
+    lab = {"learned": label_total(learned), "heuristic": label_total(heur), "best": tb,
+           "literal": label_total(np.full(nq, 1)), "compressed_lower_bound": label_total(np.full(nq, 2)),
+           "scan": label_total(np.full(nq, 3))}
+    censored_learned = int(np.sum(ms[rows, learned - 1] < 0))
+
+    # end-to-end batches (warm second run); the learned wall includes the
+    # feature extraction and inference
+    def batch(plans=None, plan=0):
+        ix.evaluate_batch(lo_te, 2, hi_te, 1, plan=plan, plans=plans)
+        t0 = time.perf_counter()
+        h, _, _ = ix.evaluate_batch(lo_te, 2, hi_te, 1, plan=plan, plans=plans)
+        return h, time.perf_counter() - t0
+    h_ref, _ = batch(plan=1)
+    wall = {}
+    for name, kw in (("literal", dict(plan=1)), ("scan", dict(plan=3)), ("heuristic", dict(plan=0)),
+                     ("best", dict(plans=best)), ("learned", dict(plans=learned))):
+        h, w = batch(**kw)
+        if not np.array_equal(h, h_ref):
+            raise RuntimeError(f"{name} plans give other counts")
+        wall[name] = w
+    wall["learned"] += choose_ms * 1e-3
+    qps = {k: round(nq / v, 1) for k, v in wall.items()}
+    fixed_best = max(qps["literal"], qps["scan"])
+    out = dict(
+        train_queries=len(lo_tr), test_queries=nq, label_timing_s=round(label_s, 1),
+        epochs=epochs, train_s=round(train_s, 2), loss_first=round(float(hist[0]), 4),
+        loss_last=round(float(hist[-1]), 4), choose_ms=round(choose_ms, 2), fallbacks=int(nfall),
+        plans_learned={N.PLAN_NAMES[k]: int(np.sum(learned == k)) for k in (1, 2, 3)},
+        plans_heuristic={N.PLAN_NAMES[k]: int(np.sum(heur == k)) for k in (1, 2, 3)},
+        plans_best={N.PLAN_NAMES[k]: int(np.sum(best == k)) for k in (1, 2, 3)},
+        censored_compressed=int(np.sum(~done)), learned_picks_censored=censored_learned,
+        label_ms=({k: round(v, 2) for k, v in lab.items()}),
+        regret_learned=round(lab["learned"] / tb - 1, 4),
+        regret_heuristic=round(lab["heuristic"] / tb - 1, 4),
+        batch_qps=qps,
+        learned_vs_heuristic=round(qps["learned"] / qps["heuristic"], 3),
+        learned_vs_best_fixed=round(qps["learned"] / fixed_best, 3),
+        learned_vs_scan_only=round(qps["learned"] / qps["scan"], 3),
+        counts_identical_all_plans=identical)
+    log("learned", out)
+    return out
+
+
 def cpu_baseline(L, ix, col, lo, hi, hits_gpu, sample):
     import oracle
     if oracle.reference() is None:
@@ -163,6 +249,10 @@ def main():
     ap.add_argument("--no-wah", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-conj", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--train-queries", type=int, default=50000)
+    ap.add_argument("--test-queries", type=int, default=8192)
+    ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="finish with 5 device-to-device copies of MB MiB (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
@@ -174,7 +264,8 @@ def main():
 
     import libfastlib_amd as L
     L.lib()
-    col, lo, hi = config3(args.log2_rows, args.queries)
+    extra = () if args.no_learned else ((5005, args.train_queries), (5006, args.test_queries))
+    col, lo, hi, more = config3(args.log2_rows, args.queries, extra=extra)
     t = time.time()
     ix = L.Index.bin_equal_weight(col, 10000)     # <binning nbins=10000 equal-weight/>
     L.lib().fbh_sync()
@@ -219,6 +310,7 @@ def main():
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(dom_kernel),
                     avg_launch_us=round(per_plan[dom]["gpu_ms"] * 1e3 / len(lo), 2))
 
+    learned = None if args.no_learned else learned_section(L, ix, more[0], more[1], args.epochs)
     wah = None if args.no_wah else wah_cells(L)
     conj = None if args.no_conj else conj_section(L, 26, 4096, 5)
     cpu = None if args.no_cpu else cpu_baseline(L, ix, col, lo, hi, hits, args.cpu_sample)
@@ -245,6 +337,7 @@ def main():
         "per_plan": per_plan,
         "best_plan_bytes_model_GB": round(float(np.minimum(bl, bs).sum()) / 1e9, 3),
         "roofline": roofline,
+        "learned_plan": learned,
         "wah_ops_2p28": wah,
         "conjunctions_4x2p26": conj,
         "cpu_baseline": cpu,

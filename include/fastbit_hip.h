@@ -267,6 +267,79 @@ void fbh_conj_free(fbh_conj* c);
 /* One conjunction: *hits receives the count. */
 int fbh_conj_evaluate(const fbh_term* terms, uint32_t nterms, int plan, uint64_t* hits);
 
+/* ---------------------------------------------------------------------- */
+/* Learned plan choice.  The reference picks a plan with fixed cost rules
+ * (column::evaluateRange, src/column.cpp:6102-6111; bin::sumBins,
+ * src/index.cpp:6640-6783); this is a learned stand-in for that choice,
+ * with no counterpart in the reference.  Every plan yields the same bits,
+ * so the choice affects only the time.
+ *
+ * Features of a range query (FBH_NFEATURES floats, from index metadata):
+ *   0 log2(1 + words of the bitmaps in the hit range)
+ *   1 log2(1 + words of the bitmaps outside it)
+ *   2 log2(1 + words the literal plan reads, edge bitmaps included)
+ *   3 bin span: hit bitmaps / bitmaps
+ *   4 density: rows of the hit bitmaps / rows
+ *   5 log2(1 + rows of the edge bitmaps)
+ *   6 log2(1 + bitmaps in the OR set)
+ *   7 1 when the OR set is the complement of the hit range
+ *   8 log2(1 + rows)   9 log2(1 + bitmaps)
+ *  10 1 for a floating-point column   11 element size in bytes */
+#define FBH_NFEATURES 12
+#define FBH_NPLANS 3
+int fbh_index_plan_features(const fbh_index* ix, uint32_t nq, const double* lo, const int* lop,
+                            const double* hi, const int* rop, float* feats);
+/* Times every query under the three plans, back to back on the device:
+ * ms[3*i + plan-1] is the device time of query i under the literal and
+ * scan plans and the host wall time of the compressed plan (which
+ * synchronises between its operations).  The compressed plan stops once
+ * it runs cap times longer than the slower of the other two, and does not
+ * start when its ORs times the fastest device round trip already exceed
+ * that; its time is then stored negated (a lower bound) and its hit count
+ * as UINT64_MAX.  hits[3*i + plan-1] receives the counts. */
+int fbh_index_time_plans(fbh_index* ix, uint32_t nq, const double* lo, const int* lop, const double* hi,
+                         const int* rop, float cap, float* ms, uint64_t* hits);
+
+/* The cost model: an FP32 MLP 12 -> 64 -> 64 -> 3 (ReLU) whose outputs are
+ * the predicted log2 device times (microseconds) of the literal,
+ * compressed and scan plans; trained and run by the library's own kernels.
+ * Parameters, flat: W1[64][12] b1[64] W2[64][64] b2[64] W3[3][64] b3[3]. */
+typedef struct fbh_planner fbh_planner;
+#define FBH_PLANNER_NPARAM 5187
+/* He-uniform weights from seed, zero biases, identity normalisation. */
+int fbh_planner_new(uint64_t seed, fbh_planner** out);
+void fbh_planner_free(fbh_planner* p);
+int fbh_planner_set_params(fbh_planner* p, const float* params, uint32_t n);
+int fbh_planner_get_params(const fbh_planner* p, float* params, uint32_t n);
+/* Input normalisation x' = (x - mu) / sd (sd <= 0 counts as 1). */
+int fbh_planner_set_norm(fbh_planner* p, const float* mu, const float* sd);
+int fbh_planner_get_norm(const fbh_planner* p, float* mu, float* sd);
+/* Scores of n samples (x: n x 12, out: n x 3). */
+int fbh_planner_forward(fbh_planner* p, const float* x, uint32_t n, float* out);
+/* Loss and gradient over n samples at the current parameters:
+ * L = sum_i sum_o w_io e_io^2 / sum |mask|, e = score - y, where mask > 0
+ * is an exact label of weight mask, mask < 0 a lower bound of weight -mask
+ * (e counts only when negative), 0 no label. */
+int fbh_planner_grad(fbh_planner* p, const float* x, const float* y, const float* mask, uint32_t n,
+                     float* loss, float* grad);
+/* Adam (beta 0.9 / 0.999, eps 1e-8) over shuffled mini-batches of the
+ * given size; normalisation and the in-range box of the features are set
+ * from x first.  epoch_loss[e] (may be NULL) receives each epoch's mean
+ * loss. */
+int fbh_planner_train(fbh_planner* p, const float* x, const float* y, const float* mask, uint32_t n,
+                      uint32_t epochs, uint32_t batch, float lr, uint64_t seed, float* epoch_loss);
+/* One Adam step on one batch of n samples (the test of the update). */
+int fbh_planner_step(fbh_planner* p, const float* x, const float* y, const float* mask, uint32_t n, float lr);
+/* Plans of n feature rows: the lowest score, or fallback[i] when a
+ * feature lies outside the training box (and for every row of an
+ * untrained planner).  *nfallback (may be NULL) counts the fallbacks. */
+int fbh_planner_choose(fbh_planner* p, const float* x, uint32_t n, const int* fallback, int* plans,
+                       uint32_t* nfallback);
+/* Features, the ibis heuristic's plan as the fallback and the learned
+ * choice for queries on one index. */
+int fbh_index_choose_plans(fbh_index* ix, fbh_planner* p, uint32_t nq, const double* lo, const int* lop,
+                           const double* hi, const int* rop, int* plans, uint32_t* nfallback);
+
 #ifdef __cplusplus
 }
 #endif

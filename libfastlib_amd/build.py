@@ -22,7 +22,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              "-Wno-unused-command-line-argument", "-I" + os.path.join(ROOT, "include")]
 
 SOURCES = ["wah_kernels.hip", "canon_kernels.hip", "index_kernels.hip", "runtime.cpp", "bitvector_api.cpp",
-           "index_api.cpp", "bin_bounds.cpp"]
+           "index_api.cpp", "bin_bounds.cpp", "mlp_kernels.hip", "planner_api.cpp"]
 
 
 def _run(cmd):

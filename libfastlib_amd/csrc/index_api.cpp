@@ -17,6 +17,7 @@
 #include "index_host.h"
 #include <algorithm>
 #include <cfloat>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -105,6 +106,7 @@ struct fbh_index {
     double* dbounds = nullptr;    // bin bounds (device)
     std::vector<double> cumbytes; // prefix sums of bytes(b)
     std::vector<double> cumser;   // prefix sums of serialized bytes
+    std::vector<double> cumrows;  // prefix sums of rows per bitmap
     bool offsets_built = false;   // the reference constructor fills offset64
     int cost_model = FBH_COST_IAPI;
     double cmin = DBL_MAX, cmax = -DBL_MAX;   // column min / max
@@ -252,6 +254,9 @@ int encode_bitmaps(fbh_index* ix, const uint32_t* skeys_d, const uint32_t* srows
     if (rc) return rc;
     if ((rc = sync())) return rc;
     // install
+    ix->cumrows.assign(nk + 1, 0.0);
+    for (uint32_t k = 0; k < nk; ++k)
+        ix->cumrows[k + 1] = ix->cumrows[k] + (double)(start[keep[k] + 1] - start[keep[k]]);
     ix->nobs = nk;
     ix->off = off;
     ix->nw = knw;
@@ -549,8 +554,29 @@ int view_aval(const fbh_index* ix, uint32_t b, uint32_t* aval, uint32_t* w0, uin
     return FBH_OK;
 }
 
-// Compressed plan: the OR chain through the WAH operator kernels.
-int eval_compressed(fbh_index* ix, double lo, int lop, double hi, int rop, fbh_bitvector* res) {
+using Clock = std::chrono::steady_clock;
+constexpr int CENSORED = 1;   // eval_compressed stopped at its deadline
+
+// Number of ORs eval_compressed performs for the range (bitmaps of the OR
+// set that are not null, plus one for the edge rows of a bin).
+uint32_t compressed_ops(const fbh_index* ix, double lo, int lop, double hi, int rop) {
+    Locate L;
+    locate(ix, lo, lop, hi, rop, L);
+    const SumPlan sp = sum_plan(ix, L.hit0, L.hit1);
+    auto nonnull = [&](uint32_t b0, uint32_t b1) {
+        uint32_t k = 0;
+        for (uint32_t b = b0; b < std::min(b1, ix->nobs); ++b) k += ix->null_bitmap(b) ? 0u : 1u;
+        return k;
+    };
+    uint32_t k = sp.kind == 2 ? nonnull(L.hit0, L.hit1) : sp.kind == 3 ? nonnull(0, L.hit0) + nonnull(L.hit1, ix->nobs) : 0u;
+    if (ix->kind == FBH_BIN && (L.cand0 < L.hit0 || (L.cand1 > L.hit1 && L.hit1 < ix->nobs))) ++k;
+    return k;
+}
+
+// Compressed plan: the OR chain through the WAH operator kernels.  With a
+// deadline, returns CENSORED once an operation ends after it.
+int eval_compressed(fbh_index* ix, double lo, int lop, double hi, int rop, fbh_bitvector* res,
+                    const Clock::time_point* deadline = nullptr) {
     Locate L;
     locate(ix, lo, lop, hi, rop, L);
     const SumPlan sp = sum_plan(ix, L.hit0, L.hit1);
@@ -567,6 +593,7 @@ int eval_compressed(fbh_index* ix, double lo, int lop, double hi, int rop, fbh_b
             v.d = nullptr;        // borrowed
             if (v.ck) { dev_free(v.ck); v.ck = nullptr; }
             if (r) return r;
+            if (deadline && Clock::now() > *deadline) return CENSORED;
         }
         return FBH_OK;
     };
@@ -951,6 +978,134 @@ int fbh_index_column_bytes(const fbh_index* ix, uint64_t* bytes) {
     if (!ix || !bytes) return fail(FBH_ERR_ARG, "null argument");
     *bytes = (uint64_t)col_esize(ix->type) * ix->nrows;
     return FBH_OK;
+}
+
+int fbh_index_plan_features(const fbh_index* ix, uint32_t nq, const double* lo, const int* lop,
+                            const double* hi, const int* rop, float* feats) {
+    if (!ix || (nq && (!lo || !lop || !hi || !rop || !feats))) return fail(FBH_ERR_ARG, "null argument");
+    const uint32_t nobs = ix->nobs;
+    auto words = [&](uint32_t b0, uint32_t b1) {
+        b1 = std::min(b1, nobs);
+        return b1 > b0 ? (ix->cumser[b1] - ix->cumser[b0]) / 4.0 : 0.0;
+    };
+    auto rows = [&](uint32_t b0, uint32_t b1) {
+        b1 = std::min(b1, nobs);
+        return b1 > b0 ? ix->cumrows[b1] - ix->cumrows[b0] : 0.0;
+    };
+    for (uint32_t i = 0; i < nq; ++i) {
+        Locate L;
+        locate(ix, lo[i], lop[i], hi[i], rop[i], L);
+        const uint32_t h0 = std::min(L.hit0, nobs), h1 = std::min(L.hit1, nobs);
+        const SumPlan sp = sum_plan(ix, h0, h1);
+        const double win = words(h0, h1), wout = words(0, h0) + words(h1, nobs);
+        double wread = sp.kind == 2 ? win : sp.kind == 3 ? wout : 0.0;
+        const double na = sp.kind == 2 ? (double)(h1 - h0) : sp.kind == 3 ? (double)(h0 + nobs - h1) : 0.0;
+        double erows = 0.0;
+        if (ix->kind == FBH_BIN) {
+            if (L.cand0 < L.hit0) { erows += rows(L.cand0, L.cand0 + 1); wread += words(L.cand0, L.cand0 + 1); }
+            if (L.cand1 > L.hit1 && L.hit1 < nobs) { erows += rows(L.hit1, L.hit1 + 1); wread += words(L.hit1, L.hit1 + 1); }
+        }
+        float* f = feats + (size_t)i * FBH_NFEATURES;
+        f[0] = (float)std::log2(1.0 + win);
+        f[1] = (float)std::log2(1.0 + wout);
+        f[2] = (float)std::log2(1.0 + wread);
+        f[3] = nobs ? (float)((double)(h1 - h0) / nobs) : 0.f;
+        f[4] = ix->nrows ? (float)(rows(h0, h1) / ix->nrows) : 0.f;
+        f[5] = (float)std::log2(1.0 + erows);
+        f[6] = (float)std::log2(1.0 + na);
+        f[7] = sp.kind == 3 ? 1.f : 0.f;
+        f[8] = (float)std::log2(1.0 + ix->nrows);
+        f[9] = (float)std::log2(1.0 + nobs);
+        f[10] = is_int_type(ix->type) ? 0.f : 1.f;
+        f[11] = (float)col_esize(ix->type);
+    }
+    return FBH_OK;
+}
+
+int fbh_index_time_plans(fbh_index* ix, uint32_t nq, const double* lo, const int* lop, const double* hi,
+                         const int* rop, float cap, float* ms, uint64_t* hits) {
+    if (!ix || (nq && (!lo || !lop || !hi || !rop || !ms || !hits))) return fail(FBH_ERR_ARG, "null argument");
+    for (uint32_t i = 0; i < nq; ++i)
+        if (lop[i] < 0 || lop[i] > 2 || rop[i] < 0 || rop[i] > 2) return fail(FBH_ERR_ARG, "bad operator");
+    int rc = ensure_init();
+    if (rc || nq == 0) return rc;
+    constexpr uint32_t CH = 256;                       // queries per event chunk
+    const uint32_t cmax = std::min(nq, CH);
+    DBuf<uint32_t> lit;
+    PartCounts pc;
+    if ((rc = lit.alloc(ix->groups + 1)) || (rc = pc.alloc(ix, cmax))) return rc;
+    std::vector<hipEvent_t> ev(cmax + 1);
+    for (auto& e : ev)
+        if ((rc = check(hipEventCreate(&e), "event"))) return rc;
+    std::vector<uint32_t> np(cmax), cnt(cmax);
+    const int dev_plans[2] = {FBH_PLAN_LITERAL, FBH_PLAN_SCAN};
+    for (int pi = 0; pi < 2 && !rc; ++pi) {
+        const int plan = dev_plans[pi];
+        // untimed warm-up of the plan's kernels
+        for (uint32_t i = 0; i < std::min(nq, 4u) && !rc; ++i)
+            rc = index_eval_literal(ix, lo[i], lop[i], hi[i], rop[i], plan, lit.p, pc.row(0), &np[0], stream());
+        if (!rc) rc = sync();
+        for (uint32_t c0 = 0; c0 < nq && !rc; c0 += CH) {
+            const uint32_t k = std::min(CH, nq - c0);
+            rc = check(hipEventRecord(ev[0], stream()), "event");
+            for (uint32_t i = 0; i < k && !rc; ++i) {
+                const uint32_t q = c0 + i;
+                rc = index_eval_literal(ix, lo[q], lop[q], hi[q], rop[q], plan, lit.p, pc.row(i), &np[i], stream());
+                if (!rc) rc = check(hipEventRecord(ev[i + 1], stream()), "event");
+            }
+            if (!rc) rc = pc.launch_sum(np.data(), k);
+            if (!rc) rc = sync();
+            if (!rc) rc = to_host(cnt.data(), pc.counts.p, k);
+            for (uint32_t i = 0; i < k && !rc; ++i) {
+                float t = 0.f;
+                rc = check(hipEventElapsedTime(&t, ev[i], ev[i + 1]), "elapsed");
+                ms[3 * (size_t)(c0 + i) + (plan - 1)] = t;
+                hits[3 * (size_t)(c0 + i) + (plan - 1)] = cnt[i];
+            }
+        }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    if (rc) return rc;
+    // compressed plan: host wall time of the synchronising OR chain.  Every
+    // OR of the chain waits for the device at least once, so the chain
+    // takes at least (its ORs) x (the fastest launch-and-wait round trip
+    // measured here); a chain whose floor already exceeds the limit is
+    // censored at that floor without running.
+    double rt = 1e30;
+    for (int k = 0; k < 16 && !rc; ++k) {
+        const Clock::time_point t0 = Clock::now();
+        rc = check(fbk::launch_zero_u32(lit.p, 1, stream()), "zero");
+        if (!rc) rc = sync();
+        rt = std::min(rt, std::chrono::duration<double, std::milli>(Clock::now() - t0).count());
+    }
+    for (uint32_t i = 0; i < nq && !rc; ++i) {
+        const double limit = (double)cap * std::max(ms[3 * (size_t)i], ms[3 * (size_t)i + 2]);
+        const double floor_ms = rt * compressed_ops(ix, lo[i], lop[i], hi[i], rop[i]);
+        if (floor_ms > limit) {
+            ms[3 * (size_t)i + 1] = (float)-floor_ms;
+            hits[3 * (size_t)i + 1] = UINT64_MAX;
+            continue;
+        }
+        if ((rc = sync())) break;
+        fbh_bitvector* res = nullptr;
+        if ((rc = fbh_bv_new(&res))) break;
+        const Clock::time_point t0 = Clock::now();
+        const Clock::time_point dl = t0 + std::chrono::duration_cast<Clock::duration>(
+                                              std::chrono::duration<double, std::milli>(limit));
+        rc = eval_compressed(ix, lo[i], lop[i], hi[i], rop[i], res, &dl);
+        uint32_t c = 0;
+        if (rc == CENSORED) {
+            rc = sync();
+            ms[3 * (size_t)i + 1] = (float)-limit;
+            hits[3 * (size_t)i + 1] = UINT64_MAX;
+        } else if (!rc && !(rc = fbh_bv_cnt(res, &c))) {
+            const double t = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+            ms[3 * (size_t)i + 1] = (float)t;
+            hits[3 * (size_t)i + 1] = c;
+        }
+        fbh_bv_free(res);
+    }
+    return rc;
 }
 
 } // extern "C"

@@ -224,6 +224,8 @@ _dp = C.POINTER(C.c_double)
 _i64p = C.POINTER(C.c_int64)
 _u64p = C.POINTER(C.c_uint64)
 _ip = C.POINTER(C.c_int)
+_fp = C.POINTER(C.c_float)
+NFEATURES, NPLANS, NPARAM = 12, 3, 5187
 
 
 def _index_sigs(L):
@@ -250,6 +252,20 @@ def _index_sigs(L):
         "fbh_index_set_cost_model": [vp, C.c_int],
         "fbh_index_build_bin_nbins": [vp, C.c_int, C.c_uint32, C.c_uint32, C.POINTER(vp)],
         "fbh_index_bounds": [vp, _dp, C.c_uint32, _u32p],
+        "fbh_index_plan_features": [vp, C.c_uint32, _dp, _ip, _dp, _ip, _fp],
+        "fbh_index_time_plans": [vp, C.c_uint32, _dp, _ip, _dp, _ip, C.c_float, _fp, _u64p],
+        "fbh_index_choose_plans": [vp, vp, C.c_uint32, _dp, _ip, _dp, _ip, _ip, _u32p],
+        "fbh_planner_new": [C.c_uint64, C.POINTER(vp)],
+        "fbh_planner_set_params": [vp, _fp, C.c_uint32],
+        "fbh_planner_get_params": [vp, _fp, C.c_uint32],
+        "fbh_planner_set_norm": [vp, _fp, _fp],
+        "fbh_planner_get_norm": [vp, _fp, _fp],
+        "fbh_planner_forward": [vp, _fp, C.c_uint32, _fp],
+        "fbh_planner_grad": [vp, _fp, _fp, _fp, C.c_uint32, _fp, _fp],
+        "fbh_planner_train": [vp, _fp, _fp, _fp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float,
+                              C.c_uint64, _fp],
+        "fbh_planner_step": [vp, _fp, _fp, _fp, C.c_uint32, C.c_float],
+        "fbh_planner_choose": [vp, _fp, C.c_uint32, _ip, _ip, _u32p],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -259,6 +275,8 @@ def _index_sigs(L):
     L.fbh_index_free.restype = None
     L.fbh_free.argtypes = [vp]
     L.fbh_free.restype = None
+    L.fbh_planner_free.argtypes = [vp]
+    L.fbh_planner_free.restype = None
 
 
 class Index:
@@ -419,6 +437,47 @@ class Index:
         check(self._lib().fbh_index_column_bytes(self._h, C.byref(v)), "fbh_index_column_bytes")
         return v.value
 
+    @staticmethod
+    def _queries(lo, lop, hi, rop):
+        lo = np.ascontiguousarray(lo, np.float64)
+        nq = lo.size
+        hi = np.ascontiguousarray(np.broadcast_to(hi, (nq,)), np.float64)
+        lop = np.ascontiguousarray(np.broadcast_to(lop, (nq,)), np.int32)
+        rop = np.ascontiguousarray(np.broadcast_to(rop, (nq,)), np.int32)
+        return nq, lo, lop, hi, rop
+
+    def plan_features(self, lo, lop, hi, rop):
+        """float32[nq, NFEATURES] plan-choice features of the ranges."""
+        nq, lo, lop, hi, rop = self._queries(lo, lop, hi, rop)
+        f = np.zeros((nq, NFEATURES), np.float32)
+        check(self._lib().fbh_index_plan_features(
+            self._h, nq, lo.ctypes.data_as(_dp), lop.ctypes.data_as(_ip), hi.ctypes.data_as(_dp),
+            rop.ctypes.data_as(_ip), f.ctypes.data_as(_fp)), "fbh_index_plan_features")
+        return f
+
+    def time_plans(self, lo, lop, hi, rop, cap=2.0):
+        """(ms float32[nq, 3], hits uint64[nq, 3]) per plan (literal,
+        compressed, scan); a censored compressed time is negative."""
+        nq, lo, lop, hi, rop = self._queries(lo, lop, hi, rop)
+        ms = np.zeros((nq, NPLANS), np.float32)
+        hits = np.zeros((nq, NPLANS), np.uint64)
+        check(self._lib().fbh_index_time_plans(
+            self._h, nq, lo.ctypes.data_as(_dp), lop.ctypes.data_as(_ip), hi.ctypes.data_as(_dp),
+            rop.ctypes.data_as(_ip), float(cap), ms.ctypes.data_as(_fp), hits.ctypes.data_as(_u64p)),
+            "fbh_index_time_plans")
+        return ms, hits
+
+    def choose_plans(self, planner, lo, lop, hi, rop):
+        """(plans int32[nq], number of heuristic fallbacks) from a Planner."""
+        nq, lo, lop, hi, rop = self._queries(lo, lop, hi, rop)
+        plans = np.zeros(nq, np.int32)
+        nf = C.c_uint32()
+        check(self._lib().fbh_index_choose_plans(
+            self._h, planner._h, nq, lo.ctypes.data_as(_dp), lop.ctypes.data_as(_ip),
+            hi.ctypes.data_as(_dp), rop.ctypes.data_as(_ip), plans.ctypes.data_as(_ip), C.byref(nf)),
+            "fbh_index_choose_plans")
+        return plans, nf.value
+
 
 # ---------------------------------------------------------------------------
 # Conjunctions
@@ -494,3 +553,108 @@ class Conjunctions:
         except Exception:
             pass
         self._h = None
+
+
+# ---------------------------------------------------------------------------
+# Learned plan choice
+def _f32(a, shape):
+    return np.ascontiguousarray(np.asarray(a, np.float32).reshape(shape))
+
+
+class Planner:
+    """The FP32 MLP cost model of the plan choice (fbh_planner_*)."""
+
+    def __init__(self, seed=0):
+        self._L = Index._lib()
+        h = C.c_void_p()
+        check(self._L.fbh_planner_new(int(seed), C.byref(h)), "fbh_planner_new")
+        self._h = h
+
+    def __del__(self):
+        try:
+            if self._h is not None:
+                self._L.fbh_planner_free(self._h)
+        except Exception:
+            pass
+        self._h = None
+
+    @property
+    def params(self):
+        out = np.zeros(NPARAM, np.float32)
+        check(self._L.fbh_planner_get_params(self._h, out.ctypes.data_as(_fp), NPARAM),
+              "fbh_planner_get_params")
+        return out
+
+    @params.setter
+    def params(self, v):
+        a = _f32(v, (-1,))
+        check(self._L.fbh_planner_set_params(self._h, a.ctypes.data_as(_fp), a.size),
+              "fbh_planner_set_params")
+
+    def set_norm(self, mu, sd):
+        mu, sd = _f32(mu, (NFEATURES,)), _f32(sd, (NFEATURES,))
+        check(self._L.fbh_planner_set_norm(self._h, mu.ctypes.data_as(_fp), sd.ctypes.data_as(_fp)),
+              "fbh_planner_set_norm")
+
+    def norm(self):
+        mu, sd = np.zeros(NFEATURES, np.float32), np.zeros(NFEATURES, np.float32)
+        check(self._L.fbh_planner_get_norm(self._h, mu.ctypes.data_as(_fp), sd.ctypes.data_as(_fp)),
+              "fbh_planner_get_norm")
+        return mu, sd
+
+    def forward(self, x):
+        x = _f32(x, (-1, NFEATURES))
+        out = np.zeros((x.shape[0], NPLANS), np.float32)
+        check(self._L.fbh_planner_forward(self._h, x.ctypes.data_as(_fp), x.shape[0],
+                                          out.ctypes.data_as(_fp)), "fbh_planner_forward")
+        return out
+
+    def grad(self, x, y, mask):
+        x = _f32(x, (-1, NFEATURES))
+        n = x.shape[0]
+        y, m = _f32(y, (n, NPLANS)), _f32(mask, (n, NPLANS))
+        g = np.zeros(NPARAM, np.float32)
+        loss = C.c_float()
+        check(self._L.fbh_planner_grad(self._h, x.ctypes.data_as(_fp), y.ctypes.data_as(_fp),
+                                       m.ctypes.data_as(_fp), n, C.byref(loss), g.ctypes.data_as(_fp)),
+              "fbh_planner_grad")
+        return loss.value, g
+
+    def step(self, x, y, mask, lr):
+        x = _f32(x, (-1, NFEATURES))
+        n = x.shape[0]
+        y, m = _f32(y, (n, NPLANS)), _f32(mask, (n, NPLANS))
+        check(self._L.fbh_planner_step(self._h, x.ctypes.data_as(_fp), y.ctypes.data_as(_fp),
+                                       m.ctypes.data_as(_fp), n, float(lr)), "fbh_planner_step")
+
+    def train(self, x, y, mask, epochs=40, batch=256, lr=1e-3, seed=0):
+        """Trains on the device; returns float32[epochs] mean losses."""
+        x = _f32(x, (-1, NFEATURES))
+        n = x.shape[0]
+        y, m = _f32(y, (n, NPLANS)), _f32(mask, (n, NPLANS))
+        hist = np.zeros(epochs, np.float32)
+        check(self._L.fbh_planner_train(self._h, x.ctypes.data_as(_fp), y.ctypes.data_as(_fp),
+                                        m.ctypes.data_as(_fp), n, int(epochs), int(batch), float(lr),
+                                        int(seed), hist.ctypes.data_as(_fp)), "fbh_planner_train")
+        return hist
+
+    def choose(self, x, fallback):
+        """(plans int32[n], fallbacks) for feature rows x."""
+        x = _f32(x, (-1, NFEATURES))
+        n = x.shape[0]
+        fb = np.ascontiguousarray(np.broadcast_to(fallback, (n,)), np.int32)
+        plans = np.zeros(n, np.int32)
+        nf = C.c_uint32()
+        check(self._L.fbh_planner_choose(self._h, x.ctypes.data_as(_fp), n, fb.ctypes.data_as(_ip),
+                                         plans.ctypes.data_as(_ip), C.byref(nf)), "fbh_planner_choose")
+        return plans, nf.value
+
+
+def plan_labels(ms):
+    """Training targets of timed queries: y = log2(microseconds) per plan,
+    mask 1 for a measured time, -1 for a censored (lower-bound) one."""
+    ms = np.asarray(ms, np.float64)
+    us = np.maximum(np.abs(ms) * 1000.0, 1e-3)
+    y = np.log2(us).astype(np.float32)
+    mask = np.where(ms < 0, -1.0, 1.0).astype(np.float32)
+    return y, mask
