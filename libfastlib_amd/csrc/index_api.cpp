@@ -321,7 +321,8 @@ int finish_index(fbh_index* ix) {
     if (!rc) rc = to_host(&total, ix->TW + nseg, 1);
     if (rc) return rc;
     ix->ntw = total;
-    ix->tw = (uint32_t*)dev_alloc(4ull * (total ? total : 1));
+    // four words of padding: the literal plan reads whole 16-byte groups
+    ix->tw = (uint32_t*)dev_alloc(4ull * (total + 4));
     if (!ix->tw) return fail(FBH_ERR_NOMEM, "device allocation failed");
     rc = check(fbk::launch_tseg_fill(ix->words, toff.p, ix->TW, ix->nobs, ix->ntiles, ix->groups, ix->anb,
                                      ix->tw, stream()), "tiled copy");

@@ -439,21 +439,35 @@ __global__ __launch_bounds__(IX_TPB) void k_tseg_fill(const uint32_t* __restrict
     }
 }
 
+// An empty use of a loaded value: placed after a batch of loads, it makes
+// the compiler issue the whole batch before waiting for any of it.
+template <class T>
+__device__ __forceinline__ void use_loaded(T x) {
+    if constexpr (sizeof(T) == 8) asm volatile("" ::"v"(__builtin_bit_cast(uint64_t, x)));
+    else if constexpr (sizeof(T) == 4) asm volatile("" ::"v"(__builtin_bit_cast(uint32_t, x)));
+    else asm volatile("" ::"v"((uint32_t)x));
+}
+
 // Literal range plan.  One block of QB_TPB threads owns tile t: ng = QT
 // groups (in the last tile the remaining groups plus the active slot).  The
 // segments of a bitmap range are contiguous in the tiled copy and each
 // covers exactly ng groups, so a word's group within the tile is the
-// running sum of the word lengths from the start of a segment, modulo ng:
-// waves stream chunks of CB bitmaps (taken from an LDS counter) 64 words a
-// round, QB_U rounds of loads in flight, with a DPP prefix sum of the word
-// lengths.  Literal words are OR-ed into the LDS tile R; 1-fills add +1/-1
-// at their ends to the LDS coverage array cov, whose prefix sums mark
-// all-one groups.  Edge bitmaps stream the same way with every row checked
-// against the raw column.
+// running sum of the word lengths from the start of any segment, modulo
+// ng.  The OR set of the tile is one or two contiguous word streams; the
+// block splits them into per-wave pieces of equal word counts at segment
+// boundaries (an LDS table of chunk starts), and each wave streams its
+// piece 64 words a sub-round with a DPP prefix sum of the word lengths,
+// QB_U sub-rounds per round and the next round's loads issued before the
+// current round is applied.  Literal words are OR-ed into the LDS tile R;
+// 1-fills add +1/-1 at their ends to the LDS coverage array cov, whose
+// prefix sums mark all-one groups.  Edge bitmaps stream one sub-round at a
+// time with every row checked against the raw column.
 constexpr int QB_TPB = 256;
 constexpr int QB_WAVES = QB_TPB / WAVE;
-constexpr int QB_U = 16;                     // rounds of loads in flight per lane
-constexpr uint32_t QB_CB = 64;               // bitmaps per chunk
+constexpr int QB_U = 2;                      // sub-rounds (16-byte loads) per round
+constexpr int CJ_U = 2;                      // the same inside the conjunction kernel
+constexpr int CT_MAX = 128;                  // chunks of the split table
+constexpr int CT_WORDS = CT_MAX + 8;         // LDS words of the split table
 
 template <class T>
 __device__ __forceinline__ word_t check_group(const QueryArgs& q, const T* col, uint32_t g, word_t bits) {
@@ -467,82 +481,192 @@ __device__ __forceinline__ word_t check_group(const QueryArgs& q, const T* col, 
     return m;
 }
 
-// Streams tiled words [ws, we) (whole segments) into the tile, by one wave.
+// Applies one sub-round of 64 words (lane's word w, act = lane holds a
+// word) at positions carry + prefix (mod ng); returns the new carry.
 template <class T, bool CHECK, bool POW2>
-__device__ void stream_words(const QueryArgs& q, uint32_t ws, uint32_t we, uint32_t g0, uint32_t ng,
-                             word_t* R, uint32_t* cov, const T* col) {
-    // the checked path (edge bitmaps, one segment) keeps one round in flight
-    constexpr int U = CHECK ? 1 : QB_U;
-    const int lane = (int)lane_id();
-    uint32_t carry = 0;                      // groups before the round, modulo ng
-    for (uint32_t base = ws; base < we; base += U * WAVE) {
-        word_t x[U];
-        #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const uint32_t i = base + (uint32_t)u * WAVE + (uint32_t)lane;
-            x[u] = i < we ? q.tw[i] : 0u;
-        }
-        #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const uint32_t r0 = base + (uint32_t)u * WAVE;
-            const bool act = r0 + (uint32_t)lane < we;
-            const word_t w = x[u];
-            const uint32_t len = act ? word_len(w) : 0u;
-            const uint32_t inc = wave_incl_sum(len);
-            uint32_t rel = carry + inc - len;
-            rel = POW2 ? (rel & (uint32_t)(QT - 1)) : (rel % ng);
-            const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)inc, WAVE - 1);
-            carry = POW2 ? ((carry + tot) & (uint32_t)(QT - 1)) : ((carry + tot) % ng);
-            if (!act) continue;
-            if (w < HEADER0) {
-                if (w) {
-                    const word_t m = CHECK ? check_group<T>(q, col, g0 + rel, w) : w;
-                    if (m) atomicOr(&R[rel], m);
-                }
-            } else if (w >= HEADER1) {
-                if (!CHECK) {
-                    atomicAdd(&cov[rel], 1u);
-                    atomicSub(&cov[rel + len], 1u);
-                } else {
-                    for (uint32_t k = 0; k < len; ++k) {
-                        const word_t m = check_group<T>(q, col, g0 + rel + k, ALLONES);
-                        if (m) atomicOr(&R[rel + k], m);
-                    }
+__device__ __forceinline__ uint32_t apply_words(const QueryArgs& q, word_t w, bool act, uint32_t carry,
+                                                uint32_t g0, uint32_t ng, word_t* R, uint32_t* cov, const T* col) {
+    const uint32_t len = act ? word_len(w) : 0u;
+    const uint32_t inc = wave_incl_sum(len);
+    uint32_t rel = carry + inc - len;
+    rel = POW2 ? (rel & (uint32_t)(QT - 1)) : (rel % ng);
+    const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)inc, WAVE - 1);
+    carry = POW2 ? ((carry + tot) & (uint32_t)(QT - 1)) : ((carry + tot) % ng);
+    if (act) {
+        if (w < HEADER0) {
+            if (w) {
+                const word_t m = CHECK ? check_group<T>(q, col, g0 + rel, w) : w;
+                if (m) atomicOr(&R[rel], m);
+            }
+        } else if (w >= HEADER1) {
+            if (!CHECK) {
+                atomicAdd(&cov[rel], 1u);
+                atomicSub(&cov[rel + len], 1u);
+            } else {
+                for (uint32_t k = 0; k < len; ++k) {
+                    const word_t m = check_group<T>(q, col, g0 + rel + k, ALLONES);
+                    if (m) atomicOr(&R[rel + k], m);
                 }
             }
         }
     }
+    return carry;
 }
 
+// Edge bitmap words [ws, we) (one segment), every row checked, by one wave.
 template <class T, bool POW2>
+__device__ void stream_checked(const QueryArgs& q, uint32_t ws, uint32_t we, uint32_t g0, uint32_t ng,
+                               word_t* R, uint32_t* cov, const T* col) {
+    const int lane = (int)lane_id();
+    uint32_t carry = 0;
+    for (uint32_t base = ws; base < we; base += WAVE) {
+        const uint32_t i = base + (uint32_t)lane;
+        const word_t w = i < we ? q.tw[i] : 0u;
+        carry = apply_words<T, true, POW2>(q, w, i < we, carry, g0, ng, R, cov, col);
+    }
+}
+
+// Applies word w (valid: in the stream) at tile position p.
+__device__ __forceinline__ void apply_one(word_t w, bool valid, uint32_t p, uint32_t len, word_t* R,
+                                          uint32_t* cov) {
+    if (valid && w != 0u && w < HEADER0) atomicOr(&R[p], w);
+    else if (valid && w >= HEADER1) {
+        atomicAdd(&cov[p], 1u);
+        atomicSub(&cov[p + len], 1u);
+    }
+}
+
+template <int U>
+__device__ __forceinline__ void load_round4(const uint4* tw4, uint32_t base4, uint32_t end4, uint4 (&x)[U]) {
+    const uint32_t lane = lane_id();
+    #pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint32_t i = base4 + (uint32_t)u * WAVE + lane;
+        x[u] = i < end4 ? tw4[i] : make_uint4(0u, 0u, 0u, 0u);
+    }
+}
+
+// Words [ws, we) (whole segments) by one wave.  Lane l of sub-round u takes
+// the four words of 16-byte group base4 + 64u + l (words outside [ws, we)
+// count as absent), so one prefix sum over the wave places 256 words; a
+// round of U sub-rounds of loads is in flight while the previous round is
+// applied.  The tiled copy is padded so that whole groups can be read.
+template <bool POW2, int U>
+__device__ void stream_words(const QueryArgs& q, uint32_t ws, uint32_t we, uint32_t ng, word_t* R,
+                             uint32_t* cov) {
+    if (ws >= we) return;
+    const uint32_t lane = lane_id();
+    const uint4* tw4 = reinterpret_cast<const uint4*>(q.tw);
+    const uint32_t a4 = ws >> 2, end4 = (we + 3u) >> 2;
+    uint32_t carry = 0;
+    uint4 cur[U], nxt[U];
+    load_round4<U>(tw4, a4, end4, cur);
+    for (uint32_t b4 = a4; b4 < end4; b4 += U * WAVE) {
+        const uint32_t n4 = b4 + U * WAVE;
+        if (n4 < end4) load_round4<U>(tw4, n4, end4, nxt);
+        #pragma unroll
+        for (int u = 0; u < U; ++u) {
+            use_loaded(cur[u].x); use_loaded(cur[u].y); use_loaded(cur[u].z); use_loaded(cur[u].w);
+        }
+        #pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t g4 = b4 + (uint32_t)u * WAVE;
+            if (g4 >= end4) break;
+            const uint32_t j = (g4 + lane) << 2;          // word index of .x
+            const word_t w0 = cur[u].x, w1 = cur[u].y, w2 = cur[u].z, w3 = cur[u].w;
+            const bool v0 = j >= ws && j < we, v1 = j + 1 >= ws && j + 1 < we;
+            const bool v2 = j + 2 >= ws && j + 2 < we, v3 = j + 3 >= ws && j + 3 < we;
+            const uint32_t l0 = v0 ? word_len(w0) : 0u, l1 = v1 ? word_len(w1) : 0u;
+            const uint32_t l2 = v2 ? word_len(w2) : 0u, l3 = v3 ? word_len(w3) : 0u;
+            const uint32_t sum = l0 + l1 + l2 + l3;
+            const uint32_t inc = wave_incl_sum(sum);
+            uint32_t p0 = carry + inc - sum, p1 = p0 + l0, p2 = p1 + l1, p3 = p2 + l2;
+            if (POW2) {
+                p0 &= (uint32_t)(QT - 1); p1 &= (uint32_t)(QT - 1);
+                p2 &= (uint32_t)(QT - 1); p3 &= (uint32_t)(QT - 1);
+            } else {
+                p0 %= ng; p1 %= ng; p2 %= ng; p3 %= ng;
+            }
+            const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)inc, WAVE - 1);
+            carry = POW2 ? ((carry + tot) & (uint32_t)(QT - 1)) : ((carry + tot) % ng);
+            apply_one(w0, v0, p0, l0, R, cov);
+            apply_one(w1, v1, p1, l1, R, cov);
+            apply_one(w2, v2, p2, l2, R, cov);
+            apply_one(w3, v3, p3, l3, R, cov);
+        }
+        #pragma unroll
+        for (int u = 0; u < U; ++u) cur[u] = nxt[u];
+    }
+}
+
+// The OR set of the tile: stream 1 = the segments of bitmaps [r0, r1),
+// stream 2 = those of [r2, r3).  The block reads the starts of up to CT_MAX
+// chunks of consecutive bitmaps into ctab; wave w streams the chunks whose
+// start lies in the w-th quarter of the words.  Called by every thread of
+// the block.
+template <class T, bool POW2, int NT, int U>
 __device__ void tile_ranges(const QueryArgs& q, uint32_t tile, uint32_t g0, uint32_t ng, word_t* R,
-                            uint32_t* cov, uint32_t* next, const T* col) {
-    const uint32_t n1 = (q.r1 - q.r0 + QB_CB - 1) / QB_CB;
-    const uint32_t n2 = (q.r3 - q.r2 + QB_CB - 1) / QB_CB;
+                            uint32_t* cov, const T* col, uint32_t* ctab) {
+    constexpr int NW = NT / WAVE;
+    const uint32_t len1 = q.r1 - q.r0, len2 = q.r3 - q.r2;
+    if (len1 + len2 == 0) return;
+    const uint32_t cb = max(16u, (len1 + len2 + CT_MAX - 1) / CT_MAX);   // bitmaps per chunk
+    const uint32_t n1 = (len1 + cb - 1) / cb, n2 = (len2 + cb - 1) / cb;
+    const uint32_t nc = n1 + n2;                                        // <= CT_MAX + 2
     const uint32_t* TWt = q.TW + (size_t)tile * q.nobs;
-    for (;;) {
-        uint32_t k = 0;
-        if (lane_id() == 0) k = atomicAdd(next, 1u);
-        k = (uint32_t)__builtin_amdgcn_readfirstlane((int)k);
-        if (k >= n1 + n2) break;
-        uint32_t bs, be;
-        if (k < n1) { bs = q.r0 + k * QB_CB; be = min(bs + QB_CB, q.r1); }
-        else { bs = q.r2 + (k - n1) * QB_CB; be = min(bs + QB_CB, q.r3); }
-        stream_words<T, false, POW2>(q, TWt[bs], TWt[be], g0, ng, R, cov, col);
+    uint32_t* st = ctab + CT_MAX + 2;        // s1, e1, s2, e2
+    for (uint32_t k = threadIdx.x; k < nc; k += NT)
+        ctab[k] = TWt[k < n1 ? q.r0 + k * cb : q.r2 + (k - n1) * cb];
+    if (threadIdx.x == 0) {
+        st[0] = len1 ? TWt[q.r0] : 0u;
+        st[1] = len1 ? TWt[q.r1] : 0u;
+        st[2] = len2 ? TWt[q.r2] : 0u;
+        st[3] = len2 ? TWt[q.r3] : 0u;
+    }
+    __syncthreads();
+    const uint32_t s1 = st[0], e1 = st[1], s2 = st[2], e2 = st[3];
+    const uint32_t W1 = e1 - s1, W = W1 + (e2 - s2);
+    // word offset of chunk k in the concatenated streams
+    auto cstart = [&](uint32_t k) -> uint32_t {
+        if (k >= nc) return W;
+        return k < n1 ? ctab[k] - s1 : W1 + (ctab[k] - s2);
+    };
+    // first chunk whose start is >= target (binary search over the table)
+    auto first_ge = [&](uint32_t target) -> uint32_t {
+        uint32_t lo = 0, hi = nc;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (cstart(mid) < target) lo = mid + 1; else hi = mid;
+        }
+        return lo;
+    };
+    const uint32_t w = threadIdx.x / WAVE;
+    const uint32_t ka = w == 0 ? 0u : first_ge((uint32_t)(((uint64_t)W * w) / NW));
+    const uint32_t kb = w + 1 == NW ? nc : first_ge((uint32_t)(((uint64_t)W * (w + 1)) / NW));
+    if (ka >= kb) return;
+    // the piece in stream 1, then the piece in stream 2
+    if (ka < n1) {
+        const uint32_t we = kb >= n1 ? e1 : ctab[kb];
+        stream_words<POW2, U>(q, ctab[ka], we, ng, R, cov);
+    }
+    if (kb > n1) {
+        const uint32_t ws = ka > n1 ? ctab[ka] : s2;
+        const uint32_t we = kb >= nc ? e2 : ctab[kb];
+        stream_words<POW2, U>(q, ws, we, ng, R, cov);
     }
 }
 
 // Literal plan of one term over tile t into R[0, ng), by a block of NT
-// threads.  R, cov and *next are zero on entry; the call ends with a
+// threads.  R and cov are zero on entry; the call ends with a
 // barrier.  red needs NT/64 + 1 words.
-template <class T, int NT>
+template <class T, int NT, int U>
 __device__ void lit_tile(const QueryArgs& q, uint32_t tile, uint32_t g0, uint32_t ng, bool last, word_t* R,
-                         uint32_t* cov, uint32_t* red, uint32_t* next) {
+                         uint32_t* cov, uint32_t* red, uint32_t* ctab) {
     const T* col = (const T*)q.col;
     const int w = threadIdx.x / WAVE;
     if (ng) {
-        if (last) tile_ranges<T, false>(q, tile, g0, ng, R, cov, next, col);
-        else tile_ranges<T, true>(q, tile, g0, ng, R, cov, next, col);
+        if (last) tile_ranges<T, false, NT, U>(q, tile, g0, ng, R, cov, col, ctab);
+        else tile_ranges<T, true, NT, U>(q, tile, g0, ng, R, cov, col, ctab);
     }
     __syncthreads();
     // coverage prefix sums: groups under some 1-fill are all ones
@@ -573,8 +697,8 @@ __device__ void lit_tile(const QueryArgs& q, uint32_t tile, uint32_t g0, uint32_
         const uint32_t cb = w == 0 ? q.cand0 : q.cand1;
         if (cb != NOKEY) {
             const uint32_t* TWt = q.TW + (size_t)tile * q.nobs;
-            if (last) stream_words<T, true, false>(q, TWt[cb], TWt[cb + 1], g0, ng, R, cov, col);
-            else stream_words<T, true, true>(q, TWt[cb], TWt[cb + 1], g0, ng, R, cov, col);
+            if (last) stream_checked<T, false>(q, TWt[cb], TWt[cb + 1], g0, ng, R, cov, col);
+            else stream_checked<T, true>(q, TWt[cb], TWt[cb + 1], g0, ng, R, cov, col);
         }
     }
     __syncthreads();
@@ -584,6 +708,34 @@ __device__ void lit_tile(const QueryArgs& q, uint32_t tile, uint32_t g0, uint32_
 // groups at a time (31 coalesced loads of 64 values, ballots staged in
 // LDS, each lane assembles its group).  bal needs NT/64 x 32 words of 64
 // bits.  Ends with a barrier.
+// Ballots of the 1984 rows rbase .. rbase+1983 (64 groups) into bal[0..30]
+// by one wave, B loads in flight at a time (all 31 for the stand-alone
+// scan; fewer inside the conjunction kernel, whose registers are shared
+// with the other plans).
+template <class T, int B>
+__device__ __forceinline__ void scan_rows64(const QueryArgs& q, const T* col, uint32_t rbase, uint32_t nrows,
+                                            uint64_t* bal) {
+    const int lane = (int)lane_id();
+    #pragma unroll
+    for (int k0 = 0; k0 < 31; k0 += B) {
+        T v[B];
+        #pragma unroll
+        for (int k = 0; k < B; ++k) {
+            const uint32_t row = rbase + (uint32_t)(k0 + k) * 64u + lane;
+            v[k] = (k0 + k < 31 && row < nrows) ? col[row] : T(0);
+        }
+        #pragma unroll
+        for (int k = 0; k < B; ++k) use_loaded(v[k]);
+        #pragma unroll
+        for (int k = 0; k < B; ++k) {
+            if (k0 + k >= 31) break;
+            const uint32_t row = rbase + (uint32_t)(k0 + k) * 64u + lane;
+            const uint64_t m = __ballot(row < nrows && in_range(q, v[k]));
+            if (lane == 0) bal[k0 + k] = m;
+        }
+    }
+}
+
 template <class T, int NT>
 __device__ void scan_tile(const QueryArgs& q, uint32_t g0, uint32_t ng, word_t* R, uint64_t (*bal)[32]) {
     const T* col = (const T*)q.col;
@@ -592,13 +744,7 @@ __device__ void scan_tile(const QueryArgs& q, uint32_t g0, uint32_t ng, word_t* 
     const uint32_t nrows = q.groups * 31u + q.anb;
     for (uint32_t k0 = (uint32_t)w * 64u; k0 < ng; k0 += (NT / WAVE) * 64u) {
         const uint32_t rbase = (g0 + k0) * 31u;
-        #pragma unroll 4
-        for (int k = 0; k < 31; ++k) {
-            const uint32_t row = rbase + (uint32_t)k * 64u + lane;
-            const bool hit = row < nrows && in_range(q, col[row]);
-            const uint64_t m = __ballot(hit);
-            if (lane == 0) bal[w][k] = m;
-        }
+        scan_rows64<T, 8>(q, col, rbase, nrows, bal[w]);
         __builtin_amdgcn_wave_barrier();
         const uint32_t k = k0 + (uint32_t)lane;
         if (k < ng) {
@@ -616,20 +762,19 @@ __device__ void scan_tile(const QueryArgs& q, uint32_t g0, uint32_t ng, word_t* 
 }
 
 template <class T>
-__global__ __launch_bounds__(QB_TPB) void k_query_lit(QueryArgs q) {
+__global__ __launch_bounds__(QB_TPB, 8) void k_query_lit(QueryArgs q) {
     __shared__ word_t R[QT + 1];
     __shared__ uint32_t cov[QT + 1];
     __shared__ uint32_t red[QB_WAVES + 1];
-    __shared__ uint32_t next;
+    __shared__ uint32_t ctab[CT_WORDS];
     const uint32_t tile = blockIdx.x;
     const bool last = tile + 1 == q.ntiles;
     const uint32_t g0 = tile * (uint32_t)QT;
     const uint32_t ng = last ? (q.groups - g0) + (q.anb ? 1u : 0u) : (uint32_t)QT;
     const int w = threadIdx.x / WAVE;
     for (uint32_t k = threadIdx.x; k <= QT; k += QB_TPB) { R[k] = 0u; cov[k] = 0u; }
-    if (threadIdx.x == 0) next = 0u;
     __syncthreads();
-    lit_tile<T, QB_TPB>(q, tile, g0, ng, last, R, cov, red, &next);
+    lit_tile<T, QB_TPB, QB_U>(q, tile, g0, ng, last, R, cov, red, ctab);
     uint32_t ones = 0;
     for (uint32_t k = threadIdx.x; k < ng; k += QB_TPB) {
         const word_t v = R[k];
@@ -656,7 +801,7 @@ __global__ __launch_bounds__(QB_TPB) void k_query_lit(QueryArgs q) {
 // through the tiled copy of its index, or the raw scan of its column).
 template <int NT>
 __device__ void term_tile(const QueryArgs& q, uint32_t tile, uint32_t g0, uint32_t ng, bool last, word_t* R,
-                          uint32_t* cov, uint32_t* red, uint32_t* next, uint64_t (*bal)[32]) {
+                          uint32_t* cov, uint32_t* red, uint64_t (*bal)[32], uint32_t* ctab) {
     if (q.plan == PLAN_SCAN_D) {
         switch (q.type) {
         case COL_I32: scan_tile<int32_t, NT>(q, g0, ng, R, bal); break;
@@ -668,14 +813,13 @@ __device__ void term_tile(const QueryArgs& q, uint32_t tile, uint32_t g0, uint32
         return;
     }
     for (uint32_t k = threadIdx.x; k <= QT; k += NT) { R[k] = 0u; cov[k] = 0u; }
-    if (threadIdx.x == 0) *next = 0u;
     __syncthreads();
     switch (q.type) {
-    case COL_I32: lit_tile<int32_t, NT>(q, tile, g0, ng, last, R, cov, red, next); break;
-    case COL_U32: lit_tile<uint32_t, NT>(q, tile, g0, ng, last, R, cov, red, next); break;
-    case COL_U16: lit_tile<uint16_t, NT>(q, tile, g0, ng, last, R, cov, red, next); break;
-    case COL_F32: lit_tile<float, NT>(q, tile, g0, ng, last, R, cov, red, next); break;
-    default: lit_tile<double, NT>(q, tile, g0, ng, last, R, cov, red, next); break;
+    case COL_I32: lit_tile<int32_t, NT, CJ_U>(q, tile, g0, ng, last, R, cov, red, ctab); break;
+    case COL_U32: lit_tile<uint32_t, NT, CJ_U>(q, tile, g0, ng, last, R, cov, red, ctab); break;
+    case COL_U16: lit_tile<uint16_t, NT, CJ_U>(q, tile, g0, ng, last, R, cov, red, ctab); break;
+    case COL_F32: lit_tile<float, NT, CJ_U>(q, tile, g0, ng, last, R, cov, red, ctab); break;
+    default: lit_tile<double, NT, CJ_U>(q, tile, g0, ng, last, R, cov, red, ctab); break;
     }
 }
 
@@ -685,7 +829,7 @@ __global__ __launch_bounds__(NT) void k_conj(ConjArgs a) {
     __shared__ uint32_t cov[QT + 1];
     __shared__ uint32_t red[NT / WAVE + 1];
     __shared__ uint64_t bal[NT / WAVE][32];
-    __shared__ uint32_t next;
+    __shared__ uint32_t ctab[CT_WORDS];
     const uint32_t nchunk = (a.ntiles + a.tpi - 1) / a.tpi;
     const uint32_t nitems = a.nq * nchunk;
     const int w = threadIdx.x / WAVE;
@@ -698,7 +842,7 @@ __global__ __launch_bounds__(NT) void k_conj(ConjArgs a) {
             const uint32_t g0 = tile * (uint32_t)QT;
             const uint32_t ng = last ? (a.groups - g0) + (a.anb ? 1u : 0u) : (uint32_t)QT;
             for (uint32_t k = k0; k < k1; ++k) {
-                term_tile<NT>(a.terms[k], tile, g0, ng, last, k == k0 ? R : R2, cov, red, &next, bal);
+                term_tile<NT>(a.terms[k], tile, g0, ng, last, k == k0 ? R : R2, cov, red, bal, ctab);
                 int any = 0;
                 if (k != k0) {
                     for (uint32_t i = threadIdx.x; i < ng; i += NT) {
@@ -748,14 +892,7 @@ __global__ __launch_bounds__(IX_TPB) void k_query_scan(QueryArgs q, uint32_t nro
     for (uint32_t gb = (blockIdx.x * (IX_TPB / WAVE) + w) * 64u; gb < ngroups_all;
          gb += gridDim.x * (IX_TPB / WAVE) * 64u) {
         const uint32_t rbase = gb * 31u;
-        #pragma unroll 4
-        for (int k = 0; k < 31; ++k) {
-            const uint32_t row = rbase + (uint32_t)k * 64u + lane;
-            bool hit = false;
-            if (row < nrows) hit = in_range(q, col[row]);
-            const uint64_t m = __ballot(hit);
-            if (lane == 0) bal[w][k] = m;
-        }
+        scan_rows64<T, 31>(q, col, rbase, nrows, bal[w]);
         __builtin_amdgcn_wave_barrier();
         const uint32_t g = gb + lane;
         if (g < ngroups_all) {
