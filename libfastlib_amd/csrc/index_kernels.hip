@@ -448,7 +448,7 @@ __device__ __forceinline__ void use_loaded(T x) {
     else asm volatile("" ::"v"((uint32_t)x));
 }
 
-// Literal range plan.  One block of QB_TPB threads owns tile t: ng = QT
+// Literal range plan.  One block of 192 or 256 threads owns tile t: ng = QT
 // groups (in the last tile the remaining groups plus the active slot).  The
 // segments of a bitmap range are contiguous in the tiled copy and each
 // covers exactly ng groups, so a word's group within the tile is the
@@ -462,8 +462,6 @@ __device__ __forceinline__ void use_loaded(T x) {
 // 1-fills add +1/-1 at their ends to the LDS coverage array cov, whose
 // prefix sums mark all-one groups.  Edge bitmaps stream one sub-round at a
 // time with every row checked against the raw column.
-constexpr int QB_TPB = 256;
-constexpr int QB_WAVES = QB_TPB / WAVE;
 constexpr int QB_U = 2;                      // sub-rounds (16-byte loads) per round
 constexpr int CJ_U = 2;                      // the same inside the conjunction kernel
 constexpr int CT_MAX = 128;                  // chunks of the split table
@@ -546,6 +544,34 @@ __device__ __forceinline__ void load_round4(const uint4* tw4, uint32_t base4, ui
     }
 }
 
+// One sub-round: lane l applies the four words of 16-byte group g4 + l at
+// carry + their prefix sum over the wave (mod ng); returns the new carry.
+// FULL: every word of the sub-round lies in [ws, we) (no checks).
+template <bool POW2, bool FULL>
+__device__ __forceinline__ uint32_t apply_group4(const uint4& x, uint32_t g4, uint32_t ws, uint32_t we,
+                                                 uint32_t carry, uint32_t ng, word_t* R, uint32_t* cov) {
+    const uint32_t j = (g4 + lane_id()) << 2;          // word index of x.x
+    const bool v0 = FULL || (j >= ws && j < we), v1 = FULL || (j + 1 >= ws && j + 1 < we);
+    const bool v2 = FULL || (j + 2 >= ws && j + 2 < we), v3 = FULL || (j + 3 >= ws && j + 3 < we);
+    const uint32_t l0 = v0 ? word_len(x.x) : 0u, l1 = v1 ? word_len(x.y) : 0u;
+    const uint32_t l2 = v2 ? word_len(x.z) : 0u, l3 = v3 ? word_len(x.w) : 0u;
+    const uint32_t sum = l0 + l1 + l2 + l3;
+    const uint32_t inc = wave_incl_sum(sum);
+    uint32_t p0 = carry + inc - sum, p1 = p0 + l0, p2 = p1 + l1, p3 = p2 + l2;
+    if (POW2) {
+        p0 &= (uint32_t)(QT - 1); p1 &= (uint32_t)(QT - 1);
+        p2 &= (uint32_t)(QT - 1); p3 &= (uint32_t)(QT - 1);
+    } else {
+        p0 %= ng; p1 %= ng; p2 %= ng; p3 %= ng;
+    }
+    const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)inc, WAVE - 1);
+    apply_one(x.x, v0, p0, l0, R, cov);
+    apply_one(x.y, v1, p1, l1, R, cov);
+    apply_one(x.z, v2, p2, l2, R, cov);
+    apply_one(x.w, v3, p3, l3, R, cov);
+    return POW2 ? ((carry + tot) & (uint32_t)(QT - 1)) : ((carry + tot) % ng);
+}
+
 // Words [ws, we) (whole segments) by one wave.  Lane l of sub-round u takes
 // the four words of 16-byte group base4 + 64u + l (words outside [ws, we)
 // count as absent), so one prefix sum over the wave places 256 words; a
@@ -555,9 +581,9 @@ template <bool POW2, int U>
 __device__ void stream_words(const QueryArgs& q, uint32_t ws, uint32_t we, uint32_t ng, word_t* R,
                              uint32_t* cov) {
     if (ws >= we) return;
-    const uint32_t lane = lane_id();
     const uint4* tw4 = reinterpret_cast<const uint4*>(q.tw);
     const uint32_t a4 = ws >> 2, end4 = (we + 3u) >> 2;
+    const uint32_t full0 = (ws + 3u) >> 2, full1 = we >> 2;   // groups wholly inside
     uint32_t carry = 0;
     uint4 cur[U], nxt[U];
     load_round4<U>(tw4, a4, end4, cur);
@@ -572,27 +598,10 @@ __device__ void stream_words(const QueryArgs& q, uint32_t ws, uint32_t we, uint3
         for (int u = 0; u < U; ++u) {
             const uint32_t g4 = b4 + (uint32_t)u * WAVE;
             if (g4 >= end4) break;
-            const uint32_t j = (g4 + lane) << 2;          // word index of .x
-            const word_t w0 = cur[u].x, w1 = cur[u].y, w2 = cur[u].z, w3 = cur[u].w;
-            const bool v0 = j >= ws && j < we, v1 = j + 1 >= ws && j + 1 < we;
-            const bool v2 = j + 2 >= ws && j + 2 < we, v3 = j + 3 >= ws && j + 3 < we;
-            const uint32_t l0 = v0 ? word_len(w0) : 0u, l1 = v1 ? word_len(w1) : 0u;
-            const uint32_t l2 = v2 ? word_len(w2) : 0u, l3 = v3 ? word_len(w3) : 0u;
-            const uint32_t sum = l0 + l1 + l2 + l3;
-            const uint32_t inc = wave_incl_sum(sum);
-            uint32_t p0 = carry + inc - sum, p1 = p0 + l0, p2 = p1 + l1, p3 = p2 + l2;
-            if (POW2) {
-                p0 &= (uint32_t)(QT - 1); p1 &= (uint32_t)(QT - 1);
-                p2 &= (uint32_t)(QT - 1); p3 &= (uint32_t)(QT - 1);
-            } else {
-                p0 %= ng; p1 %= ng; p2 %= ng; p3 %= ng;
-            }
-            const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)inc, WAVE - 1);
-            carry = POW2 ? ((carry + tot) & (uint32_t)(QT - 1)) : ((carry + tot) % ng);
-            apply_one(w0, v0, p0, l0, R, cov);
-            apply_one(w1, v1, p1, l1, R, cov);
-            apply_one(w2, v2, p2, l2, R, cov);
-            apply_one(w3, v3, p3, l3, R, cov);
+            if (g4 >= full0 && g4 + WAVE <= full1)
+                carry = apply_group4<POW2, true>(cur[u], g4, ws, we, carry, ng, R, cov);
+            else
+                carry = apply_group4<POW2, false>(cur[u], g4, ws, we, carry, ng, R, cov);
         }
         #pragma unroll
         for (int u = 0; u < U; ++u) cur[u] = nxt[u];
@@ -671,17 +680,17 @@ __device__ void lit_tile(const QueryArgs& q, uint32_t tile, uint32_t g0, uint32_
     __syncthreads();
     // coverage prefix sums: groups under some 1-fill are all ones
     {
-        constexpr int PER = QT / NT;
+        constexpr int PER = (QT + NT - 1) / NT;
         const uint32_t k0 = threadIdx.x * PER;
         uint32_t c[PER], sum = 0;
         #pragma unroll
-        for (int i = 0; i < PER; ++i) { c[i] = cov[k0 + i]; sum += c[i]; }
+        for (int i = 0; i < PER; ++i) { c[i] = k0 + i < (uint32_t)QT ? cov[k0 + i] : 0u; sum += c[i]; }
         uint32_t tot;
         uint32_t run = block_excl_sum<NT>(sum, red, &tot);
         #pragma unroll
         for (int i = 0; i < PER; ++i) {
             run += c[i];
-            if (run != 0u) R[k0 + i] = ALLONES;
+            if (run != 0u && k0 + i < (uint32_t)QT) R[k0 + i] = ALLONES;
         }
     }
     __syncthreads();
@@ -761,22 +770,23 @@ __device__ void scan_tile(const QueryArgs& q, uint32_t g0, uint32_t ng, word_t* 
     __syncthreads();
 }
 
-template <class T>
-__global__ __launch_bounds__(QB_TPB, 8) void k_query_lit(QueryArgs q) {
+template <class T, int NT>
+__global__ __launch_bounds__(NT) void k_query_lit(QueryArgs q) {
+    constexpr int NWV = NT / WAVE;
     __shared__ word_t R[QT + 1];
     __shared__ uint32_t cov[QT + 1];
-    __shared__ uint32_t red[QB_WAVES + 1];
+    __shared__ uint32_t red[NWV + 1];
     __shared__ uint32_t ctab[CT_WORDS];
     const uint32_t tile = blockIdx.x;
     const bool last = tile + 1 == q.ntiles;
     const uint32_t g0 = tile * (uint32_t)QT;
     const uint32_t ng = last ? (q.groups - g0) + (q.anb ? 1u : 0u) : (uint32_t)QT;
     const int w = threadIdx.x / WAVE;
-    for (uint32_t k = threadIdx.x; k <= QT; k += QB_TPB) { R[k] = 0u; cov[k] = 0u; }
+    for (uint32_t k = threadIdx.x; k <= QT; k += NT) { R[k] = 0u; cov[k] = 0u; }
     __syncthreads();
-    lit_tile<T, QB_TPB, QB_U>(q, tile, g0, ng, last, R, cov, red, ctab);
+    lit_tile<T, NT, QB_U>(q, tile, g0, ng, last, R, cov, red, ctab);
     uint32_t ones = 0;
-    for (uint32_t k = threadIdx.x; k < ng; k += QB_TPB) {
+    for (uint32_t k = threadIdx.x; k < ng; k += NT) {
         const word_t v = R[k];
         q.out[g0 + k] = v;
         ones += __popc(v);
@@ -786,7 +796,7 @@ __global__ __launch_bounds__(QB_TPB, 8) void k_query_lit(QueryArgs q) {
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t t = 0;
-        for (int i = 0; i < QB_WAVES; ++i) t += red[i];
+        for (int i = 0; i < NWV; ++i) t += red[i];
         q.part[tile] = t;
     }
 }
@@ -1167,17 +1177,34 @@ hipError_t launch_sum_parts(const uint32_t* parts, uint32_t stride, const uint32
     return hipGetLastError();
 }
 
-hipError_t launch_query_lit(const QueryArgs& q, int type, hipStream_t s) {
-    dim3 g(q.ntiles), b(QB_TPB);             // the last tile holds the active slot
+template <int NT>
+static hipError_t launch_query_lit_nt(const QueryArgs& q, int type, hipStream_t s) {
+    dim3 g(q.ntiles), b(NT);                 // the last tile holds the active slot
     switch (type) {
-    case COL_I32: hipLaunchKernelGGL(k_query_lit<int32_t>, g, b, 0, s, q); break;
-    case COL_U32: hipLaunchKernelGGL(k_query_lit<uint32_t>, g, b, 0, s, q); break;
-    case COL_U16: hipLaunchKernelGGL(k_query_lit<uint16_t>, g, b, 0, s, q); break;
-    case COL_F32: hipLaunchKernelGGL(k_query_lit<float>, g, b, 0, s, q); break;
-    case COL_F64: hipLaunchKernelGGL(k_query_lit<double>, g, b, 0, s, q); break;
+    case COL_I32: hipLaunchKernelGGL((k_query_lit<int32_t, NT>), g, b, 0, s, q); break;
+    case COL_U32: hipLaunchKernelGGL((k_query_lit<uint32_t, NT>), g, b, 0, s, q); break;
+    case COL_U16: hipLaunchKernelGGL((k_query_lit<uint16_t, NT>), g, b, 0, s, q); break;
+    case COL_F32: hipLaunchKernelGGL((k_query_lit<float, NT>), g, b, 0, s, q); break;
+    case COL_F64: hipLaunchKernelGGL((k_query_lit<double, NT>), g, b, 0, s, q); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+// Block size of the literal plan: 256 threads (8 blocks per CU) while all
+// tiles fit on the device at once, else 192 (9 blocks per CU by LDS), so
+// that, e.g., the 2114 tiles of 2^27 rows run in one wave of blocks on 256
+// CUs instead of 2048 blocks and a tail of 66.
+hipError_t launch_query_lit(const QueryArgs& q, int type, hipStream_t s) {
+    static const uint32_t cus = [] {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+            n = 256;
+        return (uint32_t)n;
+    }();
+    if (q.ntiles <= 8u * cus) return launch_query_lit_nt<256>(q, type, s);
+    return launch_query_lit_nt<192>(q, type, s);
 }
 
 hipError_t launch_conj(const ConjArgs& a, int nt, uint32_t blocks, hipStream_t s) {
