@@ -12,6 +12,15 @@ constexpr int WAVE = 64;
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+// An empty use of a loaded value: placed after a batch of loads, it makes
+// the compiler issue the whole batch before waiting for any of it.
+template <class T>
+__device__ __forceinline__ void use_loaded(T x) {
+    if constexpr (sizeof(T) == 8) asm volatile("" ::"v"(__builtin_bit_cast(uint64_t, x)));
+    else if constexpr (sizeof(T) == 4) asm volatile("" ::"v"(__builtin_bit_cast(uint32_t, x)));
+    else asm volatile("" ::"v"((uint32_t)x));
+}
+
 // Inclusive prefix sum across a 64-lane wavefront: row_shr 1,2,4,8 inside
 // each 16-lane row, then row_bcast 15 / 31 to carry across rows.
 __device__ __forceinline__ uint32_t wave_incl_sum(uint32_t v) {
@@ -60,8 +69,10 @@ __device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t* sh, uin
     return r;
 }
 
-// Look-back between tiles.  Tile ids are handed out in launch order by an
-// atomic ticket, so every predecessor of a tile is resident or done.  A
+// Look-back between tiles.  Tile t is block t: the dispatcher starts
+// blocks in index order, so every predecessor of a tile is resident or
+// done (a shared atomic ticket would do the same at ~20 ns per block, the
+// cost of serialised same-address atomics).  A
 // tile publishes its aggregate (status 1), reads its predecessors 64 at a
 // time until an inclusive prefix (status 2) closes the chain, then
 // publishes its own inclusive prefix.  Every value a tile needs from

@@ -439,14 +439,6 @@ __global__ __launch_bounds__(IX_TPB) void k_tseg_fill(const uint32_t* __restrict
     }
 }
 
-// An empty use of a loaded value: placed after a batch of loads, it makes
-// the compiler issue the whole batch before waiting for any of it.
-template <class T>
-__device__ __forceinline__ void use_loaded(T x) {
-    if constexpr (sizeof(T) == 8) asm volatile("" ::"v"(__builtin_bit_cast(uint64_t, x)));
-    else if constexpr (sizeof(T) == 4) asm volatile("" ::"v"(__builtin_bit_cast(uint32_t, x)));
-    else asm volatile("" ::"v"((uint32_t)x));
-}
 
 // Literal range plan.  One block of 192 or 256 threads owns tile t: ng = QT
 // groups (in the last tile the remaining groups plus the active slot).  The

@@ -102,7 +102,6 @@ struct OpArgs {
     word_t* out;
     uint32_t* outck;
     uint64_t* status;
-    uint32_t* ticket;
     uint32_t* total;      // [0] output words, [3] groups represented
 };
 
@@ -159,9 +158,7 @@ __global__ __launch_bounds__(TPB) void k_op_tiles(OpArgs p) {
     __shared__ Summary s_pre;
 
     const int tid = threadIdx.x;
-    if (tid == 0) info[0] = atomicAdd(p.ticket, 1u);
-    __syncthreads();
-    const uint32_t tile = info[0];
+    const uint32_t tile = blockIdx.x;
 
     if (tid == 0) {
         const uint32_t P0 = diag_pos(p, tile * KTILE);
@@ -321,7 +318,6 @@ struct TokArgs {
     word_t* out;
     uint32_t* outck;
     uint64_t* status;
-    uint32_t* ticket;
     uint32_t* total;
 };
 
@@ -339,9 +335,7 @@ __global__ __launch_bounds__(TPB) void k_tok_tiles(TokArgs p) {
     __shared__ Summary s_pre;
     __shared__ uint32_t s_base_pos;
     const int tid = threadIdx.x;
-    if (tid == 0) info[0] = atomicAdd(p.ticket, 1u);
-    __syncthreads();
-    const uint32_t tile = info[0];
+    const uint32_t tile = blockIdx.x;
     const uint32_t base = tile * TOK_TILE;
     const uint32_t cnt = base < p.n ? min((uint32_t)TOK_TILE, p.n - base) : 0u;
     // coalesced load of the tile's words (op applied for the c0 mode)
@@ -627,7 +621,7 @@ hipError_t launch_op(const word_t* a, const uint32_t* cka, uint32_t na,
     p.b = b; p.ckb = ckb; p.nb = nb; p.ncb = cdiv(nb, CHUNK);
     p.groups = groups; p.op = op; p.canon = canon ? 1u : 0u;
     p.out = out; p.outck = outck; p.status = status;
-    p.ticket = ctl; p.total = ctl + 1;
+    p.total = ctl + 1;
     const uint32_t nt = op_tiles(na, nb);
     hipLaunchKernelGGL(k_zero_u64, dim3(min(cdiv(nt, TPB), 4096u)), dim3(TPB), 0, s, status, nt);
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(TPB), 0, s, ctl, 8u);
@@ -659,7 +653,7 @@ hipError_t launch_tok(const word_t* a, const word_t* b, uint32_t n, uint32_t gro
                       hipStream_t s) {
     TokArgs p;
     p.a = a; p.b = b; p.n = n; p.groups = groups; p.op = op; p.mode = mode;
-    p.out = out; p.outck = outck; p.status = status; p.ticket = ctl; p.total = ctl + 1;
+    p.out = out; p.outck = outck; p.status = status; p.total = ctl + 1;
     const uint32_t nt = tok_tiles(n);
     hipLaunchKernelGGL(k_zero_u64, dim3(min(cdiv(2 * nt, TPB), 4096u)), dim3(TPB), 0, s, status,
                        2 * nt);
