@@ -1,8 +1,8 @@
 // CPU driver of the WAH step core (wah_core.h), used by the CPU test suite to
 // check the partitioned algorithm that the gfx950 kernels run: the position
 // space is cut at caller-chosen points, each piece is advanced to its first
-// common boundary, walked, summarized and written exactly as one GPU thread
-// does.  Built as a host-only library (libfbsim.so); not part of the product
+// synchronisation point, walked, summarized and written exactly as one GPU
+// thread does.  Built as a host-only library (libfbsim.so); not part of the product
 // entry points.
 #include "wah_core.h"
 #include <vector>
@@ -41,7 +41,7 @@ long fbsim_op(const uint32_t* a, uint32_t na, const uint32_t* b, uint32_t nb,
     std::vector<uint32_t> cuts(splits, splits + nsplits);
     cuts.push_back(0);
     std::vector<uint32_t> starts;   // sync points (positions)
-    std::vector<uint32_t> ias, ibs;
+    std::vector<uint32_t> ias, ibs, pas, pbs;
     for (uint32_t c : cuts) {
         if (c > groups) c = groups;
         // code word containing c in each operand, then advance to common
@@ -52,25 +52,27 @@ long fbsim_op(const uint32_t* a, uint32_t na, const uint32_t* b, uint32_t nb,
         uint32_t pa = sa[ia], pb = sb[ib];
         if (pa < c && ia < na) { pa = sa[ia + 1]; ++ia; }
         if (pb < c && ib < nb) { pb = sb[ib + 1]; ++ib; }
-        uint32_t p = find_common(&ra, &rb, ia, pa, ib, pb, groups);
+        uint32_t p = find_sync(&ra, &rb, ia, pa, ib, pb, groups);
         starts.push_back(p); ias.push_back(ia); ibs.push_back(ib);
+        pas.push_back(pa); pbs.push_back(pb);
     }
     // sort and dedupe the sync points
     std::vector<size_t> ord(starts.size());
     for (size_t i = 0; i < ord.size(); ++i) ord[i] = i;
     std::sort(ord.begin(), ord.end(), [&](size_t x, size_t y) { return starts[x] < starts[y]; });
-    std::vector<uint32_t> P, IA, IB;
+    std::vector<uint32_t> P, IA, IB, PA, PB;
     for (size_t k : ord) {
         if (!P.empty() && P.back() == starts[k]) continue;
         P.push_back(starts[k]); IA.push_back(ias[k]); IB.push_back(ibs[k]);
+        PA.push_back(pas[k]); PB.push_back(pbs[k]);
     }
-    if (P.back() != groups) { P.push_back(groups); IA.push_back(na); IB.push_back(nb); }
+    if (P.back() != groups) { P.push_back(groups); IA.push_back(na); IB.push_back(nb); PA.push_back(groups); PB.push_back(groups); }
     const size_t nr = P.size() - 1;
     // count pass
     std::vector<Summary> sum(nr);
     for (size_t r = 0; r < nr; ++r) {
         CountSink cs; cs.init();
-        walk(cs, &ra, &rb, IA[r], IB[r], P[r], P[r + 1], op, (uint32_t)canon);
+        walk(cs, &ra, &rb, IA[r], PA[r], IB[r], PB[r], P[r], P[r + 1], op, (uint32_t)canon);
         sum[r] = cs.s;
     }
     // exclusive prefixes
@@ -84,7 +86,7 @@ long fbsim_op(const uint32_t* a, uint32_t na, const uint32_t* b, uint32_t nb,
     for (size_t r = 0; r < nr; ++r) {
         WriteSink<VecWriter> ws;
         ws.init(&vw, pre[r], P[r]);
-        walk(ws, &ra, &rb, IA[r], IB[r], P[r], P[r + 1], op, (uint32_t)canon);
+        walk(ws, &ra, &rb, IA[r], PA[r], IB[r], PB[r], P[r], P[r + 1], op, (uint32_t)canon);
         if (r + 1 == nr) ws.finish();
     }
     if (res.size() != acc.count) return -2;

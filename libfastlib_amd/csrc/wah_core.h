@@ -280,123 +280,101 @@ WAH_HD void copy_runs_tok(S& sink, Run<R>& it, word_t& nw, bool neg, uint32_t ca
     }
 }
 
+// Steps of the *_c2 loops consume a span chosen by the branch; a step here
+// consumes at most `lim` groups (lim >= 1), so that a walk ends exactly at a
+// synchronisation point inside a span.  Cutting a span only ends a token
+// early; the next walk's tokens continue the same run and merge with it.
+template <class R>
+WAH_HD void take(Run<R>& r, word_t n) {          // n <= r.nWords groups of the decoded word
+    if (n == r.nWords) { r.nWords = 0; ++r.it; }
+    else r.nWords -= n;
+}
+
 // One step of the *_c2 loop of `op`; returns the groups consumed.
 // canon != 0 forces every token's representation to L (the canonical result
 // of the c0/d1/d2/c1 routines).
 template <class R, class S>
-WAH_HD word_t c2_step(S& sink, Run<R>& x, Run<R>& y, int op, uint32_t canon) {
+WAH_HD word_t c2_step(S& sink, Run<R>& x, Run<R>& y, int op, uint32_t canon, word_t lim = 0xFFFFFFFFu) {
     const uint32_t F = canon ? 0u : 1u;
     if (x.nWords == 0) x.decode();
     if (y.nWords == 0) y.decode();
-    word_t nw;
+    auto cap = [&](word_t n) { return n < lim ? n : lim; };
+    // a fill of `f` spans the step, the other operand's words are skipped
+    auto fill_skip = [&](Run<R>& f, Run<R>& o, int bit) -> word_t {
+        const word_t nw = cap(f.nWords);
+        emit_fill(sink, bit, nw, F);
+        o.minus_eq(nw);
+        take(f, nw);
+        return nw;
+    };
+    // the words of `c` are copied within the fill of `f`
+    auto copy_in = [&](Run<R>& f, Run<R>& c, bool neg) -> word_t {
+        const word_t nw = cap(f.nWords);
+        word_t r = nw;
+        copy_runs_tok(sink, c, r, neg, canon);
+        const word_t used = nw - r;
+        f.nWords -= used;
+        f.it += (f.nWords == 0);
+        return used;
+    };
+    // both fills, the shorter `s` ends first: the span has the other's bit
+    auto fill_short = [&](Run<R>& l, Run<R>& sh) -> word_t {
+        const word_t nw = cap(sh.nWords);
+        emit_fill(sink, l.fillBit, nw, F);
+        l.nWords -= nw;
+        take(sh, nw);
+        return nw;
+    };
     if (op == AND) {                                   // src/bitvector.cpp:2460-2516
         if (x.isFill) {
             if (y.isFill) {
-                if (y.fillBit == 0) {
-                    nw = y.nWords; emit_fill(sink, 0, nw, F);
-                    x.minus_eq(nw); y.nWords = 0; ++y.it; return nw;
-                } else if (y.nWords >= x.nWords) {
-                    nw = y.nWords; word_t r = y.nWords;
-                    copy_runs_tok(sink, x, r, false, canon);
-                    y.nWords = r; y.it += (r == 0); return nw - r;
-                } else {
-                    nw = y.nWords; emit_fill(sink, x.fillBit, nw, F);
-                    x.nWords -= nw; y.nWords = 0; ++y.it; return nw;
-                }
+                if (y.fillBit == 0) return fill_skip(y, x, 0);
+                else if (y.nWords >= x.nWords) return copy_in(y, x, false);
+                else return fill_short(x, y);
             } else if (x.fillBit == 0) {
-                nw = x.nWords; emit_fill(sink, 0, nw, F);
-                y.minus_eq(nw); x.nWords = 0; ++x.it; return nw;
+                return fill_skip(x, y, 0);
             } else {
-                nw = x.nWords; word_t r = x.nWords;
-                copy_runs_tok(sink, y, r, false, canon);
-                x.nWords = r; x.it += (r == 0); return nw - r;
+                return copy_in(x, y, false);
             }
         } else if (y.isFill) {
-            if (y.fillBit == 0) {
-                nw = y.nWords; emit_fill(sink, 0, nw, F);
-                x.minus_eq(nw); y.nWords = 0; ++y.it; return nw;
-            } else {
-                nw = y.nWords; word_t r = y.nWords;
-                copy_runs_tok(sink, x, r, false, canon);
-                y.nWords = r; y.it += (r == 0); return nw - r;
-            }
+            if (y.fillBit == 0) return fill_skip(y, x, 0);
+            else return copy_in(y, x, false);
         }
     } else if (op == OR) {                             // src/bitvector.cpp:2874-2934
         if (x.isFill) {
             if (y.isFill) {
-                if (y.fillBit != 0) {
-                    nw = y.nWords; emit_fill(sink, 1, nw, F);
-                    x.minus_eq(nw); y.nWords = 0; ++y.it; return nw;
-                } else if (y.nWords < x.nWords) {
-                    nw = y.nWords; emit_fill(sink, x.fillBit, nw, F);
-                    x.nWords -= nw; y.nWords = 0; ++y.it; return nw;
-                } else {
-                    nw = y.nWords; word_t r = y.nWords;
-                    copy_runs_tok(sink, x, r, false, canon);
-                    y.nWords = r; y.it += (r == 0); return nw - r;
-                }
+                if (y.fillBit != 0) return fill_skip(y, x, 1);
+                else if (y.nWords < x.nWords) return fill_short(x, y);
+                else return copy_in(y, x, false);
             } else if (x.fillBit) {
-                nw = x.nWords; emit_fill(sink, 1, nw, F);
-                y.minus_eq(nw); x.nWords = 0; ++x.it; return nw;
+                return fill_skip(x, y, 1);
             } else {
-                nw = x.nWords; word_t r = x.nWords;
-                copy_runs_tok(sink, y, r, false, canon);
-                x.nWords = r; x.it += (r == 0); return nw - r;
+                return copy_in(x, y, false);
             }
         } else if (y.isFill) {
-            if (y.fillBit) {
-                nw = y.nWords; emit_fill(sink, 1, nw, F);
-                x.minus_eq(nw); y.nWords = 0; ++y.it; return nw;
-            } else {
-                nw = y.nWords; word_t r = y.nWords;
-                copy_runs_tok(sink, x, r, false, canon);
-                y.nWords = r; y.it += (r == 0); return nw - r;
-            }
+            if (y.fillBit) return fill_skip(y, x, 1);
+            else return copy_in(y, x, false);
         }
     } else if (op == XOR) {                            // src/bitvector.cpp:3287-3325
         if (x.isFill) {
-            if (y.isFill && y.nWords >= x.nWords) {
-                nw = y.nWords; word_t r = y.nWords;
-                copy_runs_tok(sink, x, r, y.fillBit != 0, canon);
-                y.nWords = r; y.it += (r == 0); return nw - r;
-            } else {
-                nw = x.nWords; word_t r = x.nWords;
-                copy_runs_tok(sink, y, r, x.fillBit != 0, canon);
-                x.nWords = r; x.it += (r == 0); return nw - r;
-            }
+            if (y.isFill && y.nWords >= x.nWords) return copy_in(y, x, y.fillBit != 0);
+            else return copy_in(x, y, x.fillBit != 0);
         } else if (y.isFill) {
-            nw = y.nWords; word_t r = y.nWords;
-            copy_runs_tok(sink, x, r, y.fillBit != 0, canon);
-            y.nWords = r; y.it += (r == 0); return nw - r;
+            return copy_in(y, x, y.fillBit != 0);
         }
     } else {                                           // MINUS, src/bitvector.cpp:3668-3717
         if (x.isFill) {
             if (y.isFill && y.nWords >= x.nWords) {
-                if (y.fillBit == 0) {
-                    nw = y.nWords; word_t r = y.nWords;
-                    copy_runs_tok(sink, x, r, false, canon);
-                    y.nWords = r; y.it += (r == 0); return nw - r;
-                } else {
-                    nw = y.nWords; emit_fill(sink, 0, nw, F);
-                    x.minus_eq(nw); y.nWords = 0; ++y.it; return nw;
-                }
+                if (y.fillBit == 0) return copy_in(y, x, false);
+                else return fill_skip(y, x, 0);
             } else if (x.fillBit == 0) {
-                nw = x.nWords; emit_fill(sink, 0, nw, F);
-                y.minus_eq(nw); x.nWords = 0; ++x.it; return nw;
+                return fill_skip(x, y, 0);
             } else {
-                nw = x.nWords; word_t r = x.nWords;
-                copy_runs_tok(sink, y, r, true, canon);
-                x.nWords = r; x.it += (r == 0); return nw - r;
+                return copy_in(x, y, true);
             }
         } else if (y.isFill) {
-            if (y.fillBit == 0) {
-                nw = y.nWords; word_t r = y.nWords;
-                copy_runs_tok(sink, x, r, false, canon);
-                y.nWords = r; y.it += (r == 0); return nw - r;
-            } else {
-                nw = y.nWords; emit_fill(sink, 0, nw, F);
-                x.minus_eq(nw); y.nWords = 0; ++y.it; return nw;
-            }
+            if (y.fillBit == 0) return copy_in(y, x, false);
+            else return fill_skip(y, x, 0);
         }
     }
     // both literal: append_active of the combined word
@@ -405,31 +383,54 @@ WAH_HD word_t c2_step(S& sink, Run<R>& x, Run<R>& y, int op, uint32_t canon) {
     return 1;
 }
 
-// Walks the c2 loop from a common boundary (code word ia of A and ib of B,
-// both starting at position p0) until position p1 (another common boundary).
+// Walks the c2 loop over [p0, p1) between two synchronisation points (see
+// find_sync).  ia / ib are the code words of A and B at p0 and pa / pb
+// their start positions: an operand whose word starts before p0 is inside
+// a fill there, with pa + len - p0 groups of it left.
 template <class R, class S>
-WAH_HD void walk(S& sink, const R* ra, const R* rb, uint32_t ia, uint32_t ib,
+WAH_HD void walk(S& sink, const R* ra, const R* rb, uint32_t ia, uint32_t pa, uint32_t ib, uint32_t pb,
                  uint32_t p0, uint32_t p1, int op, uint32_t canon) {
     Run<R> x, y;
     x.src = ra; x.it = ia; x.nWords = 0; x.isFill = 0; x.fillBit = 0;
     y.src = rb; y.it = ib; y.nWords = 0; y.isFill = 0; y.fillBit = 0;
+    if (p0 < p1) {
+        if (pa < p0) { x.decode(); x.nWords = pa + x.nWords - p0; }
+        if (pb < p0) { y.decode(); y.nWords = pb + y.nWords - p0; }
+    }
     uint32_t p = p0;
-    while (p < p1) p += c2_step(sink, x, y, op, canon);
+    while (p < p1) p += c2_step(sink, x, y, op, canon, p1 - p);
 }
 
-// From code word ia of A starting at position pa and code word ib of B
-// starting at pb, advance to the first position >= max(pa,pb) where both
-// operands start a code word.  Stops at `end` (the total group count), which
-// is always a common boundary.
+// Synchronisation points of the c2 loop: positions where the loop's state
+// is fixed by the position alone, so that a walk may start there -- both
+// operands start a code word, or one starts a code word (not a fill of one
+// group) strictly inside a fill of the other.  (A fill of one group at the
+// start of a copied span could be appended instead of pushed raw.)  From
+// code word ia of A starting at pa and code word ib of B starting at pb,
+// returns the first such position >= max(pa, pb), visiting every word start
+// in order (so that the search is monotone in its start), with ia / pa and
+// ib / pb the words holding it and their starts.  `end` (the total group
+// count, or any synchronisation point) bounds the search.
 template <class R>
-WAH_HD uint32_t find_common(const R* ra, const R* rb, uint32_t& ia, uint32_t& pa,
-                            uint32_t& ib, uint32_t& pb, uint32_t end) {
-    while (pa != pb) {
-        if (pa < pb) { pa += word_len(ra->at(ia)); ++ia; }
-        else { pb += word_len(rb->at(ib)); ++ib; }
-        if (pa >= end && pb >= end) break;
+WAH_HD uint32_t find_sync(const R* ra, const R* rb, uint32_t& ia, uint32_t& pa, uint32_t& ib, uint32_t& pb,
+                          uint32_t end) {
+    uint32_t q = pa > pb ? pa : pb;
+    if (q >= end) return end;
+    // the words holding q
+    while (pa < q) { const uint32_t l = word_len(ra->at(ia)); if (pa + l > q) break; pa += l; ++ia; }
+    while (pb < q) { const uint32_t l = word_len(rb->at(ib)); if (pb + l > q) break; pb += l; ++ib; }
+    for (;;) {
+        if (q >= end) return end;
+        const word_t wa = ra->at(ia), wb = rb->at(ib);
+        const uint32_t la = word_len(wa), lb = word_len(wb);
+        if (pa == q && pb == q) return q;
+        if (pa == q && !(wa > ALLONES && la == 1u)) return q;
+        if (pb == q && !(wb > ALLONES && lb == 1u)) return q;
+        const uint32_t na = pa + la, nb = pb + lb;
+        q = na < nb ? na : nb;
+        if (na == q) { pa = na; ++ia; }
+        if (nb == q) { pb = nb; ++ib; }
     }
-    return pa;
 }
 
 struct ArrayReader {

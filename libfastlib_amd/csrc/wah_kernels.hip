@@ -3,7 +3,8 @@
 //  k_chunk_len      group count of every 32-word chunk (input of the chunk
 //                   position index "ck": start group of code word 32*j)
 //  k_op_tiles       a OP b over two compressed streams: merge-path over the
-//                   chunk index, resynchronisation at common boundaries,
+//                   chunk index, resynchronisation at points where the
+//                   step state is fixed by the position (find_sync),
 //                   the *_c2 step walk per thread, run-summary block scan and
 //                   single-pass look-back, then the word write
 //  k_tok_tiles      compressed output from a literal-domain token source:
@@ -133,16 +134,20 @@ struct GlobWriter {
 // First code word starting at or after position P: looked up in the LDS
 // window (start positions pw[0..cnt], pw[cnt] = end of the window) or, past
 // the window, found by walking the global words.
-__device__ __forceinline__ void first_start_ge(const uint32_t* pw, uint32_t cnt, uint32_t base,
-                                               const word_t* g, uint32_t n, uint32_t P,
-                                               uint32_t& i, uint32_t& pos) {
-    if (P <= pw[cnt]) {
-        uint32_t j = last_le(pw, cnt + 1, P);
-        if (pw[j] < P) ++j;
+__device__ __forceinline__ void holding_word(const uint32_t* pw, uint32_t cnt, uint32_t base,
+                                             const word_t* g, uint32_t n, uint32_t P,
+                                             uint32_t& i, uint32_t& pos) {
+    // the word holding group P: pw[j] <= P < pw[j + 1]
+    if (P < pw[cnt]) {
+        const uint32_t j = last_le(pw, cnt + 1, P);
         i = base + j; pos = pw[j];
     } else {
         i = base + cnt; pos = pw[cnt];
-        while (pos < P && i < n) { pos += word_len(g[i]); ++i; }
+        while (i < n) {
+            const uint32_t l = word_len(g[i]);
+            if (pos + l > P) break;
+            pos += l; ++i;
+        }
     }
 }
 
@@ -151,9 +156,9 @@ __global__ __launch_bounds__(TPB) void k_op_tiles(OpArgs p) {
     __shared__ word_t sB[WIN];
     __shared__ uint32_t posA[WIN + 1];
     __shared__ uint32_t posB[WIN + 1];
-    __shared__ uint32_t tc[TPB + 1], tia[TPB + 1], tib[TPB + 1];
+    __shared__ uint32_t tc[TPB + 1], tia[TPB + 1], tib[TPB + 1], tpa[TPB + 1], tpb[TPB + 1];
     __shared__ Summary sbuf[TPB / WAVE];
-    __shared__ uint32_t info[16];
+    __shared__ uint32_t info[20];
     __shared__ uint32_t shs[TPB / WAVE + 1];
     __shared__ Summary s_pre;
 
@@ -225,20 +230,25 @@ __global__ __launch_bounds__(TPB) void k_op_tiles(OpArgs p) {
     WinReader ra{sA, p.a, baseA, cntA};
     WinReader rb{sB, p.b, baseB, cntB};
 
-    // tile synchronisation points S0 = c(P0), S1 = c(P1)
+    // tile synchronisation points S0 = sync(P0), S1 = sync(P1), with the
+    // words holding them and those words' starts
     if (tid == 0) {
-        uint32_t S[2], IA[2], IB[2];
+        uint32_t S[2], IA[2], IB[2], PA[2], PB[2];
         const uint32_t Pq[2] = {P0, P1};
         for (int q = 0; q < 2; ++q) {
-            if (Pq[q] >= p.groups) { S[q] = p.groups; IA[q] = p.na; IB[q] = p.nb; continue; }
+            if (Pq[q] >= p.groups) {
+                S[q] = p.groups; IA[q] = p.na; IB[q] = p.nb; PA[q] = p.groups; PB[q] = p.groups;
+                continue;
+            }
             uint32_t ia, pa, ib, pb;
-            first_start_ge(posA, cntA, baseA, p.a, p.na, Pq[q], ia, pa);
-            first_start_ge(posB, cntB, baseB, p.b, p.nb, Pq[q], ib, pb);
-            S[q] = find_common(&ra, &rb, ia, pa, ib, pb, p.groups);
-            IA[q] = ia; IB[q] = ib;
+            holding_word(posA, cntA, baseA, p.a, p.na, Pq[q], ia, pa);
+            holding_word(posB, cntB, baseB, p.b, p.nb, Pq[q], ib, pb);
+            S[q] = find_sync(&ra, &rb, ia, pa, ib, pb, p.groups);
+            IA[q] = ia; IB[q] = ib; PA[q] = pa; PB[q] = pb;
         }
         info[9] = S[0]; info[10] = IA[0]; info[11] = IB[0];
         info[12] = S[1]; info[13] = IA[1]; info[14] = IB[1];
+        info[15] = PA[0]; info[16] = PB[0]; info[17] = PA[1]; info[18] = PB[1];
     }
     __syncthreads();
     const uint32_t S0 = info[9], S1 = info[12];
@@ -254,23 +264,32 @@ __global__ __launch_bounds__(TPB) void k_op_tiles(OpArgs p) {
         const uint32_t tot = nA + nB;
         const uint32_t E = (tot + TPB - 1) / TPB;
         const uint32_t d = min((uint32_t)tid * E, tot);
-        uint32_t c, ia, ib;
-        if (d == 0 || S0 >= S1) {
-            c = S0; ia = info[10]; ib = info[11];
-        } else {
+        uint32_t c = S0, ia = info[10], ib = info[11], pa = info[15], pb = info[16];
+        if (d != 0 && S0 < S1) {
             uint32_t lo = d > nB ? d - nB : 0u, hi = d < nA ? d : nA;
             while (lo < hi) {
                 const uint32_t mid = (lo + hi) >> 1;
                 if (posA[ia0 + mid] <= posB[ib0 + d - mid - 1]) lo = mid + 1; else hi = mid;
             }
-            ia = ia0 + lo; ib = ib0 + (d - lo);
-            uint32_t pa = posA[ia], pb = posB[ib];
-            ia += baseA; ib += baseB;
-            c = find_common(&ra, &rb, ia, pa, ib, pb, S1);
-            if (c > S1) c = S1;
+            uint32_t ja = ia0 + lo, jb = ib0 + (d - lo);
+            uint32_t qa = posA[ja], qb = posB[jb];
+            // the split position is the next word start in merged order; the
+            // other operand's previous word holds it.  A split at or before
+            // S0 keeps the tile's own first point (the search is monotone).
+            const uint32_t q0 = qa < qb ? qa : qb;
+            if (q0 > S0) {
+                if (qa < qb) { jb -= 1; qb = posB[jb]; }
+                else if (qb < qa) { ja -= 1; qa = posA[ja]; }
+                ja += baseA; jb += baseB;
+                c = find_sync(&ra, &rb, ja, qa, jb, qb, S1);
+                ia = ja; ib = jb; pa = qa; pb = qb;
+                if (c >= S1) { c = S1; ia = info[13]; ib = info[14]; pa = info[17]; pb = info[18]; }
+            }
         }
-        tc[tid] = c; tia[tid] = ia; tib[tid] = ib;
-        if (tid == 0) { tc[TPB] = S1; tia[TPB] = info[13]; tib[TPB] = info[14]; }
+        tc[tid] = c; tia[tid] = ia; tib[tid] = ib; tpa[tid] = pa; tpb[tid] = pb;
+        if (tid == 0) {
+            tc[TPB] = S1; tia[TPB] = info[13]; tib[TPB] = info[14]; tpa[TPB] = info[17]; tpb[TPB] = info[18];
+        }
     }
     __syncthreads();
     const uint32_t c0 = tc[tid], c1 = tc[tid + 1];
@@ -278,7 +297,7 @@ __global__ __launch_bounds__(TPB) void k_op_tiles(OpArgs p) {
     // count pass
     CountSink cs;
     cs.init();
-    if (c0 < c1) walk(cs, &ra, &rb, tia[tid], tib[tid], c0, c1, p.op, p.canon);
+    if (c0 < c1) walk(cs, &ra, &rb, tia[tid], tpa[tid], tib[tid], tpb[tid], c0, c1, p.op, p.canon);
     Summary agg;
     const Summary texcl = block_excl_summary2<TPB>(cs.s, sbuf, &agg);
     if (tid < WAVE) {
@@ -293,7 +312,7 @@ __global__ __launch_bounds__(TPB) void k_op_tiles(OpArgs p) {
     if (c0 < c1) {
         WriteSink<GlobWriter> ws;
         ws.init(&gw, pre, c0);
-        walk(ws, &ra, &rb, tia[tid], tib[tid], c0, c1, p.op, p.canon);
+        walk(ws, &ra, &rb, tia[tid], tpa[tid], tib[tid], tpb[tid], c0, c1, p.op, p.canon);
         if (c1 == p.groups) {
             ws.finish();
             const uint32_t total = combine(pre, cs.s).count;

@@ -91,3 +91,25 @@ def test_sim_every_cut_point(sim, orc):
         for op in range(4):
             got, _ = run_sim(sim, a, b, op, 0, np.arange(g + 1))
             assert np.array_equal(got, c2_oracle(orc, a, b, op))
+
+
+def test_sim_every_cut_noncanonical_chains(sim, orc):
+    """Operands that are results of earlier *_c2 ops (count-1 fills, raw
+    pushed words) and sparse operands with long fills: every cut point, all
+    four ops, against the oracle's sequential routine."""
+    rng = np.random.default_rng(24)
+    for t in range(20):
+        n = int(rng.integers(300, 30000))
+        parts = [orc.from_bits(markov_bits(n, float(rng.choice([0.002, 0.02, 0.2])),
+                                           float(rng.choice([1, 3, 40])), rng)) for _ in range(4)]
+        a = parts[0]
+        for q in parts[1:3]:
+            a = c2_oracle(orc, a, q, int(rng.integers(0, 4)))
+        b = parts[3] if t % 2 else c2_oracle(orc, parts[3], parts[1], 1)
+        g = groups_of(split_serial(a)[0])
+        for op in range(4):
+            want = c2_oracle(orc, a, b, op)
+            got, _ = run_sim(sim, a, b, op, 0, np.arange(g + 1))
+            assert np.array_equal(got, want), (t, op)
+            got, _ = run_sim(sim, b, a, op, 0, rng.integers(0, g + 1, size=40))
+            assert np.array_equal(got, c2_oracle(orc, b, a, op)), (t, op, "swapped")
