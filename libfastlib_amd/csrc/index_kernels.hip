@@ -165,7 +165,10 @@ __global__ __launch_bounds__(RS_TPB) void k_rs_scatter(const uint32_t* __restric
 }
 
 // ---------------------------------------------------------------------------
-// Per-bitmap encoding.  Bitmap b owns sorted rows [start[b], start[b+1]);
+// Per-bitmap encoding.  Bitmaps of more than ENC_HEAVY rows are encoded by a
+// whole block each (k_encode_heavy); the others by one thread each:
+constexpr uint32_t ENC_HEAVY = 1u << 15;
+// Bitmap b owns sorted rows [start[b], start[b+1]);
 // one thread walks them and writes the canonical WAH code words of the rows
 // below groups*31 into tmp[toff_b ..] (room for 2*count+2 words), the active
 // bits (rows >= groups*31) into aval[b], the code word count into nw[b] and,
@@ -183,6 +186,7 @@ __global__ __launch_bounds__(IX_TPB) void k_encode(const uint32_t* __restrict__ 
                                                    double* __restrict__ vmax) {
     const uint32_t b = blockIdx.x * IX_TPB + threadIdx.x;
     if (b >= nbm) return;
+    if (start[b + 1] - start[b] > ENC_HEAVY) return;        // k_encode_heavy's
     uint32_t* out = tmp + tmpoff[b];
     uint32_t k = 0;               // words written
     int64_t last = -1;            // last group emitted (covered)
@@ -245,6 +249,206 @@ __global__ __launch_bounds__(IX_TPB) void k_encode(const uint32_t* __restrict__ 
     nw[b] = k;
     aval[b] = av;
     if (vmin) { vmin[b] = mn; vmax[b] = mx; }
+}
+
+// A bitmap of more than ENC_HEAVY rows, by the block of blockIdx.x (other
+// blocks return): the same words as k_encode, chunk by chunk of
+// IX_TPB*ENC_K consecutive sorted rows.  In a chunk every thread takes
+// ENC_K consecutive rows; the first row of each group heads a unit (the
+// group's literal is the OR of its rows, at most 31); a unit writes a 0-gap
+// word before it when its group does not follow the previous unit's, and a
+// word of its own unless it continues a run of full groups.  The previous
+// unit of a thread's first unit comes from a block max-scan of the packed
+// (group, full) of the last units (groups increase along the rows), the
+// previous chunk's from LDS.  The head of a run of full groups counts the
+// run by checking 31 rows per group ahead.
+constexpr int ENC_K = 8;
+
+// Exclusive / inclusive block-wide max of 32-bit values (>= 0).
+template <int NT>
+__device__ __forceinline__ uint32_t block_excl_max_u32(uint32_t v, uint32_t* sh) {
+    const uint32_t w = threadIdx.x / WAVE;
+    uint32_t inc = v;
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = (uint32_t)__shfl_up((int)inc, d, 64);
+        if ((int)lane_id() >= d) inc = inc > o ? inc : o;
+    }
+    if (lane_id() == WAVE - 1) sh[w] = inc;
+    __syncthreads();
+    uint32_t before = 0;
+    for (uint32_t i = 0; i < w; ++i) before = sh[i] > before ? sh[i] : before;
+    uint32_t ex = (uint32_t)__shfl_up((int)inc, 1, 64);
+    if (lane_id() == 0) ex = 0;
+    __syncthreads();
+    return ex > before ? ex : before;
+}
+
+template <int NT>
+__device__ __forceinline__ uint32_t block_max_u32(uint32_t v, uint32_t* sh) {
+    for (int d = 32; d >= 1; d >>= 1) { const uint32_t o = (uint32_t)__shfl_xor((int)v, d, 64); v = v > o ? v : o; }
+    if (lane_id() == 0) sh[threadIdx.x / WAVE] = v;
+    __syncthreads();
+    uint32_t m = 0;
+    for (int i = 0; i < NT / WAVE; ++i) m = sh[i] > m ? sh[i] : m;
+    __syncthreads();
+    return m;
+}
+
+// First index j in [s, e) with rows[j] >= r (rows ascending), else e.
+__device__ __forceinline__ uint32_t lower_row(const uint32_t* rows, uint32_t s, uint32_t e, uint32_t r) {
+    uint32_t lo = s, hi = e;
+    while (lo < hi) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        if (rows[mid] < r) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+
+template <class T>
+__global__ __launch_bounds__(IX_TPB) void k_encode_heavy(const uint32_t* __restrict__ rows,
+                                                         const uint32_t* __restrict__ start, uint32_t groups,
+                                                         uint32_t anb, const uint64_t* __restrict__ tmpoff,
+                                                         uint32_t* __restrict__ tmp, uint32_t* __restrict__ nw,
+                                                         uint32_t* __restrict__ aval,
+                                                         const T* __restrict__ col,
+                                                         double* __restrict__ vmin, double* __restrict__ vmax) {
+    const uint32_t b = blockIdx.x;
+    const uint32_t s = start[b], e = start[b + 1];
+    if (e - s <= ENC_HEAVY) return;
+    __shared__ uint32_t sh[IX_TPB / WAVE + 1];
+    __shared__ uint64_t s_last;        // packed (group + 1) << 1 | full of the last unit so far, 0 = none
+    __shared__ uint32_t s_k;           // words written so far
+    __shared__ uint32_t s_av;
+    __shared__ double s_mn[IX_TPB / WAVE], s_mx[IX_TPB / WAVE];
+    uint32_t* out = tmp + tmpoff[b];
+    const uint32_t lim = groups * 31u;
+    if (threadIdx.x == 0) { s_last = 0; s_k = 0; s_av = 0; }
+    double mn = DBL_MAX_D, mx = -DBL_MAX_D;
+    uint32_t av = 0;
+    __syncthreads();
+    for (uint32_t base = s; base < e; base += IX_TPB * ENC_K) {
+        const uint32_t i0 = base + threadIdx.x * ENC_K;
+        // units of this thread: heads among its rows
+        uint32_t ug[ENC_K];
+        word_t uv[ENC_K];
+        int nu = 0;
+        uint32_t prevg = 0xFFFFFFFFu;
+        if (i0 > s && i0 < e) { const uint32_t r = rows[i0 - 1]; prevg = r < lim ? r / 31u : 0xFFFFFFFEu; }
+        #pragma unroll
+        for (int k = 0; k < ENC_K; ++k) {
+            const uint32_t i = i0 + (uint32_t)k;
+            if (i >= e) break;
+            const uint32_t r = rows[i];
+            if (col) {
+                const double x = (double)col[r];
+                mn = x < mn ? x : mn;
+                mx = x > mx ? x : mx;
+            }
+            if (r >= lim) { av |= 1u << (anb - 1u - (r - lim)); prevg = 0xFFFFFFFEu; continue; }
+            const uint32_t g = r / 31u;
+            if (g == prevg) continue;
+            prevg = g;
+            word_t v = 0;
+            for (uint32_t j = i; j < e && j < i + 31u; ++j) {
+                const uint32_t r2 = rows[j];
+                if (r2 / 31u != g || r2 >= lim) break;
+                v |= 1u << (30u - (r2 - g * 31u));
+            }
+            ug[nu] = g; uv[nu] = v; ++nu;
+        }
+        // previous unit of this thread's first unit
+        const uint64_t mine = nu ? ((uint64_t)(ug[nu - 1] + 1u) << 1) | (uv[nu - 1] == ALLONES ? 1u : 0u) : 0ull;
+        uint64_t before;
+        {
+            // block exclusive max of 64-bit packed values (two passes of 32 bits suffice: the
+            // group part decides, groups < 2^27)
+            const uint32_t hi = (uint32_t)(mine >> 1), lo = (uint32_t)(mine & 1u);
+            const uint32_t key = hi * 2u + lo;      // fits: (groups + 1) * 2 < 2^32
+            uint32_t ex = block_excl_max_u32<IX_TPB>(key, sh);
+            const uint64_t carry = s_last;
+            before = ex ? (uint64_t)ex : carry;
+        }
+        // words per unit
+        uint32_t cnt = 0;
+        uint64_t prev = before;
+        #pragma unroll
+        for (int u = 0; u < ENC_K; ++u) {
+            if (u >= nu) break;
+            const int64_t pg = prev ? (int64_t)(prev >> 1) - 1 : -1;
+            const bool pfull = prev & 1u;
+            const int64_t gap = (int64_t)ug[u] - pg - 1;
+            const bool merge = uv[u] == ALLONES && pfull && gap == 0;
+            cnt += (gap > 0 ? 1u : 0u) + (merge ? 0u : 1u);
+            prev = ((uint64_t)(ug[u] + 1u) << 1) | (uv[u] == ALLONES ? 1u : 0u);
+        }
+        uint32_t tot;
+        uint32_t pos = block_excl_sum<IX_TPB>(cnt, sh, &tot) + s_k;
+        prev = before;
+        #pragma unroll
+        for (int u = 0; u < ENC_K; ++u) {
+            if (u >= nu) break;
+            const int64_t pg = prev ? (int64_t)(prev >> 1) - 1 : -1;
+            const bool pfull = prev & 1u;
+            const int64_t gap = (int64_t)ug[u] - pg - 1;
+            const bool merge = uv[u] == ALLONES && pfull && gap == 0;
+            if (gap > 0) out[pos++] = gap >= 2 ? (HEADER0 | (uint32_t)gap) : 0u;
+            if (!merge) {
+                if (uv[u] == ALLONES) {
+                    // a run of full groups: count the following full groups
+                    uint32_t L = 1;
+                    uint32_t j = lower_row(rows, s, e, (ug[u] + 1u) * 31u);
+                    for (;;) {
+                        const uint32_t g2 = ug[u] + L;
+                        if (g2 >= groups || j + 30u >= e) break;
+                        if (rows[j] != g2 * 31u || rows[j + 30u] != g2 * 31u + 30u) break;
+                        ++L; j += 31u;
+                    }
+                    out[pos++] = L >= 2 ? (HEADER1 | L) : ALLONES;
+                } else {
+                    out[pos++] = uv[u];
+                }
+            }
+            prev = ((uint64_t)(ug[u] + 1u) << 1) | (uv[u] == ALLONES ? 1u : 0u);
+        }
+        __syncthreads();
+        // carry: the last unit of the chunk and the words so far
+        if (threadIdx.x == IX_TPB - 1) {
+            s_k += tot;
+        }
+        const uint32_t lastkey = block_max_u32<IX_TPB>((uint32_t)mine, sh);
+        if (threadIdx.x == 0 && lastkey) s_last = lastkey;
+        __syncthreads();
+    }
+    // active bits, min / max
+    {
+        uint32_t a = av;
+        for (int d = 32; d >= 1; d >>= 1) a |= (uint32_t)__shfl_xor((int)a, d, 64);
+        if (lane_id() == 0 && a) atomicOr(&s_av, a);
+        if (col) {
+            for (int d = 32; d >= 1; d >>= 1) {
+                const double o1 = __shfl_xor(mn, d, 64), o2 = __shfl_xor(mx, d, 64);
+                mn = o1 < mn ? o1 : mn;
+                mx = o2 > mx ? o2 : mx;
+            }
+            if (lane_id() == 0) { s_mn[threadIdx.x / WAVE] = mn; s_mx[threadIdx.x / WAVE] = mx; }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t k = s_k;
+        const uint64_t last = s_last;
+        const int64_t lg = last ? (int64_t)(last >> 1) - 1 : -1;
+        const int64_t gap = (int64_t)groups - lg - 1;
+        if (gap > 0) out[k++] = gap >= 2 ? (HEADER0 | (uint32_t)gap) : 0u;
+        nw[b] = k;
+        aval[b] = s_av;
+        if (col && vmin) {
+            double a = DBL_MAX_D, z = -DBL_MAX_D;
+            for (int w = 0; w < IX_TPB / WAVE; ++w) { a = s_mn[w] < a ? s_mn[w] : a; z = s_mx[w] > z ? s_mx[w] : z; }
+            vmin[b] = a; vmax[b] = z;
+        }
+    }
 }
 
 // One block writes the literal form of a canonical bitmap (the decompress of
@@ -1105,18 +1309,36 @@ hipError_t launch_encode(int type, const uint32_t* rows, const uint32_t* start, 
                          uint32_t groups, uint32_t anb, const uint64_t* tmpoff, uint32_t* tmp,
                          uint32_t* nw, uint32_t* aval, const void* col, double* vmin, double* vmax,
                          hipStream_t s) {
-    dim3 g(cdivu(nbm, IX_TPB)), b(IX_TPB);
+    dim3 g(cdivu(nbm, IX_TPB)), b(IX_TPB), gh(nbm);
+    if (!nbm) return hipSuccess;
     if (!col) {
         hipLaunchKernelGGL(k_encode<float>, g, b, 0, s, rows, start, nbm, groups, anb, tmpoff, tmp, nw,
+                           aval, (const float*)nullptr, (double*)nullptr, (double*)nullptr);
+        hipLaunchKernelGGL(k_encode_heavy<float>, gh, b, 0, s, rows, start, groups, anb, tmpoff, tmp, nw,
                            aval, (const float*)nullptr, (double*)nullptr, (double*)nullptr);
         return hipGetLastError();
     }
     switch (type) {
-    case COL_I32: hipLaunchKernelGGL(k_encode<int32_t>, g, b, 0, s, rows, start, nbm, groups, anb, tmpoff, tmp, nw, aval, (const int32_t*)col, vmin, vmax); break;
-    case COL_U32: hipLaunchKernelGGL(k_encode<uint32_t>, g, b, 0, s, rows, start, nbm, groups, anb, tmpoff, tmp, nw, aval, (const uint32_t*)col, vmin, vmax); break;
-    case COL_U16: hipLaunchKernelGGL(k_encode<uint16_t>, g, b, 0, s, rows, start, nbm, groups, anb, tmpoff, tmp, nw, aval, (const uint16_t*)col, vmin, vmax); break;
-    case COL_F32: hipLaunchKernelGGL(k_encode<float>, g, b, 0, s, rows, start, nbm, groups, anb, tmpoff, tmp, nw, aval, (const float*)col, vmin, vmax); break;
-    case COL_F64: hipLaunchKernelGGL(k_encode<double>, g, b, 0, s, rows, start, nbm, groups, anb, tmpoff, tmp, nw, aval, (const double*)col, vmin, vmax); break;
+    case COL_I32:
+        hipLaunchKernelGGL(k_encode<int32_t>, g, b, 0, s, rows, start, nbm, groups, anb, tmpoff, tmp, nw, aval, (const int32_t*)col, vmin, vmax);
+        hipLaunchKernelGGL(k_encode_heavy<int32_t>, gh, b, 0, s, rows, start, groups, anb, tmpoff, tmp, nw, aval, (const int32_t*)col, vmin, vmax);
+        break;
+    case COL_U32:
+        hipLaunchKernelGGL(k_encode<uint32_t>, g, b, 0, s, rows, start, nbm, groups, anb, tmpoff, tmp, nw, aval, (const uint32_t*)col, vmin, vmax);
+        hipLaunchKernelGGL(k_encode_heavy<uint32_t>, gh, b, 0, s, rows, start, groups, anb, tmpoff, tmp, nw, aval, (const uint32_t*)col, vmin, vmax);
+        break;
+    case COL_U16:
+        hipLaunchKernelGGL(k_encode<uint16_t>, g, b, 0, s, rows, start, nbm, groups, anb, tmpoff, tmp, nw, aval, (const uint16_t*)col, vmin, vmax);
+        hipLaunchKernelGGL(k_encode_heavy<uint16_t>, gh, b, 0, s, rows, start, groups, anb, tmpoff, tmp, nw, aval, (const uint16_t*)col, vmin, vmax);
+        break;
+    case COL_F32:
+        hipLaunchKernelGGL(k_encode<float>, g, b, 0, s, rows, start, nbm, groups, anb, tmpoff, tmp, nw, aval, (const float*)col, vmin, vmax);
+        hipLaunchKernelGGL(k_encode_heavy<float>, gh, b, 0, s, rows, start, groups, anb, tmpoff, tmp, nw, aval, (const float*)col, vmin, vmax);
+        break;
+    case COL_F64:
+        hipLaunchKernelGGL(k_encode<double>, g, b, 0, s, rows, start, nbm, groups, anb, tmpoff, tmp, nw, aval, (const double*)col, vmin, vmax);
+        hipLaunchKernelGGL(k_encode_heavy<double>, gh, b, 0, s, rows, start, groups, anb, tmpoff, tmp, nw, aval, (const double*)col, vmin, vmax);
+        break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -334,3 +334,29 @@ def test_query_count_kat(L):
             qs = [[(idx[(table, c)], lo, lop, hi, rop) for (c, lo, lop, hi, rop) in q["terms"]] for q in sel]
             hits, _ = L.Conjunctions(qs).run()
             assert [int(h) for h in hits] == [q["count"] for q in sel], (kind, table)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_heavy_bitmaps_block_encoder(L, seed):
+    """Bitmaps of more than 2^15 rows go through the block encoder: long
+    runs of full groups, runs crossing its chunks, active-word rows, and
+    partial groups between runs; words identical to the oracle's."""
+    rng = np.random.default_rng(seed)
+    n = 31 * 9000 + 17
+    col = rng.integers(0, 6, size=n).astype(np.int32)
+    col[: n * 2 // 5] = 1                         # a long run of full groups
+    col[150000:152500] = 2                        # a shorter run, not group aligned
+    col[200000:260000:2] = 3                      # dense but never full
+    col[-40:] = 4                                 # the active word and the last groups
+    gi = L.Index.relic(col)
+    oi = O.OracleIndex.relic_values(col)
+    _same_words(gi, oi)
+    fcol = col.astype(np.float32) + 0.25
+    bounds = np.array([0.5, 1.5, 2.5, 3.5, 4.5, np.finfo(np.float64).max])
+    gb = L.Index.bin(fcol, bounds)
+    ob = O.OracleIndex.bin_bounds(fcol, bounds)
+    _same_words(gb, ob)
+    for (lo, lop, hi, rop) in [(0.0, 2, 2.0, 1), (1.0, 2, 5.0, 2), (3.2, 1, 4.6, 1)]:
+        c, r = ob.evaluate(lo, lop, hi, rop)
+        h, _, _ = gb.evaluate(lo, lop, hi, rop, 1)
+        assert h == c
