@@ -106,11 +106,24 @@ def wah_cells(L, reps=40, copies=4):
     return out
 
 
-def conj_section(L, log2n, nq, steps):
+def relic_ranges(seed, k, card=65536):
+    """k ranges lo <= x < hi over integer values [0, card): lo uniform,
+    width log-uniform in [1, card]."""
+    r = np.random.default_rng(seed)
+    lo = r.integers(0, card, size=k)
+    w = np.floor(2.0 ** r.uniform(0, np.log2(card), size=k)).astype(np.int64)
+    hi = np.minimum(lo + np.maximum(w, 1), card)
+    return lo.astype(np.float64), hi.astype(np.float64)
+
+
+def conj_section(L, log2n, nq, steps, learned=None):
     """4 columns of 2^log2n rows (uint16 uniform -> relic; int32 Zipf(1.1),
     float32 N(0,1), int32 sorted runs -> 10000 equal-weight bins), nq
     queries ANDing 2-4 range terms of selectivity log-uniform in
-    [1e-6, 0.3]; one prepared, autotuned, graph-captured batch."""
+    [1e-6, 0.3]; one prepared, autotuned, graph-captured batch.  With
+    learned = (train, test, epochs), the learned plan choice also runs on
+    the uint16 relic column (65,536 bitmaps; uniform lo, log-uniform
+    widths), where the ibis rule prices wide ranges as scans."""
     from libfastlib_amd import datagen as G
     n = 1 << log2n
     t = time.time()
@@ -130,13 +143,23 @@ def conj_section(L, log2n, nq, steps):
         hits, m = batch.run()
         ms.append(m)
     plans = batch.plans()
+    # the same batch with every term scanned gives the same counts
+    ref, _ = L.Conjunctions([[(ixs[ci], lo, 2, hi, 1) for (ci, lo, hi) in q] for q in qs], plan=3).run()
+    if not np.array_equal(ref, hits):
+        raise RuntimeError("conjunction counts differ from the scan plan's")
     out = dict(rows=n, queries=nq, terms=int(batch.nterms), index_build_s=round(build_s, 2),
                qps=round(nq / (np.median(ms) * 1e-3), 1), ms_per_batch=round(float(np.median(ms)), 3),
                tuned=cfg, term_plans={"literal": int(np.sum(plans == 1)), "scan": int(np.sum(plans == 3))},
                hits_total=int(hits.sum()))
     log("conj", out)
+    rel = None
+    if learned:
+        ntr, nte, epochs = learned
+        rel = learned_section(L, ixs[0], relic_ranges(7001, ntr), relic_ranges(7002, nte), epochs)
+        rel["workload"] = (f"relic over the uint16 column of 2^{log2n} rows (65,536 bitmaps), ranges "
+                           f"lo <= x < hi with lo uniform and width log-uniform in [1, 65536]")
     del ixs
-    return out
+    return out, rel
 
 
 def learned_section(L, ix, train, test, epochs):
@@ -312,7 +335,8 @@ def main():
 
     learned = None if args.no_learned else learned_section(L, ix, more[0], more[1], args.epochs)
     wah = None if args.no_wah else wah_cells(L)
-    conj = None if args.no_conj else conj_section(L, 26, 4096, 5)
+    conj, relic_learned = (None, None) if args.no_conj else conj_section(
+        L, 26, 4096, 5, None if args.no_learned else (args.train_queries, args.test_queries, args.epochs))
     cpu = None if args.no_cpu else cpu_baseline(L, ix, col, lo, hi, hits, args.cpu_sample)
 
     out = {
@@ -338,6 +362,7 @@ def main():
         "best_plan_bytes_model_GB": round(float(np.minimum(bl, bs).sum()) / 1e9, 3),
         "roofline": roofline,
         "learned_plan": learned,
+        "learned_plan_relic_2p26": relic_learned,
         "wah_ops_2p28": wah,
         "conjunctions_4x2p26": conj,
         "cpu_baseline": cpu,

@@ -610,8 +610,11 @@ __global__ __launch_bounds__(IX_TPB) void k_tseg_fill(const uint32_t* __restrict
                                                       const uint32_t* __restrict__ TW, uint32_t nobs,
                                                       uint32_t ntiles, uint32_t groups, uint32_t anb,
                                                       uint32_t* __restrict__ tw) {
-    const uint32_t idx = blockIdx.x * (IX_TPB / WAVE) + threadIdx.x / WAVE;
-    if (idx >= ntiles * nobs) return;
+    // segments in grid-stride order: a grid of one wave per segment can
+    // exceed 2^32 work-items (the dispatch's 32-bit grid size)
+    const uint32_t nseg = ntiles * nobs;
+    for (uint32_t idx = blockIdx.x * (IX_TPB / WAVE) + threadIdx.x / WAVE; idx < nseg;
+         idx += gridDim.x * (IX_TPB / WAVE)) {
     const int lane = (int)lane_id();
     const uint32_t t = idx / nobs, b = idx - t * nobs;
     uint32_t s, e, p;
@@ -640,6 +643,7 @@ __global__ __launch_bounds__(IX_TPB) void k_tseg_fill(const uint32_t* __restrict
     if (lane == 0 && t + 1 == ntiles && anb > 0) {
         const uint2 E = toff[(size_t)ntiles * nobs + b];
         if (E.y) tw[base + L] = words[E.x] << (31u - anb);
+    }
     }
 }
 
@@ -1373,8 +1377,8 @@ hipError_t launch_tseg_fill(const uint32_t* words, const uint2* toff, const uint
                             uint32_t ntiles, uint32_t groups, uint32_t anb, uint32_t* tw, hipStream_t s) {
     const uint32_t n = ntiles * nobs;
     if (!n) return hipSuccess;
-    hipLaunchKernelGGL(k_tseg_fill, dim3(cdivu(n, IX_TPB / WAVE)), dim3(IX_TPB), 0, s, words, toff, TW, nobs,
-                       ntiles, groups, anb, tw);
+    hipLaunchKernelGGL(k_tseg_fill, dim3(min(cdivu(n, IX_TPB / WAVE), 1u << 20)), dim3(IX_TPB), 0, s, words,
+                       toff, TW, nobs, ntiles, groups, anb, tw);
     return hipGetLastError();
 }
 

@@ -360,3 +360,22 @@ def test_heavy_bitmaps_block_encoder(L, seed):
         c, r = ob.evaluate(lo, lop, hi, rop)
         h, _, _ = gb.evaluate(lo, lop, hi, rop, 1)
         assert h == c
+
+
+def test_tiled_copy_beyond_2p32_work_items(L):
+    """65,536 bitmaps over 2^26 rows: the tiled copy has 69M segments, more
+    than 2^32 work-items at one wave per segment; every tile must be filled
+    (the literal plan agrees with a direct count in every tile)."""
+    n = 1 << 26
+    col = np.random.default_rng(26).integers(0, 65536, size=n).astype(np.uint16)
+    ix = L.Index.relic(col)
+    cnt = np.bincount(col, minlength=65536)
+    cum = np.concatenate([[0], np.cumsum(cnt)])
+    lo = np.array([0, 11143, 33960, 51824, 100, 65000], np.float64)
+    hi = np.array([7, 11165, 53459, 51825, 60000, 65536], np.float64)
+    want = cum[hi.astype(np.int64)] - cum[lo.astype(np.int64)]
+    for plan in (1, 3):
+        h, _, _ = ix.evaluate_batch(lo, 2, hi, 1, plan=plan)
+        np.testing.assert_array_equal(h, want.astype(np.uint64))
+    r = ix.rids(11143.0, 2, 11165.0, 1, 1)
+    np.testing.assert_array_equal(r, np.nonzero((col >= 11143) & (col < 11165))[0])
