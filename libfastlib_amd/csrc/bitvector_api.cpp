@@ -330,8 +330,8 @@ int launch_word_op(fbh_bitvector* a, fbh_bitvector* b, int op, bool c2, uint32_t
                                     ob.d, ob.ck, (uint64_t*)(sc + 64), *ctl_out, stream()), "op");
     }
     const uint32_t nt = fbk::canon_tiles(groups);
-    // [ctl 64 B][status 8 B per tile][tile starts 2 x (nt + 1) x 8 B]
-    const size_t ts_off = (64 + 8ull * nt + 15) & ~size_t(15);
+    // [ctl 64 B][status 8 B per entry][tile starts 2 x (nt + 1) x 8 B]
+    const size_t ts_off = (64 + 8ull * fbk::canon_status(groups) + 15) & ~size_t(15);
     uint8_t* sc = (uint8_t*)scratch(0).get(ts_off + 16ull * (nt + 1) + 64);
     if (!sc) return fail(FBH_ERR_NOMEM, "scratch allocation failed");
     *ctl_out = (uint32_t*)sc;

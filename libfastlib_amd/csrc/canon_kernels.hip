@@ -21,6 +21,7 @@
 //      dispatcher has started before it (in-order dispatch), so there is no
 //      shared ticket counter: same-address atomics serialise (~20 ns each).
 #include "dev_common.h"
+#include <mutex>
 
 namespace fbk {
 using namespace wah;
@@ -318,281 +319,567 @@ __global__ __launch_bounds__(CT_TPB) void k_canon_tiles(CanonArgs p) {
     }
 }
 
-// Literal-only operands (one word per group, the c0 routine and compress()
-// of a literal array) take three launches instead of one tile pass with a
-// look-back: with thousands of tiles resident, waiting on predecessors'
-// status words costs more than reading the operands twice (the second
-// read mostly hits the last-level cache).
-//   k_lit_count: per tile, the number of heads and the last head + 1;
-//   k_lit_scan:  one block turns them into exclusive prefixes in place;
-//   k_lit_write: per tile again, every output word at its slot.
-// Lane l of wave w owns, in slice q of a tile, the four consecutive groups
-// g0 + q*LT_SLICE + (w*64 + l)*4 + j, read with one 16-byte load per
-// operand.  Neighbours of a lane's first and last group come from the
-// adjacent lanes by shuffles, across waves and slices from two small LDS
-// tables.  Head counts are summed per (slice, wave) segment, whose order is
-// the group order.
-constexpr uint32_t LT_SLICE = CT_TPB * 4u;   // groups per slice
+// Literal-only operands (one word per group: the c0 routine and compress()
+// of a literal array).  The 16-byte chunks (four groups) of the bitvector
+// are split evenly over the waves of a grid that is resident at once, so
+// every wave streams the same number of bytes.  A wave takes its span in
+// rounds of 64 chunks, one per lane, and loads up to LF_R rounds at once.
+// A group heads an output word when it is a mixed literal or its class
+// (all-0 / all-1 / mixed) differs from the group before; it ends one when it
+// is a mixed literal or its class differs from the group after.  A round's
+// classes are wave-wide lane masks (one compare per group and class, the
+// neighbours by DPP wave shifts, the counts by scalar popcounts); a round of
+// mixed literals only, where every group is its own word, takes a shorter
+// path.  The words a round ends take consecutive slots; they are stored
+// straight from registers when the round's words are its values and the
+// first slot is 16-byte aligned, else through a per-wave LDS row.
+// Operands whose spans fit one load batch per wave take one launch
+// (k_lit_fused, below); larger ones two, with no waiting between blocks:
+//   k_lit_count: (heads, last head + 1) of every wave's span, and of every
+//                block;
+//   k_lit_write: each block sums the totals of the blocks before it (and of
+//                its own waves before each wave), reads its span again and
+//                writes the words.
+constexpr int LW_TPB = 256;
+constexpr int LW_NW = LW_TPB / WAVE;
+constexpr uint32_t LW_ROW = 4u * WAVE;       // words one round can end
+constexpr int LF_R = 5;                      // rounds a wave loads at once
+constexpr int LW_PRE = 8;                    // block totals a thread sums
+constexpr uint32_t LW_MAXB = LW_TPB * LW_PRE;
 
-template <int S>
-struct LitTile {
-    uint4 v[S];
-    uint32_t heads = 0, ends = 0;     // bit 4q+j: group j of slice q heads / ends a word
+struct LitSpan {
+    uint32_t c0, c1, nr;                     // chunks [c0, c1) in nr rounds
 };
+
+// Wave k's span: n4 / nwaves chunks, one more for the first n4 % nwaves waves.
+__device__ __forceinline__ LitSpan lit_span(uint32_t n4, uint32_t k, uint32_t nwaves) {
+    const uint32_t per = n4 / nwaves, rem = n4 % nwaves;
+    LitSpan s;
+    s.c0 = k * per + min(k, rem);
+    s.c1 = s.c0 + per + (k < rem ? 1u : 0u);
+    s.nr = (s.c1 - s.c0 + WAVE - 1) / WAVE;
+    return s;
+}
 
 __device__ __forceinline__ word_t comp(const uint4& x, int j) {
     return j == 0 ? x.x : (j == 1 ? x.y : (j == 2 ? x.z : x.w));
 }
 
-// Four groups from 16-byte group i4 of a literal operand (whole or partial
-// at the end of the bitvector: no read past groups).
-__device__ __forceinline__ uint4 load4(const word_t* w, uint32_t i4, uint32_t groups) {
-    const uint32_t g = i4 * 4u;
-    if (g + 4u <= groups) return reinterpret_cast<const uint4*>(w)[i4];
-    uint4 r = make_uint4(SENT, SENT, SENT, SENT);
-    if (g < groups) r.x = w[g];
-    if (g + 1u < groups) r.y = w[g + 1u];
-    if (g + 2u < groups) r.z = w[g + 2u];
-    return r;
+template <int OP>
+__device__ __forceinline__ word_t lit_op1(word_t a, word_t b) {
+    if constexpr (OP < 0) return a;
+    else return apply_op(a, b, OP);
 }
 
-__device__ __forceinline__ uint4 op4(const uint4& a, const uint4& b, int op, uint32_t g, uint32_t groups) {
-    uint4 r;
-    r.x = g < groups ? apply_op(a.x, b.x, op) : SENT;
-    r.y = g + 1u < groups ? apply_op(a.y, b.y, op) : SENT;
-    r.z = g + 2u < groups ? apply_op(a.z, b.z, op) : SENT;
-    r.w = g + 3u < groups ? apply_op(a.w, b.w, op) : SENT;
-    return r;
+template <int OP>
+__device__ __forceinline__ uint4 lit_op4(const uint4& a, const uint4& b) {
+    if constexpr (OP < 0) return a;
+    else return make_uint4(apply_op(a.x, b.x, OP), apply_op(a.y, b.y, OP), apply_op(a.z, b.z, OP),
+                           apply_op(a.w, b.w, OP));
 }
 
-// Loads the op's values of the lane's groups and classifies them.  The LDS
-// tables need S*CT_NW words each, s_edge 2 words.  Ends with the head count
-// and the last head + 1 of every (slice, wave) segment in s_cnt / s_lh.
-template <int S>
-struct LitRaw {
-    uint4 x[S], y[S];
+// The op's value of group g (0 <= g < groups), a uniform load.
+template <int OP>
+__device__ __forceinline__ word_t lit_group(const CanonArgs& p, uint32_t g) {
+    return lit_op1<OP>(p.a.w[g], OP < 0 ? 0u : p.b.w[g]);
+}
+
+// Uniform words of a span, every load in bounds (clamped): the groups
+// before and after the span, the partial last chunk of the bitvector.
+struct LitEdges {
+    word_t before, after;
+    uint4 tail;
+    uint32_t n4f;                            // whole chunks (>= 1)
+    uint32_t cmax;                           // last chunk a lane reads whole
 };
 
-// Issues the loads of the tile at g0 (tiles past the end load nothing).
-template <int S>
-__device__ __forceinline__ void lit_load(const CanonArgs& p, uint32_t g0, LitRaw<S>& r) {
-    const uint32_t groups = p.groups;
-    const bool two = p.b.w != nullptr;
+template <int OP>
+__device__ __forceinline__ LitEdges lit_edges(const CanonArgs& p, const LitSpan& s) {
+    LitEdges e;
+    const uint32_t groups = p.groups, gl = groups - 1u;
+    e.n4f = groups / 4u;
+    e.cmax = max(min(s.c1, e.n4f), 1u) - 1u;
+    e.before = lit_group<OP>(p, max(4u * s.c0, 1u) - 1u);
+    e.after = lit_group<OP>(p, min(4u * s.c1, gl));
+    e.tail.x = lit_group<OP>(p, min(4u * e.n4f, gl));
+    e.tail.y = lit_group<OP>(p, min(4u * e.n4f + 1u, gl));
+    e.tail.z = lit_group<OP>(p, min(4u * e.n4f + 2u, gl));
+    e.tail.w = SENT;
+    if (s.c0 == 0) e.before = SENT;
+    if (4u * s.c1 >= groups) e.after = SENT;
+    if (4u * e.n4f >= groups) e.tail.x = SENT;
+    if (4u * e.n4f + 1u >= groups) e.tail.y = SENT;
+    if (4u * e.n4f + 2u >= groups) e.tail.z = SENT;
+    return e;
+}
+
+// The op's values of rounds rb .. rb + LF_R - 1 of the span, every load in
+// flight at once (lanes past the span read its last whole chunk; `last`
+// masks them off later).
+template <int OP>
+__device__ __forceinline__ void lit_load(const CanonArgs& p, const LitSpan& s, const LitEdges& e, uint32_t rb,
+                                         uint4 (&v)[LF_R]) {
+    const uint32_t lane = lane_id();
+    const uint4* A = reinterpret_cast<const uint4*>(p.a.w);
+    const uint4* B = reinterpret_cast<const uint4*>(p.b.w);
+    uint4 y[LF_R];
     #pragma unroll
-    for (int q = 0; q < S; ++q) {
-        const uint32_t i4 = (g0 + (uint32_t)q * LT_SLICE) / 4u + (uint32_t)threadIdx.x;
-        r.x[q] = load4(p.a.w, i4, groups);
-        r.y[q] = two ? load4(p.b.w, i4, groups) : make_uint4(0u, 0u, 0u, 0u);
+    for (int r = 0; r < LF_R; ++r) {
+        const uint32_t ic = min(s.c0 + (rb + (uint32_t)r) * WAVE + lane, e.cmax);
+        v[r] = A[ic];
+        if constexpr (OP >= 0) y[r] = B[ic];
+    }
+    #pragma unroll
+    for (int r = 0; r < LF_R; ++r) {
+        v[r] = lit_op4<OP>(v[r], y[r]);
+        if (s.c0 + (rb + (uint32_t)r) * WAVE + lane == e.n4f) v[r] = e.tail;
     }
 }
 
-template <int S>
-__device__ __forceinline__ void lit_wait(const LitRaw<S>& r) {
+// Class masks of one round (bit l: lane l, group 4(i0 + l) + j).  Groups on
+// lanes past `last`, or past the bitvector (SENT), are not valid.  Two fills
+// of equal value are the same class, so a group differs in class from its
+// neighbour exactly when it is mixed or their values differ.  prev / aft:
+// the groups before lane 0 and after lane `last`.
+struct LfMasks {
+    uint64_t M[4], H[4], E[4];     // mixed, head, end
+};
+
+__device__ __forceinline__ uint64_t lanes_upto(uint32_t last) {
+    return last >= (uint32_t)WAVE - 1u ? ~0ull : ((2ull << last) - 1ull);
+}
+
+__device__ __forceinline__ void lf_masks(const uint4& v, word_t prev, word_t aft, uint32_t last, LfMasks& m) {
+    const uint64_t own = lanes_upto(last);
+    const word_t xp = (word_t)__builtin_amdgcn_update_dpp((int)prev, (int)v.w, 0x138, 0xf, 0xf, false); // wave_shr:1
+    word_t xn = (word_t)__builtin_amdgcn_update_dpp((int)aft, (int)v.x, 0x130, 0xf, 0xf, false);        // wave_shl:1
+    if (lane_id() == last) xn = aft;
+    const word_t b[6] = {xp, v.x, v.y, v.z, v.w, xn};
+    uint64_t D[5];
     #pragma unroll
-    for (int q = 0; q < S; ++q) {
-        use_loaded(r.x[q].x); use_loaded(r.x[q].y); use_loaded(r.x[q].z); use_loaded(r.x[q].w);
-        use_loaded(r.y[q].x); use_loaded(r.y[q].y); use_loaded(r.y[q].z); use_loaded(r.y[q].w);
+    for (int k = 0; k < 5; ++k) D[k] = ballot64(b[k] != b[k + 1]);
+    #pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const word_t x = b[j + 1];
+        const uint64_t V = ballot64(x != SENT) & own;
+        const uint64_t F = ballot64(x == 0u || x == ALLONES);
+        m.M[j] = V & ~F;
+        m.H[j] = m.M[j] | (V & D[j]);
+        m.E[j] = m.M[j] | (V & D[j + 1]);
     }
 }
 
-template <int S>
-__device__ __forceinline__ void lit_classify(const CanonArgs& p, uint32_t g0, LitTile<S>& T,
-                                             word_t (*s_first)[CT_NW], word_t (*s_last)[CT_NW], word_t* s_edge,
-                                             uint32_t* s_cnt, uint32_t* s_lh, const LitRaw<S>& r) {
-    const int tid = threadIdx.x;
-    const int lane = (int)lane_id();
-    const uint32_t w = (uint32_t)tid / WAVE;
-    const uint32_t groups = p.groups;
-    const bool two = p.b.w != nullptr;
-    #pragma unroll
-    for (int q = 0; q < S; ++q) {
-        const uint32_t g = g0 + (uint32_t)q * LT_SLICE + (uint32_t)tid * 4u;
-        T.v[q] = two ? op4(r.x[q], r.y[q], p.op, g, groups) : r.x[q];
+__device__ __forceinline__ uint32_t lane_bit(uint64_t mask) { return (uint32_t)(mask >> lane_id()) & 1u; }
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Every group of lanes 0..last is a mixed literal (neither fill nor SENT),
+// so each is its own word, head and end.  x is mixed iff x - 1 < ALLONES - 1.
+__device__ __forceinline__ bool lit_all_mixed(const uint4& v, uint32_t last) {
+    const bool mixed = (v.x - 1u < ALLONES - 1u) && (v.y - 1u < ALLONES - 1u) && (v.z - 1u < ALLONES - 1u) &&
+                       (v.w - 1u < ALLONES - 1u);
+    return (ballot64(!mixed) & lanes_upto(last)) == 0;
+}
+
+// Heads of a round and the last head + 1 (0 when none); uniform.
+__device__ __forceinline__ void lit_round_heads(const uint4& v, word_t prev, word_t aft, uint32_t i0,
+                                                uint32_t last, uint32_t& nh, uint32_t& lh1) {
+    if (lit_all_mixed(v, last)) {
+        nh = 4u * (last + 1u);
+        lh1 = 4u * (i0 + last) + 4u;
+        return;
     }
-    if (tid == 0) {
-        const uint32_t TILE = LT_SLICE * S;
-        word_t vp = SENT, vn = SENT;
-        if (g0 > 0) { vp = p.a.w[g0 - 1]; if (two) vp = apply_op(vp, p.b.w[g0 - 1], p.op); }
-        if (g0 + TILE < groups) { vn = p.a.w[g0 + TILE]; if (two) vn = apply_op(vn, p.b.w[g0 + TILE], p.op); }
-        s_edge[0] = vp; s_edge[1] = vn;
-    }
+    LfMasks m;
+    lf_masks(v, prev, aft, last, m);
+    nh = 0; lh1 = 0;
     #pragma unroll
-    for (int q = 0; q < S; ++q) {
-        if (lane == 0) s_first[q][w] = T.v[q].x;
-        if (lane == WAVE - 1) s_last[q][w] = T.v[q].w;
+    for (int j = 0; j < 4; ++j) {
+        nh += (uint32_t)__popcll(m.H[j]);
+        if (m.H[j]) lh1 = max(lh1, 4u * (i0 + 63u - (uint32_t)__clzll(m.H[j])) + (uint32_t)j + 1u);
     }
-    __syncthreads();
-    #pragma unroll
-    for (int q = 0; q < S; ++q) {
-        const uint32_t gl = g0 + (uint32_t)q * LT_SLICE + (uint32_t)tid * 4u;   // the lane's first group
-        word_t pv = (word_t)__shfl_up((int)T.v[q].w, 1, 64);
-        word_t nv = (word_t)__shfl_down((int)T.v[q].x, 1, 64);
-        if (lane == 0) pv = w > 0 ? s_last[q][w - 1] : (q > 0 ? s_last[q - 1][CT_NW - 1] : s_edge[0]);
-        if (lane == WAVE - 1) nv = w + 1 < CT_NW ? s_first[q][w + 1] : (q + 1 < S ? s_first[q + 1][0] : s_edge[1]);
-        uint32_t hc = 0, lh1 = 0;
+}
+
+// Writes the words that a round ends, c heads (h: last head + 1) before it;
+// `row` is the wave's LDS row; `closes` marks the round holding the
+// bitvector's last group.  Returns the round's heads and last head + 1.
+__device__ __forceinline__ void lit_round_write(const CanonArgs& p, const uint4& v, word_t prev, word_t aft,
+                                                uint32_t i0, uint32_t last, bool closes, uint32_t c, uint32_t h,
+                                                word_t* row, uint32_t& nh, uint32_t& lh1) {
+    const uint32_t lane = lane_id();
+    const uint32_t gl = 4u * (i0 + lane);
+    uint32_t ne, sfirst;
+    bool direct = false;
+    if (lit_all_mixed(v, last)) {
+        // the words are the values, from slot c
+        nh = ne = 4u * (last + 1u);
+        lh1 = 4u * (i0 + last) + 4u;
+        sfirst = c;
+        direct = (c & 3u) == 0;
+        if (lane <= last) {
+            if (direct) *reinterpret_cast<uint4*>(p.out + c + 4u * lane) = v;
+            else reinterpret_cast<uint4*>(row)[lane] = v;
+            #pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t slot = c + 4u * lane + (uint32_t)j;
+                if ((slot & (CHUNKW - 1)) == 0) p.outck[slot / CHUNKW] = gl + (uint32_t)j;
+            }
+        }
+    } else {
+        LfMasks m;
+        lf_masks(v, prev, aft, last, m);
+        nh = 0; lh1 = 0; ne = 0;
+        uint32_t at = 0;                      // the lane's first row: ends on lanes below
+        uint64_t fill_ends = 0;
         #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const uint32_t g = gl + (uint32_t)j;
-            const word_t v = comp(T.v[q], j);
-            const word_t before = j == 0 ? pv : comp(T.v[q], j - 1);
-            const word_t after = j == 3 ? nv : comp(T.v[q], j + 1);
-            bool head = false, end = false;
-            if (g < groups) {
-                const uint32_t cl = cls3(v);
-                head = (cl == 3u) || (g == 0) || (before == SENT) || (cls3(before) != cl);
-                end = (cl == 3u) || (g + 1 == groups) || (after == SENT) || (cls3(after) != cl);
-            }
-            T.heads |= (head ? 1u : 0u) << (4 * q + j);
-            T.ends |= (end ? 1u : 0u) << (4 * q + j);
-            if (head) { ++hc; lh1 = g + 1; }
+            nh += (uint32_t)__popcll(m.H[j]);
+            if (m.H[j]) lh1 = max(lh1, 4u * (i0 + 63u - (uint32_t)__clzll(m.H[j])) + (uint32_t)j + 1u);
+            at += lanes_below(m.E[j]);
+            ne += (uint32_t)__popcll(m.E[j]);
+            fill_ends |= m.E[j] & ~m.M[j];
         }
-        const uint32_t sc = wave_sum(hc);
-        const uint32_t sh = wave_max(lh1);
-        if (lane == 0) { s_cnt[q * CT_NW + w] = sc; s_lh[q * CT_NW + w] = sh; }
-    }
-    __syncthreads();
-}
-
-// The count and write kernels take tiles in grid-stride order (there is
-// no dependence between tiles here), the grid one wave of resident blocks.
-template <int S>
-__global__ __launch_bounds__(CT_TPB) void k_lit_count(CanonArgs p, uint32_t nt) {
-    __shared__ word_t s_first[S][CT_NW], s_last[S][CT_NW];
-    __shared__ uint32_t s_cnt[S * CT_NW], s_lh[S * CT_NW];
-    __shared__ word_t s_edge[2];
-    // the next tile's loads are in flight while this tile is classified
-    LitRaw<S> cur, nxt;
-    if (blockIdx.x < nt) lit_load<S>(p, blockIdx.x * LT_SLICE * S, cur);
-    for (uint32_t tile = blockIdx.x; tile < nt; tile += gridDim.x) {
-        const uint32_t g0 = tile * LT_SLICE * S;
-        const uint32_t tn = tile + gridDim.x;
-        if (tn < nt) lit_load<S>(p, tn * LT_SLICE * S, nxt);
-        lit_wait<S>(cur);
-        LitTile<S> T;
-        lit_classify<S>(p, g0, T, s_first, s_last, s_edge, s_cnt, s_lh, cur);
-        if (threadIdx.x == 0) {
-            uint32_t tc = 0, th = 0;
-            for (int i = 0; i < S * CT_NW; ++i) { tc += s_cnt[i]; th = max(th, s_lh[i]); }
-            p.status[tile] = ((uint64_t)tc << 32) | th;
-        }
-        __syncthreads();
-        cur = nxt;
-    }
-}
-
-// Exclusive (sum of counts, max of last heads) over the nt tile entries of
-// status, in place, by one block of 1024 threads: coalesced loads into
-// LDS, contiguous runs per thread, one block scan.
-constexpr int LS_TPB = 1024;
-constexpr uint32_t LS_MAX = 8192;            // tiles per pass through LDS
-
-__global__ __launch_bounds__(LS_TPB) void k_lit_scan(uint64_t* status, uint32_t nt) {
-    __shared__ uint64_t sv[LS_MAX];
-    __shared__ uint32_t sh[LS_TPB / WAVE + 1];
-    __shared__ uint32_t carry[2];
-    if (threadIdx.x == 0) { carry[0] = 0; carry[1] = 0; }
-    for (uint32_t b0 = 0; b0 < nt; b0 += LS_MAX) {
-        const uint32_t n = min(LS_MAX, nt - b0);
-        for (uint32_t i = threadIdx.x; i < n; i += LS_TPB) sv[i] = status[b0 + i];
-        __syncthreads();
-        const uint32_t per = (n + LS_TPB - 1) / LS_TPB;
-        const uint32_t i0 = threadIdx.x * per, i1 = min(i0 + per, n);
-        uint32_t c = 0, h = 0;
-        for (uint32_t i = i0; i < i1; ++i) { c += (uint32_t)(sv[i] >> 32); h = max(h, (uint32_t)sv[i]); }
-        uint32_t tot;
-        uint32_t ce = block_excl_sum<LS_TPB>(c, sh, &tot) + carry[0];
-        uint32_t he = max(block_excl_max<LS_TPB>(h, sh), carry[1]);
-        for (uint32_t i = i0; i < i1; ++i) {
-            const uint64_t v = sv[i];
-            sv[i] = ((uint64_t)ce << 32) | he;
-            ce += (uint32_t)(v >> 32);
-            he = max(he, (uint32_t)v);
-        }
-        __syncthreads();
-        if (threadIdx.x == LS_TPB - 1 || (i0 < n && i1 == n)) { carry[0] = ce; carry[1] = he; }
-        for (uint32_t i = threadIdx.x; i < n; i += LS_TPB) status[b0 + i] = sv[i];
-        __syncthreads();
-    }
-}
-
-template <int S>
-__global__ __launch_bounds__(CT_TPB) void k_lit_write(CanonArgs p, uint32_t nt) {
-    __shared__ word_t s_first[S][CT_NW], s_last[S][CT_NW];
-    __shared__ uint32_t s_cnt[S * CT_NW], s_lh[S * CT_NW];
-    __shared__ word_t s_edge[2];
-    const int tid = threadIdx.x;
-    const int lane = (int)lane_id();
-    const uint32_t w = (uint32_t)tid / WAVE;
-    const uint32_t groups = p.groups;
-    for (uint32_t tile = blockIdx.x; tile < nt; tile += gridDim.x) {
-        const uint32_t g0 = tile * LT_SLICE * S;
-        const uint64_t pre = p.status[tile];
-        LitRaw<S> raw;
-        lit_load<S>(p, g0, raw);
-        lit_wait<S>(raw);
-        LitTile<S> T;
-        lit_classify<S>(p, g0, T, s_first, s_last, s_edge, s_cnt, s_lh, raw);
-        uint32_t seg_cnt = (uint32_t)(pre >> 32), seg_lh = (uint32_t)pre;
-        #pragma unroll
-        for (int q = 0; q < S; ++q) {
-            uint32_t c = seg_cnt, h = seg_lh;
-            for (uint32_t i = 0; i < w; ++i) { c += s_cnt[q * CT_NW + i]; h = max(h, s_lh[q * CT_NW + i]); }
-            const uint32_t gl = g0 + (uint32_t)q * LT_SLICE + (uint32_t)tid * 4u;
-            const uint32_t hbits = (T.heads >> (4 * q)) & 0xFu, ebits = (T.ends >> (4 * q)) & 0xFu;
-            // heads and last head + 1 of the lanes before this one in the wave
-            const uint32_t hc = (uint32_t)__popc(hbits);
-            const uint32_t cincl = wave_incl_sum(hc);
-            const uint32_t lhl = hbits ? gl + (31u - __clz(hbits)) + 1u : 0u;
-            uint32_t lex = (uint32_t)__shfl_up((int)wave_incl_max(lhl), 1, 64);
-            if (lane == 0) lex = 0;
-            c += cincl - hc;
-            h = max(h, lex);
-            if (ebits) {
-                #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    if (!((ebits >> j) & 1u)) continue;
-                    const uint32_t g = gl + (uint32_t)j;
-                    const uint32_t mine = hbits & ((2u << j) - 1u);
-                    const uint32_t heads_le = c + (uint32_t)__popc(mine);
-                    const uint32_t lh1 = mine ? gl + (31u - __clz(mine)) + 1u : h;
-                    const word_t x = comp(T.v[q], j);
-                    word_t word = x;
-                    uint32_t headpos = g;
-                    if (cls3(x) != 3u) {
-                        headpos = lh1 - 1;
-                        const uint32_t len = g - headpos + 1;
-                        if (len >= 2) word = (x == ALLONES ? HEADER1 : HEADER0) | len;
-                    }
-                    const uint32_t slot = heads_le - 1;
-                    p.out[slot] = word;
-                    if ((slot & (CHUNKW - 1)) == 0) p.outck[slot / CHUNKW] = headpos;
-                    if (g + 1 == groups) {
-                        p.ctl[1] = heads_le;
-                        p.ctl[3] = groups;
-                        p.outck[(heads_le + CHUNKW - 1) / CHUNKW] = groups;
-                    }
+        // the round's first word is the one open at its first group, or the
+        // one that group heads
+        sfirst = c - 1u + (uint32_t)(m.H[0] & 1ull);
+        if (!fill_ends) {
+            // every word the round ends is a mixed literal: the value itself
+            #pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (lane_bit(m.E[j])) {
+                    row[at] = comp(v, j);
+                    const uint32_t slot = sfirst + at;
+                    if ((slot & (CHUNKW - 1)) == 0) p.outck[slot / CHUNKW] = gl + (uint32_t)j;
+                    ++at;
                 }
             }
-            for (uint32_t i = 0; i < CT_NW; ++i) { seg_cnt += s_cnt[q * CT_NW + i]; seg_lh = max(seg_lh, s_lh[q * CT_NW + i]); }
+        } else {
+            // a fill's word needs its run's first group: the last head at or
+            // before it, in this lane or (a max-scan) before it
+            uint32_t hb = 0;
+            #pragma unroll
+            for (int j = 0; j < 4; ++j) hb |= lane_bit(m.H[j]) << j;
+            const uint32_t lhl = hb ? gl + (32u - __clz(hb)) : 0u;
+            uint32_t lex = (uint32_t)__shfl_up((int)wave_incl_max(lhl), 1, 64);
+            if (lane == 0) lex = 0;
+            const uint32_t hl = max(h, lex);
+            #pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (lane_bit(m.E[j])) {
+                    const uint32_t g = gl + (uint32_t)j;
+                    const word_t x = comp(v, j);
+                    word_t word = x;
+                    uint32_t headpos = g;
+                    if (!lane_bit(m.M[j])) {
+                        const uint32_t mine = hb & ((2u << j) - 1u);
+                        headpos = mine ? gl + (31u - __clz(mine)) : hl - 1u;
+                        const uint32_t len = g - headpos + 1u;
+                        if (len >= 2u) word = (x == ALLONES ? HEADER1 : HEADER0) | len;
+                    }
+                    if (at < LW_ROW) row[at] = word;
+                    const uint32_t slot = sfirst + at;
+                    if ((slot & (CHUNKW - 1)) == 0) p.outck[slot / CHUNKW] = headpos;
+                    ++at;
+                }
+            }
         }
-        __syncthreads();        // the LDS tables are reused by the next tile
+    }
+    if (closes && lane == 0) {
+        // the bitvector's last group ends the last word
+        p.ctl[1] = sfirst + ne;
+        p.ctl[3] = p.groups;
+        p.outck[(sfirst + ne + CHUNKW - 1) / CHUNKW] = p.groups;
+    }
+    if (!direct) {
+        __builtin_amdgcn_wave_barrier();
+        #pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t idx = (uint32_t)q * WAVE + lane;
+            if (idx < ne && sfirst + idx < p.groups) p.out[sfirst + idx] = row[idx];
+        }
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
-template <int S>
-static void launch_canon_lit(const CanonArgs& p, uint32_t groups, hipStream_t s) {
+// The group after round R (= rb + r) of the span: the next round's first
+// group (in registers within a batch), else a uniform load, else the
+// group after the span.
+template <int OP>
+__device__ __forceinline__ word_t lit_after(const CanonArgs& p, const LitSpan& s, const LitEdges& e,
+                                            const uint4 (&v)[LF_R], int r, uint32_t R) {
+    if (R + 1u >= s.nr) return e.after;
+    const int rn = r + 1 < LF_R ? r + 1 : r;
+    if (r + 1 < LF_R) return (word_t)__builtin_amdgcn_readlane((int)v[rn].x, 0);
+    return lit_group<OP>(p, 4u * (s.c0 + (R + 1u) * WAVE));
+}
+
+// status[k] = (heads, last head + 1) of wave k's span; status[nwaves + b]
+// the same over block b.
+template <int OP>
+__global__ __launch_bounds__(LW_TPB) void k_lit_count(CanonArgs p, uint32_t n4, uint32_t nwaves) {
+    __shared__ uint32_t s_c[LW_NW], s_h[LW_NW];
+    const uint32_t tid = threadIdx.x;
+    // the wave index is uniform: scalar span arithmetic and lane masks
+    const uint32_t wv = (uint32_t)__builtin_amdgcn_readfirstlane((int)(tid / WAVE));
+    const uint32_t k = blockIdx.x * LW_NW + wv;
+    const LitSpan s = lit_span(n4, k, nwaves);
+    const LitEdges e = lit_edges<OP>(p, s);
+    uint32_t hc = 0, lh = 0;
+    word_t prev = e.before;
+    for (uint32_t rb = 0; rb < s.nr; rb += LF_R) {
+        uint4 v[LF_R];
+        lit_load<OP>(p, s, e, rb, v);
+        #pragma unroll
+        for (int r = 0; r < LF_R; ++r) {
+            const uint32_t R = rb + (uint32_t)r;
+            if (R >= s.nr) break;
+            const uint32_t i0 = s.c0 + R * WAVE;
+            const uint32_t last = min((uint32_t)WAVE - 1u, s.c1 - 1u - i0);
+            uint32_t nh, lh1;
+            lit_round_heads(v[r], prev, lit_after<OP>(p, s, e, v, r, R), i0, last, nh, lh1);
+            hc += nh;
+            lh = max(lh, lh1);
+            prev = (word_t)__builtin_amdgcn_readlane((int)v[r].w, WAVE - 1);
+        }
+    }
+    if (lane_id() == 0) {
+        p.status[k] = ((uint64_t)hc << 32) | lh;
+        s_c[wv] = hc; s_h[wv] = lh;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t c = 0, h = 0;
+        for (int w = 0; w < LW_NW; ++w) { c += s_c[w]; h = max(h, s_h[w]); }
+        p.status[nwaves + blockIdx.x] = ((uint64_t)c << 32) | h;
+    }
+}
+
+template <int OP>
+__global__ __launch_bounds__(LW_TPB) void k_lit_write(CanonArgs p, uint32_t n4, uint32_t nwaves) {
+    __shared__ __align__(16) word_t s_row[LW_NW][LW_ROW];
+    __shared__ uint32_t s_pc[LW_NW], s_ph[LW_NW], s_oc[LW_NW], s_oh[LW_NW];
+    const uint32_t tid = threadIdx.x, lane = lane_id();
+    const uint32_t wv = (uint32_t)__builtin_amdgcn_readfirstlane((int)(tid / WAVE));
+    const uint32_t k = blockIdx.x * LW_NW + wv;
+    const LitSpan s = lit_span(n4, k, nwaves);
+    const LitEdges e = lit_edges<OP>(p, s);
+    // totals of the blocks before this one (grid <= LW_MAXB) and of this
+    // block's waves, summed before the operands are loaded: they are read
+    // before the operand loads queue up, and their registers are free again
+    uint32_t pc = 0, ph = 0;
+    {
+        uint64_t t[LW_PRE];
+        const uint32_t jmax = max(blockIdx.x, 1u) - 1u;
+        #pragma unroll
+        for (int u = 0; u < LW_PRE; ++u) {
+            const uint32_t j = tid + (uint32_t)u * LW_TPB;
+            t[u] = p.status[nwaves + min(j, jmax)];
+            if (j >= blockIdx.x) t[u] = 0;
+        }
+        #pragma unroll
+        for (int u = 0; u < LW_PRE; ++u) { pc += (uint32_t)(t[u] >> 32); ph = max(ph, (uint32_t)t[u]); }
+    }
+    const uint64_t own = p.status[blockIdx.x * LW_NW + min(tid, (uint32_t)LW_NW - 1u)];
+    pc = wave_sum(pc);
+    ph = wave_max(ph);
+    if (lane == 0) { s_pc[wv] = pc; s_ph[wv] = ph; }
+    if (tid < LW_NW) { s_oc[tid] = (uint32_t)(own >> 32); s_oh[tid] = (uint32_t)own; }
+    __syncthreads();
+    uint32_t c = 0, h = 0;                    // heads before the round, last head + 1
+    for (int w = 0; w < LW_NW; ++w) { c += s_pc[w]; h = max(h, s_ph[w]); }
+    for (uint32_t w = 0; w < wv; ++w) { c += s_oc[w]; h = max(h, s_oh[w]); }
+    word_t prev = e.before;
+    for (uint32_t rb = 0; rb < s.nr; rb += LF_R) {
+        uint4 v[LF_R];
+        lit_load<OP>(p, s, e, rb, v);
+        #pragma unroll
+        for (int r = 0; r < LF_R; ++r) {
+            const uint32_t R = rb + (uint32_t)r;
+            if (R >= s.nr) break;
+            const uint32_t i0 = s.c0 + R * WAVE;
+            const uint32_t last = min((uint32_t)WAVE - 1u, s.c1 - 1u - i0);
+            const bool closes = s.c1 == n4 && R + 1u == s.nr;
+            uint32_t nh, lh1;
+            lit_round_write(p, v[r], prev, lit_after<OP>(p, s, e, v, r, R), i0, last, closes, c, h, s_row[wv],
+                            nh, lh1);
+            c += nh;
+            h = max(h, lh1);
+            prev = (word_t)__builtin_amdgcn_readlane((int)v[r].w, WAVE - 1);
+        }
+    }
+}
+
+// Blocks of the literal path: resident at once, at most LW_MAXB, and no
+// more than one round per wave needs.
+static uint32_t lit_grid(uint32_t groups) {
     static const uint32_t slots = [] {
-        int dev = 0, cus = 0;
+        int dev = 0, cus = 0, b1 = 0, b2 = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
             cus = 256;
-        return 8u * (uint32_t)cus;          // 8 blocks of 4 waves per CU
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, k_lit_count<AND>, LW_TPB, 0) != hipSuccess || b1 <= 0)
+            b1 = 1;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b2, k_lit_write<AND>, LW_TPB, 0) != hipSuccess || b2 <= 0)
+            b2 = 1;
+        return min((uint32_t)cus * (uint32_t)min(b1, b2), LW_MAXB);
     }();
-    const uint32_t nt = (groups + LT_SLICE * S - 1) / (LT_SLICE * S);
-    const uint32_t grid = min(nt, slots);
-    hipLaunchKernelGGL((k_lit_count<S>), dim3(grid), dim3(CT_TPB), 0, s, p, nt);
-    hipLaunchKernelGGL(k_lit_scan, dim3(1), dim3(LS_TPB), 0, s, p.status, nt);
-    hipLaunchKernelGGL((k_lit_write<S>), dim3(grid), dim3(CT_TPB), 0, s, p, nt);
+    const uint32_t rounds = ((groups + 3u) / 4u + WAVE - 1) / WAVE;
+    return max(1u, min(slots, (rounds + LW_NW - 1) / LW_NW));
+}
+
+template <int OP>
+static void launch_lit_op(const CanonArgs& p, uint32_t grid, hipStream_t s) {
+    const uint32_t n4 = (p.groups + 3u) / 4u, nwaves = grid * LW_NW;
+    hipLaunchKernelGGL(k_lit_count<OP>, dim3(grid), dim3(LW_TPB), 0, s, p, n4, nwaves);
+    hipLaunchKernelGGL(k_lit_write<OP>, dim3(grid), dim3(LW_TPB), 0, s, p, n4, nwaves);
+}
+
+// groups >= 4 (a whole chunk to clamp loads to)
+static void launch_canon_lit(const CanonArgs& p, hipStream_t s) {
+    const uint32_t grid = lit_grid(p.groups);
+    switch (p.b.w ? p.op : -1) {
+    case AND: launch_lit_op<AND>(p, grid, s); break;
+    case OR: launch_lit_op<OR>(p, grid, s); break;
+    case XOR: launch_lit_op<XOR>(p, grid, s); break;
+    case MINUS: launch_lit_op<MINUS>(p, grid, s); break;
+    default: launch_lit_op<-1>(p, grid, s); break;
+    }
+}
+
+// One launch for literal operands whose spans fit one load batch (LF_R
+// rounds per wave): each block publishes a tagged total and sums the
+// totals before it (blocks before it in its segment of 64 blocks, and the
+// totals of earlier segments, which the last block of each segment
+// publishes), polling with one wave and a sleep between passes; a block
+// waits only for blocks of lower index, which the dispatcher has started
+// before it.  It then writes its words from registers, so the operands
+// are read once.  A total is one 64-bit word, [63:56] launch tag, [55:28]
+// heads, [27:0] last head + 1; the tags cycle through 1..255 and the
+// tables are cleared when they wrap.
+constexpr uint32_t LF_SEG = 64;
+__device__ uint64_t lf_status[LW_MAXB];
+__device__ uint64_t lf_segtot[LW_MAXB / LF_SEG];
+
+__global__ void k_lf_clear() {
+    for (uint32_t j = threadIdx.x; j < LW_MAXB; j += blockDim.x) lf_status[j] = 0;
+    for (uint32_t j = threadIdx.x; j < LW_MAXB / LF_SEG; j += blockDim.x) lf_segtot[j] = 0;
+}
+
+__device__ __forceinline__ uint64_t lf_pack(uint32_t tag, uint32_t cnt, uint32_t lh1) {
+    return ((uint64_t)tag << 56) | ((uint64_t)(cnt & 0xFFFFFFFu) << 28) | (uint64_t)(lh1 & 0xFFFFFFFu);
+}
+
+template <int OP>
+__global__ __launch_bounds__(LW_TPB) void k_lit_fused(CanonArgs p, uint32_t n4, uint32_t nwaves, uint32_t tag) {
+    __shared__ __align__(16) word_t s_row[LW_NW][LW_ROW];
+    __shared__ uint32_t s_c[LW_NW], s_h[LW_NW], s_pc[1], s_ph[1];
+    const uint32_t tid = threadIdx.x, lane = lane_id();
+    const uint32_t wv = (uint32_t)__builtin_amdgcn_readfirstlane((int)(tid / WAVE));
+    const uint32_t k = blockIdx.x * LW_NW + wv;
+    const LitSpan s = lit_span(n4, k, nwaves);
+    const LitEdges e = lit_edges<OP>(p, s);
+    uint4 v[LF_R];
+    lit_load<OP>(p, s, e, 0, v);
+    uint32_t hc = 0, lh = 0;
+    {
+        word_t prev = e.before;
+        #pragma unroll
+        for (int r = 0; r < LF_R; ++r) {
+            const uint32_t R = (uint32_t)r;
+            if (R >= s.nr) break;
+            const uint32_t i0 = s.c0 + R * WAVE;
+            const uint32_t last = min((uint32_t)WAVE - 1u, s.c1 - 1u - i0);
+            uint32_t nh, lh1;
+            lit_round_heads(v[r], prev, lit_after<OP>(p, s, e, v, r, R), i0, last, nh, lh1);
+            hc += nh;
+            lh = max(lh, lh1);
+            prev = (word_t)__builtin_amdgcn_readlane((int)v[r].w, WAVE - 1);
+        }
+    }
+    if (lane == 0) { s_c[wv] = hc; s_h[wv] = lh; }
+    __syncthreads();
+    uint32_t bc = 0, bh = 0;
+    for (int w = 0; w < LW_NW; ++w) { bc += s_c[w]; bh = max(bh, s_h[w]); }
+    if (tid == 0)
+        __hip_atomic_store(&lf_status[blockIdx.x], lf_pack(tag, bc, bh), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (wv == 0) {
+        // lane l waits for block 64 sg + l (l < q) and segment l (l < sg)
+        const uint32_t sg = blockIdx.x / LF_SEG, q = blockIdx.x % LF_SEG;
+        bool need_b = lane < q, need_s = lane < sg;
+        bool published = q != LF_SEG - 1u;
+        uint32_t xc = 0, xh = 0, yc = 0, yh = 0;
+        for (;;) {
+            if (need_b) {
+                const uint64_t x = __hip_atomic_load(&lf_status[sg * LF_SEG + lane], __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+                if ((uint32_t)(x >> 56) == tag) {
+                    xc = (uint32_t)(x >> 28) & 0xFFFFFFFu; xh = (uint32_t)x & 0xFFFFFFFu; need_b = false;
+                }
+            }
+            if (need_s) {
+                const uint64_t x = __hip_atomic_load(&lf_segtot[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if ((uint32_t)(x >> 56) == tag) {
+                    yc = (uint32_t)(x >> 28) & 0xFFFFFFFu; yh = (uint32_t)x & 0xFFFFFFFu; need_s = false;
+                }
+            }
+            const bool pend_b = ballot64(need_b) != 0, pend_s = ballot64(need_s) != 0;
+            if (!published && !pend_b) {
+                const uint32_t tc = wave_sum(xc) + bc, th = max(wave_max(xh), bh);
+                if (lane == 0)
+                    __hip_atomic_store(&lf_segtot[sg], lf_pack(tag, tc, th), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                published = true;
+            }
+            if (!pend_b && !pend_s) break;
+            __builtin_amdgcn_s_sleep(10);
+        }
+        const uint32_t pc = wave_sum(xc) + wave_sum(yc), ph = max(wave_max(xh), wave_max(yh));
+        if (lane == 0) { s_pc[0] = pc; s_ph[0] = ph; }
+    }
+    __syncthreads();
+    uint32_t c = s_pc[0], h = s_ph[0];
+    for (uint32_t w = 0; w < wv; ++w) { c += s_c[w]; h = max(h, s_h[w]); }
+    word_t prev = e.before;
+    #pragma unroll
+    for (int r = 0; r < LF_R; ++r) {
+        const uint32_t R = (uint32_t)r;
+        if (R >= s.nr) break;
+        const uint32_t i0 = s.c0 + R * WAVE;
+        const uint32_t last = min((uint32_t)WAVE - 1u, s.c1 - 1u - i0);
+        const bool closes = s.c1 == n4 && R + 1u == s.nr;
+        uint32_t nh, lh1;
+        lit_round_write(p, v[r], prev, lit_after<OP>(p, s, e, v, r, R), i0, last, closes, c, h, s_row[wv], nh, lh1);
+        c += nh;
+        h = max(h, lh1);
+        prev = (word_t)__builtin_amdgcn_readlane((int)v[r].w, WAVE - 1);
+    }
+}
+
+// Grid of the fused launch, 0 when the operands do not fit one batch.
+static uint32_t lf_grid(uint32_t groups) {
+    const uint32_t grid = lit_grid(groups);
+    const uint32_t n4 = (groups + 3u) / 4u;
+    return (n4 + grid * LW_NW - 1) / (grid * LW_NW) <= (uint32_t)(LF_R * WAVE) ? grid : 0u;
+}
+
+static void launch_lit_fused(const CanonArgs& p, uint32_t grid, hipStream_t s) {
+    static std::mutex mu;
+    static uint32_t tags[64];
+    std::lock_guard<std::mutex> lk(mu);
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    const uint32_t tag = tags[dev] % 255u + 1u;
+    if (tag == 1u && tags[dev] != 0u) hipLaunchKernelGGL(k_lf_clear, dim3(1), dim3(1024), 0, s);
+    tags[dev] = tag;
+    const uint32_t n4 = (p.groups + 3u) / 4u, nwaves = grid * LW_NW;
+    switch (p.b.w ? p.op : -1) {
+    case AND: hipLaunchKernelGGL(k_lit_fused<AND>, dim3(grid), dim3(LW_TPB), 0, s, p, n4, nwaves, tag); break;
+    case OR: hipLaunchKernelGGL(k_lit_fused<OR>, dim3(grid), dim3(LW_TPB), 0, s, p, n4, nwaves, tag); break;
+    case XOR: hipLaunchKernelGGL(k_lit_fused<XOR>, dim3(grid), dim3(LW_TPB), 0, s, p, n4, nwaves, tag); break;
+    case MINUS: hipLaunchKernelGGL(k_lit_fused<MINUS>, dim3(grid), dim3(LW_TPB), 0, s, p, n4, nwaves, tag); break;
+    default: hipLaunchKernelGGL(k_lit_fused<-1>, dim3(grid), dim3(LW_TPB), 0, s, p, n4, nwaves, tag); break;
+    }
 }
 
 hipError_t launch_canon(const word_t* a, const uint32_t* cka, uint32_t na, int alit,
@@ -605,8 +892,10 @@ hipError_t launch_canon(const word_t* a, const uint32_t* cka, uint32_t na, int a
     p.b = Operand{b, ckb, nb, (uint32_t)(blit ? 1 : 0), ts + (nt + 1)};
     p.groups = groups; p.ntiles = groups ? nt : 0u;
     p.op = op; p.out = out; p.outck = outck; p.status = status; p.ctl = ctl;
-    if (alit && (!b || blit) && groups) {
-        launch_canon_lit<2>(p, groups, s);      // 2048-group tiles
+    if (alit && (!b || blit) && groups >= 4u) {
+        const uint32_t fg = lf_grid(groups);
+        if (fg) launch_lit_fused(p, fg, s);
+        else launch_canon_lit(p, s);
         return hipGetLastError();
     }
     if (!alit && na > 0) {
@@ -623,5 +912,9 @@ hipError_t launch_canon(const word_t* a, const uint32_t* cka, uint32_t na, int a
 }
 
 uint32_t canon_tiles(uint32_t groups) { return groups ? (groups + CT - 1) / CT : 1u; }
+
+uint32_t canon_status(uint32_t groups) {
+    return max(canon_tiles(groups), lit_grid(groups) * (LW_NW + 1));
+}
 
 } // namespace fbk

@@ -35,9 +35,10 @@ hipError_t launch_rids_lit(const word_t* lit, uint32_t n, uint32_t* out, uint32_
                            uint32_t* tmp, hipStream_t s);
 // Canonical WAH of a OP b (b == nullptr: of a alone) group by group; an
 // operand with lit != 0 holds one literal word per group.  status needs
-// canon_tiles(groups) entries; ctl 8 words; ts 2 * (canon_tiles(groups) + 1)
+// canon_status(groups) entries; ctl 8 words; ts 2 * (canon_tiles(groups) + 1)
 // entries (tile starts of the compressed operands).
 uint32_t canon_tiles(uint32_t groups);
+uint32_t canon_status(uint32_t groups);
 hipError_t launch_canon(const word_t* a, const uint32_t* cka, uint32_t na, int alit,
                         const word_t* b, const uint32_t* ckb, uint32_t nb, int blit,
                         uint32_t groups, int op, word_t* out, uint32_t* outck, uint64_t* status,

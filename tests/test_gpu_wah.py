@@ -176,3 +176,60 @@ def test_canonical_result_spanning_tiles(L, orc):
             got, gpath = gpu_op(L, x, y, op)
             assert gpath == wpath == "c0"
             assert np.array_equal(got, want), op
+
+
+def test_c0_runs_across_wave_spans(L, orc):
+    """Literal operands of ~2.1M groups whose op results hold fill runs of up
+    to 3000 groups: runs cross the rounds and the per-wave spans the literal
+    kernels split the groups into; group counts around multiples of 4 and
+    64 exercise partial chunks and rounds."""
+    rng = np.random.default_rng(41)
+    for groups in ((1 << 21) + 1, (1 << 21) + 63, 1 << 21):
+        lens = rng.integers(1, 3000, size=groups // 500 + 8)
+        cls = np.repeat(rng.integers(0, 3, size=lens.size), lens)[:groups]
+        bits = np.where(np.repeat(cls, 31) == 2, rng.integers(0, 2, size=groups * 31),
+                        np.repeat(cls, 31)).astype(np.uint8)
+        a = orc.decompress(orc.from_bits(bits))
+        b = orc.decompress(orc.from_bits(np.where(rng.random(bits.size) < 0.002, 0, 1).astype(np.uint8)))
+        for op in range(4):
+            want, wpath = orc.op(a, b, op)
+            got, gpath = gpu_op(L, a, b, op)
+            assert gpath == wpath == "c0", (groups, op)
+            assert np.array_equal(got, want), (groups, op, len(got), len(want))
+
+
+def test_c0_repeated_ops_tag_cycle(L, orc):
+    """300 literal ops in a row: the one-launch literal kernel tags its block
+    totals per launch (1..255) and clears the table when the tags wrap, so
+    every call sees only its own totals."""
+    rng = np.random.default_rng(43)
+    n = 31 * 50000 + 11
+    a = orc.decompress(orc.from_bits(markov_bits(n, 0.5, 1.5, rng)))
+    b = orc.decompress(orc.from_bits(markov_bits(n, 0.3, 40, rng)))
+    want = {op: orc.op(a, b, op)[0] for op in range(4)}
+    x = L.Bitvector.from_serial(a)
+    y = L.Bitvector.from_serial(b)
+    for i in range(300):
+        z = x.copy()
+        assert z.op(y, i % 4) == "c0"
+        if i % 37 == 0 or i >= 250:
+            assert np.array_equal(z.write(), want[i % 4]), i
+
+
+def test_c0_beyond_one_batch(L, orc):
+    """11M groups: more than the one-launch kernel holds in registers, so
+    the two-launch literal kernels (count, then write) take it."""
+    rng = np.random.default_rng(44)
+    groups = 11_000_000
+    wa = rng.integers(0, 1 << 31, size=groups, dtype=np.uint32) & np.uint32(0x7FFFFFFF)
+    wb = rng.integers(0, 1 << 31, size=groups, dtype=np.uint32) & np.uint32(0x7FFFFFFF)
+    wa[1000:90000] = 0                       # runs for the fill words
+    wb[5_000_000:5_400_000] = 0x7FFFFFFF
+    wa[5_200_000:5_300_000] = 0x7FFFFFFF
+    a = np.concatenate([wa, np.array([0], np.uint32)])        # words, then 0 active bits
+    b = np.concatenate([wb, np.array([0], np.uint32)])
+    for op in (0, 1, 3):
+        want, wpath = orc.op(a, b, op)
+        got, gpath = gpu_op(L, a, b, op)
+        assert gpath == wpath == "c0"
+        assert np.array_equal(got, want), op
