@@ -52,36 +52,57 @@ struct CanonArgs {
     uint32_t* ctl;          // [1] words [3] groups [4],[5] edge words
 };
 
-// Clears the look-back status words and the control words of a launch.
-__global__ void k_zero_canon(uint64_t* status, uint32_t n, uint32_t* ctl) {
-    const uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i0 < 8u) ctl[i0] = 0u;
-    for (uint32_t i = i0; i < n; i += gridDim.x * blockDim.x) status[i] = 0ull;
-}
-
 // Tile starts of a compressed operand from its chunk index: the thread of
 // chunk j (code words [32j, 32j+32), groups [ck[j], ck[j+1])) resolves the
 // tiles whose first group falls in its span by walking those 32 words.
 // ts[nt] = {n, groups} closes the table.
-__global__ __launch_bounds__(256) void k_tile_starts(const word_t* __restrict__ w, const uint32_t* __restrict__ ck,
-                                                     uint32_t n, uint32_t nt, uint2* __restrict__ ts) {
+__device__ __forceinline__ void tile_starts(const word_t* __restrict__ w, const uint32_t* __restrict__ ck,
+                                            uint32_t n, uint32_t nt, uint2* __restrict__ ts, uint32_t j) {
     const uint32_t nc = (n + CHUNKW - 1) / CHUNKW;
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j > nc) return;
     if (j == nc) { ts[nt] = make_uint2(n, ck[nc]); return; }
     const uint32_t c0 = ck[j], c1 = ck[j + 1];
     uint32_t t = (c0 + CT - 1) / CT;                      // first tile start >= c0
     if ((uint64_t)t * CT >= c1 || t >= nt) return;
-    uint32_t i = j * CHUNKW, pos = c0;
-    const uint32_t e = min(i + CHUNKW, n);
-    for (; i < e && t < nt; ++i) {
-        const uint32_t l = word_len(w[i]);
+    const uint32_t i0 = j * CHUNKW, e = min(i0 + CHUNKW, n);
+    word_t x[CHUNKW];
+    if (e == i0 + CHUNKW && (reinterpret_cast<uintptr_t>(w) & 15u) == 0) {
+        const uint4* w4 = reinterpret_cast<const uint4*>(w + i0);
+        #pragma unroll
+        for (int q = 0; q < (int)CHUNKW / 4; ++q) {
+            const uint4 u = w4[q];
+            x[4 * q] = u.x; x[4 * q + 1] = u.y; x[4 * q + 2] = u.z; x[4 * q + 3] = u.w;
+        }
+    } else {
+        #pragma unroll
+        for (int q = 0; q < (int)CHUNKW; ++q) x[q] = i0 + (uint32_t)q < e ? w[i0 + q] : 0u;
+    }
+    uint32_t pos = c0;
+    #pragma unroll
+    for (int q = 0; q < (int)CHUNKW; ++q) {
+        if (i0 + (uint32_t)q >= e || t >= nt) break;
+        const uint32_t l = word_len(x[q]);
         while (t < nt && (uint64_t)t * CT < (uint64_t)pos + l) {
-            ts[t] = make_uint2(i, pos);
+            ts[t] = make_uint2(i0 + (uint32_t)q, pos);
             ++t;
         }
         pos += l;
     }
+}
+
+// One launch before the tile pass: blockIdx.y = 0 / 1 builds the tile
+// starts of compressed operand a / b, 2 clears the look-back status words
+// and the control words.
+__global__ __launch_bounds__(256) void k_canon_prep(CanonArgs p, uint32_t nt) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.y == 2) {
+        if (j < 8u) p.ctl[j] = 0u;
+        for (uint32_t i = j; i < nt; i += gridDim.x * blockDim.x) p.status[i] = 0ull;
+        return;
+    }
+    const Operand& o = blockIdx.y == 0 ? p.a : p.b;
+    if (!o.w || o.literal || o.n == 0) return;
+    tile_starts(o.w, o.ck, o.n, nt, const_cast<uint2*>(o.ts), j);
 }
 
 __device__ __forceinline__ uint32_t cls3(word_t v) { return v == 0 ? 1u : (v == ALLONES ? 2u : 3u); }
@@ -112,66 +133,98 @@ __device__ __forceinline__ uint32_t block_excl_max(uint32_t v, uint32_t* sh) {
     return max(before, ex);
 }
 
-// Expands operand `o` over the tile's groups [g0, g0+cnt) into vals,
-// combining with the values there by `op` when `combine` is set, and
-// returns (to thread 0) the operand's values of groups g0-1 and g0+cnt
-// (SENT where outside the bitvector).  Ends with a barrier.
-__device__ void expand(const Operand& o, uint32_t tile, uint32_t g0, uint32_t cnt, uint32_t groups,
-                       word_t* vals, word_t* sw, uint32_t* sp, uint32_t* idx, uint32_t* shs, bool combine,
-                       int op, word_t& vprev, word_t& vnext) {
+// An operand's words for one tile, loaded before any is used so that both
+// operands' loads are in flight together.  Literal operand: x[q] holds the
+// tile's values (q < CT_ITEMS); compressed: x[r] the covering code words
+// i0 + r*CT_TPB + t (r < CT_IT), p0 the first group of word i0.  Thread 0
+// also loads the group values before and after the tile (e0 for a
+// compressed operand only when word i0 starts at the tile).
+constexpr int CT_IT = (CT + 1 + CT_TPB - 1) / CT_TPB;
+
+struct Staged {
+    uint32_t i0 = 0, p0 = 0, nw = 0, inx = 0;
+    word_t x[CT_IT];
+    word_t e0 = SENT, e1 = SENT;
+};
+
+__device__ __forceinline__ void stage_starts(const Operand& o, uint32_t tile, Staged& st) {
+    if (o.literal) return;
+    const uint2 a = o.ts[tile];
+    st.i0 = a.x; st.p0 = a.y;
+    st.inx = o.ts[tile + 1].x;
+}
+
+__device__ __forceinline__ void stage_words(const Operand& o, uint32_t g0, uint32_t cnt, uint32_t groups,
+                                            Staged& st) {
     const int tid = threadIdx.x;
     const uint32_t g1 = g0 + cnt;
     if (o.literal) {
-        word_t x[CT_ITEMS];
         #pragma unroll
         for (int q = 0; q < CT_ITEMS; ++q) {
             const uint32_t k = (uint32_t)q * CT_TPB + tid;
-            x[q] = k < cnt ? o.w[g0 + k] : 0u;
-        }
-        #pragma unroll
-        for (int q = 0; q < CT_ITEMS; ++q) {
-            const uint32_t k = (uint32_t)q * CT_TPB + tid;
-            if (k < cnt) vals[k] = combine ? apply_op(vals[k], x[q], op) : x[q];
+            st.x[q] = k < cnt ? o.w[g0 + k] : 0u;
         }
         if (tid == 0) {
-            vprev = g0 > 0 ? o.w[g0 - 1] : SENT;
-            vnext = g1 < groups ? o.w[g1] : SENT;
+            st.e0 = g0 > 0 ? o.w[g0 - 1] : SENT;
+            st.e1 = g1 < groups ? o.w[g1] : SENT;
         }
+        return;
+    }
+    // words covering the tile: i0 .. the word holding group g1 - 1
+    st.nw = min(st.inx, o.n - 1u) + 1u - st.i0;          // <= CT + 1
+    #pragma unroll
+    for (int r = 0; r < CT_IT; ++r) {
+        const uint32_t k = (uint32_t)r * CT_TPB + tid;
+        st.x[r] = k < st.nw ? o.w[st.i0 + k] : 0u;
+    }
+    if (tid == 0) {
+        st.e0 = (g0 > 0 && st.p0 == g0) ? group_value(o.w[st.i0 - 1u]) : SENT;
+        st.e1 = g1 < groups ? group_value(o.w[st.inx]) : SENT;
+    }
+}
+
+// Expands a staged operand over the tile's groups [g0, g0+cnt) into vals,
+// combining with the values there by `op` when `combine` is set, and
+// returns (to thread 0) the operand's values of groups g0-1 and g0+cnt
+// (SENT where outside the bitvector).  Ends with a barrier.
+__device__ void expand(const Operand& o, const Staged& st, uint32_t g0, uint32_t cnt, word_t* vals, word_t* sw,
+                       uint32_t* sp, uint32_t* idx, uint32_t* shs, bool combine, int op, word_t& vprev,
+                       word_t& vnext) {
+    const int tid = threadIdx.x;
+    if (o.literal) {
+        #pragma unroll
+        for (int q = 0; q < CT_ITEMS; ++q) {
+            const uint32_t k = (uint32_t)q * CT_TPB + tid;
+            if (k < cnt) vals[k] = combine ? apply_op(vals[k], st.x[q], op) : st.x[q];
+        }
+        if (tid == 0) { vprev = st.e0; vnext = st.e1; }
         __syncthreads();
         return;
     }
-    const uint2 st = o.ts[tile];
-    const uint32_t i0 = st.x, p0 = st.y;
-    // words covering the tile: i0 .. the word holding group g1 - 1
-    const uint32_t ilast = min(o.ts[tile + 1].x, o.n - 1u);
-    const uint32_t nw = ilast + 1u - i0;                   // <= CT + 1
-    constexpr int IT = (CT + 1 + CT_TPB - 1) / CT_TPB;
+    const uint32_t nw = st.nw, p0 = st.p0;
     #pragma unroll
-    for (int r = 0; r < IT; ++r) {
+    for (int r = 0; r < CT_IT; ++r) {
         const uint32_t k = (uint32_t)r * CT_TPB + tid;
-        if (k < nw) sw[k] = o.w[i0 + k];
+        if (k < nw) sw[k] = st.x[r];
     }
     for (uint32_t k = tid; k < cnt; k += CT_TPB) idx[k] = 0u;
-    if (tid == 0) {
-        vprev = g0 == 0 ? SENT : (p0 < g0 ? 0u : group_value(o.w[i0 - 1]));
-        vnext = g1 < groups ? group_value(o.w[o.ts[tile + 1].x]) : SENT;
-    }
+    if (tid == 0) { vprev = st.e0; vnext = st.e1; }
     __syncthreads();
     if (tid == 0 && g0 > 0 && p0 < g0) vprev = group_value(sw[0]);
-    // start groups of the staged words (thread-contiguous runs of IT)
-    uint32_t len[IT];
+    // start groups of the staged words (thread-contiguous runs of CT_IT)
+    uint32_t len[CT_IT];
     uint32_t s = 0;
     #pragma unroll
-    for (int k = 0; k < IT; ++k) {
-        const uint32_t j = tid * IT + k;
+    for (int k = 0; k < CT_IT; ++k) {
+        const uint32_t j = tid * CT_IT + k;
         len[k] = j < nw ? word_len(sw[j]) : 0u;
         s += len[k];
     }
     uint32_t tot;
     uint32_t ex = block_excl_sum<CT_TPB>(s, shs, &tot) + p0;
     #pragma unroll
-    for (int k = 0; k < IT; ++k) {
-        const uint32_t j = tid * IT + k;
+    for (int k = 0; k < CT_IT; ++k) {
+        const uint32_t j = tid * CT_IT + k;
         if (j <= nw) sp[j] = ex;
         ex += len[k];
     }
@@ -212,7 +265,7 @@ __device__ void expand(const Operand& o, uint32_t tile, uint32_t g0, uint32_t cn
 
 __global__ __launch_bounds__(CT_TPB) void k_canon_tiles(CanonArgs p) {
     __shared__ word_t s_vals[CT + 2];       // [0] = group before the tile, [cnt+1] after
-    __shared__ word_t s_sw[CT + 1];         // staged code words
+    __shared__ word_t s_sw[2][CT + 1];      // staged code words of each operand
     __shared__ uint32_t s_sp[CT + 2];       // their start groups
     __shared__ uint32_t s_idx[CT];          // covering word of each group
     __shared__ uint32_t s_shs[CT_NW + 1];
@@ -229,9 +282,15 @@ __global__ __launch_bounds__(CT_TPB) void k_canon_tiles(CanonArgs p) {
         const uint32_t g0 = tile * (uint32_t)CT;
         const uint32_t cnt = min((uint32_t)CT, p.groups - g0);
 
+        // both operands' tile starts, then both operands' words in flight
+        Staged sa, sb;
+        stage_starts(p.a, tile, sa);
+        if (p.b.w) stage_starts(p.b, tile, sb);
+        stage_words(p.a, g0, cnt, p.groups, sa);
+        if (p.b.w) stage_words(p.b, g0, cnt, p.groups, sb);
         word_t vpa = SENT, vna = SENT, vpb = SENT, vnb = SENT;
-        expand(p.a, tile, g0, cnt, p.groups, vals, s_sw, s_sp, s_idx, s_shs, false, p.op, vpa, vna);
-        if (p.b.w) expand(p.b, tile, g0, cnt, p.groups, vals, s_sw, s_sp, s_idx, s_shs, true, p.op, vpb, vnb);
+        expand(p.a, sa, g0, cnt, vals, s_sw[0], s_sp, s_idx, s_shs, false, p.op, vpa, vna);
+        if (p.b.w) expand(p.b, sb, g0, cnt, vals, s_sw[1], s_sp, s_idx, s_shs, true, p.op, vpb, vnb);
         if (tid == 0) {
             word_t vp = vpa, vn = vna;
             if (p.b.w) {
@@ -892,21 +951,19 @@ hipError_t launch_canon(const word_t* a, const uint32_t* cka, uint32_t na, int a
     p.b = Operand{b, ckb, nb, (uint32_t)(blit ? 1 : 0), ts + (nt + 1)};
     p.groups = groups; p.ntiles = groups ? nt : 0u;
     p.op = op; p.out = out; p.outck = outck; p.status = status; p.ctl = ctl;
-    if (alit && (!b || blit) && groups >= 4u) {
+    // (16-byte loads: operands at 16-byte aligned addresses; borrowed views
+    // of index bitmaps may start anywhere and take the tile pass)
+    const bool al16 = ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b)) & 15u) == 0;
+    if (alit && (!b || blit) && groups >= 4u && al16) {
         const uint32_t fg = lf_grid(groups);
         if (fg) launch_lit_fused(p, fg, s);
         else launch_canon_lit(p, s);
         return hipGetLastError();
     }
-    if (!alit && na > 0) {
-        const uint32_t nc = (na + CHUNKW - 1) / CHUNKW;
-        hipLaunchKernelGGL(k_tile_starts, dim3(nc / 256 + 1), dim3(256), 0, s, a, cka, na, nt, ts);
-    }
-    if (b && !blit && nb > 0) {
-        const uint32_t nc = (nb + CHUNKW - 1) / CHUNKW;
-        hipLaunchKernelGGL(k_tile_starts, dim3(nc / 256 + 1), dim3(256), 0, s, b, ckb, nb, nt, ts + (nt + 1));
-    }
-    hipLaunchKernelGGL(k_zero_canon, dim3(min((nt + 255) / 256, 4096u)), dim3(256), 0, s, status, nt, ctl);
+    const uint32_t nca = !alit ? (na + CHUNKW - 1) / CHUNKW + 1 : 0u;
+    const uint32_t ncb = b && !blit ? (nb + CHUNKW - 1) / CHUNKW + 1 : 0u;
+    const uint32_t gx = max(max(nca, ncb), min(nt, 4096u * 256u));
+    hipLaunchKernelGGL(k_canon_prep, dim3((gx + 255) / 256, 3), dim3(256), 0, s, p, nt);
     if (p.ntiles) hipLaunchKernelGGL(k_canon_tiles, dim3(p.ntiles), dim3(CT_TPB), 0, s, p);
     return hipGetLastError();
 }
