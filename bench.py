@@ -270,6 +270,8 @@ def main():
     ap.add_argument("--queries", type=int, default=1024)
     ap.add_argument("--cpu-sample", type=int, default=16)
     ap.add_argument("--no-wah", action="store_true")
+    ap.add_argument("--wah-only", type=int, default=0, metavar="REPS",
+                    help="time only the WAH AND/OR cells (REPS ops each) and print them as JSON")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-conj", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
@@ -287,6 +289,9 @@ def main():
 
     import libfastlib_amd as L
     L.lib()
+    if args.wah_only:
+        print(json.dumps({"wah_ops_2p28": wah_cells(L, reps=args.wah_only)}))
+        return
     extra = () if args.no_learned else ((5005, args.train_queries), (5006, args.test_queries))
     col, lo, hi, more = config3(args.log2_rows, args.queries, extra=extra)
     t = time.time()

@@ -133,26 +133,25 @@ WAH_HD uint32_t status_of(uint64_t v) { return (uint32_t)(v >> 62); }
 // Token consumers.  A token is a span of positions with one class: a literal
 // (len 1, any value) or a fill-valued span with a representation.
 
-// Builds the context-free summary of a thread's range.
+// Builds the context-free summary of a thread's range.  Every field is
+// assigned once per token from selects (stores chosen by branch would be
+// merged into one store through a chosen field address, which keeps the
+// summary in memory).
 struct CountSink {
     Summary s;
     WAH_HD void init() { s = summary_empty(); }
     WAH_HD void tok(uint32_t cls, uint32_t len, uint32_t repF, word_t /*lit*/) {
-        if (s.first == C_NONE) {
-            s.first = cls; s.count = 1; s.trail = cls;
-            s.tlen = (cls == C_MIXED) ? 0u : len;
-            s.trep = repF; s.uniform = (cls != C_MIXED) ? 1u : 0u;
-            return;
-        }
-        if (cls != C_MIXED && cls == s.trail) {
-            s.tlen += len; s.trep = 0;
-            return;
-        }
-        s.uniform = 0;
-        s.count += 1;
-        s.trail = cls;
-        s.tlen = (cls == C_MIXED) ? 0u : len;
-        s.trep = repF;
+        const bool empty = s.first == C_NONE;
+        const bool extend = !empty && cls != C_MIXED && cls == s.trail;   // the open run goes on
+        const uint32_t nlen = cls == C_MIXED ? 0u : len;
+        Summary n;
+        n.first = empty ? cls : s.first;
+        n.count = empty ? 1u : (extend ? s.count : s.count + 1u);
+        n.trail = extend ? s.trail : cls;
+        n.tlen = extend ? s.tlen + len : nlen;
+        n.trep = extend ? 0u : repF;
+        n.uniform = empty ? (cls != C_MIXED ? 1u : 0u) : (extend ? s.uniform : 0u);
+        s = n;
     }
 };
 
@@ -163,14 +162,14 @@ struct CountSink {
 // of the bitvector.  The word emitter W is called as w(slot, word, headpos).
 template <class W>
 struct WriteSink {
-    W* w;
+    W w;                // the word emitter (held by value)
     uint32_t next;      // next free output slot
     uint32_t rcls;      // class of the open run (C_NONE if none)
     uint32_t rlen;      // its length so far
     uint32_t rrep;      // its representation if rlen == 1
     uint32_t pos;       // position of the next token
-    WAH_HD void init(W* ww, const Summary& pre, uint32_t startpos) {
-        w = ww; next = pre.count; pos = startpos;
+    WAH_HD void init(const W* ww, const Summary& pre, uint32_t startpos) {
+        w = *ww; next = pre.count; pos = startpos;
         if (pre.first != C_NONE && (pre.trail == C_ZERO || pre.trail == C_ONE)) {
             rcls = pre.trail; rlen = pre.tlen; rrep = pre.trep;
         } else {
@@ -183,22 +182,22 @@ struct WriteSink {
             word_t word;
             if (rlen >= 2 || rrep) word = (b ? HEADER1 : HEADER0) | rlen;
             else word = b ? ALLONES : 0u;
-            (*w)(next - 1, word, pos - rlen);
+            w(next - 1, word, pos - rlen);
         }
         rcls = C_NONE; rlen = 0; rrep = 0;
     }
     WAH_HD void tok(uint32_t cls, uint32_t len, uint32_t repF, word_t lit) {
-        if (cls != C_MIXED && cls == rcls) {
-            rlen += len; rrep = 0; pos += len;
-            return;
+        const bool extend = cls != C_MIXED && cls == rcls;    // the open run goes on
+        const uint32_t olen = rlen;
+        if (!extend) {
+            close_run();
+            ++next;
+            if (cls == C_MIXED) w(next - 1, lit, pos);
         }
-        close_run();
-        ++next;
-        if (cls == C_MIXED) {
-            (*w)(next - 1, lit, pos);
-        } else {
-            rcls = cls; rlen = len; rrep = repF;
-        }
+        // fields assigned once, from selects (see CountSink)
+        rcls = extend ? cls : (cls == C_MIXED ? (uint32_t)C_NONE : cls);
+        rlen = extend ? olen + len : (cls == C_MIXED ? 0u : len);
+        rrep = extend || cls == C_MIXED ? 0u : repF;
         pos += len;
     }
     WAH_HD void finish() { close_run(); }
@@ -209,12 +208,12 @@ struct WriteSink {
 // accessor R (R::at(index) returns the code word at that index).
 template <class R>
 struct Run {
-    const R* src;
+    R src;              // the words (held by value: runs are swapped by value)
     uint32_t it;        // index of the current code word
     word_t nWords;      // remaining groups of the current code word
     int isFill;
     int fillBit;
-    WAH_HD word_t cur() const { return src->at(it); }
+    WAH_HD word_t cur() const { return src.at(it); }
     WAH_HD void decode() {
         const word_t w = cur();
         fillBit = (w > HEADER1);
@@ -232,6 +231,19 @@ struct Run {
         }
     }
 };
+
+// Field-wise choice between two runs (readers provide R::pick): plain
+// value selects, so that choosing a run never takes its address.
+template <class R>
+WAH_HD Run<R> pick(bool c, const Run<R>& a, const Run<R>& b) {
+    Run<R> r;
+    r.src = R::pick(c, a.src, b.src);
+    r.it = c ? a.it : b.it;
+    r.nWords = c ? a.nWords : b.nWords;
+    r.isFill = c ? a.isFill : b.isFill;
+    r.fillBit = c ? a.fillBit : b.fillBit;
+    return r;
+}
 
 // Emits the token of one group-valued span (class from the value).
 template <class S>
@@ -292,95 +304,96 @@ WAH_HD void take(Run<R>& r, word_t n) {          // n <= r.nWords groups of the 
 
 // One step of the *_c2 loop of `op`; returns the groups consumed.
 // canon != 0 forces every token's representation to L (the canonical result
-// of the c0/d1/d2/c1 routines).
+// of the c0/d1/d2/c1 routines).  The branch of the reference's loop is first
+// reduced to an action and the operand it is anchored on (f, the other o);
+// f and o are chosen from x and y field by field and written back the same
+// way, so that each action runs on fixed operands (a run chosen through a
+// reference would keep both runs in memory instead of registers).
+enum StepAct : int { S_LIT = 0, S_FILL_SKIP = 1, S_COPY_IN = 2, S_FILL_SHORT = 3 };
+
 template <class R, class S>
 WAH_HD word_t c2_step(S& sink, Run<R>& x, Run<R>& y, int op, uint32_t canon, word_t lim = 0xFFFFFFFFu) {
     const uint32_t F = canon ? 0u : 1u;
     if (x.nWords == 0) x.decode();
     if (y.nWords == 0) y.decode();
-    auto cap = [&](word_t n) { return n < lim ? n : lim; };
-    // a fill of `f` spans the step, the other operand's words are skipped
-    auto fill_skip = [&](Run<R>& f, Run<R>& o, int bit) -> word_t {
-        const word_t nw = cap(f.nWords);
-        emit_fill(sink, bit, nw, F);
-        o.minus_eq(nw);
-        take(f, nw);
-        return nw;
-    };
-    // the words of `c` are copied within the fill of `f`
-    auto copy_in = [&](Run<R>& f, Run<R>& c, bool neg) -> word_t {
-        const word_t nw = cap(f.nWords);
-        word_t r = nw;
-        copy_runs_tok(sink, c, r, neg, canon);
-        const word_t used = nw - r;
-        f.nWords -= used;
-        f.it += (f.nWords == 0);
-        return used;
-    };
-    // both fills, the shorter `s` ends first: the span has the other's bit
-    auto fill_short = [&](Run<R>& l, Run<R>& sh) -> word_t {
-        const word_t nw = cap(sh.nWords);
-        emit_fill(sink, l.fillBit, nw, F);
-        l.nWords -= nw;
-        take(sh, nw);
-        return nw;
-    };
+    // act on (f, o): f = x, or f = y when `swp`
+    int act = S_LIT, bit = 0;
+    bool swp = false, neg = false;
     if (op == AND) {                                   // src/bitvector.cpp:2460-2516
         if (x.isFill) {
             if (y.isFill) {
-                if (y.fillBit == 0) return fill_skip(y, x, 0);
-                else if (y.nWords >= x.nWords) return copy_in(y, x, false);
-                else return fill_short(x, y);
-            } else if (x.fillBit == 0) {
-                return fill_skip(x, y, 0);
-            } else {
-                return copy_in(x, y, false);
-            }
+                if (y.fillBit == 0) { act = S_FILL_SKIP; swp = true; bit = 0; }
+                else if (y.nWords >= x.nWords) { act = S_COPY_IN; swp = true; }
+                else act = S_FILL_SHORT;
+            } else if (x.fillBit == 0) { act = S_FILL_SKIP; bit = 0; }
+            else act = S_COPY_IN;
         } else if (y.isFill) {
-            if (y.fillBit == 0) return fill_skip(y, x, 0);
-            else return copy_in(y, x, false);
+            if (y.fillBit == 0) { act = S_FILL_SKIP; swp = true; bit = 0; }
+            else { act = S_COPY_IN; swp = true; }
         }
     } else if (op == OR) {                             // src/bitvector.cpp:2874-2934
         if (x.isFill) {
             if (y.isFill) {
-                if (y.fillBit != 0) return fill_skip(y, x, 1);
-                else if (y.nWords < x.nWords) return fill_short(x, y);
-                else return copy_in(y, x, false);
-            } else if (x.fillBit) {
-                return fill_skip(x, y, 1);
-            } else {
-                return copy_in(x, y, false);
-            }
+                if (y.fillBit != 0) { act = S_FILL_SKIP; swp = true; bit = 1; }
+                else if (y.nWords < x.nWords) act = S_FILL_SHORT;
+                else { act = S_COPY_IN; swp = true; }
+            } else if (x.fillBit) { act = S_FILL_SKIP; bit = 1; }
+            else act = S_COPY_IN;
         } else if (y.isFill) {
-            if (y.fillBit) return fill_skip(y, x, 1);
-            else return copy_in(y, x, false);
+            if (y.fillBit) { act = S_FILL_SKIP; swp = true; bit = 1; }
+            else { act = S_COPY_IN; swp = true; }
         }
     } else if (op == XOR) {                            // src/bitvector.cpp:3287-3325
         if (x.isFill) {
-            if (y.isFill && y.nWords >= x.nWords) return copy_in(y, x, y.fillBit != 0);
-            else return copy_in(x, y, x.fillBit != 0);
+            if (y.isFill && y.nWords >= x.nWords) { act = S_COPY_IN; swp = true; neg = y.fillBit != 0; }
+            else { act = S_COPY_IN; neg = x.fillBit != 0; }
         } else if (y.isFill) {
-            return copy_in(y, x, y.fillBit != 0);
+            act = S_COPY_IN; swp = true; neg = y.fillBit != 0;
         }
     } else {                                           // MINUS, src/bitvector.cpp:3668-3717
         if (x.isFill) {
             if (y.isFill && y.nWords >= x.nWords) {
-                if (y.fillBit == 0) return copy_in(y, x, false);
-                else return fill_skip(y, x, 0);
-            } else if (x.fillBit == 0) {
-                return fill_skip(x, y, 0);
-            } else {
-                return copy_in(x, y, true);
-            }
+                if (y.fillBit == 0) { act = S_COPY_IN; swp = true; }
+                else { act = S_FILL_SKIP; swp = true; bit = 0; }
+            } else if (x.fillBit == 0) { act = S_FILL_SKIP; bit = 0; }
+            else { act = S_COPY_IN; neg = true; }
         } else if (y.isFill) {
-            if (y.fillBit == 0) return copy_in(y, x, false);
-            else return fill_skip(y, x, 0);
+            if (y.fillBit == 0) { act = S_COPY_IN; swp = true; }
+            else { act = S_FILL_SKIP; swp = true; bit = 0; }
         }
     }
-    // both literal: append_active of the combined word
-    emit_lit(sink, apply_op(x.cur(), y.cur(), op), canon);
-    x.nWords = 0; y.nWords = 0; ++x.it; ++y.it;
-    return 1;
+    if (act == S_LIT) {
+        // both literal: append_active of the combined word
+        emit_lit(sink, apply_op(x.cur(), y.cur(), op), canon);
+        x.nWords = 0; y.nWords = 0; ++x.it; ++y.it;
+        return 1;
+    }
+    Run<R> f = pick(swp, y, x), o = pick(swp, x, y);
+    word_t used;
+    if (act == S_FILL_SKIP) {
+        // a fill of f spans the step, o's words are skipped
+        used = f.nWords < lim ? f.nWords : lim;
+        emit_fill(sink, bit, used, F);
+        o.minus_eq(used);
+        take(f, used);
+    } else if (act == S_COPY_IN) {
+        // the words of o are copied within the fill of f
+        const word_t nw = f.nWords < lim ? f.nWords : lim;
+        word_t r = nw;
+        copy_runs_tok(sink, o, r, neg, canon);
+        used = nw - r;
+        f.nWords -= used;
+        f.it += (f.nWords == 0);
+    } else {
+        // both fills, o is the shorter and ends first: the span has f's bit
+        used = o.nWords < lim ? o.nWords : lim;
+        emit_fill(sink, f.fillBit, used, F);
+        f.nWords -= used;
+        take(o, used);
+    }
+    x = pick(swp, o, f);
+    y = pick(swp, f, o);
+    return used;
 }
 
 // Walks the c2 loop over [p0, p1) between two synchronisation points (see
@@ -391,8 +404,8 @@ template <class R, class S>
 WAH_HD void walk(S& sink, const R* ra, const R* rb, uint32_t ia, uint32_t pa, uint32_t ib, uint32_t pb,
                  uint32_t p0, uint32_t p1, int op, uint32_t canon) {
     Run<R> x, y;
-    x.src = ra; x.it = ia; x.nWords = 0; x.isFill = 0; x.fillBit = 0;
-    y.src = rb; y.it = ib; y.nWords = 0; y.isFill = 0; y.fillBit = 0;
+    x.src = *ra; x.it = ia; x.nWords = 0; x.isFill = 0; x.fillBit = 0;
+    y.src = *rb; y.it = ib; y.nWords = 0; y.isFill = 0; y.fillBit = 0;
     if (p0 < p1) {
         if (pa < p0) { x.decode(); x.nWords = pa + x.nWords - p0; }
         if (pb < p0) { y.decode(); y.nWords = pb + y.nWords - p0; }
@@ -436,6 +449,9 @@ WAH_HD uint32_t find_sync(const R* ra, const R* rb, uint32_t& ia, uint32_t& pa, 
 struct ArrayReader {
     const word_t* p;
     WAH_HD word_t at(uint32_t i) const { return p[i]; }
+    static WAH_HD ArrayReader pick(bool c, const ArrayReader& a, const ArrayReader& b) {
+        return ArrayReader{c ? a.p : b.p};
+    }
 };
 
 } // namespace wah

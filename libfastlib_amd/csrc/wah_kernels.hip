@@ -34,11 +34,17 @@ struct WinReader {
         const uint32_t k = i - base;
         return k < cnt ? lds[k] : g[i];
     }
+    static __device__ __forceinline__ WinReader pick(bool c, const WinReader& a, const WinReader& b) {
+        return WinReader{c ? a.lds : b.lds, c ? a.g : b.g, c ? a.base : b.base, c ? a.cnt : b.cnt};
+    }
 };
 
 struct GlobReader {
     const word_t* g;
     __device__ __forceinline__ word_t at(uint32_t i) const { return g[i]; }
+    static __device__ __forceinline__ GlobReader pick(bool c, const GlobReader& a, const GlobReader& b) {
+        return GlobReader{c ? a.g : b.g};
+    }
 };
 
 // ---------------------------------------------------------------------------
