@@ -734,14 +734,15 @@ __device__ __forceinline__ void apply_one(word_t w, bool valid, uint32_t p, uint
     }
 }
 
+// Loads of one round: every load is issued (lanes past the stream read its
+// last group again, and apply_group4 drops words outside [ws, we)), so the
+// loads are straight-line code and each wait counts only the loads issued
+// after the ones it needs.
 template <int U>
 __device__ __forceinline__ void load_round4(const uint4* tw4, uint32_t base4, uint32_t end4, uint4 (&x)[U]) {
     const uint32_t lane = lane_id();
     #pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const uint32_t i = base4 + (uint32_t)u * WAVE + lane;
-        x[u] = i < end4 ? tw4[i] : make_uint4(0u, 0u, 0u, 0u);
-    }
+    for (int u = 0; u < U; ++u) x[u] = tw4[min(base4 + (uint32_t)u * WAVE + lane, end4 - 1u)];
 }
 
 // One sub-round: lane l applies the four words of 16-byte group g4 + l at
@@ -778,33 +779,38 @@ __device__ __forceinline__ uint32_t apply_group4(const uint4& x, uint32_t g4, ui
 // round of U sub-rounds of loads is in flight while the previous round is
 // applied.  The tiled copy is padded so that whole groups can be read.
 template <bool POW2, int U>
-__device__ void stream_words(const QueryArgs& q, uint32_t ws, uint32_t we, uint32_t ng, word_t* R,
-                             uint32_t* cov) {
+__device__ __forceinline__ uint32_t apply_round4(const uint4 (&x)[U], uint32_t b4, uint32_t end4, uint32_t full0,
+                                                 uint32_t full1, uint32_t ws, uint32_t we, uint32_t carry,
+                                                 uint32_t ng, word_t* R, uint32_t* cov) {
+    #pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint32_t g4 = b4 + (uint32_t)u * WAVE;
+        if (g4 >= end4) break;
+        if (g4 >= full0 && g4 + WAVE <= full1)
+            carry = apply_group4<POW2, true>(x[u], g4, ws, we, carry, ng, R, cov);
+        else
+            carry = apply_group4<POW2, false>(x[u], g4, ws, we, carry, ng, R, cov);
+    }
+    return carry;
+}
+
+template <bool POW2, int U>
+__device__ __forceinline__ void stream_words(const QueryArgs& q, uint32_t ws, uint32_t we, uint32_t ng,
+                                             word_t* R, uint32_t* cov) {
     if (ws >= we) return;
     const uint4* tw4 = reinterpret_cast<const uint4*>(q.tw);
     const uint32_t a4 = ws >> 2, end4 = (we + 3u) >> 2;
     const uint32_t full0 = (ws + 3u) >> 2, full1 = we >> 2;   // groups wholly inside
     uint32_t carry = 0;
-    uint4 cur[U], nxt[U];
-    load_round4<U>(tw4, a4, end4, cur);
-    for (uint32_t b4 = a4; b4 < end4; b4 += U * WAVE) {
-        const uint32_t n4 = b4 + U * WAVE;
-        if (n4 < end4) load_round4<U>(tw4, n4, end4, nxt);
-        #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            use_loaded(cur[u].x); use_loaded(cur[u].y); use_loaded(cur[u].z); use_loaded(cur[u].w);
-        }
-        #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const uint32_t g4 = b4 + (uint32_t)u * WAVE;
-            if (g4 >= end4) break;
-            if (g4 >= full0 && g4 + WAVE <= full1)
-                carry = apply_group4<POW2, true>(cur[u], g4, ws, we, carry, ng, R, cov);
-            else
-                carry = apply_group4<POW2, false>(cur[u], g4, ws, we, carry, ng, R, cov);
-        }
-        #pragma unroll
-        for (int u = 0; u < U; ++u) cur[u] = nxt[u];
+    // two batches of U rounds alternate: one in flight while the other is
+    // applied
+    uint4 xa[U], xb[U];
+    load_round4<U>(tw4, a4, end4, xa);
+    for (uint32_t b4 = a4; b4 < end4; b4 += 2u * U * WAVE) {
+        load_round4<U>(tw4, b4 + U * WAVE, end4, xb);
+        carry = apply_round4<POW2, U>(xa, b4, end4, full0, full1, ws, we, carry, ng, R, cov);
+        load_round4<U>(tw4, b4 + 2u * U * WAVE, end4, xa);
+        carry = apply_round4<POW2, U>(xb, b4 + U * WAVE, end4, full0, full1, ws, we, carry, ng, R, cov);
     }
 }
 
