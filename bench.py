@@ -83,26 +83,42 @@ def query_bytes(ix, lo, hi, plan):
     return np.array([ix.query_bytes(float(lo[i]), 2, float(hi[i]), 1, plan) for i in range(len(lo))])
 
 
+WAH_CELLS = [(d, run) for d in (1e-4, 1e-2, 0.5) for run in (1, 64, 4096)]
+OP_NAMES = ("AND", "OR", "XOR", "MINUS")
+
+
 def wah_cells(L, reps=40, copies=4):
+    """The nine Markov cells (density d, mean 1-run length L) of two 2^28-bit
+    vectors: AND / OR / XOR / MINUS in place plus compress(), and cnt(), each
+    timed with inputs resident over `copies` rotating copies (working set
+    beyond the last-level cache).  Algorithmic bytes: 4 x (words of A + of B
+    + of the result) per op, 4 x words of A per cnt."""
     from libfastlib_amd import datagen as G
     out = []
-    for d, run in ((0.5, 1), (0.5, 64), (0.01, 1)):
+    for d, run in WAH_CELLS:
         wa = G.markov_wah(100, 1 << 28, d, run)
         wb = G.markov_wah(101, 1 << 28, d, run)
         A = [L.Bitvector.from_serial(wa) for _ in range(copies)]
         B = [L.Bitvector.from_serial(wb) for _ in range(copies)]
-        for op, name in ((0, "AND"), (1, "OR")):
+        cell = f"d={d:g},L={run}"
+        for op, name in enumerate(OP_NAMES):
             z = A[0].copy()
             path = z.op(B[0], op)
             outw = z.nwords + 1 + (1 if z.write()[-1] else 0)
             ms, _ = L.bench_op_rot(A, B, op, reps)
             byts = 4.0 * (len(wa) + len(wb) + outw)
             gbs = byts / (ms * 1e-3) / 1e9 if ms > 0 else None
-            out.append(dict(cell=f"d={d},L={run}", op=name, path=path, us=round(ms * 1e3, 2),
+            out.append(dict(cell=cell, op=name, path=path, us=round(ms * 1e3, 2),
                             in_words=[int(len(wa)), int(len(wb))], out_words=int(outw),
                             GBs=None if gbs is None else round(gbs, 1),
                             hbm_frac=None if gbs is None else round(gbs / HBM_PEAK_GBS, 3)))
             log("wah", out[-1])
+        ms, ones = L.bench_cnt_rot(A, reps)
+        gbs = 4.0 * len(wa) / (ms * 1e-3) / 1e9 if ms > 0 else None
+        out.append(dict(cell=cell, op="cnt", path="cnt", us=round(ms * 1e3, 2), in_words=[int(len(wa))],
+                        ones=int(ones), GBs=None if gbs is None else round(gbs, 1),
+                        hbm_frac=None if gbs is None else round(gbs / HBM_PEAK_GBS, 3)))
+        log("wah", out[-1])
     return out
 
 

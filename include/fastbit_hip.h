@@ -119,6 +119,10 @@ int fbh_bench_op(const fbh_bitvector* a, const fbh_bitvector* b, int op, int rep
  * the working set exceeds the last-level cache. */
 int fbh_bench_op_rot(fbh_bitvector* const* a, fbh_bitvector* const* b, int npairs, int op,
                      int reps, float* ms_per_op, uint32_t* out_words);
+/* cnt() (src/bitvector.h:616-656) timed as fbh_bench_op_rot times an op:
+ * the device count of a[r % n] `reps` times without host synchronisation;
+ * mean ms per count, and the ones of a[0]. */
+int fbh_bench_cnt_rot(fbh_bitvector* const* a, int n, int reps, float* ms_per_cnt, uint32_t* ones);
 
 /* ---------------------------------------------------------------------- */
 /* Indexes: ibis::relic (equality index, one bitmap per distinct value) and

@@ -773,6 +773,31 @@ int fbh_bench_op(const fbh_bitvector* a, const fbh_bitvector* b, int op, int rep
 // Like fbh_bench_op, cycling over npairs operand pairs of the same shape so
 // that the working set exceeds the last-level cache and every op streams
 // its operands from HBM.
+int fbh_bench_cnt_rot(fbh_bitvector* const* a, int n, int reps, float* ms, uint32_t* ones) {
+    if (!a || n <= 0 || reps <= 0) return fail(FBH_ERR_ARG, "bad argument");
+    uint32_t* ctl = (uint32_t*)scratch(2).get(4ull * fbk::cnt_scratch_words());
+    if (!ctl) return fail(FBH_ERR_NOMEM, "scratch allocation failed");
+    int rc = FBH_OK;
+    for (int i = 0; i < n && !rc; ++i)
+        rc = check(fbk::launch_cnt(a[i]->d, a[i]->nw, ctl, stream()), "cnt");
+    uint32_t h[2] = {0, 0};
+    if (!rc) rc = check(hipMemcpyAsync(h, ctl, sizeof(h), hipMemcpyDeviceToHost, stream()), "cnt copy");
+    if (!rc) rc = check(hipStreamSynchronize(stream()), "cnt sync");
+    if (ones) *ones = h[0];
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0); (void)hipEventCreate(&e1);
+    if (!rc) rc = check(hipEventRecord(e0, stream()), "event");
+    for (int r = 0; r < reps && !rc; ++r)
+        rc = check(fbk::launch_cnt(a[r % n]->d, a[r % n]->nw, ctl, stream()), "cnt");
+    if (!rc) rc = check(hipEventRecord(e1, stream()), "event");
+    if (!rc) rc = check(hipEventSynchronize(e1), "event sync");
+    float t = 0.f;
+    if (!rc) (void)hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+    if (ms) *ms = t / reps;
+    return rc;
+}
+
 int fbh_bench_op_rot(fbh_bitvector* const* a, fbh_bitvector* const* b, int npairs, int op, int reps,
                      float* ms, uint32_t* out_words) {
     if (!a || !b || npairs <= 0 || reps <= 0) return fail(FBH_ERR_ARG, "bad argument");

@@ -54,6 +54,8 @@ def lib():
             "fbh_bv_rids": [vp, _u32p, C.c_uint32, _u32p],
             "fbh_bv_nwords": [vp, _u32p],
             "fbh_bench_op": [vp, vp, C.c_int, C.c_int, C.POINTER(C.c_float), _u32p],
+            "fbh_bench_cnt_rot": [C.POINTER(vp), C.c_int, C.c_int, C.POINTER(C.c_float),
+                                  C.POINTER(C.c_uint32)],
             "fbh_bench_op_rot": [C.POINTER(vp), C.POINTER(vp), C.c_int, C.c_int, C.c_int,
                                  C.POINTER(C.c_float), _u32p],
         }.items():
@@ -207,6 +209,19 @@ def bench_op_rot(a, b, op: int, reps: int):
     check(lib().fbh_bench_op_rot(pa, pb, n, int(op), int(reps), C.byref(ms), C.byref(nw)),
           "fbh_bench_op_rot")
     return ms.value, nw.value
+
+
+def bench_cnt_rot(a, reps: int):
+    """cnt() on the device, cycling over the bitvectors in a: (ms per
+    count, ones of a[0])."""
+    n = len(a)
+    if n == 0:
+        raise ValueError("bench_cnt_rot needs a non-empty list")
+    pa = (C.c_void_p * n)(*[x._h for x in a])
+    ms = C.c_float()
+    ones = C.c_uint32()
+    check(lib().fbh_bench_cnt_rot(pa, n, int(reps), C.byref(ms), C.byref(ones)), "fbh_bench_cnt_rot")
+    return ms.value, ones.value
 
 
 # ---------------------------------------------------------------------------
