@@ -365,7 +365,7 @@ __global__ __launch_bounds__(CT_TPB) void k_canon_tiles(CanonArgs p) {
                 if (emit) {
                     const uint32_t slot = heads_le - 1;
                     p.out[slot] = word;
-                    if ((slot & (CHUNKW - 1)) == 0) p.outck[slot / CHUNKW] = headpos;
+                    if ((slot & (CHUNKW - 1)) == 0 && slot < p.groups) p.outck[slot / CHUNKW] = headpos;
                 }
                 if (g + 1 == p.groups) {
                     p.ctl[1] = heads_le;
@@ -582,7 +582,7 @@ __device__ __forceinline__ void lit_round_write(const CanonArgs& p, const uint4&
             #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const uint32_t slot = c + 4u * lane + (uint32_t)j;
-                if ((slot & (CHUNKW - 1)) == 0) p.outck[slot / CHUNKW] = gl + (uint32_t)j;
+                if ((slot & (CHUNKW - 1)) == 0 && slot < p.groups) p.outck[slot / CHUNKW] = gl + (uint32_t)j;
             }
         }
     } else {
@@ -609,7 +609,7 @@ __device__ __forceinline__ void lit_round_write(const CanonArgs& p, const uint4&
                 if (lane_bit(m.E[j])) {
                     row[at] = comp(v, j);
                     const uint32_t slot = sfirst + at;
-                    if ((slot & (CHUNKW - 1)) == 0) p.outck[slot / CHUNKW] = gl + (uint32_t)j;
+                    if ((slot & (CHUNKW - 1)) == 0 && slot < p.groups) p.outck[slot / CHUNKW] = gl + (uint32_t)j;
                     ++at;
                 }
             }
@@ -638,13 +638,13 @@ __device__ __forceinline__ void lit_round_write(const CanonArgs& p, const uint4&
                     }
                     if (at < LW_ROW) row[at] = word;
                     const uint32_t slot = sfirst + at;
-                    if ((slot & (CHUNKW - 1)) == 0) p.outck[slot / CHUNKW] = headpos;
+                    if ((slot & (CHUNKW - 1)) == 0 && slot < p.groups) p.outck[slot / CHUNKW] = headpos;
                     ++at;
                 }
             }
         }
     }
-    if (closes && lane == 0) {
+    if (closes && lane == 0 && sfirst + ne <= p.groups) {
         // the bitvector's last group ends the last word
         p.ctl[1] = sfirst + ne;
         p.ctl[3] = p.groups;
@@ -829,6 +829,70 @@ __device__ __forceinline__ uint64_t lf_pack(uint32_t tag, uint32_t cnt, uint32_t
     return ((uint64_t)tag << 56) | ((uint64_t)(cnt & 0xFFFFFFFu) << 28) | (uint64_t)(lh1 & 0xFFFFFFFu);
 }
 
+// Block prefix of the one-launch kernels: tid 0 publishes the block's
+// tagged total (bc, bh); wave 0 sums the totals before the block (those of
+// the blocks before it in its segment, and the segment totals that the
+// last block of every earlier segment publishes once its segment is in),
+// polling with a sleep between passes and re-reading only what is missing.
+// A block waits only for blocks of lower index, which the dispatcher has
+// started before it.  Returns the sum in s_pc[0] / s_ph[0] to every thread.
+__device__ __forceinline__ void lf_block_prefix(uint32_t tag, uint32_t bc, uint32_t bh, uint32_t wv,
+                                                uint32_t* s_pc, uint32_t* s_ph) {
+    const uint32_t tid = threadIdx.x, lane = lane_id();
+    if (tid == 0)
+        __hip_atomic_store(&lf_status[blockIdx.x], lf_pack(tag, bc, bh), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (wv == 0) {
+        // lane l waits for block 64 sg + l (l < q) and segment l (l < sg)
+        const uint32_t sg = blockIdx.x / LF_SEG, q = blockIdx.x % LF_SEG;
+        bool need_b = lane < q, need_s = lane < sg;
+        bool published = q != LF_SEG - 1u;
+        uint32_t xc = 0, xh = 0, yc = 0, yh = 0;
+        for (;;) {
+            if (need_b) {
+                const uint64_t x = __hip_atomic_load(&lf_status[sg * LF_SEG + lane], __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+                if ((uint32_t)(x >> 56) == tag) {
+                    xc = (uint32_t)(x >> 28) & 0xFFFFFFFu; xh = (uint32_t)x & 0xFFFFFFFu; need_b = false;
+                }
+            }
+            if (need_s) {
+                const uint64_t x = __hip_atomic_load(&lf_segtot[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if ((uint32_t)(x >> 56) == tag) {
+                    yc = (uint32_t)(x >> 28) & 0xFFFFFFFu; yh = (uint32_t)x & 0xFFFFFFFu; need_s = false;
+                }
+            }
+            const bool pend_b = ballot64(need_b) != 0, pend_s = ballot64(need_s) != 0;
+            if (!published && !pend_b) {
+                // the last block of a segment publishes the segment's total
+                const uint32_t tc = wave_sum(xc) + bc, th = max(wave_max(xh), bh);
+                if (lane == 0)
+                    __hip_atomic_store(&lf_segtot[sg], lf_pack(tag, tc, th), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                published = true;
+            }
+            if (!pend_b && !pend_s) break;
+            __builtin_amdgcn_s_sleep(10);
+        }
+        const uint32_t pc = wave_sum(xc) + wave_sum(yc), ph = max(wave_max(xh), wave_max(yh));
+        if (lane == 0) { s_pc[0] = pc; s_ph[0] = ph; }
+    }
+    __syncthreads();
+}
+
+// Launch tags of the one-launch kernels (shared table, so one sequence):
+// 1..255 per device, the table cleared on the stream when they wrap.
+static uint32_t lf_next_tag(hipStream_t s) {
+    static std::mutex mu;
+    static uint32_t tags[64];
+    std::lock_guard<std::mutex> lk(mu);
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    const uint32_t tag = tags[dev] % 255u + 1u;
+    if (tag == 1u && tags[dev] != 0u) hipLaunchKernelGGL(k_lf_clear, dim3(1), dim3(1024), 0, s);
+    tags[dev] = tag;
+    return tag;
+}
+
 template <int OP>
 __global__ __launch_bounds__(LW_TPB) void k_lit_fused(CanonArgs p, uint32_t n4, uint32_t nwaves, uint32_t tag) {
     __shared__ __align__(16) word_t s_row[LW_NW][LW_ROW];
@@ -860,43 +924,7 @@ __global__ __launch_bounds__(LW_TPB) void k_lit_fused(CanonArgs p, uint32_t n4, 
     __syncthreads();
     uint32_t bc = 0, bh = 0;
     for (int w = 0; w < LW_NW; ++w) { bc += s_c[w]; bh = max(bh, s_h[w]); }
-    if (tid == 0)
-        __hip_atomic_store(&lf_status[blockIdx.x], lf_pack(tag, bc, bh), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (wv == 0) {
-        // lane l waits for block 64 sg + l (l < q) and segment l (l < sg)
-        const uint32_t sg = blockIdx.x / LF_SEG, q = blockIdx.x % LF_SEG;
-        bool need_b = lane < q, need_s = lane < sg;
-        bool published = q != LF_SEG - 1u;
-        uint32_t xc = 0, xh = 0, yc = 0, yh = 0;
-        for (;;) {
-            if (need_b) {
-                const uint64_t x = __hip_atomic_load(&lf_status[sg * LF_SEG + lane], __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_AGENT);
-                if ((uint32_t)(x >> 56) == tag) {
-                    xc = (uint32_t)(x >> 28) & 0xFFFFFFFu; xh = (uint32_t)x & 0xFFFFFFFu; need_b = false;
-                }
-            }
-            if (need_s) {
-                const uint64_t x = __hip_atomic_load(&lf_segtot[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if ((uint32_t)(x >> 56) == tag) {
-                    yc = (uint32_t)(x >> 28) & 0xFFFFFFFu; yh = (uint32_t)x & 0xFFFFFFFu; need_s = false;
-                }
-            }
-            const bool pend_b = ballot64(need_b) != 0, pend_s = ballot64(need_s) != 0;
-            if (!published && !pend_b) {
-                const uint32_t tc = wave_sum(xc) + bc, th = max(wave_max(xh), bh);
-                if (lane == 0)
-                    __hip_atomic_store(&lf_segtot[sg], lf_pack(tag, tc, th), __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-                published = true;
-            }
-            if (!pend_b && !pend_s) break;
-            __builtin_amdgcn_s_sleep(10);
-        }
-        const uint32_t pc = wave_sum(xc) + wave_sum(yc), ph = max(wave_max(xh), wave_max(yh));
-        if (lane == 0) { s_pc[0] = pc; s_ph[0] = ph; }
-    }
-    __syncthreads();
+    lf_block_prefix(tag, bc, bh, wv, s_pc, s_ph);
     uint32_t c = s_pc[0], h = s_ph[0];
     for (uint32_t w = 0; w < wv; ++w) { c += s_c[w]; h = max(h, s_h[w]); }
     word_t prev = e.before;
@@ -923,14 +951,7 @@ static uint32_t lf_grid(uint32_t groups) {
 }
 
 static void launch_lit_fused(const CanonArgs& p, uint32_t grid, hipStream_t s) {
-    static std::mutex mu;
-    static uint32_t tags[64];
-    std::lock_guard<std::mutex> lk(mu);
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
-    const uint32_t tag = tags[dev] % 255u + 1u;
-    if (tag == 1u && tags[dev] != 0u) hipLaunchKernelGGL(k_lf_clear, dim3(1), dim3(1024), 0, s);
-    tags[dev] = tag;
+    const uint32_t tag = lf_next_tag(s);
     const uint32_t n4 = (p.groups + 3u) / 4u, nwaves = grid * LW_NW;
     switch (p.b.w ? p.op : -1) {
     case AND: hipLaunchKernelGGL(k_lit_fused<AND>, dim3(grid), dim3(LW_TPB), 0, s, p, n4, nwaves, tag); break;
@@ -938,6 +959,239 @@ static void launch_lit_fused(const CanonArgs& p, uint32_t grid, hipStream_t s) {
     case XOR: hipLaunchKernelGGL(k_lit_fused<XOR>, dim3(grid), dim3(LW_TPB), 0, s, p, n4, nwaves, tag); break;
     case MINUS: hipLaunchKernelGGL(k_lit_fused<MINUS>, dim3(grid), dim3(LW_TPB), 0, s, p, n4, nwaves, tag); break;
     default: hipLaunchKernelGGL(k_lit_fused<-1>, dim3(grid), dim3(LW_TPB), 0, s, p, n4, nwaves, tag); break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// One launch for canonical ops with a compressed operand (the d1 / d2 / c1
+// routines).  Rounds of 256 groups; wave k takes rounds [kR, kR + R).  A
+// compressed operand's values of a round come from the 256 code words
+// starting at the word holding the round's first group (256 words cover at
+// least 256 groups): their start groups by a wave prefix sum, a mark at the
+// start of each word in a per-wave LDS row, a max-scan of the marks for the
+// word holding each group.  The word holding the next round's first group
+// falls out of the same window; the span's first one comes from a 64-way
+// search of the chunk index.  Classification, block totals and writes are
+// those of k_lit_fused, from the values held in registers.
+
+// The code word of compressed operand o holding group g (< its groups) and
+// that word's first group, by one wave (uniform results).
+__device__ __forceinline__ void mix_find_cover(const Operand& o, uint32_t g, uint32_t& wi, uint32_t& ws) {
+    const uint32_t lane = lane_id();
+    const uint32_t nc = (o.n + CHUNKW - 1) / CHUNKW;
+    uint32_t lo = 0, hi = nc;               // ck[lo] <= g < ck[hi]; ck[nc] = groups
+    while (hi - lo > 1u) {
+        const uint32_t step = (hi - lo + WAVE - 1) / WAVE;
+        const uint32_t j = lo + lane * step;
+        const uint64_t m = ballot64(j < hi && o.ck[min(j, nc)] <= g);   // a prefix of the lanes
+        lo += (63u - (uint32_t)__clzll(m)) * step;
+        hi = min(lo + step, hi);
+    }
+    const uint32_t i = lo * CHUNKW + lane;
+    const bool in = lane < CHUNKW && i < o.n;
+    const uint32_t len = in ? word_len(o.w[min(i, o.n - 1u)]) : 0u;
+    const uint32_t start = o.ck[lo] + wave_incl_sum(len) - len;
+    const uint64_t cm = ballot64(in && start <= g && g - start < len);
+    const uint32_t L = (uint32_t)__ffsll((unsigned long long)cm) - 1u;
+    wi = lo * CHUNKW + L;
+    ws = (uint32_t)__builtin_amdgcn_readlane((int)start, (int)L);
+}
+
+// Values of groups g0 + 4 lane + j of compressed operand o, whose word wi
+// (starting at group ws <= g0) holds g0; wi / ws move on to the word holding
+// g0 + 256.  mk / ww: the wave's LDS rows.
+__device__ __forceinline__ uint4 mix_expand_round(const Operand& o, uint32_t g0, uint32_t& wi, uint32_t& ws,
+                                                  uint32_t* mk, word_t* ww) {
+    const uint32_t lane = lane_id();
+    word_t w[4];
+    uint32_t len[4], sum = 0;
+    #pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t i = wi + 4u * lane + (uint32_t)j;
+        w[j] = o.w[min(i, o.n - 1u)];
+        len[j] = i < o.n ? word_len(w[j]) : 0u;
+        sum += len[j];
+    }
+    uint32_t st = ws + wave_incl_sum(sum) - sum;          // first group of the lane's word 0
+    #pragma unroll
+    for (int j = 0; j < 4; ++j) { ww[4u * lane + j] = w[j]; mk[4u * lane + j] = 0u; }
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t gn = g0 + 256u;
+    uint32_t cand = 0, cst = 0, cen = 0;                  // last word starting at or before gn
+    #pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t k = 4u * lane + (uint32_t)j;
+        if (len[j] && st > g0 && st - g0 < 256u) mk[st - g0] = k;
+        if (len[j] && st <= gn) { cand = k + 1u; cst = st; cen = st + len[j]; }
+        st += len[j];
+    }
+    const uint32_t kn = wave_max(cand);                   // >= 1: word 0 starts at or before g0
+    const uint32_t wsn = (uint32_t)__builtin_amdgcn_readlane((int)cst, (int)((kn - 1u) / 4u));
+    const uint32_t wen = (uint32_t)__builtin_amdgcn_readlane((int)cen, (int)((kn - 1u) / 4u));
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t m0 = mk[4u * lane], m1 = max(m0, mk[4u * lane + 1]);
+    const uint32_t m2 = max(m1, mk[4u * lane + 2]), m3 = max(m2, mk[4u * lane + 3]);
+    uint32_t ex = (uint32_t)__shfl_up((int)wave_incl_max(m3), 1, 64);
+    if (lane == 0) ex = 0;
+    uint4 r;
+    r.x = group_value(ww[max(ex, m0)]);
+    r.y = group_value(ww[max(ex, m1)]);
+    r.z = group_value(ww[max(ex, m2)]);
+    r.w = group_value(ww[max(ex, m3)]);
+    __builtin_amdgcn_wave_barrier();
+    // the window reaches gn (its words cover [ws, >= g0 + 256)): the word
+    // found holds gn, or ends exactly there and the next word does
+    if (wen <= gn) { wi += kn; ws = wen; }
+    else { wi += kn - 1u; ws = wsn; }
+    return r;
+}
+
+// Values of groups g0 + 4 lane + j of a literal operand (clamped reads).
+__device__ __forceinline__ uint4 mix_lit_round(const Operand& o, uint32_t g0, uint32_t groups) {
+    const uint32_t g = g0 + 4u * lane_id(), gl = groups - 1u;
+    return make_uint4(o.w[min(g, gl)], o.w[min(g + 1u, gl)], o.w[min(g + 2u, gl)], o.w[min(g + 3u, gl)]);
+}
+
+// Value of group g of an operand, given for a compressed one the word wi
+// (starting at ws) that holds group g + 1.
+__device__ __forceinline__ word_t mix_group_before(const Operand& o, uint32_t g, uint32_t wi, uint32_t ws) {
+    if (o.literal) return o.w[g];
+    return group_value(o.w[ws <= g ? wi : (wi ? wi - 1u : 0u)]);
+}
+
+template <int OP>
+__global__ __launch_bounds__(LW_TPB) void k_mix_fused(CanonArgs p, uint32_t nrounds, uint32_t R, uint32_t tag) {
+    __shared__ __align__(16) word_t s_row[LW_NW][LW_ROW];
+    __shared__ uint32_t s_mk[LW_NW][LW_ROW];
+    __shared__ uint32_t s_c[LW_NW], s_h[LW_NW], s_pc[1], s_ph[1];
+    const uint32_t lane = lane_id();
+    const uint32_t wv = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+    const uint32_t k = blockIdx.x * LW_NW + wv;
+    const uint32_t groups = p.groups, n4 = (groups + 3u) / 4u;
+    // the span: rounds [m0, m0 + R) as 4-group chunks
+    const uint32_t m0 = min(k * R, nrounds);
+    LitSpan s;
+    s.c0 = min(m0 * WAVE, n4);
+    s.c1 = min((m0 + R) * WAVE, n4);
+    s.nr = (s.c1 - s.c0 + WAVE - 1) / WAVE;
+    const uint32_t G0 = 4u * s.c0, G1 = 4u * s.c1;
+    const bool two = OP >= 0;
+    uint32_t wia = 0, wsa = 0, wib = 0, wsb = 0;
+    if (s.nr && !p.a.literal) mix_find_cover(p.a, G0, wia, wsa);
+    if (two && s.nr && !p.b.literal) mix_find_cover(p.b, G0, wib, wsb);
+    word_t before = SENT;
+    if (s.nr && G0 > 0) {
+        const word_t x = mix_group_before(p.a, G0 - 1u, wia, wsa);
+        before = two ? lit_op1<OP>(x, mix_group_before(p.b, G0 - 1u, wib, wsb)) : x;
+    }
+
+    // phase 1: the span's values
+    uint4 v[LF_R];
+    #pragma unroll
+    for (int r = 0; r < LF_R; ++r) {
+        if ((uint32_t)r >= s.nr) break;
+        const uint32_t g0 = G0 + 256u * (uint32_t)r;
+        const uint4 va = p.a.literal ? mix_lit_round(p.a, g0, groups)
+                                     : mix_expand_round(p.a, g0, wia, wsa, s_mk[wv], s_row[wv]);
+        uint4 x = va;
+        if (two) {
+            const uint4 vb = p.b.literal ? mix_lit_round(p.b, g0, groups)
+                                         : mix_expand_round(p.b, g0, wib, wsb, s_mk[wv], s_row[wv]);
+            x = lit_op4<OP>(va, vb);
+        }
+        const uint32_t g = g0 + 4u * lane;
+        if (g >= groups) x.x = SENT;
+        if (g + 1u >= groups) x.y = SENT;
+        if (g + 2u >= groups) x.z = SENT;
+        if (g + 3u >= groups) x.w = SENT;
+        v[r] = x;
+    }
+    // the group after the span: held by the words the expansion moved to
+    word_t after = SENT;
+    if (s.nr && G1 < groups) {
+        const word_t x = p.a.literal ? p.a.w[G1] : group_value(p.a.w[min(wia, p.a.n - 1u)]);
+        after = x;
+        if (two) after = lit_op1<OP>(x, p.b.literal ? p.b.w[G1] : group_value(p.b.w[min(wib, p.b.n - 1u)]));
+    }
+
+    // phase 2: heads of the span
+    uint32_t hc = 0, lh = 0;
+    {
+        word_t prev = before;
+        #pragma unroll
+        for (int r = 0; r < LF_R; ++r) {
+            const uint32_t Rr = (uint32_t)r;
+            if (Rr >= s.nr) break;
+            const uint32_t i0 = s.c0 + Rr * WAVE;
+            const uint32_t last = min((uint32_t)WAVE - 1u, s.c1 - 1u - i0);
+            const int rn = r + 1 < LF_R ? r + 1 : r;
+            const word_t aft = Rr + 1u < s.nr ? (word_t)__builtin_amdgcn_readlane((int)v[rn].x, 0) : after;
+            uint32_t nh, lh1;
+            lit_round_heads(v[r], prev, aft, i0, last, nh, lh1);
+            hc += nh;
+            lh = max(lh, lh1);
+            prev = (word_t)__builtin_amdgcn_readlane((int)v[r].w, WAVE - 1);
+        }
+    }
+    if (lane == 0) { s_c[wv] = hc; s_h[wv] = lh; }
+    __syncthreads();
+    uint32_t bc = 0, bh = 0;
+    for (int w = 0; w < LW_NW; ++w) { bc += s_c[w]; bh = max(bh, s_h[w]); }
+
+    // phase 3: the totals before this block
+    lf_block_prefix(tag, bc, bh, wv, s_pc, s_ph);
+    uint32_t c = s_pc[0], h = s_ph[0];
+    for (uint32_t w = 0; w < wv; ++w) { c += s_c[w]; h = max(h, s_h[w]); }
+
+    // phase 4: the words, from registers
+    word_t prev = before;
+    #pragma unroll
+    for (int r = 0; r < LF_R; ++r) {
+        const uint32_t Rr = (uint32_t)r;
+        if (Rr >= s.nr) break;
+        const uint32_t i0 = s.c0 + Rr * WAVE;
+        const uint32_t last = min((uint32_t)WAVE - 1u, s.c1 - 1u - i0);
+        const int rn = r + 1 < LF_R ? r + 1 : r;
+        const word_t aft = Rr + 1u < s.nr ? (word_t)__builtin_amdgcn_readlane((int)v[rn].x, 0) : after;
+        const bool closes = s.c1 == n4 && Rr + 1u == s.nr;
+        uint32_t nh, lh1;
+        lit_round_write(p, v[r], prev, aft, i0, last, closes, c, h, s_row[wv], nh, lh1);
+        c += nh;
+        h = max(h, lh1);
+        prev = (word_t)__builtin_amdgcn_readlane((int)v[r].w, WAVE - 1);
+    }
+}
+
+// Rounds per wave of the mixed one-launch kernel (0: the operands need more
+// than LF_R rounds per resident wave) and its grid.
+static bool mix_grid(uint32_t groups, uint32_t& R, uint32_t& grid) {
+    static const uint32_t slots = [] {
+        int dev = 0, cus = 0, b = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_mix_fused<AND>, LW_TPB, 0) != hipSuccess || b <= 0)
+            b = 1;
+        return min((uint32_t)cus * (uint32_t)b, LW_MAXB);
+    }();
+    const uint32_t nrounds = (groups + 255u) / 256u;
+    R = (nrounds + slots * LW_NW - 1) / (slots * LW_NW);
+    if (R > (uint32_t)LF_R) return false;
+    R = max(R, 1u);
+    const uint32_t nwaves = (nrounds + R - 1) / R;
+    grid = (nwaves + LW_NW - 1) / LW_NW;
+    return grid <= LW_MAXB;
+}
+
+static void launch_mix_fused(const CanonArgs& p, uint32_t R, uint32_t grid, hipStream_t s) {
+    const uint32_t tag = lf_next_tag(s);
+    const uint32_t nrounds = (p.groups + 255u) / 256u;
+    switch (p.b.w ? p.op : -1) {
+    case AND: hipLaunchKernelGGL(k_mix_fused<AND>, dim3(grid), dim3(LW_TPB), 0, s, p, nrounds, R, tag); break;
+    case OR: hipLaunchKernelGGL(k_mix_fused<OR>, dim3(grid), dim3(LW_TPB), 0, s, p, nrounds, R, tag); break;
+    case XOR: hipLaunchKernelGGL(k_mix_fused<XOR>, dim3(grid), dim3(LW_TPB), 0, s, p, nrounds, R, tag); break;
+    case MINUS: hipLaunchKernelGGL(k_mix_fused<MINUS>, dim3(grid), dim3(LW_TPB), 0, s, p, nrounds, R, tag); break;
+    default: hipLaunchKernelGGL(k_mix_fused<-1>, dim3(grid), dim3(LW_TPB), 0, s, p, nrounds, R, tag); break;
     }
 }
 
@@ -958,6 +1212,11 @@ hipError_t launch_canon(const word_t* a, const uint32_t* cka, uint32_t na, int a
         const uint32_t fg = lf_grid(groups);
         if (fg) launch_lit_fused(p, fg, s);
         else launch_canon_lit(p, s);
+        return hipGetLastError();
+    }
+    uint32_t mr = 0, mg = 0;
+    if (groups && mix_grid(groups, mr, mg)) {
+        launch_mix_fused(p, mr, mg, s);
         return hipGetLastError();
     }
     const uint32_t nca = !alit ? (na + CHUNKW - 1) / CHUNKW + 1 : 0u;
