@@ -127,6 +127,47 @@ __device__ uint32_t diag_pos(const OpArgs& p, uint32_t d) {
     return pa < pb ? pa : pb;
 }
 
+// 64-way searches by one wave (uniform results; a few dependent loads
+// instead of one per halving).
+// The largest j < n with a[j] <= key (0 when none or n <= 1; a ascending).
+__device__ __forceinline__ uint32_t wave_last_le(const uint32_t* a, uint32_t n, uint32_t key) {
+    const uint32_t lane = lane_id();
+    uint32_t lo = 0, hi = n;                 // a[lo] <= key (or lo = 0) < a[hi]
+    while (hi - lo > 1u) {
+        const uint32_t step = (hi - lo + WAVE - 1) / WAVE;
+        const uint32_t j = lo + lane * step;
+        const uint64_t m = ballot64(lane > 0 && j < hi && a[min(j, n - 1u)] <= key);
+        const uint32_t L = m ? 63u - (uint32_t)__clzll(m) : 0u;
+        lo += L * step;
+        hi = min(lo + step, hi);
+    }
+    return lo;
+}
+
+// diag_pos by one wave: the first ja in [lo, hi] with cka[ja] > ckb[d - ja - 1].
+__device__ uint32_t wave_diag_pos(const OpArgs& p, uint32_t d) {
+    if (d >= p.nca + p.ncb) return p.groups;
+    const uint32_t lane = lane_id();
+    uint32_t lo = d > p.ncb ? d - p.ncb : 0u;
+    uint32_t hi = d < p.nca ? d : p.nca;
+    while (lo < hi) {
+        const uint32_t step = (hi - lo + WAVE - 1) / WAVE;
+        const uint32_t mid = lo + lane * step;
+        const bool in = mid < hi;
+        const uint32_t mc = in ? mid : lo;
+        const uint64_t m = ballot64(in && p.cka[mc] <= p.ckb[d - mc - 1u]);   // a prefix of the lanes
+        const uint32_t cnt = (uint32_t)__popcll(m);
+        if (cnt == 0) { hi = lo; break; }
+        const uint32_t nlo = lo + (cnt - 1u) * step + 1u;
+        hi = min(lo + cnt * step, hi);
+        lo = nlo;
+    }
+    const uint32_t ja = lo, jb = d - lo;
+    const uint32_t pa = ja < p.nca ? p.cka[ja] : 0xFFFFFFFFu;
+    const uint32_t pb = jb < p.ncb ? p.ckb[jb] : 0xFFFFFFFFu;
+    return pa < pb ? pa : pb;
+}
+
 // Emitter for the write pass: result words and the result chunk index.
 struct GlobWriter {
     word_t* out;
@@ -171,16 +212,23 @@ __global__ __launch_bounds__(TPB) void k_op_tiles(OpArgs p) {
     const int tid = threadIdx.x;
     const uint32_t tile = blockIdx.x;
 
+    // the tile's two ends: wave 0 searches the first, wave 1 the second
+    if (tid < 2 * WAVE) {
+        const uint32_t e = (uint32_t)tid / WAVE;
+        const uint32_t P = wave_diag_pos(p, (tile + e) * KTILE);
+        const uint32_t ca = wave_last_le(p.cka, p.nca, P), cb = wave_last_le(p.ckb, p.ncb, P);
+        if ((tid & (WAVE - 1)) == 0) { info[12 + 3 * e] = P; info[13 + 3 * e] = ca; info[14 + 3 * e] = cb; }
+    }
+    __syncthreads();
     if (tid == 0) {
-        const uint32_t P0 = diag_pos(p, tile * KTILE);
-        const uint32_t P1 = diag_pos(p, (tile + 1) * KTILE);
+        const uint32_t P0 = info[12], P1 = info[15];
         // chunk windows: from the chunk holding P0 through one chunk past
         // the chunk holding P1
-        uint32_t ca0 = last_le(p.cka, p.nca, P0);
-        uint32_t ca1 = last_le(p.cka, p.nca, P1) + 2;
+        uint32_t ca0 = info[13];
+        uint32_t ca1 = info[16] + 2;
         if (ca1 > p.nca) ca1 = p.nca;
-        uint32_t cb0 = last_le(p.ckb, p.ncb, P0);
-        uint32_t cb1 = last_le(p.ckb, p.ncb, P1) + 2;
+        uint32_t cb0 = info[14];
+        uint32_t cb1 = info[17] + 2;
         if (cb1 > p.ncb) cb1 = p.ncb;
         uint32_t baseA = ca0 * CHUNK, cntA = min(ca1 * CHUNK, p.na) - baseA;
         uint32_t baseB = cb0 * CHUNK, cntB = min(cb1 * CHUNK, p.nb) - baseB;
@@ -637,6 +685,13 @@ uint32_t op_tiles(uint32_t na, uint32_t nb) {
     return items ? cdiv(items, KTILE) : 1u;
 }
 
+// Clears the look-back status words and the 8 control words of a launch.
+__global__ void k_zero_status_ctl(uint64_t* status, uint32_t n, uint32_t* ctl) {
+    const uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i0 < 8u) ctl[i0] = 0u;
+    for (uint32_t i = i0; i < n; i += gridDim.x * blockDim.x) status[i] = 0ull;
+}
+
 hipError_t launch_op(const word_t* a, const uint32_t* cka, uint32_t na,
                      const word_t* b, const uint32_t* ckb, uint32_t nb,
                      uint32_t groups, int op, int canon, word_t* out, uint32_t* outck,
@@ -648,8 +703,7 @@ hipError_t launch_op(const word_t* a, const uint32_t* cka, uint32_t na,
     p.out = out; p.outck = outck; p.status = status;
     p.total = ctl + 1;
     const uint32_t nt = op_tiles(na, nb);
-    hipLaunchKernelGGL(k_zero_u64, dim3(min(cdiv(nt, TPB), 4096u)), dim3(TPB), 0, s, status, nt);
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(TPB), 0, s, ctl, 8u);
+    hipLaunchKernelGGL(k_zero_status_ctl, dim3(min(cdiv(nt, TPB), 4096u)), dim3(TPB), 0, s, status, nt, ctl);
     hipLaunchKernelGGL(k_op_tiles, dim3(nt), dim3(TPB), 0, s, p);
     return hipGetLastError();
 }
