@@ -233,3 +233,35 @@ def test_c0_beyond_one_batch(L, orc):
         got, gpath = gpu_op(L, a, b, op)
         assert gpath == wpath == "c0"
         assert np.array_equal(got, want), op
+
+
+def test_d2_beyond_one_batch(L, orc):
+    """Compressed operands of 11.3M groups (more rounds than the one-launch
+    kernel holds per resident wave): the tile pass takes them; its words
+    equal the oracle's for every op."""
+    from libfastlib_amd import datagen as G
+    nbits = 31 * 11_300_000
+    a = G.markov_wah(7, nbits, 0.5, 64)
+    b = G.markov_wah(8, nbits, 0.5, 64)
+    for op in range(4):
+        want, wpath = orc.op(a, b, op)
+        got, gpath = gpu_op(L, a, b, op)
+        assert gpath == wpath == "d2", (op, gpath, wpath)
+        assert np.array_equal(got, want), op
+
+
+def test_d2_mixed_kernel_shapes(L, orc):
+    """d1 / d2 ops at group counts around round (256), chunk (32 words) and
+    segment boundaries, operands from runs of every kind."""
+    from libfastlib_amd import datagen as G
+    for k, groups in enumerate((255, 256, 257, 4096 + 3, 256 * 64 * 4 + 1, 700_001)):
+        nbits = 31 * groups + (k % 3) * 7
+        a = G.markov_wah(100 + k, nbits, 0.5, 48)
+        b = G.markov_wah(200 + k, nbits, 0.4, 20)
+        lit = orc.decompress(b)
+        for x, y in ((a, b), (a, lit), (lit, a)):
+            for op in range(4):
+                want, wpath = orc.op(x, y, op)
+                got, gpath = gpu_op(L, x, y, op)
+                assert gpath == wpath, (groups, op)
+                assert np.array_equal(got, want), (groups, op, gpath)
