@@ -190,6 +190,28 @@ int fbh_index_deconstruct(const fbh_index* ix, double** keys, uint64_t* nkeys,
                           int64_t** offsets, uint64_t* noffsets, uint32_t** bms,
                           uint64_t* nbms);
 void fbh_free(void* p);
+/* fastbit_iapi_attach_full_index (src/iapi.cpp:2037-2055) over the column
+ * it indexes (the array fastbit_iapi_register_array names): the index in
+ * the deconstructed form above, bin when nkeys == 2*(noffsets-1) and relic
+ * when nkeys+1 == noffsets (column::attachIndex, src/column.cpp:5637-5677;
+ * bin boundaries from the minima and maxima as src/ibin.cpp:358-373 makes
+ * them).  Unlike the reference, which keeps pointers to the caller's
+ * arrays, the words are checked and copied into device memory, so the
+ * arrays may be released when this returns.  A bitmap that does not cover
+ * exactly nrows rows is an FBH_ERR_ARG error. */
+int fbh_index_attach(const void* col, int type, uint32_t nrows, const double* keys, uint64_t nkeys,
+                     const int64_t* offsets, uint64_t noffsets, const uint32_t* bms, uint64_t nbms,
+                     fbh_index** out);
+/* relic::read / bin::read of an index file (src/irelic.cpp:690-786,
+ * src/ibin.cpp:468-553): the "#IBIS" header, 4- or 8-byte bitmap
+ * positions, the keys, and the serialized bitmaps, mapped from the file and
+ * copied to device memory once.  Relic (type 7) and bin (type 0) files;
+ * nrows must equal the row count in the file. */
+int fbh_index_read_file(const char* path, const void* col, int type, uint32_t nrows, fbh_index** out);
+/* fastbit_iapi_get_number_of_hits (src/iapi.cpp:2011-2017): the ones of
+ * index::sumBins(ib, ie) (src/index.cpp:6450-6491) over the index's
+ * bitmaps. */
+int fbh_index_sum_bins(fbh_index* ix, uint32_t ib, uint32_t ie, uint64_t* hits);
 /* fastbit_iapi_resolve_range (src/iapi.cpp:1981-2009): relic::locate /
  * bin::locate of the range. */
 int fbh_index_resolve_range(const fbh_index* ix, double lo, int lop, double hi, int rop,

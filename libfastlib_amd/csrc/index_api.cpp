@@ -23,6 +23,10 @@
 #include <cstring>
 #include <string>
 #include <vector>
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
 using namespace fbh_rt;
 
@@ -69,7 +73,7 @@ struct DBuf {
     T* release() { T* q = p; p = nullptr; return q; }
 };
 
-int sync() { return check(hipStreamSynchronize(stream()), "sync"); }
+int stream_sync() { return check(hipStreamSynchronize(stream()), "sync"); }
 
 template <class T>
 int to_dev(T* d, const T* h, size_t n) {
@@ -81,7 +85,7 @@ template <class T>
 int to_host(T* h, const T* d, size_t n) {
     if (!n) return FBH_OK;
     int rc = check(hipMemcpyAsync(h, d, sizeof(T) * n, hipMemcpyDeviceToHost, stream()), "download");
-    return rc ? rc : sync();
+    return rc ? rc : stream_sync();
 }
 
 } // namespace
@@ -252,7 +256,7 @@ int encode_bitmaps(fbh_index* ix, const uint32_t* skeys_d, const uint32_t* srows
                                   any_unpack ? dunpack.p : nullptr, nk, ix->groups, ix->anb, words.p,
                                   stream()), "concat");
     if (rc) return rc;
-    if ((rc = sync())) return rc;
+    if ((rc = stream_sync())) return rc;
     // install
     ix->cumrows.assign(nk + 1, 0.0);
     for (uint32_t k = 0; k < nk; ++k)
@@ -327,7 +331,7 @@ int finish_index(fbh_index* ix) {
     rc = check(fbk::launch_tseg_fill(ix->words, toff.p, ix->TW, ix->nobs, ix->ntiles, ix->groups, ix->anb,
                                      ix->tw, stream()), "tiled copy");
     if (rc) return rc;
-    return sync();
+    return stream_sync();
 }
 
 int upload_column(fbh_index* ix, const void* col, int type, uint32_t nrows) {
@@ -420,7 +424,9 @@ struct Locate {
     uint32_t cand0 = 0, hit0 = 0, hit1 = 0, cand1 = 0;
 };
 
-void locate(const fbh_index* ix, double lo, int lop, double hi, int rop, Locate& L) {
+// relic::locate / bin::locate as they return (an empty range can come
+// back with hit1 < hit0)
+void locate_raw(const fbh_index* ix, double lo, int lop, double hi, int rop, Locate& L) {
     if (ix->kind == FBH_RELIC) {
         fbh_ix::relic_locate_range(ix->keys, lo, lop, hi, rop, L.hit0, L.hit1);
         L.cand0 = L.hit0; L.cand1 = L.hit1;
@@ -428,11 +434,17 @@ void locate(const fbh_index* ix, double lo, int lop, double hi, int rop, Locate&
         fbh_ix::bin_locate_range(ix->keys, ix->minval, ix->maxval, lo, lop, hi, rop, L.cand0,
                                  L.cand1, L.hit0, L.hit1);
     }
+}
+
+void locate(const fbh_index* ix, double lo, int lop, double hi, int rop, Locate& L) {
+    locate_raw(ix, lo, lop, hi, rop, L);
     if (L.hit1 < L.hit0) L.hit1 = L.hit0;
 }
 
 // sumBins' choice between OR of [ib,ie) and the complement of the OR of
-// the outside (src/index.cpp:6450-6491), by bitmap bytes.
+// the outside (src/index.cpp:6450-6491).  The measure is offset64: the
+// serialized sizes where a constructor or a file filled it, else the
+// bitmap bytes sumBins fills it with.
 struct SumPlan {
     int kind;         // 0 none (all zero), 1 all ones, 2 straight, 3 complement
 };
@@ -441,8 +453,9 @@ SumPlan sum_plan(const fbh_index* ix, uint32_t ib, uint32_t ie) {
     SumPlan p{2};
     if (ie > ix->nobs) ie = ix->nobs;
     if (ib >= ie) { p.kind = 0; return p; }
-    const double all = ix->cumbytes[ix->nobs];
-    const double mid = ix->cumbytes[ie] - ix->cumbytes[ib];
+    const std::vector<double>& m = ix->offsets_built ? ix->cumser : ix->cumbytes;
+    const double all = m[ix->nobs];
+    const double mid = m[ie] - m[ib];
     if (mid == 0) p.kind = 0;
     else if (all == mid) p.kind = 1;
     else p.kind = (mid <= std::floor(all / 2)) ? 2 : 3;
@@ -464,6 +477,15 @@ void fill_query(const fbh_index* ix, fbk::QueryArgs& q, double lo, int lop, doub
         else fbh_ix::int_bounds<uint16_t>(lo, l, hi, r, ilo, ihi);
     }
     q.lop = l; q.rop = r; q.lo = lo; q.hi = hi; q.ilo = ilo; q.ihi = ihi;
+}
+
+// The bitmap ranges of sumBins(ib, ie) as the literal plan ORs them.
+void set_sum_ranges(const fbh_index* ix, fbk::QueryArgs& q, uint32_t ib, uint32_t ie) {
+    const SumPlan sp = sum_plan(ix, ib, ie);
+    q.r0 = q.r1 = q.r2 = q.r3 = 0;
+    if (sp.kind == 1) { q.r0 = 0; q.r1 = 0; q.flip = 1; }
+    else if (sp.kind == 2) { q.r0 = ib; q.r1 = std::min(ie, ix->nobs); }
+    else if (sp.kind == 3) { q.r0 = 0; q.r1 = ib; q.r2 = std::min(ie, ix->nobs); q.r3 = ix->nobs; q.flip = 1; }
 }
 
 // Per-block partial counts of evaluated queries (one row per query), summed
@@ -513,11 +535,7 @@ int index_eval_literal(fbh_index* ix, double lo, int lop, double hi, int rop, in
         // doubles
         return check(fbk::launch_query_scan(q, dev_type(ix->type), ix->nrows, s), "scan");
     }
-    const SumPlan sp = sum_plan(ix, L.hit0, L.hit1);
-    q.r0 = q.r1 = q.r2 = q.r3 = 0;
-    if (sp.kind == 1) { q.r0 = 0; q.r1 = 0; q.flip = 1; }
-    else if (sp.kind == 2) { q.r0 = L.hit0; q.r1 = L.hit1; }
-    else if (sp.kind == 3) { q.r0 = 0; q.r1 = L.hit0; q.r2 = L.hit1; q.r3 = ix->nobs; q.flip = 1; }
+    set_sum_ranges(ix, q, L.hit0, L.hit1);
     if (ix->kind == FBH_BIN) {
         if (L.cand0 < L.hit0 && !ix->null_bitmap(L.cand0)) q.cand0 = L.cand0;
         if (L.cand1 > L.hit1 && L.hit1 < ix->nobs && !ix->null_bitmap(L.hit1)) q.cand1 = L.hit1;
@@ -640,6 +658,135 @@ int plan_heuristic(const fbh_index* ix, double lo, int lop, double hi, int rop);
 double index_cost(const fbh_index* ix, double lo, int lop, double hi, int rop);
 double scan_cost(const fbh_index* ix);
 
+// Installs bitmaps given in the serialized index form (src/iapi.h:160-176):
+// bitmap b is bms[offsets[b], offsets[b+1]); an empty range is a null
+// bitmap, any other is code words ++ [active value when the active bit
+// count is not 0] ++ active bit count (bitvector::write(array_t&),
+// src/bitvector.cpp:2283).  The words are checked on the host before a
+// kernel reads them, so that a damaged index is an error and not an access
+// beyond a bitmap: every bitmap has to cover exactly the column's rows.
+// The ones of every bitmap come out of the same pass (the rows-per-bitmap
+// table of the cost model).
+int attach_bitmaps(fbh_index* ix, const int64_t* offsets, uint32_t nobs, const uint32_t* bms,
+                   uint64_t nbms) {
+    if (offsets[0] < 0) return fail(FBH_ERR_ARG, "negative bitmap offset");
+    for (uint32_t b = 0; b < nobs; ++b)
+        if (offsets[b + 1] < offsets[b]) return fail(FBH_ERR_ARG, "bitmap offsets are not ascending");
+    if ((uint64_t)offsets[nobs] > nbms) return fail(FBH_ERR_ARG, "bitmap offsets beyond the words");
+    const uint64_t base = (uint64_t)offsets[0];
+    const uint32_t trailer = ix->anb > 0 ? 2u : 1u;
+    std::vector<uint64_t> off(nobs + 1);
+    std::vector<uint32_t> nw(nobs);
+    ix->cumrows.assign(nobs + 1, 0.0);
+    for (uint32_t b = 0; b < nobs; ++b) {
+        const uint64_t s = (uint64_t)offsets[b], e = (uint64_t)offsets[b + 1];
+        off[b] = s - base;
+        nw[b] = 0;
+        uint64_t ones = 0;
+        if (e > s) {
+            if (e - s < trailer) return fail(FBH_ERR_ARG, "bitmap " + std::to_string(b) + " is truncated");
+            if (bms[e - 1] != ix->anb)
+                return fail(FBH_ERR_ARG, "bitmap " + std::to_string(b) + " does not have the column's rows");
+            if (ix->anb) {
+                const uint32_t av = bms[e - 2];
+                if (av >> ix->anb) return fail(FBH_ERR_ARG, "bitmap " + std::to_string(b) + " has a bad active word");
+                ones += (uint32_t)__builtin_popcount(av);
+            }
+            const uint64_t ncode = e - s - trailer;
+            if (ncode > 0xFFFFFFFFull) return fail(FBH_ERR_SIZE, "bitmap larger than 2^32 words");
+            uint64_t groups = 0;
+            for (uint64_t i = s; i < s + ncode; ++i) {
+                const uint32_t w = bms[i];
+                if (w & 0x80000000u) {
+                    const uint32_t n = w & 0x3FFFFFFFu;
+                    if (n == 0) return fail(FBH_ERR_ARG, "bitmap " + std::to_string(b) + " has an empty fill");
+                    groups += n;
+                    if (w & 0x40000000u) ones += (uint64_t)MAXBITS * n;
+                } else {
+                    groups += 1;
+                    ones += (uint32_t)__builtin_popcount(w);
+                }
+            }
+            if (groups != ix->groups)
+                return fail(FBH_ERR_ARG, "bitmap " + std::to_string(b) + " does not have the column's rows");
+            nw[b] = (uint32_t)ncode;
+        }
+        ix->cumrows[b + 1] = ix->cumrows[b] + (double)ones;
+    }
+    off[nobs] = (uint64_t)offsets[nobs] - base;
+    const uint64_t total = off[nobs];
+    // one word beyond the last bitmap, like the built index
+    DBuf<uint32_t> words;
+    int rc = words.alloc(total + 1);
+    const uint32_t zero = 0;
+    if (!rc) rc = to_dev(words.p, bms + base, total);
+    if (!rc) rc = to_dev(words.p + total, &zero, 1);
+    if (!rc) rc = stream_sync();
+    if (rc) return rc;
+    ix->nobs = nobs;
+    ix->off.swap(off);
+    ix->nw.swap(nw);
+    ix->nwords = total;
+    ix->words = words.release();
+    ix->offsets_built = true;      // the key/offset constructors fill offset64
+    return FBH_OK;
+}
+
+// Keys of an attached index: relic values (sorted), or bin boundaries with
+// the per-bin minima and maxima; then the bitmaps and the tables the plans
+// read.  The column range of the overlap test comes from the keys, as for
+// the built index.
+int attach_index(fbh_index* ix, const double* vals, const double* bounds, const double* minv,
+                 const double* maxv, uint32_t nobs, const int64_t* offsets, const uint32_t* bms,
+                 uint64_t nbms) {
+    if (ix->kind == FBH_RELIC) {
+        ix->keys.assign(vals, vals + nobs);
+        for (uint32_t b = 1; b < nobs; ++b)
+            if (!(ix->keys[b - 1] < ix->keys[b])) return fail(FBH_ERR_ARG, "relic values are not ascending");
+    } else {
+        ix->keys.assign(bounds, bounds + nobs);
+        ix->minval.assign(minv, minv + nobs);
+        ix->maxval.assign(maxv, maxv + nobs);
+        for (uint32_t b = 1; b < nobs; ++b)
+            if (!(ix->keys[b - 1] <= ix->keys[b])) return fail(FBH_ERR_ARG, "bin boundaries are not ascending");
+    }
+    int rc = attach_bitmaps(ix, offsets, nobs, bms, nbms);
+    if (rc) return rc;
+    if (ix->kind == FBH_RELIC) {
+        if (nobs) { ix->cmin = ix->keys.front(); ix->cmax = ix->keys.back(); }
+    } else {
+        for (uint32_t b = 0; b < nobs; ++b) {
+            if (ix->null_bitmap(b)) continue;
+            ix->cmin = std::min(ix->cmin, ix->minval[b]);
+            ix->cmax = std::max(ix->cmax, ix->maxval[b]);
+        }
+    }
+    return finish_index(ix);
+}
+
+// Bytes of an index file, read through a private file mapping.
+struct FileMap {
+    const uint8_t* p = nullptr;
+    size_t n = 0;
+    int open(const char* path) {
+        const int fd = ::open(path, O_RDONLY);
+        if (fd < 0) return fail(FBH_ERR_ARG, std::string("cannot open ") + path);
+        struct stat st;
+        if (fstat(fd, &st) != 0 || st.st_size < 0) { ::close(fd); return fail(FBH_ERR_ARG, "cannot stat the file"); }
+        n = (size_t)st.st_size;
+        if (n) {
+            void* m = mmap(nullptr, n, PROT_READ, MAP_PRIVATE, fd, 0);
+            if (m == MAP_FAILED) { ::close(fd); n = 0; return fail(FBH_ERR_ARG, "cannot map the file"); }
+            p = (const uint8_t*)m;
+        }
+        ::close(fd);
+        return FBH_OK;
+    }
+    ~FileMap() { if (p) munmap((void*)p, n); }
+    template <class T>
+    T at(size_t pos) const { T v; memcpy(&v, p + pos, sizeof(T)); return v; }
+};
+
 } // namespace
 
 extern "C" {
@@ -753,6 +900,135 @@ int fbh_index_build_bin_nbins(const void* col, int type, uint32_t nrows, uint32_
     return fbh_index_build_bin(col, type, nrows, bounds.data(), (uint32_t)bounds.size(), out);
 }
 
+int fbh_index_attach(const void* col, int type, uint32_t nrows, const double* keys, uint64_t nkeys,
+                     const int64_t* offsets, uint64_t noffsets, const uint32_t* bms, uint64_t nbms,
+                     fbh_index** out) {
+    // the argument checks of fastbit_iapi_attach_full_index
+    // (src/iapi.cpp:2037-2044)
+    if (!out || (nrows && !col) || !keys || nkeys == 0 || !offsets || noffsets == 0 || !bms || nbms == 0)
+        return fail(FBH_ERR_ARG, "bad argument");
+    if ((uint64_t)offsets[noffsets - 1] > nbms) return fail(FBH_ERR_ARG, "bitmap offsets beyond the words");
+    // column::attachIndex (src/column.cpp:5637-5677): bin when there are
+    // two keys per bitmap (minima ++ maxima), relic when there is one
+    int kind;
+    if (nkeys == 2 * (noffsets - 1)) kind = FBH_BIN;
+    else if (nkeys + 1 == noffsets) kind = FBH_RELIC;
+    else return fail(FBH_ERR_ARG, "mismatching nkeys and noffsets");
+    if (noffsets - 1 > 0xFFFFFFFFull) return fail(FBH_ERR_SIZE, "too many bitmaps");
+    const uint32_t nobs = (uint32_t)(noffsets - 1);
+    int rc = ensure_init();
+    if (rc) return rc;
+    fbh_index* ix = new fbh_index();
+    ix->kind = kind;
+    if ((rc = upload_column(ix, col, type, nrows))) { delete ix; return rc; }
+    std::vector<double> bounds;
+    if (kind == FBH_BIN) {
+        // bin(col, nb, keys, offs, ...) (src/ibin.cpp:358-373): the
+        // boundary between two bins is the shortest value between the
+        // maximum of one and the minimum of the next; the last is DBL_MAX
+        bounds.resize(nobs);
+        for (uint32_t j = 0; j + 1 < nobs; ++j) bounds[j] = fbh_ix::compact_value(keys[nobs + j], keys[j + 1]);
+        bounds[nobs - 1] = DBL_MAX;
+        rc = attach_index(ix, nullptr, bounds.data(), keys, keys + nobs, nobs, offsets, bms, nbms);
+    } else {
+        rc = attach_index(ix, keys, nullptr, nullptr, nullptr, nobs, offsets, bms, nbms);
+    }
+    if (rc) { delete ix; return rc; }
+    *out = ix;
+    return FBH_OK;
+}
+
+int fbh_index_read_file(const char* path, const void* col, int type, uint32_t nrows, fbh_index** out) {
+    if (!path || !out || (nrows && !col)) return fail(FBH_ERR_ARG, "null argument");
+    FileMap f;
+    int rc = f.open(path);
+    if (rc) return rc;
+    // 8-byte header: "#IBIS", index type, offset width, 0
+    // (src/irelic.cpp:704-712, src/ibin.cpp:479-487)
+    if (f.n < 16 || memcmp(f.p, "#IBIS", 5) != 0 || (f.p[6] != 4 && f.p[6] != 8) || f.p[7] != 0)
+        return fail(FBH_ERR_ARG, std::string(path) + " is not an index file");
+    const int itype = f.p[5];
+    const size_t w = f.p[6];
+    constexpr int BINNING = 0, RELIC = 7;   // ibis::index::INDEX_TYPE (src/index.h:91-105)
+    const uint32_t frows = f.at<uint32_t>(8), nobs = f.at<uint32_t>(12);
+    size_t obeg, oend, kend;
+    std::vector<double> vals, bounds, minv, maxv;
+    auto doubles = [&](size_t pos, std::vector<double>& v) {
+        v.resize(nobs);
+        if (nobs) memcpy(v.data(), f.p + pos, 8ull * nobs);
+    };
+    if (itype == RELIC) {
+        // relic::read (src/irelic.cpp:727-749): nrows, nobs, card; the
+        // values at byte 24; the nobs+1 bitmap positions after them
+        if (f.n < 20) return fail(FBH_ERR_ARG, "truncated index file");
+        if (f.at<uint32_t>(16) != nobs) return fail(FBH_ERR_ARG, "relic file with card != nobs");
+        obeg = 24 + 8ull * nobs;
+        oend = obeg + w * (nobs + 1ull);
+        kend = oend;
+        if (kend > f.n) return fail(FBH_ERR_ARG, "truncated index file");
+        doubles(24, vals);
+    } else if (itype == BINNING) {
+        // bin::read (src/ibin.cpp:501-540): nrows, nobs, the nobs+1 bitmap
+        // positions, then at the next 8-byte boundary bounds, maxval, minval
+        obeg = 16;
+        oend = obeg + w * (nobs + 1ull);
+        const size_t kbeg = 8 * ((7 + oend) / 8);
+        kend = kbeg + 24ull * nobs;
+        if (kend > f.n) return fail(FBH_ERR_ARG, "truncated index file");
+        doubles(kbeg, bounds);
+        doubles(kbeg + 8ull * nobs, maxv);
+        doubles(kbeg + 16ull * nobs, minv);
+    } else {
+        return fail(FBH_ERR_ARG, "index type " + std::to_string(itype) + " is not a relic or a bin index");
+    }
+    if (nobs == 0) return fail(FBH_ERR_ARG, "index file without bitmaps");
+    if (frows != nrows)
+        return fail(FBH_ERR_ARG, "the file indexes " + std::to_string(frows) + " rows, the column has " +
+                                     std::to_string(nrows));
+    // bitmap positions are byte offsets in the file; the in-memory form
+    // counts words from the first bitmap
+    std::vector<int64_t> pos(nobs + 1ull);
+    for (uint32_t b = 0; b <= nobs; ++b)
+        pos[b] = w == 8 ? f.at<int64_t>(obeg + 8ull * b) : (int64_t)f.at<int32_t>(obeg + 4ull * b);
+    if (pos[0] < (int64_t)kend || pos[0] % 4 != 0 || (uint64_t)pos[nobs] > f.n)
+        return fail(FBH_ERR_ARG, "bitmap positions outside the file");
+    std::vector<int64_t> woff(nobs + 1ull);
+    for (uint32_t b = 0; b <= nobs; ++b) {
+        if (pos[b] < pos[0] || (pos[b] - pos[0]) % 4 != 0) return fail(FBH_ERR_ARG, "bad bitmap position");
+        woff[b] = (pos[b] - pos[0]) / 4;
+    }
+    if ((rc = ensure_init())) return rc;
+    fbh_index* ix = new fbh_index();
+    ix->kind = itype == RELIC ? FBH_RELIC : FBH_BIN;
+    if ((rc = upload_column(ix, col, type, nrows))) { delete ix; return rc; }
+    const uint32_t* bms = (const uint32_t*)(f.p + pos[0]);
+    rc = attach_index(ix, vals.data(), bounds.data(), minv.data(), maxv.data(), nobs, woff.data(), bms,
+                      (uint64_t)woff[nobs]);
+    if (rc) { delete ix; return rc; }
+    *out = ix;
+    return FBH_OK;
+}
+
+int fbh_index_sum_bins(fbh_index* ix, uint32_t ib, uint32_t ie, uint64_t* hits) {
+    if (!ix || !hits) return fail(FBH_ERR_ARG, "null argument");
+    int rc = ensure_init();
+    if (rc) return rc;
+    DBuf<uint32_t> lit;
+    PartCounts pc;
+    if ((rc = lit.alloc(ix->groups + 1)) || (rc = pc.alloc(ix, 1))) return rc;
+    fbk::QueryArgs q;
+    fill_query(ix, q, 0.0, FBH_OP_NONE, 0.0, FBH_OP_NONE);
+    q.out = lit.p;
+    q.part = pc.row(0);
+    set_sum_ranges(ix, q, ib, ie);
+    uint32_t np = ix->ntiles, c = 0;
+    rc = check(fbk::launch_query_lit(q, dev_type(ix->type), stream()), "sum bins");
+    if (!rc) rc = pc.sum(&np, 1, &c);
+    if (rc) return rc;
+    *hits = c;
+    return FBH_OK;
+}
+
 int fbh_index_bounds(const fbh_index* ix, double* bounds, uint32_t cap, uint32_t* n) {
     if (!ix || !n) return fail(FBH_ERR_ARG, "null argument");
     if (ix->kind != FBH_BIN) return fail(FBH_ERR_ARG, "not a bin index");
@@ -796,7 +1072,7 @@ int fbh_index_resolve_range(const fbh_index* ix, double lo, int lop, double hi, 
                             uint32_t* cand0, uint32_t* hit0, uint32_t* hit1, uint32_t* cand1) {
     if (!ix) return fail(FBH_ERR_ARG, "null index");
     Locate L;
-    locate(ix, lo, lop, hi, rop, L);
+    locate_raw(ix, lo, lop, hi, rop, L);
     if (cand0) *cand0 = L.cand0;
     if (hit0) *hit0 = L.hit0;
     if (hit1) *hit1 = L.hit1;
@@ -1045,7 +1321,7 @@ int fbh_index_time_plans(fbh_index* ix, uint32_t nq, const double* lo, const int
         // untimed warm-up of the plan's kernels
         for (uint32_t i = 0; i < std::min(nq, 4u) && !rc; ++i)
             rc = index_eval_literal(ix, lo[i], lop[i], hi[i], rop[i], plan, lit.p, pc.row(0), &np[0], stream());
-        if (!rc) rc = sync();
+        if (!rc) rc = stream_sync();
         for (uint32_t c0 = 0; c0 < nq && !rc; c0 += CH) {
             const uint32_t k = std::min(CH, nq - c0);
             rc = check(hipEventRecord(ev[0], stream()), "event");
@@ -1055,7 +1331,7 @@ int fbh_index_time_plans(fbh_index* ix, uint32_t nq, const double* lo, const int
                 if (!rc) rc = check(hipEventRecord(ev[i + 1], stream()), "event");
             }
             if (!rc) rc = pc.launch_sum(np.data(), k);
-            if (!rc) rc = sync();
+            if (!rc) rc = stream_sync();
             if (!rc) rc = to_host(cnt.data(), pc.counts.p, k);
             for (uint32_t i = 0; i < k && !rc; ++i) {
                 float t = 0.f;
@@ -1076,7 +1352,7 @@ int fbh_index_time_plans(fbh_index* ix, uint32_t nq, const double* lo, const int
     for (int k = 0; k < 16 && !rc; ++k) {
         const Clock::time_point t0 = Clock::now();
         rc = check(fbk::launch_zero_u32(lit.p, 1, stream()), "zero");
-        if (!rc) rc = sync();
+        if (!rc) rc = stream_sync();
         rt = std::min(rt, std::chrono::duration<double, std::milli>(Clock::now() - t0).count());
     }
     for (uint32_t i = 0; i < nq && !rc; ++i) {
@@ -1087,7 +1363,7 @@ int fbh_index_time_plans(fbh_index* ix, uint32_t nq, const double* lo, const int
             hits[3 * (size_t)i + 1] = UINT64_MAX;
             continue;
         }
-        if ((rc = sync())) break;
+        if ((rc = stream_sync())) break;
         fbh_bitvector* res = nullptr;
         if ((rc = fbh_bv_new(&res))) break;
         const Clock::time_point t0 = Clock::now();
@@ -1096,7 +1372,7 @@ int fbh_index_time_plans(fbh_index* ix, uint32_t nq, const double* lo, const int
         rc = eval_compressed(ix, lo[i], lop[i], hi[i], rop[i], res, &dl);
         uint32_t c = 0;
         if (rc == CENSORED) {
-            rc = sync();
+            rc = stream_sync();
             ms[3 * (size_t)i + 1] = (float)-limit;
             hits[3 * (size_t)i + 1] = UINT64_MAX;
         } else if (!rc && !(rc = fbh_bv_cnt(res, &c))) {
@@ -1255,7 +1531,7 @@ int fbh_conj_prepare(const fbh_term* terms, const uint32_t* qoff, uint32_t nq, i
     rc = to_dev(c->dterms, h.data(), nterms);
     if (!rc) rc = to_dev(c->dqoff, qoff, nq + 1);
     if (!rc) rc = to_dev(c->dnp, np.data(), nq);
-    if (!rc) rc = sync();
+    if (!rc) rc = stream_sync();
     c->blocks = 4u * (uint32_t)cu_count();
     if (!rc) rc = conj_capture(c);
     if (rc) { delete c; return rc; }

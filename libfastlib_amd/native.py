@@ -267,6 +267,10 @@ def _index_sigs(L):
         "fbh_index_set_cost_model": [vp, C.c_int],
         "fbh_index_build_bin_nbins": [vp, C.c_int, C.c_uint32, C.c_uint32, C.POINTER(vp)],
         "fbh_index_bounds": [vp, _dp, C.c_uint32, _u32p],
+        "fbh_index_attach": [vp, C.c_int, C.c_uint32, _dp, C.c_uint64, _i64p, C.c_uint64, _u32p,
+                             C.c_uint64, C.POINTER(vp)],
+        "fbh_index_read_file": [C.c_char_p, vp, C.c_int, C.c_uint32, C.POINTER(vp)],
+        "fbh_index_sum_bins": [vp, C.c_uint32, C.c_uint32, _u64p],
         "fbh_index_plan_features": [vp, C.c_uint32, _dp, _ip, _dp, _ip, _fp],
         "fbh_index_time_plans": [vp, C.c_uint32, _dp, _ip, _dp, _ip, C.c_float, _fp, _u64p],
         "fbh_index_choose_plans": [vp, vp, C.c_uint32, _dp, _ip, _dp, _ip, _ip, _u32p],
@@ -346,6 +350,37 @@ class Index:
         check(L.fbh_index_build_bin_nbins(a.ctypes.data_as(C.c_void_p), _COLT[a.dtype], a.size, int(nbins),
                                           C.byref(h)), "fbh_index_build_bin_nbins")
         return cls(h)
+
+    @classmethod
+    def attach(cls, col, keys, offsets, bms):
+        """fastbit_iapi_attach_full_index: an index given as its keys, word
+        offsets and serialized bitmaps, over the column it indexes."""
+        L = cls._lib()
+        a = np.ascontiguousarray(col)
+        k = np.ascontiguousarray(keys, np.float64)
+        o = np.ascontiguousarray(offsets, np.int64)
+        b = np.ascontiguousarray(bms, np.uint32)
+        h = C.c_void_p()
+        check(L.fbh_index_attach(a.ctypes.data_as(C.c_void_p), _COLT[a.dtype], a.size, k.ctypes.data_as(_dp),
+                                 k.size, o.ctypes.data_as(_i64p), o.size, b.ctypes.data_as(_u32p), b.size,
+                                 C.byref(h)), "fbh_index_attach")
+        return cls(h)
+
+    @classmethod
+    def read_file(cls, path, col):
+        """relic::read / bin::read of an index file, over its column."""
+        L = cls._lib()
+        a = np.ascontiguousarray(col)
+        h = C.c_void_p()
+        check(L.fbh_index_read_file(os.fsencode(path), a.ctypes.data_as(C.c_void_p), _COLT[a.dtype], a.size,
+                                    C.byref(h)), "fbh_index_read_file")
+        return cls(h)
+
+    def sum_bins(self, ib, ie):
+        """fastbit_iapi_get_number_of_hits: ones of sumBins(ib, ie)."""
+        n = C.c_uint64()
+        check(self._lib().fbh_index_sum_bins(self._h, int(ib), int(ie), C.byref(n)), "fbh_index_sum_bins")
+        return n.value
 
     def bounds(self):
         L = self._lib()

@@ -459,3 +459,62 @@ class RefAttached:
         if n < 0:
             raise RuntimeError(f"ref_iapi_query failed {n}")
         return int(n)
+
+
+# ---------------------------------------------------------------------------
+# Index files written and read by the reference (oracle/refdrv.cpp).
+def ref_write_index_file(name, path):
+    """relic::write / bin::write of the index the reference built for the
+    registered array `name` (see RefIapi)."""
+    rc = reference().lib.ref_iapi_write_index_file(name.encode(), os.fsencode(path))
+    if rc < 0:
+        raise RuntimeError(f"ref_iapi_write_index_file failed rc={rc}")
+
+
+def ref_relic_card_file(ind, card, path):
+    """relic(0, card, ind) written to an index file."""
+    a = np.ascontiguousarray(ind, np.uint32)
+    rc = reference().lib.ref_relic_card_file(a.ctypes.data_as(_u32p), C.c_size_t(a.size), C.c_uint32(card),
+                                             os.fsencode(path))
+    if rc < 0:
+        raise RuntimeError(f"ref_relic_card_file failed rc={rc}")
+
+
+def ref_file_triple(path):
+    """(kind, keys, offsets, bitmaps) of an index file as the reference reads
+    it; kind is "relic" or "bin"."""
+    L = reference().lib
+    kind = C.c_int()
+    k, o, b = _dp(), _i64p(), _u32p()
+    nk, no, nb = C.c_uint64(), C.c_uint64(), C.c_uint64()
+    rc = L.ref_file_triple(os.fsencode(path), C.byref(kind), C.byref(k), C.byref(nk), C.byref(o), C.byref(no),
+                           C.byref(b), C.byref(nb))
+    if rc < 0:
+        raise RuntimeError(f"ref_file_triple failed rc={rc}")
+    out = ("relic" if kind.value == 1 else "bin",
+           np.ctypeslib.as_array(k, (nk.value,)).copy() if nk.value else np.zeros(0),
+           np.ctypeslib.as_array(o, (no.value,)).copy(),
+           np.ctypeslib.as_array(b, (nb.value,)).copy() if nb.value else np.zeros(0, np.uint32))
+    for p in (k, o, b):
+        L.ref_free(p)
+    return out
+
+
+def ref_file_locate(path, lo, lop, hi, rop):
+    """(cand0, hit0, hit1, cand1) of lo OP1 x OP2 hi on an index file."""
+    out = np.zeros(4, np.uint32)
+    rc = reference().lib.ref_file_locate(os.fsencode(path), C.c_double(lo), C.c_int(lop), C.c_double(hi),
+                                         C.c_int(rop), out.ctypes.data_as(_u32p))
+    if rc < 0:
+        raise RuntimeError(f"ref_file_locate failed rc={rc}")
+    return tuple(int(x) for x in out)
+
+
+def ref_file_sum_bins(path, ib, ie):
+    """Ones of index::sumBins(ib, ie) on an index file."""
+    L = reference().lib
+    L.ref_file_sum_bins.restype = C.c_int64
+    n = L.ref_file_sum_bins(os.fsencode(path), C.c_uint32(ib), C.c_uint32(ie))
+    if n < 0:
+        raise RuntimeError(f"ref_file_sum_bins failed {n}")
+    return int(n)

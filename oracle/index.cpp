@@ -166,6 +166,12 @@ static uint32_t relic_locate(const std::vector<double>& vals, double val) {
 // Bytes of a bitmap as index::sumBins counts them (bitvector::bytes()).
 static double bm_bytes(const BV* b) { return b ? 4.0 * b->v.size() + 40.0 : 0.0; }
 
+// bitvector::getSerialSize() in bytes (src/bitvector.h:150-154)
+static double serial_bytes(const Index& ix, uint32_t b) {
+    if (!ix.bits[b]) return 0.0;
+    return 4.0 * (double)(ix.bits[b]->v.size() + 1 + (ix.nrows % 31u ? 1u : 0u));
+}
+
 // index::sumBins(ib, ie, res) -- src/index.cpp:6450-7019, by its result:
 // OR of bits[ib, ie), or the complement of the OR of the rest.
 static BV sum_bins(const Index& ix, uint32_t ib, uint32_t ie) {
@@ -173,10 +179,13 @@ static BV sum_bins(const Index& ix, uint32_t ib, uint32_t ie) {
     BV res;
     if (ie > nobs) ie = nobs;
     if (ib >= ie) { res.set(0, ix.nrows); return res; }
+    // the measure is offset64 (src/index.cpp:6463-6477): the serialized
+    // sizes a constructor or a file put there, else bitvector::bytes()
     double all = 0, mid = 0;
     for (uint32_t i = 0; i < nobs; ++i) {
-        all += bm_bytes(ix.bits[i]);
-        if (i >= ib && i < ie) mid += bm_bytes(ix.bits[i]);
+        const double m = ix.offsets_built ? serial_bytes(ix, i) : bm_bytes(ix.bits[i]);
+        all += m;
+        if (i >= ib && i < ie) mid += m;
     }
     if (mid == 0) { res.set(0, ix.nrows); return res; }
     if (all == mid) { res.set(1, ix.nrows); return res; }
@@ -345,12 +354,6 @@ BV scan(const void* col, int type, uint32_t n, double lo, int lop, double hi, in
 
 static uint32_t elem_size(int type) {
     return type == COL_U16 ? 2u : type == COL_F64 ? 8u : 4u;
-}
-
-// bitvector::getSerialSize() in bytes (src/bitvector.h:150-154)
-static double serial_bytes(const Index& ix, uint32_t b) {
-    if (!ix.bits[b]) return 0.0;
-    return 4.0 * (double)(ix.bits[b]->v.size() + 1 + (ix.nrows % 31u ? 1u : 0u));
 }
 
 double estimate_cost(const Index& ix, int type, double lo, int lop, double hi, int rop, int model) {

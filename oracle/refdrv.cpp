@@ -263,3 +263,167 @@ extern "C" int ref_iapi_bin_bounds(const char* name, double** out, uint64_t* n) 
     *out = dupv(b.data(), b.size());
     return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Index files.  An index with no column attached writes to a path that ends
+// in ".idx" as given (index::indexFileName, src/index.cpp:1481-1487), so the
+// index built for a registered array is copied into an object with col == 0
+// and written from there.
+namespace {
+struct FileRelic : ibis::relic {
+    explicit FileRelic(const ibis::relic& r) : ibis::relic(r) { col = 0; }
+};
+struct FileBin : ibis::bin {
+    explicit FileBin(const ibis::bin& b) : ibis::bin(b) { col = 0; }
+};
+
+// An index read from a file whose bitmaps are loaded as the file holds
+// them: a zero-length range stays a null bitmap (index::activate would make
+// it an all-zero one, src/index.cpp:4984-4987).
+template <class B>
+struct Opened : B {
+    explicit Opened(const char* path) : B(0, path) {}
+    void load_present() {
+        const size_t n = this->bits.size();
+        for (size_t i = 0; i < n; ++i) {
+            int64_t a = 0, e = 0;
+            if (this->offset64.size() > n) { a = this->offset64[i]; e = this->offset64[i + 1]; }
+            else if (this->offset32.size() > n) { a = this->offset32[i]; e = this->offset32[i + 1]; }
+            if (e > a) this->activate((uint32_t)i);
+        }
+    }
+};
+
+ibis::qExpr::COMPARE cmp_op(int o) {
+    return o == 1 ? ibis::qExpr::OP_LT : o == 2 ? ibis::qExpr::OP_LE : ibis::qExpr::OP_UNDEFINED;
+}
+
+// The index file at path, read by relic::read or bin::read by its header.
+ibis::index* read_file(const char* path) {
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return 0;
+    char h[8];
+    const size_t got = std::fread(h, 1, 8, f);
+    std::fclose(f);
+    if (got != 8) return 0;
+    if (h[5] == (char)ibis::index::RELIC) return new ibis::relic(0, path);
+    if (h[5] == (char)ibis::index::BINNING) return new ibis::bin(0, path);
+    return 0;
+}
+} // namespace
+
+extern "C" {
+
+// relic::write / bin::write (src/irelic.cpp:354-456, src/ibin.cpp:5530-5654)
+// of the index built for the registered array `name`.
+int ref_iapi_write_index_file(const char* name, const char* path) {
+    ensure_init();
+    ibis::bord::column* c = __fastbit_iapi_array_by_name(name);
+    if (c == 0) return -1;
+    ibis::column::indexLock lock(c, "ref_iapi_write_index_file");
+    const ibis::index* ix = lock.getIndex();
+    if (ix == 0) return -2;
+    try {
+        if (ix->type() == ibis::index::RELIC) {
+            FileRelic r(*static_cast<const ibis::relic*>(ix));
+            return r.write(path);
+        }
+        if (ix->type() == ibis::index::BINNING) {
+            FileBin b(*static_cast<const ibis::bin*>(ix));
+            return b.write(path);
+        }
+    } catch (...) {
+        return -4;
+    }
+    return -3;
+}
+
+// relic(0, card, ind) (src/irelic.cpp:160-211) written to an index file.
+int ref_relic_card_file(const uint32_t* ind, size_t n, uint32_t card, const char* path) {
+    ensure_init();
+    try {
+        ibis::array_t<uint32_t> arr(n);
+        if (n) std::memcpy(arr.begin(), ind, 4 * n);
+        ibis::relic rl(0, card, arr);
+        return rl.write(path);
+    } catch (...) {
+        return -2;
+    }
+}
+
+// The in-memory form (relic::write / bin::write into arrays,
+// src/irelic.cpp:644-667, src/ibin.cpp:5836-5860) of an index file read by
+// the reference; *kind is 0 for bin, 1 for relic.
+int ref_file_triple(const char* path, int* kind, double** keys, uint64_t* nk, int64_t** offs,
+                    uint64_t* no, uint32_t** bms, uint64_t* nb) {
+    ensure_init();
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return -1;
+    char h[8];
+    const size_t got = std::fread(h, 1, 8, f);
+    std::fclose(f);
+    if (got != 8) return -1;
+    ibis::array_t<double> k;
+    ibis::array_t<int64_t> o;
+    ibis::array_t<uint32_t> b;
+    int rc = -1;
+    try {
+        if (h[5] == (char)ibis::index::RELIC) {
+            Opened<ibis::relic> ix(path);
+            ix.load_present();
+            rc = ix.write(k, o, b);
+            *kind = 1;
+        } else if (h[5] == (char)ibis::index::BINNING) {
+            Opened<ibis::bin> ix(path);
+            ix.load_present();
+            rc = ix.write(k, o, b);
+            *kind = 0;
+        }
+    } catch (...) {
+        rc = -2;
+    }
+    if (rc < 0) return rc;
+    *keys = dupv(k.begin(), k.size()); *nk = k.size();
+    *offs = dupv(o.begin(), o.size()); *no = o.size();
+    *bms = dupv(b.begin(), b.size()); *nb = b.size();
+    return 0;
+}
+
+// relic::locate / bin::locate of lo OP1 x OP2 hi on an index file read by
+// the reference: out = cand0, hit0, hit1, cand1 (fastbit_iapi_resolve_range's
+// order, src/iapi.cpp:1981-2009).
+int ref_file_locate(const char* path, double lo, int lop, double hi, int rop, uint32_t* out) {
+    ensure_init();
+    ibis::index* ix = read_file(path);
+    if (ix == 0) return -1;
+    ibis::qContinuousRange cr(lo, cmp_op(lop), "_", cmp_op(rop), hi);
+    uint32_t c0 = 0, h0 = 0, h1 = 0, c1 = 0;
+    if (ix->type() == ibis::index::RELIC) {
+        static_cast<const ibis::relic*>(ix)->locate(cr, h0, h1);
+        c0 = h0; c1 = h1;
+    } else {
+        static_cast<const ibis::bin*>(ix)->locate(cr, c0, c1, h0, h1);
+    }
+    out[0] = c0; out[1] = h0; out[2] = h1; out[3] = c1;
+    delete ix;
+    return 0;
+}
+
+// index::sumBins(ib, ie) (src/index.cpp:6450-7019) of an index file read by
+// the reference: the number of ones.
+int64_t ref_file_sum_bins(const char* path, uint32_t ib, uint32_t ie) {
+    ensure_init();
+    ibis::index* ix = read_file(path);
+    if (ix == 0) return -1;
+    int64_t cnt = -2;
+    try {
+        ibis::bitvector res;
+        ix->sumBins(ib, ie, res);
+        cnt = res.cnt();
+    } catch (...) {
+    }
+    delete ix;
+    return cnt;
+}
+
+} // extern "C"
