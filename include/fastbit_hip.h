@@ -293,6 +293,28 @@ void fbh_conj_free(fbh_conj* c);
 /* One conjunction: *hits receives the count. */
 int fbh_conj_evaluate(const fbh_term* terms, uint32_t nterms, int plan, uint64_t* hits);
 
+/* Boolean expressions over range terms: countQuery::doEvaluate over the
+ * LOGICAL_AND / OR / XOR / NOT / MINUS tree (src/countQuery.cpp:944-1003)
+ * that fastbit_selection_combine builds (src/iapi.cpp:1285-1339; NAND and
+ * NOR are NOT over AND and OR there).  Query q is the postfix program
+ * prog[poff[q] .. poff[q+1]) over its terms terms[toff[q] .. toff[q+1]):
+ * an item i >= 0 pushes the rows of term toff[q] + i, an FBH_EXPR_* item
+ * pops two results (NOT: one) and pushes the combination.  At most
+ * FBH_EXPR_DEPTH results are on the stack; a program must leave one.
+ * NOT complements within the column's rows (every row is valid for arrays
+ * registered in memory).  The returned batch runs, tunes and frees like a
+ * conjunction batch. */
+#define FBH_EXPR_DEPTH 4
+#define FBH_EXPR_AND (-1)
+#define FBH_EXPR_OR (-2)
+#define FBH_EXPR_XOR (-3)
+#define FBH_EXPR_NAND (-4)
+#define FBH_EXPR_NOR (-5)
+#define FBH_EXPR_NOT (-6)
+#define FBH_EXPR_MINUS (-7)
+int fbh_expr_prepare(const fbh_term* terms, const uint32_t* toff, const int32_t* prog,
+                     const uint32_t* poff, uint32_t nq, int plan, fbh_conj** out);
+
 /* ---------------------------------------------------------------------- */
 /* Learned plan choice.  The reference picks a plan with fixed cost rules
  * (column::evaluateRange, src/column.cpp:6102-6111; bin::sumBins,

@@ -1396,6 +1396,8 @@ struct fbh_conj {
     uint32_t* dnp = nullptr;             // ntiles per query (for the sums)
     uint32_t* parts = nullptr;           // nq * ntiles
     uint32_t* counts = nullptr;          // nq
+    int32_t* dprog = nullptr;            // postfix programs (expression batches)
+    uint32_t* dpoff = nullptr;
     std::vector<int> plans;              // plan of every term (caller's order)
     std::vector<uint32_t> order;         // evaluation order: order[k] = caller's index
     int nt = 256;                        // threads per block
@@ -1407,6 +1409,7 @@ struct fbh_conj {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         dev_free(dterms); dev_free(dqoff); dev_free(dnp); dev_free(parts); dev_free(counts);
+        dev_free(dprog); dev_free(dpoff);
     }
 };
 
@@ -1422,10 +1425,57 @@ int cu_count() {
     return n;
 }
 
+// Device descriptor of one range term under the given plan.
+void term_args(const fbh_term& t, int p, fbk::QueryArgs& q) {
+    fill_query(t.ix, q, t.lo, t.lop, t.hi, t.rop);
+    q.type = dev_type(t.ix->type);
+    q.plan = p == FBH_PLAN_SCAN ? fbk::PLAN_SCAN_D : fbk::PLAN_LIT_D;
+    Locate L;
+    locate(t.ix, t.lo, t.lop, t.hi, t.rop, L);
+    set_sum_ranges(t.ix, q, L.hit0, L.hit1);
+    if (t.ix->kind == FBH_BIN) {
+        if (L.cand0 < L.hit0 && !t.ix->null_bitmap(L.cand0)) q.cand0 = L.cand0;
+        if (L.cand1 > L.hit1 && L.hit1 < t.ix->nobs && !t.ix->null_bitmap(L.hit1)) q.cand1 = L.hit1;
+    }
+}
+
+int conj_launch(fbh_conj* c, hipStream_t s);
+int conj_capture(fbh_conj* c);
+
+// Term descriptors, query offsets and programs to the device, then the
+// launch sequence captured.
+int conj_upload(fbh_conj* c, const std::vector<fbk::QueryArgs>& h, const uint32_t* qoff, const int32_t* prog,
+                const uint32_t* poff, uint32_t nprog) {
+    const uint32_t nq = c->nq;
+    std::vector<uint32_t> np(nq, c->ntiles);
+    c->dterms = (fbk::QueryArgs*)dev_alloc(sizeof(fbk::QueryArgs) * c->nterms);
+    c->dqoff = (uint32_t*)dev_alloc(4ull * (nq + 1));
+    c->dnp = (uint32_t*)dev_alloc(4ull * nq);
+    c->parts = (uint32_t*)dev_alloc(4ull * nq * c->ntiles);
+    c->counts = (uint32_t*)dev_alloc(4ull * nq);
+    if (!c->dterms || !c->dqoff || !c->dnp || !c->parts || !c->counts)
+        return fail(FBH_ERR_NOMEM, "device allocation failed");
+    if (prog) {
+        c->dprog = (int32_t*)dev_alloc(4ull * nprog);
+        c->dpoff = (uint32_t*)dev_alloc(4ull * (nq + 1));
+        if (!c->dprog || !c->dpoff) return fail(FBH_ERR_NOMEM, "device allocation failed");
+    }
+    int rc = to_dev(c->dterms, h.data(), c->nterms);
+    if (!rc) rc = to_dev(c->dqoff, qoff, nq + 1);
+    if (!rc) rc = to_dev(c->dnp, np.data(), nq);
+    if (!rc && prog) rc = to_dev(c->dprog, prog, nprog);
+    if (!rc && prog) rc = to_dev(c->dpoff, poff, nq + 1);
+    if (!rc) rc = stream_sync();
+    c->blocks = 4u * (uint32_t)cu_count();
+    if (!rc) rc = conj_capture(c);
+    return rc;
+}
+
 int conj_launch(fbh_conj* c, hipStream_t s) {
     fbk::ConjArgs a;
     a.terms = c->dterms; a.qoff = c->dqoff; a.nq = c->nq; a.ntiles = c->ntiles;
     a.groups = c->groups; a.anb = c->anb; a.tpi = c->tpi; a.part = c->parts; a.out = nullptr;
+    a.prog = c->dprog; a.poff = c->dpoff;
     int rc = check(fbk::launch_conj(a, c->nt, c->blocks, s), "conjunctions");
     if (!rc) rc = check(fbk::launch_sum_parts(c->parts, c->ntiles, c->dnp, c->nq, c->counts, s), "sum parts");
     return rc;
@@ -1499,41 +1549,67 @@ int fbh_conj_prepare(const fbh_term* terms, const uint32_t* qoff, uint32_t nq, i
         for (uint32_t i = 0; i < (uint32_t)w.size(); ++i) c->order[qoff[q] + i] = w[i].second;
     }
     std::vector<fbk::QueryArgs> h(nterms);
+    for (uint32_t k = 0; k < nterms; ++k) term_args(terms[c->order[k]], c->plans[c->order[k]], h[k]);
+    rc = conj_upload(c, h, qoff, nullptr, nullptr, 0);
+    if (rc) { delete c; return rc; }
+    *out = c;
+    return FBH_OK;
+}
+
+int fbh_expr_prepare(const fbh_term* terms, const uint32_t* toff, const int32_t* prog, const uint32_t* poff,
+                     uint32_t nq, int plan, fbh_conj** out) {
+    if (!terms || !toff || !prog || !poff || !out || nq == 0) return fail(FBH_ERR_ARG, "bad argument");
+    if (plan < FBH_PLAN_AUTO || plan > FBH_PLAN_SCAN) return fail(FBH_ERR_ARG, "bad plan");
+    int rc = ensure_init();
+    if (rc) return rc;
+    const uint32_t nterms = toff[nq];
+    if (nterms == 0) return fail(FBH_ERR_ARG, "no terms");
+    const fbh_index* ix0 = terms[0].ix;
+    if (!ix0) return fail(FBH_ERR_ARG, "null index");
     for (uint32_t k = 0; k < nterms; ++k) {
-        const fbh_term& t = terms[c->order[k]];
-        fbk::QueryArgs& q = h[k];
-        fill_query(t.ix, q, t.lo, t.lop, t.hi, t.rop);
-        const int p = c->plans[c->order[k]];
-        q.type = dev_type(t.ix->type);
-        q.plan = p == FBH_PLAN_SCAN ? fbk::PLAN_SCAN_D : fbk::PLAN_LIT_D;
-        Locate L;
-        locate(t.ix, t.lo, t.lop, t.hi, t.rop, L);
-        const SumPlan sp = sum_plan(t.ix, L.hit0, L.hit1);
-        q.r0 = q.r1 = q.r2 = q.r3 = 0;
-        if (sp.kind == 1) { q.flip = 1; }
-        else if (sp.kind == 2) { q.r0 = L.hit0; q.r1 = L.hit1; }
-        else if (sp.kind == 3) { q.r0 = 0; q.r1 = L.hit0; q.r2 = L.hit1; q.r3 = t.ix->nobs; q.flip = 1; }
-        if (t.ix->kind == FBH_BIN) {
-            if (L.cand0 < L.hit0 && !t.ix->null_bitmap(L.cand0)) q.cand0 = L.cand0;
-            if (L.cand1 > L.hit1 && L.hit1 < t.ix->nobs && !t.ix->null_bitmap(L.hit1)) q.cand1 = L.hit1;
+        const fbh_term& t = terms[k];
+        if (!t.ix || t.ix->nrows != ix0->nrows) return fail(FBH_ERR_ARG, "terms over different row counts");
+        if (t.lop < 0 || t.lop > 2 || t.rop < 0 || t.rop > 2) return fail(FBH_ERR_ARG, "bad operator");
+    }
+    // every program leaves exactly one result and stays within the stack;
+    // its term items refer to the query's own terms
+    std::vector<int32_t> aprog(poff[nq]);
+    for (uint32_t q = 0; q < nq; ++q) {
+        if (toff[q + 1] < toff[q] || poff[q + 1] <= poff[q]) return fail(FBH_ERR_ARG, "bad offsets");
+        int sp = 0;
+        for (uint32_t p = poff[q]; p < poff[q + 1]; ++p) {
+            const int32_t op = prog[p];
+            if (op >= 0) {
+                if ((uint32_t)op >= toff[q + 1] - toff[q]) return fail(FBH_ERR_ARG, "term index out of range");
+                if (++sp > fbk::EXPR_DEPTH) return fail(FBH_ERR_ARG, "expression deeper than the stack");
+                aprog[p] = (int32_t)(toff[q] + (uint32_t)op);
+            } else if (op == FBH_EXPR_NOT) {
+                if (sp < 1) return fail(FBH_ERR_ARG, "NOT of an empty stack");
+                aprog[p] = op;
+            } else if (op <= FBH_EXPR_AND && op >= FBH_EXPR_MINUS) {
+                if (sp < 2) return fail(FBH_ERR_ARG, "operator without two operands");
+                --sp;
+                aprog[p] = op;
+            } else {
+                return fail(FBH_ERR_ARG, "unknown operator");
+            }
         }
+        if (sp != 1) return fail(FBH_ERR_ARG, "expression does not leave one result");
     }
-    std::vector<uint32_t> np(nq, c->ntiles);
-    c->dterms = (fbk::QueryArgs*)dev_alloc(sizeof(fbk::QueryArgs) * nterms);
-    c->dqoff = (uint32_t*)dev_alloc(4ull * (nq + 1));
-    c->dnp = (uint32_t*)dev_alloc(4ull * nq);
-    c->parts = (uint32_t*)dev_alloc(4ull * nq * c->ntiles);
-    c->counts = (uint32_t*)dev_alloc(4ull * nq);
-    if (!c->dterms || !c->dqoff || !c->dnp || !c->parts || !c->counts) {
-        delete c;
-        return fail(FBH_ERR_NOMEM, "device allocation failed");
+    fbh_conj* c = new fbh_conj();
+    c->nq = nq; c->nterms = nterms; c->ntiles = ix0->ntiles; c->groups = ix0->groups; c->anb = ix0->anb;
+    c->plans.resize(nterms);
+    c->order.resize(nterms);
+    std::vector<fbk::QueryArgs> h(nterms);
+    for (uint32_t k = 0; k < nterms; ++k) {
+        const fbh_term& t = terms[k];
+        int p = plan == FBH_PLAN_AUTO ? plan_heuristic(t.ix, t.lo, t.lop, t.hi, t.rop) : plan;
+        if (p == FBH_PLAN_COMPRESSED) p = FBH_PLAN_LITERAL;
+        c->plans[k] = p;
+        c->order[k] = k;
+        term_args(t, p, h[k]);
     }
-    rc = to_dev(c->dterms, h.data(), nterms);
-    if (!rc) rc = to_dev(c->dqoff, qoff, nq + 1);
-    if (!rc) rc = to_dev(c->dnp, np.data(), nq);
-    if (!rc) rc = stream_sync();
-    c->blocks = 4u * (uint32_t)cu_count();
-    if (!rc) rc = conj_capture(c);
+    rc = conj_upload(c, h, toff, aprog.data(), poff, poff[nq]);
     if (rc) { delete c; return rc; }
     *out = c;
     return FBH_OK;

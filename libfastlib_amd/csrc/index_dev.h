@@ -50,7 +50,17 @@ struct ConjArgs {
     uint32_t tpi;              // tiles per work item
     uint32_t* part;
     uint32_t* out;
+    // boolean expressions instead of conjunctions when prog is not null:
+    // query q runs the postfix program prog[poff[q] .. poff[q+1]); an item
+    // >= 0 pushes the result of terms[item], an EXPR_* item combines the
+    // top of the stack (at most EXPR_DEPTH results)
+    const int32_t* prog;
+    const uint32_t* poff;
 };
+
+constexpr int EXPR_DEPTH = 4;
+constexpr int32_t EXPR_AND = -1, EXPR_OR = -2, EXPR_XOR = -3, EXPR_NAND = -4, EXPR_NOR = -5,
+                  EXPR_NOT = -6, EXPR_MINUS = -7;
 
 hipError_t launch_keys_relic(const void* col, int type, uint32_t n, const double* vals, uint32_t nv,
                              uint32_t* keys, hipStream_t s);

@@ -1094,6 +1094,83 @@ __global__ __launch_bounds__(NT) void k_conj(ConjArgs a) {
     }
 }
 
+// Boolean expressions over range terms (countQuery::doEvaluate,
+// src/countQuery.cpp:944-1003, with the mask of every row): per tile the
+// block runs the query's postfix program over a stack of tile results in
+// LDS.  NOT complements within the rows (an active group keeps only its
+// nbits), NAND / NOR are NOT of AND / OR as fastbit_selection_combine
+// builds them (src/iapi.cpp:1313-1326), MINUS is AND NOT.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_expr(ConjArgs a) {
+    __shared__ word_t S[EXPR_DEPTH][QT + 1];
+    __shared__ uint32_t cov[QT + 1];
+    __shared__ uint32_t red[NT / WAVE + 1];
+    __shared__ uint64_t bal[NT / WAVE][32];
+    __shared__ uint32_t ctab[CT_WORDS];
+    const uint32_t nchunk = (a.ntiles + a.tpi - 1) / a.tpi;
+    const uint32_t nitems = a.nq * nchunk;
+    const int w = threadIdx.x / WAVE;
+    const word_t amask = a.anb ? (((1u << a.anb) - 1u) << (31u - a.anb)) : 0u;
+    for (uint32_t item = blockIdx.x; item < nitems; item += gridDim.x) {
+        const uint32_t qi = item / nchunk;
+        const uint32_t c = item - qi * nchunk;
+        const uint32_t p0 = a.poff[qi], p1 = a.poff[qi + 1];
+        for (uint32_t tile = c * a.tpi; tile < min((c + 1) * a.tpi, a.ntiles); ++tile) {
+            const bool last = tile + 1 == a.ntiles;
+            const uint32_t g0 = tile * (uint32_t)QT;
+            const uint32_t ng = last ? (a.groups - g0) + (a.anb ? 1u : 0u) : (uint32_t)QT;
+            int sp = 0;
+            for (uint32_t p = p0; p < p1; ++p) {
+                const int32_t op = a.prog[p];
+                if (op >= 0) {
+                    term_tile<NT>(a.terms[op], tile, g0, ng, last, S[sp], cov, red, bal, ctab);
+                    ++sp;
+                    continue;
+                }
+                if (op == EXPR_NOT) {
+                    word_t* x = S[sp - 1];
+                    for (uint32_t i = threadIdx.x; i < ng; i += NT)
+                        x[i] = ~x[i] & (g0 + i == a.groups ? amask : 0x7FFFFFFFu);
+                } else {
+                    word_t* x = S[sp - 2];
+                    const word_t* y = S[sp - 1];
+                    for (uint32_t i = threadIdx.x; i < ng; i += NT) {
+                        const word_t u = x[i], v = y[i];
+                        const word_t m = g0 + i == a.groups ? amask : 0x7FFFFFFFu;
+                        word_t r;
+                        switch (op) {
+                        case EXPR_AND: r = u & v; break;
+                        case EXPR_OR: r = u | v; break;
+                        case EXPR_XOR: r = u ^ v; break;
+                        case EXPR_NAND: r = ~(u & v) & m; break;
+                        case EXPR_NOR: r = ~(u | v) & m; break;
+                        default: r = u & ~v; break;          // EXPR_MINUS
+                        }
+                        x[i] = r;
+                    }
+                    --sp;
+                }
+                __syncthreads();
+            }
+            uint32_t ones = 0;
+            for (uint32_t i = threadIdx.x; i < ng; i += NT) {
+                const word_t v = S[0][i];
+                if (a.out) a.out[g0 + i] = v;
+                ones += __popc(v);
+            }
+            ones = wave_sum(ones);
+            if (lane_id() == 0) red[w] = ones;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                uint32_t t = 0;
+                for (int i = 0; i < NT / WAVE; ++i) t += red[i];
+                a.part[(size_t)qi * a.ntiles + tile] = t;
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // Raw scan plan: R[g] = rows 31g..31g+30 that satisfy the range.  A wave
 // covers 64 groups (1984 rows): 31 coalesced loads of 64 values, 31
 // ballots staged in LDS, each lane assembles its group.
@@ -1433,6 +1510,15 @@ hipError_t launch_query_lit(const QueryArgs& q, int type, hipStream_t s) {
 
 hipError_t launch_conj(const ConjArgs& a, int nt, uint32_t blocks, hipStream_t s) {
     if (!a.nq) return hipSuccess;
+    if (a.prog) {
+        switch (nt) {
+        case 128: hipLaunchKernelGGL(k_expr<128>, dim3(blocks), dim3(128), 0, s, a); break;
+        case 256: hipLaunchKernelGGL(k_expr<256>, dim3(blocks), dim3(256), 0, s, a); break;
+        case 512: hipLaunchKernelGGL(k_expr<512>, dim3(blocks), dim3(512), 0, s, a); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     switch (nt) {
     case 128: hipLaunchKernelGGL(k_conj<128>, dim3(blocks), dim3(128), 0, s, a); break;
     case 256: hipLaunchKernelGGL(k_conj<256>, dim3(blocks), dim3(256), 0, s, a); break;

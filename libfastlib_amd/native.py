@@ -544,6 +544,8 @@ def _conj_sigs(L):
         "fbh_conj_tune": [vp, C.POINTER(C.c_int), _u32p, _u32p, C.POINTER(C.c_float)],
         "fbh_conj_plans": [vp, C.POINTER(C.c_int)],
         "fbh_conj_evaluate": [C.POINTER(_Term), C.c_uint32, C.c_int, _u64p],
+        "fbh_expr_prepare": [C.POINTER(_Term), _u32p, C.POINTER(C.c_int32), _u32p, C.c_uint32, C.c_int,
+                             C.POINTER(vp)],
     }.items():
         f = getattr(L, name)
         f.argtypes = args
@@ -610,6 +612,43 @@ class Conjunctions:
 def _f32(a, shape):
     return np.ascontiguousarray(np.asarray(a, np.float32).reshape(shape))
 
+
+
+EXPR_AND, EXPR_OR, EXPR_XOR, EXPR_NAND, EXPR_NOR, EXPR_NOT, EXPR_MINUS = -1, -2, -3, -4, -5, -6, -7
+
+
+class Expressions(Conjunctions):
+    """A prepared batch of boolean expressions over range terms
+    (fbh_expr_prepare).  queries is a list of (terms, program): terms a list
+    of (Index, lo, lop, hi, rop), program a postfix list of term positions
+    (>= 0) and EXPR_* operators."""
+
+    def __init__(self, queries, plan=PLAN_AUTO):
+        L = Index._lib()
+        if not getattr(L, "_conj_sigs", False):
+            _conj_sigs(L)
+            L._conj_sigs = True
+        self._L = L
+        self._keep = []
+        terms, toff, prog, poff = [], [0], [], [0]
+        for qterms, qprog in queries:
+            for (ix, lo, lop, hi, rop) in qterms:
+                terms.append(_Term(ix._h, float(lo), int(lop), float(hi), int(rop)))
+                self._keep.append(ix)
+            toff.append(len(terms))
+            prog.extend(int(x) for x in qprog)
+            poff.append(len(prog))
+        self.nq = len(queries)
+        self.nterms = len(terms)
+        arr = (_Term * max(len(terms), 1))(*terms)
+        to = np.ascontiguousarray(toff, np.uint32)
+        pr = np.ascontiguousarray(prog if prog else [0], np.int32)
+        po = np.ascontiguousarray(poff, np.uint32)
+        h = C.c_void_p()
+        self._h = None
+        check(L.fbh_expr_prepare(arr, to.ctypes.data_as(_u32p), pr.ctypes.data_as(C.POINTER(C.c_int32)),
+                                 po.ctypes.data_as(_u32p), self.nq, int(plan), C.byref(h)), "fbh_expr_prepare")
+        self._h = h
 
 class Planner:
     """The FP32 MLP cost model of the plan choice (fbh_planner_*)."""

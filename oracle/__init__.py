@@ -518,3 +518,43 @@ def ref_file_sum_bins(path, ib, ie):
     if n < 0:
         raise RuntimeError(f"ref_file_sum_bins failed {n}")
     return int(n)
+
+
+def ref_iapi_expr(terms, prog):
+    """Hit count of a postfix boolean program over range terms on arrays
+    registered with RefIapi: terms is a list of (name, lo, lop, hi, rop),
+    prog items >= 0 push a term, -1..-7 are AND OR XOR NAND NOR NOT MINUS;
+    built with fastbit_selection_combine, counted by
+    fastbit_selection_evaluate."""
+    L = reference().lib
+    L.ref_iapi_expr.restype = C.c_int64
+    n = len(terms)
+    names = (C.c_char_p * n)(*[t[0].encode() for t in terms])
+    lo = np.array([t[1] for t in terms], np.float64)
+    lop = np.array([t[2] for t in terms], np.int32)
+    hi = np.array([t[3] for t in terms], np.float64)
+    rop = np.array([t[4] for t in terms], np.int32)
+    pr = np.ascontiguousarray(prog, np.int32)
+    cnt = L.ref_iapi_expr(names, lo.ctypes.data_as(_dp), lop.ctypes.data_as(C.POINTER(C.c_int)),
+                          hi.ctypes.data_as(_dp), rop.ctypes.data_as(C.POINTER(C.c_int)),
+                          pr.ctypes.data_as(C.POINTER(C.c_int32)), C.c_uint32(pr.size))
+    if cnt < 0:
+        raise RuntimeError(f"ref_iapi_expr failed {cnt}")
+    return int(cnt)
+
+
+def expr_rows(masks, prog):
+    """The rows a postfix program selects, from the boolean row masks of its
+    terms (NOT complements over every row)."""
+    st = []
+    for op in prog:
+        if op >= 0:
+            st.append(masks[op])
+        elif op == -6:
+            st.append(~st.pop())
+        else:
+            b, a = st.pop(), st.pop()
+            st.append({-1: lambda: a & b, -2: lambda: a | b, -3: lambda: a ^ b, -4: lambda: ~(a & b),
+                       -5: lambda: ~(a | b), -7: lambda: a & ~b}[op]())
+    assert len(st) == 1
+    return st[0]

@@ -427,3 +427,54 @@ int64_t ref_file_sum_bins(const char* path, uint32_t ib, uint32_t ie) {
 }
 
 } // extern "C"
+
+extern "C" {
+
+// A boolean expression over range terms on registered arrays, built with
+// fastbit_selection_combine (src/iapi.cpp:1285-1339) from a postfix program
+// (item >= 0: term; -1 AND, -2 OR, -3 XOR, -4 NAND, -5 NOR, -6 NOT,
+// -7 MINUS) and counted by fastbit_selection_evaluate (countQuery).  Each
+// term is one qContinuousRange lo OP1 names[i] OP2 hi.  Returns the hit
+// count, or a negative code.
+int64_t ref_iapi_expr(const char* const* names, const double* lo, const int* lop, const double* hi,
+                      const int* rop, const int32_t* prog, uint32_t nprog) {
+    ensure_init();
+    std::vector<ibis::qExpr*> st;
+    for (uint32_t p = 0; p < nprog; ++p) {
+        const int32_t op = prog[p];
+        if (op >= 0) {
+            st.push_back(new ibis::qContinuousRange(lo[op], cmp_op(lop[op]), names[op], cmp_op(rop[op]), hi[op]));
+            continue;
+        }
+        if (op == -6) {
+            if (st.empty()) return -1;
+            ibis::qExpr* n = new ibis::qExpr(ibis::qExpr::LOGICAL_NOT);
+            n->setLeft(st.back());
+            st.back() = n;
+            continue;
+        }
+        if (st.size() < 2) return -1;
+        ibis::qExpr* b = st.back(); st.pop_back();
+        ibis::qExpr* a = st.back(); st.pop_back();
+        ibis::qExpr* r = 0;
+        if (op == -7) {
+            r = new ibis::qExpr(ibis::qExpr::LOGICAL_MINUS);
+            r->setLeft(a);
+            r->setRight(b);
+        } else {
+            const FastBitCombineType t = op == -1 ? FastBitCombineAnd : op == -2 ? FastBitCombineOr
+                                         : op == -3 ? FastBitCombineXor : op == -4 ? FastBitCombineNand
+                                         : FastBitCombineNor;
+            r = static_cast<ibis::qExpr*>(fastbit_selection_combine(a, t, b));
+        }
+        if (r == 0) return -2;
+        st.push_back(r);
+    }
+    if (st.size() != 1) return -1;
+    FastBitSelectionHandle h = static_cast<FastBitSelectionHandle>(st[0]);
+    const int64_t cnt = fastbit_selection_evaluate(h);
+    fastbit_selection_free(h);
+    return cnt;
+}
+
+} // extern "C"
