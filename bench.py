@@ -349,6 +349,10 @@ def main():
     # the kernel the heuristic step runs for most of its queries
     dom = max(("literal", "scan"), key=lambda k: plan_mix.get(k, 0))
     dom_kernel = {"literal": "k_query_lit", "scan": "k_query_scan"}[dom]
+    # consecutive queries of a batch alternate between two streams, so two
+    # launches overlap: the time per launch is the batch's device time (HIP
+    # events on the library stream, which joins both) over its launches --
+    # the busy time of the kernel per launch, not one dispatch's span
     achieved = per_plan[dom]["GBs"]
     roofline = dict(bound="hbm", kernel=dom_kernel, achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(dom_kernel),

@@ -875,7 +875,7 @@ __device__ void tile_ranges(const QueryArgs& q, uint32_t tile, uint32_t g0, uint
 // threads.  R and cov are zero on entry; the call ends with a
 // barrier.  red needs NT/64 + 1 words.
 template <class T, int NT, int U>
-__device__ void lit_tile(const QueryArgs& q, uint32_t tile, uint32_t g0, uint32_t ng, bool last, word_t* R,
+__device__ __forceinline__ void lit_tile(const QueryArgs& q, uint32_t tile, uint32_t g0, uint32_t ng, bool last, word_t* R,
                          uint32_t* cov, uint32_t* red, uint32_t* ctab) {
     const T* col = (const T*)q.col;
     const int w = threadIdx.x / WAVE;
@@ -952,7 +952,7 @@ __device__ __forceinline__ void scan_rows64(const QueryArgs& q, const T* col, ui
 }
 
 template <class T, int NT>
-__device__ void scan_tile(const QueryArgs& q, uint32_t g0, uint32_t ng, word_t* R, uint64_t (*bal)[32]) {
+__device__ __forceinline__ void scan_tile(const QueryArgs& q, uint32_t g0, uint32_t ng, word_t* R, uint64_t (*bal)[32]) {
     const T* col = (const T*)q.col;
     const int lane = (int)lane_id();
     const int w = threadIdx.x / WAVE;
@@ -1016,7 +1016,7 @@ __global__ __launch_bounds__(NT) void k_query_lit(QueryArgs q) {
 // empty.  Each term runs the plan chosen for it on the host (literal
 // through the tiled copy of its index, or the raw scan of its column).
 template <int NT>
-__device__ void term_tile(const QueryArgs& q, uint32_t tile, uint32_t g0, uint32_t ng, bool last, word_t* R,
+__device__ __forceinline__ void term_tile(const QueryArgs& q, uint32_t tile, uint32_t g0, uint32_t ng, bool last, word_t* R,
                           uint32_t* cov, uint32_t* red, uint64_t (*bal)[32], uint32_t* ctab) {
     if (q.plan == PLAN_SCAN_D) {
         switch (q.type) {

@@ -8,6 +8,7 @@ namespace fbh_rt {
 
 static int g_device = -1;
 static hipStream_t g_stream = nullptr;
+static hipStream_t g_side[2] = {nullptr, nullptr};
 static std::mutex g_mu;
 static std::multimap<size_t, void*> g_free;        // rounded size -> buffer
 static std::map<void*, size_t> g_size;             // live buffer -> rounded size
@@ -35,13 +36,19 @@ int ensure_init(int device) {
     if (device >= n) { set_error("device index out of range"); return FBH_ERR_ARG; }
     if ((e = hipSetDevice(device)) != hipSuccess) return check(e, "hipSetDevice");
     if (g_stream) { (void)hipStreamDestroy(g_stream); g_stream = nullptr; }
+    for (hipStream_t& t : g_side)
+        if (t) { (void)hipStreamDestroy(t); t = nullptr; }
     if ((e = hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking)) != hipSuccess)
         return check(e, "hipStreamCreate");
+    for (hipStream_t& t : g_side)
+        if ((e = hipStreamCreateWithFlags(&t, hipStreamNonBlocking)) != hipSuccess)
+            return check(e, "hipStreamCreate");
     g_device = device;
     return FBH_OK;
 }
 
 hipStream_t stream() { return g_stream; }
+hipStream_t side_stream(int k) { return g_side[k & 1]; }
 
 static size_t round_size(size_t b) {
     if (b < 256) return 256;
