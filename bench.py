@@ -349,8 +349,8 @@ def main():
     # the kernel the heuristic step runs for most of its queries
     dom = max(("literal", "scan"), key=lambda k: plan_mix.get(k, 0))
     dom_kernel = {"literal": "k_query_lit", "scan": "k_query_scan"}[dom]
-    # consecutive queries of a batch alternate between two streams, so two
-    # launches overlap: the time per launch is the batch's device time (HIP
+    # consecutive queries of a batch rotate over three streams, so launches
+    # overlap: the time per launch is the batch's device time (HIP
     # events on the library stream, which joins both) over its launches --
     # the busy time of the kernel per launch, not one dispatch's span
     achieved = per_plan[dom]["GBs"]

@@ -1183,22 +1183,22 @@ int fbh_index_evaluate_batch(fbh_index* ix, uint32_t nq, const double* lo, const
     if (!ix || !lo || !lop || !hi || !rop || !hits) return fail(FBH_ERR_ARG, "null argument");
     int rc = ensure_init();
     if (rc) return rc;
-    // consecutive literal / scan queries alternate between two streams, so
+    // consecutive literal / scan queries rotate over the side streams, so
     // that the last blocks of one query overlap the first of the next (each
     // stream has its own literal result buffer)
-    DBuf<uint32_t> lit[2];
+    DBuf<uint32_t> lit[NSIDE];
     PartCounts pc;
-    if ((rc = lit[0].alloc(ix->groups + 1)) || (rc = lit[1].alloc(ix->groups + 1)) || (rc = pc.alloc(ix, nq)))
-        return rc;
+    for (int k = 0; k < NSIDE; ++k)
+        if ((rc = lit[k].alloc(ix->groups + 1))) return rc;
+    if ((rc = pc.alloc(ix, nq))) return rc;
     std::vector<uint32_t> np(nq, 0);
     std::vector<int> used(nq);
     std::vector<uint64_t> sync_hits(nq, 0);
-    hipEvent_t e0, e1, ej[2];
+    hipEvent_t e0, e1, ej[NSIDE];
     (void)hipEventCreate(&e0); (void)hipEventCreate(&e1);
-    (void)hipEventCreateWithFlags(&ej[0], hipEventDisableTiming);
-    (void)hipEventCreateWithFlags(&ej[1], hipEventDisableTiming);
+    for (int k = 0; k < NSIDE; ++k) (void)hipEventCreateWithFlags(&ej[k], hipEventDisableTiming);
     rc = check(hipEventRecord(e0, stream()), "event");
-    for (int k = 0; k < 2 && !rc; ++k) rc = check(hipStreamWaitEvent(side_stream(k), e0, 0), "stream wait");
+    for (int k = 0; k < NSIDE && !rc; ++k) rc = check(hipStreamWaitEvent(side_stream(k), e0, 0), "stream wait");
     uint32_t nlaunch = 0;
     for (uint32_t i = 0; i < nq && !rc; ++i) {
         int p = plans ? plans[i] : plan;
@@ -1213,13 +1213,13 @@ int fbh_index_evaluate_batch(fbh_index* ix, uint32_t nq, const double* lo, const
             sync_hits[i] = c;
             fbh_bv_free(res);
         } else {
-            const int k = (int)(nlaunch++ & 1u);
+            const int k = (int)(nlaunch++ % (uint32_t)NSIDE);
             rc = index_eval_literal(ix, lo[i], lop[i], hi[i], rop[i], p, lit[k].p, pc.row(i), &np[i],
                                     side_stream(k));
         }
     }
-    // join: stream() waits for both side streams whatever happened above
-    for (int k = 0; k < 2; ++k) {
+    // join: stream() waits for every side stream whatever happened above
+    for (int k = 0; k < NSIDE; ++k) {
         const int jrc = check(hipEventRecord(ej[k], side_stream(k)), "event");
         const int wrc = jrc ? jrc : check(hipStreamWaitEvent(stream(), ej[k], 0), "stream wait");
         if (!rc) rc = wrc;
@@ -1232,7 +1232,7 @@ int fbh_index_evaluate_batch(fbh_index* ix, uint32_t nq, const double* lo, const
     float ms = 0.f;
     if (!rc) (void)hipEventElapsedTime(&ms, e0, e1);
     (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
-    (void)hipEventDestroy(ej[0]); (void)hipEventDestroy(ej[1]);
+    for (int k = 0; k < NSIDE; ++k) (void)hipEventDestroy(ej[k]);
     if (rc) return rc;
     if ((rc = to_host(c.data(), pc.counts.p, nq))) return rc;
     for (uint32_t i = 0; i < nq; ++i) hits[i] = used[i] == FBH_PLAN_COMPRESSED ? sync_hits[i] : c[i];

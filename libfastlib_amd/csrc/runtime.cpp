@@ -8,7 +8,7 @@ namespace fbh_rt {
 
 static int g_device = -1;
 static hipStream_t g_stream = nullptr;
-static hipStream_t g_side[2] = {nullptr, nullptr};
+static hipStream_t g_side[NSIDE] = {};
 static std::mutex g_mu;
 static std::multimap<size_t, void*> g_free;        // rounded size -> buffer
 static std::map<void*, size_t> g_size;             // live buffer -> rounded size
@@ -48,7 +48,7 @@ int ensure_init(int device) {
 }
 
 hipStream_t stream() { return g_stream; }
-hipStream_t side_stream(int k) { return g_side[k & 1]; }
+hipStream_t side_stream(int k) { return g_side[k % NSIDE]; }
 
 static size_t round_size(size_t b) {
     if (b < 256) return 256;

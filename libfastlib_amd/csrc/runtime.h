@@ -11,9 +11,10 @@ namespace fbh_rt {
 // Select the device and create the stream once; returns 0 or an FBH error.
 int ensure_init(int device = -1);
 hipStream_t stream();
-// Two more streams (default priority) for overlapping independent launches
-// of one call; the call joins them back into stream() before it returns, so
-// buffer reuse stays ordered on stream().
+// NSIDE more streams (default priority) for overlapping independent
+// launches of one call; the call joins them back into stream() before it
+// returns, so buffer reuse stays ordered on stream().
+constexpr int NSIDE = 3;
 hipStream_t side_stream(int k);
 
 // Caching allocator.  Buffers are handed back to a free list keyed by the
