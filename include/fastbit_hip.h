@@ -243,8 +243,10 @@ int fbh_index_column_bytes(const fbh_index* ix, uint64_t* bytes);
 #define FBH_COST_PARTITION 1
 int fbh_index_set_cost_model(fbh_index* ix, int model);
 
-/* Many range queries on one index, queued back to back on the device with
- * one synchronisation at the end.  plans[i] (or `plan` when plans is NULL)
+/* Many range queries on one index, queued on the device with one
+ * synchronisation at the end; consecutive queries rotate over three
+ * streams so that their launches overlap.  plans[i] (or `plan` when plans
+ * is NULL)
  * picks the plan per query; hits[i] receives the counts, plans_used[i] the
  * plan that ran, *gpu_ms the device time from the first to the last query.
  * (No counterpart in the reference, which evaluates one selection at a
