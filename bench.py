@@ -168,6 +168,30 @@ def conj_section(L, log2n, nq, steps, learned=None):
                tuned=cfg, term_plans={"literal": int(np.sum(plans == 1)), "scan": int(np.sum(plans == 3))},
                hits_total=int(hits.sum()))
     log("conj", out)
+    # the same terms combined by boolean programs (fastbit_selection_combine
+    # trees): the first two terms ORed, a third one ANDed NOT, a fourth ANDed
+    def program(k):
+        p = [0] if k == 1 else [0, 1, -2]
+        if k >= 3:
+            p += [2, -7]
+        if k >= 4:
+            p += [3, -1]
+        return p
+    eq = [([(ixs[ci], lo, 2, hi, 1) for (ci, lo, hi) in q], program(len(q))) for q in qs]
+    ebatch = L.Expressions(eq)
+    ecfg = ebatch.tune()
+    ebatch.run()
+    ems = []
+    for _ in range(steps):
+        ehits, m = ebatch.run()
+        ems.append(m)
+    eref, _ = L.Expressions(eq, plan=3).run()
+    if not np.array_equal(eref, ehits):
+        raise RuntimeError("expression counts differ from the scan plan's")
+    out["expressions"] = dict(program="(t0 OR t1) AND NOT t2 AND t3", qps=round(nq / (np.median(ems) * 1e-3), 1),
+                              ms_per_batch=round(float(np.median(ems)), 3), tuned=ecfg,
+                              hits_total=int(ehits.sum()))
+    log("expressions", out["expressions"])
     rel = None
     if learned:
         ntr, nte, epochs = learned

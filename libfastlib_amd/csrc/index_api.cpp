@@ -1416,6 +1416,7 @@ struct fbh_conj {
     uint32_t* counts = nullptr;          // nq
     int32_t* dprog = nullptr;            // postfix programs (expression batches)
     uint32_t* dpoff = nullptr;
+    uint32_t depth = 0;                  // deepest stack of the programs
     std::vector<int> plans;              // plan of every term (caller's order)
     std::vector<uint32_t> order;         // evaluation order: order[k] = caller's index
     int nt = 256;                        // threads per block
@@ -1493,7 +1494,7 @@ int conj_launch(fbh_conj* c, hipStream_t s) {
     fbk::ConjArgs a;
     a.terms = c->dterms; a.qoff = c->dqoff; a.nq = c->nq; a.ntiles = c->ntiles;
     a.groups = c->groups; a.anb = c->anb; a.tpi = c->tpi; a.part = c->parts; a.out = nullptr;
-    a.prog = c->dprog; a.poff = c->dpoff;
+    a.prog = c->dprog; a.poff = c->dpoff; a.depth = c->depth;
     int rc = check(fbk::launch_conj(a, c->nt, c->blocks, s), "conjunctions");
     if (!rc) rc = check(fbk::launch_sum_parts(c->parts, c->ntiles, c->dnp, c->nq, c->counts, s), "sum parts");
     return rc;
@@ -1592,6 +1593,7 @@ int fbh_expr_prepare(const fbh_term* terms, const uint32_t* toff, const int32_t*
     // every program leaves exactly one result and stays within the stack;
     // its term items refer to the query's own terms
     std::vector<int32_t> aprog(poff[nq]);
+    int maxsp = 0;
     for (uint32_t q = 0; q < nq; ++q) {
         if (toff[q + 1] < toff[q] || poff[q + 1] <= poff[q]) return fail(FBH_ERR_ARG, "bad offsets");
         int sp = 0;
@@ -1600,6 +1602,7 @@ int fbh_expr_prepare(const fbh_term* terms, const uint32_t* toff, const int32_t*
             if (op >= 0) {
                 if ((uint32_t)op >= toff[q + 1] - toff[q]) return fail(FBH_ERR_ARG, "term index out of range");
                 if (++sp > fbk::EXPR_DEPTH) return fail(FBH_ERR_ARG, "expression deeper than the stack");
+                maxsp = std::max(maxsp, sp);
                 aprog[p] = (int32_t)(toff[q] + (uint32_t)op);
             } else if (op == FBH_EXPR_NOT) {
                 if (sp < 1) return fail(FBH_ERR_ARG, "NOT of an empty stack");
@@ -1616,6 +1619,7 @@ int fbh_expr_prepare(const fbh_term* terms, const uint32_t* toff, const int32_t*
     }
     fbh_conj* c = new fbh_conj();
     c->nq = nq; c->nterms = nterms; c->ntiles = ix0->ntiles; c->groups = ix0->groups; c->anb = ix0->anb;
+    c->depth = (uint32_t)maxsp;
     c->plans.resize(nterms);
     c->order.resize(nterms);
     std::vector<fbk::QueryArgs> h(nterms);

@@ -56,6 +56,7 @@ struct ConjArgs {
     // top of the stack (at most EXPR_DEPTH results)
     const int32_t* prog;
     const uint32_t* poff;
+    uint32_t depth;            // deepest stack any program reaches
 };
 
 constexpr int EXPR_DEPTH = 4;

@@ -1100,9 +1100,9 @@ __global__ __launch_bounds__(NT) void k_conj(ConjArgs a) {
 // LDS.  NOT complements within the rows (an active group keeps only its
 // nbits), NAND / NOR are NOT of AND / OR as fastbit_selection_combine
 // builds them (src/iapi.cpp:1313-1326), MINUS is AND NOT.
-template <int NT>
+template <int NT, int D>
 __global__ __launch_bounds__(NT) void k_expr(ConjArgs a) {
-    __shared__ word_t S[EXPR_DEPTH][QT + 1];
+    __shared__ word_t S[D][QT + 1];      // D >= a.depth (the launcher's choice)
     __shared__ uint32_t cov[QT + 1];
     __shared__ uint32_t red[NT / WAVE + 1];
     __shared__ uint64_t bal[NT / WAVE][32];
@@ -1511,10 +1511,23 @@ hipError_t launch_query_lit(const QueryArgs& q, int type, hipStream_t s) {
 hipError_t launch_conj(const ConjArgs& a, int nt, uint32_t blocks, hipStream_t s) {
     if (!a.nq) return hipSuccess;
     if (a.prog) {
+        // the stack in LDS only as deep as the batch's programs need: a
+        // shallower stack leaves room for more blocks per CU
+        if (a.depth == 0 || a.depth > (uint32_t)EXPR_DEPTH) return hipErrorInvalidValue;
+        const bool d2 = a.depth <= 2;
         switch (nt) {
-        case 128: hipLaunchKernelGGL(k_expr<128>, dim3(blocks), dim3(128), 0, s, a); break;
-        case 256: hipLaunchKernelGGL(k_expr<256>, dim3(blocks), dim3(256), 0, s, a); break;
-        case 512: hipLaunchKernelGGL(k_expr<512>, dim3(blocks), dim3(512), 0, s, a); break;
+        case 128:
+            if (d2) hipLaunchKernelGGL((k_expr<128, 2>), dim3(blocks), dim3(128), 0, s, a);
+            else hipLaunchKernelGGL((k_expr<128, EXPR_DEPTH>), dim3(blocks), dim3(128), 0, s, a);
+            break;
+        case 256:
+            if (d2) hipLaunchKernelGGL((k_expr<256, 2>), dim3(blocks), dim3(256), 0, s, a);
+            else hipLaunchKernelGGL((k_expr<256, EXPR_DEPTH>), dim3(blocks), dim3(256), 0, s, a);
+            break;
+        case 512:
+            if (d2) hipLaunchKernelGGL((k_expr<512, 2>), dim3(blocks), dim3(512), 0, s, a);
+            else hipLaunchKernelGGL((k_expr<512, EXPR_DEPTH>), dim3(blocks), dim3(512), 0, s, a);
+            break;
         default: return hipErrorInvalidValue;
         }
         return hipGetLastError();
