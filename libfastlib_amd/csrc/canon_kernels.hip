@@ -817,6 +817,8 @@ static void launch_canon_lit(const CanonArgs& p, hipStream_t s) {
 // heads, [27:0] last head + 1; the tags cycle through 1..255 and the
 // tables are cleared when they wrap.
 constexpr uint32_t LF_SEG = 64;
+// one polling lane per block of a segment
+static_assert(LF_SEG <= (uint32_t)WAVE && LW_MAXB % LF_SEG == 0, "a segment is at most one wave of blocks");
 __device__ uint64_t lf_status[LW_MAXB];
 __device__ uint64_t lf_segtot[LW_MAXB / LF_SEG];
 
