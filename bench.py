@@ -192,6 +192,21 @@ def conj_section(L, log2n, nq, steps, learned=None):
                               ms_per_batch=round(float(np.median(ems)), 3), tuned=ecfg,
                               hits_total=int(ehits.sum()))
     log("expressions", out["expressions"])
+    # the conjunctions themselves as programs (t0 AND t1 AND ...): the
+    # expression kernel's short-circuit against the conjunction kernel
+    aq = [([(ixs[ci], lo, 2, hi, 1) for (ci, lo, hi) in q], [0] + [x for k in range(1, len(q)) for x in (k, -1)])
+          for q in qs]
+    abatch = L.Expressions(aq)
+    abatch.tune()
+    ams = []
+    for _ in range(steps):
+        ahits, m = abatch.run()
+        ams.append(m)
+    if not np.array_equal(ahits, hits):
+        raise RuntimeError("AND programs differ from the conjunction batch")
+    out["expressions_and"] = dict(program="t0 AND t1 AND ...", qps=round(nq / (np.median(ams) * 1e-3), 1),
+                                  ms_per_batch=round(float(np.median(ams)), 3))
+    log("expressions_and", out["expressions_and"])
     rel = None
     if learned:
         ntr, nte, epochs = learned

@@ -1415,6 +1415,7 @@ struct fbh_conj {
     uint32_t* parts = nullptr;           // nq * ntiles
     uint32_t* counts = nullptr;          // nq
     int32_t* dprog = nullptr;            // postfix programs (expression batches)
+    int32_t* dskip = nullptr;            // short-circuit targets of the programs
     uint32_t* dpoff = nullptr;
     uint32_t depth = 0;                  // deepest stack of the programs
     std::vector<int> plans;              // plan of every term (caller's order)
@@ -1428,7 +1429,7 @@ struct fbh_conj {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         dev_free(dterms); dev_free(dqoff); dev_free(dnp); dev_free(parts); dev_free(counts);
-        dev_free(dprog); dev_free(dpoff);
+        dev_free(dprog); dev_free(dskip); dev_free(dpoff);
     }
 };
 
@@ -1494,7 +1495,7 @@ int conj_launch(fbh_conj* c, hipStream_t s) {
     fbk::ConjArgs a;
     a.terms = c->dterms; a.qoff = c->dqoff; a.nq = c->nq; a.ntiles = c->ntiles;
     a.groups = c->groups; a.anb = c->anb; a.tpi = c->tpi; a.part = c->parts; a.out = nullptr;
-    a.prog = c->dprog; a.poff = c->dpoff; a.depth = c->depth;
+    a.prog = c->dprog; a.poff = c->dpoff; a.depth = c->depth; a.skip = c->dskip;
     int rc = check(fbk::launch_conj(a, c->nt, c->blocks, s), "conjunctions");
     if (!rc) rc = check(fbk::launch_sum_parts(c->parts, c->ntiles, c->dnp, c->nq, c->counts, s), "sum parts");
     return rc;
@@ -1592,16 +1593,18 @@ int fbh_expr_prepare(const fbh_term* terms, const uint32_t* toff, const int32_t*
     }
     // every program leaves exactly one result and stays within the stack;
     // its term items refer to the query's own terms
-    std::vector<int32_t> aprog(poff[nq]);
+    std::vector<int32_t> aprog(poff[nq]), skip(poff[nq], -1);
     int maxsp = 0;
     for (uint32_t q = 0; q < nq; ++q) {
         if (toff[q + 1] < toff[q] || poff[q + 1] <= poff[q]) return fail(FBH_ERR_ARG, "bad offsets");
         int sp = 0;
+        uint32_t start[fbk::EXPR_DEPTH];      // first item of each stacked result's subtree
         for (uint32_t p = poff[q]; p < poff[q + 1]; ++p) {
             const int32_t op = prog[p];
             if (op >= 0) {
                 if ((uint32_t)op >= toff[q + 1] - toff[q]) return fail(FBH_ERR_ARG, "term index out of range");
-                if (++sp > fbk::EXPR_DEPTH) return fail(FBH_ERR_ARG, "expression deeper than the stack");
+                if (sp >= fbk::EXPR_DEPTH) return fail(FBH_ERR_ARG, "expression deeper than the stack");
+                start[sp++] = p;
                 maxsp = std::max(maxsp, sp);
                 aprog[p] = (int32_t)(toff[q] + (uint32_t)op);
             } else if (op == FBH_EXPR_NOT) {
@@ -1609,6 +1612,9 @@ int fbh_expr_prepare(const fbh_term* terms, const uint32_t* toff, const int32_t*
                 aprog[p] = op;
             } else if (op <= FBH_EXPR_AND && op >= FBH_EXPR_MINUS) {
                 if (sp < 2) return fail(FBH_ERR_ARG, "operator without two operands");
+                // AND / MINUS with an empty left operand give the empty
+                // left operand: its right operand need not run
+                if (op == FBH_EXPR_AND || op == FBH_EXPR_MINUS) skip[start[sp - 1]] = (int32_t)(p + 1);
                 --sp;
                 aprog[p] = op;
             } else {
@@ -1631,7 +1637,10 @@ int fbh_expr_prepare(const fbh_term* terms, const uint32_t* toff, const int32_t*
         c->order[k] = k;
         term_args(t, p, h[k]);
     }
-    rc = conj_upload(c, h, toff, aprog.data(), poff, poff[nq]);
+    // the skip table first: conj_upload captures the launch with its address
+    c->dskip = (int32_t*)dev_alloc(4ull * poff[nq]);
+    rc = c->dskip ? to_dev(c->dskip, skip.data(), poff[nq]) : fail(FBH_ERR_NOMEM, "device allocation failed");
+    if (!rc) rc = conj_upload(c, h, toff, aprog.data(), poff, poff[nq]);
     if (rc) { delete c; return rc; }
     *out = c;
     return FBH_OK;

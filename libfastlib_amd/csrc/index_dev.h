@@ -57,6 +57,10 @@ struct ConjArgs {
     const int32_t* prog;
     const uint32_t* poff;
     uint32_t depth;            // deepest stack any program reaches
+    // skip[p] >= 0: item p starts the right operand of an AND or MINUS;
+    // when the left operand's tile is empty the program resumes at skip[p]
+    // (the empty left result is the result of the operator)
+    const int32_t* skip;
 };
 
 constexpr int EXPR_DEPTH = 4;

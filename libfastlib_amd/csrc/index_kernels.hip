@@ -1121,6 +1121,15 @@ __global__ __launch_bounds__(NT) void k_expr(ConjArgs a) {
             const uint32_t ng = last ? (a.groups - g0) + (a.anb ? 1u : 0u) : (uint32_t)QT;
             int sp = 0;
             for (uint32_t p = p0; p < p1; ++p) {
+                const int32_t sk = a.skip[p];
+                if (sk >= 0) {
+                    int any = 0;
+                    for (uint32_t i = threadIdx.x; i < ng; i += NT) any |= S[sp - 1][i] != 0u;
+                    if (!__syncthreads_or(any)) {
+                        p = (uint32_t)sk - 1u;          // the loop's ++p resumes at sk
+                        continue;
+                    }
+                }
                 const int32_t op = a.prog[p];
                 if (op >= 0) {
                     term_tile<NT>(a.terms[op], tile, g0, ng, last, S[sp], cov, red, bal, ctab);
