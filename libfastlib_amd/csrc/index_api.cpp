@@ -1637,6 +1637,29 @@ int fbh_expr_prepare(const fbh_term* terms, const uint32_t* toff, const int32_t*
         c->order[k] = k;
         term_args(t, p, h[k]);
     }
+    // a program that is one chain of ANDs runs its terms cheapest first, as
+    // a conjunction does (qExpr::reorder by weight, src/qExpr.cpp:1022; AND
+    // commutes, so the rows do not change and the chain keeps its shape)
+    for (uint32_t q = 0; q < nq; ++q) {
+        const uint32_t p0 = poff[q], p1 = poff[q + 1];
+        bool chain = (p1 - p0) >= 3 && (p1 - p0) % 2 == 1 && prog[p0] >= 0;
+        for (uint32_t p = p0 + 1; chain && p < p1; p += 2)
+            chain = prog[p] >= 0 && prog[p + 1] == FBH_EXPR_AND;
+        if (!chain) continue;
+        std::vector<std::pair<double, int32_t>> w;
+        for (uint32_t p = p0; p < p1; p += (p == p0 ? 1u : 2u)) {
+            const fbh_term& t = terms[aprog[p]];
+            double bytes = 0;
+            fbh_index_query_bytes(t.ix, t.lo, t.lop, t.hi, t.rop, c->plans[aprog[p]], &bytes);
+            w.push_back({bytes, aprog[p]});
+        }
+        std::stable_sort(w.begin(), w.end(),
+                         [](const std::pair<double, int32_t>& x, const std::pair<double, int32_t>& y) {
+                             return x.first < y.first;
+                         });
+        uint32_t i = 0;
+        for (uint32_t p = p0; p < p1; p += (p == p0 ? 1u : 2u)) aprog[p] = w[i++].second;
+    }
     // the skip table first: conj_upload captures the launch with its address
     c->dskip = (int32_t*)dev_alloc(4ull * poff[nq]);
     rc = c->dskip ? to_dev(c->dskip, skip.data(), poff[nq]) : fail(FBH_ERR_NOMEM, "device allocation failed");
