@@ -42,24 +42,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def config3(log2n, nq, seed=3, extra=()):
+def config3(log2n, nq, extra=()):
     """The column, its nq quantile queries (lo, hi), and one more (lo, hi)
     query set per (seed, count) in `extra`."""
     from libfastlib_amd import datagen as G
     n = 1 << log2n
     t = time.time()
-    col = G.float_mix(seed, n, pool=100000, wnorm=0.7, s=1.2)
+    col = G.bin_f32_column(log2n)
     log(f"column: {n} rows in {time.time() - t:.1f}s")
     t = time.time()
     srt = np.sort(col)
-
-    def quantile_ranges(qseed, k):
-        u = G.uniform_f64(qseed, 2 * k)
-        a = srt[np.minimum((u[:k] * n).astype(np.int64), n - 1)].astype(np.float64)
-        b = srt[np.minimum((u[k:] * n).astype(np.int64), n - 1)].astype(np.float64)
-        return np.minimum(a, b), np.maximum(a, b)
-    lo, hi = quantile_ranges(seed + 1000, nq)
-    more = [quantile_ranges(s, k) for s, k in extra]
+    lo, hi = G.quantile_ranges(srt, G.BIN_F32_QUERY_SEED, nq)
+    more = [G.quantile_ranges(srt, s, k) for s, k in extra]
     log(f"queries: {nq} + {sum(k for _, k in extra)} in {time.time() - t:.1f}s")
     del srt
     return col, lo, hi, more
@@ -143,8 +137,8 @@ def conj_section(L, log2n, nq, steps, learned=None):
     from libfastlib_amd import datagen as G
     n = 1 << log2n
     t = time.time()
-    cols = G.table4(41, n)
-    qs = G.conj_queries(42, cols, nq)
+    cols = G.table4(G.CONJ_TABLE_SEED, n)
+    qs = G.conj_queries(G.CONJ_QUERY_SEED, cols, nq)
     log(f"conj table: {time.time() - t:.1f}s")
     t = time.time()
     ixs = [L.Index.relic(cols[0]), L.Index.bin_equal_weight(cols[1], 10000),
@@ -343,11 +337,13 @@ def main():
         sys.exit(2)
 
     import libfastlib_amd as L
+    from libfastlib_amd import datagen as G
     L.lib()
     if args.wah_only:
         print(json.dumps({"wah_ops_2p28": wah_cells(L, reps=args.wah_only)}))
         return
-    extra = () if args.no_learned else ((5005, args.train_queries), (5006, args.test_queries))
+    extra = () if args.no_learned else ((G.BIN_F32_TRAIN_SEED, args.train_queries),
+                                        (G.BIN_F32_TEST_SEED, args.test_queries))
     col, lo, hi, more = config3(args.log2_rows, args.queries, extra=extra)
     t = time.time()
     ix = L.Index.bin_equal_weight(col, 10000)     # <binning nbins=10000 equal-weight/>

@@ -8,7 +8,9 @@
  *     no C++ exception crosses this boundary;  fbh_last_error() describes
  *     the last failure of the calling thread;
  *   - handles are opaque; a handle is not shared between host threads
- *     without an external lock (as ibis::bitvector);
+ *     without an external lock (as ibis::bitvector); calls on different
+ *     handles may come from several threads: the library holds one lock
+ *     for the whole of every call, so such calls run one after another;
  *   - arrays passed in are host memory unless the name says "dev";
  *   - bitvectors cross the boundary in the reference's serialized form
  *     (src/bitvector.cpp:2283-2313): the code words, then active.val when

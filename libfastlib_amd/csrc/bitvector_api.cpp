@@ -489,15 +489,20 @@ using namespace fbh_impl;
 // ---------------------------------------------------------------------------
 extern "C" {
 
-int fbh_init(int device) { return ensure_init(device); }
+int fbh_init(int device) {
+    FBH_API_LOCK();
+    return ensure_init(device);
+}
 const char* fbh_last_error(void) { return last_error(); }
 int fbh_sync(void) {
+    FBH_API_LOCK();
     int rc = ensure_init();
     if (rc) return rc;
     return check(hipStreamSynchronize(stream()), "sync");
 }
 
 int fbh_bv_new(fbh_bitvector** out) {
+    FBH_API_LOCK();
     if (!out) return fail(FBH_ERR_ARG, "null output");
     int rc = ensure_init();
     if (rc) return rc;
@@ -506,6 +511,7 @@ int fbh_bv_new(fbh_bitvector** out) {
 }
 
 int fbh_bv_from_serial(const uint32_t* arr, uint32_t n, fbh_bitvector** out) {
+    FBH_API_LOCK();
     if (!out || (n && !arr)) return fail(FBH_ERR_ARG, "null argument");
     int rc = ensure_init();
     if (rc) return rc;
@@ -531,6 +537,7 @@ int fbh_bv_from_serial(const uint32_t* arr, uint32_t n, fbh_bitvector** out) {
 }
 
 int fbh_bv_set(fbh_bitvector* bv, int val, uint32_t n) {
+    FBH_API_LOCK();
     if (!bv) return fail(FBH_ERR_ARG, "null handle");
     // src/bitvector.cpp:256-276
     bv->nw = 0; bv->nbits = 0; bv->nset = 0; bv->aval = 0; bv->anb = 0; bv->ck_ok = false;
@@ -545,6 +552,7 @@ int fbh_bv_set(fbh_bitvector* bv, int val, uint32_t n) {
 }
 
 int fbh_bv_copy(const fbh_bitvector* src, fbh_bitvector** out) {
+    FBH_API_LOCK();
     if (!src || !out) return fail(FBH_ERR_ARG, "null argument");
     fbh_bitvector* bv = new fbh_bitvector();
     bv->nbits = src->nbits; bv->nset = src->nset; bv->aval = src->aval; bv->anb = src->anb;
@@ -555,12 +563,14 @@ int fbh_bv_copy(const fbh_bitvector* src, fbh_bitvector** out) {
 }
 
 void fbh_bv_free(fbh_bitvector* bv) {
+    FBH_API_LOCK();
     if (!bv) return;
     bv_release(bv);
     delete bv;
 }
 
 int fbh_bv_write(fbh_bitvector* bv, uint32_t* arr, uint32_t cap, uint32_t* n) {
+    FBH_API_LOCK();
     if (!bv || !n) return fail(FBH_ERR_ARG, "null argument");
     const uint32_t need = bv->nw + 1 + (bv->anb > 0 ? 1 : 0);
     *n = need;
@@ -579,6 +589,7 @@ int fbh_bv_write(fbh_bitvector* bv, uint32_t* arr, uint32_t cap, uint32_t* n) {
 }
 
 int fbh_bv_op(fbh_bitvector* a, const fbh_bitvector* b, int op, int* path) {
+    FBH_API_LOCK();
     if (!a || !b) return fail(FBH_ERR_ARG, "null handle");
     try { return bv_op(a, b, op, path); }
     catch (...) { return fail(FBH_ERR_INTERNAL, "exception in fbh_bv_op"); }
@@ -586,6 +597,7 @@ int fbh_bv_op(fbh_bitvector* a, const fbh_bitvector* b, int op, int* path) {
 
 int fbh_bv_op_new(const fbh_bitvector* a, const fbh_bitvector* b, int op, fbh_bitvector** out,
                   int* path) {
+    FBH_API_LOCK();
     if (!a || !b || !out) return fail(FBH_ERR_ARG, "null argument");
     fbh_bitvector* r = nullptr;
     int rc = fbh_bv_copy(a, &r);
@@ -597,11 +609,13 @@ int fbh_bv_op_new(const fbh_bitvector* a, const fbh_bitvector* b, int op, fbh_bi
 }
 
 int fbh_bv_compress(fbh_bitvector* bv) {
+    FBH_API_LOCK();
     if (!bv) return fail(FBH_ERR_ARG, "null handle");
     return bv_compress(bv);
 }
 
 int fbh_bv_decompress(fbh_bitvector* bv) {
+    FBH_API_LOCK();
     if (!bv) return fail(FBH_ERR_ARG, "null handle");
     // src/bitvector.cpp:474-512
     if (bv->nbits == 0 && bv->nw > 0) {
@@ -633,6 +647,7 @@ int fbh_bv_decompress(fbh_bitvector* bv) {
 }
 
 int fbh_bv_flip(fbh_bitvector* bv) {
+    FBH_API_LOCK();
     if (!bv) return fail(FBH_ERR_ARG, "null handle");
     // src/bitvector.cpp:1128-1174
     int rc = check(fbk::launch_flip(bv->d, bv->nw, stream()), "flip");
@@ -656,6 +671,7 @@ int fbh_bv_flip(fbh_bitvector* bv) {
 }
 
 int fbh_bv_cnt(fbh_bitvector* bv, uint32_t* cnt) {
+    FBH_API_LOCK();
     if (!bv || !cnt) return fail(FBH_ERR_ARG, "null argument");
     // src/bitvector.h:652-656
     if (bv->nset == 0 && bv->nw > 0) {
@@ -669,22 +685,26 @@ int fbh_bv_cnt(fbh_bitvector* bv, uint32_t* cnt) {
 }
 
 int fbh_bv_size(fbh_bitvector* bv, uint32_t* size) {
+    FBH_API_LOCK();
     if (!bv || !size) return fail(FBH_ERR_ARG, "null argument");
     return bv_size(bv, size);
 }
 
 int fbh_bv_adjust_size(fbh_bitvector* bv, uint32_t nv, uint32_t nt) {
+    FBH_API_LOCK();
     if (!bv) return fail(FBH_ERR_ARG, "null handle");
     return bv_adjust(bv, nv, nt);
 }
 
 int fbh_bv_nwords(const fbh_bitvector* bv, uint32_t* n) {
+    FBH_API_LOCK();
     if (!bv || !n) return fail(FBH_ERR_ARG, "null argument");
     *n = bv->nw;
     return FBH_OK;
 }
 
 int fbh_bv_rids(fbh_bitvector* bv, uint32_t* rids, uint32_t cap, uint32_t* n) {
+    FBH_API_LOCK();
     if (!bv || !n) return fail(FBH_ERR_ARG, "null argument");
     uint32_t c;
     int rc = fbh_bv_cnt(bv, &c);
@@ -728,6 +748,7 @@ int fbh_bv_rids(fbh_bitvector* bv, uint32_t* rids, uint32_t cap, uint32_t* n) {
 
 int fbh_bench_op(const fbh_bitvector* a, const fbh_bitvector* b, int op, int reps, float* ms,
                  uint32_t* out_words) {
+    FBH_API_LOCK();
     if (!a || !b || reps <= 0) return fail(FBH_ERR_ARG, "bad argument");
     // decide the routine on a throw-away copy (metadata + shortcut work)
     fbh_bitvector* ta = nullptr;
@@ -774,6 +795,7 @@ int fbh_bench_op(const fbh_bitvector* a, const fbh_bitvector* b, int op, int rep
 // that the working set exceeds the last-level cache and every op streams
 // its operands from HBM.
 int fbh_bench_cnt_rot(fbh_bitvector* const* a, int n, int reps, float* ms, uint32_t* ones) {
+    FBH_API_LOCK();
     if (!a || n <= 0 || reps <= 0) return fail(FBH_ERR_ARG, "bad argument");
     uint32_t* ctl = (uint32_t*)scratch(2).get(4ull * fbk::cnt_scratch_words());
     if (!ctl) return fail(FBH_ERR_NOMEM, "scratch allocation failed");
@@ -800,6 +822,7 @@ int fbh_bench_cnt_rot(fbh_bitvector* const* a, int n, int reps, float* ms, uint3
 
 int fbh_bench_op_rot(fbh_bitvector* const* a, fbh_bitvector* const* b, int npairs, int op, int reps,
                      float* ms, uint32_t* out_words) {
+    FBH_API_LOCK();
     if (!a || !b || npairs <= 0 || reps <= 0) return fail(FBH_ERR_ARG, "bad argument");
     std::vector<int> paths(npairs);
     for (int i = 0; i < npairs; ++i) {

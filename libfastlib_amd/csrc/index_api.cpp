@@ -794,6 +794,7 @@ extern "C" {
 void fbh_free(void* p) { std::free(p); }
 
 int fbh_index_build_relic(const void* col, int type, uint32_t nrows, fbh_index** out) {
+    FBH_API_LOCK();
     if (!out || (nrows && !col)) return fail(FBH_ERR_ARG, "null argument");
     int rc = ensure_init();
     if (rc) return rc;
@@ -819,6 +820,7 @@ int fbh_index_build_relic(const void* col, int type, uint32_t nrows, fbh_index**
 }
 
 int fbh_index_build_relic_card(const uint32_t* ind, uint32_t nrows, uint32_t card, fbh_index** out) {
+    FBH_API_LOCK();
     if (!out || (nrows && !ind)) return fail(FBH_ERR_ARG, "null argument");
     int rc = ensure_init();
     if (rc) return rc;
@@ -844,6 +846,7 @@ int fbh_index_build_relic_card(const uint32_t* ind, uint32_t nrows, uint32_t car
 
 int fbh_index_build_bin(const void* col, int type, uint32_t nrows, const double* bounds,
                         uint32_t nbounds, fbh_index** out) {
+    FBH_API_LOCK();
     if (!out || (nrows && !col) || !bounds || nbounds == 0) return fail(FBH_ERR_ARG, "bad argument");
     int rc = ensure_init();
     if (rc) return rc;
@@ -872,6 +875,7 @@ int fbh_index_build_bin(const void* col, int type, uint32_t nrows, const double*
 
 int fbh_index_build_bin_nbins(const void* col, int type, uint32_t nrows, uint32_t nbins,
                               fbh_index** out) {
+    FBH_API_LOCK();
     if (!out || (nrows && !col) || nbins == 0) return fail(FBH_ERR_ARG, "bad argument");
     if (type == FBH_COL_DOUBLE) return fail(FBH_ERR_ARG, "equal-weight bins over double columns are not supported");
     int rc = ensure_init();
@@ -903,6 +907,7 @@ int fbh_index_build_bin_nbins(const void* col, int type, uint32_t nrows, uint32_
 int fbh_index_attach(const void* col, int type, uint32_t nrows, const double* keys, uint64_t nkeys,
                      const int64_t* offsets, uint64_t noffsets, const uint32_t* bms, uint64_t nbms,
                      fbh_index** out) {
+    FBH_API_LOCK();
     // the argument checks of fastbit_iapi_attach_full_index
     // (src/iapi.cpp:2037-2044)
     if (!out || (nrows && !col) || !keys || nkeys == 0 || !offsets || noffsets == 0 || !bms || nbms == 0)
@@ -939,6 +944,7 @@ int fbh_index_attach(const void* col, int type, uint32_t nrows, const double* ke
 }
 
 int fbh_index_read_file(const char* path, const void* col, int type, uint32_t nrows, fbh_index** out) {
+    FBH_API_LOCK();
     if (!path || !out || (nrows && !col)) return fail(FBH_ERR_ARG, "null argument");
     FileMap f;
     int rc = f.open(path);
@@ -1010,6 +1016,7 @@ int fbh_index_read_file(const char* path, const void* col, int type, uint32_t nr
 }
 
 int fbh_index_sum_bins(fbh_index* ix, uint32_t ib, uint32_t ie, uint64_t* hits) {
+    FBH_API_LOCK();
     if (!ix || !hits) return fail(FBH_ERR_ARG, "null argument");
     int rc = ensure_init();
     if (rc) return rc;
@@ -1030,6 +1037,7 @@ int fbh_index_sum_bins(fbh_index* ix, uint32_t ib, uint32_t ie, uint64_t* hits) 
 }
 
 int fbh_index_bounds(const fbh_index* ix, double* bounds, uint32_t cap, uint32_t* n) {
+    FBH_API_LOCK();
     if (!ix || !n) return fail(FBH_ERR_ARG, "null argument");
     if (ix->kind != FBH_BIN) return fail(FBH_ERR_ARG, "not a bin index");
     *n = (uint32_t)ix->keys.size();
@@ -1037,9 +1045,13 @@ int fbh_index_bounds(const fbh_index* ix, double* bounds, uint32_t cap, uint32_t
     return FBH_OK;
 }
 
-void fbh_index_free(fbh_index* ix) { delete ix; }
+void fbh_index_free(fbh_index* ix) {
+    FBH_API_LOCK();
+    delete ix;
+}
 
 int fbh_index_info(const fbh_index* ix, uint32_t* nobs, uint32_t* nrows, uint64_t* nwords) {
+    FBH_API_LOCK();
     if (!ix) return fail(FBH_ERR_ARG, "null index");
     if (nobs) *nobs = ix->nobs;
     if (nrows) *nrows = ix->nrows;
@@ -1049,6 +1061,7 @@ int fbh_index_info(const fbh_index* ix, uint32_t* nobs, uint32_t* nrows, uint64_
 
 int fbh_index_deconstruct(const fbh_index* ix, double** keys, uint64_t* nkeys, int64_t** offsets,
                           uint64_t* noffsets, uint32_t** bms, uint64_t* nbms) {
+    FBH_API_LOCK();
     if (!ix || !keys || !nkeys || !offsets || !noffsets || !bms || !nbms)
         return fail(FBH_ERR_ARG, "null argument");
     // relic: keys = vals; bin: keys = minval ++ maxval
@@ -1070,6 +1083,7 @@ int fbh_index_deconstruct(const fbh_index* ix, double** keys, uint64_t* nkeys, i
 
 int fbh_index_resolve_range(const fbh_index* ix, double lo, int lop, double hi, int rop,
                             uint32_t* cand0, uint32_t* hit0, uint32_t* hit1, uint32_t* cand1) {
+    FBH_API_LOCK();
     if (!ix) return fail(FBH_ERR_ARG, "null index");
     Locate L;
     locate_raw(ix, lo, lop, hi, rop, L);
@@ -1082,6 +1096,7 @@ int fbh_index_resolve_range(const fbh_index* ix, double lo, int lop, double hi, 
 
 int fbh_index_estimate_cost(const fbh_index* ix, double lo, int lop, double hi, int rop,
                             double* icost, double* scost, int* plan) {
+    FBH_API_LOCK();
     if (!ix) return fail(FBH_ERR_ARG, "null index");
     const double ic = index_cost(ix, lo, lop, hi, rop);
     const double sc = scan_cost(ix);
@@ -1092,6 +1107,7 @@ int fbh_index_estimate_cost(const fbh_index* ix, double lo, int lop, double hi, 
 }
 
 int fbh_index_set_cost_model(fbh_index* ix, int model) {
+    FBH_API_LOCK();
     if (!ix) return fail(FBH_ERR_ARG, "null index");
     if (model != FBH_COST_IAPI && model != FBH_COST_PARTITION) return fail(FBH_ERR_ARG, "unknown cost model");
     ix->cost_model = model;
@@ -1100,6 +1116,7 @@ int fbh_index_set_cost_model(fbh_index* ix, int model) {
 
 int fbh_index_evaluate(fbh_index* ix, double lo, int lop, double hi, int rop, int plan,
                        uint64_t* hits, fbh_bitvector** result, int* plan_used) {
+    FBH_API_LOCK();
     if (!ix) return fail(FBH_ERR_ARG, "null index");
     if (lop < 0 || lop > 2 || rop < 0 || rop > 2) return fail(FBH_ERR_ARG, "bad operator");
     int rc = ensure_init();
@@ -1153,6 +1170,7 @@ int fbh_index_evaluate(fbh_index* ix, double lo, int lop, double hi, int rop, in
 
 int fbh_index_evaluate_rids(fbh_index* ix, double lo, int lop, double hi, int rop, int plan,
                             uint32_t* rids, uint64_t cap, uint64_t* n) {
+    FBH_API_LOCK();
     if (!ix || !n) return fail(FBH_ERR_ARG, "null argument");
     int rc = ensure_init();
     if (rc) return rc;
@@ -1180,6 +1198,7 @@ int fbh_index_evaluate_rids(fbh_index* ix, double lo, int lop, double hi, int ro
 int fbh_index_evaluate_batch(fbh_index* ix, uint32_t nq, const double* lo, const int* lop,
                              const double* hi, const int* rop, const int* plans, int plan,
                              uint64_t* hits, int* plans_used, float* gpu_ms) {
+    FBH_API_LOCK();
     if (!ix || !lo || !lop || !hi || !rop || !hits) return fail(FBH_ERR_ARG, "null argument");
     int rc = ensure_init();
     if (rc) return rc;
@@ -1228,7 +1247,12 @@ int fbh_index_evaluate_batch(fbh_index* ix, uint32_t nq, const double* lo, const
     if (!rc) rc = pc.launch_sum(np.data(), nq);
     if (!rc) rc = check(hipEventRecord(e1, stream()), "event");
     if (!rc) rc = check(hipEventSynchronize(e1), "event sync");
-    else (void)hipStreamSynchronize(stream());
+    if (rc) {
+        // a failed join leaves side-stream kernels that may still write lit[]
+        // and the partial counts: drain every stream before they are freed
+        (void)hipStreamSynchronize(stream());
+        for (int k = 0; k < NSIDE; ++k) (void)hipStreamSynchronize(side_stream(k));
+    }
     float ms = 0.f;
     if (!rc) (void)hipEventElapsedTime(&ms, e0, e1);
     (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
@@ -1246,6 +1270,7 @@ int fbh_index_evaluate_batch(fbh_index* ix, uint32_t nq, const double* lo, const
 // literal result it writes.
 int fbh_index_query_bytes(const fbh_index* ix, double lo, int lop, double hi, int rop, int plan,
                           double* bytes) {
+    FBH_API_LOCK();
     if (!ix || !bytes) return fail(FBH_ERR_ARG, "null argument");
     const double res = 4.0 * (ix->groups + 1);
     const double es = (double)col_esize(ix->type);
@@ -1270,6 +1295,7 @@ int fbh_index_query_bytes(const fbh_index* ix, double lo, int lop, double hi, in
 }
 
 int fbh_index_column_bytes(const fbh_index* ix, uint64_t* bytes) {
+    FBH_API_LOCK();
     if (!ix || !bytes) return fail(FBH_ERR_ARG, "null argument");
     *bytes = (uint64_t)col_esize(ix->type) * ix->nrows;
     return FBH_OK;
@@ -1277,6 +1303,7 @@ int fbh_index_column_bytes(const fbh_index* ix, uint64_t* bytes) {
 
 int fbh_index_plan_features(const fbh_index* ix, uint32_t nq, const double* lo, const int* lop,
                             const double* hi, const int* rop, float* feats) {
+    FBH_API_LOCK();
     if (!ix || (nq && (!lo || !lop || !hi || !rop || !feats))) return fail(FBH_ERR_ARG, "null argument");
     const uint32_t nobs = ix->nobs;
     auto words = [&](uint32_t b0, uint32_t b1) {
@@ -1319,6 +1346,7 @@ int fbh_index_plan_features(const fbh_index* ix, uint32_t nq, const double* lo, 
 
 int fbh_index_time_plans(fbh_index* ix, uint32_t nq, const double* lo, const int* lop, const double* hi,
                          const int* rop, float cap, float* ms, uint64_t* hits) {
+    FBH_API_LOCK();
     if (!ix || (nq && (!lo || !lop || !hi || !rop || !ms || !hits))) return fail(FBH_ERR_ARG, "null argument");
     for (uint32_t i = 0; i < nq; ++i)
         if (lop[i] < 0 || lop[i] > 2 || rop[i] < 0 || rop[i] > 2) return fail(FBH_ERR_ARG, "bad operator");
@@ -1534,6 +1562,7 @@ int conj_run(fbh_conj* c, float* ms) {
 extern "C" {
 
 int fbh_conj_prepare(const fbh_term* terms, const uint32_t* qoff, uint32_t nq, int plan, fbh_conj** out) {
+    FBH_API_LOCK();
     if (!terms || !qoff || !out || nq == 0) return fail(FBH_ERR_ARG, "bad argument");
     if (plan < FBH_PLAN_AUTO || plan > FBH_PLAN_SCAN) return fail(FBH_ERR_ARG, "bad plan");
     int rc = ensure_init();
@@ -1578,6 +1607,7 @@ int fbh_conj_prepare(const fbh_term* terms, const uint32_t* qoff, uint32_t nq, i
 
 int fbh_expr_prepare(const fbh_term* terms, const uint32_t* toff, const int32_t* prog, const uint32_t* poff,
                      uint32_t nq, int plan, fbh_conj** out) {
+    FBH_API_LOCK();
     if (!terms || !toff || !prog || !poff || !out || nq == 0) return fail(FBH_ERR_ARG, "bad argument");
     if (plan < FBH_PLAN_AUTO || plan > FBH_PLAN_SCAN) return fail(FBH_ERR_ARG, "bad plan");
     int rc = ensure_init();
@@ -1670,6 +1700,7 @@ int fbh_expr_prepare(const fbh_term* terms, const uint32_t* toff, const int32_t*
 }
 
 int fbh_conj_run(fbh_conj* c, uint64_t* hits, float* gpu_ms) {
+    FBH_API_LOCK();
     if (!c) return fail(FBH_ERR_ARG, "null batch");
     int rc = conj_run(c, gpu_ms);
     if (rc || !hits) return rc;
@@ -1680,6 +1711,7 @@ int fbh_conj_run(fbh_conj* c, uint64_t* hits, float* gpu_ms) {
 }
 
 int fbh_conj_tune(fbh_conj* c, int* nt, uint32_t* blocks, uint32_t* tpi, float* best_ms) {
+    FBH_API_LOCK();
     if (!c) return fail(FBH_ERR_ARG, "null batch");
     static const int NTS[] = {128, 256, 512};
     static const uint32_t TPIS[] = {1, 2, 4};
@@ -1708,14 +1740,19 @@ int fbh_conj_tune(fbh_conj* c, int* nt, uint32_t* blocks, uint32_t* tpi, float* 
 }
 
 int fbh_conj_plans(const fbh_conj* c, int* plans) {
+    FBH_API_LOCK();
     if (!c || !plans) return fail(FBH_ERR_ARG, "null argument");
     for (uint32_t k = 0; k < c->nterms; ++k) plans[k] = c->plans[k];
     return FBH_OK;
 }
 
-void fbh_conj_free(fbh_conj* c) { delete c; }
+void fbh_conj_free(fbh_conj* c) {
+    FBH_API_LOCK();
+    delete c;
+}
 
 int fbh_conj_evaluate(const fbh_term* terms, uint32_t nterms, int plan, uint64_t* hits) {
+    FBH_API_LOCK();
     if (!terms || nterms == 0 || !hits) return fail(FBH_ERR_ARG, "bad argument");
     const uint32_t qoff[2] = {0, nterms};
     fbh_conj* c = nullptr;

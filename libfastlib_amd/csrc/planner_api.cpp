@@ -123,6 +123,7 @@ double mask_weight(const float* mask, const uint32_t* rows, uint32_t n) {
 extern "C" {
 
 int fbh_planner_new(uint64_t seed, fbh_planner** out) {
+    FBH_API_LOCK();
     if (!out) return fail(FBH_ERR_ARG, "null argument");
     int rc = ensure_init();
     if (rc) return rc;
@@ -150,9 +151,13 @@ int fbh_planner_new(uint64_t seed, fbh_planner** out) {
     return FBH_OK;
 }
 
-void fbh_planner_free(fbh_planner* p) { delete p; }
+void fbh_planner_free(fbh_planner* p) {
+    FBH_API_LOCK();
+    delete p;
+}
 
 int fbh_planner_set_params(fbh_planner* p, const float* params, uint32_t n) {
+    FBH_API_LOCK();
     if (!p || !params) return fail(FBH_ERR_ARG, "null argument");
     if (n != (uint32_t)fbm::NPARAM) return fail(FBH_ERR_SIZE, "parameter count is FBH_PLANNER_NPARAM");
     int rc = p->params.put(params, n);
@@ -160,12 +165,14 @@ int fbh_planner_set_params(fbh_planner* p, const float* params, uint32_t n) {
 }
 
 int fbh_planner_get_params(const fbh_planner* p, float* params, uint32_t n) {
+    FBH_API_LOCK();
     if (!p || !params) return fail(FBH_ERR_ARG, "null argument");
     if (n != (uint32_t)fbm::NPARAM) return fail(FBH_ERR_SIZE, "parameter count is FBH_PLANNER_NPARAM");
     return p->params.get(params, n);
 }
 
 int fbh_planner_set_norm(fbh_planner* p, const float* mu, const float* sd) {
+    FBH_API_LOCK();
     if (!p || !mu || !sd) return fail(FBH_ERR_ARG, "null argument");
     std::memcpy(p->hmu, mu, sizeof(p->hmu));
     std::memcpy(p->hsd, sd, sizeof(p->hsd));
@@ -174,6 +181,7 @@ int fbh_planner_set_norm(fbh_planner* p, const float* mu, const float* sd) {
 }
 
 int fbh_planner_get_norm(const fbh_planner* p, float* mu, float* sd) {
+    FBH_API_LOCK();
     if (!p || !mu || !sd) return fail(FBH_ERR_ARG, "null argument");
     std::memcpy(mu, p->hmu, sizeof(p->hmu));
     std::memcpy(sd, p->hsd, sizeof(p->hsd));
@@ -181,6 +189,7 @@ int fbh_planner_get_norm(const fbh_planner* p, float* mu, float* sd) {
 }
 
 int fbh_planner_forward(fbh_planner* p, const float* x, uint32_t n, float* out) {
+    FBH_API_LOCK();
     if (!p || (n && (!x || !out))) return fail(FBH_ERR_ARG, "null argument");
     if (n == 0) return FBH_OK;
     int rc = p->x.put(x, (size_t)n * fbm::NF);
@@ -193,6 +202,7 @@ int fbh_planner_forward(fbh_planner* p, const float* x, uint32_t n, float* out) 
 
 int fbh_planner_grad(fbh_planner* p, const float* x, const float* y, const float* mask, uint32_t n,
                      float* loss, float* grad) {
+    FBH_API_LOCK();
     if (!p || !x || !y || !mask || !grad || n == 0) return fail(FBH_ERR_ARG, "null argument or no samples");
     const double mw = mask_weight(mask, nullptr, n);
     const float inv_m = mw > 0 ? (float)(1.0 / mw) : 0.f;
@@ -215,6 +225,7 @@ int fbh_planner_grad(fbh_planner* p, const float* x, const float* y, const float
 }
 
 int fbh_planner_step(fbh_planner* p, const float* x, const float* y, const float* mask, uint32_t n, float lr) {
+    FBH_API_LOCK();
     if (!p || !x || !y || !mask || n == 0) return fail(FBH_ERR_ARG, "null argument or no samples");
     const double mw = mask_weight(mask, nullptr, n);
     const float inv_m = mw > 0 ? (float)(1.0 / mw) : 0.f;
@@ -231,6 +242,7 @@ int fbh_planner_step(fbh_planner* p, const float* x, const float* y, const float
 
 int fbh_planner_train(fbh_planner* p, const float* x, const float* y, const float* mask, uint32_t n,
                       uint32_t epochs, uint32_t batch, float lr, uint64_t seed, float* epoch_loss) {
+    FBH_API_LOCK();
     if (!p || !x || !y || !mask || n == 0 || batch == 0) return fail(FBH_ERR_ARG, "null argument or no samples");
     // normalisation and the in-range box from the training features
     float flo[fbm::NF], fhi[fbm::NF];
@@ -298,6 +310,7 @@ int fbh_planner_train(fbh_planner* p, const float* x, const float* y, const floa
 
 int fbh_planner_choose(fbh_planner* p, const float* x, uint32_t n, const int* fallback, int* plans,
                        uint32_t* nfallback) {
+    FBH_API_LOCK();
     if (!p || (n && (!x || !fallback || !plans))) return fail(FBH_ERR_ARG, "null argument");
     if (n == 0) { if (nfallback) *nfallback = 0; return FBH_OK; }
     if (!p->trained) {
@@ -324,6 +337,7 @@ int fbh_planner_choose(fbh_planner* p, const float* x, uint32_t n, const int* fa
 
 int fbh_index_choose_plans(fbh_index* ix, fbh_planner* p, uint32_t nq, const double* lo, const int* lop,
                            const double* hi, const int* rop, int* plans, uint32_t* nfallback) {
+    FBH_API_LOCK();
     if (!ix || !p || (nq && (!lo || !lop || !hi || !rop || !plans))) return fail(FBH_ERR_ARG, "null argument");
     std::vector<float> f((size_t)nq * fbm::NF);
     std::vector<int> fb(nq);

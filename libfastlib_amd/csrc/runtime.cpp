@@ -14,6 +14,11 @@ static std::multimap<size_t, void*> g_free;        // rounded size -> buffer
 static std::map<void*, size_t> g_size;             // live buffer -> rounded size
 static thread_local std::string t_err;
 
+std::recursive_mutex& api_mutex() {
+    static std::recursive_mutex m;
+    return m;
+}
+
 void set_error(const std::string& s) { t_err = s; }
 const char* last_error() { return t_err.c_str(); }
 

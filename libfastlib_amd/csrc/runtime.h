@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stddef.h>
+#include <mutex>
 #include <string>
 
 namespace fbh_rt {
@@ -22,6 +23,14 @@ hipStream_t side_stream(int k);
 // stream, so reuse is ordered after the work that used the buffer.
 void* dev_alloc(size_t bytes);
 void dev_free(void* p);
+
+// Library-wide lock.  Every C-ABI entry point holds it for its whole call:
+// the library stream, the scratch slots and the look-back status tables
+// are shared by all handles, so two host threads working on different
+// handles must not interleave their launches and read-backs.  Recursive
+// because entry points call one another.
+std::recursive_mutex& api_mutex();
+#define FBH_API_LOCK() std::lock_guard<std::recursive_mutex> fbh_api_guard_(fbh_rt::api_mutex())
 
 void set_error(const std::string& s);
 const char* last_error();

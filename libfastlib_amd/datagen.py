@@ -126,3 +126,34 @@ def conj_queries(seed, cols, nq, min_terms=2, max_terms=4, smin=1e-6, smax=0.3):
             q.append((int(ci), float(a), float(b)))
         out.append(q)
     return out
+
+
+def quantile_ranges(srt, seed, k):
+    """k ranges lo <= x < hi whose endpoints are empirical quantiles of a
+    column at uniform draws: srt is the sorted column; lo and hi are the
+    values at ranks floor(u * n) of 2k uniform draws, ordered per pair."""
+    n = len(srt)
+    u = uniform_f64(seed, 2 * k)
+    a = srt[np.minimum((u[:k] * n).astype(np.int64), n - 1)].astype(np.float64)
+    b = srt[np.minimum((u[k:] * n).astype(np.int64), n - 1)].astype(np.float64)
+    return np.minimum(a, b), np.maximum(a, b)
+
+
+# The binned float workload: a float32 column of 2^27 rows from a pool of
+# 10^5 distinct values (70% N(0,1) draws, 30% Zipf(1.2) support), binned
+# with 10,000 equal-weight bins; 1024 quantile ranges for the range-query
+# batch, and 50,000 training / 8192 test ranges for the learned plan choice.
+BIN_F32_SEED = 3
+BIN_F32_QUERY_SEED = 1003
+BIN_F32_TRAIN_SEED = 5005
+BIN_F32_TEST_SEED = 5006
+
+
+def bin_f32_column(log2n=27):
+    return float_mix(BIN_F32_SEED, 1 << log2n, pool=100000, wnorm=0.7, s=1.2)
+
+
+# The conjunctive workload: table4(CONJ_TABLE_SEED, 2^26) and
+# conj_queries(CONJ_QUERY_SEED, ..., 4096).
+CONJ_TABLE_SEED = 41
+CONJ_QUERY_SEED = 42

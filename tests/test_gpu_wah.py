@@ -265,3 +265,37 @@ def test_d2_mixed_kernel_shapes(L, orc):
                 got, gpath = gpu_op(L, x, y, op)
                 assert gpath == wpath, (groups, op)
                 assert np.array_equal(got, want), (groups, op, gpath)
+
+
+def test_ops_from_several_host_threads(L, orc):
+    """Four host threads run operators on handles of their own at the same
+    time (ctypes releases the interpreter lock during each call); the
+    library's lock keeps their use of the shared stream and scratch apart,
+    so every result matches the oracle."""
+    import threading
+    rng = np.random.default_rng(77)
+    work = []
+    for t in range(4):
+        cases = []
+        for _ in range(12):
+            a, b, n = random_pair(rng, orc, 2000, 40000)
+            op = int(rng.integers(0, 4))
+            cases.append((a, b, op, orc.op(a, b, op)[0]))
+        work.append(cases)
+    bad = []
+
+    def run(cases, tag):
+        try:
+            for rep in range(3):
+                for i, (a, b, op, want) in enumerate(cases):
+                    got, _ = gpu_op(L, a, b, op)
+                    if not np.array_equal(got, want):
+                        bad.append((tag, rep, i, op))
+        except Exception as e:          # surfaced below, in the test's thread
+            bad.append((tag, repr(e)))
+    ths = [threading.Thread(target=run, args=(c, k)) for k, c in enumerate(work)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    assert not bad, bad[:5]
