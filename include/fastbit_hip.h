@@ -232,6 +232,23 @@ int fbh_index_evaluate(fbh_index* ix, double lo, int lop, double hi, int rop, in
  * src/countQuery.cpp:387-418). */
 int fbh_index_evaluate_rids(fbh_index* ix, double lo, int lop, double hi, int rop, int plan,
                             uint32_t* rids, uint64_t cap, uint64_t* n);
+/* No counterpart in the reference: fbh_index_evaluate for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  It calls no hipMalloc,
+ * hipFree or other allocating routine.  The literal result (groups + 1
+ * words: one word per 31-row group, then the active rows left-aligned in
+ * the last word) goes to the caller's device buffer dev_out of out_words
+ * words; *hits receives the count.  Plans FBH_PLAN_LITERAL and
+ * FBH_PLAN_SCAN; FBH_PLAN_AUTO runs the heuristic's plan, the literal plan
+ * where that is the compressed one (the same bits); FBH_PLAN_COMPRESSED is
+ * an FBH_ERR_ARG error.  The per-block partial counts live in a static
+ * device array of 65,537 words that each GPU holds until the process ends,
+ * so indexes of more than 65,536 tiles (2^27 groups) are an FBH_ERR_SIZE
+ * error.  Every call on one GPU shares that array: calls that overlap on a
+ * GPU (from other processes' threads or captured graphs replayed
+ * concurrently) race. */
+int fbh_index_evaluate_noalloc(fbh_index* ix, double lo, int lop, double hi, int rop, int plan,
+                               uint32_t* dev_out, uint64_t out_words, uint64_t* hits, int* plan_used);
 int fbh_index_column_bytes(const fbh_index* ix, uint64_t* bytes);
 /* The cost rule of the plan choice (FBH_PLAN_AUTO):
  *   FBH_COST_IAPI       arrays registered through the in-memory API: the

@@ -1195,6 +1195,42 @@ int fbh_index_evaluate_rids(fbh_index* ix, double lo, int lop, double hi, int ro
     return to_host(rids, out.p, k);
 }
 
+// fbh_index_evaluate without any device allocation: the literal result goes
+// to the caller's device buffer, the per-block partial counts and the count
+// to the library's static scratch array (NOALLOC_PARTS + 1 words per GPU).
+int fbh_index_evaluate_noalloc(fbh_index* ix, double lo, int lop, double hi, int rop, int plan,
+                               uint32_t* dev_out, uint64_t out_words, uint64_t* hits, int* plan_used) {
+    FBH_API_LOCK();
+    if (!ix || !dev_out || !hits) return fail(FBH_ERR_ARG, "null argument");
+    if (lop < 0 || lop > 2 || rop < 0 || rop > 2) return fail(FBH_ERR_ARG, "bad operator");
+    if (plan == FBH_PLAN_COMPRESSED)
+        return fail(FBH_ERR_ARG, "the compressed plan allocates; use the literal or scan plan");
+    if (plan < FBH_PLAN_AUTO || plan > FBH_PLAN_SCAN) return fail(FBH_ERR_ARG, "bad plan");
+    int rc = ensure_init();
+    if (rc) return rc;
+    uint32_t* sc = noalloc_scratch_ptr();
+    if (!sc) return fail(FBH_ERR_DEVICE, "no scratch address for this device");
+    if (plan == FBH_PLAN_AUTO) {
+        plan = plan_heuristic(ix, lo, lop, hi, rop);
+        if (plan == FBH_PLAN_COMPRESSED) plan = FBH_PLAN_LITERAL;     // the same bits
+    }
+    if (out_words < (uint64_t)ix->groups + 1)
+        return fail(FBH_ERR_SIZE, "output buffer smaller than the literal result (groups + 1 words)");
+    const uint32_t nparts = plan == FBH_PLAN_SCAN ? fbk::scan_blocks(ix->groups, ix->anb) : ix->ntiles;
+    if (nparts > NOALLOC_PARTS)
+        return fail(FBH_ERR_SIZE, "index beyond the allocation-free limit of 65536 tiles (2^27 groups)");
+    uint32_t np = 0;
+    rc = index_eval_literal(ix, lo, lop, hi, rop, plan, dev_out, sc, &np, stream());
+    if (!rc) rc = check(fbk::launch_sum_parts_n(sc, np, sc + NOALLOC_PARTS, stream()), "sum parts");
+    uint32_t c = 0;
+    if (!rc) rc = check(hipMemcpyAsync(&c, sc + NOALLOC_PARTS, 4, hipMemcpyDeviceToHost, stream()), "count copy");
+    if (!rc) rc = stream_sync();
+    if (rc) return rc;
+    *hits = c;
+    if (plan_used) *plan_used = plan;
+    return FBH_OK;
+}
+
 int fbh_index_evaluate_batch(fbh_index* ix, uint32_t nq, const double* lo, const int* lop,
                              const double* hi, const int* rop, const int* plans, int plan,
                              uint64_t* hits, int* plans_used, float* gpu_ms) {

@@ -113,6 +113,8 @@ uint32_t scan_blocks(uint32_t groups, uint32_t anb);
 // counts[i] = sum of parts[i*stride, i*stride + np[i])
 hipError_t launch_sum_parts(const uint32_t* parts, uint32_t stride, const uint32_t* np, uint32_t nq,
                             uint32_t* counts, hipStream_t s);
+// *count = sum of parts[0 .. n) (one query, n given on the host)
+hipError_t launch_sum_parts_n(const uint32_t* parts, uint32_t n, uint32_t* count, hipStream_t s);
 hipError_t launch_scan_excl(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp,
                             hipStream_t s);
 uint32_t scan_tmp_words(uint32_t n);

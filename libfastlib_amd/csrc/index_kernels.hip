@@ -1222,13 +1222,13 @@ __global__ __launch_bounds__(IX_TPB) void k_query_scan(QueryArgs q, uint32_t nro
 }
 
 // counts[i] = sum of parts[i * stride .. i * stride + np[i]), one block per
-// query
+// query (np == nullptr: nfixed parts for every query)
 __global__ __launch_bounds__(IX_TPB) void k_sum_parts(const uint32_t* __restrict__ parts, uint32_t stride,
-                                                      const uint32_t* __restrict__ np,
+                                                      const uint32_t* __restrict__ np, uint32_t nfixed,
                                                       uint32_t* __restrict__ counts) {
     __shared__ uint32_t red[IX_TPB / WAVE];
     const uint32_t qi = blockIdx.x;
-    const uint32_t n = np[qi];
+    const uint32_t n = np ? np[qi] : nfixed;
     uint32_t t = 0;
     for (uint32_t k = threadIdx.x; k < n; k += IX_TPB) t += parts[(size_t)qi * stride + k];
     t = wave_sum(t);
@@ -1483,7 +1483,13 @@ uint32_t scan_blocks(uint32_t groups, uint32_t anb) {
 hipError_t launch_sum_parts(const uint32_t* parts, uint32_t stride, const uint32_t* np, uint32_t nq,
                             uint32_t* counts, hipStream_t s) {
     if (!nq) return hipSuccess;
-    hipLaunchKernelGGL(k_sum_parts, dim3(nq), dim3(IX_TPB), 0, s, parts, stride, np, counts);
+    hipLaunchKernelGGL(k_sum_parts, dim3(nq), dim3(IX_TPB), 0, s, parts, stride, np, 0u, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_parts_n(const uint32_t* parts, uint32_t n, uint32_t* count, hipStream_t s) {
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(IX_TPB), 0, s, parts, n, (const uint32_t*)nullptr, n,
+                       count);
     return hipGetLastError();
 }
 

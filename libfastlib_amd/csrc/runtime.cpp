@@ -14,6 +14,16 @@ static std::multimap<size_t, void*> g_free;        // rounded size -> buffer
 static std::map<void*, size_t> g_size;             // live buffer -> rounded size
 static thread_local std::string t_err;
 
+// Scratch of the allocation-free entry points: a file-scope device array of
+// NOALLOC_WORDS words that every GPU the library runs on holds until the
+// process ends.  In host code its name stands for a host-side shadow, not
+// for device memory, so its device address is looked up once per device
+// (at initialisation, with that device current) and only that address is
+// handed to kernels.
+static __device__ __align__(256) uint32_t noalloc_scratch[NOALLOC_WORDS];
+constexpr int MAX_DEVICES = 64;
+static uint32_t* g_noalloc[MAX_DEVICES] = {};
+
 std::recursive_mutex& api_mutex() {
     static std::recursive_mutex m;
     return m;
@@ -48,8 +58,18 @@ int ensure_init(int device) {
     for (hipStream_t& t : g_side)
         if ((e = hipStreamCreateWithFlags(&t, hipStreamNonBlocking)) != hipSuccess)
             return check(e, "hipStreamCreate");
+    if (device < MAX_DEVICES && !g_noalloc[device]) {
+        void* p = nullptr;
+        if ((e = hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch))) != hipSuccess)
+            return check(e, "hipGetSymbolAddress");
+        g_noalloc[device] = (uint32_t*)p;
+    }
     g_device = device;
     return FBH_OK;
+}
+
+uint32_t* noalloc_scratch_ptr() {
+    return (g_device >= 0 && g_device < MAX_DEVICES) ? g_noalloc[g_device] : nullptr;
 }
 
 hipStream_t stream() { return g_stream; }

@@ -18,6 +18,13 @@ hipStream_t stream();
 constexpr int NSIDE = 3;
 hipStream_t side_stream(int k);
 
+// Device address of the allocation-free entry points' scratch array on the
+// current device (NOALLOC_WORDS words, taken once per device by
+// ensure_init; nullptr before it).  Every call on a GPU shares it.
+constexpr uint32_t NOALLOC_PARTS = 65536;             // partial counts (tiles or scan blocks)
+constexpr uint32_t NOALLOC_WORDS = NOALLOC_PARTS + 64; // + the count
+uint32_t* noalloc_scratch_ptr();
+
 // Caching allocator.  Buffers are handed back to a free list keyed by the
 // rounded size and reused by later requests; all library work runs on one
 // stream, so reuse is ordered after the work that used the buffer.

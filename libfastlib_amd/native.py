@@ -263,6 +263,8 @@ def _index_sigs(L):
         "fbh_index_evaluate_batch": [vp, C.c_uint32, _dp, _ip, _dp, _ip, _ip, C.c_int, _u64p, _ip,
                                      C.POINTER(C.c_float)],
         "fbh_index_query_bytes": [vp, C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, _dp],
+        "fbh_index_evaluate_noalloc": [vp, C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, vp,
+                                       C.c_uint64, _u64p, C.POINTER(C.c_int)],
         "fbh_index_column_bytes": [vp, _u64p],
         "fbh_index_set_cost_model": [vp, C.c_int],
         "fbh_index_build_bin_nbins": [vp, C.c_int, C.c_uint32, C.c_uint32, C.POINTER(vp)],
@@ -448,6 +450,16 @@ class Index:
         check(L.fbh_index_evaluate_rids(self._h, lo, lop, hi, rop, plan, out.ctypes.data_as(_u32p),
                                         n.value, C.byref(n)), "fbh_index_evaluate_rids")
         return out[: n.value]
+
+    def evaluate_noalloc(self, lo, lop, hi, rop, plan, dev_out, out_words):
+        """fbh_index_evaluate_noalloc: the literal result into the device
+        buffer at address dev_out (out_words words); (hits, plan used)."""
+        hits = C.c_uint64()
+        used = C.c_int()
+        check(self._lib().fbh_index_evaluate_noalloc(self._h, lo, lop, hi, rop, int(plan), C.c_void_p(dev_out),
+                                                     int(out_words), C.byref(hits), C.byref(used)),
+              "fbh_index_evaluate_noalloc")
+        return hits.value, used.value
 
     def evaluate_batch(self, lo, lop, hi, rop, plan=PLAN_AUTO, plans=None):
         """Evaluate many ranges back to back on the device.
