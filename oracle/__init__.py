@@ -86,6 +86,21 @@ class _Lib:
         return self._call_out("wah_from_bits", b.ctypes.data_as(C.POINTER(C.c_uint8)),
                               C.c_size_t(b.size))
 
+    def count(self, a, mask):
+        """bitvector::count(mask): the ones of a AND mask."""
+        a, pa, na = _arr(a)
+        b, pb, nb = _arr(mask)
+        c = C.c_uint32()
+        rc = getattr(self.lib, self.prefix + "wah_count")(pa, na, pb, nb, C.byref(c))
+        if rc != 0:
+            raise RuntimeError(f"{self.prefix}wah_count failed rc={rc}")
+        return c.value
+
+    def erase(self, a, i, j):
+        """bitvector::erase(i, j): the words after removing bits [i, j)."""
+        a, p, n = _arr(a)
+        return self._call_out("wah_erase", p, n, C.c_uint32(i), C.c_uint32(j))
+
     def cnt(self, a):
         """Return (cnt, size, words after the call)."""
         a, p, n = _arr(a)
@@ -118,6 +133,19 @@ class Oracle(_Lib):
             raise RuntimeError(f"orc_wah_op failed rc={rc}")
         return self._take(out, n), PATH_NAMES.get(path.value, "?")
 
+    def op_new(self, a, b, op):
+        """The out-of-place operator (a new bitvector, not compressed);
+        returns (serialized words, path name)."""
+        a, pa, na = _arr(a)
+        b, pb, nb = _arr(b)
+        path = C.c_int(-1)
+        out = _u32p()
+        n = C.c_size_t()
+        rc = self.lib.orc_wah_op_new(pa, na, pb, nb, C.c_int(op), C.byref(out), C.byref(n), C.byref(path))
+        if rc != 0:
+            raise RuntimeError(f"orc_wah_op_new failed rc={rc}")
+        return self._take(out, n), PATH_NAMES.get(path.value, "?")
+
     def from_runs(self, first_bit, lens):
         lens = np.ascontiguousarray(lens, dtype=np.uint32)
         return self._call_out("wah_from_runs", C.c_int(int(first_bit)),
@@ -136,6 +164,11 @@ class Reference(_Lib):
         a, pa, na = _arr(a)
         b, pb, nb = _arr(b)
         return self._call_out("wah_op", pa, na, pb, nb, C.c_int(op), C.c_int(int(compress)))
+
+    def op_new(self, a, b, op):
+        a, pa, na = _arr(a)
+        b, pb, nb = _arr(b)
+        return self._call_out("wah_op_new", pa, na, pb, nb, C.c_int(op))
 
 
 _oracle = None

@@ -120,3 +120,47 @@ int orc_wah_literal(const uint32_t* a, size_t na, uint32_t** out, size_t* nout) 
 }
 
 } // extern "C"
+
+extern "C" {
+
+// The out-of-place operator a OP b (not compressed); *path receives the
+// routine (orc::Path).
+int orc_wah_op_new(const uint32_t* a, size_t na, const uint32_t* b, size_t nb, int op,
+                   uint32_t** out, size_t* nout, int* path) {
+    try {
+        BV x = BV::from_serial(a, na);
+        BV y = BV::from_serial(b, nb);
+        Path p = PATH_NONE;
+        BV r = x.op_new(y, (Op)op, &p);
+        if (path) *path = (int)p;
+        return hand_out(r.serial(), out, nout);
+    } catch (...) {
+        return -2;
+    }
+}
+
+// count(mask): the ones of a AND mask.
+int orc_wah_count(const uint32_t* a, size_t na, const uint32_t* b, size_t nb, uint32_t* cnt) {
+    try {
+        BV x = BV::from_serial(a, na);
+        BV y = BV::from_serial(b, nb);
+        x.size(); y.size();
+        *cnt = x.count_mask(y);
+        return 0;
+    } catch (...) {
+        return -2;
+    }
+}
+
+// erase(i, j)
+int orc_wah_erase(const uint32_t* a, size_t na, uint32_t i, uint32_t j, uint32_t** out, size_t* nout) {
+    try {
+        BV x = BV::from_serial(a, na);
+        x.erase(i, j);
+        return hand_out(x.serial(), out, nout);
+    } catch (...) {
+        return -2;
+    }
+}
+
+} // extern "C"

@@ -61,6 +61,64 @@ int ref_wah_op(const uint32_t* a, size_t na, const uint32_t* b, size_t nb,
     }
 }
 
+// The out-of-place operators a & b, a | b, a ^ b, a - b
+// (src/bitvector.cpp:1292, 1488, 1624, 1768): a new bitvector, not
+// compressed.
+int ref_wah_op_new(const uint32_t* a, size_t na, const uint32_t* b, size_t nb,
+                   int op, uint32_t** out, size_t* nout) {
+    ensure_init();
+    try {
+        ibis::bitvector x, y;
+        load(x, a, na);
+        load(y, b, nb);
+        ibis::bitvector* r = nullptr;
+        switch (op) {
+        case 0: r = x & y; break;
+        case 1: r = x | y; break;
+        case 2: r = x ^ y; break;
+        default: r = x - y; break;
+        }
+        ibis::array_t<ibis::bitvector::word_t> arr;
+        r->write(arr);
+        delete r;
+        return hand_out(arr.begin(), arr.size(), out, nout);
+    } catch (...) {
+        return -2;
+    }
+}
+
+// bitvector::count(mask) (src/bitvector.cpp:1088-1124)
+int ref_wah_count(const uint32_t* a, size_t na, const uint32_t* b, size_t nb, uint32_t* cnt) {
+    ensure_init();
+    try {
+        ibis::bitvector x, y;
+        load(x, a, na);
+        load(y, b, nb);
+        (void) x.size(); (void) y.size();
+        *cnt = x.count(y);
+        return 0;
+    } catch (...) {
+        return -2;
+    }
+}
+
+// bitvector::erase(i, j) (src/bitvector.cpp:1012-1081) of a vector whose size
+// has been counted (erase tests j against nbits as it stands).
+int ref_wah_erase(const uint32_t* a, size_t na, uint32_t i, uint32_t j, uint32_t** out, size_t* nout) {
+    ensure_init();
+    try {
+        ibis::bitvector x;
+        load(x, a, na);
+        (void) x.size();       // erase reads nbits without counting it
+        x.erase(i, j);
+        ibis::array_t<ibis::bitvector::word_t> arr;
+        x.write(arr);
+        return hand_out(arr.begin(), arr.size(), out, nout);
+    } catch (...) {
+        return -2;
+    }
+}
+
 int ref_wah_from_bits(const uint8_t* bits, size_t n, uint32_t** out, size_t* nout) {
     ensure_init();
     ibis::bitvector x;

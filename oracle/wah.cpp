@@ -795,3 +795,389 @@ void BV::rids(std::vector<word_t>& out) const {
 }
 
 } // namespace orc
+
+namespace orc {
+
+// ---------------------------------------------------------------------------
+// Out-of-place operators.
+
+// src/bitvector.cpp:2545-2610 (rhs is literal-only)
+void BV::and_c1(const BV& rhs, BV& res) const {
+    res.clear();
+    if (v.size() == 1) {
+        const word_t w = v[0];
+        if (w > HEADER1) { res.v = rhs.v; res.nbits = rhs.nbits; res.nset = rhs.nset; }
+        else if (w > ALLONES) { res.v = v; res.nbits = nbits; res.nset = 0; }
+        else { res.v.push_back(w & rhs.v[0]); res.nbits = nbits; }
+    } else if (v.size() > 1) {
+        size_t i1 = 0;
+        for (size_t i0 = 0; i0 < v.size(); ++i0) {
+            const word_t w = v[i0];
+            if (w > ALLONES) {
+                const word_t s0 = w & MAXCNT;
+                if (w < HEADER1) {
+                    if (s0 > 1) res.append_counter(0, s0);
+                    else res.append_active();
+                    i1 += s0;
+                } else {
+                    for (word_t k = 0; k < s0; ++k) res.v.push_back(rhs.v[i1++]);
+                    res.nbits += s0 * MAXBITS;
+                }
+            } else {
+                res.aval = w & rhs.v[i1];
+                res.append_active();
+                ++i1;
+            }
+        }
+        if (i1 != rhs.v.size()) throw std::runtime_error("and_c1 internal error");
+    }
+    res.aval = aval & rhs.aval;
+    res.anb = anb;
+}
+
+// src/bitvector.cpp:2981-3050 (rhs is literal-only)
+void BV::or_c1(const BV& rhs, BV& res) const {
+    res.clear();
+    if (v.size() == 1) {
+        const word_t w = v[0];
+        if (w > HEADER1) { res.v = v; res.nbits = nbits; res.nset = nbits; }
+        else if (w > ALLONES) { res.v = rhs.v; res.nbits = rhs.nbits; res.nset = rhs.nset; }
+        else { res.v.push_back(w | rhs.v[0]); res.nbits = nbits; }
+    } else if (v.size() > 1) {
+        size_t i1 = 0;
+        for (size_t i0 = 0; i0 < v.size(); ++i0) {
+            const word_t w = v[i0];
+            if (w > ALLONES) {
+                const word_t s0 = w & MAXCNT;
+                if (w >= HEADER1) {
+                    if (s0 > 1) res.append_counter(1, s0);
+                    else { res.aval = ALLONES; res.append_active(); }
+                    i1 += s0;
+                } else {
+                    for (word_t k = 0; k < s0; ++k) res.v.push_back(rhs.v[i1++]);
+                    res.nbits += s0 * MAXBITS;
+                }
+            } else {
+                res.aval = w | rhs.v[i1];
+                res.append_active();
+                ++i1;
+            }
+        }
+        if (i1 != rhs.v.size()) throw std::runtime_error("or_c1 internal error");
+    }
+    res.aval = aval | rhs.aval;
+    res.anb = anb;
+}
+
+// src/bitvector.cpp:3821-3869 (this is literal-only, rhs drives the loop)
+void BV::minus_c1x(const BV& rhs, BV& res) const {
+    res.clear();
+    size_t i0 = 0;
+    for (size_t i1 = 0; i1 < rhs.v.size(); ++i1) {
+        const word_t w = rhs.v[i1];
+        if (w > ALLONES) {
+            const word_t s0 = w & MAXCNT;
+            if (w >= HEADER1) {
+                i0 += s0;
+                if (s0 > 1) res.append_counter(0, s0);
+                else res.append_active();
+            } else {
+                for (word_t k = 0; k < s0; ++k) res.v.push_back(v[i0++]);
+                res.nbits += s0 * MAXBITS;
+            }
+        } else {
+            res.aval = v[i0] & ~w;
+            res.append_active();
+            ++i0;
+        }
+    }
+    if (i0 != v.size()) throw std::runtime_error("minus_c1x internal error");
+    res.aval = aval & ~(rhs.aval);
+    res.anb = anb;
+}
+
+// The operator's dispatch: operands of different sizes go through a copy of
+// the smaller one, extended, and the in-place operator (with the copy on
+// the left, except for MINUS); otherwise one routine writes a new vector.
+BV BV::op_new(const BV& rhs, Op op, Path* path) const {
+    BV res;
+    Path p = PATH_NONE;
+    if (size() > rhs.size()) {
+        if (op == OP_MINUS) {
+            BV tmp(rhs);
+            tmp.adjustSize(0, size());
+            res = *this;
+            p = res.op_inplace(tmp, op);
+        } else {
+            res = rhs;
+            res.adjustSize(0, size());
+            p = res.op_inplace(*this, op);
+        }
+        if (path) *path = p;
+        return res;
+    } else if (size() < rhs.size()) {
+        res = *this;
+        res.adjustSize(0, rhs.size());
+        p = res.op_inplace(rhs, op);
+        if (path) *path = p;
+        return res;
+    }
+    // MINUS tests literal-ness without nbits > 0 (src/bitvector.cpp:1788-1789)
+    const bool ca = (v.size() * MAXBITS == nbits) && (nbits > 0 || op == OP_MINUS);
+    const bool cb = (rhs.v.size() * MAXBITS == rhs.nbits) && (rhs.nbits > 0 || op == OP_MINUS);
+    const bool d2 = (v.size() + rhs.v.size()) * MAXBITS > nbits;
+    auto literal = [&](Path q) {
+        std::vector<word_t> la = literal_words(), lb = rhs.literal_words();
+        literal_op(la, lb, op, res.v);
+        res.nbits = nbits;
+        res.aval = op_word(aval, rhs.aval, op);
+        res.anb = anb;
+        p = q;
+    };
+    switch (op) {
+    case OP_AND:
+        if (ca && cb) literal(PATH_C0);
+        else if (ca) { rhs.and_c1(*this, res); p = PATH_C1; }
+        else if (cb) { and_c1(rhs, res); p = PATH_C1; }
+        else if (all0s() || rhs.all1s()) { res = *this; res.aval &= rhs.aval; p = PATH_SHORTCUT; }
+        else if (all1s() || rhs.all0s()) { res = rhs; res.aval &= aval; p = PATH_SHORTCUT; }
+        else if (d2) literal(PATH_D2);
+        else { and_c2(rhs, res); p = PATH_C2; }
+        break;
+    case OP_OR:
+        if (ca && cb) literal(PATH_C0);
+        else if (ca) { rhs.or_c1(*this, res); p = PATH_C1; }
+        else if (cb) { or_c1(rhs, res); p = PATH_C1; }
+        else if (all1s() || rhs.all0s()) { res = *this; res.aval |= rhs.aval; p = PATH_SHORTCUT; }
+        else if (all0s() || rhs.all1s()) { res = rhs; res.aval |= aval; p = PATH_SHORTCUT; }
+        else if (d2) literal(PATH_D2);
+        else { or_c2(rhs, res); p = PATH_C2; }
+        break;
+    case OP_XOR:
+        if (ca && cb) literal(PATH_C0);
+        else if (ca) { rhs.xor_c1(*this, res); p = PATH_C1; }
+        else if (cb) { xor_c1(rhs, res); p = PATH_C1; }
+        else if (d2) literal(PATH_D2);
+        else { xor_c2(rhs, res); p = PATH_C2; }
+        break;
+    default:
+        if (ca && cb) literal(PATH_C0);
+        else if (ca) { minus_c1x(rhs, res); p = PATH_C1; }
+        else if (cb) { minus_c1(rhs, res); p = PATH_C1; }
+        else if (all0s() || rhs.all0s()) { res = *this; res.aval &= ~rhs.aval; p = PATH_SHORTCUT; }
+        else if (rhs.all1s()) {
+            res.append_counter(0, rhs.v[0] & MAXCNT);
+            res.anb = anb;
+            res.aval = aval & ~rhs.aval;
+            p = PATH_SHORTCUT;
+        } else if (all1s()) { res = rhs; res.flip(); res.aval &= aval; p = PATH_SHORTCUT; }
+        else if (d2) literal(PATH_D2);
+        else { minus_c2(rhs, res); p = PATH_C2; }
+        break;
+    }
+    if (path) *path = p;
+    return res;
+}
+
+// ---------------------------------------------------------------------------
+// count(mask)
+
+word_t BV::count_mask(const BV& mask) const {
+    word_t cnt = 0;
+    const bool ca = (v.size() * MAXBITS == nbits && nbits > 0);
+    const bool cb = (mask.v.size() * MAXBITS == mask.nbits && mask.nbits > 0);
+    if (ca && cb) {
+        for (size_t j = 0; j < v.size(); ++j) cnt += cnt_ones(v[j] & mask.v[j]);
+        cnt += cnt_ones(aval & mask.aval);
+    } else if (ca) {
+        cnt = mask.count_c1(*this);
+    } else if (cb) {
+        cnt = count_c1(mask);
+    } else if (all0s() || mask.all0s()) {
+        cnt = cnt_ones(aval & mask.aval);
+    } else if (all1s()) {
+        if (mask.nset == 0 && !mask.v.empty()) mask.nbits = mask.do_cnt();
+        cnt = mask.nset + cnt_ones(aval & mask.aval);
+    } else if (mask.all1s()) {
+        if (nbits == 0 && !v.empty()) nbits = do_cnt();
+        cnt = nset + cnt_ones(aval & mask.aval);
+    } else {
+        cnt = count_c2(mask);
+    }
+    return cnt;
+}
+
+// src/bitvector.cpp:2317-2387
+word_t BV::count_c2(const BV& mask) const {
+    word_t cnt = cnt_ones(aval & mask.aval);
+    Run x, y;
+    x.it = v.data();
+    y.it = mask.v.data();
+    const word_t* xe = v.data() + v.size();
+    const word_t* ye = mask.v.data() + mask.v.size();
+    while (x.it < xe && y.it < ye) {
+        if (x.nWords == 0) x.decode();
+        if (y.nWords == 0) y.decode();
+        if (x.isFill != 0) {
+            if (y.isFill != 0) {
+                if (x.fillBit != 0 && y.fillBit != 0) {
+                    if (x.nWords <= y.nWords) {
+                        cnt += MAXBITS * x.nWords;
+                        y.nWords -= x.nWords; x.nWords = 0; ++x.it;
+                        y.it += (y.nWords == 0);
+                    } else {
+                        cnt += MAXBITS * y.nWords;
+                        x.nWords -= y.nWords; y.nWords = 0; ++y.it;
+                    }
+                } else if (x.nWords <= y.nWords) {
+                    y.nWords -= x.nWords; y.it += (y.nWords == 0);
+                    x.nWords = 0; ++x.it;
+                } else {
+                    x.nWords -= y.nWords; y.nWords = 0; ++y.it;
+                }
+            } else {
+                if (x.fillBit != 0) cnt += cnt_ones(*y.it);
+                ++y.it; --x.nWords; y.nWords = 0;
+                x.it += (x.nWords == 0);
+            }
+        } else if (y.isFill != 0) {
+            if (y.fillBit != 0) cnt += cnt_ones(*x.it);
+            x.nWords = 0; --y.nWords; ++x.it;
+            y.it += (y.nWords == 0);
+        } else {
+            cnt += cnt_ones(*x.it & *y.it);
+            x.nWords = 0; y.nWords = 0; ++x.it; ++y.it;
+        }
+    }
+    return cnt;
+}
+
+// src/bitvector.cpp:2390-2416 (mask is literal-only)
+word_t BV::count_c1(const BV& mask) const {
+    word_t cnt = cnt_ones(aval & mask.aval);
+    size_t it = 0;
+    for (size_t i = 0; i < v.size() && it < mask.v.size(); ++i) {
+        const word_t w = v[i];
+        if (w > ALLONES) {
+            word_t n = w & MAXCNT;
+            if (w > HEADER1) { while (n > 0) { cnt += cnt_ones(mask.v[it]); ++it; --n; } }
+            else it += n;
+        } else {
+            cnt += cnt_ones(mask.v[it] & w);
+            ++it;
+        }
+    }
+    return cnt;
+}
+
+// ---------------------------------------------------------------------------
+// erase
+
+// src/bitvector.cpp:280-331
+void BV::appendWord(word_t w) {
+    const int cps = (int)(w >> MAXBITS);
+    nset = 0;
+    if (anb) {
+        const word_t nb1 = anb;
+        word_t nb2 = MAXBITS - anb;
+        aval <<= nb2;
+        if (cps != 0) {
+            const int b2 = (w >= HEADER1);
+            if (b2 != 0) aval |= (1U << nb2) - 1;
+            append_active();
+            nb2 = (w & MAXCNT) - 1;
+            if (nb2 > 1) append_counter(b2, nb2);
+            else if (nb2 == 1) { if (b2 != 0) aval = ALLONES; append_active(); }
+            anb = nb1;
+            aval = ((1U << nb1) - 1) * b2;
+        } else {
+            aval |= (w >> nb1);
+            append_active();
+            aval = (w & ((1U << nb1) - 1));
+            anb = nb1;
+        }
+    } else if (cps != 0) {
+        const int b2 = (w >= HEADER1);
+        const word_t n = w & MAXCNT;
+        if (n > 1) append_counter(b2, n);
+        else if (n == 1) { if (b2) aval = ALLONES; append_active(); }
+    } else {
+        aval = w;
+        append_active();
+    }
+}
+
+namespace {
+// The bit iterator of src/bitvector.cpp:4437-4518 (forward steps only):
+// the code word holding a position, the offset in it, and the word's kind.
+struct BitPos {
+    const BV* bv;
+    size_t it;              // index into m_vec; m_vec.size() = the active word
+    word_t ind = 0;         // offset of the position in the word
+    word_t nb = 0;          // bits in the word
+    int compressed = 0, fillbit = 0;
+    word_t literal = 0;     // literal value (the active word left-aligned)
+    void decode() {
+        const size_t n = bv->v.size();
+        if (it < n) {
+            const word_t w = bv->v[it];
+            if (w > HEADER1) { fillbit = 1; compressed = 1; nb = (w & MAXCNT) * MAXBITS; }
+            else if (w > HEADER0) { fillbit = 0; compressed = 1; nb = (w & MAXCNT) * MAXBITS; }
+            else { compressed = 0; nb = MAXBITS; literal = w; }
+        } else if (it == n) {
+            compressed = 0; nb = bv->anb;
+            literal = bv->aval << (MAXBITS - nb);
+            it += (nb == 0);
+        } else {
+            it = n + 1; compressed = 0; nb = 0; literal = 0; fillbit = 0;
+        }
+        ind = 0;
+    }
+    void advance(word_t incr) {
+        if (incr == 0) return;
+        if (ind + incr < nb) { ind += incr; return; }
+        long long rest = (long long)incr + ind - nb;
+        while (rest >= 0 && it < bv->v.size()) {
+            ++it;
+            decode();
+            if ((long long)nb > rest) { ind = (word_t)rest; rest = -1; }
+            else rest -= nb;
+        }
+    }
+};
+} // namespace
+
+// src/bitvector.cpp:1012-1081
+// The size is counted first (do_cnt, with its collapse of an all-zero
+// vector), as erase reads nbits as it stands.
+void BV::erase(word_t i, word_t j) {
+    size();
+    if (i >= j) return;
+    BV res;
+    if (i > 0) {
+        BitPos ip{this, 0};
+        ip.decode();
+        ip.advance(i);
+        for (size_t c = 0; c < ip.it && c < v.size(); ++c) res.v.push_back(v[c]);
+        res.nbits = i - ip.ind;
+        if (ip.compressed) res.appendFill(ip.fillbit, ip.ind);
+        else { res.aval = ip.literal >> (MAXBITS - ip.ind); res.anb = ip.ind; }
+    }
+    if (j < nbits) {
+        BitPos iq{this, 0};
+        iq.decode();
+        iq.advance(j);
+        if (iq.compressed) {
+            res.appendFill(iq.fillbit, iq.nb - iq.ind);
+        } else {
+            for (long ii = (long)(iq.nb - iq.ind - 1); ii >= 0; --ii) res.push_bit(1 & (iq.literal >> ii));
+        }
+        for (size_t k = iq.it + 1; k < v.size(); ++k) res.appendWord(v[k]);
+        for (long ii = (long)anb - 1; ii >= 0; --ii) res.push_bit((int)((aval >> ii) & 1));
+    } else if (j < nbits + anb) {
+        for (long ii = (long)(anb - j + nbits - 1); ii >= 0; --ii) res.push_bit((int)((aval >> ii) & 1));
+    }
+    *this = res;
+}
+
+} // namespace orc

@@ -118,6 +118,23 @@ struct BV {
 
     // --- rid extraction (src/bitvector.cpp:4524-4590, ascending positions)
     void rids(std::vector<word_t>& out) const;
+
+    // --- out-of-place operators: a new bitvector, not compressed
+    // (src/bitvector.cpp:1292-1407 &, 1488-1551 |, 1624-1679 ^, 1768-1838 -)
+    BV op_new(const BV& rhs, Op op, Path* path) const;
+    void and_c1(const BV& rhs, BV& res) const;           // src/bitvector.cpp:2545-2610
+    void or_c1(const BV& rhs, BV& res) const;            // src/bitvector.cpp:2981-3050
+    void minus_c1x(const BV& rhs, BV& res) const;        // src/bitvector.cpp:3821-3869
+
+    // --- ones of this AND mask without the result (src/bitvector.cpp:1088-1124)
+    word_t count_mask(const BV& mask) const;
+    word_t count_c1(const BV& mask) const;               // src/bitvector.cpp:2390-2416
+    word_t count_c2(const BV& mask) const;               // src/bitvector.cpp:2317-2387
+
+    // --- erase bits [i, j) (src/bitvector.cpp:1012-1081) and the word
+    // append it uses (src/bitvector.cpp:280-331)
+    void erase(word_t i, word_t j);
+    void appendWord(word_t w);
 };
 
 // Literal-array result of one op, as *_c0 / *_d1 / *_d2 produce it.

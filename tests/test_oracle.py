@@ -169,3 +169,69 @@ def test_oracle_matches_config2_digests_small(orc):
         assert path == r["path"]
         checked += 1
     assert checked > 0
+
+
+def _bits_of(orc, words, n):
+    bits = np.zeros(n, np.uint8)
+    bits[orc.rids(words)] = 1
+    return bits
+
+
+def _operand_forms(orc, a, b):
+    """a in its canonical, decompressed and uncompressed-c2-output forms."""
+    return [a, orc.decompress(a), orc.op(a, b, 0, compress=False)[0], orc.op(a, b, 1, compress=False)[0]]
+
+
+def test_op_new_bits_and_count_brute_force(orc):
+    """Out-of-place results decode to the op's bits (sizes equal or not: the
+    shorter operand is extended with zeros); count(mask) is the popcount of
+    the AND; erase removes exactly the bits [i, j)."""
+    rng = np.random.default_rng(21)
+    fns = [np.bitwise_and, np.bitwise_or, np.bitwise_xor, lambda x, y: x & (1 - y)]
+    for t in range(60):
+        n = int(rng.integers(1, 2500))
+        m = n if t % 3 else max(1, n + int(rng.integers(-70, 70)))
+        ba, bb = markov_bits(n, 0.2, 20, rng), markov_bits(m, 0.6, 5, rng)
+        a, b = orc.from_bits(ba), orc.from_bits(bb)
+        # forms 2 and 3 are a AND c / a OR c, uncompressed: their own bits
+        x = _operand_forms(orc, a, orc.from_bits(markov_bits(n, 0.3, 3, rng)))[t % 4]
+        ba = _bits_of(orc, x, n)
+        k = max(n, m)
+        ea, eb = np.zeros(k, np.uint8), np.zeros(k, np.uint8)
+        ea[:n], eb[:m] = ba, bb
+        for op in range(4):
+            r, _ = orc.op_new(x, b, op)
+            assert np.array_equal(_bits_of(orc, r, k), fns[op](ea, eb).astype(np.uint8)), (t, op)
+        if n == m:
+            assert orc.count(x, b) == int(np.sum(ba & bb))
+        i = int(rng.integers(0, n))
+        j = int(rng.integers(i, n + 40))
+        e = orc.erase(x, i, j)
+        want = np.concatenate([ba[:i], ba[j:]])
+        assert orc.cnt(e)[1] == len(want) and np.array_equal(_bits_of(orc, e, len(want)), want), (t, i, j)
+
+
+def test_op_new_count_erase_match_reference(orc, ref):
+    """The out-of-place operators (every routine, including the c1 loops and
+    unequal sizes), count(mask) and erase, word for word against the
+    reference on canonical, decompressed and uncompressed operands."""
+    rng = np.random.default_rng(22)
+    paths = set()
+    for t in range(300):
+        a, b, n = random_pair(rng, orc, 1, 3000)
+        if t % 5 == 0 and n > 40:
+            b = orc.from_bits(markov_bits(n + int(rng.integers(-40, 40)), 0.2, 5, rng))
+        va, vb = _operand_forms(orc, a, b), _operand_forms(orc, b, a)
+        x, y = va[int(rng.integers(0, 4))], vb[int(rng.integers(0, 4))]
+        for op in range(4):
+            w, p = orc.op_new(x, y, op)
+            paths.add(p)
+            assert np.array_equal(w, ref.op_new(x, y, op)), (t, op, p)
+        if orc.cnt(x)[1] == orc.cnt(y)[1]:
+            assert orc.count(x, y) == ref.count(x, y), t
+        sz = orc.cnt(x)[1]
+        if sz > 1:
+            i = int(rng.integers(0, sz))
+            j = int(rng.integers(i, sz + 40))
+            assert np.array_equal(orc.erase(x, i, j), ref.erase(x, i, j)), (t, i, j)
+    assert {"c0", "c1", "c2", "d2", "shortcut"} <= paths
