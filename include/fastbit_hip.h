@@ -88,11 +88,28 @@ int fbh_bv_write(fbh_bitvector* bv, uint32_t* arr, uint32_t cap, uint32_t* n);
  * The routine chosen by the reference's dispatch decides the result words;
  * *path (may be NULL) receives it (FBH_PATH_*). */
 int fbh_bv_op(fbh_bitvector* a, const fbh_bitvector* b, int op, int* path);
-/* Out-of-place form: *out = new bitvector equal to (copy of a) op= b,
- * compressed.  The caller frees *out (as operator& returns a new
- * bitvector that the caller deletes, src/bitvector.cpp:1286-1291). */
+/* The out-of-place operators a & b, a | b, a ^ b, a - b
+ * (src/bitvector.cpp:1292-1407, 1488-1551, 1624-1679, 1768-1838): *out is a
+ * new bitvector (the caller frees it, as the caller deletes the one
+ * operator& returns) holding the words the reference's own dispatch writes,
+ * not compressed: literal arrays from the c0 / d2 routines (the d2 test is
+ * the strict (|a|w + |b|w) * 31 > nbits), the c1 loops' words when one
+ * operand is literal, the c2 loops' words, or a copy of one operand for the
+ * all0s / all1s shortcuts.  Operands of different sizes go through a copy
+ * of the shorter one extended with zeros and the in-place operator, as the
+ * reference does.  *path (may be NULL) receives the routine. */
 int fbh_bv_op_new(const fbh_bitvector* a, const fbh_bitvector* b, int op,
                   fbh_bitvector** out, int* path);
+/* count(mask) -- src/bitvector.cpp:1088-1124: the ones of a AND mask,
+ * without forming the result.  The operands must have the same size
+ * (FBH_ERR_SIZE otherwise; the reference does not check it). */
+int fbh_bv_count(fbh_bitvector* a, fbh_bitvector* mask, uint32_t* cnt);
+/* erase(i, j) -- src/bitvector.cpp:1012-1081: removes bits [i, j) (j is
+ * clamped to the size; i >= j does nothing; i past the end is an
+ * FBH_ERR_ARG error).  The size is counted first (the reference tests j
+ * against nbits as it stands), so the result is the reference's for a
+ * vector whose size() was taken. */
+int fbh_bv_erase(fbh_bitvector* bv, uint32_t i, uint32_t j);
 
 int fbh_bv_compress(fbh_bitvector* bv);   /* src/bitvector.cpp:378-469 */
 int fbh_bv_decompress(fbh_bitvector* bv); /* src/bitvector.cpp:474-512 */
@@ -232,6 +249,19 @@ int fbh_index_evaluate(fbh_index* ix, double lo, int lop, double hi, int rop, in
  * src/countQuery.cpp:387-418). */
 int fbh_index_evaluate_rids(fbh_index* ix, double lo, int lop, double hi, int rop, int plan,
                             uint32_t* rids, uint64_t cap, uint64_t* n);
+/* index::estimate(expr, lower, upper) (src/index.h:191-194): rows of *lower
+ * are hits, rows of *upper candidates; an *upper of size 0 (the reference's
+ * cleared bitvector) stands for *lower itself.  relic: lower is evaluate's
+ * result and upper is cleared (src/irelic.h:58-62).  bin
+ * (src/ibin.cpp:6722-6829): lower = the hit bins, upper = lower and the
+ * edge bins; no hit bin: lower empty, upper the candidate bins; a range
+ * whose bitmap bytes plus edge-bin values cost more than 0.75 bytes per row
+ * gives lower = none, upper = every row.  The cost reads the bitmap sizes as
+ * the reference's offset64 holds them once sumBins has run (serialized
+ * sizes for an index that came with offsets, else bitvector::bytes()).
+ * Both bitvectors are new (canonical words); the caller frees them. */
+int fbh_index_estimate(fbh_index* ix, double lo, int lop, double hi, int rop, fbh_bitvector** lower,
+                       fbh_bitvector** upper);
 /* No counterpart in the reference: fbh_index_evaluate for callers that
  * cannot allocate device memory (code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc).  It calls no hipMalloc,

@@ -55,6 +55,14 @@ public:
     void adjustSize(uint32_t nv, uint32_t nt) { check(fbh_bv_adjust_size(h_, nv, nt), "fbh_bv_adjust_size"); }
     uint32_t cnt() const { uint32_t c = 0; check(fbh_bv_cnt(h_, &c), "fbh_bv_cnt"); return c; }
     uint32_t size() const { uint32_t n = 0; check(fbh_bv_size(h_, &n), "fbh_bv_size"); return n; }
+    /* count(mask), src/bitvector.cpp:1088-1124: ones of *this AND mask */
+    uint32_t count(const bitvector& mask) const {
+        uint32_t c = 0;
+        check(fbh_bv_count(h_, mask.h_, &c), "fbh_bv_count");
+        return c;
+    }
+    /* erase(i, j), src/bitvector.cpp:1012-1081: removes bits [i, j) */
+    void erase(uint32_t i, uint32_t j) { check(fbh_bv_erase(h_, i, j), "fbh_bv_erase"); }
     /* the routine the last operator took (FBH_PATH_*) */
     int lastPath() const { return path_; }
     void write(std::vector<uint32_t>& out) const {
@@ -92,8 +100,9 @@ private:
     }
 };
 
-/* out-of-place operators: a new bitvector (the reference returns one the
- * caller deletes, src/bitvector.cpp:1286-1291) */
+/* out-of-place operators: a new bitvector holding the words the
+ * reference's out-of-place dispatch writes, not compressed (the reference
+ * returns one the caller deletes, src/bitvector.cpp:1286-1291) */
 inline bitvector operator&(const bitvector& a, const bitvector& b) { return bitvector::op_new(a, b, FBH_AND); }
 inline bitvector operator|(const bitvector& a, const bitvector& b) { return bitvector::op_new(a, b, FBH_OR); }
 inline bitvector operator^(const bitvector& a, const bitvector& b) { return bitvector::op_new(a, b, FBH_XOR); }

@@ -21,7 +21,8 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-command-line-argument", "-I" + os.path.join(ROOT, "include")]
 
-SOURCES = ["wah_kernels.hip", "canon_kernels.hip", "index_kernels.hip", "runtime.cpp", "bitvector_api.cpp",
+SOURCES = ["wah_kernels.hip", "canon_kernels.hip", "index_kernels.hip", "aux_kernels.hip", "runtime.cpp",
+           "bitvector_api.cpp",
            "index_api.cpp", "bin_bounds.cpp", "mlp_kernels.hip", "planner_api.cpp"]
 
 

@@ -15,6 +15,8 @@
 #include "runtime.h"
 #include "launch.h"
 #include "bitvector_impl.h"
+#include "index_dev.h"
+#include "wah_core.h"
 #include <cstring>
 #include <vector>
 #include <string>
@@ -313,7 +315,7 @@ int bv_compress(fbh_bitvector* bv) {
 // Canonical WAH (or the exact c2 words) of a OP b into `res` (a, b have
 // equal group counts; `groups` > 0).
 int launch_word_op(fbh_bitvector* a, fbh_bitvector* b, int op, bool c2, uint32_t groups,
-                   OutBuf& ob, uint32_t** ctl_out) {
+                   OutBuf& ob, uint32_t** ctl_out, int c2_mode = 0) {
     if (groups == 0 && (a->nw > 1 || b->nw > 1))
         return fail(FBH_ERR_INTERNAL, "operand sizes not counted before the word op");
     const bool la = a->nw == groups;
@@ -326,7 +328,7 @@ int launch_word_op(fbh_bitvector* a, fbh_bitvector* b, int op, bool c2, uint32_t
         uint8_t* sc = (uint8_t*)scratch(0).get(64 + 8ull * nt + 64);
         if (!sc) return fail(FBH_ERR_NOMEM, "scratch allocation failed");
         *ctl_out = (uint32_t*)sc;
-        return check(fbk::launch_op(a->d, a->ck, a->nw, b->d, b->ck, b->nw, groups, op, 0,
+        return check(fbk::launch_op(a->d, a->ck, a->nw, b->d, b->ck, b->nw, groups, op, c2_mode,
                                     ob.d, ob.ck, (uint64_t*)(sc + 64), *ctl_out, stream()), "op");
     }
     const uint32_t nt = fbk::canon_tiles(groups);
@@ -341,7 +343,7 @@ int launch_word_op(fbh_bitvector* a, fbh_bitvector* b, int op, bool c2, uint32_t
 }
 
 int run_word_op(const fbh_bitvector* a, const fbh_bitvector* b, int op, bool c2,
-                fbh_bitvector* res, uint32_t groups) {
+                fbh_bitvector* res, uint32_t groups, int c2_mode = 0) {
     OutBuf ob;
     // c2 emits at most one word per consumed code word; a canonical result
     // has at most one word per group
@@ -350,7 +352,7 @@ int run_word_op(const fbh_bitvector* a, const fbh_bitvector* b, int op, bool c2,
     if (rc) return rc;
     uint32_t* ctl = nullptr;
     rc = launch_word_op(const_cast<fbh_bitvector*>(a), const_cast<fbh_bitvector*>(b), op, c2,
-                        groups, ob, &ctl);
+                        groups, ob, &ctl, c2_mode);
     if (rc) { dev_free(ob.d); dev_free(ob.ck); return rc; }
     return finish_words(res, ob, ctl, nullptr);
 }
@@ -482,6 +484,398 @@ int bv_op(fbh_bitvector* a, const fbh_bitvector* b_in, int op, int* path_out) {
     return FBH_OK;
 }
 
+
+// ---------------------------------------------------------------------------
+// Raw results: the words a routine writes, not compressed (the out-of-place
+// operators, and the in-place operator they fall back on for operands of
+// different sizes).
+
+// Installs d[0, nw) (capacity cap) as the handle's words.
+int install_words(fbh_bitvector* bv, uint32_t* d, uint32_t cap, uint32_t nw) {
+    uint32_t e[2] = {0, 0};
+    int rc = FBH_OK;
+    if (nw) {
+        rc = check(hipMemcpyAsync(&e[0], d, 4, hipMemcpyDeviceToHost, stream()), "edge copy");
+        if (!rc) rc = check(hipMemcpyAsync(&e[1], d + nw - 1, 4, hipMemcpyDeviceToHost, stream()), "edge copy");
+        if (!rc) rc = check(hipStreamSynchronize(stream()), "edge sync");
+        if (rc) { dev_free(d); return rc; }
+    }
+    dev_free(bv->d); dev_free(bv->ck);
+    bv->d = d; bv->cap = cap; bv->nw = nw; bv->ck = nullptr; bv->ckcap = 0; bv->ck_ok = false;
+    bv->w0 = e[0]; bv->wl = e[1];
+    return FBH_OK;
+}
+
+// One literal word per group of a handle (groups words, then `extra` zero
+// words), in a new device buffer.
+int literal_groups(fbh_bitvector* bv, uint32_t groups, uint32_t extra, uint32_t** out) {
+    uint32_t* d = (uint32_t*)dev_alloc(4ull * (groups + extra + 1));
+    if (!d) return fail(FBH_ERR_NOMEM, "device allocation failed");
+    int rc = FBH_OK;
+    if (extra) rc = check(fbk::launch_zero_u32(d + groups, extra, stream()), "zero");
+    if (!rc && groups) {
+        if (bv->nw == groups) {
+            rc = check(hipMemcpyAsync(d, bv->d, 4ull * groups, hipMemcpyDeviceToDevice, stream()), "copy");
+        } else {
+            rc = bv_ensure_ck(bv);
+            if (!rc) rc = check(fbk::launch_decompress(bv->d, bv->ck, bv->nw, groups, d, 0, stream()), "decompress");
+        }
+    }
+    if (rc) { dev_free(d); return rc; }
+    *out = d;
+    return FBH_OK;
+}
+
+// a := a OP b as a literal array (*_c0, *_d1, *_d2).
+int raw_literal(fbh_bitvector* a, fbh_bitvector* b, int op, uint32_t groups) {
+    uint32_t *la = nullptr, *lb = nullptr;
+    int rc = literal_groups(a, groups, 0, &la);
+    if (!rc) rc = literal_groups(b, groups, 0, &lb);
+    if (!rc) rc = check(fbk::launch_lit_op(la, lb, groups, op, la, stream()), "literal op");
+    dev_free(lb);
+    if (rc) { dev_free(la); return rc; }
+    return install_words(a, la, groups + 1, groups);
+}
+
+// The words of a *_c1 loop (x compressed, y literal, both of `groups`
+// groups) into res: per group value and word start, then the words.
+int raw_c1(fbh_bitvector* x, fbh_bitvector* y, int rk, uint32_t groups, fbh_bitvector* res) {
+    int rc = bv_ensure_ck(x);
+    if (rc) return rc;
+    uint32_t* V = (uint32_t*)dev_alloc(4ull * groups);
+    uint32_t* H = (uint32_t*)dev_alloc(4ull * groups);
+    uint32_t* O = (uint32_t*)dev_alloc(4ull * (groups + fbk::scan_tmp_words(groups)));
+    if (!V || !H || !O) { dev_free(V); dev_free(H); dev_free(O); return fail(FBH_ERR_NOMEM, "device allocation failed"); }
+    rc = check(fbk::launch_c1_heads(x->d, x->ck, x->nw, y->d, groups, rk, V, H, stream()), "c1 heads");
+    uint32_t* tmp = (uint32_t*)dev_alloc(4ull * (fbk::scan_tmp_words(groups) + 4));
+    if (!tmp && !rc) rc = fail(FBH_ERR_NOMEM, "device allocation failed");
+    if (!rc) rc = check(fbk::launch_scan_excl(H, O, groups, tmp, stream()), "scan");
+    uint32_t last[2] = {0, 0};
+    if (!rc) rc = check(hipMemcpyAsync(&last[0], O + groups - 1, 4, hipMemcpyDeviceToHost, stream()), "copy");
+    if (!rc) rc = check(hipMemcpyAsync(&last[1], H + groups - 1, 4, hipMemcpyDeviceToHost, stream()), "copy");
+    if (!rc) rc = check(hipStreamSynchronize(stream()), "sync");
+    const uint32_t nw = last[0] + last[1];
+    uint32_t* out = nullptr;
+    uint32_t* HP = nullptr;
+    if (!rc) {
+        out = (uint32_t*)dev_alloc(4ull * (nw + 1));
+        HP = (uint32_t*)dev_alloc(4ull * (nw + 1));
+        if (!out || !HP) rc = fail(FBH_ERR_NOMEM, "device allocation failed");
+    }
+    if (!rc) rc = check(fbk::launch_head_words(V, H, O, groups, nw, HP, out, stream()), "c1 words");
+    if (!rc) rc = check(hipStreamSynchronize(stream()), "sync");
+    dev_free(V); dev_free(H); dev_free(O); dev_free(tmp); dev_free(HP);
+    if (rc) { dev_free(out); return rc; }
+    return install_words(res, out, nw + 1, nw);
+}
+
+// The *_c1 routine `rk` with x compressed and y literal into res (x and y
+// of `groups` groups), including the one-word cases of and_c1 / or_c1 /
+// minus_c1 (src/bitvector.cpp:2548-2563, 2984-2999, 3751-3769), which copy
+// one operand's words.
+int raw_c1_routine(fbh_bitvector* x, fbh_bitvector* y, int rk, uint32_t groups, fbh_bitvector* res) {
+    if (x->nw == 1 && (rk == fbk::C1_AND || rk == fbk::C1_OR || rk == fbk::C1_MINUS)) {
+        const uint32_t w = x->w0;
+        const bool one = w > HEADER1, zero = w > ALLONES && !one;
+        fbh_bitvector* src = nullptr;
+        bool flip = false;
+        if (rk == fbk::C1_AND) src = one ? y : x;
+        else if (rk == fbk::C1_OR) src = zero ? y : x;
+        else { src = one ? y : x; flip = one; }
+        if (w <= ALLONES) {
+            // one literal word each (one group)
+            const uint32_t yv = y->w0;
+            const uint32_t v = rk == fbk::C1_AND ? (w & yv) : rk == fbk::C1_OR ? (w | yv) : (w & ~yv & ALLONES);
+            uint32_t* d = (uint32_t*)dev_alloc(8);
+            if (!d) return fail(FBH_ERR_NOMEM, "device allocation failed");
+            int rc = check(hipMemcpyAsync(d, &v, 4, hipMemcpyHostToDevice, stream()), "upload");
+            if (!rc) rc = check(hipStreamSynchronize(stream()), "sync");
+            if (rc) { dev_free(d); return rc; }
+            return install_words(res, d, 2, 1);
+        }
+        int rc = src == res ? FBH_OK : bv_copy_words(res, src);
+        if (!rc && flip) {
+            rc = check(fbk::launch_flip(res->d, res->nw, stream()), "flip");
+            res->w0 ^= ALLONES; res->wl ^= ALLONES;      // literal words
+        }
+        return rc;
+    }
+    return raw_c1(x, y, rk, groups, res);
+}
+
+// a OP= b without compress(): the in-place dispatch (src/bitvector.cpp:
+// 1194-1762) when `inplace`, else the out-of-place operator's
+// (src/bitvector.cpp:1292-1407, 1488-1551, 1624-1679, 1768-1838), for
+// operands of equal size; the result replaces a's words.
+int bv_op_raw(fbh_bitvector* a, fbh_bitvector* b, int op, bool inplace, int* path_out) {
+    const uint32_t groups = a->nbits / MAXBITS;
+    const bool lit_a = a->nw * MAXBITS == a->nbits, lit_b = b->nw * MAXBITS == b->nbits;
+    const bool nz = !(inplace == false && op == FBH_MINUS);        // out-of-place MINUS: no nbits > 0
+    const bool ca = lit_a && (a->nbits > 0 || !nz);
+    const bool cb = lit_b && (b->nbits > 0 || !nz);
+    const uint32_t dsum = (a->nw + b->nw) * MAXBITS;
+    const bool d2 = inplace ? dsum >= b->nbits : dsum > a->nbits;
+    const uint32_t act = active_op(a->aval, b->aval, op);
+    int path = 0, rc = FBH_OK;
+    auto literal = [&](int p) {
+        path = p;
+        if (groups) rc = raw_literal(a, b, op, groups);
+        a->aval = act; a->nset = 0;
+    };
+    auto c1 = [&](fbh_bitvector* x, fbh_bitvector* y, int rk) {
+        path = FBH_PATH_C1;
+        if (groups) rc = raw_c1_routine(x, y, rk, groups, a);
+        a->aval = act; a->nset = 0;
+    };
+    auto c2 = [&]() {
+        path = FBH_PATH_C2;
+        if (groups) rc = run_word_op(a, b, op, true, a, groups, (int)wah::MODE_RAW);
+        a->aval = act; a->nset = 0;
+    };
+    auto copy_b = [&]() {
+        path = FBH_PATH_SHORTCUT;
+        rc = bv_copy_words(a, b);
+        a->nset = b->nset; a->aval = act;
+    };
+    const int rk_x = op == FBH_AND ? fbk::C1_AND : op == FBH_OR ? fbk::C1_OR : op == FBH_XOR ? fbk::C1_XOR
+                                                                                       : fbk::C1_MINUSX;
+    const int rk_y = op == FBH_MINUS ? fbk::C1_MINUS : rk_x;
+    switch (op) {
+    case FBH_AND:
+    case FBH_OR: {
+        const bool and_ = op == FBH_AND;
+        if (ca && cb) literal(FBH_PATH_C0);
+        else if (ca) { if (inplace) literal(FBH_PATH_D1); else c1(b, a, rk_x); }
+        else if (cb) { if (inplace) literal(FBH_PATH_D1SWAP); else c1(a, b, rk_y); }
+        else if (and_ ? (all0s(a) || all1s(b)) : (all1s(a) || all0s(b))) { path = FBH_PATH_SHORTCUT; a->aval = act; }
+        else if (and_ ? (all0s(b) || all1s(a)) : (all0s(a) || all1s(b))) copy_b();
+        else if (d2) literal(FBH_PATH_D2);
+        else c2();
+        break;
+    }
+    case FBH_XOR:
+        if (ca && cb) literal(FBH_PATH_C0);
+        else if (ca) { if (inplace) literal(FBH_PATH_D1); else c1(b, a, rk_x); }
+        else if (cb) c1(a, b, rk_y);
+        else if (d2) literal(FBH_PATH_D2);
+        else c2();
+        break;
+    default:
+        if (ca && cb) literal(FBH_PATH_C0);
+        else if (ca) { if (inplace) literal(FBH_PATH_D1); else c1(b, a, rk_x); }
+        else if (cb) c1(a, b, rk_y);
+        else if (all0s(a) || all0s(b)) { path = FBH_PATH_SHORTCUT; a->aval = act; }
+        else if (all1s(b)) {
+            // one 0-fill of b's fill count (append_counter into an empty vector)
+            path = FBH_PATH_SHORTCUT;
+            const uint32_t w = HEADER0 | (b->w0 & MAXCNT);
+            uint32_t* d = (uint32_t*)dev_alloc(8);
+            if (!d) return fail(FBH_ERR_NOMEM, "device allocation failed");
+            rc = check(hipMemcpyAsync(d, &w, 4, hipMemcpyHostToDevice, stream()), "upload");
+            if (!rc) rc = check(hipStreamSynchronize(stream()), "sync");
+            if (rc) { dev_free(d); return rc; }
+            rc = install_words(a, d, 2, 1);
+            a->nset = 0; a->aval = act;
+        } else if (all1s(a)) {
+            // ~b, active bits kept where a's are set
+            path = FBH_PATH_SHORTCUT;
+            const uint32_t keep = a->aval;
+            rc = bv_copy_words(a, b);
+            if (!rc) rc = check(fbk::launch_flip(a->d, a->nw, stream()), "flip");
+            auto fl = [](uint32_t w) { return w > ALLONES ? (w ^ FILLBIT) : (w ^ ALLONES); };
+            a->w0 = fl(a->w0); a->wl = fl(a->wl); a->ck_ok = false;
+            a->nset = 0;
+            a->aval = (b->aval ^ (b->anb ? ((1U << b->anb) - 1) : 0u)) & keep;
+        }
+        else if (d2) literal(FBH_PATH_D2);
+        else c2();
+        break;
+    }
+    if (rc) return rc;
+    a->nbits = groups * MAXBITS;
+    if (path_out) *path_out = path;
+    return FBH_OK;
+}
+
+// The out-of-place operator a OP b into a new handle (src/bitvector.cpp:
+// 1292-1838): operands of different sizes go through a copy of the shorter
+// one extended with zeros and the in-place operator (the copy on the left,
+// except for MINUS), without compress().
+int bv_op_new(const fbh_bitvector* a_in, const fbh_bitvector* b_in, int op, fbh_bitvector** out, int* path) {
+    if (op < 0 || op > 3) return fail(FBH_ERR_ARG, "unknown operator");
+    fbh_bitvector* a = const_cast<fbh_bitvector*>(a_in);
+    fbh_bitvector* b = const_cast<fbh_bitvector*>(b_in);
+    uint32_t sa, sb;
+    int rc = bv_size(a, &sa);
+    if (!rc) rc = bv_size(b, &sb);
+    if (rc) return rc;
+    fbh_bitvector* res = new fbh_bitvector();
+    auto copy_meta = [](fbh_bitvector* d, const fbh_bitvector* s) {
+        d->nbits = s->nbits; d->nset = s->nset; d->aval = s->aval; d->anb = s->anb;
+    };
+    fbh_bitvector tmp;
+    if (sa != sb) {
+        const bool b_short = sa > sb;
+        const fbh_bitvector* first = (b_short && op != FBH_MINUS) ? b : a;     // res starts as this copy
+        fbh_bitvector* other = (b_short && op != FBH_MINUS) ? a : b;
+        copy_meta(res, first);
+        rc = bv_copy_words(res, first);
+        if (!rc && first->nbits + first->anb < (sa > sb ? sa : sb)) rc = bv_adjust(res, 0, sa > sb ? sa : sb);
+        if (!rc && b_short && op == FBH_MINUS) {
+            // src/bitvector.cpp:1778-1782: a copy of the shorter rhs, extended
+            copy_meta(&tmp, b);
+            rc = bv_copy_words(&tmp, b);
+            if (!rc) rc = bv_adjust(&tmp, 0, sa);
+            other = &tmp;
+        }
+        uint32_t so = 0, sr = 0;
+        if (!rc) rc = bv_size(other, &so);
+        if (!rc) rc = bv_size(res, &sr);
+        if (!rc) rc = bv_op_raw(res, other, op, true, path);
+    } else {
+        copy_meta(res, a);
+        rc = bv_copy_words(res, a);
+        if (!rc) rc = bv_op_raw(res, b, op, false, path);
+    }
+    bv_release(&tmp);
+    if (rc) { bv_release(res); delete res; return rc; }
+    *out = res;
+    return FBH_OK;
+}
+
+// count(mask) (src/bitvector.cpp:1088-1124): the ones of a AND mask,
+// computed tile by tile without writing the result.  The operands must
+// have the same size.
+int bv_count_mask(fbh_bitvector* a, fbh_bitvector* m, uint32_t* out) {
+    uint32_t sa, sm;
+    int rc = bv_size(a, &sa);
+    if (!rc) rc = bv_size(m, &sm);
+    if (rc) return rc;
+    if (sa != sm) return fail(FBH_ERR_SIZE, "count(mask): the operands differ in size");
+    const uint32_t groups = a->nbits / MAXBITS;
+    const uint32_t act = popc(a->aval & m->aval);
+    if (groups == 0 || all0s(a) || all0s(m)) { *out = act; return FBH_OK; }
+    uint32_t c = 0;
+    if (all1s(a) || all1s(m)) {
+        fbh_bitvector* o = all1s(a) ? m : a;
+        rc = fbh_bv_cnt(o, &c);
+        if (rc) return rc;
+        *out = c - popc(o->aval) + act;
+        return FBH_OK;
+    }
+    const bool la = a->nw == groups, lm = m->nw == groups;
+    if (!la && (rc = bv_ensure_ck(a))) return rc;
+    if (!lm && (rc = bv_ensure_ck(m))) return rc;
+    const uint32_t nb = fbk::count_and_blocks(groups);
+    uint32_t* part = (uint32_t*)dev_alloc(4ull * (nb + 1));
+    if (!part) return fail(FBH_ERR_NOMEM, "device allocation failed");
+    rc = check(fbk::launch_count_and(a->d, a->ck, a->nw, la ? 1 : 0, m->d, m->ck, m->nw, lm ? 1 : 0, groups, part,
+                                     stream()), "count");
+    if (!rc) rc = check(fbk::launch_sum_parts_n(part, nb, part + nb, stream()), "count sum");
+    if (!rc) rc = check(hipMemcpyAsync(&c, part + nb, 4, hipMemcpyDeviceToHost, stream()), "count copy");
+    if (!rc) rc = check(hipStreamSynchronize(stream()), "count sync");
+    dev_free(part);
+    if (rc) return rc;
+    *out = c + act;
+    return FBH_OK;
+}
+
+// erase(i, j) (src/bitvector.cpp:1012-1081), with the size counted first.
+// The words before the one holding bit i are kept as they are; everything
+// the reference appends after them (the bits [p0, i) of that word, bits
+// [j, size)) goes through append_active / append_counter and appendWord,
+// which merge every token into the word before it, so it is the canonical
+// compression of those bits, its first run merged into the last kept word
+// when their classes match.
+int bv_erase(fbh_bitvector* bv, uint32_t i, uint32_t j) {
+    uint32_t size;
+    int rc = bv_size(bv, &size);
+    if (rc) return rc;
+    if (i >= j) return FBH_OK;
+    if (i >= size) return fail(FBH_ERR_ARG, "erase: the first position is past the end");
+    if (j > size) j = size;
+    const uint32_t groups = bv->nbits / MAXBITS;
+    // the code word holding bit i: the kept prefix ends before it
+    uint32_t pw = bv->nw, p0 = bv->nbits;
+    if (i < bv->nbits) {
+        rc = bv_ensure_ck(bv);
+        if (rc) return rc;
+        uint32_t ws[2] = {0, 0};
+        uint32_t* dws = (uint32_t*)dev_alloc(16);
+        if (!dws) return fail(FBH_ERR_NOMEM, "device allocation failed");
+        rc = check(fbk::launch_word_at(bv->d, bv->ck, bv->nw, i / MAXBITS, dws, stream()), "locate");
+        if (!rc) rc = check(hipMemcpyAsync(ws, dws, 8, hipMemcpyDeviceToHost, stream()), "copy");
+        if (!rc) rc = check(hipStreamSynchronize(stream()), "sync");
+        dev_free(dws);
+        if (rc) return rc;
+        pw = ws[0]; p0 = ws[1] * MAXBITS;
+    }
+    const uint32_t ind = i - p0;
+    const uint32_t nS = ind + (size - j);
+    const uint32_t whole = nS / MAXBITS, rem = nS % MAXBITS;
+    // the literal groups, the active bits as one more group, a zero word
+    uint32_t* L = nullptr;
+    rc = literal_groups(bv, groups, 2, &L);
+    if (rc) return rc;
+    if (bv->anb) {
+        const uint32_t act = bv->aval << (MAXBITS - bv->anb);
+        rc = check(hipMemcpyAsync(L + groups, &act, 4, hipMemcpyHostToDevice, stream()), "upload");
+        if (!rc) rc = check(hipStreamSynchronize(stream()), "sync");
+        if (rc) { dev_free(L); return rc; }
+    }
+    fbh_bitvector S;                      // the appended groups, canonical
+    S.d = (uint32_t*)dev_alloc(4ull * (whole + 2));
+    if (!S.d) { dev_free(L); return fail(FBH_ERR_NOMEM, "device allocation failed"); }
+    S.cap = whole + 2;
+    rc = check(fbk::launch_shift_groups(L, p0, ind, j, nS, S.d, stream()), "shift");
+    uint32_t tail = 0;
+    if (!rc && rem) rc = check(hipMemcpyAsync(&tail, S.d + whole, 4, hipMemcpyDeviceToHost, stream()), "copy");
+    if (!rc) rc = check(hipStreamSynchronize(stream()), "sync");
+    dev_free(L);
+    if (!rc && whole) {
+        S.nw = whole; S.nbits = whole * MAXBITS;
+        rc = check(hipMemcpy(&S.w0, S.d, 4, hipMemcpyDeviceToHost), "copy");
+        if (!rc) rc = check(hipMemcpy(&S.wl, S.d + whole - 1, 4, hipMemcpyDeviceToHost), "copy");
+        if (!rc) rc = bv_compress(&S);
+    }
+    if (rc) { bv_release(&S); return rc; }
+    // stitch: kept words ++ S's words, S's first run merged into the last
+    // kept word when both are all-zero or all-one groups
+    auto cls = [](uint32_t w) -> int {
+        if (w == 0 || (w >= HEADER0 && w < HEADER1)) return 0;
+        if (w == ALLONES || w >= HEADER1) return 1;
+        return -1;
+    };
+    auto count_of = [](uint32_t w) { return w > ALLONES ? (w & MAXCNT) : 1u; };
+    uint32_t last = 0;
+    if (pw) {
+        rc = check(hipMemcpy(&last, bv->d + pw - 1, 4, hipMemcpyDeviceToHost), "copy");
+        if (rc) { bv_release(&S); return rc; }
+    }
+    const bool merge = pw && S.nw && cls(last) >= 0 && cls(last) == cls(S.w0);
+    const uint32_t nw = pw + S.nw - (merge ? 1u : 0u);
+    uint32_t* d = (uint32_t*)dev_alloc(4ull * (nw + 1));
+    if (!d) { bv_release(&S); return fail(FBH_ERR_NOMEM, "device allocation failed"); }
+    if (pw) rc = check(hipMemcpyAsync(d, bv->d, 4ull * pw, hipMemcpyDeviceToDevice, stream()), "copy");
+    const uint32_t skip = merge ? 1u : 0u;
+    if (!rc && S.nw > skip)
+        rc = check(hipMemcpyAsync(d + pw, S.d + skip, 4ull * (S.nw - skip), hipMemcpyDeviceToDevice, stream()), "copy");
+    if (!rc && merge) {
+        const uint32_t w = (cls(last) ? HEADER1 : HEADER0) | (count_of(last) + count_of(S.w0));
+        rc = check(hipMemcpyAsync(d + pw - 1, &w, 4, hipMemcpyHostToDevice, stream()), "upload");
+    }
+    if (!rc) rc = check(hipStreamSynchronize(stream()), "sync");
+    bv_release(&S);
+    if (rc) { dev_free(d); return rc; }
+    rc = install_words(bv, d, nw + 1, nw);
+    if (rc) return rc;
+    bv->nbits = (size - (j - i)) - rem;
+    bv->nset = 0;
+    bv->anb = rem;
+    bv->aval = rem ? (tail >> (MAXBITS - rem)) : 0u;
+    return FBH_OK;
+}
+
 } // namespace fbh_impl
 
 using namespace fbh_impl;
@@ -599,13 +993,21 @@ int fbh_bv_op_new(const fbh_bitvector* a, const fbh_bitvector* b, int op, fbh_bi
                   int* path) {
     FBH_API_LOCK();
     if (!a || !b || !out) return fail(FBH_ERR_ARG, "null argument");
-    fbh_bitvector* r = nullptr;
-    int rc = fbh_bv_copy(a, &r);
-    if (rc) return rc;
-    rc = fbh_bv_op(r, b, op, path);
-    if (rc) { fbh_bv_free(r); return rc; }
-    *out = r;
-    return FBH_OK;
+    try { return bv_op_new(a, b, op, out, path); }
+    catch (...) { return fail(FBH_ERR_INTERNAL, "exception in fbh_bv_op_new"); }
+}
+
+int fbh_bv_count(fbh_bitvector* a, fbh_bitvector* mask, uint32_t* cnt) {
+    FBH_API_LOCK();
+    if (!a || !mask || !cnt) return fail(FBH_ERR_ARG, "null argument");
+    return bv_count_mask(a, mask, cnt);
+}
+
+int fbh_bv_erase(fbh_bitvector* bv, uint32_t i, uint32_t j) {
+    FBH_API_LOCK();
+    if (!bv) return fail(FBH_ERR_ARG, "null handle");
+    try { return bv_erase(bv, i, j); }
+    catch (...) { return fail(FBH_ERR_INTERNAL, "exception in fbh_bv_erase"); }
 }
 
 int fbh_bv_compress(fbh_bitvector* bv) {

@@ -658,6 +658,55 @@ int plan_heuristic(const fbh_index* ix, double lo, int lop, double hi, int rop);
 double index_cost(const fbh_index* ix, double lo, int lop, double hi, int rop);
 double scan_cost(const fbh_index* ix);
 
+// Canonical WAH of a literal result (groups words, then the active bits
+// left-aligned) as a new bitvector.
+int literal_bitvector(const fbh_index* ix, uint32_t* lit, fbh_bitvector** out) {
+    fbh_bitvector* res = nullptr;
+    int rc = fbh_bv_new(&res);
+    if (rc) return rc;
+    uint32_t act = 0;
+    if (ix->anb && (rc = to_host(&act, lit + ix->groups, 1))) { fbh_bv_free(res); return rc; }
+    fbh_bitvector tmp;
+    tmp.d = lit; tmp.nw = ix->groups; tmp.nbits = ix->groups * MAXBITS;
+    tmp.anb = ix->anb; tmp.aval = ix->anb ? (act >> (31 - ix->anb)) : 0;
+    if (ix->groups) {
+        if ((rc = to_host(&tmp.w0, lit, 1)) || (rc = to_host(&tmp.wl, lit + ix->groups - 1, 1))) {
+            fbh_bv_free(res); return rc;
+        }
+    }
+    // res = empty | tmp  (the canonical kernel runs through the c0/d1 path)
+    rc = fbh_bv_set(res, 0, ix->nrows);
+    if (!rc) rc = bv_op(res, &tmp, FBH_OR, nullptr);
+    tmp.d = nullptr;
+    if (tmp.ck) { dev_free(tmp.ck); tmp.ck = nullptr; }
+    if (rc) { fbh_bv_free(res); return rc; }
+    *out = res;
+    return FBH_OK;
+}
+
+// The OR of bitmaps [r0, r1) and [r2, r3), complemented when flip, as a new
+// bitvector (the literal kernel without an edge check).
+int ranges_bitvector(fbh_index* ix, uint32_t r0, uint32_t r1, uint32_t r2, uint32_t r3, uint32_t flip,
+                     fbh_bitvector** out) {
+    fbk::QueryArgs q;
+    fill_query(ix, q, 0.0, FBH_OP_NONE, 0.0, FBH_OP_NONE);
+    q.r0 = r0; q.r1 = r1; q.r2 = r2; q.r3 = r3; q.flip = flip;
+    DBuf<uint32_t> lit, part;
+    int rc;
+    if ((rc = lit.alloc(ix->groups + 1)) || (rc = part.alloc(ix->ntiles))) return rc;
+    q.out = lit.p; q.part = part.p;
+    if ((rc = check(fbk::launch_query_lit(q, dev_type(ix->type), stream()), "estimate"))) return rc;
+    return literal_bitvector(ix, lit.p, out);
+}
+
+// index::sumBins(ib, ie) as a new bitvector.
+int sum_bins_bitvector(fbh_index* ix, uint32_t ib, uint32_t ie, fbh_bitvector** out) {
+    fbk::QueryArgs q;
+    memset(&q, 0, sizeof(q));
+    set_sum_ranges(ix, q, ib, ie);
+    return ranges_bitvector(ix, q.r0, q.r1, q.r2, q.r3, q.flip, out);
+}
+
 // Installs bitmaps given in the serialized index form (src/iapi.h:160-176):
 // bitmap b is bms[offsets[b], offsets[b+1]); an empty range is a null
 // bitmap, any other is code words ++ [active value when the active bit
@@ -1143,28 +1192,7 @@ int fbh_index_evaluate(fbh_index* ix, double lo, int lop, double hi, int rop, in
     if (!rc) rc = pc.sum(&np, 1, &c);
     if (rc) return rc;
     if (hits) *hits = c;
-    if (result) {
-        // canonical WAH of the literal result
-        fbh_bitvector* res = nullptr;
-        if ((rc = fbh_bv_new(&res))) return rc;
-        uint32_t act = 0;
-        if (ix->anb && (rc = to_host(&act, lit.p + ix->groups, 1))) { fbh_bv_free(res); return rc; }
-        fbh_bitvector tmp;
-        tmp.d = lit.p; tmp.nw = ix->groups; tmp.nbits = ix->groups * MAXBITS;
-        tmp.anb = ix->anb; tmp.aval = ix->anb ? (act >> (31 - ix->anb)) : 0;
-        if (ix->groups) {
-            if ((rc = to_host(&tmp.w0, lit.p, 1)) || (rc = to_host(&tmp.wl, lit.p + ix->groups - 1, 1))) {
-                fbh_bv_free(res); return rc;
-            }
-        }
-        // res = empty | tmp  (the canonical kernel runs through the c0/d1 path)
-        rc = fbh_bv_set(res, 0, ix->nrows);
-        if (!rc) rc = bv_op(res, &tmp, FBH_OR, nullptr);
-        tmp.d = nullptr;
-        if (tmp.ck) { dev_free(tmp.ck); tmp.ck = nullptr; }
-        if (rc) { fbh_bv_free(res); return rc; }
-        *result = res;
-    }
+    if (result) return literal_bitvector(ix, lit.p, result);
     return FBH_OK;
 }
 
@@ -1193,6 +1221,82 @@ int fbh_index_evaluate_rids(fbh_index* ix, double lo, int lop, double hi, int ro
     if (rc) return rc;
     const uint64_t k = c < cap ? c : cap;
     return to_host(rids, out.p, k);
+}
+
+// relic::estimate (src/irelic.h:58-62: evaluate, upper cleared) and
+// bin::estimate (src/ibin.cpp:6722-6829).  The cost test reads the bitmap
+// sizes as offset64 holds them once sumBins has run: the serialized sizes
+// where the index came with offsets, else bitvector::bytes().
+int fbh_index_estimate(fbh_index* ix, double lo, int lop, double hi, int rop, fbh_bitvector** lower,
+                       fbh_bitvector** upper) {
+    FBH_API_LOCK();
+    if (!ix || !lower || !upper) return fail(FBH_ERR_ARG, "null argument");
+    if (lop < 0 || lop > 2 || rop < 0 || rop > 2) return fail(FBH_ERR_ARG, "bad operator");
+    int rc = ensure_init();
+    if (rc) return rc;
+    fbh_bitvector *lw = nullptr, *up = nullptr;
+    auto done = [&](int r) {
+        if (r) { fbh_bv_free(lw); fbh_bv_free(up); return r; }
+        *lower = lw; *upper = up;
+        return FBH_OK;
+    };
+    if ((rc = fbh_bv_new(&up))) return rc;                   // a cleared upper unless set below
+    if (ix->kind == FBH_RELIC) {
+        uint64_t h = 0;
+        return done(fbh_index_evaluate(ix, lo, lop, hi, rop, FBH_PLAN_LITERAL, &h, &lw, nullptr));
+    }
+    const uint32_t nobs = ix->nobs;
+    if (nobs == 0 || ix->nrows == 0) {
+        if ((rc = fbh_bv_new(&lw)) == 0) rc = fbh_bv_set(lw, 0, ix->nrows);
+        return done(rc);
+    }
+    Locate L;
+    locate(ix, lo, lop, hi, rop, L);
+    const uint32_t c0 = L.cand0, c1 = L.cand1, h0 = L.hit0, h1 = L.hit1;
+    double cost = 0.0;
+    if (c0 < c1 && c1 < nobs + 1) {
+        const std::vector<double>& m = ix->offsets_built ? ix->cumser : ix->cumbytes;
+        const int64_t tot = (int64_t)(m[nobs] - m[0]);
+        const int64_t mid = (int64_t)(m[c1] - m[c0]);
+        cost = (tot >> 1) >= mid ? (double)mid : (double)(tot - mid);
+    }
+    const double edge = (double)col_esize(ix->type) * (double)ix->nrows / nobs;
+    if (h0 > c0 && c1 > h1) cost += edge * 2.0;
+    else if (h0 > c0 || c1 > h1) cost += edge;
+    if (c0 >= c1) {
+        if ((rc = fbh_bv_new(&lw)) == 0) rc = fbh_bv_set(lw, 0, ix->nrows);
+        if (!rc) rc = fbh_bv_set(up, 0, ix->nrows);
+        return done(rc);
+    }
+    if (cost > ix->nrows * 0.75) {
+        // too costly: no sure hit, every row a candidate
+        if ((rc = fbh_bv_new(&lw)) == 0) rc = fbh_bv_set(lw, 0, ix->nrows);
+        if (!rc) rc = fbh_bv_set(up, 1, ix->nrows);
+        return done(rc);
+    }
+    if (h0 < h1) {
+        rc = sum_bins_bitvector(ix, h0, h1, &lw);
+        const bool e0 = c0 < h0 && !ix->null_bitmap(c0);
+        const bool e1 = c1 > h1 && h1 < nobs && !ix->null_bitmap(h1);
+        if (!rc && (c0 < h0 || (c1 > h1 && h1 < nobs))) {
+            // lower | the edge bins: the bins partition the rows, so the
+            // straight OR widens to [c0, c1) and the complement's outside
+            // set shrinks by the edge bins
+            const uint32_t b0 = e0 ? c0 : h0, b1 = e1 ? h1 + 1 : h1;
+            const SumPlan sp = sum_plan(ix, h0, h1);
+            fbh_bitvector* u = nullptr;
+            if (sp.kind == 1) rc = ranges_bitvector(ix, 0, 0, 0, 0, 1, &u);
+            else if (sp.kind == 3) rc = ranges_bitvector(ix, 0, b0, b1, nobs, 1, &u);
+            else rc = ranges_bitvector(ix, b0, b1, 0, 0, 0, &u);
+            if (!rc) { fbh_bv_free(up); up = u; }
+        }
+        return done(rc);
+    }
+    if ((rc = fbh_bv_new(&lw)) == 0) rc = fbh_bv_set(lw, 0, ix->nrows);
+    fbh_bitvector* u = nullptr;
+    if (!rc) rc = sum_bins_bitvector(ix, c0, c1, &u);
+    if (!rc) { fbh_bv_free(up); up = u; }
+    return done(rc);
 }
 
 // fbh_index_evaluate without any device allocation: the literal result goes

@@ -18,7 +18,7 @@ hipError_t launch_chunk_index(const word_t* w, uint32_t n, uint32_t* ck, uint32_
 uint32_t op_tiles(uint32_t na, uint32_t nb);
 hipError_t launch_op(const word_t* a, const uint32_t* cka, uint32_t na,
                      const word_t* b, const uint32_t* ckb, uint32_t nb,
-                     uint32_t groups, int op, int canon, word_t* out, uint32_t* outck,
+                     uint32_t groups, int op, int mode, word_t* out, uint32_t* outck,
                      uint64_t* status, uint32_t* ctl, hipStream_t s);
 hipError_t launch_edge_words(const word_t* out, uint32_t* ctl, hipStream_t s);
 uint32_t tok_tiles(uint32_t n);
@@ -43,4 +43,30 @@ hipError_t launch_canon(const word_t* a, const uint32_t* cka, uint32_t na, int a
                         const word_t* b, const uint32_t* ckb, uint32_t nb, int blit,
                         uint32_t groups, int op, word_t* out, uint32_t* outck, uint64_t* status,
                         uint32_t* ctl, uint2* ts, hipStream_t s);
+// count(mask): part[k] (count_and_blocks(groups) entries) = ones of a AND b
+// over the k-th tile of groups; lit != 0: the operand is one literal word
+// per group (no chunk index needed).
+uint32_t count_and_blocks(uint32_t groups);
+hipError_t launch_count_and(const word_t* a, const uint32_t* cka, uint32_t na, int alit, const word_t* b,
+                            const uint32_t* ckb, uint32_t nb, int blit, uint32_t groups, uint32_t* part,
+                            hipStream_t s);
+// out[i] = a[i] OP b[i]
+hipError_t launch_lit_op(const word_t* a, const word_t* b, uint32_t n, int op, word_t* out, hipStream_t s);
+// *_c1 routines: V[g] = value of result group g, H[g] = 1 where a result
+// word starts (x compressed with its chunk index, y literal).
+enum : int { C1_AND = 0, C1_OR = 1, C1_XOR = 2, C1_MINUS = 3, C1_MINUSX = 4 };
+hipError_t launch_c1_heads(const word_t* x, const uint32_t* ckx, uint32_t nx, const word_t* y, uint32_t groups,
+                           int rk, word_t* V, uint32_t* H, hipStream_t s);
+// Result words from the word starts H, their exclusive scan O and the
+// values V of n groups; nw = number of words; HP needs nw + 1 words.
+hipError_t launch_head_words(const word_t* V, const uint32_t* H, const uint32_t* O, uint32_t n, uint32_t nw,
+                             uint32_t* HP, word_t* out, hipStream_t s);
+// out[0] / out[1] = index and start group of the code word holding group G
+hipError_t launch_word_at(const word_t* w, const uint32_t* ck, uint32_t n, uint32_t G, uint32_t* out,
+                          hipStream_t s);
+// V[q] = group q of bits [p0, p0 + ind) ++ bits [j, j + nS - ind) of the
+// literal groups L (the active bits left-aligned in one more group, then a
+// zero word).
+hipError_t launch_shift_groups(const word_t* L, uint32_t p0, uint32_t ind, uint32_t j, uint32_t nS, word_t* V,
+                               hipStream_t s);
 } // namespace fbk

@@ -27,10 +27,12 @@ extern "C" {
 
 // a, b: code words only (no active word), both covering `groups` 31-bit
 // groups.  splits: nominal cut positions (any order, any value in [0,groups]).
-// Writes the compressed result words to out (capacity cap) and their start
-// positions to heads; returns the word count or -1.
+// mode: 0 the compressed *_c2 result, MODE_CANON the canonical result,
+// MODE_RAW the *_c2 loop's own words.  Writes the result words to out
+// (capacity cap) and their start positions to heads; returns the word count
+// or a negative value.
 long fbsim_op(const uint32_t* a, uint32_t na, const uint32_t* b, uint32_t nb,
-              uint32_t groups, int op, int canon, const uint32_t* splits,
+              uint32_t groups, int op, int mode, const uint32_t* splits,
               uint32_t nsplits, uint32_t* out, uint32_t* heads, uint32_t cap) {
     ArrayReader ra{a}, rb{b};
     // start positions of every code word (plus sentinel)
@@ -52,7 +54,7 @@ long fbsim_op(const uint32_t* a, uint32_t na, const uint32_t* b, uint32_t nb,
         uint32_t pa = sa[ia], pb = sb[ib];
         if (pa < c && ia < na) { pa = sa[ia + 1]; ++ia; }
         if (pb < c && ib < nb) { pb = sb[ib + 1]; ++ib; }
-        uint32_t p = find_sync(&ra, &rb, ia, pa, ib, pb, groups);
+        uint32_t p = find_sync(&ra, &rb, ia, pa, ib, pb, groups, (mode & MODE_RAW) != 0);
         starts.push_back(p); ias.push_back(ia); ibs.push_back(ib);
         pas.push_back(pa); pbs.push_back(pb);
     }
@@ -72,7 +74,7 @@ long fbsim_op(const uint32_t* a, uint32_t na, const uint32_t* b, uint32_t nb,
     std::vector<Summary> sum(nr);
     for (size_t r = 0; r < nr; ++r) {
         CountSink cs; cs.init();
-        walk(cs, &ra, &rb, IA[r], PA[r], IB[r], PB[r], P[r], P[r + 1], op, (uint32_t)canon);
+        walk(cs, &ra, &rb, IA[r], PA[r], IB[r], PB[r], P[r], P[r + 1], op, (uint32_t)mode);
         sum[r] = cs.s;
     }
     // exclusive prefixes
@@ -86,7 +88,7 @@ long fbsim_op(const uint32_t* a, uint32_t na, const uint32_t* b, uint32_t nb,
     for (size_t r = 0; r < nr; ++r) {
         WriteSink<VecWriter> ws;
         ws.init(&vw, pre[r], P[r]);
-        walk(ws, &ra, &rb, IA[r], PA[r], IB[r], PB[r], P[r], P[r + 1], op, (uint32_t)canon);
+        walk(ws, &ra, &rb, IA[r], PA[r], IB[r], PB[r], P[r], P[r + 1], op, (uint32_t)mode);
         if (r + 1 == nr) ws.finish();
     }
     if (res.size() != acc.count) return -2;

@@ -140,12 +140,15 @@ WAH_HD uint32_t status_of(uint64_t v) { return (uint32_t)(v >> 62); }
 struct CountSink {
     Summary s;
     WAH_HD void init() { s = summary_empty(); }
-    WAH_HD void tok(uint32_t cls, uint32_t len, uint32_t repF, word_t /*lit*/) {
+    // merge == 0: a word pushed raw (never merged into the word before it,
+    // though later tokens may merge into it); its class counts as MIXED
+    // for the merge test of the range's first token
+    WAH_HD void tok(uint32_t cls, uint32_t len, uint32_t repF, word_t /*lit*/, uint32_t merge) {
         const bool empty = s.first == C_NONE;
-        const bool extend = !empty && cls != C_MIXED && cls == s.trail;   // the open run goes on
+        const bool extend = !empty && merge && cls != C_MIXED && cls == s.trail;   // the open run goes on
         const uint32_t nlen = cls == C_MIXED ? 0u : len;
         Summary n;
-        n.first = empty ? cls : s.first;
+        n.first = empty ? (merge ? cls : (uint32_t)C_MIXED) : s.first;
         n.count = empty ? 1u : (extend ? s.count : s.count + 1u);
         n.trail = extend ? s.trail : cls;
         n.tlen = extend ? s.tlen + len : nlen;
@@ -186,8 +189,8 @@ struct WriteSink {
         }
         rcls = C_NONE; rlen = 0; rrep = 0;
     }
-    WAH_HD void tok(uint32_t cls, uint32_t len, uint32_t repF, word_t lit) {
-        const bool extend = cls != C_MIXED && cls == rcls;    // the open run goes on
+    WAH_HD void tok(uint32_t cls, uint32_t len, uint32_t repF, word_t lit, uint32_t merge) {
+        const bool extend = merge && cls != C_MIXED && cls == rcls;    // the open run goes on
         const uint32_t olen = rlen;
         if (!extend) {
             close_run();
@@ -246,27 +249,36 @@ WAH_HD Run<R> pick(bool c, const Run<R>& a, const Run<R>& b) {
 }
 
 // Emits the token of one group-valued span (class from the value).
+// merge: the token goes through append_counter / append_active (it merges
+// into the word before it when the classes match); 0 for a word pushed raw.
 template <class S>
-WAH_HD void emit_fill(S& sink, int b, word_t k, uint32_t repF) {
-    sink.tok(b ? C_ONE : C_ZERO, k, repF, 0u);
+WAH_HD void emit_fill(S& sink, int b, word_t k, uint32_t repF, uint32_t merge = 1u) {
+    sink.tok(b ? C_ONE : C_ZERO, k, repF, 0u, merge);
 }
 
 template <class S>
-WAH_HD void emit_lit(S& sink, word_t v, uint32_t canon) {
-    (void)canon;
+WAH_HD void emit_lit(S& sink, word_t v, uint32_t merge = 1u) {
     const uint32_t c = class_of_value(v);
-    sink.tok(c, 1u, 0u, v);
+    sink.tok(c, 1u, 0u, v, merge);
 }
+
+// Mode bits of the step walk: MODE_CANON forces every representation to L
+// (the canonical results of the c0/d1/d2/c1 routines); MODE_RAW emits the
+// *_c2 loop's own words, not compressed: raw pushes of copy_runs do not
+// merge into the word before them, and walks start only where both
+// operands start a code word.
+constexpr uint32_t MODE_CANON = 1u, MODE_RAW = 2u;
 
 // copy_runs / copy_runsn (src/bitvector.h:838-910) as tokens: the first code
 // word is appended through append_active or append_counter, the following
 // whole code words are pushed raw while they fit into nw.
 template <class R, class S>
-WAH_HD void copy_runs_tok(S& sink, Run<R>& it, word_t& nw, bool neg, uint32_t canon) {
-    const uint32_t F = canon ? 0u : 1u;
+WAH_HD void copy_runs_tok(S& sink, Run<R>& it, word_t& nw, bool neg, uint32_t mode) {
+    const uint32_t F = (mode & MODE_CANON) ? 0u : 1u;
+    const uint32_t mr = (mode & MODE_RAW) ? 0u : 1u;      // raw pushes merge only when compressed
     if (it.isFill == 0) {
         const word_t v = neg ? (ALLONES ^ it.cur()) : it.cur();
-        emit_lit(sink, v, canon);
+        emit_lit(sink, v);
         --nw;
     } else if (it.nWords > 1) {
         emit_fill(sink, neg ? !it.fillBit : it.fillBit, it.nWords, F);
@@ -281,8 +293,8 @@ WAH_HD void copy_runs_tok(S& sink, Run<R>& it, word_t& nw, bool neg, uint32_t ca
         it.decode();
         if (nw >= it.nWords) {
             const word_t w = it.cur();
-            if (it.isFill) emit_fill(sink, neg ? !it.fillBit : it.fillBit, it.nWords, F);
-            else emit_lit(sink, neg ? (ALLONES ^ w) : w, canon);
+            if (it.isFill) emit_fill(sink, neg ? !it.fillBit : it.fillBit, it.nWords, F, mr);
+            else emit_lit(sink, neg ? (ALLONES ^ w) : w, mr);
             nw -= it.nWords;
             it.nWords = 0;
             ++it.it;
@@ -303,7 +315,7 @@ WAH_HD void take(Run<R>& r, word_t n) {          // n <= r.nWords groups of the 
 }
 
 // One step of the *_c2 loop of `op`; returns the groups consumed.
-// canon != 0 forces every token's representation to L (the canonical result
+// mode & MODE_CANON forces every token's representation to L (the canonical result
 // of the c0/d1/d2/c1 routines).  The branch of the reference's loop is first
 // reduced to an action and the operand it is anchored on (f, the other o);
 // f and o are chosen from x and y field by field and written back the same
@@ -312,8 +324,8 @@ WAH_HD void take(Run<R>& r, word_t n) {          // n <= r.nWords groups of the 
 enum StepAct : int { S_LIT = 0, S_FILL_SKIP = 1, S_COPY_IN = 2, S_FILL_SHORT = 3 };
 
 template <class R, class S>
-WAH_HD word_t c2_step(S& sink, Run<R>& x, Run<R>& y, int op, uint32_t canon, word_t lim = 0xFFFFFFFFu) {
-    const uint32_t F = canon ? 0u : 1u;
+WAH_HD word_t c2_step(S& sink, Run<R>& x, Run<R>& y, int op, uint32_t mode, word_t lim = 0xFFFFFFFFu) {
+    const uint32_t F = (mode & MODE_CANON) ? 0u : 1u;
     if (x.nWords == 0) x.decode();
     if (y.nWords == 0) y.decode();
     // act on (f, o): f = x, or f = y when `swp`
@@ -364,7 +376,7 @@ WAH_HD word_t c2_step(S& sink, Run<R>& x, Run<R>& y, int op, uint32_t canon, wor
     }
     if (act == S_LIT) {
         // both literal: append_active of the combined word
-        emit_lit(sink, apply_op(x.cur(), y.cur(), op), canon);
+        emit_lit(sink, apply_op(x.cur(), y.cur(), op));
         x.nWords = 0; y.nWords = 0; ++x.it; ++y.it;
         return 1;
     }
@@ -380,7 +392,7 @@ WAH_HD word_t c2_step(S& sink, Run<R>& x, Run<R>& y, int op, uint32_t canon, wor
         // the words of o are copied within the fill of f
         const word_t nw = f.nWords < lim ? f.nWords : lim;
         word_t r = nw;
-        copy_runs_tok(sink, o, r, neg, canon);
+        copy_runs_tok(sink, o, r, neg, mode);
         used = nw - r;
         f.nWords -= used;
         f.it += (f.nWords == 0);
@@ -402,7 +414,7 @@ WAH_HD word_t c2_step(S& sink, Run<R>& x, Run<R>& y, int op, uint32_t canon, wor
 // a fill there, with pa + len - p0 groups of it left.
 template <class R, class S>
 WAH_HD void walk(S& sink, const R* ra, const R* rb, uint32_t ia, uint32_t pa, uint32_t ib, uint32_t pb,
-                 uint32_t p0, uint32_t p1, int op, uint32_t canon) {
+                 uint32_t p0, uint32_t p1, int op, uint32_t mode) {
     Run<R> x, y;
     x.src = *ra; x.it = ia; x.nWords = 0; x.isFill = 0; x.fillBit = 0;
     y.src = *rb; y.it = ib; y.nWords = 0; y.isFill = 0; y.fillBit = 0;
@@ -411,7 +423,7 @@ WAH_HD void walk(S& sink, const R* ra, const R* rb, uint32_t ia, uint32_t pa, ui
         if (pb < p0) { y.decode(); y.nWords = pb + y.nWords - p0; }
     }
     uint32_t p = p0;
-    while (p < p1) p += c2_step(sink, x, y, op, canon, p1 - p);
+    while (p < p1) p += c2_step(sink, x, y, op, mode, p1 - p);
 }
 
 // Synchronisation points of the c2 loop: positions where the loop's state
@@ -426,7 +438,7 @@ WAH_HD void walk(S& sink, const R* ra, const R* rb, uint32_t ia, uint32_t pa, ui
 // count, or any synchronisation point) bounds the search.
 template <class R>
 WAH_HD uint32_t find_sync(const R* ra, const R* rb, uint32_t& ia, uint32_t& pa, uint32_t& ib, uint32_t& pb,
-                          uint32_t end) {
+                          uint32_t end, bool raw = false) {
     uint32_t q = pa > pb ? pa : pb;
     if (q >= end) return end;
     // the words holding q
@@ -437,8 +449,10 @@ WAH_HD uint32_t find_sync(const R* ra, const R* rb, uint32_t& ia, uint32_t& pa, 
         const word_t wa = ra->at(ia), wb = rb->at(ib);
         const uint32_t la = word_len(wa), lb = word_len(wb);
         if (pa == q && pb == q) return q;
-        if (pa == q && !(wa > ALLONES && la == 1u)) return q;
-        if (pb == q && !(wb > ALLONES && lb == 1u)) return q;
+        // raw words: a copied span may not restart at a word start (its
+        // first word would be appended, not pushed raw)
+        if (!raw && pa == q && !(wa > ALLONES && la == 1u)) return q;
+        if (!raw && pb == q && !(wb > ALLONES && lb == 1u)) return q;
         const uint32_t na = pa + la, nb = pb + lb;
         q = na < nb ? na : nb;
         if (na == q) { pa = na; ++ia; }

@@ -105,7 +105,7 @@ struct OpArgs {
     const word_t* b; const uint32_t* ckb; uint32_t nb; uint32_t ncb;
     uint32_t groups;
     int op;
-    uint32_t canon;
+    uint32_t mode;        // MODE_CANON / MODE_RAW bits of the step walk
     word_t* out;
     uint32_t* outck;
     uint64_t* status;
@@ -297,7 +297,7 @@ __global__ __launch_bounds__(TPB) void k_op_tiles(OpArgs p) {
             uint32_t ia, pa, ib, pb;
             holding_word(posA, cntA, baseA, p.a, p.na, Pq[q], ia, pa);
             holding_word(posB, cntB, baseB, p.b, p.nb, Pq[q], ib, pb);
-            S[q] = find_sync(&ra, &rb, ia, pa, ib, pb, p.groups);
+            S[q] = find_sync(&ra, &rb, ia, pa, ib, pb, p.groups, (p.mode & MODE_RAW) != 0);
             IA[q] = ia; IB[q] = ib; PA[q] = pa; PB[q] = pb;
         }
         info[9] = S[0]; info[10] = IA[0]; info[11] = IB[0];
@@ -335,7 +335,7 @@ __global__ __launch_bounds__(TPB) void k_op_tiles(OpArgs p) {
                 if (qa < qb) { jb -= 1; qb = posB[jb]; }
                 else if (qb < qa) { ja -= 1; qa = posA[ja]; }
                 ja += baseA; jb += baseB;
-                c = find_sync(&ra, &rb, ja, qa, jb, qb, S1);
+                c = find_sync(&ra, &rb, ja, qa, jb, qb, S1, (p.mode & MODE_RAW) != 0);
                 ia = ja; ib = jb; pa = qa; pb = qb;
                 if (c >= S1) { c = S1; ia = info[13]; ib = info[14]; pa = info[17]; pb = info[18]; }
             }
@@ -351,7 +351,7 @@ __global__ __launch_bounds__(TPB) void k_op_tiles(OpArgs p) {
     // count pass
     CountSink cs;
     cs.init();
-    if (c0 < c1) walk(cs, &ra, &rb, tia[tid], tpa[tid], tib[tid], tpb[tid], c0, c1, p.op, p.canon);
+    if (c0 < c1) walk(cs, &ra, &rb, tia[tid], tpa[tid], tib[tid], tpb[tid], c0, c1, p.op, p.mode);
     Summary agg;
     const Summary texcl = block_excl_summary2<TPB>(cs.s, sbuf, &agg);
     if (tid < WAVE) {
@@ -366,7 +366,7 @@ __global__ __launch_bounds__(TPB) void k_op_tiles(OpArgs p) {
     if (c0 < c1) {
         WriteSink<GlobWriter> ws;
         ws.init(&gw, pre, c0);
-        walk(ws, &ra, &rb, tia[tid], tpa[tid], tib[tid], tpb[tid], c0, c1, p.op, p.canon);
+        walk(ws, &ra, &rb, tia[tid], tpa[tid], tib[tid], tpb[tid], c0, c1, p.op, p.mode);
         if (c1 == p.groups) {
             ws.finish();
             const uint32_t total = combine(pre, cs.s).count;
@@ -694,12 +694,12 @@ __global__ void k_zero_status_ctl(uint64_t* status, uint32_t n, uint32_t* ctl) {
 
 hipError_t launch_op(const word_t* a, const uint32_t* cka, uint32_t na,
                      const word_t* b, const uint32_t* ckb, uint32_t nb,
-                     uint32_t groups, int op, int canon, word_t* out, uint32_t* outck,
+                     uint32_t groups, int op, int mode, word_t* out, uint32_t* outck,
                      uint64_t* status, uint32_t* ctl, hipStream_t s) {
     OpArgs p;
     p.a = a; p.cka = cka; p.na = na; p.nca = cdiv(na, CHUNK);
     p.b = b; p.ckb = ckb; p.nb = nb; p.ncb = cdiv(nb, CHUNK);
-    p.groups = groups; p.op = op; p.canon = canon ? 1u : 0u;
+    p.groups = groups; p.op = op; p.mode = (uint32_t)mode;
     p.out = out; p.outck = outck; p.status = status;
     p.total = ctl + 1;
     const uint32_t nt = op_tiles(na, nb);

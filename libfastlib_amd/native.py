@@ -45,6 +45,8 @@ def lib():
             "fbh_bv_write": [vp, _u32p, C.c_uint32, _u32p],
             "fbh_bv_op": [vp, vp, C.c_int, C.POINTER(C.c_int)],
             "fbh_bv_op_new": [vp, vp, C.c_int, C.POINTER(vp), C.POINTER(C.c_int)],
+            "fbh_bv_count": [vp, vp, _u32p],
+            "fbh_bv_erase": [vp, C.c_uint32, C.c_uint32],
             "fbh_bv_compress": [vp],
             "fbh_bv_decompress": [vp],
             "fbh_bv_flip": [vp],
@@ -133,6 +135,35 @@ class Bitvector:
         check(lib().fbh_bv_op(self._h, other._h, int(op), C.byref(path)), "fbh_bv_op")
         self.last_path = PATHS.get(path.value, "?")
         return self.last_path
+
+    def op_new(self, other: "Bitvector", op: int):
+        """The out-of-place operator (a new, uncompressed Bitvector, as
+        ibis::bitvector::operator& returns it) and the routine's name."""
+        h = C.c_void_p()
+        path = C.c_int(0)
+        check(lib().fbh_bv_op_new(self._h, other._h, int(op), C.byref(h), C.byref(path)), "fbh_bv_op_new")
+        return Bitvector(h), PATHS.get(path.value, "?")
+
+    def __and__(self, o):
+        return self.op_new(o, AND)[0]
+
+    def __or__(self, o):
+        return self.op_new(o, OR)[0]
+
+    def __xor__(self, o):
+        return self.op_new(o, XOR)[0]
+
+    def __sub__(self, o):
+        return self.op_new(o, MINUS)[0]
+
+    def count(self, mask: "Bitvector") -> int:
+        """count(mask): the ones of self AND mask."""
+        c = C.c_uint32()
+        check(lib().fbh_bv_count(self._h, mask._h, C.byref(c)), "fbh_bv_count")
+        return c.value
+
+    def erase(self, i: int, j: int) -> None:
+        check(lib().fbh_bv_erase(self._h, int(i), int(j)), "fbh_bv_erase")
 
     def __iand__(self, o):
         self.op(o, AND)
@@ -263,6 +294,7 @@ def _index_sigs(L):
         "fbh_index_evaluate_batch": [vp, C.c_uint32, _dp, _ip, _dp, _ip, _ip, C.c_int, _u64p, _ip,
                                      C.POINTER(C.c_float)],
         "fbh_index_query_bytes": [vp, C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, _dp],
+        "fbh_index_estimate": [vp, C.c_double, C.c_int, C.c_double, C.c_int, C.POINTER(vp), C.POINTER(vp)],
         "fbh_index_evaluate_noalloc": [vp, C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, vp,
                                        C.c_uint64, _u64p, C.POINTER(C.c_int)],
         "fbh_index_column_bytes": [vp, _u64p],
@@ -450,6 +482,14 @@ class Index:
         check(L.fbh_index_evaluate_rids(self._h, lo, lop, hi, rop, plan, out.ctypes.data_as(_u32p),
                                         n.value, C.byref(n)), "fbh_index_evaluate_rids")
         return out[: n.value]
+
+    def estimate(self, lo, lop, hi, rop):
+        """index::estimate(expr, lower, upper): (lower, upper) Bitvectors;
+        an upper of size 0 stands for lower."""
+        lw, up = C.c_void_p(), C.c_void_p()
+        check(self._lib().fbh_index_estimate(self._h, lo, lop, hi, rop, C.byref(lw), C.byref(up)),
+              "fbh_index_estimate")
+        return Bitvector(lw), Bitvector(up)
 
     def evaluate_noalloc(self, lo, lop, hi, rop, plan, dev_out, out_words):
         """fbh_index_evaluate_noalloc: the literal result into the device

@@ -288,6 +288,20 @@ class OracleIndex:
         L.orc_free(r)
         return int(cnt.value), rids
 
+    def estimate(self, lo, lop, hi, rop):
+        """index::estimate(expr, lower, upper): the serialized words of lower
+        and upper (a cleared upper is [0])."""
+        L = oracle().lib
+        lw, uw = _u32p(), _u32p()
+        nl, nu = C.c_size_t(), C.c_size_t()
+        rc = L.orc_index_estimate(C.c_void_p(self._h), self.col.ctypes.data_as(C.c_void_p), self.type,
+                                  C.c_double(lo), C.c_int(lop), C.c_double(hi), C.c_int(rop),
+                                  C.byref(lw), C.byref(nl), C.byref(uw), C.byref(nu))
+        if rc != 0:
+            raise RuntimeError("orc_index_estimate failed")
+        o = oracle()
+        return o._take(lw, nl), o._take(uw, nu)
+
     def evaluate_plan(self, lo, lop, hi, rop, plan=0, model=COST_IAPI):
         """(hit count, ascending rids, plan used) under `plan`: 0 the
         reference's own choice under the cost `model`, 1 the index, 3 the
@@ -368,6 +382,9 @@ class RefIapi:
         for p in (k, o, b):
             L.ref_free(p)
 
+    def estimate(self, lo, lop, hi, rop):
+        return ref_iapi_estimate(self.name, lo, lop, hi, rop)
+
     def query_range(self, lo, lop, hi, rop, want_rids=True):
         """lo OP1 x OP2 hi as one range term (as a where clause gives it)."""
         L = reference().lib
@@ -394,6 +411,21 @@ class RefIapi:
         if n < 0:
             raise RuntimeError(f"ref_iapi_query failed {n}")
         return int(n), (buf[:n].astype(np.uint32) if want_rids else None)
+
+
+def ref_iapi_estimate(name, lo, lop, hi, rop):
+    """The reference's index::estimate(expr, lower, upper) on the index of a
+    registered array: the serialized words of lower and upper."""
+    R = reference()
+    L = R.lib
+    lw, uw = _u32p(), _u32p()
+    nl, nu = C.c_size_t(), C.c_size_t()
+    nm = name if isinstance(name, bytes) else name.encode()
+    rc = L.ref_iapi_estimate(nm, C.c_double(lo), C.c_int(lop), C.c_double(hi), C.c_int(rop),
+                             C.byref(lw), C.byref(nl), C.byref(uw), C.byref(nu))
+    if rc != 0:
+        raise RuntimeError(f"ref_iapi_estimate failed rc={rc}")
+    return R._take(lw, nl), R._take(uw, nu)
 
 
 def ref_bin_bounds(name):
@@ -485,6 +517,9 @@ class RefAttached:
                                self.bitmaps.ctypes.data_as(_u32p), C.c_uint64(self.bitmaps.size))
         if rc < 0:
             raise RuntimeError(f"ref_iapi_attach failed rc={rc}")
+
+    def estimate(self, lo, lop, hi, rop):
+        return ref_iapi_estimate(self.name, lo, lop, hi, rop)
 
     def count(self, lo, lop, hi, rop):
         n = reference().lib.ref_iapi_query(self.name, C.c_double(lo), C.c_int(lop), C.c_double(hi),

@@ -342,6 +342,56 @@ BV evaluate(const Index& ix, const void* col, int type, double lo, int lop, doub
     return lower;
 }
 
+static uint32_t elem_size(int type);
+
+// src/ibin.cpp:6722-6829.  The cost test reads offset64: the serialized
+// sizes where a file or the keys/offsets constructor filled it, else the
+// bitvector::bytes() that sumBins fills it with (src/index.cpp:6463-6477);
+// the state of an index that has evaluated a range (the reference's first
+// estimate on a fresh in-memory index, before any sumBins, sees no
+// offsets and keeps only the edge term).
+void estimate(const Index& ix, const void* col, int type, double lo, int lop, double hi, int rop,
+              BV& lower, BV& upper) {
+    upper.clear();
+    if (ix.kind == 1) {
+        lower = evaluate(ix, col, type, lo, lop, hi, rop);
+        return;
+    }
+    const uint32_t nobs = (uint32_t)ix.bits.size();
+    if (nobs == 0 || ix.nrows == 0) { lower.set(0, ix.nrows); return; }
+    uint32_t cand0, cand1, hit0, hit1;
+    bin_range(ix, lo, lop, hi, rop, cand0, cand1, hit0, hit1);
+    double cost = 0.0;
+    if (cand0 < cand1 && cand1 < nobs + 1) {
+        std::vector<int64_t> off(nobs + 1, 0);
+        for (uint32_t i = 0; i < nobs; ++i)
+            off[i + 1] = off[i] + (int64_t)(ix.offsets_built ? serial_bytes(ix, i) : bm_bytes(ix.bits[i]));
+        const int64_t tot = off[nobs] - off[0];
+        const int64_t mid = off[cand1] - off[cand0];
+        cost = (tot >> 1) >= mid ? (double)mid : (double)(tot - mid);
+    }
+    const double edge = (double)elem_size(type) * (double)ix.nrows / nobs;
+    if (hit0 > cand0 && cand1 > hit1) cost += edge * 2.0;
+    else if (hit0 > cand0 || cand1 > hit1) cost += edge;
+    if (cand0 >= cand1) {
+        lower.set(0, ix.nrows);
+        upper.set(0, ix.nrows);
+    } else if (cost > ix.nrows * 0.75) {
+        lower.set(0, ix.nrows);
+        upper.set(1, ix.nrows);
+    } else if (hit0 < hit1) {
+        lower = sum_bins(ix, hit0, hit1);
+        if (cand0 < hit0 || (cand1 > hit1 && hit1 < nobs)) {
+            upper = lower;
+            if (cand0 < hit0 && ix.bits[cand0]) { upper.op_inplace(*ix.bits[cand0], OP_OR); upper.compress(); }
+            if (cand1 > hit1 && hit1 < nobs && ix.bits[hit1]) { upper.op_inplace(*ix.bits[hit1], OP_OR); upper.compress(); }
+        }
+    } else {
+        lower.set(0, ix.nrows);
+        upper = sum_bins(ix, cand0, cand1);
+    }
+}
+
 // part::doScan over every row (src/part.cpp:11130-11237 for integer
 // columns, 12094-12219 / 12849- for float / double).
 BV scan(const void* col, int type, uint32_t n, double lo, int lop, double hi, int rop) {

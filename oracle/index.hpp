@@ -56,6 +56,12 @@ double scan_cost(const Index& ix, int type, int model);
 // misses the column's [min, max] and nothing is evaluated.
 int auto_plan(const Index& ix, const void* col, int type, double lo, int lop, double hi, int rop,
               int model, bool* empty);
+// bin::estimate(expr, lower, upper) -- src/ibin.cpp:6722-6829; for a relic
+// (src/irelic.h:58-62) lower is evaluate's result and upper is cleared.
+// Rows of lower are hits, rows of upper candidates; a cleared upper (size
+// 0) stands for lower itself.
+void estimate(const Index& ix, const void* col, int type, double lo, int lop, double hi, int rop,
+              BV& lower, BV& upper);
 // plan 0 (the reference's choice), PLAN_INDEX or PLAN_SCAN.
 BV evaluate_plan(const Index& ix, const void* col, int type, double lo, int lop, double hi, int rop,
                  int plan, int model, int* used);

@@ -57,6 +57,22 @@ int orc_index_eval(void* ix, const void* col, int type, double lo, int lop, doub
     }
 }
 
+// estimate: the serialized words of lower and upper (upper cleared: one
+// word, 0)
+int orc_index_estimate(void* ix, const void* col, int type, double lo, int lop, double hi, int rop,
+                       uint32_t** lw, size_t* nl, uint32_t** uw, size_t* nu) {
+    try {
+        BV lower, upper;
+        estimate(*(Index*)ix, col, type, lo, lop, hi, rop, lower, upper);
+        std::vector<word_t> a = lower.serial(), b = upper.serial();
+        *lw = dup(a); *nl = a.size();
+        *uw = dup(b); *nu = b.size();
+        return 0;
+    } catch (...) {
+        return -2;
+    }
+}
+
 // plan: 0 the reference's choice under `model`, 1 index, 3 scan
 int orc_index_eval_plan(void* ix, const void* col, int type, double lo, int lop, double hi, int rop,
                         int plan, int model, uint64_t* count, uint32_t** rids, size_t* nr,

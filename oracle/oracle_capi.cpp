@@ -60,6 +60,26 @@ int orc_wah_c2(const uint32_t* a, size_t na, const uint32_t* b, size_t nb,
     }
 }
 
+// One *_c2 routine forced, its words as the loop emits them (no compress()).
+int orc_wah_c2_raw(const uint32_t* a, size_t na, const uint32_t* b, size_t nb,
+                   int op, uint32_t** out, size_t* nout) {
+    try {
+        BV x = BV::from_serial(a, na);
+        BV y = BV::from_serial(b, nb);
+        x.size(); y.size();
+        BV res;
+        switch (op) {
+        case OP_AND: x.and_c2(y, res); break;
+        case OP_OR: x.or_c2(y, res); break;
+        case OP_XOR: x.xor_c2(y, res); break;
+        default: x.minus_c2(y, res); break;
+        }
+        return hand_out(res.serial(), out, nout);
+    } catch (...) {
+        return -2;
+    }
+}
+
 // Canonical WAH of a bit string given as bytes of 0/1 (operator+= per bit).
 int orc_wah_from_bits(const uint8_t* bits, size_t n, uint32_t** out, size_t* nout) {
     BV x;

@@ -308,6 +308,34 @@ extern "C" void ref_set_verbose(int v) { ensure_init(); ibis::gVerbose = v; }
 #include "bord.h"
 ibis::bord::column* __fastbit_iapi_array_by_name(const char* name);
 
+// index::estimate(expr, lower, upper) of the index of a registered array
+// (bin::estimate, src/ibin.cpp:6722-6829; relic: src/irelic.h:58-62): the
+// serialized words of lower and upper.
+extern "C" int ref_iapi_estimate(const char* name, double lo, int lop, double hi, int rop, uint32_t** lw,
+                                 size_t* nl, uint32_t** uw, size_t* nu) {
+    ensure_init();
+    ibis::bord::column* col = __fastbit_iapi_array_by_name(name);
+    if (col == 0) return -1;
+    ibis::column::indexLock lock(col, "ref_iapi_estimate");
+    const ibis::index* ix = lock.getIndex();
+    if (ix == 0) return -2;
+    auto op = [](int o) {
+        return o == 1 ? ibis::qExpr::OP_LT : o == 2 ? ibis::qExpr::OP_LE : ibis::qExpr::OP_UNDEFINED;
+    };
+    try {
+        ibis::qContinuousRange q(lo, op(lop), name, op(rop), hi);
+        ibis::bitvector lower, upper;
+        ix->estimate(q, lower, upper);
+        ibis::array_t<ibis::bitvector::word_t> a, b;
+        lower.write(a);
+        upper.write(b);
+        if (hand_out(a.begin(), a.size(), lw, nl)) return -3;
+        return hand_out(b.begin(), b.size(), uw, nu);
+    } catch (...) {
+        return -4;
+    }
+}
+
 extern "C" int ref_iapi_bin_bounds(const char* name, double** out, uint64_t* n) {
     ensure_init();
     ibis::bord::column* col = __fastbit_iapi_array_by_name(name);

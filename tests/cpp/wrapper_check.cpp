@@ -73,9 +73,13 @@ void check_ops(uint32_t n) {
         EXPECT(c.size() == n && d.size() == n);
         EXPECT(c.cnt() == want.size() && d.cnt() == want.size());
         EXPECT(c.rids() == want && d.rids() == want);
+        // the out-of-place result is the routine's words, not compressed:
+        // compressed, it is the in-place result
         std::vector<uint32_t> sc, sd;
         c.write(sc);
-        d.write(sd);
+        fbh::bitvector dc(d);
+        dc.compress();
+        dc.write(sd);
         EXPECT(sc == sd);
         c.flip();
         EXPECT(c.cnt() == n - want.size());

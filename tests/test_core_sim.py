@@ -50,6 +50,12 @@ def c2_oracle(orc, a, b, op):
     return orc._call_out("wah_c2", pa, na, pb, nb, C.c_int(op))
 
 
+def c2_raw_oracle(orc, a, b, op):
+    pa, na = O._arr(a)[1:]
+    pb, nb = O._arr(b)[1:]
+    return orc._call_out("wah_c2_raw", pa, na, pb, nb, C.c_int(op))
+
+
 def test_sim_c2_matches_oracle(sim, orc):
     rng = np.random.default_rng(21)
     for t in range(120):
@@ -113,3 +119,28 @@ def test_sim_every_cut_noncanonical_chains(sim, orc):
             assert np.array_equal(got, want), (t, op)
             got, _ = run_sim(sim, b, a, op, 0, rng.integers(0, g + 1, size=40))
             assert np.array_equal(got, c2_oracle(orc, b, a, op)), (t, op, "swapped")
+
+
+def test_sim_raw_c2_words(sim, orc):
+    """The raw mode (MODE_RAW = 2): the *_c2 loop's own words, not
+    compressed (the out-of-place operators return them), from walks that
+    start only where both operands start a code word; every cut point on
+    small pairs, random cuts on larger and non-canonical ones."""
+    rng = np.random.default_rng(25)
+    for t in range(60):
+        n = int(rng.integers(100, 20000))
+        a = orc.from_bits(markov_bits(n, float(rng.choice([0.002, 0.02, 0.2, 0.6])),
+                                      float(rng.choice([1, 3, 40, 300])), rng))
+        b = orc.from_bits(markov_bits(n, float(rng.choice([0.002, 0.02, 0.2, 0.6])),
+                                      float(rng.choice([1, 3, 40, 300])), rng))
+        if t % 3 == 0:
+            a = c2_raw_oracle(orc, a, b, int(rng.integers(0, 4)))
+        g = groups_of(split_serial(a)[0])
+        cuts = np.arange(g + 1) if g < 700 else rng.integers(0, g + 1, size=200)
+        for op in range(4):
+            want = c2_raw_oracle(orc, a, b, op)
+            got, heads = run_sim(sim, a, b, op, 2, cuts)
+            assert np.array_equal(got, want), (t, op)
+            words = split_serial(got)[0]
+            lens = np.where(words > 0x7FFFFFFF, words & 0x3FFFFFFF, 1)
+            assert np.array_equal(heads, np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.uint32))

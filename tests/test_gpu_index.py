@@ -379,3 +379,39 @@ def test_tiled_copy_beyond_2p32_work_items(L):
         np.testing.assert_array_equal(h, want.astype(np.uint64))
     r = ix.rids(11143.0, 2, 11165.0, 1, 1)
     np.testing.assert_array_equal(r, np.nonzero((col >= 11143) & (col < 11165))[0])
+
+
+def test_estimate_matches_oracle(L):
+    """fbh_index_estimate against the oracle's bin::estimate / relic::estimate
+    (pinned against the reference): the rows of lower and upper, and a
+    cleared upper where the reference clears it."""
+    rng = np.random.default_rng(61)
+
+    def rows(words):
+        return O.oracle().rids(words)
+    seen = set()
+    for t in range(12):
+        n = int(rng.integers(2000, 60000))
+        col = rng.normal(size=n).astype(np.float32) if t % 2 else rng.integers(0, 300, size=n).astype(np.int32)
+        if t % 4 == 3:
+            gi, oi = L.Index.relic(col), O.OracleIndex.relic_values(col)
+        else:
+            gi = L.Index.bin_equal_weight(col, int(rng.integers(5, 150)))
+            oi = O.OracleIndex.bin_bounds(col, gi.bounds())
+        for q in range(8):
+            a, b = sorted(rng.choice(col, 2).astype(np.float64))
+            if q % 3 == 0:
+                a -= 0.25
+                b += 0.5
+            lop, rop = int(rng.integers(1, 3)), int(rng.integers(1, 3))
+            glo, gup = gi.estimate(a, lop, b, rop)
+            olo, oup = oi.estimate(a, lop, b, rop)
+            assert np.array_equal(glo.rids(), rows(olo)), (t, q, "lower")
+            assert glo.size() == n
+            if len(oup) == 1:
+                assert gup.size() == 0, (t, q)
+                seen.add("cleared")
+            else:
+                assert gup.size() == n and np.array_equal(gup.rids(), rows(oup)), (t, q, "upper")
+                seen.add("all" if len(rows(oup)) == n else "upper")
+    assert seen == {"cleared", "all", "upper"}

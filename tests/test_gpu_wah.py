@@ -299,3 +299,81 @@ def test_ops_from_several_host_threads(L, orc):
     for th in ths:
         th.join()
     assert not bad, bad[:5]
+
+
+def _forms(orc, a, b):
+    """a canonical, decompressed, and as uncompressed c2 outputs (count-1
+    fills, raw-pushed words)."""
+    return [a, orc.decompress(a), orc.op(a, b, 0, compress=False)[0], orc.op(a, b, 1, compress=False)[0]]
+
+
+def test_out_of_place_operators_match_oracle(L, orc):
+    """a & b, a | b, a ^ b, a - b through fbh_bv_op_new: the words of the
+    reference's out-of-place dispatch (c0 / c1 / d2 with the strict size
+    test / c2 loop words / shortcut copies, nothing compressed; unequal
+    sizes through the extended copy and the in-place operator), word for
+    word against the oracle, which is pinned against the reference."""
+    rng = np.random.default_rng(41)
+    paths = set()
+    for t in range(150):
+        a, b, n = random_pair(rng, orc, 1, 6000)
+        if t % 5 == 0 and n > 40:
+            b = orc.from_bits(markov_bits(n + int(rng.integers(-40, 40)), 0.2, 5, rng))
+        x = _forms(orc, a, b)[int(rng.integers(0, 4))]
+        y = _forms(orc, b, a)[int(rng.integers(0, 4))]
+        X, Y = L.Bitvector.from_serial(x), L.Bitvector.from_serial(y)
+        for op in range(4):
+            want, wpath = orc.op_new(x, y, op)
+            r, gpath = X.op_new(Y, op)
+            assert gpath == wpath, (t, op, gpath, wpath)
+            assert np.array_equal(r.write(), want), (t, op, gpath)
+            paths.add(gpath)
+        # the operands are not modified
+        assert np.array_equal(X.write(), x) or orc.cnt(x)[2].tolist() == X.write().tolist()
+    assert {"c0", "c1", "c2", "d2", "shortcut"} <= paths
+
+
+def test_out_of_place_size_boundary(L, orc):
+    """Pairs with (|a|w + |b|w) * 31 == nbits exactly: the out-of-place
+    operator takes the c2 loop where the in-place one takes d2."""
+    rng = np.random.default_rng(42)
+    hit = 0
+    for t in range(4000):
+        g = int(rng.integers(4, 60))
+        n = 31 * g
+        a = orc.from_bits(markov_bits(n, 0.3, 40, rng))
+        b = orc.from_bits(markov_bits(n, 0.3, 40, rng))
+        na, nb = len(a) - 1, len(b) - 1          # no active word: n is whole groups
+        if (na + nb) != g:
+            continue
+        hit += 1
+        X, Y = L.Bitvector.from_serial(a), L.Bitvector.from_serial(b)
+        for op in range(4):
+            want, wpath = orc.op_new(a, b, op)
+            r, gpath = X.op_new(Y, op)
+            assert gpath == wpath and np.array_equal(r.write(), want), (t, op, gpath)
+            _, ipath = orc.op(a, b, op)
+            assert ipath in ("d2", "shortcut"), ipath
+        if hit >= 40:
+            break
+    assert hit >= 10
+
+
+def test_count_mask_and_erase_match_oracle(L, orc):
+    """count(mask) against the oracle (pinned against the reference and the
+    popcount of the AND), every operand form; erase(i, j) word for word,
+    cuts inside literals, inside fills and in the active word."""
+    rng = np.random.default_rng(43)
+    for t in range(150):
+        a, b, n = random_pair(rng, orc, 1, 8000)
+        x = _forms(orc, a, b)[t % 4]
+        y = _forms(orc, b, a)[(t // 4) % 4]
+        X, Y = L.Bitvector.from_serial(x), L.Bitvector.from_serial(y)
+        assert X.count(Y) == orc.count(x, y), t
+        sz = orc.cnt(x)[1]
+        for _ in range(3):
+            i = int(rng.integers(0, sz))
+            j = int(rng.integers(i, sz + 40)) if t % 3 else int(rng.integers(i, min(sz, i + 64) + 1))
+            Z = L.Bitvector.from_serial(x)
+            Z.erase(i, j)
+            assert np.array_equal(Z.write(), orc.erase(x, i, j)), (t, i, j, sz)

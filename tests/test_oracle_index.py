@@ -256,3 +256,45 @@ def test_query_count_kat():
             rr = idx[(q["table"], c)].evaluate_plan(lo, lop, hi, rop)[1]
             r = rr if r is None else np.intersect1d(r, rr)
         assert len(r) == q["count"], q["text"]
+
+
+def test_estimate_matches_reference(orc, ref):
+    """index::estimate(expr, lower, upper): the oracle's lower and upper rows
+    against the reference's bin::estimate on the same bins (float and int32
+    columns, equal-weight bins, every branch: no candidate, the cost
+    give-up, hits with edge bins, edge bins only).  The reference index is
+    first warmed by one estimate, so that sumBins has filled offset64 (the
+    state the restatement describes)."""
+    import oracle as O
+    rng = np.random.default_rng(3)
+
+    def bits(words, n):
+        b = np.zeros(n, np.uint8)
+        b[orc.rids(words)] = 1
+        return b
+    kinds = set()
+    for t in range(24):
+        n = int(rng.integers(1000, 40000))
+        col = (rng.normal(size=n) if t % 2 else rng.integers(0, 500, size=n)).astype(np.float32 if t % 2 else np.int32)
+        R = O.RefIapi(f"est{t}", col, "<binning nbins=%d equal-weight/>" % int(rng.integers(5, 200)))
+        m = float(np.median(col))
+        R.estimate(m, 2, m + 1e-3, 1)
+        oi = O.OracleIndex.bin_bounds(col, O.bounds_from_keys(R.keys))
+        for q in range(8):
+            a, b = sorted(rng.choice(col, 2).astype(np.float64))
+            if t % 3 == 0:
+                a -= 0.25
+                b += 0.5
+            if q == 7:
+                b = a                      # an empty range
+            lop, rop = int(rng.integers(1, 3)), int(rng.integers(1, 3))
+            rl, ru = R.estimate(a, lop, b, rop)
+            ol, ou = oi.estimate(a, lop, b, rop)
+            assert np.array_equal(bits(rl, n), bits(ol, n)), (t, q)
+            assert (len(ru) == 1) == (len(ou) == 1), (t, q)
+            if len(ru) > 1:
+                assert np.array_equal(bits(ru, n), bits(ou, n)), (t, q)
+                kinds.add("all" if orc.cnt(ru)[0] == n else "upper")
+            else:
+                kinds.add("cleared")
+    assert kinds == {"all", "upper", "cleared"}
