@@ -341,6 +341,16 @@ int fbh_conj_tune(fbh_conj* c, int* threads, uint32_t* blocks, uint32_t* tiles_p
  * caller's term order. */
 int fbh_conj_plans(const fbh_conj* c, int* plans);
 void fbh_conj_free(fbh_conj* c);
+/* A batch of single range queries on one index as a conjunction batch of
+ * one-term queries (run, tune and free it with fbh_conj_*): one persistent
+ * launch of the range kernel over (query, tiles) work items, captured in a
+ * hipGraph and autotuned by fbh_conj_tune; counts only.  plans[i] (or
+ * `plan` when plans is NULL) picks each query's plan; FBH_PLAN_AUTO applies
+ * the heuristic, and the compressed plan runs as the literal one (the same
+ * bits).  (No counterpart in the reference, which evaluates one selection
+ * at a time, src/iapi.cpp:1383-1418.) */
+int fbh_index_batch_prepare(fbh_index* ix, uint32_t nq, const double* lo, const int* lop, const double* hi,
+                            const int* rop, const int* plans, int plan, fbh_conj** out);
 /* One conjunction: *hits receives the count. */
 int fbh_conj_evaluate(const fbh_term* terms, uint32_t nterms, int plan, uint64_t* hits);
 

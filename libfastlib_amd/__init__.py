@@ -6,10 +6,10 @@ HIP kernels behind a C ABI (``include/fastbit_hip.h``,
 ABI; the C++ host binding is ``include/fastbit_hip.hpp``.
 """
 from . import native  # noqa: F401
-from .native import (AND, OR, XOR, MINUS, Bitvector, FastbitError, Index, Conjunctions, Expressions,
+from .native import (AND, OR, XOR, MINUS, Bitvector, FastbitError, Index, Conjunctions, Expressions, RangeBatch,
                      EXPR_AND, EXPR_OR, EXPR_XOR, EXPR_NAND, EXPR_NOR, EXPR_NOT, EXPR_MINUS, bench_op,
                      bench_op_rot, bench_cnt_rot, lib)  # noqa: F401
 
-__all__ = ["AND", "OR", "XOR", "MINUS", "Bitvector", "FastbitError", "Index", "Conjunctions", "Expressions",
+__all__ = ["AND", "OR", "XOR", "MINUS", "Bitvector", "FastbitError", "Index", "Conjunctions", "Expressions", "RangeBatch",
            "EXPR_AND", "EXPR_OR", "EXPR_XOR", "EXPR_NAND", "EXPR_NOR", "EXPR_NOT", "EXPR_MINUS", "bench_op",
            "bench_op_rot", "bench_cnt_rot", "lib"]

@@ -1589,6 +1589,8 @@ struct fbh_conj {
     std::vector<int> plans;              // plan of every term (caller's order)
     std::vector<uint32_t> order;         // evaluation order: order[k] = caller's index
     int nt = 256;                        // threads per block
+    int single_type = -1;                // >= 0: single-term queries on one column type
+    bool has_scan = false;               // some term runs the scan plan
     uint32_t blocks = 0;                 // persistent grid
     uint32_t tpi = 1;                    // tiles per work item
     hipGraph_t graph = nullptr;
@@ -1654,7 +1656,21 @@ int conj_upload(fbh_conj* c, const std::vector<fbk::QueryArgs>& h, const uint32_
     if (!rc && prog) rc = to_dev(c->dprog, prog, nprog);
     if (!rc && prog) rc = to_dev(c->dpoff, poff, nq + 1);
     if (!rc) rc = stream_sync();
-    c->blocks = 4u * (uint32_t)cu_count();
+    // single-term queries on one column type run the persistent range
+    // kernel: 256 threads, 2 tiles per work item, 9 blocks per CU (the
+    // tuner's choice on 2^27 rows with 10,000 bins; all resident by LDS)
+    bool single = !prog;
+    for (uint32_t q = 0; single && q < nq; ++q) single = qoff[q + 1] == qoff[q] + 1;
+    for (uint32_t k = 0; single && k < c->nterms; ++k) single = h[k].type == h[0].type;
+    if (single) {
+        c->single_type = h[0].type;
+        for (uint32_t k = 0; k < c->nterms; ++k) c->has_scan |= h[k].plan == fbk::PLAN_SCAN_D;
+        c->nt = 256;
+        c->tpi = 2;
+        c->blocks = 9u * (uint32_t)cu_count();
+    } else {
+        c->blocks = 4u * (uint32_t)cu_count();
+    }
     if (!rc) rc = conj_capture(c);
     return rc;
 }
@@ -1664,7 +1680,9 @@ int conj_launch(fbh_conj* c, hipStream_t s) {
     a.terms = c->dterms; a.qoff = c->dqoff; a.nq = c->nq; a.ntiles = c->ntiles;
     a.groups = c->groups; a.anb = c->anb; a.tpi = c->tpi; a.part = c->parts; a.out = nullptr;
     a.prog = c->dprog; a.poff = c->dpoff; a.depth = c->depth; a.skip = c->dskip;
-    int rc = check(fbk::launch_conj(a, c->nt, c->blocks, s), "conjunctions");
+    int rc = c->single_type >= 0
+                 ? check(fbk::launch_range(a, c->single_type, c->has_scan, c->nt, c->blocks, s), "range batch")
+                 : check(fbk::launch_conj(a, c->nt, c->blocks, s), "conjunctions");
     if (!rc) rc = check(fbk::launch_sum_parts(c->parts, c->ntiles, c->dnp, c->nq, c->counts, s), "sum parts");
     return rc;
 }
@@ -1854,8 +1872,10 @@ int fbh_conj_tune(fbh_conj* c, int* nt, uint32_t* blocks, uint32_t* tpi, float* 
     FBH_API_LOCK();
     if (!c) return fail(FBH_ERR_ARG, "null batch");
     static const int NTS[] = {128, 256, 512};
+    static const int NTS1[] = {192, 256, 256};        // single-term batches
     static const uint32_t TPIS[] = {1, 2, 4};
     static const uint32_t BPC[] = {2, 4, 8};
+    static const uint32_t BPC1[] = {4, 8, 9};
     int bnt = c->nt;
     uint32_t bblocks = c->blocks, btpi = c->tpi;
     float best = 0.f;
@@ -1864,7 +1884,10 @@ int fbh_conj_tune(fbh_conj* c, int* nt, uint32_t* blocks, uint32_t* tpi, float* 
     for (int a = 0; a < 3 && !rc; ++a)
         for (int b = 0; b < 3 && !rc; ++b)
             for (int d = 0; d < 3 && !rc; ++d) {
-                c->nt = NTS[a]; c->tpi = TPIS[b]; c->blocks = BPC[d] * (uint32_t)cu_count();
+                if (c->single_type >= 0 && a == 2) continue;
+                c->nt = c->single_type >= 0 ? NTS1[a] : NTS[a];
+                c->tpi = TPIS[b];
+                c->blocks = (c->single_type >= 0 ? BPC1[d] : BPC[d]) * (uint32_t)cu_count();
                 rc = conj_capture(c);
                 float ms = 0.f;
                 if (!rc) rc = conj_run(c, &ms);
@@ -1889,6 +1912,41 @@ int fbh_conj_plans(const fbh_conj* c, int* plans) {
 void fbh_conj_free(fbh_conj* c) {
     FBH_API_LOCK();
     delete c;
+}
+
+int fbh_index_batch_prepare(fbh_index* ix, uint32_t nq, const double* lo, const int* lop, const double* hi,
+                            const int* rop, const int* plans, int plan, fbh_conj** out) {
+    FBH_API_LOCK();
+    if (!ix || !lo || !lop || !hi || !rop || !out || nq == 0) return fail(FBH_ERR_ARG, "bad argument");
+    std::vector<fbh_term> t(nq);
+    std::vector<uint32_t> qoff(nq + 1);
+    for (uint32_t i = 0; i < nq; ++i) {
+        if (plans && (plans[i] < FBH_PLAN_AUTO || plans[i] > FBH_PLAN_SCAN)) return fail(FBH_ERR_ARG, "bad plan");
+        t[i] = fbh_term{ix, lo[i], lop[i], hi[i], rop[i]};
+        qoff[i] = i;
+    }
+    qoff[nq] = nq;
+    if (!plans) return fbh_conj_prepare(t.data(), qoff.data(), nq, plan, out);
+    // per-query plans: prepare under the literal plan, then switch the scan
+    // terms (the device descriptors carry the plan)
+    int rc = fbh_conj_prepare(t.data(), qoff.data(), nq, FBH_PLAN_LITERAL, out);
+    if (rc) return rc;
+    fbh_conj* c = *out;
+    std::vector<fbk::QueryArgs> h(nq);
+    bool scan = false;
+    for (uint32_t i = 0; i < nq; ++i) {
+        int p = plans[i] == FBH_PLAN_AUTO ? plan_heuristic(ix, lo[i], lop[i], hi[i], rop[i]) : plans[i];
+        if (p != FBH_PLAN_SCAN) p = FBH_PLAN_LITERAL;
+        c->plans[i] = p;
+        scan |= p == FBH_PLAN_SCAN;
+        term_args(t[i], p, h[i]);
+    }
+    c->has_scan = scan;
+    rc = to_dev(c->dterms, h.data(), nq);
+    if (!rc) rc = stream_sync();
+    if (!rc) rc = conj_capture(c);
+    if (rc) { delete c; *out = nullptr; }
+    return rc;
 }
 
 int fbh_conj_evaluate(const fbh_term* terms, uint32_t nterms, int plan, uint64_t* hits) {

@@ -109,6 +109,10 @@ hipError_t launch_query_lit(const QueryArgs& q, int type, hipStream_t s);
 hipError_t launch_query_scan(const QueryArgs& q, int type, uint32_t nrows, hipStream_t s);
 // nt: threads per block (128, 256 or 512); blocks: persistent grid size
 hipError_t launch_conj(const ConjArgs& a, int nt, uint32_t blocks, hipStream_t s);
+// A batch whose queries are single terms on one index of column `type`
+// (qoff[q + 1] = qoff[q] + 1): the persistent range kernel; scan: some
+// term runs the scan plan.  nt: 192 or 256.
+hipError_t launch_range(const ConjArgs& a, int type, bool scan, int nt, uint32_t blocks, hipStream_t s);
 uint32_t scan_blocks(uint32_t groups, uint32_t anb);
 // counts[i] = sum of parts[i*stride, i*stride + np[i])
 hipError_t launch_sum_parts(const uint32_t* parts, uint32_t stride, const uint32_t* np, uint32_t nq,

@@ -1094,6 +1094,58 @@ __global__ __launch_bounds__(NT) void k_conj(ConjArgs a) {
     }
 }
 
+// Batches of single range terms (the range-query batch): the persistent
+// form of k_query_lit over (query, TPI consecutive tiles) work items in
+// grid-stride order, so that one launch serves the whole batch and the
+// blocks of one query's last tiles overlap the next query's first.  The
+// column type is a template parameter (one index per batch); SCAN adds the
+// ballot staging of the raw-scan plan for batches that mix the plans.
+// Counts only: each tile's ones go to part[q * ntiles + tile], the literal
+// result is not written unless a.out is set (single-query batches).
+template <class T, int NT, bool SCAN>
+__global__ __launch_bounds__(NT) void k_range(ConjArgs a) {
+    __shared__ word_t R[QT + 1];
+    __shared__ uint32_t cov[QT + 1];
+    __shared__ uint32_t red[NT / WAVE + 1];
+    __shared__ uint64_t bal[SCAN ? NT / WAVE : 1][32];
+    __shared__ uint32_t ctab[CT_WORDS];
+    const uint32_t nchunk = (a.ntiles + a.tpi - 1) / a.tpi;
+    const uint32_t nitems = a.nq * nchunk;
+    const int w = threadIdx.x / WAVE;
+    for (uint32_t item = blockIdx.x; item < nitems; item += gridDim.x) {
+        const uint32_t qi = item / nchunk;
+        const uint32_t c = item - qi * nchunk;
+        const QueryArgs& q = a.terms[a.qoff[qi]];
+        for (uint32_t tile = c * a.tpi; tile < min((c + 1) * a.tpi, a.ntiles); ++tile) {
+            const bool last = tile + 1 == a.ntiles;
+            const uint32_t g0 = tile * (uint32_t)QT;
+            const uint32_t ng = last ? (a.groups - g0) + (a.anb ? 1u : 0u) : (uint32_t)QT;
+            if (SCAN && q.plan == PLAN_SCAN_D) {
+                scan_tile<T, NT>(q, g0, ng, R, bal);
+            } else {
+                for (uint32_t k = threadIdx.x; k <= QT; k += NT) { R[k] = 0u; cov[k] = 0u; }
+                __syncthreads();
+                lit_tile<T, NT, QB_U>(q, tile, g0, ng, last, R, cov, red, ctab);
+            }
+            uint32_t ones = 0;
+            for (uint32_t i = threadIdx.x; i < ng; i += NT) {
+                const word_t v = R[i];
+                if (a.out) a.out[g0 + i] = v;
+                ones += __popc(v);
+            }
+            ones = wave_sum(ones);
+            if (lane_id() == 0) red[w] = ones;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                uint32_t t = 0;
+                for (int i = 0; i < NT / WAVE; ++i) t += red[i];
+                a.part[(size_t)qi * a.ntiles + tile] = t;
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // Boolean expressions over range terms (countQuery::doEvaluate,
 // src/countQuery.cpp:944-1003, with the mask of every row): per tile the
 // block runs the query's postfix program over a stack of tile results in
@@ -1554,6 +1606,33 @@ hipError_t launch_conj(const ConjArgs& a, int nt, uint32_t blocks, hipStream_t s
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+template <class T, bool SCAN>
+static hipError_t launch_range_t(const ConjArgs& a, int nt, uint32_t blocks, hipStream_t s) {
+    switch (nt) {
+    case 192: hipLaunchKernelGGL((k_range<T, 192, SCAN>), dim3(blocks), dim3(192), 0, s, a); break;
+    case 256: hipLaunchKernelGGL((k_range<T, 256, SCAN>), dim3(blocks), dim3(256), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <bool SCAN>
+static hipError_t launch_range_s(const ConjArgs& a, int type, int nt, uint32_t blocks, hipStream_t s) {
+    switch (type) {
+    case COL_I32: return launch_range_t<int32_t, SCAN>(a, nt, blocks, s);
+    case COL_U32: return launch_range_t<uint32_t, SCAN>(a, nt, blocks, s);
+    case COL_U16: return launch_range_t<uint16_t, SCAN>(a, nt, blocks, s);
+    case COL_F32: return launch_range_t<float, SCAN>(a, nt, blocks, s);
+    case COL_F64: return launch_range_t<double, SCAN>(a, nt, blocks, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_range(const ConjArgs& a, int type, bool scan, int nt, uint32_t blocks, hipStream_t s) {
+    if (!a.nq) return hipSuccess;
+    return scan ? launch_range_s<true>(a, type, nt, blocks, s) : launch_range_s<false>(a, type, nt, blocks, s);
 }
 
 hipError_t launch_query_scan(const QueryArgs& q, int type, uint32_t nrows, hipStream_t s) {

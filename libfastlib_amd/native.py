@@ -598,6 +598,7 @@ def _conj_sigs(L):
         "fbh_conj_evaluate": [C.POINTER(_Term), C.c_uint32, C.c_int, _u64p],
         "fbh_expr_prepare": [C.POINTER(_Term), _u32p, C.POINTER(C.c_int32), _u32p, C.c_uint32, C.c_int,
                              C.POINTER(vp)],
+        "fbh_index_batch_prepare": [vp, C.c_uint32, _dp, _ip, _dp, _ip, _ip, C.c_int, C.POINTER(vp)],
     }.items():
         f = getattr(L, name)
         f.argtypes = args
@@ -667,6 +668,32 @@ def _f32(a, shape):
 
 
 EXPR_AND, EXPR_OR, EXPR_XOR, EXPR_NAND, EXPR_NOR, EXPR_NOT, EXPR_MINUS = -1, -2, -3, -4, -5, -6, -7
+
+
+class RangeBatch(Conjunctions):
+    """A prepared batch of single range queries on one index
+    (fbh_index_batch_prepare): one persistent, graph-captured launch;
+    run / tune / plans as for Conjunctions."""
+
+    def __init__(self, ix, lo, lop, hi, rop, plan=PLAN_AUTO, plans=None):
+        L = Index._lib()
+        if not getattr(L, "_conj_sigs", False):
+            _conj_sigs(L)
+            L._conj_sigs = True
+        self._L = L
+        self._keep = [ix]
+        nq, lo, lop, hi, rop = Index._queries(lo, lop, hi, rop)
+        pp = None if plans is None else np.ascontiguousarray(np.broadcast_to(plans, (nq,)), np.int32)
+        self._arrays = (lo, lop, hi, rop, pp)
+        self.nq = nq
+        self.nterms = nq
+        h = C.c_void_p()
+        self._h = None
+        check(L.fbh_index_batch_prepare(ix._h, nq, lo.ctypes.data_as(_dp), lop.ctypes.data_as(_ip),
+                                        hi.ctypes.data_as(_dp), rop.ctypes.data_as(_ip),
+                                        None if pp is None else pp.ctypes.data_as(_ip), int(plan), C.byref(h)),
+              "fbh_index_batch_prepare")
+        self._h = h
 
 
 class Expressions(Conjunctions):

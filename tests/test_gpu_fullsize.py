@@ -66,6 +66,21 @@ def test_bin_f32_batch_counts(binf32):
             assert np.all(used == plan)
 
 
+def test_bin_f32_range_batch(L, binf32):
+    """The 1024 ranges as one persistent graph-captured launch
+    (fbh_index_batch_prepare) under the heuristic and scan plans, untuned
+    and tuned: the reference's counts."""
+    col, ix, g = binf32
+    want = g["counts"].astype(np.uint64)
+    for plan in (0, 3):
+        rb = L.RangeBatch(ix, g["lo"], 2, g["hi"], 1, plan=plan)
+        h, _ = rb.run()
+        assert np.array_equal(h, want), plan
+        if plan == 0:
+            rb.tune()
+            assert np.array_equal(rb.run()[0], want)
+
+
 def test_bin_f32_rids(binf32):
     """Row ids of every 16th range equal the reference's (SHA-256), through
     the literal plan; the scan plan on a quarter of them."""

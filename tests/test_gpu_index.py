@@ -415,3 +415,40 @@ def test_estimate_matches_oracle(L):
                 assert gup.size() == n and np.array_equal(gup.rids(), rows(oup)), (t, q, "upper")
                 seen.add("all" if len(rows(oup)) == n else "upper")
     assert seen == {"cleared", "all", "upper"}
+
+
+def test_range_batch_matches_evaluate_batch(L):
+    """fbh_index_batch_prepare (one persistent graph-captured launch) gives
+    the counts of the per-query path on relic and bin indexes of every
+    column type, under the heuristic, literal and scan plans, per-query
+    plan mixes (the kernel with scan staging), after tuning, and on
+    repeated replays."""
+    rng = np.random.default_rng(71)
+    cols = [rng.integers(-50, 400, size=31 * 3000 + 17).astype(np.int32),
+            rng.integers(0, 600, size=31 * 2500 + 3).astype(np.uint32),
+            rng.integers(0, 65535, size=31 * 4000 + 29).astype(np.uint16),
+            rng.normal(size=31 * 5000 + 11).astype(np.float32),
+            (rng.normal(size=31 * 2200 + 5) * 10).astype(np.float64)]
+    for ci, col in enumerate(cols):
+        if ci < 3:
+            ix = L.Index.relic(col)
+        elif ci == 3:
+            ix = L.Index.bin_equal_weight(col, 200)
+        else:                                   # double: explicit bounds
+            qb = np.unique(np.quantile(col, np.linspace(0, 1, 150)[1:-1]))
+            ix = L.Index.bin(col, np.concatenate([qb, [np.finfo(np.float64).max]]))
+        srt = np.sort(col.astype(np.float64))
+        a = srt[rng.integers(0, len(col), 96)]
+        b = srt[rng.integers(0, len(col), 96)]
+        lo, hi = np.minimum(a, b), np.maximum(a, b) + (0.5 if ci < 3 else 0.0)
+        want, _, _ = ix.evaluate_batch(lo, 2, hi, 1, plan=1)
+        for plan in (0, 1, 3):
+            rb = L.RangeBatch(ix, lo, 2, hi, 1, plan=plan)
+            h, _ = rb.run()
+            assert np.array_equal(h, want), (ci, plan)
+        mix = rng.integers(0, 4, size=len(lo)).astype(np.int32)
+        rb = L.RangeBatch(ix, lo, 2, hi, 1, plans=mix)
+        assert np.array_equal(rb.run()[0], want), (ci, "mix")
+        cfg = rb.tune()
+        for _ in range(3):
+            assert np.array_equal(rb.run()[0], want), (ci, "tuned", cfg)
