@@ -126,7 +126,7 @@ def relic_ranges(seed, k, card=65536):
     return lo.astype(np.float64), hi.astype(np.float64)
 
 
-def conj_section(L, log2n, nq, steps, learned=None):
+def conj_section(L, log2n, nq, steps, learned=None, cpu=None):
     """4 columns of 2^log2n rows (uint16 uniform -> relic; int32 Zipf(1.1),
     float32 N(0,1), int32 sorted runs -> 10000 equal-weight bins), nq
     queries ANDing 2-4 range terms of selectivity log-uniform in
@@ -162,6 +162,9 @@ def conj_section(L, log2n, nq, steps, learned=None):
                tuned=cfg, term_plans={"literal": int(np.sum(plans == 1)), "scan": int(np.sum(plans == 3))},
                hits_total=int(hits.sum()))
     log("conj", out)
+    if cpu:
+        out["cpu_baseline"] = cpu_baseline_conj(ixs, cols, qs, hits, *cpu)
+        log("conj cpu", out["cpu_baseline"])
     # the same terms combined by boolean programs (fastbit_selection_combine
     # trees): the first two terms ORed, a third one ANDed NOT, a fourth ANDed
     def program(k):
@@ -291,23 +294,152 @@ def learned_section(L, ix, train, test, epochs):
     return out
 
 
-def cpu_baseline(L, ix, col, lo, hi, hits_gpu, sample):
+def host_cpu():
+    """(CPU model, host threads the benchmark may use)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return model, n
+
+
+_RW = None
+
+
+def _ref_worker_init(d, kind):
+    """A host worker of the CPU baseline: the reference library with the
+    GPU-built index(es) attached from the arrays in directory d."""
+    global _RW
+    import oracle
+    if kind == "range":
+        a = {k: np.load(os.path.join(d, k + ".npy")) for k in ("col", "keys", "offs", "bms")}
+        _RW = oracle.RefAttached("cpu_range", a["col"], a["keys"], a["offs"], a["bms"])
+        _RW.count_range(float(a["col"][0]), 2, float(a["col"][0]) + 1.0, 1)     # warm the caches
+    else:
+        _RW = []
+        for i in range(4):
+            a = {k: np.load(os.path.join(d, f"{k}{i}.npy")) for k in ("col", "keys", "offs", "bms")}
+            _RW.append(oracle.RefAttached(f"cpu_t{i}", a["col"], a["keys"], a["offs"], a["bms"]))
+
+
+def _ref_worker_ranges(q):
+    lo, hi = q
+    t = time.perf_counter()
+    c = [_RW.count_range(float(a), 2, float(b), 1) for a, b in zip(lo, hi)]
+    return c, time.perf_counter() - t
+
+
+def _ref_worker_conj(qs):
+    import oracle
+    t = time.perf_counter()
+    c = []
+    for q in qs:
+        terms = [(f"cpu_t{ci}", a, 2, b, 1) for (ci, a, b) in q]
+        c.append(oracle.ref_iapi_expr(terms, [0] + [x for k in range(1, len(q)) for x in (k, -1)]))
+    return c, time.perf_counter() - t
+
+
+def _spawn_pool(arrays, kind, workers):
+    """A pool of spawned host processes, each attaching the arrays (written
+    to a temporary directory) to the reference library."""
+    import multiprocessing as mp
+    import tempfile
+    d = tempfile.mkdtemp(prefix="fbh_cpu_")
+    for k, v in arrays.items():
+        np.save(os.path.join(d, k + ".npy"), v)
+    pool = mp.get_context("spawn").Pool(workers, initializer=_ref_worker_init, initargs=(d, kind))
+    return pool, d
+
+
+def cpu_baseline(ix, col, lo, hi, hits_gpu, sample, workers):
+    """The reference library answering `sample` of the same ranges, each as
+    one two-sided range (a qContinuousRange through
+    fastbit_selection_evaluate) with the GPU-built index attached, the
+    queries spread over host worker processes (one thread each)."""
     import oracle
     if oracle.reference() is None:
         return None
+    import shutil
     keys, offs, bms = ix.deconstruct()
-    R = oracle.RefAttached("bench_c3", col, keys, offs, bms)
-    R.count(float(lo[0]), 2, float(hi[0]), 1)   # warm the reference's caches
-    t = time.time()
-    for i in range(sample):
-        c = R.count(float(lo[i]), 2, float(hi[i]), 1)
-        if c != int(hits_gpu[i]):
-            raise RuntimeError(f"reference count {c} != GPU count {hits_gpu[i]} (query {i})")
-    dt = time.time() - t
-    return dict(value=round(sample / dt, 4), unit="queries/s", cores=1, kind="reference",
-                sample=f"{sample} of the 1024 queries (lo <= x < hi) on the same 2^27-row column "
-                       f"with the GPU-built index attached through fastbit_iapi_attach_full_index; "
-                       f"counts checked equal to the GPU's; {dt:.1f}s on one host thread")
+    pool, d = _spawn_pool(dict(col=col, keys=keys, offs=offs, bms=bms), "range", workers)
+    try:
+        parts = np.array_split(np.arange(sample), workers)
+        t = time.perf_counter()
+        res = pool.map(_ref_worker_ranges, [(lo[p], hi[p]) for p in parts])
+        dt = time.perf_counter() - t
+    finally:
+        pool.close()
+        pool.join()
+        shutil.rmtree(d, ignore_errors=True)
+    counts = [c for r in res for c in r[0]]
+    if not np.array_equal(np.array(counts, np.uint64), hits_gpu[:sample]):
+        raise RuntimeError("reference counts differ from the GPU's")
+    model, _ = host_cpu()
+    busy = sum(r[1] for r in res)
+    return dict(value=round(sample / dt, 3), unit="queries/s", cores=workers, kind="reference",
+                cpu=model, per_thread_qps=round(sample / busy, 4),
+                sample=f"{sample} of the 1024 ranges lo <= x < hi, each one qContinuousRange through "
+                       f"fastbit_selection_evaluate on the same 2^27-row column with the GPU-built index "
+                       f"attached (fastbit_iapi_attach_full_index), spread over {workers} worker processes; "
+                       f"counts checked equal to the GPU's; {dt:.1f}s wall")
+
+
+def cpu_baseline_wah(cells):
+    """The reference's in-place operator plus compress() (one host thread) on
+    the same 2^28-bit Markov operands as the GPU's bandwidth cells."""
+    import oracle
+    R = oracle.reference()
+    if R is None:
+        return None
+    from libfastlib_amd import datagen as G
+    model, _ = host_cpu()
+    out = []
+    for d, run in cells:
+        wa = G.markov_wah(100, 1 << 28, d, run)
+        wb = G.markov_wah(101, 1 << 28, d, run)
+        for op, name in ((0, "AND"), (1, "OR")):
+            ms, nw = R.time_op(wa, wb, op, reps=2)
+            byts = 4.0 * (len(wa) + len(wb) + nw)
+            out.append(dict(cell=f"d={d:g},L={run}", op=name, ms=round(ms, 3), GBs=round(byts / (ms * 1e-3) / 1e9, 2)))
+            log("cpu wah", out[-1])
+    return dict(unit="ms per op (in place + compress)", cores=1, kind="reference", cpu=model, cells=out)
+
+
+def cpu_baseline_conj(ixs, cols, qs, hits_gpu, sample, workers):
+    """The reference answering `sample` of the conjunctions (terms ANDed by
+    fastbit_selection_combine, counted by fastbit_selection_evaluate) with
+    the four GPU-built indexes attached, over host worker processes."""
+    import oracle
+    if oracle.reference() is None:
+        return None
+    import shutil
+    arrays = {}
+    for i, (ix, c) in enumerate(zip(ixs, cols)):
+        k, o, b = ix.deconstruct()
+        arrays.update({f"col{i}": c, f"keys{i}": k, f"offs{i}": o, f"bms{i}": b})
+    pool, d = _spawn_pool(arrays, "conj", workers)
+    try:
+        pick = np.arange(0, len(qs), max(1, len(qs) // sample))[:sample]
+        parts = np.array_split(pick, workers)
+        t = time.perf_counter()
+        res = pool.map(_ref_worker_conj, [[qs[i] for i in p] for p in parts])
+        dt = time.perf_counter() - t
+    finally:
+        pool.close()
+        pool.join()
+        shutil.rmtree(d, ignore_errors=True)
+    counts = np.array([c for r in res for c in r[0]], np.uint64)
+    if not np.array_equal(counts, hits_gpu[pick]):
+        raise RuntimeError("reference conjunction counts differ from the GPU's")
+    model, _ = host_cpu()
+    return dict(value=round(len(pick) / dt, 3), unit="queries/s", cores=workers, kind="reference", cpu=model,
+                sample=f"{len(pick)} of the 4096 conjunctions, the four GPU-built indexes attached, spread over "
+                       f"{workers} worker processes; counts checked equal to the GPU's; {dt:.1f}s wall")
 
 
 def main():
@@ -317,7 +449,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--log2-rows", type=int, default=27)
     ap.add_argument("--queries", type=int, default=1024)
-    ap.add_argument("--cpu-sample", type=int, default=16)
+    ap.add_argument("--cpu-sample", type=int, default=64)
+    ap.add_argument("--cpu-workers", type=int, default=0, help="host processes of the CPU baseline "
+                    "(default: the host threads available, at most 16)")
     ap.add_argument("--no-wah", action="store_true")
     ap.add_argument("--wah-only", type=int, default=0, metavar="REPS",
                     help="time only the WAH AND/OR cells (REPS ops each) and print them as JSON")
@@ -327,9 +461,6 @@ def main():
     ap.add_argument("--train-queries", type=int, default=50000)
     ap.add_argument("--test-queries", type=int, default=8192)
     ap.add_argument("--epochs", type=int, default=40)
-    ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
-                    help="finish with 5 device-to-device copies of MB MiB (calibrates the "
-                         "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; multi-GPU sharding is a design "
@@ -352,52 +483,89 @@ def main():
     nobs, nrows, nwords = ix.info()
     log(f"index: {nobs} bins, {nwords} words, GPU build {build_s:.2f}s")
 
+    # the headline: the ranges as one persistent, graph-captured launch
+    # (fbh_index_batch_prepare) under the ibis heuristic's plans, autotuned
+    # once before the timed steps; a step is one replay (counts to host)
+    rb = L.RangeBatch(ix, lo, 2, hi, 1, plan=0)
+    tuned = rb.tune()
+    log(f"range batch tuned: {tuned}")
     for _ in range(args.warmup):
-        ix.evaluate_batch(lo, 2, hi, 1, 0)
+        rb.run()
     walls, gms = [], []
-    hits = used = None
+    hits = None
     for _ in range(args.steps):
         t = time.perf_counter()
-        hits, used, gm = ix.evaluate_batch(lo, 2, hi, 1, 0)
+        hits, gm = rb.run()
         walls.append(time.perf_counter() - t)
         gms.append(gm)
     wall = float(np.sum(walls))
     qps = args.queries * args.steps / wall
+    used = rb.plans()
     plan_mix = {L.native.PLAN_NAMES[k]: int(np.sum(used == k)) for k in (1, 2, 3)}
-    log(f"heuristic plan: {qps:.1f} q/s, mix {plan_mix}")
+    log(f"heuristic plans, one launch: {qps:.1f} q/s, mix {plan_mix}")
+    # the per-query path (a launch per query over three streams) gives the
+    # same counts
+    h_pq, _, gm_pq = ix.evaluate_batch(lo, 2, hi, 1, 0)
+    if not np.array_equal(h_pq, hits):
+        raise RuntimeError("the per-query path's counts differ from the batch's")
 
-    # per-plan batches (identical counts required)
+    # algorithmic bytes of a counting query: the bitmap words its plan reads
+    # (OR set and edge bitmaps) and the raw values it reads (edge rows, or
+    # the whole column for the scan); the batch writes no literal result
+    groups = nrows // 31
+    res_bytes = 4.0 * (groups + 1)
+    bl = query_bytes(ix, lo, hi, 1) - res_bytes
+    bs = query_bytes(ix, lo, hi, 3) - res_bytes
     per_plan = {}
-    for plan in (1, 3):
-        h, u, gm = ix.evaluate_batch(lo, 2, hi, 1, plan)
+    for plan, qb in ((1, bl), (3, bs)):
+        pb = L.RangeBatch(ix, lo, 2, hi, 1, plan=plan)
+        pb.tune()
+        h, _ = pb.run()
         if not np.array_equal(h, hits):
             raise RuntimeError(f"plan {plan} counts differ from the heuristic plan's")
-        qb = query_bytes(ix, lo, hi, plan)
-        gbs = qb.sum() / (gm * 1e-3) / 1e9
+        gm = float(np.median([pb.run()[1] for _ in range(3)]))
         per_plan[L.native.PLAN_NAMES[plan]] = dict(
             qps=round(len(lo) / (gm * 1e-3), 1), gpu_ms=round(gm, 3),
-            algo_GB=round(qb.sum() / 1e9, 3), GBs=round(gbs, 1))
+            algo_GB=round(qb.sum() / 1e9, 3), GBs=round(qb.sum() / (gm * 1e-3) / 1e9, 1))
         log("plan", L.native.PLAN_NAMES[plan], per_plan[L.native.PLAN_NAMES[plan]])
-    # best single plan per query (oracle of the plan choice, by the bytes model)
-    bl = query_bytes(ix, lo, hi, 1)
-    bs = query_bytes(ix, lo, hi, 3)
-    # the kernel the heuristic step runs for most of its queries
-    dom = max(("literal", "scan"), key=lambda k: plan_mix.get(k, 0))
-    dom_kernel = {"literal": "k_query_lit", "scan": "k_query_scan"}[dom]
-    # consecutive queries of a batch rotate over three streams, so launches
-    # overlap: the time per launch is the batch's device time (HIP
-    # events on the library stream, which joins both) over its launches --
-    # the busy time of the kernel per launch, not one dispatch's span
-    achieved = per_plan[dom]["GBs"]
-    roofline = dict(bound="hbm", kernel=dom_kernel, achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(dom_kernel),
-                    avg_launch_us=round(per_plan[dom]["gpu_ms"] * 1e3 / len(lo), 2))
+        del pb
+    per_query_launches = dict(qps=round(len(lo) / (gm_pq * 1e-3), 1), gpu_ms=round(gm_pq, 3),
+                              note="fbh_index_evaluate_batch: one launch per query rotating over three "
+                                   "streams, literal results written")
+    # roofline of the kernel of the timed step: k_range, one launch per
+    # step, its bytes those of each query's chosen plan, its time the HIP
+    # events around the replay on the library stream (the replay is k_range
+    # and the few-microsecond k_sum_parts)
+    qb_auto = np.where(used == 3, bs, bl)
+    gm_med = float(np.median(gms))
+    achieved = round(qb_auto.sum() / (gm_med * 1e-3) / 1e9, 1)
+    roofline = dict(bound="hbm", kernel="k_range", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic("k_range"),
+                    avg_launch_us=round(gm_med * 1e3, 1), algo_bytes_per_launch=round(float(qb_auto.sum()), 0))
 
+    # row ids (countQuery::getHitRows): every 64th range through the
+    # literal plan, ids copied to host memory (PCIe included)
+    rid_q = np.arange(0, len(lo), 64)
+    ix.rids(float(lo[0]), 2, float(hi[0]), 1)
+    t = time.perf_counter()
+    nr = sum(len(ix.rids(float(lo[i]), 2, float(hi[i]), 1)) for i in rid_q)
+    dt = time.perf_counter() - t
+    rid_output = dict(queries=int(len(rid_q)), rids=int(nr), rids_per_s=round(nr / dt, 0),
+                      GBs=round(4.0 * nr / dt / 1e9, 2),
+                      note="fbh_index_evaluate_rids: literal plan, row-id kernel, ids copied to host "
+                           "memory (PCIe-inclusive wall time)")
+    log("rids", rid_output)
+
+    _, ncpu = host_cpu()
+    workers = max(1, min(args.cpu_workers or ncpu, 16))
     learned = None if args.no_learned else learned_section(L, ix, more[0], more[1], args.epochs)
     wah = None if args.no_wah else wah_cells(L)
     conj, relic_learned = (None, None) if args.no_conj else conj_section(
-        L, 26, 4096, 5, None if args.no_learned else (args.train_queries, args.test_queries, args.epochs))
-    cpu = None if args.no_cpu else cpu_baseline(L, ix, col, lo, hi, hits, args.cpu_sample)
+        L, 26, 4096, 5, None if args.no_learned else (args.train_queries, args.test_queries, args.epochs),
+        cpu=None if args.no_cpu else (32, workers))
+    cpu = None if args.no_cpu else cpu_baseline(ix, col, lo, hi, hits, args.cpu_sample, workers)
+    cpu_wah = None if (args.no_cpu or args.no_wah) else cpu_baseline_wah(
+        [(0.5, 1), (0.5, 64), (0.01, 1)])
 
     out = {
         "metric": "range queries/s at 2^27 rows; WAH op GB/s (% HBM peak); learned vs fixed plan",
@@ -418,7 +586,10 @@ def main():
                    "rows": int(nrows), "bins": int(nobs), "queries": args.queries,
                    "gpu_index_build_s": round(build_s, 2)},
         "plan_mix": plan_mix,
+        "tuned": tuned,
         "per_plan": per_plan,
+        "per_query_launches": per_query_launches,
+        "rid_output": rid_output,
         "best_plan_bytes_model_GB": round(float(np.minimum(bl, bs).sum()) / 1e9, 3),
         "roofline": roofline,
         "learned_plan": learned,
@@ -426,18 +597,9 @@ def main():
         "wah_ops_2p28": wah,
         "conjunctions_4x2p26": conj,
         "cpu_baseline": cpu,
+        "cpu_baseline_wah": cpu_wah,
     }
     print(json.dumps(out))
-    if args.copy_probe:
-        # an elementwise kernel that reads and writes MB MiB each (a plain
-        # copy through the shader, unlike hipMemcpy's DMA path)
-        import torch
-        n = (args.copy_probe << 20) // 4
-        src = torch.zeros(n, dtype=torch.int32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(5):
-            torch.bitwise_not(src, out=dst)
-        torch.cuda.synchronize()
 
 
 if __name__ == "__main__":

@@ -1590,7 +1590,7 @@ struct fbh_conj {
     std::vector<uint32_t> order;         // evaluation order: order[k] = caller's index
     int nt = 256;                        // threads per block
     int single_type = -1;                // >= 0: single-term queries on one column type
-    bool has_scan = false;               // some term runs the scan plan
+    int has_scan = 0;                    // 0 no term runs the scan plan, 1 some, 2 all
     uint32_t blocks = 0;                 // persistent grid
     uint32_t tpi = 1;                    // tiles per work item
     hipGraph_t graph = nullptr;
@@ -1664,7 +1664,9 @@ int conj_upload(fbh_conj* c, const std::vector<fbk::QueryArgs>& h, const uint32_
     for (uint32_t k = 0; single && k < c->nterms; ++k) single = h[k].type == h[0].type;
     if (single) {
         c->single_type = h[0].type;
-        for (uint32_t k = 0; k < c->nterms; ++k) c->has_scan |= h[k].plan == fbk::PLAN_SCAN_D;
+        uint32_t ns = 0;
+        for (uint32_t k = 0; k < c->nterms; ++k) ns += h[k].plan == fbk::PLAN_SCAN_D ? 1u : 0u;
+        c->has_scan = ns == 0 ? 0 : ns == c->nterms ? 2 : 1;
         c->nt = 256;
         c->tpi = 2;
         c->blocks = 9u * (uint32_t)cu_count();
@@ -1933,15 +1935,15 @@ int fbh_index_batch_prepare(fbh_index* ix, uint32_t nq, const double* lo, const 
     if (rc) return rc;
     fbh_conj* c = *out;
     std::vector<fbk::QueryArgs> h(nq);
-    bool scan = false;
+    uint32_t ns = 0;
     for (uint32_t i = 0; i < nq; ++i) {
         int p = plans[i] == FBH_PLAN_AUTO ? plan_heuristic(ix, lo[i], lop[i], hi[i], rop[i]) : plans[i];
         if (p != FBH_PLAN_SCAN) p = FBH_PLAN_LITERAL;
         c->plans[i] = p;
-        scan |= p == FBH_PLAN_SCAN;
+        ns += p == FBH_PLAN_SCAN ? 1u : 0u;
         term_args(t[i], p, h[i]);
     }
-    c->has_scan = scan;
+    c->has_scan = ns == 0 ? 0 : ns == nq ? 2 : 1;
     rc = to_dev(c->dterms, h.data(), nq);
     if (!rc) rc = stream_sync();
     if (!rc) rc = conj_capture(c);

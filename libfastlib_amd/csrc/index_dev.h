@@ -110,9 +110,9 @@ hipError_t launch_query_scan(const QueryArgs& q, int type, uint32_t nrows, hipSt
 // nt: threads per block (128, 256 or 512); blocks: persistent grid size
 hipError_t launch_conj(const ConjArgs& a, int nt, uint32_t blocks, hipStream_t s);
 // A batch whose queries are single terms on one index of column `type`
-// (qoff[q + 1] = qoff[q] + 1): the persistent range kernel; scan: some
-// term runs the scan plan.  nt: 192 or 256.
-hipError_t launch_range(const ConjArgs& a, int type, bool scan, int nt, uint32_t blocks, hipStream_t s);
+// (qoff[q + 1] = qoff[q] + 1): the persistent range kernel; scan: 0 no
+// term runs the scan plan, 1 some do, 2 all do.  nt: 192 or 256.
+hipError_t launch_range(const ConjArgs& a, int type, int scan, int nt, uint32_t blocks, hipStream_t s);
 uint32_t scan_blocks(uint32_t groups, uint32_t anb);
 // counts[i] = sum of parts[i*stride, i*stride + np[i])
 hipError_t launch_sum_parts(const uint32_t* parts, uint32_t stride, const uint32_t* np, uint32_t nq,

@@ -951,7 +951,7 @@ __device__ __forceinline__ void scan_rows64(const QueryArgs& q, const T* col, ui
     }
 }
 
-template <class T, int NT>
+template <class T, int NT, int B = 8>
 __device__ __forceinline__ void scan_tile(const QueryArgs& q, uint32_t g0, uint32_t ng, word_t* R, uint64_t (*bal)[32]) {
     const T* col = (const T*)q.col;
     const int lane = (int)lane_id();
@@ -959,7 +959,7 @@ __device__ __forceinline__ void scan_tile(const QueryArgs& q, uint32_t g0, uint3
     const uint32_t nrows = q.groups * 31u + q.anb;
     for (uint32_t k0 = (uint32_t)w * 64u; k0 < ng; k0 += (NT / WAVE) * 64u) {
         const uint32_t rbase = (g0 + k0) * 31u;
-        scan_rows64<T, 8>(q, col, rbase, nrows, bal[w]);
+        scan_rows64<T, B>(q, col, rbase, nrows, bal[w]);
         __builtin_amdgcn_wave_barrier();
         const uint32_t k = k0 + (uint32_t)lane;
         if (k < ng) {
@@ -1058,7 +1058,8 @@ __global__ __launch_bounds__(NT) void k_conj(ConjArgs a) {
             const uint32_t g0 = tile * (uint32_t)QT;
             const uint32_t ng = last ? (a.groups - g0) + (a.anb ? 1u : 0u) : (uint32_t)QT;
             for (uint32_t k = k0; k < k1; ++k) {
-                term_tile<NT>(a.terms[k], tile, g0, ng, last, k == k0 ? R : R2, cov, red, bal, ctab);
+                const QueryArgs q = a.terms[k];        // in registers (see k_range)
+                term_tile<NT>(q, tile, g0, ng, last, k == k0 ? R : R2, cov, red, bal, ctab);
                 int any = 0;
                 if (k != k0) {
                     for (uint32_t i = threadIdx.x; i < ng; i += NT) {
@@ -1098,11 +1099,13 @@ __global__ __launch_bounds__(NT) void k_conj(ConjArgs a) {
 // form of k_query_lit over (query, TPI consecutive tiles) work items in
 // grid-stride order, so that one launch serves the whole batch and the
 // blocks of one query's last tiles overlap the next query's first.  The
-// column type is a template parameter (one index per batch); SCAN adds the
-// ballot staging of the raw-scan plan for batches that mix the plans.
+// column type is a template parameter (one index per batch); SCAN 1 adds
+// the raw-scan plan for batches that mix the plans (8 row loads in flight,
+// registers shared with the literal plan), SCAN 2 is a batch of raw scans
+// only (31 loads in flight).
 // Counts only: each tile's ones go to part[q * ntiles + tile], the literal
 // result is not written unless a.out is set (single-query batches).
-template <class T, int NT, bool SCAN>
+template <class T, int NT, int SCAN>
 __global__ __launch_bounds__(NT) void k_range(ConjArgs a) {
     __shared__ word_t R[QT + 1];
     __shared__ uint32_t cov[QT + 1];
@@ -1115,12 +1118,47 @@ __global__ __launch_bounds__(NT) void k_range(ConjArgs a) {
     for (uint32_t item = blockIdx.x; item < nitems; item += gridDim.x) {
         const uint32_t qi = item / nchunk;
         const uint32_t c = item - qi * nchunk;
-        const QueryArgs& q = a.terms[a.qoff[qi]];
+        // a copy in registers: a reference into device memory would be
+        // re-read after every store the compiler cannot tell apart from it
+        const QueryArgs q = a.terms[a.qoff[qi]];
         for (uint32_t tile = c * a.tpi; tile < min((c + 1) * a.tpi, a.ntiles); ++tile) {
             const bool last = tile + 1 == a.ntiles;
             const uint32_t g0 = tile * (uint32_t)QT;
             const uint32_t ng = last ? (a.groups - g0) + (a.anb ? 1u : 0u) : (uint32_t)QT;
-            if (SCAN && q.plan == PLAN_SCAN_D) {
+            if (SCAN == 2) {
+                // raw scan only: each wave's groups assembled from ballots and
+                // counted in registers (the structure of k_query_scan)
+                const T* col = (const T*)q.col;
+                const uint32_t nrows = q.groups * 31u + q.anb;
+                const uint32_t lane = lane_id();
+                uint32_t ones = 0;
+                for (uint32_t k0 = (uint32_t)w * 64u; k0 < ng; k0 += (NT / WAVE) * 64u) {
+                    scan_rows64<T, 31>(q, col, (g0 + k0) * 31u, nrows, bal[w]);
+                    __builtin_amdgcn_wave_barrier();
+                    const uint32_t k = k0 + lane;
+                    if (k < ng) {
+                        const uint32_t bit0 = lane * 31u;
+                        const uint32_t kk = bit0 >> 6, s0 = bit0 & 63u;
+                        uint64_t lo = bal[w][kk] >> s0;
+                        if (s0 > 33u && kk + 1 < 31u) lo |= bal[w][kk + 1] << (64u - s0);
+                        word_t v = __brev((uint32_t)(lo & 0x7FFFFFFFull)) >> 1;
+                        if (g0 + k == q.groups) v &= (((1u << q.anb) - 1u) << (31u - q.anb));
+                        if (a.out) a.out[g0 + k] = v;
+                        ones += __popc(v);
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                }
+                ones = wave_sum(ones);
+                if (lane == 0) red[w] = ones;
+                __syncthreads();
+                if (threadIdx.x == 0) {
+                    uint32_t t = 0;
+                    for (int i = 0; i < NT / WAVE; ++i) t += red[i];
+                    a.part[(size_t)qi * a.ntiles + tile] = t;
+                }
+                __syncthreads();
+                continue;
+            } else if (SCAN == 1 && q.plan == PLAN_SCAN_D) {
                 scan_tile<T, NT>(q, g0, ng, R, bal);
             } else {
                 for (uint32_t k = threadIdx.x; k <= QT; k += NT) { R[k] = 0u; cov[k] = 0u; }
@@ -1184,7 +1222,8 @@ __global__ __launch_bounds__(NT) void k_expr(ConjArgs a) {
                 }
                 const int32_t op = a.prog[p];
                 if (op >= 0) {
-                    term_tile<NT>(a.terms[op], tile, g0, ng, last, S[sp], cov, red, bal, ctab);
+                    const QueryArgs q = a.terms[op];       // in registers (see k_range)
+                    term_tile<NT>(q, tile, g0, ng, last, S[sp], cov, red, bal, ctab);
                     ++sp;
                     continue;
                 }
@@ -1608,7 +1647,7 @@ hipError_t launch_conj(const ConjArgs& a, int nt, uint32_t blocks, hipStream_t s
     return hipGetLastError();
 }
 
-template <class T, bool SCAN>
+template <class T, int SCAN>
 static hipError_t launch_range_t(const ConjArgs& a, int nt, uint32_t blocks, hipStream_t s) {
     switch (nt) {
     case 192: hipLaunchKernelGGL((k_range<T, 192, SCAN>), dim3(blocks), dim3(192), 0, s, a); break;
@@ -1618,7 +1657,7 @@ static hipError_t launch_range_t(const ConjArgs& a, int nt, uint32_t blocks, hip
     return hipGetLastError();
 }
 
-template <bool SCAN>
+template <int SCAN>
 static hipError_t launch_range_s(const ConjArgs& a, int type, int nt, uint32_t blocks, hipStream_t s) {
     switch (type) {
     case COL_I32: return launch_range_t<int32_t, SCAN>(a, nt, blocks, s);
@@ -1630,9 +1669,13 @@ static hipError_t launch_range_s(const ConjArgs& a, int type, int nt, uint32_t b
     }
 }
 
-hipError_t launch_range(const ConjArgs& a, int type, bool scan, int nt, uint32_t blocks, hipStream_t s) {
+hipError_t launch_range(const ConjArgs& a, int type, int scan, int nt, uint32_t blocks, hipStream_t s) {
     if (!a.nq) return hipSuccess;
-    return scan ? launch_range_s<true>(a, type, nt, blocks, s) : launch_range_s<false>(a, type, nt, blocks, s);
+    switch (scan) {
+    case 0: return launch_range_s<0>(a, type, nt, blocks, s);
+    case 1: return launch_range_s<1>(a, type, nt, blocks, s);
+    default: return launch_range_s<2>(a, type, nt, blocks, s);
+    }
 }
 
 hipError_t launch_query_scan(const QueryArgs& q, int type, uint32_t nrows, hipStream_t s) {

@@ -170,6 +170,17 @@ class Reference(_Lib):
         b, pb, nb = _arr(b)
         return self._call_out("wah_op_new", pa, na, pb, nb, C.c_int(op))
 
+    def time_op(self, a, b, op, reps=3):
+        """(mean ms of a OP= b; a.compress() on copies of a, result words)."""
+        a, pa, na = _arr(a)
+        b, pb, nb = _arr(b)
+        ms = C.c_double()
+        nw = C.c_uint32()
+        rc = self.lib.ref_wah_time_op(pa, na, pb, nb, C.c_int(op), C.c_int(reps), C.byref(ms), C.byref(nw))
+        if rc != 0:
+            raise RuntimeError(f"ref_wah_time_op failed rc={rc}")
+        return ms.value, nw.value
+
 
 _oracle = None
 _ref = None
@@ -520,6 +531,17 @@ class RefAttached:
 
     def estimate(self, lo, lop, hi, rop):
         return ref_iapi_estimate(self.name, lo, lop, hi, rop)
+
+    def count_range(self, lo, lop, hi, rop):
+        """Hits of lo OP1 x OP2 hi as one range term (one qContinuousRange
+        through fastbit_selection_evaluate)."""
+        L = reference().lib
+        L.ref_iapi_query_range.restype = C.c_int64
+        n = L.ref_iapi_query_range(self.name, C.c_double(lo), C.c_int(lop), C.c_double(hi), C.c_int(rop), None,
+                                   C.c_uint64(0))
+        if n < 0:
+            raise RuntimeError(f"ref_iapi_query_range failed {n}")
+        return int(n)
 
     def count(self, lo, lop, hi, rop):
         n = reference().lib.ref_iapi_query(self.name, C.c_double(lo), C.c_int(lop), C.c_double(hi),

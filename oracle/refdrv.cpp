@@ -6,6 +6,7 @@
 #include "irelic.h"
 #include "ibin.h"
 #include "iapi.h"
+#include <chrono>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -114,6 +115,43 @@ int ref_wah_erase(const uint32_t* a, size_t na, uint32_t i, uint32_t j, uint32_t
         ibis::array_t<ibis::bitvector::word_t> arr;
         x.write(arr);
         return hand_out(arr.begin(), arr.size(), out, nout);
+    } catch (...) {
+        return -2;
+    }
+}
+
+// The in-place operator followed by compress() on copies of a, `reps`
+// times: the mean milliseconds of the op and compress alone (the copy of a
+// is made before the clock starts), and the result's word count.
+int ref_wah_time_op(const uint32_t* a, size_t na, const uint32_t* b, size_t nb, int op, int reps,
+                    double* ms, uint32_t* nwords) {
+    ensure_init();
+    try {
+        ibis::bitvector x0, y;
+        load(x0, a, na);
+        load(y, b, nb);
+        (void) x0.size(); (void) y.size();
+        double tot = 0.0;
+        for (int r = 0; r < reps; ++r) {
+            ibis::bitvector x;
+            x.copy(x0);
+            const auto t0 = std::chrono::steady_clock::now();
+            switch (op) {
+            case 0: x &= y; break;
+            case 1: x |= y; break;
+            case 2: x ^= y; break;
+            default: x -= y; break;
+            }
+            x.compress();
+            tot += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            if (r == 0 && nwords) {
+                ibis::array_t<ibis::bitvector::word_t> arr;
+                x.write(arr);
+                *nwords = (uint32_t)arr.size();
+            }
+        }
+        *ms = tot / (reps > 0 ? reps : 1);
+        return 0;
     } catch (...) {
         return -2;
     }
