@@ -340,6 +340,17 @@ int fbh_conj_tune(fbh_conj* c, int* threads, uint32_t* blocks, uint32_t* tiles_p
 /* The plan each term runs (FBH_PLAN_LITERAL or FBH_PLAN_SCAN), in the
  * caller's term order. */
 int fbh_conj_plans(const fbh_conj* c, int* plans);
+/* Row ids of every query of the batch (countQuery::getHitRows,
+ * src/countQuery.cpp:387-418, for each query): replays the batch for the
+ * counts, then one more persistent launch writes every tile's ids.
+ * offsets[0 .. nq] (required) receives the exclusive prefix of the counts;
+ * the ids of query q, ascending, go to rids[offsets[q] .. offsets[q+1]).
+ * At most `cap` ids are written (rids may be NULL: offsets only). */
+int fbh_conj_rids(fbh_conj* c, uint64_t* offsets, uint32_t* rids, uint64_t cap);
+/* The hit vector of query q (countQuery::getHitVector,
+ * src/countQuery.h:64) as a new bitvector in canonical compressed form
+ * (the caller frees it). */
+int fbh_conj_result(fbh_conj* c, uint32_t q, fbh_bitvector** out);
 void fbh_conj_free(fbh_conj* c);
 /* A batch of single range queries on one index as a conjunction batch of
  * one-term queries (run, tune and free it with fbh_conj_*): one persistent

@@ -93,6 +93,7 @@ private:
     friend bitvector operator|(const bitvector&, const bitvector&);
     friend bitvector operator^(const bitvector&, const bitvector&);
     friend bitvector operator-(const bitvector&, const bitvector&);
+    friend class batch;
     static bitvector op_new(const bitvector& a, const bitvector& b, int which) {
         fbh_bitvector* r = nullptr;
         check(fbh_bv_op_new(a.h_, b.h_, which, &r, nullptr), "fbh_bv_op_new");
@@ -252,6 +253,20 @@ public:
     void tune() {
         int t = 0; uint32_t b = 0, tpi = 0; float ms = 0;
         check(fbh_conj_tune(h_, &t, &b, &tpi, &ms), "fbh_conj_tune");
+    }
+    /* row ids of every query (countQuery::getHitRows per query): query q's
+     * ids are rids[offsets[q] .. offsets[q+1]) */
+    void hit_rows(std::vector<uint64_t>& offsets, std::vector<uint32_t>& rids) {
+        offsets.assign(nq_ + 1, 0);
+        check(fbh_conj_rids(h_, offsets.data(), nullptr, 0), "fbh_conj_rids");
+        rids.resize(offsets[nq_]);
+        check(fbh_conj_rids(h_, offsets.data(), rids.data(), rids.size()), "fbh_conj_rids");
+    }
+    /* the hit vector of query q (countQuery::getHitVector) */
+    bitvector hit_vector(uint32_t q) {
+        fbh_bitvector* v = nullptr;
+        check(fbh_conj_result(h_, q, &v), "fbh_conj_result");
+        return bitvector(v);
     }
 
 private:

@@ -660,22 +660,22 @@ double scan_cost(const fbh_index* ix);
 
 // Canonical WAH of a literal result (groups words, then the active bits
 // left-aligned) as a new bitvector.
-int literal_bitvector(const fbh_index* ix, uint32_t* lit, fbh_bitvector** out) {
+int literal_bitvector(uint32_t groups, uint32_t anb, uint32_t* lit, fbh_bitvector** out) {
     fbh_bitvector* res = nullptr;
     int rc = fbh_bv_new(&res);
     if (rc) return rc;
     uint32_t act = 0;
-    if (ix->anb && (rc = to_host(&act, lit + ix->groups, 1))) { fbh_bv_free(res); return rc; }
+    if (anb && (rc = to_host(&act, lit + groups, 1))) { fbh_bv_free(res); return rc; }
     fbh_bitvector tmp;
-    tmp.d = lit; tmp.nw = ix->groups; tmp.nbits = ix->groups * MAXBITS;
-    tmp.anb = ix->anb; tmp.aval = ix->anb ? (act >> (31 - ix->anb)) : 0;
-    if (ix->groups) {
-        if ((rc = to_host(&tmp.w0, lit, 1)) || (rc = to_host(&tmp.wl, lit + ix->groups - 1, 1))) {
+    tmp.d = lit; tmp.nw = groups; tmp.nbits = groups * MAXBITS;
+    tmp.anb = anb; tmp.aval = anb ? (act >> (31 - anb)) : 0;
+    if (groups) {
+        if ((rc = to_host(&tmp.w0, lit, 1)) || (rc = to_host(&tmp.wl, lit + groups - 1, 1))) {
             fbh_bv_free(res); return rc;
         }
     }
     // res = empty | tmp  (the canonical kernel runs through the c0/d1 path)
-    rc = fbh_bv_set(res, 0, ix->nrows);
+    rc = fbh_bv_set(res, 0, groups * MAXBITS + anb);
     if (!rc) rc = bv_op(res, &tmp, FBH_OR, nullptr);
     tmp.d = nullptr;
     if (tmp.ck) { dev_free(tmp.ck); tmp.ck = nullptr; }
@@ -696,7 +696,11 @@ int ranges_bitvector(fbh_index* ix, uint32_t r0, uint32_t r1, uint32_t r2, uint3
     if ((rc = lit.alloc(ix->groups + 1)) || (rc = part.alloc(ix->ntiles))) return rc;
     q.out = lit.p; q.part = part.p;
     if ((rc = check(fbk::launch_query_lit(q, dev_type(ix->type), stream()), "estimate"))) return rc;
-    return literal_bitvector(ix, lit.p, out);
+    return literal_bitvector(ix->groups, ix->anb, lit.p, out);
+}
+
+int literal_bitvector(const fbh_index* ix, uint32_t* lit, fbh_bitvector** out) {
+    return literal_bitvector(ix->groups, ix->anb, lit, out);
 }
 
 // index::sumBins(ib, ie) as a new bitvector.
@@ -1677,14 +1681,31 @@ int conj_upload(fbh_conj* c, const std::vector<fbk::QueryArgs>& h, const uint32_
     return rc;
 }
 
-int conj_launch(fbh_conj* c, hipStream_t s) {
+// Kernel arguments of queries [q0, q0 + nq) of the batch (part / roff rows
+// start at query q0).
+fbk::ConjArgs conj_args(const fbh_conj* c, uint32_t q0, uint32_t nq) {
     fbk::ConjArgs a;
-    a.terms = c->dterms; a.qoff = c->dqoff; a.nq = c->nq; a.ntiles = c->ntiles;
-    a.groups = c->groups; a.anb = c->anb; a.tpi = c->tpi; a.part = c->parts; a.out = nullptr;
-    a.prog = c->dprog; a.poff = c->dpoff; a.depth = c->depth; a.skip = c->dskip;
-    int rc = c->single_type >= 0
-                 ? check(fbk::launch_range(a, c->single_type, c->has_scan, c->nt, c->blocks, s), "range batch")
-                 : check(fbk::launch_conj(a, c->nt, c->blocks, s), "conjunctions");
+    a.terms = c->dterms; a.qoff = c->dqoff + q0; a.nq = nq; a.ntiles = c->ntiles;
+    a.groups = c->groups; a.anb = c->anb; a.tpi = c->tpi; a.part = c->parts + (size_t)q0 * c->ntiles;
+    a.out = nullptr;
+    a.prog = c->dprog; a.poff = c->dpoff ? c->dpoff + q0 : nullptr; a.depth = c->depth; a.skip = c->dskip;
+    a.rids = nullptr; a.rbase = nullptr; a.roff = nullptr;
+    return a;
+}
+
+// The batch kernel over the queries of `a`.  The scan-only form of the range
+// kernel counts in registers; passes that need each tile's result in LDS
+// (row ids) take the mixed form.
+int conj_kernel(const fbh_conj* c, const fbk::ConjArgs& a, hipStream_t s) {
+    if (c->single_type >= 0) {
+        const int scan = (a.rids && c->has_scan == 2) ? 1 : c->has_scan;
+        return check(fbk::launch_range(a, c->single_type, scan, c->nt, c->blocks, s), "range batch");
+    }
+    return check(fbk::launch_conj(a, c->nt, c->blocks, s), "conjunctions");
+}
+
+int conj_launch(fbh_conj* c, hipStream_t s) {
+    int rc = conj_kernel(c, conj_args(c, 0, c->nq), s);
     if (!rc) rc = check(fbk::launch_sum_parts(c->parts, c->ntiles, c->dnp, c->nq, c->counts, s), "sum parts");
     return rc;
 }
@@ -1902,6 +1923,62 @@ int fbh_conj_tune(fbh_conj* c, int* nt, uint32_t* blocks, uint32_t* tpi, float* 
     if (tpi) *tpi = btpi;
     if (best_ms) *best_ms = best;
     return rc;
+}
+
+int fbh_conj_rids(fbh_conj* c, uint64_t* offsets, uint32_t* rids, uint64_t cap) {
+    FBH_API_LOCK();
+    if (!c || !offsets) return fail(FBH_ERR_ARG, "null argument");
+    // the counts, and each tile's ones in parts
+    int rc = conj_run(c, nullptr);
+    if (rc) return rc;
+    const uint32_t nq = c->nq;
+    std::vector<uint32_t> cnt(nq);
+    if ((rc = to_host(cnt.data(), c->counts, nq))) return rc;
+    offsets[0] = 0;
+    for (uint32_t q = 0; q < nq; ++q) offsets[q + 1] = offsets[q] + cnt[q];
+    if (!rids || cap == 0 || offsets[nq] == 0) return FBH_OK;
+    // each tile's offset within its query
+    DBuf<uint32_t> roff;
+    DBuf<uint64_t> rbase;
+    if ((rc = roff.alloc((size_t)nq * c->ntiles)) || (rc = rbase.alloc(nq))) return rc;
+    rc = check(fbk::launch_scan_parts(c->parts, c->ntiles, c->ntiles, nq, roff.p, stream()), "scan parts");
+    if (rc) return rc;
+    // queries in chunks of at most 2^28 ids on the device (1 GiB), at
+    // least one query each; ids past `cap` are not written
+    constexpr uint64_t CHUNK_IDS = 1ull << 28;
+    std::vector<uint64_t> base(nq);
+    uint32_t q0 = 0;
+    while (q0 < nq && offsets[q0] < cap) {
+        uint32_t q1 = q0 + 1;
+        while (q1 < nq && offsets[q1 + 1] - offsets[q0] <= CHUNK_IDS && offsets[q1] < cap) ++q1;
+        const uint64_t n = offsets[q1] - offsets[q0];
+        if (n) {
+            for (uint32_t q = q0; q < q1; ++q) base[q] = offsets[q] - offsets[q0];
+            DBuf<uint32_t> out;
+            if ((rc = out.alloc(n))) return rc;
+            if ((rc = to_dev(rbase.p + q0, base.data() + q0, q1 - q0))) return rc;
+            fbk::ConjArgs a = conj_args(c, q0, q1 - q0);
+            a.rids = out.p; a.rbase = rbase.p + q0; a.roff = roff.p + (size_t)q0 * c->ntiles;
+            if ((rc = conj_kernel(c, a, stream()))) return rc;
+            const uint64_t k = std::min(n, cap - offsets[q0]);
+            if ((rc = to_host(rids + offsets[q0], out.p, k))) return rc;
+        }
+        q0 = q1;
+    }
+    return FBH_OK;
+}
+
+int fbh_conj_result(fbh_conj* c, uint32_t q, fbh_bitvector** out) {
+    FBH_API_LOCK();
+    if (!c || !out) return fail(FBH_ERR_ARG, "null argument");
+    if (q >= c->nq) return fail(FBH_ERR_ARG, "query index out of range");
+    DBuf<uint32_t> lit, part;
+    int rc;
+    if ((rc = lit.alloc(c->groups + 1)) || (rc = part.alloc(c->ntiles))) return rc;
+    fbk::ConjArgs a = conj_args(c, q, 1);
+    a.part = part.p; a.out = lit.p;
+    if ((rc = conj_kernel(c, a, stream()))) return rc;
+    return literal_bitvector(c->groups, c->anb, lit.p, out);
 }
 
 int fbh_conj_plans(const fbh_conj* c, int* plans) {

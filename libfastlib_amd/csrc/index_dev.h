@@ -61,6 +61,12 @@ struct ConjArgs {
     // when the left operand's tile is empty the program resumes at skip[p]
     // (the empty left result is the result of the operator)
     const int32_t* skip;
+    // row-id pass (countQuery::getHitRows): when rids is set, tile t of
+    // query q writes its row ids, ascending, from
+    // rids[rbase[q] + roff[q*ntiles + t]] on, and part is not written
+    uint32_t* rids;
+    const uint64_t* rbase;
+    const uint32_t* roff;
 };
 
 constexpr int EXPR_DEPTH = 4;
@@ -119,6 +125,10 @@ hipError_t launch_sum_parts(const uint32_t* parts, uint32_t stride, const uint32
                             uint32_t* counts, hipStream_t s);
 // *count = sum of parts[0 .. n) (one query, n given on the host)
 hipError_t launch_sum_parts_n(const uint32_t* parts, uint32_t n, uint32_t* count, hipStream_t s);
+// roff[q*stride + t] = sum of parts[q*stride .. q*stride + t), t < n, per
+// query q < nq (the row offsets of each query's tiles)
+hipError_t launch_scan_parts(const uint32_t* parts, uint32_t stride, uint32_t n, uint32_t nq, uint32_t* roff,
+                             hipStream_t s);
 hipError_t launch_scan_excl(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp,
                             hipStream_t s);
 uint32_t scan_tmp_words(uint32_t n);

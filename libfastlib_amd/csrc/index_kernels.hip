@@ -1039,6 +1039,56 @@ __device__ __forceinline__ void term_tile(const QueryArgs& q, uint32_t tile, uin
     }
 }
 
+// End of a (query, tile) work item of the batch kernels: the tile's result
+// R[0, ng) goes to a.out (single-query launches), its ones to
+// part[q*ntiles + tile], or, in the row-id pass, its row ids to a.rids.  A
+// set bit b of group g is row 31g + 30 - b.  In the row-id pass thread i
+// takes the PER consecutive groups from i*PER, so that a block-wide prefix
+// sum of the threads' ones places every id.  red needs NT/64 + 1 words.
+// Ends with a barrier.
+template <int NT>
+__device__ __forceinline__ void tile_epilogue(const ConjArgs& a, uint32_t qi, uint32_t tile, uint32_t g0,
+                                              uint32_t ng, const word_t* R, uint32_t* red) {
+    const int w = threadIdx.x / WAVE;
+    if (a.rids) {
+        constexpr int PER = (QT + NT - 1) / NT;
+        const uint32_t k0 = threadIdx.x * (uint32_t)PER;
+        uint32_t c = 0;
+        #pragma unroll
+        for (int i = 0; i < PER; ++i) c += k0 + i < ng ? (uint32_t)__popc(R[k0 + i]) : 0u;
+        uint32_t tot;
+        const uint32_t off = block_excl_sum<NT>(c, red, &tot);
+        uint32_t* dst = a.rids + a.rbase[qi] + a.roff[(size_t)qi * a.ntiles + tile] + off;
+        for (int i = 0; i < PER; ++i) {
+            if (k0 + i >= ng) break;
+            word_t v = R[k0 + i];
+            const uint32_t r30 = (g0 + k0 + (uint32_t)i) * 31u + 30u;
+            while (v) {
+                const uint32_t b = 31u - (uint32_t)__clz(v);
+                *dst++ = r30 - b;
+                v &= ~(1u << b);
+            }
+        }
+        __syncthreads();
+        return;
+    }
+    uint32_t ones = 0;
+    for (uint32_t i = threadIdx.x; i < ng; i += NT) {
+        const word_t v = R[i];
+        if (a.out) a.out[g0 + i] = v;
+        ones += __popc(v);
+    }
+    ones = wave_sum(ones);
+    if (lane_id() == 0) red[w] = ones;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int i = 0; i < NT / WAVE; ++i) t += red[i];
+        a.part[(size_t)qi * a.ntiles + tile] = t;
+    }
+    __syncthreads();
+}
+
 template <int NT>
 __global__ __launch_bounds__(NT) void k_conj(ConjArgs a) {
     __shared__ word_t R[QT + 1], R2[QT + 1];
@@ -1048,7 +1098,6 @@ __global__ __launch_bounds__(NT) void k_conj(ConjArgs a) {
     __shared__ uint32_t ctab[CT_WORDS];
     const uint32_t nchunk = (a.ntiles + a.tpi - 1) / a.tpi;
     const uint32_t nitems = a.nq * nchunk;
-    const int w = threadIdx.x / WAVE;
     for (uint32_t item = blockIdx.x; item < nitems; item += gridDim.x) {
         const uint32_t qi = item / nchunk;
         const uint32_t c = item - qi * nchunk;
@@ -1076,21 +1125,7 @@ __global__ __launch_bounds__(NT) void k_conj(ConjArgs a) {
                     break;
                 }
             }
-            uint32_t ones = 0;
-            for (uint32_t i = threadIdx.x; i < ng; i += NT) {
-                const word_t v = R[i];
-                if (a.out) a.out[g0 + i] = v;
-                ones += __popc(v);
-            }
-            ones = wave_sum(ones);
-            if (lane_id() == 0) red[w] = ones;
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                uint32_t t = 0;
-                for (int i = 0; i < NT / WAVE; ++i) t += red[i];
-                a.part[(size_t)qi * a.ntiles + tile] = t;
-            }
-            __syncthreads();
+            tile_epilogue<NT>(a, qi, tile, g0, ng, R, red);
         }
     }
 }
@@ -1165,21 +1200,7 @@ __global__ __launch_bounds__(NT) void k_range(ConjArgs a) {
                 __syncthreads();
                 lit_tile<T, NT, QB_U>(q, tile, g0, ng, last, R, cov, red, ctab);
             }
-            uint32_t ones = 0;
-            for (uint32_t i = threadIdx.x; i < ng; i += NT) {
-                const word_t v = R[i];
-                if (a.out) a.out[g0 + i] = v;
-                ones += __popc(v);
-            }
-            ones = wave_sum(ones);
-            if (lane_id() == 0) red[w] = ones;
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                uint32_t t = 0;
-                for (int i = 0; i < NT / WAVE; ++i) t += red[i];
-                a.part[(size_t)qi * a.ntiles + tile] = t;
-            }
-            __syncthreads();
+            tile_epilogue<NT>(a, qi, tile, g0, ng, R, red);
         }
     }
 }
@@ -1199,7 +1220,6 @@ __global__ __launch_bounds__(NT) void k_expr(ConjArgs a) {
     __shared__ uint32_t ctab[CT_WORDS];
     const uint32_t nchunk = (a.ntiles + a.tpi - 1) / a.tpi;
     const uint32_t nitems = a.nq * nchunk;
-    const int w = threadIdx.x / WAVE;
     const word_t amask = a.anb ? (((1u << a.anb) - 1u) << (31u - a.anb)) : 0u;
     for (uint32_t item = blockIdx.x; item < nitems; item += gridDim.x) {
         const uint32_t qi = item / nchunk;
@@ -1252,21 +1272,7 @@ __global__ __launch_bounds__(NT) void k_expr(ConjArgs a) {
                 }
                 __syncthreads();
             }
-            uint32_t ones = 0;
-            for (uint32_t i = threadIdx.x; i < ng; i += NT) {
-                const word_t v = S[0][i];
-                if (a.out) a.out[g0 + i] = v;
-                ones += __popc(v);
-            }
-            ones = wave_sum(ones);
-            if (lane_id() == 0) red[w] = ones;
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                uint32_t t = 0;
-                for (int i = 0; i < NT / WAVE; ++i) t += red[i];
-                a.part[(size_t)qi * a.ntiles + tile] = t;
-            }
-            __syncthreads();
+            tile_epilogue<NT>(a, qi, tile, g0, ng, S[0], red);
         }
     }
 }
@@ -1329,6 +1335,23 @@ __global__ __launch_bounds__(IX_TPB) void k_sum_parts(const uint32_t* __restrict
         uint32_t x = 0;
         for (int i = 0; i < IX_TPB / WAVE; ++i) x += red[i];
         counts[qi] = x;
+    }
+}
+
+// roff[q*stride + t] = parts[q*stride] + ... + parts[q*stride + t - 1] for
+// t < n, one block per query, IX_TPB entries per round.
+__global__ __launch_bounds__(IX_TPB) void k_scan_parts(const uint32_t* __restrict__ parts, uint32_t stride,
+                                                       uint32_t n, uint32_t* __restrict__ roff) {
+    __shared__ uint32_t sh[IX_TPB / WAVE + 1];
+    const size_t base = (size_t)blockIdx.x * stride;
+    uint32_t carry = 0;
+    for (uint32_t k0 = 0; k0 < n; k0 += IX_TPB) {
+        const uint32_t k = k0 + threadIdx.x;
+        const uint32_t v = k < n ? parts[base + k] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_excl_sum<IX_TPB>(v, sh, &tot);
+        if (k < n) roff[base + k] = carry + ex;
+        carry += tot;
     }
 }
 
@@ -1581,6 +1604,13 @@ hipError_t launch_sum_parts(const uint32_t* parts, uint32_t stride, const uint32
 hipError_t launch_sum_parts_n(const uint32_t* parts, uint32_t n, uint32_t* count, hipStream_t s) {
     hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(IX_TPB), 0, s, parts, n, (const uint32_t*)nullptr, n,
                        count);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan_parts(const uint32_t* parts, uint32_t stride, uint32_t n, uint32_t nq, uint32_t* roff,
+                             hipStream_t s) {
+    if (!nq || !n) return hipSuccess;
+    hipLaunchKernelGGL(k_scan_parts, dim3(nq), dim3(IX_TPB), 0, s, parts, stride, n, roff);
     return hipGetLastError();
 }
 

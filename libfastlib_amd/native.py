@@ -595,6 +595,8 @@ def _conj_sigs(L):
         "fbh_conj_run": [vp, _u64p, C.POINTER(C.c_float)],
         "fbh_conj_tune": [vp, C.POINTER(C.c_int), _u32p, _u32p, C.POINTER(C.c_float)],
         "fbh_conj_plans": [vp, C.POINTER(C.c_int)],
+        "fbh_conj_rids": [vp, _u64p, _u32p, C.c_uint64],
+        "fbh_conj_result": [vp, C.c_uint32, C.POINTER(vp)],
         "fbh_conj_evaluate": [C.POINTER(_Term), C.c_uint32, C.c_int, _u64p],
         "fbh_expr_prepare": [C.POINTER(_Term), _u32p, C.POINTER(C.c_int32), _u32p, C.c_uint32, C.c_int,
                              C.POINTER(vp)],
@@ -650,6 +652,22 @@ class Conjunctions:
         out = np.zeros(self.nterms, np.int32)
         check(self._L.fbh_conj_plans(self._h, out.ctypes.data_as(C.POINTER(C.c_int))), "fbh_conj_plans")
         return out
+
+    def rids(self):
+        """(offsets uint64[nq+1], rids uint32[offsets[-1]]): the row ids of
+        every query, query q's at rids[offsets[q]:offsets[q+1]]."""
+        off = np.zeros(self.nq + 1, np.uint64)
+        check(self._L.fbh_conj_rids(self._h, off.ctypes.data_as(_u64p), None, 0), "fbh_conj_rids")
+        out = np.zeros(max(int(off[-1]), 1), np.uint32)
+        check(self._L.fbh_conj_rids(self._h, off.ctypes.data_as(_u64p), out.ctypes.data_as(_u32p),
+                                    int(off[-1])), "fbh_conj_rids")
+        return off, out[:int(off[-1])]
+
+    def result(self, q):
+        """The hit vector of query q as a Bitvector (canonical WAH)."""
+        h = C.c_void_p()
+        check(self._L.fbh_conj_result(self._h, int(q), C.byref(h)), "fbh_conj_result")
+        return Bitvector(h)
 
     def __del__(self):
         try:

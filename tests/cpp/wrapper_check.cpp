@@ -119,6 +119,15 @@ void check_index() {
                                {ix.handle(), 0, FBH_OP_NONE, 25, FBH_OP_LE}};
     fbh::batch e = fbh::batch::expressions(t, {0, 3}, {0, 1, FBH_EXPR_OR, 2, FBH_EXPR_NOT, FBH_EXPR_AND}, {0, 6});
     EXPECT(e.run()[0] == count(26, 30));
+    std::vector<uint64_t> off;
+    std::vector<uint32_t> rids;
+    e.hit_rows(off, rids);
+    EXPECT(off.size() == 2 && off[1] == count(26, 30) && rids.size() == off[1]);
+    bool rows_ok = true;
+    for (size_t i = 0; i < rids.size(); ++i)
+        rows_ok = rows_ok && col[rids[i]] >= 26 && col[rids[i]] < 30 && (i == 0 || rids[i] > rids[i - 1]);
+    EXPECT(rows_ok);
+    EXPECT(e.hit_vector(0).cnt() == count(26, 30));
     fbh::batch cj = fbh::batch::conjunctions(t, {0, 2});
     EXPECT(cj.run()[0] == 0);
     bool threw = false;

@@ -100,3 +100,33 @@ def test_expression_program_errors(L):
     for prog in ([0, 1], [-1], [0, -6, -1], [0, 2, -1], [0, 1, -9], [0, 0, 0, 0, 0, -1, -1, -1, -1]):
         with pytest.raises(L.FastbitError):
             L.Expressions([(t, prog)])
+
+
+def test_expression_short_circuit_and_rids(L):
+    """Programs whose left operand of AND / MINUS is empty on some tiles
+    only (its term is a range of a sorted column, so its rows are one
+    contiguous run): counts and row ids against brute force, as a batch
+    that needs a stack of two results and as one that needs four."""
+    n = 31 * 2048 * 4 + 99
+    rng = np.random.default_rng(17)
+    cols = {"ef": np.sort(rng.standard_normal(n)).astype(np.float32),
+            "ei": rng.integers(-300, 300, n).astype(np.int32),
+            "eu": rng.integers(0, 2000, n).astype(np.uint16)}
+    ixs = _indexes(L, cols)
+    progs2 = ([0, 1, -1], [0, 1, -7], [0, 1, -1, -6])
+    progs4 = progs2 + ([0, 1, 2, -1, -1],)
+    cases = []
+    for lo in (-2.5, -0.3, 0.9):
+        terms = [("ef", lo, 2, lo + 0.4, 1), ("ei", -100.0, 2, 150.0, 1), ("eu", 10.0, 1, 1500.0, 2)]
+        cases.append(terms)
+    for progs in (progs2, progs4):
+        qs = [(terms, p) for terms in cases for p in progs]
+        want = [O.expr_rows([term_mask(cols[t[0]], *t[1:]) for t in terms], p) for terms, p in qs]
+        E = L.Expressions([([(ixs[t[0]],) + t[1:] for t in terms], p) for terms, p in qs])
+        hits, _ = E.run()
+        assert [int(h) for h in hits] == [int(w.sum()) for w in want], len(progs)
+        off, rids = E.rids()
+        for i, w in enumerate(want):
+            assert np.array_equal(rids[off[i]:off[i + 1]], np.nonzero(w)[0].astype(np.uint32)), (len(progs), i)
+        bits = want[1].astype(np.uint8)
+        assert int(E.result(1).cnt()) == int(bits.sum())

@@ -186,3 +186,30 @@ def test_conj_fullsize_and_programs(L, table4):
     batch = L.Expressions(progs)
     hits, _ = batch.run()
     assert np.array_equal(hits, want)
+
+
+def test_conj_fullsize_rids(L, table4):
+    """Row ids of all 4096 conjunctions from one more persistent launch:
+    per-query lengths equal the committed counts, ids ascend within each
+    query, and every 256th query's ids equal a scan of the columns."""
+    from libfastlib_amd import datagen as G
+    ixs, qs, g = table4
+    want = g["counts"].astype(np.uint64)
+    batch = L.Conjunctions([[(ixs[ci], a, 2, b, 1) for (ci, a, b) in q] for q in qs])
+    _say("conj rids")
+    off, rids = batch.rids()
+    assert np.array_equal(np.diff(off), want)
+    d = np.diff(rids.astype(np.int64))
+    starts = off[1:-1][off[1:-1] < len(rids)].astype(np.int64)
+    ok = d > 0
+    ok[starts[starts > 0] - 1] = True          # a new query may restart lower
+    assert ok.all()
+    cols = G.table4(G.CONJ_TABLE_SEED, 1 << 26)
+    for i in range(0, len(qs), 256):
+        m = None
+        for ci, a, b in qs[i]:
+            c = cols[ci]
+            v = c.astype(np.int64) if c.dtype.kind in "iu" else c.astype(np.float64)
+            t = (v >= (int(a) if c.dtype.kind in "iu" else a)) & (v < (int(b) if c.dtype.kind in "iu" else b))
+            m = t if m is None else (m & t)
+        assert np.array_equal(rids[off[i]:off[i + 1]], np.nonzero(m)[0].astype(np.uint32)), i

@@ -316,6 +316,57 @@ def test_conjunctions(L, plan):
     assert np.array_equal(hits, hits2)
 
 
+@pytest.mark.parametrize("plan", [0, 1, 3])
+def test_conjunction_rids_and_hit_vectors(L, orc, plan):
+    """Row ids of every query of a conjunction batch (one more persistent
+    launch after the counts) equal the intersection of the oracle's
+    per-term rows; the hit vector of a query is the canonical WAH of those
+    rows.  Four tiles of rows, with empty and non-empty tiles per query."""
+    from libfastlib_amd import datagen as G
+    n = 31 * 2048 * 3 + 31 * 700 + 13
+    cols = G.table4(31, n)
+    gis = _table_indexes(L, cols)
+    ois = _oracle_indexes(gis, cols)
+    qs = G.conj_queries(32, cols, 40, smax=0.6)
+    batch = L.Conjunctions([[(gis[ci], lo, 2, hi, 1) for (ci, lo, hi) in q] for q in qs], plan)
+    hits, _ = batch.run()
+    off, rids = batch.rids()
+    assert np.array_equal(np.diff(off), hits)
+    plans = batch.plans()
+    k = 0
+    for qi, q in enumerate(qs):
+        r = None
+        for (ci, lo, hi) in q:
+            p = int(plans[k]); k += 1
+            rr = ois[ci].evaluate_plan(lo, 2, hi, 1, plan=p)[1]
+            r = rr if r is None else np.intersect1d(r, rr)
+        got = rids[off[qi]:off[qi + 1]]
+        assert np.array_equal(got, r.astype(np.uint32)), (qi, len(got), len(r))
+        if qi % 5 == 0:
+            bits = np.zeros(n, np.uint8)
+            bits[r] = 1
+            assert np.array_equal(batch.result(qi).write(), orc.from_bits(bits)), qi
+
+
+def test_range_batch_rids(L):
+    """Row ids of a range batch (the persistent range kernel) under the
+    literal, mixed and scan-only plans equal the single-query row ids."""
+    rng = np.random.default_rng(8)
+    n = 31 * 2048 * 2 + 31 * 5 + 7
+    col = rng.normal(size=n).astype(np.float32)
+    ix = L.Index.bin_equal_weight(col, 300)
+    lo = rng.normal(size=24)
+    hi = lo + np.abs(rng.normal(size=24))
+    mixed = np.where(np.arange(24) % 3 == 0, 3, 1).astype(np.int32)
+    for kw in (dict(plan=1), dict(plans=mixed), dict(plan=3)):
+        rb = L.RangeBatch(ix, lo, 2, hi, 1, **kw)
+        off, rids = rb.rids()
+        for i in range(24):
+            want = np.nonzero((col >= lo[i]) & (col < hi[i]))[0].astype(np.uint32)
+            assert np.array_equal(rids[off[i]:off[i + 1]], want), (kw, i)
+            assert np.array_equal(ix.rids(float(lo[i]), 2, float(hi[i]), 1, 1), want)
+
+
 def test_query_count_kat(L):
     """The reference's tests/scripts/query-count.list conjunctions of range
     terms (tests/golden/kat_query_count.json) through the batched
