@@ -171,7 +171,7 @@ typedef struct fbh_index fbh_index;
  * FBH_PLAN_SCAN, exactly as it does in the reference. */
 #define FBH_PLAN_AUTO 0        /* the ibis heuristic (column::evaluateRange) */
 #define FBH_PLAN_LITERAL 1     /* LDS multiway OR into a literal result      */
-#define FBH_PLAN_COMPRESSED 2  /* pairwise ORs of the compressed bitmaps     */
+#define FBH_PLAN_COMPRESSED 2  /* OR merged per tile, compressed result     */
 #define FBH_PLAN_SCAN 3        /* raw column compare (part::doScan)          */
 
 /* relic over the distinct values of a column: index::mapValues +
@@ -410,13 +410,12 @@ int fbh_expr_prepare(const fbh_term* terms, const uint32_t* toff, const int32_t*
 int fbh_index_plan_features(const fbh_index* ix, uint32_t nq, const double* lo, const int* lop,
                             const double* hi, const int* rop, float* feats);
 /* Times every query under the three plans, back to back on the device:
- * ms[3*i + plan-1] is the device time of query i under the literal and
- * scan plans and the host wall time of the compressed plan (which
- * synchronises between its operations).  The compressed plan stops once
- * it runs cap times longer than the slower of the other two, and does not
- * start when its ORs times the fastest device round trip already exceed
- * that; its time is then stored negated (a lower bound) and its hit count
- * as UINT64_MAX.  hits[3*i + plan-1] receives the counts. */
+ * ms[3*i + plan-1] is the device time (HIP events around the query's
+ * launches) of query i under the literal, compressed and scan plans, and
+ * hits[3*i + plan-1] its count.  The literal and scan plans write the
+ * literal result, the compressed plan the compressed one.  A negative time
+ * would mark a lower bound; the device plans never produce one.  cap is
+ * kept for the interface and not used. */
 int fbh_index_time_plans(fbh_index* ix, uint32_t nq, const double* lo, const int* lop, const double* hi,
                          const int* rop, float cap, float* ms, uint64_t* hits);
 

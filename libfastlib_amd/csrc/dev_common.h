@@ -254,4 +254,50 @@ __device__ __forceinline__ void lookback_ch_wave(uint64_t* status, uint32_t tile
                            __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Run summary of one wave's lanes in lane order, valid in every lane.
+__device__ __forceinline__ wah::Summary wave_total_summary(const wah::Summary& x) {
+    return shfl_summary_idx(wave_incl_summary(x), 63);
+}
+
+// Spins until the summary entry p carries `tag`, then returns it.
+__device__ __forceinline__ wah::Summary wait_summary(const uint64_t* p, uint32_t tag) {
+    uint64_t v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (wah::status_of(v) != tag) {
+        __builtin_amdgcn_s_sleep(1);
+        v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return wah::unpack(v);
+}
+
+// Two-level exclusive prefix of run summaries for kernels whose tile t is
+// block t (blocks start in index order).  Tile t publishes its aggregate in
+// aggs[t]; the last tile of every segment of 64 tiles publishes the
+// segment's aggregate in segs[t / 64] as soon as the tiles before it in the
+// segment have published theirs, before it waits for anything else.  A
+// tile then combines the aggregates of the tiles before it in its segment
+// and of the segments before its own: two rounds of independent loads,
+// where a walk back window by window takes a round trip per 64 tiles while
+// every tile is still waiting.  Entries carry `tag` (1 or 2): callers that
+// alternate the tag between launches on the same buffers, all of one tile
+// count, start from zeroed buffers and never clear them again.  Called by
+// all 64 lanes of one wave; the result is valid in every lane.
+__device__ __forceinline__ wah::Summary prefix_summary_seg(uint64_t* aggs, uint64_t* segs, uint32_t tile,
+                                                           uint32_t tag, const wah::Summary& agg) {
+    const uint32_t lane = lane_id();
+    const uint32_t s = tile >> 6, r = tile & 63u;
+    if (lane == 0) __hip_atomic_store(&aggs[tile], wah::pack(agg, tag), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const wah::Summary mine = lane < r ? wait_summary(&aggs[(s << 6) + lane], tag) : wah::summary_empty();
+    const wah::Summary in_seg = wave_total_summary(mine);
+    if (r == 63u && lane == 0)
+        __hip_atomic_store(&segs[s], wah::pack(wah::combine(in_seg, agg), tag), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    wah::Summary before = wah::summary_empty();
+    for (uint32_t k0 = 0; k0 < s; k0 += 64u) {
+        const uint32_t j = k0 + lane;
+        const wah::Summary x = j < s ? wait_summary(&segs[j], tag) : wah::summary_empty();
+        before = wah::combine(before, wave_total_summary(x));
+    }
+    return wah::combine(before, in_seg);
+}
+
 } // namespace fbk

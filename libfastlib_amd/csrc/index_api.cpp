@@ -549,108 +549,72 @@ using namespace fbh_impl;
 
 namespace {
 
-// View of bitmap b of the index as a (borrowed) device bitvector.
-void bitmap_view(const fbh_index* ix, uint32_t b, fbh_bitvector& v) {
-    v.d = ix->words + ix->off[b];
-    v.cap = 0;
-    v.nw = ix->nw[b];
-    v.nbits = ix->groups * MAXBITS;
-    v.nset = 0;
-    v.anb = ix->anb;
-    v.aval = 0;
-}
-
-int view_aval(const fbh_index* ix, uint32_t b, uint32_t* aval, uint32_t* w0, uint32_t* wl) {
-    uint32_t h[2] = {0, 0};
-    int rc;
-    if (ix->nw[b]) {
-        if ((rc = to_host(&h[0], ix->words + ix->off[b], 1))) return rc;
-        if ((rc = to_host(&h[1], ix->words + ix->off[b] + ix->nw[b] - 1, 1))) return rc;
+// Device output of the compressed plan (launch_query_wah): result words and
+// chunk index sized for the worst case, tile look-back status, control words.
+struct WahBuf {
+    DBuf<uint32_t> out, ck, ctl;
+    DBuf<uint64_t> status;
+    uint32_t tag = 2;          // the tag of the last launch on these buffers
+    int alloc(const fbh_index* ix) {
+        const uint32_t ns = fbk::wah_status_words(ix->ntiles);
+        int rc = out.alloc(ix->groups + 2);
+        if (!rc) rc = ck.alloc(ix->groups / fbk::CHUNK_WORDS + 3);
+        if (!rc) rc = ctl.alloc(8);
+        if (!rc) rc = status.alloc(ns);
+        if (!rc) rc = check(fbk::launch_zero_u64(status.p, ns, stream()), "zero");
+        if (!rc) rc = check(fbk::launch_zero_u32(ctl.p, 8, stream()), "zero");
+        return rc;
     }
-    *w0 = h[0]; *wl = h[1];
-    *aval = 0;
-    if (ix->anb && !ix->null_bitmap(b)) return to_host(aval, ix->words + ix->off[b] + ix->nw[b], 1);
-    return FBH_OK;
-}
+    // arguments of the next launch (tags alternate)
+    fbk::WahOut args() {
+        tag = 3u - tag;
+        return fbk::WahOut{out.p, ck.p, status.p, ctl.p, tag};
+    }
+};
 
-using Clock = std::chrono::steady_clock;
-constexpr int CENSORED = 1;   // eval_compressed stopped at its deadline
-
-// Number of ORs eval_compressed performs for the range (bitmaps of the OR
-// set that are not null, plus one for the edge rows of a bin).
-uint32_t compressed_ops(const fbh_index* ix, double lo, int lop, double hi, int rop) {
+// Compressed plan (sumBins leaving its result compressed, src/index.cpp:
+// 6687-6760): the OR set and the edge rows merged tile by tile on the
+// device, the canonical WAH of the result written directly; part[] receives
+// ntiles per-tile ones.
+int launch_compressed(fbh_index* ix, double lo, int lop, double hi, int rop, WahBuf& wb, uint32_t* part,
+                      hipStream_t s) {
+    fbk::QueryArgs q;
+    fill_query(ix, q, lo, lop, hi, rop);
+    q.out = nullptr;
+    q.part = part;
     Locate L;
     locate(ix, lo, lop, hi, rop, L);
-    const SumPlan sp = sum_plan(ix, L.hit0, L.hit1);
-    auto nonnull = [&](uint32_t b0, uint32_t b1) {
-        uint32_t k = 0;
-        for (uint32_t b = b0; b < std::min(b1, ix->nobs); ++b) k += ix->null_bitmap(b) ? 0u : 1u;
-        return k;
-    };
-    uint32_t k = sp.kind == 2 ? nonnull(L.hit0, L.hit1) : sp.kind == 3 ? nonnull(0, L.hit0) + nonnull(L.hit1, ix->nobs) : 0u;
-    if (ix->kind == FBH_BIN && (L.cand0 < L.hit0 || (L.cand1 > L.hit1 && L.hit1 < ix->nobs))) ++k;
-    return k;
-}
-
-// Compressed plan: the OR chain through the WAH operator kernels.  With a
-// deadline, returns CENSORED once an operation ends after it.
-int eval_compressed(fbh_index* ix, double lo, int lop, double hi, int rop, fbh_bitvector* res,
-                    const Clock::time_point* deadline = nullptr) {
-    Locate L;
-    locate(ix, lo, lop, hi, rop, L);
-    const SumPlan sp = sum_plan(ix, L.hit0, L.hit1);
-    int rc = fbh_bv_set(res, 0, ix->nrows);
-    if (rc) return rc;
-    auto or_range = [&](uint32_t b0, uint32_t b1) -> int {
-        for (uint32_t b = b0; b < b1; ++b) {
-            if (ix->null_bitmap(b)) continue;
-            fbh_bitvector v;
-            bitmap_view(ix, b, v);
-            int r = view_aval(ix, b, &v.aval, &v.w0, &v.wl);
-            if (r) return r;
-            r = bv_op(res, &v, FBH_OR, nullptr);
-            v.d = nullptr;        // borrowed
-            if (v.ck) { dev_free(v.ck); v.ck = nullptr; }
-            if (r) return r;
-            if (deadline && Clock::now() > *deadline) return CENSORED;
-        }
-        return FBH_OK;
-    };
-    if (sp.kind == 1) {
-        rc = fbh_bv_set(res, 1, ix->nrows);
-    } else if (sp.kind == 2) {
-        rc = or_range(L.hit0, L.hit1);
-    } else if (sp.kind == 3) {
-        rc = or_range(0, L.hit0);
-        if (!rc) rc = or_range(L.hit1, ix->nobs);
-        if (!rc) rc = fbh_bv_flip(res);
-    }
-    if (rc) return rc;
-    if (ix->kind == FBH_BIN && (L.cand0 < L.hit0 || (L.cand1 > L.hit1 && L.hit1 < ix->nobs))) {
-        // edge rows through the literal kernel restricted to the candidates
-        fbk::QueryArgs q;
-        fill_query(ix, q, lo, lop, hi, rop);
+    set_sum_ranges(ix, q, L.hit0, L.hit1);
+    if (ix->kind == FBH_BIN) {
         if (L.cand0 < L.hit0 && !ix->null_bitmap(L.cand0)) q.cand0 = L.cand0;
         if (L.cand1 > L.hit1 && L.hit1 < ix->nobs && !ix->null_bitmap(L.hit1)) q.cand1 = L.hit1;
-        DBuf<uint32_t> lit, part;
-        if ((rc = lit.alloc(ix->groups + 1)) || (rc = part.alloc(ix->ntiles))) return rc;
-        q.out = lit.p; q.part = part.p;
-        if ((rc = check(fbk::launch_query_lit(q, dev_type(ix->type), stream()), "edge"))) return rc;
-        fbh_bitvector delta;
-        uint32_t act = 0;
-        if (ix->anb) { if ((rc = to_host(&act, lit.p + ix->groups, 1))) return rc; }
-        // wrap the literal array as a literal-only bitvector (no copy)
-        delta.d = lit.p; delta.nw = ix->groups; delta.nbits = ix->groups * MAXBITS;
-        delta.anb = ix->anb; delta.aval = ix->anb ? (act >> (31 - ix->anb)) : 0;
-        if (ix->groups) {
-            if ((rc = to_host(&delta.w0, lit.p, 1)) || (rc = to_host(&delta.wl, lit.p + ix->groups - 1, 1)))
-                return rc;
-        }
-        rc = bv_op(res, &delta, FBH_OR, nullptr);
-        delta.d = nullptr;
-        if (delta.ck) { dev_free(delta.ck); delta.ck = nullptr; }
-        if (rc) return rc;
     }
+    return check(fbk::launch_query_wah(q, dev_type(ix->type), wb.args(), s), "compressed");
+}
+
+// The compressed plan's result as the words of `res` (its previous words are
+// released).
+int eval_compressed(fbh_index* ix, double lo, int lop, double hi, int rop, fbh_bitvector* res) {
+    WahBuf wb;
+    DBuf<uint32_t> part;
+    int rc = wb.alloc(ix);
+    if (!rc) rc = part.alloc(ix->ntiles);
+    if (!rc) rc = launch_compressed(ix, lo, lop, hi, rop, wb, part.p, stream());
+    if (!rc) rc = check(fbk::launch_wah_edges(wb.out.p, wb.ctl.p, stream()), "edge words");
+    uint32_t h[8];
+    if (!rc) rc = to_host(h, wb.ctl.p, 8);
+    if (rc) return rc;
+    dev_free(res->d); dev_free(res->ck);
+    res->cap = ix->groups + 2;
+    res->d = wb.out.release();
+    res->ckcap = ix->groups / fbk::CHUNK_WORDS + 3;
+    res->ck = wb.ck.release();
+    res->ck_ok = true;
+    res->nw = h[1]; res->w0 = h[4]; res->wl = h[5];
+    res->nbits = ix->groups * MAXBITS;
+    res->nset = 0;
+    res->anb = ix->anb;
+    res->aval = ix->anb ? (h[2] >> (31 - ix->anb)) : 0;
     return FBH_OK;
 }
 
@@ -1356,7 +1320,7 @@ int fbh_index_evaluate_batch(fbh_index* ix, uint32_t nq, const double* lo, const
     if ((rc = pc.alloc(ix, nq))) return rc;
     std::vector<uint32_t> np(nq, 0);
     std::vector<int> used(nq);
-    std::vector<uint64_t> sync_hits(nq, 0);
+    WahBuf wbs[NSIDE];
     hipEvent_t e0, e1, ej[NSIDE];
     (void)hipEventCreate(&e0); (void)hipEventCreate(&e1);
     for (int k = 0; k < NSIDE; ++k) (void)hipEventCreateWithFlags(&ej[k], hipEventDisableTiming);
@@ -1367,16 +1331,13 @@ int fbh_index_evaluate_batch(fbh_index* ix, uint32_t nq, const double* lo, const
         int p = plans ? plans[i] : plan;
         if (p == FBH_PLAN_AUTO) p = plan_heuristic(ix, lo[i], lop[i], hi[i], rop[i]);
         used[i] = p;
+        const int k = (int)(nlaunch++ % (uint32_t)NSIDE);
         if (p == FBH_PLAN_COMPRESSED) {
-            fbh_bitvector* res = nullptr;
-            rc = fbh_bv_new(&res);
-            if (!rc) rc = eval_compressed(ix, lo[i], lop[i], hi[i], rop[i], res);
-            uint32_t c = 0;
-            if (!rc) rc = fbh_bv_cnt(res, &c);
-            sync_hits[i] = c;
-            fbh_bv_free(res);
+            // the compressed result of each side stream in its own buffers
+            if (!wbs[k].out.p) rc = wbs[k].alloc(ix);
+            np[i] = ix->ntiles;
+            if (!rc) rc = launch_compressed(ix, lo[i], lop[i], hi[i], rop[i], wbs[k], pc.row(i), side_stream(k));
         } else {
-            const int k = (int)(nlaunch++ % (uint32_t)NSIDE);
             rc = index_eval_literal(ix, lo[i], lop[i], hi[i], rop[i], p, lit[k].p, pc.row(i), &np[i],
                                     side_stream(k));
         }
@@ -1403,7 +1364,7 @@ int fbh_index_evaluate_batch(fbh_index* ix, uint32_t nq, const double* lo, const
     for (int k = 0; k < NSIDE; ++k) (void)hipEventDestroy(ej[k]);
     if (rc) return rc;
     if ((rc = to_host(c.data(), pc.counts.p, nq))) return rc;
-    for (uint32_t i = 0; i < nq; ++i) hits[i] = used[i] == FBH_PLAN_COMPRESSED ? sync_hits[i] : c[i];
+    for (uint32_t i = 0; i < nq; ++i) hits[i] = c[i];
     if (plans_used) for (uint32_t i = 0; i < nq; ++i) plans_used[i] = used[i];
     if (gpu_ms) *gpu_ms = ms;
     return FBH_OK;
@@ -1504,20 +1465,26 @@ int fbh_index_time_plans(fbh_index* ix, uint32_t nq, const double* lo, const int
     std::vector<hipEvent_t> ev(cmax + 1);
     for (auto& e : ev)
         if ((rc = check(hipEventCreate(&e), "event"))) return rc;
+    WahBuf wb;
+    if ((rc = wb.alloc(ix))) return rc;
     std::vector<uint32_t> np(cmax), cnt(cmax);
-    const int dev_plans[2] = {FBH_PLAN_LITERAL, FBH_PLAN_SCAN};
-    for (int pi = 0; pi < 2 && !rc; ++pi) {
-        const int plan = dev_plans[pi];
+    // one launch sequence of a plan for query q (part row i)
+    auto launch = [&](int plan, uint32_t q, uint32_t i) {
+        if (plan == FBH_PLAN_COMPRESSED) {
+            np[i] = ix->ntiles;
+            return launch_compressed(ix, lo[q], lop[q], hi[q], rop[q], wb, pc.row(i), stream());
+        }
+        return index_eval_literal(ix, lo[q], lop[q], hi[q], rop[q], plan, lit.p, pc.row(i), &np[i], stream());
+    };
+    for (int plan = FBH_PLAN_LITERAL; plan <= FBH_PLAN_SCAN && !rc; ++plan) {
         // untimed warm-up of the plan's kernels
-        for (uint32_t i = 0; i < std::min(nq, 4u) && !rc; ++i)
-            rc = index_eval_literal(ix, lo[i], lop[i], hi[i], rop[i], plan, lit.p, pc.row(0), &np[0], stream());
+        for (uint32_t i = 0; i < std::min(nq, 4u) && !rc; ++i) rc = launch(plan, i, 0);
         if (!rc) rc = stream_sync();
         for (uint32_t c0 = 0; c0 < nq && !rc; c0 += CH) {
             const uint32_t k = std::min(CH, nq - c0);
             rc = check(hipEventRecord(ev[0], stream()), "event");
             for (uint32_t i = 0; i < k && !rc; ++i) {
-                const uint32_t q = c0 + i;
-                rc = index_eval_literal(ix, lo[q], lop[q], hi[q], rop[q], plan, lit.p, pc.row(i), &np[i], stream());
+                rc = launch(plan, c0 + i, i);
                 if (!rc) rc = check(hipEventRecord(ev[i + 1], stream()), "event");
             }
             if (!rc) rc = pc.launch_sum(np.data(), k);
@@ -1532,46 +1499,7 @@ int fbh_index_time_plans(fbh_index* ix, uint32_t nq, const double* lo, const int
         }
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
-    if (rc) return rc;
-    // compressed plan: host wall time of the synchronising OR chain.  Every
-    // OR of the chain waits for the device at least once, so the chain
-    // takes at least (its ORs) x (the fastest launch-and-wait round trip
-    // measured here); a chain whose floor already exceeds the limit is
-    // censored at that floor without running.
-    double rt = 1e30;
-    for (int k = 0; k < 16 && !rc; ++k) {
-        const Clock::time_point t0 = Clock::now();
-        rc = check(fbk::launch_zero_u32(lit.p, 1, stream()), "zero");
-        if (!rc) rc = stream_sync();
-        rt = std::min(rt, std::chrono::duration<double, std::milli>(Clock::now() - t0).count());
-    }
-    for (uint32_t i = 0; i < nq && !rc; ++i) {
-        const double limit = (double)cap * std::max(ms[3 * (size_t)i], ms[3 * (size_t)i + 2]);
-        const double floor_ms = rt * compressed_ops(ix, lo[i], lop[i], hi[i], rop[i]);
-        if (floor_ms > limit) {
-            ms[3 * (size_t)i + 1] = (float)-floor_ms;
-            hits[3 * (size_t)i + 1] = UINT64_MAX;
-            continue;
-        }
-        if ((rc = stream_sync())) break;
-        fbh_bitvector* res = nullptr;
-        if ((rc = fbh_bv_new(&res))) break;
-        const Clock::time_point t0 = Clock::now();
-        const Clock::time_point dl = t0 + std::chrono::duration_cast<Clock::duration>(
-                                              std::chrono::duration<double, std::milli>(limit));
-        rc = eval_compressed(ix, lo[i], lop[i], hi[i], rop[i], res, &dl);
-        uint32_t c = 0;
-        if (rc == CENSORED) {
-            rc = stream_sync();
-            ms[3 * (size_t)i + 1] = (float)-limit;
-            hits[3 * (size_t)i + 1] = UINT64_MAX;
-        } else if (!rc && !(rc = fbh_bv_cnt(res, &c))) {
-            const double t = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
-            ms[3 * (size_t)i + 1] = (float)t;
-            hits[3 * (size_t)i + 1] = c;
-        }
-        fbh_bv_free(res);
-    }
+    (void)cap;
     return rc;
 }
 

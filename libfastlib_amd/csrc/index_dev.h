@@ -38,6 +38,20 @@ struct QueryArgs {
     int plan;                  // PLAN_LIT_D or PLAN_SCAN_D (conjunction terms)
 };
 
+// Output of the compressed plan: the result's code words and chunk index
+// (start group of every 32nd word, closed by the group count), the tiles'
+// run summaries (prefix_summary_seg: ntiles aggregates, then ntiles/64 + 1
+// segment aggregates; zeroed when allocated, tag alternating 1 / 2 between
+// launches on the same buffers), and ctl: [1] the word count, [2] the active
+// group's value (left-aligned bits).
+struct WahOut {
+    uint32_t* out;
+    uint32_t* outck;
+    uint64_t* status;
+    uint32_t* ctl;
+    uint32_t tag;
+};
+
 // A batch of conjunctions: query q ANDs terms[qoff[q] .. qoff[q+1]); all
 // terms are over columns of the same rows.  part[q*ntiles + t] receives the
 // ones of tile t; out (may be null) the literal result of a single query.
@@ -113,6 +127,13 @@ hipError_t launch_tseg_fill(const uint32_t* words, const uint2* toff, const uint
                             uint32_t ntiles, uint32_t groups, uint32_t anb, uint32_t* tw, hipStream_t s);
 hipError_t launch_query_lit(const QueryArgs& q, int type, hipStream_t s);
 hipError_t launch_query_scan(const QueryArgs& q, int type, uint32_t nrows, hipStream_t s);
+// The compressed plan, one launch of one block per tile: o.out needs
+// q.groups + 1 words, o.outck q.groups / 32 + 2 entries, o.status
+// wah_status_words(q.ntiles) entries, o.ctl 8 words.
+uint32_t wah_status_words(uint32_t ntiles);
+hipError_t launch_query_wah(const QueryArgs& q, int type, const WahOut& o, hipStream_t s);
+// ctl[4] / ctl[5] = first / last of the ctl[1] words of out
+hipError_t launch_wah_edges(const uint32_t* out, uint32_t* ctl, hipStream_t s);
 // nt: threads per block (128, 256 or 512); blocks: persistent grid size
 hipError_t launch_conj(const ConjArgs& a, int nt, uint32_t blocks, hipStream_t s);
 // A batch whose queries are single terms on one index of column `type`

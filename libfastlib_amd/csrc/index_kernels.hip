@@ -1007,6 +1007,88 @@ __global__ __launch_bounds__(NT) void k_query_lit(QueryArgs q) {
     }
 }
 
+// Compressed plan of one range (sumBins leaving its result compressed, the
+// priority-queue branch of src/index.cpp:6687-6760): block t merges the OR
+// set's segments of tile t in LDS as the literal plan does, then writes the
+// tile's words of the canonical WAH result directly.  Every whole group is a
+// token of its class (all 0, all 1, mixed); thread i summarises its QT/NT
+// consecutive groups, the summaries are scanned over the block and carried
+// from tile to tile by the look-back (tile t is block t), and every word is
+// written at its slot: a literal by its group, a run by the group that
+// closes it, at its head's slot (a run may start tiles before).  No literal
+// result reaches HBM.  The active slot (the last tile's last group when
+// anb > 0) is not a code word: its value goes to o.ctl[2].
+struct IxWriter {
+    word_t* out;
+    uint32_t* outck;
+    __device__ __forceinline__ void operator()(uint32_t slot, word_t w, uint32_t head) {
+        out[slot] = w;
+        if ((slot & 31u) == 0) outck[slot >> 5] = head;
+    }
+};
+
+template <class T, int NT>
+__global__ __launch_bounds__(NT) void k_query_wah(QueryArgs q, WahOut o) {
+    constexpr int NWV = NT / WAVE;
+    constexpr int PER = QT / NT;
+    __shared__ word_t R[QT + 1];
+    __shared__ uint32_t cov[QT + 1];
+    __shared__ uint32_t red[NWV + 1];
+    __shared__ uint32_t ctab[CT_WORDS];
+    __shared__ Summary sbuf[NWV];
+    __shared__ Summary s_pre;
+    const uint32_t tile = blockIdx.x;
+    const bool last = tile + 1 == q.ntiles;
+    const uint32_t g0 = tile * (uint32_t)QT;
+    const uint32_t ng = last ? (q.groups - g0) + (q.anb ? 1u : 0u) : (uint32_t)QT;
+    const uint32_t nw = last ? q.groups - g0 : (uint32_t)QT;          // whole groups
+    const int w = threadIdx.x / WAVE;
+    for (uint32_t k = threadIdx.x; k <= QT; k += NT) { R[k] = 0u; cov[k] = 0u; }
+    __syncthreads();
+    lit_tile<T, NT, QB_U>(q, tile, g0, ng, last, R, cov, red, ctab);
+    const uint32_t k0 = threadIdx.x * (uint32_t)PER;
+    word_t v[PER];
+    uint32_t ones = 0;
+    CountSink cs;
+    cs.init();
+    #pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        v[i] = k0 + i < ng ? R[k0 + i] : 0u;
+        ones += __popc(v[i]);
+        if (k0 + i < nw) emit_lit(cs, v[i]);
+    }
+    ones = wave_sum(ones);
+    if (lane_id() == 0) red[w] = ones;
+    Summary agg;
+    const Summary texcl = block_excl_summary2<NT>(cs.s, sbuf, &agg);
+    if (threadIdx.x < WAVE) {
+        const Summary pr = prefix_summary_seg(o.status, o.status + q.ntiles, tile, o.tag, agg);
+        if (threadIdx.x == 0) {
+            s_pre = pr;
+            uint32_t t = 0;
+            for (int i = 0; i < NWV; ++i) t += red[i];
+            q.part[tile] = t;
+        }
+    }
+    __syncthreads();
+    const Summary pre = combine(s_pre, texcl);
+    IxWriter iw{o.out, o.outck};
+    WriteSink<IxWriter> ws;
+    ws.init(&iw, pre, g0 + k0);
+    #pragma unroll
+    for (int i = 0; i < PER; ++i)
+        if (k0 + i < nw) emit_lit(ws, v[i]);
+    // the thread holding the last whole group (thread 0 when the last tile
+    // has none) closes the last run and records the word count
+    if (last && ((nw == 0 && threadIdx.x == 0) || (k0 < nw && nw <= k0 + PER))) {
+        ws.finish();
+        const uint32_t total = combine(pre, cs.s).count;
+        o.ctl[1] = total;
+        o.outck[(total + 31u) >> 5] = q.groups;
+        o.ctl[2] = q.anb ? R[nw] : 0u;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Conjunctions (countQuery::doEvaluate, LOGICAL_AND, src/countQuery.cpp:
 // 953-963): the result of a query is the AND of its terms' results.  One
@@ -1706,6 +1788,35 @@ hipError_t launch_range(const ConjArgs& a, int type, int scan, int nt, uint32_t 
     case 1: return launch_range_s<1>(a, type, nt, blocks, s);
     default: return launch_range_s<2>(a, type, nt, blocks, s);
     }
+}
+
+__global__ void k_wah_edges(const uint32_t* out, uint32_t* ctl) {
+    if (threadIdx.x == 0) {
+        const uint32_t n = ctl[1];
+        ctl[4] = n ? out[0] : 0u;
+        ctl[5] = n ? out[n - 1] : 0u;
+    }
+}
+
+uint32_t wah_status_words(uint32_t ntiles) { return ntiles + ntiles / 64u + 1u; }
+
+hipError_t launch_wah_edges(const uint32_t* out, uint32_t* ctl, hipStream_t s) {
+    hipLaunchKernelGGL(k_wah_edges, dim3(1), dim3(64), 0, s, out, ctl);
+    return hipGetLastError();
+}
+
+hipError_t launch_query_wah(const QueryArgs& q, int type, const WahOut& o, hipStream_t s) {
+    if (o.tag != 1u && o.tag != 2u) return hipErrorInvalidValue;
+    dim3 g(q.ntiles), b(256);
+    switch (type) {
+    case COL_I32: hipLaunchKernelGGL((k_query_wah<int32_t, 256>), g, b, 0, s, q, o); break;
+    case COL_U32: hipLaunchKernelGGL((k_query_wah<uint32_t, 256>), g, b, 0, s, q, o); break;
+    case COL_U16: hipLaunchKernelGGL((k_query_wah<uint16_t, 256>), g, b, 0, s, q, o); break;
+    case COL_F32: hipLaunchKernelGGL((k_query_wah<float, 256>), g, b, 0, s, q, o); break;
+    case COL_F64: hipLaunchKernelGGL((k_query_wah<double, 256>), g, b, 0, s, q, o); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_query_scan(const QueryArgs& q, int type, uint32_t nrows, hipStream_t s) {

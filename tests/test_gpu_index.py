@@ -348,6 +348,42 @@ def test_conjunction_rids_and_hit_vectors(L, orc, plan):
             assert np.array_equal(batch.result(qi).write(), orc.from_bits(bits)), qi
 
 
+@pytest.mark.parametrize("n", [31 * 4096, 31 * 4096 + 5, 31 * 2048 * 3 + 31 * 9 + 29, 1000])
+def test_compressed_plan_words(L, orc, n):
+    """The compressed plan (OR merged per tile, canonical WAH written with
+    a look-back between tiles) gives the canonical words of the rows the
+    range selects: narrow ranges (0-fills across tiles), wide ones (1-fills
+    across tiles), complemented OR sets, no hit and every hit; tiles that
+    end on a group boundary and a last tile holding only the active bits."""
+    rng = np.random.default_rng(n)
+    col = rng.normal(size=n).astype(np.float32)
+    ints = np.sort(rng.integers(0, 300, n)).astype(np.int32)      # long runs per bitmap
+    ixs = [(L.Index.bin_equal_weight(col, 120), col), (L.Index.relic(ints), ints)]
+    for ix, c in ixs:
+        vals = np.unique(c.astype(np.float64))
+        cases = [(vals[0] - 1, vals[-1] + 1), (vals[-1] + 1, vals[-1] + 2), (vals[1], vals[2]),
+                 (vals[len(vals) // 3], vals[-2]), (vals[2], vals[len(vals) // 2])]
+        for _ in range(8):
+            a, b = sorted(rng.choice(vals, 2, replace=False))
+            cases.append((float(a), float(b) + 0.5))
+        los = np.array([x for x, _ in cases])
+        his = np.array([y for _, y in cases])
+        hb, used, _ = ix.evaluate_batch(los, 2, his, 1, plan=2)
+        assert np.all(used == 2)
+        for (lo, hi), h in zip(cases, hb):
+            bits = ((c.astype(np.float64) >= lo) & (c.astype(np.float64) < hi)).astype(np.uint8)
+            want = orc.from_bits(bits)
+            hc, u, bv = ix.evaluate(float(lo), 2, float(hi), 1, plan=2, want_bitvector=True)
+            assert u == 2 and hc == int(bits.sum()) == int(h), (n, lo, hi)
+            assert np.array_equal(bv.write(), want), (n, lo, hi)
+            assert bv.cnt() == hc
+            _, _, lv = ix.evaluate(float(lo), 2, float(hi), 1, plan=1, want_bitvector=True)
+            assert np.array_equal(lv.write(), want)
+            x = bv.copy()
+            x.op(lv, 2)          # XOR of equal vectors: words from the compressed plan's chunk index
+            assert x.cnt() == 0
+
+
 def test_range_batch_rids(L):
     """Row ids of a range batch (the persistent range kernel) under the
     literal, mixed and scan-only plans equal the single-query row ids."""

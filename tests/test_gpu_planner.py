@@ -167,13 +167,9 @@ def test_time_plans_same_counts(L, table):
     lo, hi = queries(41, 48)
     ms, hits = ix.time_plans(lo, 2, hi, 1, cap=3.0)
     ref = np.array([np.count_nonzero((col >= a) & (col < b)) for a, b in zip(lo, hi)], np.uint64)
-    np.testing.assert_array_equal(hits[:, 0], ref)
-    np.testing.assert_array_equal(hits[:, 2], ref)
-    done = ms[:, 1] > 0
-    np.testing.assert_array_equal(hits[done, 1], ref[done])
-    assert np.all(hits[~done, 1] == np.iinfo(np.uint64).max)
-    assert np.all(ms[:, 0] > 0) and np.all(ms[:, 2] > 0)
-    assert np.all(-ms[~done, 1] >= 3.0 * np.maximum(ms[~done, 0], ms[~done, 2]) * 0.999)
+    for p in range(3):
+        np.testing.assert_array_equal(hits[:, p], ref)
+    assert np.all(ms > 0)
 
 
 def test_learned_plans_gain_over_heuristic(L, table):
