@@ -325,7 +325,7 @@ int launch_word_op(fbh_bitvector* a, fbh_bitvector* b, int op, bool c2, uint32_t
     if (!lb || c2) { rc = bv_ensure_ck(b); if (rc) return rc; }
     if (c2) {
         const uint32_t nt = fbk::op_tiles(a->nw, b->nw);
-        uint8_t* sc = (uint8_t*)scratch(0).get(64 + 8ull * nt + 64);
+        uint8_t* sc = (uint8_t*)scratch(0).get(64 + 8ull * fbk::op_status_words(nt) + 64);
         if (!sc) return fail(FBH_ERR_NOMEM, "scratch allocation failed");
         *ctl_out = (uint32_t*)sc;
         return check(fbk::launch_op(a->d, a->ck, a->nw, b->d, b->ck, b->nw, groups, op, c2_mode,

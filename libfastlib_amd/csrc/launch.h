@@ -16,6 +16,8 @@ hipError_t launch_scan_excl(const uint32_t* in, uint32_t* out, uint32_t n, uint3
 hipError_t launch_chunk_index(const word_t* w, uint32_t n, uint32_t* ck, uint32_t* tmp,
                               hipStream_t s);
 uint32_t op_tiles(uint32_t na, uint32_t nb);
+// status entries launch_op needs for nt = op_tiles(na, nb) tiles
+uint32_t op_status_words(uint32_t nt);
 hipError_t launch_op(const word_t* a, const uint32_t* cka, uint32_t na,
                      const word_t* b, const uint32_t* ckb, uint32_t nb,
                      uint32_t groups, int op, int mode, word_t* out, uint32_t* outck,

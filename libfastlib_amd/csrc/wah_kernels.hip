@@ -355,7 +355,7 @@ __global__ __launch_bounds__(TPB) void k_op_tiles(OpArgs p) {
     Summary agg;
     const Summary texcl = block_excl_summary2<TPB>(cs.s, sbuf, &agg);
     if (tid < WAVE) {
-        const Summary pr = lookback_summary_wave(p.status, tile, agg);
+        const Summary pr = prefix_summary_seg(p.status, p.status + gridDim.x, tile, 1u, agg);
         if (tid == 0) s_pre = pr;
     }
     __syncthreads();
@@ -438,7 +438,7 @@ __global__ __launch_bounds__(TPB) void k_tok_tiles(TokArgs p) {
     Summary agg;
     const Summary texcl = block_excl_summary2<TPB>(cs.s, sbuf, &agg);
     if (tid < WAVE) {
-        const Summary pr = lookback_summary_wave(p.status, tile, agg);
+        const Summary pr = prefix_summary_seg(p.status, p.status + gridDim.x, tile, 1u, agg);
         if (tid == 0) { s_pre = pr; s_base_pos = (p.mode == 0) ? 0u : base; }
     }
     __syncthreads();
@@ -685,6 +685,10 @@ uint32_t op_tiles(uint32_t na, uint32_t nb) {
     return items ? cdiv(items, KTILE) : 1u;
 }
 
+// Run-summary entries of an op launch of nt tiles (prefix_summary_seg: the
+// tiles' aggregates, then their segments').
+uint32_t op_status_words(uint32_t nt) { return nt + nt / 64u + 1u; }
+
 // Clears the look-back status words and the 8 control words of a launch.
 __global__ void k_zero_status_ctl(uint64_t* status, uint32_t n, uint32_t* ctl) {
     const uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x;
@@ -703,7 +707,8 @@ hipError_t launch_op(const word_t* a, const uint32_t* cka, uint32_t na,
     p.out = out; p.outck = outck; p.status = status;
     p.total = ctl + 1;
     const uint32_t nt = op_tiles(na, nb);
-    hipLaunchKernelGGL(k_zero_status_ctl, dim3(min(cdiv(nt, TPB), 4096u)), dim3(TPB), 0, s, status, nt, ctl);
+    const uint32_t ns = op_status_words(nt);
+    hipLaunchKernelGGL(k_zero_status_ctl, dim3(min(cdiv(ns, TPB), 4096u)), dim3(TPB), 0, s, status, ns, ctl);
     hipLaunchKernelGGL(k_op_tiles, dim3(nt), dim3(TPB), 0, s, p);
     return hipGetLastError();
 }
