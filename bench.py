@@ -59,10 +59,10 @@ def config3(log2n, nq, extra=()):
     return col, lo, hi, more
 
 
-def measured_traffic(kernel):
+def measured_traffic(kernel, summary="traffic_bin2p27_1024q.txt"):
     """HBM bytes per launch of `kernel` on this workload, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counter-scaled), or None."""
-    path = os.path.join(ROOT, "profiles", "traffic_bin2p27_1024q.txt")
+    path = os.path.join(ROOT, "profiles", summary)
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -153,6 +153,20 @@ def conj_section(L, log2n, nq, steps, learned=None, cpu=None):
         hits, m = batch.run()
         ms.append(m)
     plans = batch.plans()
+    # bytes the terms' plans read with no early exit (bitmap words of the OR
+    # sets and edge bins, raw values of edge rows or of the scanned column):
+    # the kernel stops a tile at its first empty term, so it reads less
+    res_bytes = 4.0 * (n // 31 + 1)
+    flat = [(ci, lo, hi) for q in qs for (ci, lo, hi) in q]
+    nominal = sum(ixs[ci].query_bytes(float(lo), 2, float(hi), 1, int(p)) - res_bytes
+                  for (ci, lo, hi), p in zip(flat, plans))
+    # row ids of every query: a count replay, one more launch writing the
+    # ids, the copy to host memory
+    t = time.perf_counter()
+    roff, rids = batch.rids()
+    rid_s = time.perf_counter() - t
+    if not np.array_equal(np.diff(roff), hits):
+        raise RuntimeError("row-id offsets differ from the counts")
     # the same batch with every term scanned gives the same counts
     ref, _ = L.Conjunctions([[(ixs[ci], lo, 2, hi, 1) for (ci, lo, hi) in q] for q in qs], plan=3).run()
     if not np.array_equal(ref, hits):
@@ -160,7 +174,18 @@ def conj_section(L, log2n, nq, steps, learned=None, cpu=None):
     out = dict(rows=n, queries=nq, terms=int(batch.nterms), index_build_s=round(build_s, 2),
                qps=round(nq / (np.median(ms) * 1e-3), 1), ms_per_batch=round(float(np.median(ms)), 3),
                tuned=cfg, term_plans={"literal": int(np.sum(plans == 1)), "scan": int(np.sum(plans == 3))},
-               hits_total=int(hits.sum()))
+               hits_total=int(hits.sum()),
+               nominal_term_GB=round(nominal / 1e9, 3),
+               nominal_GBs=round(nominal / (np.median(ms) * 1e-3) / 1e9, 1),
+               traffic_per_launch=measured_traffic("k_conj", "traffic_conj_4x2p26.txt"),
+               rid_output=dict(rids=int(len(rids)), seconds=round(rid_s, 3),
+                               rids_per_s=round(len(rids) / rid_s, 0),
+                               note="fbh_conj_rids: count replay, row-id launch, ids copied to host "
+                                    "memory (PCIe-inclusive wall time)"))
+    tr = out["traffic_per_launch"]
+    if tr:
+        out["hbm_GBs_measured"] = round(tr / (np.median(ms) * 1e-3) / 1e9, 1)
+        out["hbm_frac_measured"] = round(out["hbm_GBs_measured"] / HBM_PEAK_GBS, 4)
     log("conj", out)
     if cpu:
         out["cpu_baseline"] = cpu_baseline_conj(ixs, cols, qs, hits, *cpu)
@@ -457,6 +482,8 @@ def main():
                     help="time only the WAH AND/OR cells (REPS ops each) and print them as JSON")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-conj", action="store_true")
+    ap.add_argument("--conj-only", action="store_true",
+                    help="run only the conjunction / expression batches (no CPU baseline) and print them as JSON")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--train-queries", type=int, default=50000)
     ap.add_argument("--test-queries", type=int, default=8192)
@@ -472,6 +499,10 @@ def main():
     L.lib()
     if args.wah_only:
         print(json.dumps({"wah_ops_2p28": wah_cells(L, reps=args.wah_only)}))
+        return
+    if args.conj_only:
+        conj, _ = conj_section(L, 26, 4096, args.steps)
+        print(json.dumps({"conjunctions_4x2p26": conj}))
         return
     extra = () if args.no_learned else ((G.BIN_F32_TRAIN_SEED, args.train_queries),
                                         (G.BIN_F32_TEST_SEED, args.test_queries))
