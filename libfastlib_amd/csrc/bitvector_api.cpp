@@ -323,7 +323,11 @@ int launch_word_op(fbh_bitvector* a, fbh_bitvector* b, int op, bool c2, uint32_t
     int rc;
     if (!la || c2) { rc = bv_ensure_ck(a); if (rc) return rc; }
     if (!lb || c2) { rc = bv_ensure_ck(b); if (rc) return rc; }
-    if (c2) {
+    // dense c2 operands (code words at least 1/16 of the groups): the
+    // canonical group-domain kernel, whose cost follows the groups rather
+    // than the words, then the c2 routines' count-1 fills written in
+    const bool c2dense = c2 && c2_mode == 0 && 16ull * ((uint64_t)a->nw + b->nw) >= groups;
+    if (c2 && !c2dense) {
         const uint32_t nt = fbk::op_tiles(a->nw, b->nw);
         uint8_t* sc = (uint8_t*)scratch(0).get(64 + 8ull * fbk::op_status_words(nt) + 64);
         if (!sc) return fail(FBH_ERR_NOMEM, "scratch allocation failed");
@@ -337,9 +341,13 @@ int launch_word_op(fbh_bitvector* a, fbh_bitvector* b, int op, bool c2, uint32_t
     uint8_t* sc = (uint8_t*)scratch(0).get(ts_off + 16ull * (nt + 1) + 64);
     if (!sc) return fail(FBH_ERR_NOMEM, "scratch allocation failed");
     *ctl_out = (uint32_t*)sc;
-    return check(fbk::launch_canon(a->d, a->ck, a->nw, la ? 1 : 0, b->d, b->ck, b->nw, lb ? 1 : 0,
-                                   groups, op, ob.d, ob.ck, (uint64_t*)(sc + 64), *ctl_out,
-                                   (uint2*)(sc + ts_off), stream()), "canonical op");
+    int rc2 = check(fbk::launch_canon(a->d, a->ck, a->nw, la ? 1 : 0, b->d, b->ck, b->nw, lb ? 1 : 0,
+                                      groups, op, ob.d, ob.ck, (uint64_t*)(sc + 64), *ctl_out,
+                                      (uint2*)(sc + ts_off), stream()), "canonical op");
+    if (!rc2 && c2dense)
+        rc2 = check(fbk::launch_c2_fix(a->d, a->ck, a->nw, b->d, b->ck, b->nw, ob.d, ob.ck, *ctl_out,
+                                       a->nw + b->nw, op, stream()), "c2 representations");
+    return rc2;
 }
 
 int run_word_op(const fbh_bitvector* a, const fbh_bitvector* b, int op, bool c2,

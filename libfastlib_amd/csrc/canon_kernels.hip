@@ -999,18 +999,55 @@ __device__ __forceinline__ void mix_find_cover(const Operand& o, uint32_t g, uin
     ws = (uint32_t)__builtin_amdgcn_readlane((int)start, (int)L);
 }
 
+// mix_find_cover for two compressed operands at once: the levels of both
+// searches advance together, so their dependent loads overlap.
+__device__ __forceinline__ void mix_find_cover2(const Operand& a, const Operand& b, uint32_t g, uint32_t& wia,
+                                                uint32_t& wsa, uint32_t& wib, uint32_t& wsb) {
+    const uint32_t lane = lane_id();
+    const uint32_t nca = (a.n + CHUNKW - 1) / CHUNKW, ncb = (b.n + CHUNKW - 1) / CHUNKW;
+    uint32_t loa = 0, hia = nca, lob = 0, hib = ncb;
+    while (hia - loa > 1u || hib - lob > 1u) {
+        const uint32_t sa = (hia - loa + WAVE - 1) / WAVE, sb = (hib - lob + WAVE - 1) / WAVE;
+        const uint32_t ja = loa + lane * sa, jb = lob + lane * sb;
+        const bool ta = hia - loa > 1u && ja < hia && a.ck[min(ja, nca)] <= g;
+        const bool tb = hib - lob > 1u && jb < hib && b.ck[min(jb, ncb)] <= g;
+        const uint64_t ma = ballot64(ta), mb = ballot64(tb);
+        if (hia - loa > 1u) { loa += (63u - (uint32_t)__clzll(ma)) * sa; hia = min(loa + sa, hia); }
+        if (hib - lob > 1u) { lob += (63u - (uint32_t)__clzll(mb)) * sb; hib = min(lob + sb, hib); }
+    }
+    const uint32_t ia = loa * CHUNKW + lane, ib = lob * CHUNKW + lane;
+    const bool ina = lane < CHUNKW && ia < a.n, inb = lane < CHUNKW && ib < b.n;
+    const uint32_t lena = ina ? word_len(a.w[min(ia, a.n - 1u)]) : 0u;
+    const uint32_t lenb = inb ? word_len(b.w[min(ib, b.n - 1u)]) : 0u;
+    const uint32_t sta = a.ck[loa] + wave_incl_sum(lena) - lena;
+    const uint32_t stb = b.ck[lob] + wave_incl_sum(lenb) - lenb;
+    const uint64_t cma = ballot64(ina && sta <= g && g - sta < lena);
+    const uint64_t cmb = ballot64(inb && stb <= g && g - stb < lenb);
+    const uint32_t La = (uint32_t)__ffsll((unsigned long long)cma) - 1u;
+    const uint32_t Lb = (uint32_t)__ffsll((unsigned long long)cmb) - 1u;
+    wia = loa * CHUNKW + La;
+    wsa = (uint32_t)__builtin_amdgcn_readlane((int)sta, (int)La);
+    wib = lob * CHUNKW + Lb;
+    wsb = (uint32_t)__builtin_amdgcn_readlane((int)stb, (int)Lb);
+}
+
 // Values of groups g0 + 4 lane + j of compressed operand o, whose word wi
 // (starting at group ws <= g0) holds g0; wi / ws move on to the word holding
 // g0 + 256.  mk / ww: the wave's LDS rows.
+// The 256 words from wi (clamped to the operand's last word), 4 per lane.
+__device__ __forceinline__ uint4 mix_window(const Operand& o, uint32_t wi) {
+    const uint32_t i = wi + 4u * lane_id(), nl = o.n - 1u;
+    return make_uint4(o.w[min(i, nl)], o.w[min(i + 1u, nl)], o.w[min(i + 2u, nl)], o.w[min(i + 3u, nl)]);
+}
+
 __device__ __forceinline__ uint4 mix_expand_round(const Operand& o, uint32_t g0, uint32_t& wi, uint32_t& ws,
-                                                  uint32_t* mk, word_t* ww) {
+                                                  const uint4& wv4, uint32_t* mk, word_t* ww) {
     const uint32_t lane = lane_id();
-    word_t w[4];
+    const word_t w[4] = {wv4.x, wv4.y, wv4.z, wv4.w};
     uint32_t len[4], sum = 0;
     #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const uint32_t i = wi + 4u * lane + (uint32_t)j;
-        w[j] = o.w[min(i, o.n - 1u)];
         len[j] = i < o.n ? word_len(w[j]) : 0u;
         sum += len[j];
     }
@@ -1079,8 +1116,12 @@ __global__ __launch_bounds__(LW_TPB) void k_mix_fused(CanonArgs p, uint32_t nrou
     const uint32_t G0 = 4u * s.c0, G1 = 4u * s.c1;
     const bool two = OP >= 0;
     uint32_t wia = 0, wsa = 0, wib = 0, wsb = 0;
-    if (s.nr && !p.a.literal) mix_find_cover(p.a, G0, wia, wsa);
-    if (two && s.nr && !p.b.literal) mix_find_cover(p.b, G0, wib, wsb);
+    if (s.nr && two && !p.a.literal && !p.b.literal) {
+        mix_find_cover2(p.a, p.b, G0, wia, wsa, wib, wsb);
+    } else {
+        if (s.nr && !p.a.literal) mix_find_cover(p.a, G0, wia, wsa);
+        if (two && s.nr && !p.b.literal) mix_find_cover(p.b, G0, wib, wsb);
+    }
     word_t before = SENT;
     if (s.nr && G0 > 0) {
         const word_t x = mix_group_before(p.a, G0 - 1u, wia, wsa);
@@ -1093,12 +1134,15 @@ __global__ __launch_bounds__(LW_TPB) void k_mix_fused(CanonArgs p, uint32_t nrou
     for (int r = 0; r < LF_R; ++r) {
         if ((uint32_t)r >= s.nr) break;
         const uint32_t g0 = G0 + 256u * (uint32_t)r;
-        const uint4 va = p.a.literal ? mix_lit_round(p.a, g0, groups)
-                                     : mix_expand_round(p.a, g0, wia, wsa, s_mk[wv], s_row[wv]);
+        // both operands' windows (or literal groups) in flight before either
+        // is expanded
+        const uint4 wa = p.a.literal ? mix_lit_round(p.a, g0, groups) : mix_window(p.a, wia);
+        uint4 wb = make_uint4(0, 0, 0, 0);
+        if (two) wb = p.b.literal ? mix_lit_round(p.b, g0, groups) : mix_window(p.b, wib);
+        const uint4 va = p.a.literal ? wa : mix_expand_round(p.a, g0, wia, wsa, wa, s_mk[wv], s_row[wv]);
         uint4 x = va;
         if (two) {
-            const uint4 vb = p.b.literal ? mix_lit_round(p.b, g0, groups)
-                                         : mix_expand_round(p.b, g0, wib, wsb, s_mk[wv], s_row[wv]);
+            const uint4 vb = p.b.literal ? wb : mix_expand_round(p.b, g0, wib, wsb, wb, s_mk[wv], s_row[wv]);
             x = lit_op4<OP>(va, vb);
         }
         const uint32_t g = g0 + 4u * lane;

@@ -39,6 +39,13 @@ hipError_t launch_rids_lit(const word_t* lit, uint32_t n, uint32_t* out, uint32_
 // operand with lit != 0 holds one literal word per group.  status needs
 // canon_status(groups) entries; ctl 8 words; ts 2 * (canon_tiles(groups) + 1)
 // entries (tile starts of the compressed operands).
+// After a canonical op kernel wrote the canonical words of a OP b (both
+// operands compressed, with chunk indexes) to out / outck with their count
+// in ctl[1] (at most maxwords): the *_c2 routines' words, every literal
+// 0 / all-ones word that the routine writes as a count-1 fill rewritten.
+hipError_t launch_c2_fix(const word_t* a, const uint32_t* cka, uint32_t na, const word_t* b, const uint32_t* ckb,
+                         uint32_t nb, word_t* out, const uint32_t* outck, const uint32_t* ctl, uint32_t maxwords,
+                         int op, hipStream_t s);
 uint32_t canon_tiles(uint32_t groups);
 uint32_t canon_status(uint32_t groups);
 hipError_t launch_canon(const word_t* a, const uint32_t* cka, uint32_t na, int alit,

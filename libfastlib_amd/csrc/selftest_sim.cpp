@@ -98,4 +98,43 @@ long fbsim_op(const uint32_t* a, uint32_t na, const uint32_t* b, uint32_t nb,
     return (long)res.size();
 }
 
+// The *_c2 result as the group-domain kernels write it: the canonical words
+// of the op's group values, with every single-group run of all-0 or all-1
+// groups written as a count-1 fill where c2_rep_at finds a fill token
+// there.  Returns the word count or a negative value.
+long fbsim_c2_canon_rep(const uint32_t* a, uint32_t na, const uint32_t* b, uint32_t nb, uint32_t groups,
+                        int op, uint32_t* out, uint32_t cap) {
+    ArrayReader ra{a}, rb{b};
+    std::vector<uint32_t> sa(na + 1), sb(nb + 1);
+    sa[0] = 0; for (uint32_t i = 0; i < na; ++i) sa[i + 1] = sa[i] + word_len(a[i]);
+    sb[0] = 0; for (uint32_t i = 0; i < nb; ++i) sb[i + 1] = sb[i] + word_len(b[i]);
+    if (sa[na] != groups || sb[nb] != groups) return -1;
+    std::vector<word_t> v(groups);
+    for (uint32_t i = 0, g = 0; i < na; ++i)
+        for (uint32_t k = 0; k < word_len(a[i]); ++k) v[g++] = group_value(a[i]);
+    for (uint32_t i = 0, g = 0; i < nb; ++i)
+        for (uint32_t k = 0; k < word_len(b[i]); ++k, ++g) v[g] = apply_op(v[g], group_value(b[i]), op);
+    std::vector<word_t> res;
+    for (uint32_t g = 0; g < groups;) {
+        const uint32_t c = class_of_value(v[g]);
+        uint32_t e = g + 1;
+        if (c != C_MIXED)
+            while (e < groups && v[e] == v[g]) ++e;
+        if (c == C_MIXED) {
+            res.push_back(v[g]);
+        } else if (e - g >= 2) {
+            res.push_back((c == C_ONE ? HEADER1 : HEADER0) | (e - g));
+        } else {
+            const uint32_t ia = (uint32_t)(std::upper_bound(sa.begin(), sa.end(), g) - sa.begin()) - 1;
+            const uint32_t ib = (uint32_t)(std::upper_bound(sb.begin(), sb.end(), g) - sb.begin()) - 1;
+            const bool fill = c2_rep_at(&ra, &rb, ia, sa[ia], ib, sb[ib], g, op);
+            res.push_back(fill ? ((c == C_ONE ? HEADER1 : HEADER0) | 1u) : v[g]);
+        }
+        g = e;
+    }
+    if (res.size() > cap) return -3;
+    std::memcpy(out, res.data(), res.size() * sizeof(word_t));
+    return (long)res.size();
+}
+
 } // extern "C"

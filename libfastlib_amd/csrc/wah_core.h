@@ -460,6 +460,48 @@ WAH_HD uint32_t find_sync(const R* ra, const R* rb, uint32_t& ia, uint32_t& pa, 
     }
 }
 
+// The *_c2 loop of op writes the canonical words of its result except for
+// a result group g that is a run of its own (all 0 or all 1, its neighbours
+// of another class): that group's word is a count-1 fill when its token is
+// a fill and a literal when it is a literal.  c2_rep_at returns true for a
+// fill.  (ia, pa) / (ib, pb): the code words of A / B holding g and their
+// first groups.  The walk starts at the last synchronisation point at or
+// before g (find_sync's rule, taken backwards: the later of the two word
+// starts, unless that word is a fill of one group starting inside the
+// other operand's fill) and runs the step logic up to g.  The token that
+// covers g has length 1, since g is a run of its own, so cutting the walk
+// after g leaves it as the full loop emits it.
+struct RepSink {
+    uint32_t pos, g, rep;
+    WAH_HD void tok(uint32_t /*cls*/, uint32_t len, uint32_t repF, word_t /*lit*/, uint32_t /*merge*/) {
+        if (pos <= g && g - pos < len) rep = repF;
+        pos += len;
+    }
+};
+
+template <class R>
+WAH_HD bool c2_rep_at(const R* ra, const R* rb, uint32_t ia, uint32_t pa, uint32_t ib, uint32_t pb, uint32_t g,
+                      int op) {
+    for (;;) {
+        if (pa == pb) break;
+        if (pa > pb) {
+            const word_t w = ra->at(ia);
+            if (!(w > ALLONES && word_len(w) == 1u)) break;
+            --ia;
+            pa -= word_len(ra->at(ia));
+        } else {
+            const word_t w = rb->at(ib);
+            if (!(w > ALLONES && word_len(w) == 1u)) break;
+            --ib;
+            pb -= word_len(rb->at(ib));
+        }
+    }
+    const uint32_t s = pa > pb ? pa : pb;
+    RepSink sink{s, g, 0u};
+    walk(sink, ra, rb, ia, pa, ib, pb, s, g + 1u, op, 0u);
+    return sink.rep != 0u;
+}
+
 struct ArrayReader {
     const word_t* p;
     WAH_HD word_t at(uint32_t i) const { return p[i]; }

@@ -722,6 +722,127 @@ __global__ void k_edge_words(const word_t* out, uint32_t* ctl) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// *_c2 results from the group domain.  A canonical op kernel has written the
+// canonical words of a OP b; they are the *_c2 routines' words except that
+// a result group forming a run of its own (a literal 0 / all-ones word) is a
+// count-1 fill where the routine's token there is a fill (c2_rep_at).  Wave
+// w fixes result words [512w, 512w + 512), lane l the eight from 512w + 8l:
+// their first groups from the result's chunk index and a wave prefix sum;
+// the chunks of A and B that hold the wave's groups by 64-way searches (both
+// ends, both operands, at once); then each lane's candidate words by a
+// binary search within those chunks, one chunk's words, and the walk.
+constexpr uint32_t FIX_WORDS = 512;
+
+// The chunk j of an operand with ck[j] <= g < ck[j + 1] by one wave.
+__device__ __forceinline__ uint32_t wave_chunk_of(const uint32_t* ck, uint32_t nc, uint32_t g) {
+    return wave_last_le(ck, nc + 1u, g);
+}
+
+// The chunk j of chunks [c0, c1) with ck[j] <= g < ck[j + 1], by one lane.
+__device__ __forceinline__ uint32_t lane_chunk_of(const uint32_t* ck, uint32_t c0, uint32_t c1, uint32_t g) {
+    uint32_t lo = c0, hi = c1;                 // ck[lo] <= g < ck[hi]
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (ck[mid] <= g) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// The code word holding group g, in chunk j (first group c), and its first
+// group, by one lane: the chunk's words eight at a time (two 16-byte loads;
+// clamped scalar loads past the last word or at an unaligned address).
+__device__ __forceinline__ void word_in_chunk(const word_t* w, uint32_t n, uint32_t j, uint32_t c, uint32_t g,
+                                              uint32_t& wi, uint32_t& ws) {
+    uint32_t pos = c, i = j * CHUNK;
+    wi = i; ws = c;
+    for (int piece = 0; piece < CHUNK / 8; ++piece, i += 8u) {
+        word_t x[8];
+        if (i + 8u <= n && (reinterpret_cast<uintptr_t>(w + i) & 15u) == 0) {
+            const uint4* w4 = reinterpret_cast<const uint4*>(w + i);
+            const uint4 u = w4[0], v = w4[1];
+            x[0] = u.x; x[1] = u.y; x[2] = u.z; x[3] = u.w; x[4] = v.x; x[5] = v.y; x[6] = v.z; x[7] = v.w;
+        } else {
+            #pragma unroll
+            for (int q = 0; q < 8; ++q) x[q] = w[min(i + (uint32_t)q, n - 1u)];
+        }
+        bool found = false;
+        #pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const uint32_t l = i + (uint32_t)q < n ? word_len(x[q]) : 0u;
+            const bool here = !found && l && pos + l > g;
+            wi = here ? i + (uint32_t)q : wi;
+            ws = here ? pos : ws;
+            found = found || here;
+            pos += l;
+        }
+        if (found) break;
+    }
+}
+
+__global__ __launch_bounds__(TPB) void k_c2_fix(const word_t* __restrict__ a, const uint32_t* __restrict__ cka,
+                                                uint32_t na, const word_t* __restrict__ b,
+                                                const uint32_t* __restrict__ ckb, uint32_t nb, word_t* out,
+                                                const uint32_t* __restrict__ outck, const uint32_t* ctl, int op) {
+    const uint32_t n = ctl[1];
+    const uint32_t lane = lane_id();
+    const uint32_t wbase = (blockIdx.x * (TPB / WAVE) + threadIdx.x / WAVE) * FIX_WORDS;
+    if (wbase >= n) return;                                   // uniform per wave
+    const uint32_t i0 = wbase + 8u * lane;
+    word_t w[8];
+    uint32_t sum = 0, cand = 0;
+    #pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        w[k] = i0 + k < n ? out[i0 + k] : 0u;
+        const uint32_t l = i0 + k < n ? word_len(w[k]) : 0u;
+        sum += l;
+        if (i0 + k < n && (w[k] == 0u || w[k] == ALLONES)) cand |= 1u << k;
+    }
+    if (!ballot64(cand != 0)) return;
+    const uint32_t incl = wave_incl_sum(sum);
+    const uint32_t P0 = outck[wbase / CHUNK];
+    const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)incl, WAVE - 1);
+    // chunks of A and B that hold the wave's groups [P0, P0 + tot)
+    const uint32_t nca = (na + CHUNK - 1) / CHUNK, ncb = (nb + CHUNK - 1) / CHUNK;
+    const uint32_t glast = P0 + tot - 1u;
+    const uint32_t ca0 = wave_chunk_of(cka, nca, P0), ca1 = min(wave_chunk_of(cka, nca, glast) + 1u, nca);
+    const uint32_t cb0 = wave_chunk_of(ckb, ncb, P0), cb1 = min(wave_chunk_of(ckb, ncb, glast) + 1u, ncb);
+    const GlobReader ra{a}, rb{b};
+    uint32_t pk[8];                                           // first group of each word
+    uint32_t pos = P0 + incl - sum;
+    #pragma unroll
+    for (int k = 0; k < 8; ++k) { pk[k] = pos; pos += word_len(w[k]); }
+    // one candidate at a time (a select, not an indexed load, picks its
+    // group and word), so the walk is not repeated per unrolled slot
+    while (cand) {
+        const uint32_t k = (uint32_t)__ffs(cand) - 1u;
+        cand &= cand - 1u;
+        uint32_t g = pk[0];
+        word_t wk = w[0];
+        #pragma unroll
+        for (int q = 1; q < 8; ++q) {
+            g = k == (uint32_t)q ? pk[q] : g;
+            wk = k == (uint32_t)q ? w[q] : wk;
+        }
+        // both operands' chunks, then the words in them
+        const uint32_t ja = lane_chunk_of(cka, ca0, ca1, g), jb = lane_chunk_of(ckb, cb0, cb1, g);
+        uint32_t ia, pa, ib, pb;
+        word_in_chunk(a, na, ja, cka[ja], g, ia, pa);
+        word_in_chunk(b, nb, jb, ckb[jb], g, ib, pb);
+        if (c2_rep_at(&ra, &rb, ia, pa, ib, pb, g, op)) out[i0 + k] = (wk == ALLONES ? HEADER1 : HEADER0) | 1u;
+    }
+}
+
+hipError_t launch_c2_fix(const word_t* a, const uint32_t* cka, uint32_t na, const word_t* b, const uint32_t* ckb,
+                         uint32_t nb, word_t* out, const uint32_t* outck, const uint32_t* ctl, uint32_t maxwords,
+                         int op, hipStream_t s) {
+    const uint32_t waves = cdiv(maxwords, FIX_WORDS);
+    if (!waves) return hipSuccess;
+    hipLaunchKernelGGL(k_c2_fix, dim3(cdiv(waves, TPB / WAVE)), dim3(TPB), 0, s, a, cka, na, b, ckb, nb, out, outck,
+                       ctl, op);
+    return hipGetLastError();
+}
+
 hipError_t launch_edge_words(const word_t* out, uint32_t* ctl, hipStream_t s) {
     hipLaunchKernelGGL(k_edge_words, dim3(1), dim3(64), 0, s, out, ctl);
     return hipGetLastError();

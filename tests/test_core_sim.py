@@ -144,3 +144,63 @@ def test_sim_raw_c2_words(sim, orc):
             words = split_serial(got)[0]
             lens = np.where(words > 0x7FFFFFFF, words & 0x3FFFFFFF, 1)
             assert np.array_equal(heads, np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.uint32))
+
+
+def test_sim_c2_canonical_with_representations(sim, orc):
+    """The *_c2 result from the group domain (canonical words of the op's
+    values, a single all-0 / all-1 group written as a count-1 fill where
+    c2_rep_at walks to a fill token from the last synchronisation point)
+    equals the oracle's *_c2 words, on operands with count-1 fills and
+    literal 0 / all-ones words inside runs of fills."""
+    sim.fbsim_c2_canon_rep.restype = C.c_long
+    rng = np.random.default_rng(23)
+    for t in range(160):
+        a, b, n = random_pair(rng, orc)
+        if t % 3 == 0:
+            a = orc.op(a, orc.from_bits(markov_bits(n, 0.1, 7, rng)), 1, compress=False)[0]
+            a = orc.compress(a) if t % 2 else a
+        if t % 4 == 1:
+            b = orc.op(b, orc.from_bits(markov_bits(n, 0.3, 3, rng)), 0, compress=False)[0]
+        wa, av, an = split_serial(a)
+        wb, bv, bn = split_serial(b)
+        g = groups_of(wa)
+        wa = np.ascontiguousarray(wa)
+        wb = np.ascontiguousarray(wb)
+        for op in range(4):
+            cap = g + 8
+            out = np.zeros(cap, np.uint32)
+            k = sim.fbsim_c2_canon_rep(wa.ctypes.data_as(u32p), len(wa), wb.ctypes.data_as(u32p), len(wb), g,
+                                       op, out.ctypes.data_as(u32p), cap)
+            assert k >= 0, k
+            act = [av & bv, av | bv, av ^ bv, av & ~bv & 0xFFFFFFFF][op]
+            got = np.array(list(out[:k]) + ([act] if an else []) + [an], np.uint32)
+            assert np.array_equal(got, c2_oracle(orc, a, b, op)), (t, op)
+
+
+@pytest.mark.parametrize("d,run", [(0.01, 64), (0.01, 1), (0.1, 8), (0.5, 64)])
+def test_sim_c2_representations_markov(sim, orc, d, run):
+    """The same on 2^20-bit Markov operands, where the *_c2 loops emit
+    count-1 fills (checked to occur in the expected words)."""
+    sim.fbsim_c2_canon_rep.restype = C.c_long
+    rng = np.random.default_rng(int(d * 1000) + run)
+    n = 1 << 20
+    a = orc.from_bits(markov_bits(n, d, run, rng))
+    b = orc.from_bits(markov_bits(n, d, run, rng))
+    wa, av, an = split_serial(a)
+    wb, bv, bn = split_serial(b)
+    g = groups_of(wa)
+    wa = np.ascontiguousarray(wa)
+    wb = np.ascontiguousarray(wb)
+    ones1 = 0
+    for op in range(4):
+        want = c2_oracle(orc, a, b, op)
+        ww = split_serial(want)[0]
+        ones1 += int(np.sum((ww == 0x80000001) | (ww == 0xC0000001)))
+        out = np.zeros(g + 8, np.uint32)
+        k = sim.fbsim_c2_canon_rep(wa.ctypes.data_as(u32p), len(wa), wb.ctypes.data_as(u32p), len(wb), g, op,
+                                   out.ctypes.data_as(u32p), g + 8)
+        act = [av & bv, av | bv, av ^ bv, av & ~bv & 0xFFFFFFFF][op]
+        got = np.array(list(out[:k]) + ([act] if an else []) + [an], np.uint32)
+        assert np.array_equal(got, want), op
+    if (d, run) == (0.01, 64):
+        assert ones1 > 0
