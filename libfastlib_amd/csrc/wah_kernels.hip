@@ -727,12 +727,13 @@ __global__ void k_edge_words(const word_t* out, uint32_t* ctl) {
 // canonical words of a OP b; they are the *_c2 routines' words except that
 // a result group forming a run of its own (a literal 0 / all-ones word) is a
 // count-1 fill where the routine's token there is a fill (c2_rep_at).  Wave
-// w fixes result words [512w, 512w + 512), lane l the eight from 512w + 8l:
+// w fixes result words [256w, 256w + 256), lane l the four from 256w + 4l:
 // their first groups from the result's chunk index and a wave prefix sum;
 // the chunks of A and B that hold the wave's groups by 64-way searches (both
 // ends, both operands, at once); then each lane's candidate words by a
 // binary search within those chunks, one chunk's words, and the walk.
-constexpr uint32_t FIX_WORDS = 512;
+constexpr uint32_t FIX_WORDS = 256;
+constexpr int FIX_PER = (int)FIX_WORDS / WAVE;          // result words per lane
 
 // The chunk j of an operand with ck[j] <= g < ck[j + 1] by one wave.
 __device__ __forceinline__ uint32_t wave_chunk_of(const uint32_t* ck, uint32_t nc, uint32_t g) {
@@ -780,19 +781,66 @@ __device__ __forceinline__ void word_in_chunk(const word_t* w, uint32_t n, uint3
     }
 }
 
+// The words of chunks [c0, c1) of an operand staged in a wave's LDS rows
+// (sw: words, sp: their first groups, then the end), when they number at
+// most FIX_WIN; returns the count staged (0: too many, read from HBM).
+// Lane l takes words 8l .. 8l + 7; all 64 lanes call it.
+constexpr uint32_t FIX_WIN = 512;
+
+__device__ __forceinline__ uint32_t stage_chunks(const word_t* w, const uint32_t* ck, uint32_t n, uint32_t c0,
+                                                 uint32_t c1, word_t* sw, uint32_t* sp) {
+    const uint32_t base = c0 * CHUNK, cnt = min(c1 * CHUNK, n) - base;
+    if (cnt > FIX_WIN) return 0;
+    const uint32_t lane = lane_id();
+    word_t x[8];
+    uint32_t len[8], sum = 0;
+    #pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t j = 8u * lane + (uint32_t)k;
+        x[k] = j < cnt ? w[base + j] : 0u;
+        len[k] = j < cnt ? word_len(x[k]) : 0u;
+        sum += len[k];
+    }
+    const uint32_t incl = wave_incl_sum(sum);
+    uint32_t pos = ck[c0] + incl - sum;
+    #pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t j = 8u * lane + (uint32_t)k;
+        if (j < cnt) { sw[j] = x[k]; sp[j] = pos; }
+        pos += len[k];
+    }
+    if (lane == WAVE - 1) sp[cnt] = pos;
+    __builtin_amdgcn_wave_barrier();
+    return cnt;
+}
+
+// The staged word holding group g (sp[0] <= g < sp[cnt]) and its first group.
+__device__ __forceinline__ void staged_word_of(const uint32_t* sp, uint32_t cnt, uint32_t base, uint32_t g,
+                                               uint32_t& wi, uint32_t& ws) {
+    uint32_t lo = 0, hi = cnt;                 // sp[lo] <= g < sp[hi]
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (sp[mid] <= g) lo = mid; else hi = mid;
+    }
+    wi = base + lo; ws = sp[lo];
+}
+
 __global__ __launch_bounds__(TPB) void k_c2_fix(const word_t* __restrict__ a, const uint32_t* __restrict__ cka,
                                                 uint32_t na, const word_t* __restrict__ b,
                                                 const uint32_t* __restrict__ ckb, uint32_t nb, word_t* out,
                                                 const uint32_t* __restrict__ outck, const uint32_t* ctl, int op) {
+    __shared__ word_t s_w[TPB / WAVE][2][FIX_WIN];
+    __shared__ uint32_t s_p[TPB / WAVE][2][FIX_WIN + 1];
     const uint32_t n = ctl[1];
     const uint32_t lane = lane_id();
-    const uint32_t wbase = (blockIdx.x * (TPB / WAVE) + threadIdx.x / WAVE) * FIX_WORDS;
+    const uint32_t wv = threadIdx.x / WAVE;
+    const uint32_t wbase = (blockIdx.x * (TPB / WAVE) + wv) * FIX_WORDS;
     if (wbase >= n) return;                                   // uniform per wave
-    const uint32_t i0 = wbase + 8u * lane;
-    word_t w[8];
+    const uint32_t i0 = wbase + (uint32_t)FIX_PER * lane;
+    word_t w[FIX_PER];
     uint32_t sum = 0, cand = 0;
     #pragma unroll
-    for (int k = 0; k < 8; ++k) {
+    for (int k = 0; k < FIX_PER; ++k) {
         w[k] = i0 + k < n ? out[i0 + k] : 0u;
         const uint32_t l = i0 + k < n ? word_len(w[k]) : 0u;
         sum += l;
@@ -802,16 +850,19 @@ __global__ __launch_bounds__(TPB) void k_c2_fix(const word_t* __restrict__ a, co
     const uint32_t incl = wave_incl_sum(sum);
     const uint32_t P0 = outck[wbase / CHUNK];
     const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)incl, WAVE - 1);
-    // chunks of A and B that hold the wave's groups [P0, P0 + tot)
+    // chunks of A and B that hold the wave's groups [P0, P0 + tot), staged
+    // in LDS when they are few (dense results); else read from HBM
     const uint32_t nca = (na + CHUNK - 1) / CHUNK, ncb = (nb + CHUNK - 1) / CHUNK;
     const uint32_t glast = P0 + tot - 1u;
     const uint32_t ca0 = wave_chunk_of(cka, nca, P0), ca1 = min(wave_chunk_of(cka, nca, glast) + 1u, nca);
     const uint32_t cb0 = wave_chunk_of(ckb, ncb, P0), cb1 = min(wave_chunk_of(ckb, ncb, glast) + 1u, ncb);
-    const GlobReader ra{a}, rb{b};
-    uint32_t pk[8];                                           // first group of each word
+    const uint32_t sa = stage_chunks(a, cka, na, ca0, ca1, s_w[wv][0], s_p[wv][0]);
+    const uint32_t sb = stage_chunks(b, ckb, nb, cb0, cb1, s_w[wv][1], s_p[wv][1]);
+    const WinReader ra{s_w[wv][0], a, ca0 * CHUNK, sa}, rb{s_w[wv][1], b, cb0 * CHUNK, sb};
+    uint32_t pk[FIX_PER];                                     // first group of each word
     uint32_t pos = P0 + incl - sum;
     #pragma unroll
-    for (int k = 0; k < 8; ++k) { pk[k] = pos; pos += word_len(w[k]); }
+    for (int k = 0; k < FIX_PER; ++k) { pk[k] = pos; pos += word_len(w[k]); }
     // one candidate at a time (a select, not an indexed load, picks its
     // group and word), so the walk is not repeated per unrolled slot
     while (cand) {
@@ -820,15 +871,15 @@ __global__ __launch_bounds__(TPB) void k_c2_fix(const word_t* __restrict__ a, co
         uint32_t g = pk[0];
         word_t wk = w[0];
         #pragma unroll
-        for (int q = 1; q < 8; ++q) {
+        for (int q = 1; q < FIX_PER; ++q) {
             g = k == (uint32_t)q ? pk[q] : g;
             wk = k == (uint32_t)q ? w[q] : wk;
         }
-        // both operands' chunks, then the words in them
-        const uint32_t ja = lane_chunk_of(cka, ca0, ca1, g), jb = lane_chunk_of(ckb, cb0, cb1, g);
         uint32_t ia, pa, ib, pb;
-        word_in_chunk(a, na, ja, cka[ja], g, ia, pa);
-        word_in_chunk(b, nb, jb, ckb[jb], g, ib, pb);
+        if (sa) staged_word_of(s_p[wv][0], sa, ca0 * CHUNK, g, ia, pa);
+        else word_in_chunk(a, na, lane_chunk_of(cka, ca0, ca1, g), cka[lane_chunk_of(cka, ca0, ca1, g)], g, ia, pa);
+        if (sb) staged_word_of(s_p[wv][1], sb, cb0 * CHUNK, g, ib, pb);
+        else word_in_chunk(b, nb, lane_chunk_of(ckb, cb0, cb1, g), ckb[lane_chunk_of(ckb, cb0, cb1, g)], g, ib, pb);
         if (c2_rep_at(&ra, &rb, ia, pa, ib, pb, g, op)) out[i0 + k] = (wk == ALLONES ? HEADER1 : HEADER0) | 1u;
     }
 }
