@@ -257,6 +257,12 @@ class OracleIndex:
         return cls(L.orc_bin_bounds(v.ctypes.data_as(_dp), C.c_uint32(v.size), b.ctypes.data_as(_dp),
                                     C.c_uint32(b.size)), col)
 
+    def as_file(self):
+        """Marks the index as one read from a file: its bitmaps are measured
+        by their serialized sizes (the file's positions)."""
+        oracle().lib.orc_index_set_offsets_built(C.c_void_p(self._h), C.c_int(1))
+        return self
+
     def __del__(self):
         try:
             oracle().lib.orc_index_free(C.c_void_p(self._h))
@@ -648,3 +654,144 @@ def expr_rows(masks, prog):
                        -5: lambda: ~(a | b), -7: lambda: a & ~b}[op]())
     assert len(st) == 1
     return st[0]
+
+
+# ---------------------------------------------------------------------------
+# Two-level encodings over a relic (ibis::fuzz, ibis::zona;
+# src/irelic.h:540-712) restated in oracle/index.cpp.
+ENC_FUZZ, ENC_ZONA = 1, 2
+IDX_FUZZ, IDX_ZONA = 24, 25          # ibis::index::INDEX_TYPE (src/index.h:139-143)
+TL_EMPTY, TL_FINE, TL_ALL, TL_EQUAL, TL_SAME_BIN = 0, 1, 6, 7, 8
+
+
+def two_level_file_parts(path_or_bytes):
+    """The parts of a fuzz / zona index file (src/ixfuzz.cpp:46-143,
+    src/ixzona.cpp:40-100): (encoding, fine positions, nc, cbounds, coarse
+    positions, coarse words, word offsets of the coarse bitmaps)."""
+    import struct
+    b = path_or_bytes if isinstance(path_or_bytes, (bytes, bytearray)) else open(path_or_bytes, "rb").read()
+    if b[5] not in (IDX_FUZZ, IDX_ZONA):
+        raise ValueError("not a fuzz / zona index file")
+    w = b[6]
+    it = np.int32 if w == 4 else np.int64
+    nobs = struct.unpack_from("<I", b, 12)[0]
+    fine = np.frombuffer(b, it, nobs + 1, 24 + 8 * nobs).astype(np.int64)
+    p = int(fine[-1])
+    nc = struct.unpack_from("<I", b, p)[0]
+    cb = np.frombuffer(b, np.uint32, nc + 1, p + 4).copy()
+    ncb = nc - (nc + 1) // 2 + 1 if b[5] == IDX_FUZZ else nc
+    co = np.frombuffer(b, it, ncb + 1, p + 4 + 4 * (nc + 1)).astype(np.int64)
+    words = np.frombuffer(b[int(co[0]):int(co[-1])], np.uint32).copy()
+    enc = ENC_FUZZ if b[5] == IDX_FUZZ else ENC_ZONA
+    return enc, fine, nc, cb, co, words, (co - co[0]) // 4
+
+
+def two_level_widen(src, dst):
+    """The same fuzz / zona index file with 8-byte bitmap positions: the relic
+    part as relic::write64 lays it out (src/irelic.cpp:456-642), then nc,
+    cbounds, 8-byte coarse positions and the coarse bitmaps
+    (src/ixfuzz.cpp:1272-1331)."""
+    import struct
+    b = open(src, "rb").read()
+    assert b[6] == 4
+    nobs = struct.unpack_from("<I", b, 12)[0]
+    obeg = 24 + 8 * nobs
+    pos = np.frombuffer(b, np.int32, nobs + 1, obeg).astype(np.int64)
+    p = int(pos[-1])
+    nc = struct.unpack_from("<I", b, p)[0]
+    ncb = nc - (nc + 1) // 2 + 1 if b[5] == IDX_FUZZ else nc
+    cpos_at = p + 4 + 4 * (nc + 1)
+    co = np.frombuffer(b, np.int32, ncb + 1, cpos_at).astype(np.int64)
+    s1 = 4 * (nobs + 1)                 # growth of the fine positions
+    s2 = s1 + 4 * (ncb + 1)             # growth up to the coarse bitmaps
+    head = bytearray(b[:obeg])
+    head[6] = 8
+    out = (bytes(head) + (pos + s1).astype("<i8").tobytes() + b[obeg + s1:p]
+           + b[p:cpos_at] + (co + s2).astype("<i8").tobytes() + b[int(co[0]):])
+    with open(dst, "wb") as f:
+        f.write(out)
+
+
+class OracleTwoLevel:
+    """The coarse level of ibis::fuzz / ibis::zona over an OracleIndex relic."""
+
+    def __init__(self, fine, handle):
+        self.fine = fine
+        self._h = handle
+        if not handle:
+            raise RuntimeError("two-level oracle failed")
+
+    @classmethod
+    def build(cls, fine, enc, ncoarse=0):
+        """coarsen() on the index built in memory."""
+        L = oracle().lib
+        L.orc_two_level_build.restype = C.c_void_p
+        return cls(fine, L.orc_two_level_build(C.c_void_p(fine._h), C.c_int(enc), C.c_uint32(ncoarse)))
+
+    @classmethod
+    def from_file(cls, fine, path):
+        """The coarse level as the index file at path holds it; `fine` is the
+        relic over the same column with file offsets (offsets_built)."""
+        L = oracle().lib
+        L.orc_two_level_file.restype = C.c_void_p
+        enc, fp, nc, cb, co, words, woff = two_level_file_parts(path)
+        fp = np.ascontiguousarray(fp, np.int64)
+        cb = np.ascontiguousarray(cb, np.uint32)
+        co = np.ascontiguousarray(co, np.int64)
+        words = np.ascontiguousarray(words, np.uint32)
+        woff = np.ascontiguousarray(woff, np.int64)
+        h = L.orc_two_level_file(C.c_void_p(fine._h), C.c_int(enc), fp.ctypes.data_as(_i64p),
+                                 C.c_uint32(fp.size - 1), cb.ctypes.data_as(_u32p), C.c_uint32(nc),
+                                 co.ctypes.data_as(_i64p), C.c_uint32(co.size - 1),
+                                 words.ctypes.data_as(_u32p), woff.ctypes.data_as(_i64p))
+        return cls(fine, h)
+
+    def __del__(self):
+        try:
+            oracle().lib.orc_two_level_free(C.c_void_p(self._h))
+        except Exception:
+            pass
+
+    def info(self):
+        """(cbounds, coarse offsets, fine offsets) in bytes."""
+        L = oracle().lib
+        a, b, c = _u32p(), _i64p(), _i64p()
+        na, nb, nc = C.c_size_t(), C.c_size_t(), C.c_size_t()
+        L.orc_two_level_info(C.c_void_p(self._h), C.byref(a), C.byref(na), C.byref(b), C.byref(nb), C.byref(c),
+                             C.byref(nc))
+        out = (np.ctypeslib.as_array(a, (na.value,)).copy() if na.value else np.zeros(0, np.uint32),
+               np.ctypeslib.as_array(b, (nb.value,)).copy() if nb.value else np.zeros(0, np.int64),
+               np.ctypeslib.as_array(c, (nc.value,)).copy() if nc.value else np.zeros(0, np.int64))
+        for q in (a, b, c):
+            L.orc_free(q)
+        return out
+
+    def coarse_bitmap(self, i):
+        """Serialized words of coarse bitmap i."""
+        L = oracle().lib
+        p, n = _u32p(), C.c_size_t()
+        if L.orc_two_level_cbit(C.c_void_p(self._h), C.c_uint32(i), C.byref(p), C.byref(n)) != 0:
+            raise IndexError(i)
+        out = np.ctypeslib.as_array(p, (n.value,)).copy()
+        L.orc_free(p)
+        return out
+
+    def cost(self, lo, lop, hi, rop):
+        L = oracle().lib
+        L.orc_two_level_cost.restype = C.c_double
+        return L.orc_two_level_cost(C.c_void_p(self.fine._h), C.c_void_p(self._h), self.fine.type, C.c_double(lo),
+                                    C.c_int(lop), C.c_double(hi), C.c_int(rop))
+
+    def evaluate(self, lo, lop, hi, rop):
+        """(hit count, ascending rids, option, coarse bitmaps read as a mask)."""
+        L = oracle().lib
+        cnt, opt, ld = C.c_uint64(), C.c_int(), C.c_uint64()
+        r, nr = _u32p(), C.c_size_t()
+        rc = L.orc_two_level_eval(C.c_void_p(self.fine._h), C.c_void_p(self._h), C.c_double(lo), C.c_int(lop),
+                                  C.c_double(hi), C.c_int(rop), C.byref(cnt), C.byref(r), C.byref(nr),
+                                  C.byref(opt), C.byref(ld))
+        if rc != 0:
+            raise RuntimeError("orc_two_level_eval failed")
+        rids = np.ctypeslib.as_array(r, (nr.value,)).copy() if nr.value else np.zeros(0, np.uint32)
+        L.orc_free(r)
+        return int(cnt.value), rids, int(opt.value), int(ld.value)

@@ -536,4 +536,453 @@ BV evaluate_plan(const Index& ix, const void* col, int type, double lo, int lop,
     return evaluate(ix, col, type, lo, lop, hi, rop);
 }
 
+// ---------------------------------------------------------------------------
+// Two-level encodings (ibis::fuzz, ibis::zona).
+
+// array_t<T>::find (src/array_t.cpp:507-528): the first position whose
+// value is not less than v.
+template <class T>
+static uint32_t find_first(const std::vector<T>& a, T v) {
+    return (uint32_t)(std::lower_bound(a.begin(), a.end(), v) - a.begin());
+}
+
+// The boundaries of the coarse bins: fine bitmaps split into groups of
+// nearly equal offset64 bytes (src/ixfuzz.cpp:215-231, src/ixzona.cpp:
+// 146-161; fuzz keeps the target in 32 bits).
+static std::vector<uint32_t> coarse_bounds(const std::vector<int64_t>& off, uint32_t ncoarse, bool i32) {
+    const uint32_t nbits = (uint32_t)off.size() - 1;
+    std::vector<uint32_t> cb(ncoarse + 1);
+    cb[0] = 0;
+    for (uint32_t i = 1; i < ncoarse; ++i) {
+        int64_t target = off[cb[i - 1]] + (off.back() - off[cb[i - 1]]) / (int64_t)(ncoarse - i + 1);
+        if (i32) target = (int32_t)target;
+        cb[i] = find_first(off, target);
+        if (cb[i] > cb[i - 1] + 1 && off[cb[i]] - target > target - off[cb[i] - 1]) --cb[i];
+        else if (cb[i] <= cb[i - 1]) cb[i] = cb[i - 1] + 1;
+    }
+    cb[ncoarse] = nbits;
+    for (uint32_t i = ncoarse - 1; i > 0 && cb[i + 1] < cb[i]; --i) cb[i] = cb[i + 1] - 1;
+    return cb;
+}
+
+// OR of fine bitmaps [a, b) as a compressed bitvector of nrows bits.
+static BV or_range(const Index& ix, uint32_t a, uint32_t b) {
+    BV r;
+    r.set(0, ix.nrows);
+    for (uint32_t i = a; i < b && i < ix.bits.size(); ++i)
+        if (ix.bits[i]) { r.op_inplace(*ix.bits[i], OP_OR); r.compress(); }
+    return r;
+}
+
+TwoLevel* coarsen(const Index& ix, int enc, uint32_t ncoarse) {
+    TwoLevel* tl = new TwoLevel();
+    tl->enc = enc;
+    const uint32_t nbits = (uint32_t)ix.bits.size();
+    // offset64 (src/ixfuzz.cpp:166-180): the file's positions where the
+    // index has them, else bitvector::bytes() of every bitmap
+    tl->off.assign(nbits + 1, 0);
+    for (uint32_t i = 0; i < nbits; ++i)
+        tl->off[i + 1] = tl->off[i] + (int64_t)(ix.offsets_built ? serial_bytes(ix, i) : bm_bytes(ix.bits[i]));
+    if (ix.vals.size() < 32) return tl;              // no coarse level for fewer than 32 values
+    const uint32_t nrows = ix.nrows;
+    if (enc == ENC_FUZZ) {
+        // src/ixfuzz.cpp:182-208: sf = fine-level words; the default count
+        // minimises the product of index size and query cost
+        if (ncoarse < 5u && tl->off.back() > tl->off[0] + (int64_t)(nrows / 31u)) {
+            const int wm1 = 31;
+            const long sf = (long)((tl->off.back() - tl->off[0]) / 4);
+            unsigned nc = (unsigned)(wm1 * sf / (std::sqrt(2.0) * nrows));
+            const unsigned ncmax = (unsigned)std::sqrt(2.0 * ix.vals.size());
+            if (nc < ncmax) {
+                const double obj1 = (sf + (nc + 1 - std::ceil(0.5 * nc)) * nrows / wm1) *
+                                    (sf * 0.5 / nc + 2.0 * nrows / wm1);
+                const double obj2 = (sf + (nc + 2 - std::ceil(0.5 * nc + 0.5)) * nrows / wm1) *
+                                    (sf * 0.5 / (nc + 1.0) + 2.0 * nrows / wm1);
+                nc += (obj2 < obj1);
+            } else {
+                nc = ncmax;
+            }
+            ncoarse = nc;
+        }
+        if (ncoarse < 5 || ncoarse >= ix.vals.size()) return tl;
+    } else {
+        ncoarse = 11;                                 // src/ixzona.cpp:128
+    }
+    tl->cbounds = coarse_bounds(tl->off, ncoarse, enc == ENC_FUZZ);
+    const std::vector<uint32_t>& cb = tl->cbounds;
+    if (enc == ENC_FUZZ) {
+        // interval encoding (src/ixfuzz.cpp:242-253): bitmap i covers the
+        // nc2 coarse bins from i
+        const uint32_t nc2 = (ncoarse + 1) / 2, ncb = ncoarse - nc2 + 1;
+        for (uint32_t i = 0; i < ncb; ++i) tl->cbits.push_back(new BV(or_range(ix, cb[i], cb[i + nc2])));
+    } else {
+        for (uint32_t i = 0; i < ncoarse; ++i) tl->cbits.push_back(new BV(or_range(ix, cb[i], cb[i + 1])));
+    }
+    tl->coff.assign(tl->cbits.size() + 1, 0);
+    for (size_t i = 0; i < tl->cbits.size(); ++i) {
+        tl->cbits[i]->compress();
+        tl->coff[i + 1] = tl->coff[i] + (int64_t)bm_bytes(tl->cbits[i]);
+    }
+    return tl;
+}
+
+TwoLevel* two_level_from_file(const Index& ix, int enc, const std::vector<int64_t>& fine_pos,
+                              const std::vector<uint32_t>& cbounds, const std::vector<int64_t>& coff,
+                              const std::vector<std::vector<word_t>>& cbits_serial) {
+    (void)ix;
+    TwoLevel* tl = new TwoLevel();
+    tl->enc = enc;
+    tl->off = fine_pos;
+    tl->cbounds = cbounds;
+    tl->coff = coff;
+    for (const auto& w : cbits_serial) tl->cbits.push_back(new BV(BV::from_serial(w.data(), w.size())));
+    return tl;
+}
+
+namespace {
+
+// One evaluation: the fine level, the coarse level, and the bitmaps read.
+struct TLEval {
+    const Index& ix;
+    const TwoLevel& tl;
+    uint64_t loads = 0;
+    const std::vector<int64_t>& off;
+    const std::vector<int64_t>& coff;
+    const std::vector<uint32_t>& cb;
+    TLEval(const Index& i, const TwoLevel& t) : ix(i), tl(t), off(t.off), coff(t.coff), cb(t.cbounds) {}
+    uint32_t nobs() const { return (uint32_t)ix.bits.size(); }
+    BV none() const { BV r; r.set(0, ix.nrows); return r; }
+    BV all() const { BV r; r.set(1, ix.nrows); return r; }
+    const BV* coarse(uint32_t i) {
+        if (i >= tl.cbits.size()) return nullptr;
+        loads |= 1ull << i;
+        return tl.cbits[i];
+    }
+    void load_range(uint32_t i, uint32_t j) {
+        for (; i < j && i < tl.cbits.size(); ++i) loads |= 1ull << i;
+    }
+    // the fine level's cost of [h0, h1): the smaller of the range and its
+    // complement (src/ixfuzz.cpp:793-804)
+    long fine_cost(uint32_t h0, uint32_t h1) const {
+        const long in = (long)(off[h1] - off[h0]);
+        const long out = (long)((off.back() - off[h1]) + (off[h0] - off[0]));
+        return in <= out ? in : out;
+    }
+    // index::addBins (src/index.cpp:5537-5580): OR of [ib, ie) into res;
+    // the whole fine level sets every row
+    void add_bins(uint32_t ib, uint32_t ie, BV& res) const {
+        if (ie > nobs()) ie = nobs();
+        if (ib >= ie) return;
+        if (ib == 0 && ie == nobs()) { res = all(); return; }
+        BV r = or_range(ix, ib, ie);
+        res.op_inplace(r, OP_OR);
+        res.compress();
+    }
+    void minus(BV& res, const BV& b) const { res.op_inplace(b, OP_MINUS); res.compress(); }
+    void andw(BV& res, const BV& b) const { res.op_inplace(b, OP_AND); res.compress(); }
+    void orw(BV& res, const BV& b) const { res.op_inplace(b, OP_OR); res.compress(); }
+
+    // fuzz::coarseEstimate (src/ixfuzz.cpp:626-674): bytes of the coarse
+    // bitmaps coarseEvaluate reads for coarse bins [lo, hi)
+    long fuzz_coarse_cost(uint32_t lo, uint32_t hi) const {
+        const uint32_t mid = (uint32_t)cb.size() / 2;
+        auto sz = [&](uint32_t i) { return (long)(coff[i + 1] - coff[i]); };
+        if (lo >= cb.size() || lo >= hi) return 0;
+        if (hi > mid) {
+            long c = sz(hi - mid);
+            if (lo > hi - mid) c += lo >= mid ? sz(lo - mid) : sz(lo);
+            else if (lo < hi - mid) c += sz(lo);
+            return c;
+        }
+        if (hi < mid) return sz(lo) + sz(hi);
+        return sz(0) + (lo > 0 ? sz(lo) : 0);
+    }
+    // fuzz::coarseEvaluate (src/ixfuzz.cpp:676-765): rows of coarse bins
+    // [lo, hi) from at most two interval bitmaps
+    BV fuzz_coarse(uint32_t lo, uint32_t hi) {
+        const uint32_t mid = (uint32_t)cb.size() / 2;
+        const uint32_t ncb = (uint32_t)tl.cbits.size();
+        if (lo >= cb.size() || lo >= hi) return none();
+        if (lo + 1 == hi) {
+            const uint32_t a = hi < ncb ? lo : hi - mid, b = hi < ncb ? hi : lo - mid;
+            load_range(std::min(a, b), std::min(a, b) + 2);
+            const BV* x = tl.cbits[a];
+            if (!x) return none();
+            BV r = *x;
+            if (tl.cbits[b]) minus(r, *tl.cbits[b]);
+            return r;
+        }
+        if (hi > mid) {
+            const BV* x = coarse(hi - mid);
+            BV r = x ? *x : none();
+            if (lo > hi - mid) {
+                if (lo >= mid) {
+                    const BV* y = coarse(lo - mid);
+                    if (y) minus(r, *y);
+                } else {
+                    const BV* y = coarse(lo);
+                    if (y) andw(r, *y); else r = none();
+                }
+            } else if (lo < hi - mid) {
+                const BV* y = coarse(lo);
+                if (y) orw(r, *y);
+            }
+            return r;
+        }
+        if (hi < mid) {
+            const BV* x = coarse(lo);
+            if (!x) return none();
+            BV r = *x;
+            const BV* y = coarse(hi);
+            if (y) minus(r, *y);
+            return r;
+        }
+        const BV* x = coarse(0);
+        BV r = x ? *x : none();
+        if (lo > 0) {
+            const BV* y = coarse(lo);
+            if (y) andw(r, *y);
+        }
+        return r;
+    }
+    // zona: OR of coarse bitmaps [a, b) (index::addBits) read straight, or
+    // the rows outside [a, b) complemented, whichever is smaller
+    // (src/ixzona.cpp:889-913)
+    BV zona_coarse(uint32_t a, uint32_t b, uint32_t out_lo_end, uint32_t out_hi_begin, uint32_t ncoarse,
+                   bool lo_cond, bool hi_cond) {
+        const long csize = (long)(coff.back() - coff[0]);
+        const long tmp = (long)(coff[b] - coff[a]);
+        BV r;
+        if (tmp + tmp <= csize) {
+            r = none();
+            load_range(a, b);
+            for (uint32_t i = a; i < b; ++i)
+                if (tl.cbits[i]) orw(r, *tl.cbits[i]);
+        } else {
+            BV bv = none();
+            if (lo_cond) {
+                load_range(0, out_lo_end);
+                for (uint32_t i = 0; i < out_lo_end; ++i)
+                    if (tl.cbits[i]) orw(bv, *tl.cbits[i]);
+            }
+            if (hi_cond) {
+                load_range(out_hi_begin, ncoarse);
+                for (uint32_t i = out_hi_begin; i < ncoarse; ++i)
+                    if (tl.cbits[i]) orw(bv, *tl.cbits[i]);
+            }
+            r = all();
+            minus(r, bv);
+        }
+        return r;
+    }
+    long zona_part(uint32_t a, uint32_t b) const {
+        const long csize = (long)(coff.back() - coff[0]);
+        const long t = (long)(coff[b] - coff[a]);
+        return t + t <= csize ? t : csize - t;
+    }
+};
+
+} // namespace
+
+double two_level_cost(const Index& ix, const TwoLevel& tl, int type, double lo, int lop, double hi, int rop) {
+    TLEval E(ix, tl);
+    const uint32_t nobs = E.nobs();
+    const double scan = (double)elem_size(type) * ix.nrows;
+    if (nobs == 0 || tl.off.size() != nobs + 1) return scan;
+    uint32_t h0 = 0, h1 = 0;
+    relic_hits(ix, lo, lop, hi, rop, h0, h1);
+    const auto& off = tl.off;
+    const auto& coff = tl.coff;
+    const auto& cb = tl.cbounds;
+    const bool have_coff = !tl.cbits.empty() && coff.size() == tl.cbits.size() + 1;
+    if (tl.enc == ENC_FUZZ) {
+        if (h1 <= h0 || h0 >= nobs) return 0.0;
+        if (h0 == 0 && h1 >= nobs) return 0.0;
+        const uint32_t ncoarse = cb.empty() ? 0u : (uint32_t)cb.size() - 1;
+        const long fine = E.fine_cost(h0, h1);
+        if (h0 + 3 >= h1 || ncoarse == 0 || !have_coff) return (double)fine;
+        const uint32_t c0 = find_first(cb, h0), c1 = find_first(cb, h1);
+        if (c0 >= c1) {
+            const long tmp = E.fuzz_coarse_cost(c1 - 1, c1) + (long)((off[h0] - off[cb[c1 - 1]]) + (off[cb[c1]] - off[h1]));
+            return (double)(tmp / 100 >= fine / 99 ? fine : tmp);
+        }
+        long cost = E.fuzz_coarse_cost(c0, c1 - 1) + (long)((off[cb[c0]] - off[h0]) + (off[h1] - off[cb[c1 - 1]]));
+        if (c0 > 0)
+            cost = std::min(cost, E.fuzz_coarse_cost(c0 - 1, c1 - 1) +
+                                      (long)((off[h0] - off[cb[c0 - 1]]) + (off[h1] - off[cb[c1 - 1]])));
+        cost = std::min(cost, E.fuzz_coarse_cost(c0, c1) + (long)((off[cb[c0]] - off[h0]) + (off[cb[c1]] - off[h1])));
+        if (c0 > 0)
+            cost = std::min(cost, E.fuzz_coarse_cost(c0 - 1, c1) +
+                                      (long)((off[h0] - off[cb[c0 - 1]]) + (off[cb[c1]] - off[h1])));
+        return (double)(cost / 100 >= fine / 99 ? fine : cost);
+    }
+    // zona
+    if (h1 <= h0) return 0.0;
+    if (h0 == 0 && h1 >= nobs) return 0.0;
+    const long fine = E.fine_cost(h0, h1);
+    const uint32_t ncoarse = (tl.cbits.empty() || cb.empty()) ? 0u
+                             : tl.cbits.size() + 1 <= cb.size() ? (uint32_t)tl.cbits.size() : (uint32_t)cb.size() - 1;
+    if (h0 + 1 == h1) return (double)fine;
+    if (h0 + 3 >= h1 || !have_coff) return (double)fine;
+    const uint32_t c0 = find_first(cb, h0), c1 = find_first(cb, h1);
+    if (c0 >= c1) {
+        const long tmp = (long)(coff[c0] - coff[c0 - 1]) + (long)(off[cb[c0]] - off[cb[c0 - 1]]) - fine;
+        return (double)((long)(0.99 * fine) >= tmp ? tmp : fine);
+    }
+    if (c1 < ncoarse && cb[c1] == h1 && cb[c0] == h0) return (double)E.zona_part(c0, c1);
+    long cost = E.zona_part(c0, c1 - 1) + (long)((off[cb[c0]] - off[h0]) + (off[h1] - off[cb[c1 - 1]]));
+    if (c0 > 0)
+        cost = std::min(cost, E.zona_part(c0 - 1, c1 - 1) + (long)((off[h0] - off[cb[c0 - 1]]) + (off[h1] - off[cb[c1 - 1]])));
+    cost = std::min(cost, E.zona_part(c0, c1) + (long)((off[cb[c0]] - off[h0]) + (off[cb[c1]] - off[h1])));
+    if (c0 > 0)
+        cost = std::min(cost, E.zona_part(c0 - 1, c1) + (long)((off[h0] - off[cb[c0 - 1]]) + (off[cb[c1]] - off[h1])));
+    if (cost > (long)(0.99 * fine)) cost = fine;
+    return (double)cost;
+}
+
+BV two_level_eval(const Index& ix, const TwoLevel& tl, double lo, int lop, double hi, int rop, int* option,
+                  uint64_t* loads) {
+    TLEval E(ix, tl);
+    const uint32_t nobs = E.nobs();
+    *loads = 0;
+    *option = TL_EMPTY;
+    if (nobs == 0) return E.none();
+    uint32_t h0 = 0, h1 = 0;
+    relic_hits(ix, lo, lop, hi, rop, h0, h1);
+    if (h1 <= h0 || (tl.enc == ENC_FUZZ && h0 >= nobs)) return E.none();
+    if (h0 == 0 && h1 >= nobs) { *option = TL_ALL; return E.all(); }
+    if (h0 + 1 == h1) {
+        *option = TL_EQUAL;
+        return ix.bits[h0] ? *ix.bits[h0] : E.none();
+    }
+    const auto& off = tl.off;
+    const auto& coff = tl.coff;
+    const auto& cb = tl.cbounds;
+    const bool have_coff = !tl.cbits.empty() && coff.size() == tl.cbits.size() + 1;
+    const uint32_t ncoarse = tl.enc == ENC_FUZZ
+        ? (cb.empty() ? 0u : (uint32_t)cb.size() - 1)
+        : ((tl.cbits.empty() || cb.empty()) ? 0u
+           : tl.cbits.size() + 1 <= cb.size() ? (uint32_t)tl.cbits.size() : (uint32_t)cb.size() - 1);
+    if (h0 + 3 >= h1 || ncoarse == 0 || !have_coff) {
+        *option = TL_FINE;
+        return sum_bins(ix, h0, h1);
+    }
+    const uint32_t c0 = find_first(cb, h0), c1 = find_first(cb, h1);
+    BV lower;
+    if (tl.enc == ENC_FUZZ) {
+        const long finec = E.fine_cost(h0, h1);
+        if (c0 >= c1) {
+            const long tmp = E.fuzz_coarse_cost(c1 - 1, c1) + (long)((off[h0] - off[cb[c1 - 1]]) + (off[cb[c1]] - off[h1]));
+            if (finec <= (long)(0.99 * tmp)) { *option = TL_FINE; lower = sum_bins(ix, h0, h1); }
+            else {
+                *option = TL_SAME_BIN;
+                lower = E.fuzz_coarse(c1 - 1, c1);
+                if (h0 > cb[c1 - 1]) E.minus(lower, sum_bins(ix, cb[c1 - 1], h0));
+                if (cb[c1] > h1) E.minus(lower, sum_bins(ix, h1, cb[c1]));
+            }
+            *loads = E.loads;
+            return lower;
+        }
+        // the five options (src/ixfuzz.cpp:943-1011); option 4's coarse
+        // cost starts at c0-1 here, where estimateCost starts it at c0
+        int opt = 2;
+        long cost = E.fuzz_coarse_cost(c0, c1 - 1) + (long)((off[cb[c0]] - off[h0]) + (off[h1] - off[cb[c1 - 1]]));
+        long tmp;
+        if (c0 > 0) {
+            tmp = E.fuzz_coarse_cost(c0 - 1, c1 - 1) + (long)((off[h0] - off[cb[c0 - 1]]) + (off[h1] - off[cb[c1 - 1]]));
+            if (tmp < cost) { cost = tmp; opt = 3; }
+        }
+        tmp = E.fuzz_coarse_cost(c0 > 0 ? c0 - 1 : 0, c1) + (long)((off[cb[c0]] - off[h0]) + (off[cb[c1]] - off[h1]));
+        if (tmp < cost) { cost = tmp; opt = 4; }
+        if (c0 > 0) {
+            tmp = E.fuzz_coarse_cost(c0 - 1, c1) + (long)((off[h0] - off[cb[c0 - 1]]) + (off[cb[c1]] - off[h1]));
+            if (tmp < cost) { cost = tmp; opt = 5; }
+        }
+        if (cost > (long)(0.99 * finec)) opt = 1;
+        *option = opt;
+        switch (opt) {
+        case 2:
+            lower = E.fuzz_coarse(c0, c1 - 1);
+            if (h0 < cb[c0]) E.add_bins(h0, cb[c0], lower);
+            if (cb[c1 - 1] < h1) E.add_bins(cb[c1 - 1], h1, lower);
+            break;
+        case 3:
+            lower = E.fuzz_coarse(c0 - 1, c1 - 1);
+            if (cb[c0 - 1] < h0) E.minus(lower, sum_bins(ix, cb[c0 - 1], h0));
+            if (cb[c1 - 1] < h1) E.add_bins(cb[c1 - 1], h1, lower);
+            break;
+        case 4:
+            lower = E.fuzz_coarse(c0, c1);
+            if (h0 < cb[c0]) E.add_bins(h0, cb[c0], lower);
+            if (cb[c1] > h1) E.minus(lower, sum_bins(ix, h1, cb[c1]));
+            break;
+        case 5:
+            lower = E.fuzz_coarse(c0 - 1, c1);
+            if (h0 > cb[c0 - 1]) E.minus(lower, sum_bins(ix, cb[c0 - 1], h0));
+            if (cb[c1] > h1) E.minus(lower, sum_bins(ix, h1, cb[c1]));
+            break;
+        default:
+            lower = sum_bins(ix, h0, h1);
+        }
+        *loads = E.loads;
+        return lower;
+    }
+    // zona
+    if (c0 >= c1) {
+        const long fine = (long)(off[h1] - off[h0]);
+        const long tmp = (long)(coff[c0] - coff[c0 - 1]) + (long)(off[cb[c0]] - off[cb[c0 - 1]]) - fine;
+        if ((long)(0.99 * fine) <= tmp) { *option = TL_FINE; return sum_bins(ix, h0, h1); }
+        *option = TL_SAME_BIN;
+        const BV* x = E.coarse(c0 - 1);
+        lower = x ? *x : E.all();
+        if (h0 > cb[c0 - 1]) E.minus(lower, sum_bins(ix, cb[c0 - 1], h0));
+        if (h1 < cb[c0]) E.minus(lower, sum_bins(ix, h1, cb[c0]));
+        *loads = E.loads;
+        return lower;
+    }
+    // five pairs (src/ixzona.cpp:803-882)
+    int opt = 2;
+    long cost = E.zona_part(c0, c1 - 1) + (long)((off[cb[c0]] - off[h0]) + (off[h1] - off[cb[c1 - 1]]));
+    long tmp;
+    if (c0 > 0) {
+        tmp = E.zona_part(c0 - 1, c1 - 1) + (long)((off[h0] - off[cb[c0 - 1]]) + (off[h1] - off[cb[c1 - 1]]));
+        if (tmp < cost) { cost = tmp; opt = 3; }
+    }
+    tmp = E.zona_part(c0, c1) + (long)((off[cb[c0]] - off[h0]) + (off[cb[c1]] - off[h1]));
+    if (tmp < cost) { cost = tmp; opt = 4; }
+    if (c0 > 0) {
+        tmp = E.zona_part(c0 - 1, c1) + (long)((off[h0] - off[cb[c0 - 1]]) + (off[cb[c1]] - off[h1]));
+        if (tmp < cost) { cost = tmp; opt = 5; }
+    }
+    if (cost > (long)(0.99 * E.fine_cost(h0, h1))) opt = 1;
+    *option = opt;
+    switch (opt) {
+    case 2:
+        lower = E.none();
+        if (c0 < c1 - 1) lower = E.zona_coarse(c0, c1 - 1, c0, c1 - 1, ncoarse, c0 > 0, c1 <= ncoarse);
+        if (h0 < cb[c0]) E.add_bins(h0, cb[c0], lower);
+        if (cb[c1 - 1] < h1) E.add_bins(cb[c1 - 1], h1, lower);
+        break;
+    case 3:
+        lower = E.zona_coarse(c0 - 1, c1 - 1, c0 - 1, c1 - 1, ncoarse, c0 > 1, c1 <= ncoarse);
+        if (cb[c0 - 1] < h0) E.minus(lower, sum_bins(ix, cb[c0 - 1], h0));
+        if (cb[c1 - 1] < h1) E.add_bins(cb[c1 - 1], h1, lower);
+        break;
+    case 4:
+        lower = E.zona_coarse(c0, c1, c0, c1, ncoarse, c0 > 0, c1 < ncoarse);
+        if (h0 < cb[c0]) E.add_bins(h0, cb[c0], lower);
+        if (cb[c1] > h1) E.minus(lower, sum_bins(ix, h1, cb[c1]));
+        break;
+    case 5:
+        lower = E.zona_coarse(c0 - 1, c1, c0 - 1, c1, ncoarse, c0 > 1, c1 < ncoarse);
+        if (h0 > cb[c0 - 1]) E.minus(lower, sum_bins(ix, cb[c0 - 1], h0));
+        if (cb[c1] > h1) E.minus(lower, sum_bins(ix, h1, cb[c1]));
+        break;
+    default:
+        lower = sum_bins(ix, h0, h1);
+    }
+    *loads = E.loads;
+    return lower;
+}
+
 } // namespace orc

@@ -66,4 +66,42 @@ void estimate(const Index& ix, const void* col, int type, double lo, int lop, do
 BV evaluate_plan(const Index& ix, const void* col, int type, double lo, int lop, double hi, int rop,
                  int plan, int model, int* used);
 
+// ---------------------------------------------------------------------------
+// Two-level encodings over a relic: ibis::fuzz (interval-equality coarse
+// level) and ibis::zona (equality-equality), src/irelic.h:540-712.
+enum { ENC_FUZZ = 1, ENC_ZONA = 2 };
+// What an evaluation did (the option numbers of fuzz::evaluate /
+// zona::evaluate, 1..5; the other branches numbered here).
+enum { TL_EMPTY = 0, TL_FINE = 1, TL_ALL = 6, TL_EQUAL = 7, TL_SAME_BIN = 8 };
+
+struct TwoLevel {
+    int enc = 0;
+    std::vector<uint32_t> cbounds;     // coarse bin c holds fine bitmaps [cbounds[c], cbounds[c+1])
+    std::vector<BV*> cbits;            // fuzz: ncoarse - ceil(ncoarse/2) + 1 bitmaps, zona: ncoarse
+    std::vector<int64_t> off;          // offset64 of the fine level (bytes)
+    std::vector<int64_t> coff;         // coffset64 (bytes)
+    ~TwoLevel() { for (auto b : cbits) delete b; }
+};
+
+// fuzz::coarsen / zona::coarsen (src/ixfuzz.cpp:161-263, src/ixzona.cpp:
+// 124-182) on the index built in memory: offset64 from bitvector::bytes(),
+// coffset64 from the compressed coarse bitmaps' bytes().  ncoarse is the
+// "ncoarse=" value of the index spec (fuzz only; values below 5 select the
+// default rule).  Returns a level with no coarse bins where coarsen
+// declines to build one.
+TwoLevel* coarsen(const Index& ix, int enc, uint32_t ncoarse);
+// The coarse level as an index file holds it (src/ixfuzz.cpp:46-143): fine
+// and coarse positions from the file.
+TwoLevel* two_level_from_file(const Index& ix, int enc, const std::vector<int64_t>& fine_pos,
+                              const std::vector<uint32_t>& cbounds, const std::vector<int64_t>& coff,
+                              const std::vector<std::vector<word_t>>& cbits_serial);
+// fuzz::estimateCost / zona::estimateCost (src/ixfuzz.cpp:767-870,
+// src/ixzona.cpp:580-711)
+double two_level_cost(const Index& ix, const TwoLevel& tl, int type, double lo, int lop, double hi, int rop);
+// fuzz::evaluate / zona::evaluate (src/ixfuzz.cpp:872-1095, src/ixzona.cpp:
+// 713-1017): the hits; *option what it did, *loads bit c for every coarse
+// bitmap it reads.
+BV two_level_eval(const Index& ix, const TwoLevel& tl, double lo, int lop, double hi, int rop, int* option,
+                  uint64_t* loads);
+
 } // namespace orc

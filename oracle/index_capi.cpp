@@ -21,6 +21,8 @@ void* orc_bin_bounds(const double* v, uint32_t n, const double* b, uint32_t nb) 
     return bin_bounds(v, n, b, nb);
 }
 void orc_index_free(void* ix) { delete (Index*)ix; }
+// the index as one read from a file (offset64 holds serialized positions)
+void orc_index_set_offsets_built(void* ix, int v) { ((Index*)ix)->offsets_built = v != 0; }
 
 int orc_index_write(void* ix, double** keys, size_t* nk, int64_t** offs, size_t* no, uint32_t** bms,
                     size_t* nb) {
@@ -99,6 +101,66 @@ int orc_index_cost(void* ix, const void* col, int type, double lo, int lop, doub
     *plan = auto_plan(x, col, type, lo, lop, hi, rop, model, &e);
     *empty = e ? 1 : 0;
     return 0;
+}
+
+// --- two-level encodings (fuzz / zona) over a relic
+void* orc_two_level_build(void* ix, int enc, uint32_t ncoarse) {
+    try {
+        return coarsen(*(Index*)ix, enc, ncoarse);
+    } catch (...) {
+        return nullptr;
+    }
+}
+
+// The coarse level as a file holds it: fine bitmap positions (bytes, nobs+1),
+// cbounds (nc+1), coarse positions (bytes, ncb+1) and the coarse bitmaps'
+// serialized words (bitmap i at words[woff[i], woff[i+1])).
+void* orc_two_level_file(void* ix, int enc, const int64_t* fine_pos, uint32_t nobs, const uint32_t* cbounds,
+                         uint32_t nc, const int64_t* coff, uint32_t ncb, const uint32_t* words, const int64_t* woff) {
+    std::vector<int64_t> fp(fine_pos, fine_pos + nobs + 1), co(coff, coff + ncb + 1);
+    std::vector<uint32_t> cb(cbounds, cbounds + nc + 1);
+    std::vector<std::vector<word_t>> cw(ncb);
+    for (uint32_t i = 0; i < ncb; ++i) cw[i].assign(words + woff[i], words + woff[i + 1]);
+    return two_level_from_file(*(Index*)ix, enc, fp, cb, co, cw);
+}
+
+void orc_two_level_free(void* tl) { delete (TwoLevel*)tl; }
+
+int orc_two_level_info(void* tl, uint32_t** cbounds, size_t* ncb, int64_t** coff, size_t* nco, int64_t** off,
+                       size_t* noff) {
+    const TwoLevel& t = *(TwoLevel*)tl;
+    *cbounds = dup(t.cbounds); *ncb = t.cbounds.size();
+    *coff = dup(t.coff); *nco = t.coff.size();
+    *off = dup(t.off); *noff = t.off.size();
+    return 0;
+}
+
+// serialized words of coarse bitmap i
+int orc_two_level_cbit(void* tl, uint32_t i, uint32_t** out, size_t* n) {
+    const TwoLevel& t = *(TwoLevel*)tl;
+    if (i >= t.cbits.size()) return -1;
+    std::vector<word_t> v = t.cbits[i]->serial();
+    *out = dup(v); *n = v.size();
+    return 0;
+}
+
+double orc_two_level_cost(void* ix, void* tl, int type, double lo, int lop, double hi, int rop) {
+    return two_level_cost(*(Index*)ix, *(TwoLevel*)tl, type, lo, lop, hi, rop);
+}
+
+int orc_two_level_eval(void* ix, void* tl, double lo, int lop, double hi, int rop, uint64_t* count,
+                       uint32_t** rids, size_t* nr, int* option, uint64_t* loads) {
+    try {
+        BV r = two_level_eval(*(Index*)ix, *(TwoLevel*)tl, lo, lop, hi, rop, option, loads);
+        std::vector<word_t> v;
+        r.rids(v);
+        *count = r.cnt();
+        *rids = dup(v);
+        *nr = v.size();
+        return 0;
+    } catch (...) {
+        return -2;
+    }
 }
 
 } // extern "C"

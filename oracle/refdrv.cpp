@@ -7,6 +7,8 @@
 #include "ibin.h"
 #include "iapi.h"
 #include <chrono>
+#include <cstdio>
+#include <string>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -598,6 +600,135 @@ int64_t ref_iapi_expr(const char* const* names, const double* lo, const int* lop
     FastBitSelectionHandle h = static_cast<FastBitSelectionHandle>(st[0]);
     const int64_t cnt = fastbit_selection_evaluate(h);
     fastbit_selection_free(h);
+    return cnt;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// Two-level encodings over a relic: ibis::fuzz (interval-equality) and
+// ibis::zona (equality-equality), src/irelic.h:540-712.
+namespace {
+
+// ibis::index::col is protected; a pointer to it taken inside a derived
+// class reaches it on any index.
+struct IndexCol : ibis::index {
+    static const ibis::column* ibis::index::*member() { return &IndexCol::col; }
+};
+
+// An index file through the file manager (fileManager::getFile, then the
+// storage constructors fuzz(c, st) / zona(c, st) / relic(c, st) as
+// index::create uses them, src/index.cpp:781-795): the bitmaps are copied
+// out of the mapped file when an evaluation first needs them.  (Read with
+// fuzz::read(name) instead, which loads bitmaps by file name, the fuzz_i32
+// fixture's file gives 521 hits for 283.48 <= x <= 313.36, where 1,233 rows
+// qualify; through the file manager every term matches the column.)
+ibis::relic* read_two_level(const char* path) {
+    ibis::fileManager::storage* st = 0;
+    if (ibis::fileManager::instance().getFile(path, &st) != 0 || st == 0 || st->size() < 8) return 0;
+    const char t = st->begin()[5];
+    if (t == (char)ibis::index::FUZZ) return new ibis::fuzz(0, st);
+    if (t == (char)ibis::index::ZONA) return new ibis::zona(0, st);
+    if (t == (char)ibis::index::RELIC) return new ibis::relic(0, st);
+    return 0;
+}
+
+// fuzz::cbits / zona::cbits are private; an explicit instantiation may name
+// a private member, and the friend function it defines hands out the
+// pointer to it.
+template <class Tag, typename Tag::type M>
+struct Rob {
+    friend typename Tag::type get(Tag) { return M; }
+};
+struct FuzzCbits {
+    typedef ibis::array_t<ibis::bitvector*> ibis::fuzz::*type;
+    friend type get(FuzzCbits);
+};
+struct ZonaCbits {
+    typedef ibis::array_t<ibis::bitvector*> ibis::zona::*type;
+    friend type get(ZonaCbits);
+};
+template struct Rob<FuzzCbits, &ibis::fuzz::cbits>;
+template struct Rob<ZonaCbits, &ibis::zona::cbits>;
+
+// Bit i set when coarse bitmap i is in memory.
+uint64_t coarse_loaded(const ibis::relic* r) {
+    const ibis::array_t<ibis::bitvector*>* cb = 0;
+    if (r->type() == ibis::index::FUZZ) cb = &(static_cast<const ibis::fuzz*>(r)->*get(FuzzCbits()));
+    else if (r->type() == ibis::index::ZONA) cb = &(static_cast<const ibis::zona*>(r)->*get(ZonaCbits()));
+    uint64_t m = 0;
+    if (cb)
+        for (size_t i = 0; i < cb->size() && i < 64; ++i)
+            if ((*cb)[i] != 0) m |= 1ull << i;
+    return m;
+}
+
+} // namespace
+
+extern "C" {
+
+// fuzz::write / zona::write (src/ixfuzz.cpp:1098-1209, src/ixzona.cpp:
+// 1060-1130) of the index built for the registered array `name` (built with
+// an "<encoding interval-equality/>" or "equality-equality" spec).  The
+// index is written as one with no column attached, so that the path is used
+// as given.
+int ref_iapi_write_coarse_file(const char* name, const char* path) {
+    ensure_init();
+    ibis::bord::column* c = __fastbit_iapi_array_by_name(name);
+    if (c == 0) return -1;
+    ibis::column::indexLock lock(c, "ref_iapi_write_coarse_file");
+    ibis::index* ix = const_cast<ibis::index*>(lock.getIndex());
+    if (ix == 0) return -2;
+    if (ix->type() != ibis::index::FUZZ && ix->type() != ibis::index::ZONA) return -3;
+    const ibis::column* saved = ix->*IndexCol::member();
+    ix->*IndexCol::member() = 0;
+    int rc = -4;
+    try {
+        rc = ix->write(path);
+    } catch (...) {
+    }
+    ix->*IndexCol::member() = saved;
+    return rc;
+}
+
+// fuzz::evaluate / zona::evaluate and estimateCost (src/ixfuzz.cpp:767-1095,
+// src/ixzona.cpp:580-1040) of lo OP1 x OP2 hi, on the index built for the
+// registered array `name` (path null) or on the index file at `path` (the
+// array `name` is attached as its column, for the null mask).  An index
+// read from a file loads its bitmaps when an evaluation first needs them:
+// *loaded gets bit i set for every coarse bitmap evaluate read, which names
+// the option it took.  Returns the hit count.
+int64_t ref_two_level_eval(const char* name, const char* path, double lo, int lop, double hi, int rop,
+                           double* cost, uint64_t* loaded) {
+    ensure_init();
+    ibis::bord::column* c = __fastbit_iapi_array_by_name(name);
+    if (c == 0) return -1;
+    ibis::qContinuousRange cr(lo, cmp_op(lop), name, cmp_op(rop), hi);
+    int64_t cnt = -2;
+    *loaded = 0;
+    ibis::relic* owned = 0;
+    const ibis::relic* ix = 0;
+    ibis::column::indexLock* lock = 0;
+    if (path) {
+        owned = read_two_level(path);
+        if (owned == 0) return -3;
+        owned->*IndexCol::member() = c;
+        ix = owned;
+    } else {
+        lock = new ibis::column::indexLock(c, "ref_two_level_eval");
+        ix = static_cast<const ibis::relic*>(lock->getIndex());
+        if (ix == 0) { delete lock; return -4; }
+    }
+    try {
+        *cost = ix->estimateCost(cr);
+        ibis::bitvector res;
+        cnt = ix->evaluate(cr, res);
+        if (path) *loaded = coarse_loaded(ix);
+    } catch (...) {
+        cnt = -5;
+    }
+    if (owned) { owned->*IndexCol::member() = 0; delete owned; }
+    delete lock;
     return cnt;
 }
 
