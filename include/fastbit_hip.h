@@ -180,6 +180,48 @@ int fbh_index_build_relic(const void* col, int type, uint32_t nrows, fbh_index**
 /* relic(col, card, ind) (src/irelic.cpp:160-211): values 0..card-1. */
 int fbh_index_build_relic_card(const uint32_t* ind, uint32_t nrows, uint32_t card,
                                fbh_index** out);
+/* Two-level indexes over a relic (src/irelic.h:540-712): the relic is the
+ * fine level, and a coarse level groups its bitmaps into ncoarse bins of
+ * nearly equal bytes.  ibis::fuzz (interval-equality) keeps
+ * ncoarse - ceil(ncoarse/2) + 1 coarse bitmaps, bitmap i holding coarse
+ * bins [i, i + ceil(ncoarse/2)); ibis::zona (equality-equality) keeps one
+ * per coarse bin.  A range query then reads one or two coarse bitmaps and
+ * the fine bitmaps at its edges where the reference's cost rule says that
+ * is cheaper (fuzz::evaluate, src/ixfuzz.cpp:872-1095; zona::evaluate,
+ * src/ixzona.cpp:713-1017); its rows are those of the relic.  The coarse
+ * bitmaps are the canonical WAH of their rows (the reference compresses
+ * the result of its own op sequence, which may keep a lone count-1 fill
+ * where the canonical form has a literal word). */
+#define FBH_ENC_FUZZ 1
+#define FBH_ENC_ZONA 2
+/* ibis::fuzz(col) (src/ixfuzz.cpp:20-44, coarsen 161-263): ncoarse is the
+ * index spec's "ncoarse=" value; 0 (or below 5) picks the reference's
+ * default count.  With fewer than 32 distinct values, or where the rule
+ * gives fewer than 5 bins, there is no coarse level and queries run on the
+ * relic, as in the reference. */
+int fbh_index_build_fuzz(const void* col, int type, uint32_t nrows, uint32_t ncoarse, fbh_index** out);
+/* ibis::zona(col) (src/ixzona.cpp, coarsen 124-182): 11 coarse bins. */
+int fbh_index_build_zona(const void* col, int type, uint32_t nrows, fbh_index** out);
+/* The coarse level: the encoding (0 for a plain relic or bin index), the
+ * number of coarse bins, their boundaries cbounds[0..ncoarse] (up to cap),
+ * the coarse bitmaps' byte offsets as coffset64 holds them (up to ccap;
+ * bitvector::bytes() sums for a built index, file positions for one read
+ * from a file) and the number of coarse bitmaps. */
+int fbh_index_coarse(const fbh_index* ix, int* enc, uint32_t* ncoarse, uint32_t* cbounds, uint32_t cap,
+                     int64_t* coffsets, uint32_t ccap, uint32_t* ncbits);
+/* Coarse bitmap i as a new bitvector. */
+int fbh_index_coarse_bitmap(const fbh_index* ix, uint32_t i, fbh_bitvector** out);
+/* What fuzz::evaluate / zona::evaluate does for a range: *option 1 (fine
+ * level only) or 2..5 (the coarse options, numbered as in the reference),
+ * or one of FBH_TL_*; *coarse_read bit i for every coarse bitmap it reads
+ * (the bitmaps a file-backed reference index loads). */
+#define FBH_TL_EMPTY 0
+#define FBH_TL_FINE 1
+#define FBH_TL_ALL 6
+#define FBH_TL_EQUAL 7
+#define FBH_TL_SAME_BIN 8
+int fbh_index_two_level_option(const fbh_index* ix, double lo, int lop, double hi, int rop, int* option,
+                               uint64_t* coarse_read);
 /* bin with the given boundaries: bin::binning (src/ibin.cpp:3350-3449),
  * empty inner bins dropped.  The bounds are the reference's bounds array
  * (bounds[nobs-1] is DBL_MAX). */
@@ -224,8 +266,10 @@ int fbh_index_attach(const void* col, int type, uint32_t nrows, const double* ke
 /* relic::read / bin::read of an index file (src/irelic.cpp:690-786,
  * src/ibin.cpp:468-553): the "#IBIS" header, 4- or 8-byte bitmap
  * positions, the keys, and the serialized bitmaps, mapped from the file and
- * copied to device memory once.  Relic (type 7) and bin (type 0) files;
- * nrows must equal the row count in the file. */
+ * copied to device memory once.  Relic (type 7), bin (type 0), fuzz (type
+ * 24; fuzz::read, src/ixfuzz.cpp:1333-1584) and zona (type 25) files; the
+ * two-level files carry their coarse level after the relic part.  nrows
+ * must equal the row count in the file. */
 int fbh_index_read_file(const char* path, const void* col, int type, uint32_t nrows, fbh_index** out);
 /* fastbit_iapi_get_number_of_hits (src/iapi.cpp:2011-2017): the ones of
  * index::sumBins(ib, ie) (src/index.cpp:6450-6491) over the index's
@@ -358,8 +402,11 @@ void fbh_conj_free(fbh_conj* c);
  * hipGraph and autotuned by fbh_conj_tune; counts only.  plans[i] (or
  * `plan` when plans is NULL) picks each query's plan; FBH_PLAN_AUTO applies
  * the heuristic, and the compressed plan runs as the literal one (the same
- * bits).  (No counterpart in the reference, which evaluates one selection
- * at a time, src/iapi.cpp:1383-1418.) */
+ * bits).  On a fuzz / zona index each query is the program of its option
+ * (coarse and fine bitmaps combined) in one launch of the expression
+ * kernel, and fbh_conj_plans gives one plan per query.  (No counterpart in
+ * the reference, which evaluates one selection at a time,
+ * src/iapi.cpp:1383-1418.) */
 int fbh_index_batch_prepare(fbh_index* ix, uint32_t nq, const double* lo, const int* lop, const double* hi,
                             const int* rop, const int* plans, int plan, fbh_conj** out);
 /* One conjunction: *hits receives the count. */

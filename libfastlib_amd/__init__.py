@@ -8,8 +8,10 @@ ABI; the C++ host binding is ``include/fastbit_hip.hpp``.
 from . import native  # noqa: F401
 from .native import (AND, OR, XOR, MINUS, Bitvector, FastbitError, Index, Conjunctions, Expressions, RangeBatch,
                      EXPR_AND, EXPR_OR, EXPR_XOR, EXPR_NAND, EXPR_NOR, EXPR_NOT, EXPR_MINUS, bench_op,
-                     bench_op_rot, bench_cnt_rot, lib)  # noqa: F401
+                     bench_op_rot, bench_cnt_rot, lib, PLAN_AUTO, PLAN_LITERAL, PLAN_COMPRESSED, PLAN_SCAN,
+                     ENC_FUZZ, ENC_ZONA, TL_EMPTY, TL_FINE, TL_ALL, TL_EQUAL, TL_SAME_BIN)  # noqa: F401
 
 __all__ = ["AND", "OR", "XOR", "MINUS", "Bitvector", "FastbitError", "Index", "Conjunctions", "Expressions", "RangeBatch",
            "EXPR_AND", "EXPR_OR", "EXPR_XOR", "EXPR_NAND", "EXPR_NOR", "EXPR_NOT", "EXPR_MINUS", "bench_op",
-           "bench_op_rot", "bench_cnt_rot", "lib"]
+           "bench_op_rot", "bench_cnt_rot", "lib", "PLAN_AUTO", "PLAN_LITERAL", "PLAN_COMPRESSED", "PLAN_SCAN",
+           "ENC_FUZZ", "ENC_ZONA", "TL_EMPTY", "TL_FINE", "TL_ALL", "TL_EQUAL", "TL_SAME_BIN"]

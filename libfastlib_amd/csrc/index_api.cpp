@@ -114,10 +114,20 @@ struct fbh_index {
     bool offsets_built = false;   // the reference constructor fills offset64
     int cost_model = FBH_COST_IAPI;
     double cmin = DBL_MAX, cmax = -DBL_MAX;   // column min / max
+    // two-level encodings over a relic (ibis::fuzz / ibis::zona): the coarse
+    // bins, the byte offsets the cost rules read, and the coarse bitmaps as
+    // an index of their own (its words, tiled copy and tables)
+    int enc = 0;                          // FBH_ENC_FUZZ, FBH_ENC_ZONA or 0
+    std::vector<uint32_t> cbounds;        // coarse bin c: fine bitmaps [cbounds[c], cbounds[c+1])
+    std::vector<int64_t> toff;            // offset64 of the fine level (bytes)
+    std::vector<int64_t> tcoff;           // coffset64 of the coarse bitmaps (bytes)
+    fbh_index* coarse = nullptr;
     ~fbh_index() {
         dev_free(words); dev_free(doff); dev_free(dnw); dev_free(tw); dev_free(TW); dev_free(col);
         dev_free(dbounds);
+        delete coarse;
     }
+    bool two_level() const { return coarse != nullptr && !cbounds.empty(); }
     bool null_bitmap(uint32_t b) const { return off[b + 1] == off[b]; }
     // ibis::bitvector::bytes(): code words * 4 + sizeof(bitvector) (40 on x86-64)
     double bytes(uint32_t b) const { return null_bitmap(b) ? 0.0 : 4.0 * nw[b] + 40.0; }
@@ -621,6 +631,25 @@ int eval_compressed(fbh_index* ix, double lo, int lop, double hi, int rop, fbh_b
 int plan_heuristic(const fbh_index* ix, double lo, int lop, double hi, int rop);
 double index_cost(const fbh_index* ix, double lo, int lop, double hi, int rop);
 double scan_cost(const fbh_index* ix);
+double two_level_cost(const fbh_index* ix, double lo, int lop, double hi, int rop);
+int tl_build(fbh_index* ix, int enc, uint32_t spec);
+int tl_read(fbh_index* ix, int enc, const uint8_t* file, size_t n, size_t pos, size_t w);
+double tl_query_bytes(const fbh_index* ix, double lo, int lop, double hi, int rop);
+int tl_evaluate(fbh_index* ix, double lo, int lop, double hi, int rop, uint64_t* hits, fbh_bitvector** result,
+                uint32_t* rids, uint64_t cap);
+int tl_batch_counts(fbh_index* ix, uint32_t nq, const double* lo, const int* lop, const double* hi,
+                    const int* rop, const int* plans, int plan, uint64_t* hits, int* plans_used, float* gpu_ms);
+int tl_batch_prepare(fbh_index* ix, uint32_t nq, const double* lo, const int* lop, const double* hi,
+                     const int* rop, const int* plans, int plan, fbh_conj** out);
+// One evaluation as a postfix program over terms of the fine and the
+// coarse level (operators as the expression kernel takes them).
+struct TLProgram {
+    int option = FBH_TL_EMPTY;
+    uint64_t loads = 0;                      // coarse bitmaps read
+    std::vector<fbk::QueryArgs> terms;
+    std::vector<int32_t> prog;
+};
+void two_level_program(const fbh_index* ix, double lo, int lop, double hi, int rop, TLProgram& P);
 
 // Canonical WAH of a literal result (groups words, then the active bits
 // left-aligned) as a new bitvector.
@@ -820,6 +849,18 @@ int fbh_index_build_relic(const void* col, int type, uint32_t nrows, fbh_index**
     if ((rc = upload_column(ix, col, type, nrows))) { delete ix; return rc; }
     std::vector<double> vals;
     if ((rc = distinct_values(ix, vals))) { delete ix; return rc; }
+    // -0.0 and +0.0 are one value of index::mapValues' std::map
+    // (src/index.cpp:2552-2611), keyed by the zero that comes first
+    for (size_t i = 1; i < vals.size(); ++i) {
+        if (vals[i] == 0.0 && vals[i - 1] == 0.0) {
+            const float* fc = (const float*)col;
+            uint32_t r = 0;
+            while (r < nrows && fc[r] != 0.0f) ++r;
+            vals[i - 1] = (r < nrows) ? (double)fc[r] : 0.0;
+            vals.erase(vals.begin() + i);
+            break;
+        }
+    }
     ix->keys = vals;
     ix->nobs = (uint32_t)vals.size();
     if (!vals.empty()) { ix->cmin = vals.front(); ix->cmax = vals.back(); }
@@ -858,6 +899,69 @@ int fbh_index_build_relic_card(const uint32_t* ind, uint32_t nrows, uint32_t car
         ix->cmax = std::max(ix->cmax, (double)ind[i]);
     }
     *out = ix;
+    return FBH_OK;
+}
+
+// ibis::fuzz(col) / ibis::zona(col) (src/ixfuzz.cpp:20-44, src/ixzona.cpp):
+// the relic, then coarsen() adds the coarse level.  ncoarse is the index
+// spec's "ncoarse=" value (fuzz; 0 or below 5 for the default rule).
+static int build_two_level(const void* col, int type, uint32_t nrows, int enc, uint32_t ncoarse, fbh_index** out) {
+    fbh_index* ix = nullptr;
+    int rc = fbh_index_build_relic(col, type, nrows, &ix);
+    if (rc) return rc;
+    rc = tl_build(ix, enc, ncoarse);
+    if (rc) { delete ix; return rc; }
+    *out = ix;
+    return FBH_OK;
+}
+
+int fbh_index_build_fuzz(const void* col, int type, uint32_t nrows, uint32_t ncoarse, fbh_index** out) {
+    FBH_API_LOCK();
+    if (!out || (nrows && !col)) return fail(FBH_ERR_ARG, "null argument");
+    return build_two_level(col, type, nrows, FBH_ENC_FUZZ, ncoarse, out);
+}
+
+int fbh_index_build_zona(const void* col, int type, uint32_t nrows, fbh_index** out) {
+    FBH_API_LOCK();
+    if (!out || (nrows && !col)) return fail(FBH_ERR_ARG, "null argument");
+    return build_two_level(col, type, nrows, FBH_ENC_ZONA, 0, out);
+}
+
+int fbh_index_coarse(const fbh_index* ix, int* enc, uint32_t* ncoarse, uint32_t* cbounds, uint32_t cap,
+                     int64_t* coffsets, uint32_t ccap, uint32_t* ncbits) {
+    FBH_API_LOCK();
+    if (!ix) return fail(FBH_ERR_ARG, "null index");
+    const uint32_t nc = ix->cbounds.empty() ? 0u : (uint32_t)ix->cbounds.size() - 1;
+    const uint32_t nb = ix->coarse ? ix->coarse->nobs : 0u;
+    if (enc) *enc = ix->enc;
+    if (ncoarse) *ncoarse = nc;
+    if (ncbits) *ncbits = nb;
+    if (cbounds) for (uint32_t i = 0; i < std::min<uint32_t>(cap, (uint32_t)ix->cbounds.size()); ++i) cbounds[i] = ix->cbounds[i];
+    if (coffsets) for (uint32_t i = 0; i < std::min<uint32_t>(ccap, (uint32_t)ix->tcoff.size()); ++i) coffsets[i] = ix->tcoff[i];
+    return FBH_OK;
+}
+
+int fbh_index_coarse_bitmap(const fbh_index* ix, uint32_t i, fbh_bitvector** out) {
+    FBH_API_LOCK();
+    if (!ix || !out) return fail(FBH_ERR_ARG, "null argument");
+    if (!ix->coarse || i >= ix->coarse->nobs) return fail(FBH_ERR_ARG, "no such coarse bitmap");
+    const fbh_index* c = ix->coarse;
+    const uint32_t trailer = c->anb ? 2u : 1u;
+    if (c->null_bitmap(i)) return fbh_bv_new(out);
+    std::vector<uint32_t> w(c->nw[i] + trailer);
+    int rc = to_host(w.data(), c->words + c->off[i], w.size());
+    return rc ? rc : fbh_bv_from_serial(w.data(), (uint32_t)w.size(), out);
+}
+
+int fbh_index_two_level_option(const fbh_index* ix, double lo, int lop, double hi, int rop, int* option,
+                               uint64_t* coarse_read) {
+    FBH_API_LOCK();
+    if (!ix || !option) return fail(FBH_ERR_ARG, "null argument");
+    if (!ix->enc) return fail(FBH_ERR_ARG, "not a fuzz or zona index");
+    TLProgram P;
+    two_level_program(ix, lo, lop, hi, rop, P);
+    *option = P.option;
+    if (coarse_read) *coarse_read = P.loads;
     return FBH_OK;
 }
 
@@ -972,7 +1076,8 @@ int fbh_index_read_file(const char* path, const void* col, int type, uint32_t nr
         return fail(FBH_ERR_ARG, std::string(path) + " is not an index file");
     const int itype = f.p[5];
     const size_t w = f.p[6];
-    constexpr int BINNING = 0, RELIC = 7;   // ibis::index::INDEX_TYPE (src/index.h:91-105)
+    // ibis::index::INDEX_TYPE (src/index.h:91-143)
+    constexpr int BINNING = 0, RELIC = 7, FUZZ = 24, ZONA = 25;
     const uint32_t frows = f.at<uint32_t>(8), nobs = f.at<uint32_t>(12);
     size_t obeg, oend, kend;
     std::vector<double> vals, bounds, minv, maxv;
@@ -980,7 +1085,7 @@ int fbh_index_read_file(const char* path, const void* col, int type, uint32_t nr
         v.resize(nobs);
         if (nobs) memcpy(v.data(), f.p + pos, 8ull * nobs);
     };
-    if (itype == RELIC) {
+    if (itype == RELIC || itype == FUZZ || itype == ZONA) {
         // relic::read (src/irelic.cpp:727-749): nrows, nobs, card; the
         // values at byte 24; the nobs+1 bitmap positions after them
         if (f.n < 20) return fail(FBH_ERR_ARG, "truncated index file");
@@ -1002,7 +1107,8 @@ int fbh_index_read_file(const char* path, const void* col, int type, uint32_t nr
         doubles(kbeg + 8ull * nobs, maxv);
         doubles(kbeg + 16ull * nobs, minv);
     } else {
-        return fail(FBH_ERR_ARG, "index type " + std::to_string(itype) + " is not a relic or a bin index");
+        return fail(FBH_ERR_ARG, "index type " + std::to_string(itype) +
+                                     " is not a relic, fuzz, zona or bin index");
     }
     if (nobs == 0) return fail(FBH_ERR_ARG, "index file without bitmaps");
     if (frows != nrows)
@@ -1022,11 +1128,14 @@ int fbh_index_read_file(const char* path, const void* col, int type, uint32_t nr
     }
     if ((rc = ensure_init())) return rc;
     fbh_index* ix = new fbh_index();
-    ix->kind = itype == RELIC ? FBH_RELIC : FBH_BIN;
+    ix->kind = itype == BINNING ? FBH_BIN : FBH_RELIC;
     if ((rc = upload_column(ix, col, type, nrows))) { delete ix; return rc; }
     const uint32_t* bms = (const uint32_t*)(f.p + pos[0]);
     rc = attach_index(ix, vals.data(), bounds.data(), minv.data(), maxv.data(), nobs, woff.data(), bms,
                       (uint64_t)woff[nobs]);
+    // fuzz / zona: the coarse level follows the last fine bitmap
+    if (!rc && (itype == FUZZ || itype == ZONA))
+        rc = tl_read(ix, itype == FUZZ ? FBH_ENC_FUZZ : FBH_ENC_ZONA, f.p, f.n, (size_t)pos[nobs], w);
     if (rc) { delete ix; return rc; }
     *out = ix;
     return FBH_OK;
@@ -1140,6 +1249,7 @@ int fbh_index_evaluate(fbh_index* ix, double lo, int lop, double hi, int rop, in
     if (rc) return rc;
     if (plan == FBH_PLAN_AUTO) plan = plan_heuristic(ix, lo, lop, hi, rop);
     if (plan_used) *plan_used = plan;
+    if (ix->enc && plan == FBH_PLAN_LITERAL) return tl_evaluate(ix, lo, lop, hi, rop, hits, result, nullptr, 0);
     if (plan == FBH_PLAN_COMPRESSED) {
         fbh_bitvector* res = nullptr;
         if ((rc = fbh_bv_new(&res))) return rc;
@@ -1171,6 +1281,7 @@ int fbh_index_evaluate_rids(fbh_index* ix, double lo, int lop, double hi, int ro
     int rc = ensure_init();
     if (rc) return rc;
     if (plan == FBH_PLAN_AUTO || plan == FBH_PLAN_COMPRESSED) plan = FBH_PLAN_LITERAL;
+    if (ix->enc && plan == FBH_PLAN_LITERAL) return tl_evaluate(ix, lo, lop, hi, rop, n, nullptr, rids, cap);
     DBuf<uint32_t> lit;
     PartCounts pc;
     if ((rc = lit.alloc(ix->groups + 1)) || (rc = pc.alloc(ix, 1))) return rc;
@@ -1310,6 +1421,10 @@ int fbh_index_evaluate_batch(fbh_index* ix, uint32_t nq, const double* lo, const
     if (!ix || !lo || !lop || !hi || !rop || !hits) return fail(FBH_ERR_ARG, "null argument");
     int rc = ensure_init();
     if (rc) return rc;
+    if (ix->enc) {
+        if (nq == 0) return FBH_OK;
+        return tl_batch_counts(ix, nq, lo, lop, hi, rop, plans, plan, hits, plans_used, gpu_ms);
+    }
     // consecutive literal / scan queries rotate over the side streams, so
     // that the last blocks of one query overlap the first of the next (each
     // stream has its own literal result buffer)
@@ -1380,6 +1495,7 @@ int fbh_index_query_bytes(const fbh_index* ix, double lo, int lop, double hi, in
     const double res = 4.0 * (ix->groups + 1);
     const double es = (double)col_esize(ix->type);
     if (plan == FBH_PLAN_SCAN) { *bytes = es * ix->nrows + res; return FBH_OK; }
+    if (ix->enc && plan != FBH_PLAN_COMPRESSED) { *bytes = tl_query_bytes(ix, lo, lop, hi, rop); return FBH_OK; }
     Locate L;
     locate(ix, lo, lop, hi, rop, L);
     const SumPlan sp = sum_plan(ix, L.hit0, L.hit1);
@@ -1912,7 +2028,7 @@ int fbh_conj_result(fbh_conj* c, uint32_t q, fbh_bitvector** out) {
 int fbh_conj_plans(const fbh_conj* c, int* plans) {
     FBH_API_LOCK();
     if (!c || !plans) return fail(FBH_ERR_ARG, "null argument");
-    for (uint32_t k = 0; k < c->nterms; ++k) plans[k] = c->plans[k];
+    for (size_t k = 0; k < c->plans.size(); ++k) plans[k] = c->plans[k];
     return FBH_OK;
 }
 
@@ -1925,6 +2041,15 @@ int fbh_index_batch_prepare(fbh_index* ix, uint32_t nq, const double* lo, const 
                             const int* rop, const int* plans, int plan, fbh_conj** out) {
     FBH_API_LOCK();
     if (!ix || !lo || !lop || !hi || !rop || !out || nq == 0) return fail(FBH_ERR_ARG, "bad argument");
+    if (plan < FBH_PLAN_AUTO || plan > FBH_PLAN_SCAN) return fail(FBH_ERR_ARG, "bad plan");
+    for (uint32_t i = 0; i < nq; ++i) {
+        if (plans && (plans[i] < FBH_PLAN_AUTO || plans[i] > FBH_PLAN_SCAN)) return fail(FBH_ERR_ARG, "bad plan");
+        if (lop[i] < 0 || lop[i] > 2 || rop[i] < 0 || rop[i] > 2) return fail(FBH_ERR_ARG, "bad operator");
+    }
+    if (ix->enc) {
+        int rc = ensure_init();
+        return rc ? rc : tl_batch_prepare(ix, nq, lo, lop, hi, rop, plans, plan, out);
+    }
     std::vector<fbh_term> t(nq);
     std::vector<uint32_t> qoff(nq + 1);
     for (uint32_t i = 0; i < nq; ++i) {
@@ -1978,6 +2103,7 @@ namespace {
 // ind); the in-memory constructors leave it empty, and then relic sums
 // bitvector::bytes() over the hit bitmaps and bin keeps only the edge term.
 double index_cost(const fbh_index* ix, double lo, int lop, double hi, int rop) {
+    if (ix->enc) return two_level_cost(ix, lo, lop, hi, rop);
     const bool offsets = ix->cost_model == FBH_COST_PARTITION || ix->offsets_built;
     const uint32_t nobs = ix->nobs;
     auto ser = [&](uint32_t b0, uint32_t b1) {
@@ -2074,6 +2200,7 @@ bool overlap(const fbh_index* ix, double lo, int lop, double hi, int rop) {
 int plan_heuristic(const fbh_index* ix, double lo, int lop, double hi, int rop) {
     if (ix->cost_model == FBH_COST_PARTITION && !overlap(ix, lo, lop, hi, rop)) return FBH_PLAN_LITERAL;
     if (!(index_cost(ix, lo, lop, hi, rop) < scan_cost(ix))) return FBH_PLAN_SCAN;
+    if (ix->enc) return FBH_PLAN_LITERAL;      // fuzz / zona: the option program
     Locate L;
     locate(ix, lo, lop, hi, rop, L);
     const SumPlan sp = sum_plan(ix, L.hit0, L.hit1);
@@ -2093,6 +2220,622 @@ int plan_heuristic(const fbh_index* ix, double lo, int lop, double hi, int rop) 
     if (na > 2 && sum2 < uncomp && bytes * (double)na * na <= std::log(2.0) * uncomp)
         return FBH_PLAN_COMPRESSED;
     return FBH_PLAN_LITERAL;
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// Two-level encodings over a relic: ibis::fuzz (the coarse level interval
+// encoded: bitmap i holds coarse bins [i, i + ceil(nc/2))) and ibis::zona
+// (equality encoded: bitmap i holds coarse bin i), src/irelic.h:540-712.
+// The fine level is the relic itself; the coarse bitmaps form an index of
+// their own (words, tiled copy), and an evaluation is a short postfix
+// program over terms of both levels run by the expression kernel.
+namespace {
+
+// offset64 of the fine level in bytes: the file's positions where the index
+// came with them, else bitvector::bytes() of every bitmap, as coarsen fills
+// it (src/ixfuzz.cpp:166-180, src/ixzona.cpp:130-142)
+std::vector<int64_t> tl_fine_offsets(const fbh_index* ix) {
+    std::vector<int64_t> off(ix->nobs + 1, 0);
+    for (uint32_t b = 0; b < ix->nobs; ++b)
+        off[b + 1] = off[b] + (int64_t)(ix->offsets_built ? 4.0 * (double)(ix->off[b + 1] - ix->off[b]) : ix->bytes(b));
+    return off;
+}
+
+// array_t::find (src/array_t.cpp:507-528): the first position whose value
+// is not less than v
+template <class T>
+uint32_t tl_find(const std::vector<T>& a, T v) {
+    return (uint32_t)(std::lower_bound(a.begin(), a.end(), v) - a.begin());
+}
+
+// The coarse bin boundaries: the fine bitmaps in ncoarse groups of nearly
+// equal bytes, each boundary the fine position nearest its byte target
+// (src/ixfuzz.cpp:215-231, src/ixzona.cpp:146-161; fuzz holds the target in
+// 32 bits)
+std::vector<uint32_t> tl_bounds(const std::vector<int64_t>& off, uint32_t ncoarse, bool target32) {
+    const uint32_t nbits = (uint32_t)off.size() - 1;
+    std::vector<uint32_t> cb(ncoarse + 1, 0);
+    for (uint32_t i = 1; i < ncoarse; ++i) {
+        const int64_t base = off[cb[i - 1]];
+        int64_t t = base + (off.back() - base) / (int64_t)(ncoarse - i + 1);
+        if (target32) t = (int32_t)t;
+        uint32_t k = tl_find(off, t);
+        if (k > cb[i - 1] + 1 && off[k] - t > t - off[k - 1]) --k;
+        else if (k <= cb[i - 1]) k = cb[i - 1] + 1;
+        cb[i] = k;
+    }
+    cb[ncoarse] = nbits;
+    for (uint32_t i = ncoarse - 1; i > 0 && cb[i + 1] < cb[i]; --i) cb[i] = cb[i + 1] - 1;
+    return cb;
+}
+
+// Number of coarse bins fuzz::coarsen picks (src/ixfuzz.cpp:182-210): the
+// "ncoarse=" value when above 4, else the count that balances the coarse
+// level's size against the bytes a query reads; 0 for none
+uint32_t tl_fuzz_ncoarse(const fbh_index* ix, const std::vector<int64_t>& off, uint32_t spec) {
+    uint32_t nc = spec > 4 ? spec : 0;
+    const uint32_t nrows = ix->nrows;
+    if (nc < 5u && off.back() > off[0] + (int64_t)(nrows / 31u)) {
+        const int wm1 = 31;                                       // bits per literal word
+        const long sf = (long)((off.back() - off[0]) / 4);        // fine-level words
+        nc = (unsigned)(wm1 * sf / (std::sqrt(2.0) * nrows));
+        const unsigned ncmax = (unsigned)std::sqrt(2.0 * ix->nobs);
+        if (nc < ncmax) {
+            const double obj1 = (sf + (nc + 1 - std::ceil(0.5 * nc)) * nrows / wm1) * (sf * 0.5 / nc + 2.0 * nrows / wm1);
+            const double obj2 = (sf + (nc + 2 - std::ceil(0.5 * nc + 0.5)) * nrows / wm1) *
+                                (sf * 0.5 / (nc + 1.0) + 2.0 * nrows / wm1);
+            nc += (obj2 < obj1) ? 1u : 0u;
+        } else {
+            nc = ncmax;
+        }
+    }
+    if (nc < 5 || nc >= ix->nobs) return 0;
+    return nc;
+}
+
+// The coarse bitmaps of an index from their serialized words on the host
+// (file layout): an index of their own over the same rows.
+int tl_attach(fbh_index* ix, int enc, std::vector<uint32_t> cbounds, const std::vector<int64_t>& coff_bytes,
+              const std::vector<int64_t>& woff, const uint32_t* words) {
+    fbh_index* c = new fbh_index();
+    c->kind = FBH_RELIC;
+    c->type = ix->type;
+    c->nrows = ix->nrows; c->groups = ix->groups; c->anb = ix->anb;
+    const uint32_t ncb = (uint32_t)woff.size() - 1;
+    int rc = attach_bitmaps(c, woff.data(), ncb, words, (uint64_t)woff.back());
+    if (!rc) rc = finish_index(c);
+    if (rc) { delete c; return rc; }
+    delete ix->coarse;
+    ix->coarse = c;
+    ix->enc = enc;
+    ix->cbounds = std::move(cbounds);
+    ix->tcoff = coff_bytes;
+    return FBH_OK;
+}
+
+// fuzz::coarsen / zona::coarsen (src/ixfuzz.cpp:161-263, src/ixzona.cpp:
+// 124-182): the coarse bins, then each coarse bitmap as the canonical WAH
+// of an OR of consecutive fine bitmaps (the literal kernel, then the
+// canonical compression), its bytes() into coffset64.
+int tl_build(fbh_index* ix, int enc, uint32_t spec) {
+    ix->enc = enc;
+    if (ix->nobs < 32) return FBH_OK;                     // no coarse level (nor offsets) below 32 values
+    ix->toff = tl_fine_offsets(ix);
+    const uint32_t nc = enc == FBH_ENC_FUZZ ? tl_fuzz_ncoarse(ix, ix->toff, spec) : 11u;
+    if (nc == 0) return FBH_OK;
+    std::vector<uint32_t> cb = tl_bounds(ix->toff, nc, enc == FBH_ENC_FUZZ);
+    const uint32_t nc2 = (nc + 1) / 2;
+    const uint32_t ncb = enc == FBH_ENC_FUZZ ? nc - nc2 + 1 : nc;
+    const uint32_t trailer = ix->anb ? 2u : 1u;
+    std::vector<fbh_bitvector*> bvs(ncb, nullptr);
+    std::vector<int64_t> woff(ncb + 1, 0), coff(ncb + 1, 0);
+    int rc = FBH_OK;
+    for (uint32_t i = 0; i < ncb && !rc; ++i) {
+        const uint32_t a = cb[i], b = enc == FBH_ENC_FUZZ ? cb[i + nc2] : cb[i + 1];
+        rc = ranges_bitvector(ix, a, b, 0, 0, 0, &bvs[i]);
+        if (!rc) {
+            woff[i + 1] = woff[i] + bvs[i]->nw + trailer;
+            coff[i + 1] = coff[i] + 4 * (int64_t)bvs[i]->nw + 40;      // bitvector::bytes()
+        }
+    }
+    // the serialized coarse bitmaps, assembled on the host
+    std::vector<uint32_t> words(rc ? 0 : (size_t)woff[ncb]);
+    for (uint32_t i = 0; i < ncb && !rc; ++i) {
+        fbh_bitvector* v = bvs[i];
+        uint32_t* w = words.data() + woff[i];
+        if (v->nw) rc = to_host(w, v->d, v->nw);
+        if (ix->anb) w[v->nw] = v->aval;
+        w[v->nw + trailer - 1] = ix->anb;
+    }
+    for (fbh_bitvector* v : bvs) if (v) fbh_bv_free(v);
+    if (rc) return rc;
+    return tl_attach(ix, enc, std::move(cb), coff, woff, words.data());
+}
+
+
+struct TLBuilder {
+    const fbh_index* ix;
+    TLProgram& P;
+    const std::vector<int64_t>& off;
+    const std::vector<int64_t>& coff;
+    const std::vector<uint32_t>& cb;
+    TLBuilder(const fbh_index* x, TLProgram& p)
+        : ix(x), P(p), off(x->toff), coff(x->tcoff), cb(x->cbounds) {}
+    uint32_t nobs() const { return ix->nobs; }
+    uint32_t ncb() const { return ix->coarse ? ix->coarse->nobs : 0u; }
+
+    void push(const fbk::QueryArgs& q) {
+        P.prog.push_back((int32_t)P.terms.size());
+        P.terms.push_back(q);
+    }
+    void op(int32_t o) { P.prog.push_back(o); }
+    fbk::QueryArgs base(const fbh_index* lvl) const {
+        fbk::QueryArgs q;
+        fill_query(ix, q, 0.0, FBH_OP_NONE, 0.0, FBH_OP_NONE);
+        q.tw = lvl->tw; q.TW = lvl->TW; q.nobs = lvl->nobs;
+        q.type = dev_type(ix->type);
+        q.plan = fbk::PLAN_LIT_D;
+        return q;
+    }
+    void none() { push(base(ix)); }
+    void all() { fbk::QueryArgs q = base(ix); q.flip = 1; push(q); }
+    // fine bitmaps [a, b) straight
+    void fine(uint32_t a, uint32_t b) {
+        fbk::QueryArgs q = base(ix);
+        q.r0 = a; q.r1 = std::min(b, nobs());
+        push(q);
+    }
+    // index::sumBins(a, b): the range or the complement of the rest
+    void sum_bins(uint32_t a, uint32_t b) {
+        fbk::QueryArgs q = base(ix);
+        set_sum_ranges(ix, q, a, b);
+        push(q);
+    }
+    // index::addBins(a, b, res) (src/index.cpp:5537-5580): OR of [a, b),
+    // every row when that is the whole fine level
+    void add_bins(uint32_t a, uint32_t b) {
+        b = std::min(b, nobs());
+        if (a >= b) return;
+        if (a == 0 && b == nobs()) all(); else fine(a, b);
+        op(fbk::EXPR_OR);
+    }
+    void minus_sum(uint32_t a, uint32_t b) { sum_bins(a, b); op(fbk::EXPR_MINUS); }
+    // coarse bitmap i; false when it is null
+    bool have(uint32_t i) const { return i < ncb() && !ix->coarse->null_bitmap(i); }
+    void coarse(uint32_t i) {
+        P.loads |= (i < 64) ? (1ull << i) : 0ull;
+        fbk::QueryArgs q = base(ix->coarse);
+        q.r0 = i; q.r1 = i + 1;
+        push(q);
+    }
+    // OR of coarse bitmaps [a, b), or with flip the rows outside [a0, a1) and
+    // [b0, b1)
+    void coarse_ranges(uint32_t a0, uint32_t a1, uint32_t b0, uint32_t b1, bool flip) {
+        fbk::QueryArgs q = base(ix->coarse);
+        q.r0 = a0; q.r1 = std::max(a0, a1); q.r2 = b0; q.r3 = std::max(b0, b1); q.flip = flip ? 1u : 0u;
+        for (uint32_t i = a0; i < a1 && i < 64; ++i) if (have(i)) P.loads |= 1ull << i;
+        for (uint32_t i = b0; i < b1 && i < 64; ++i) if (have(i)) P.loads |= 1ull << i;
+        push(q);
+    }
+
+    long fine_cost(uint32_t h0, uint32_t h1) const {
+        const long in = (long)(off[h1] - off[h0]);
+        const long out = (long)((off.back() - off[h1]) + (off[h0] - off[0]));
+        return in <= out ? in : out;
+    }
+    long csize(uint32_t i) const { return (long)(coff[i + 1] - coff[i]); }
+    // fuzz::coarseEstimate (src/ixfuzz.cpp:626-674)
+    long fuzz_ccost(uint32_t lo, uint32_t hi) const {
+        const uint32_t mid = (uint32_t)cb.size() / 2;
+        if (lo >= cb.size() || lo >= hi) return 0;
+        if (hi > mid) {
+            long c = csize(hi - mid);
+            if (lo > hi - mid) c += lo >= mid ? csize(lo - mid) : csize(lo);
+            else if (lo < hi - mid) c += csize(lo);
+            return c;
+        }
+        if (hi < mid) return csize(lo) + csize(hi);
+        return csize(0) + (lo > 0 ? csize(lo) : 0);
+    }
+    // fuzz::coarseEvaluate (src/ixfuzz.cpp:676-765): coarse bins [lo, hi)
+    // from one or two interval bitmaps
+    void fuzz_coarse(uint32_t lo, uint32_t hi) {
+        const uint32_t mid = (uint32_t)cb.size() / 2;
+        if (lo >= cb.size() || lo >= hi) { none(); return; }
+        if (lo + 1 == hi) {
+            const uint32_t a = hi < ncb() ? lo : hi - mid, b = hi < ncb() ? hi : lo - mid;
+            if (!have(a)) { none(); return; }
+            coarse(a);
+            if (have(b)) { coarse(b); op(fbk::EXPR_MINUS); }
+            return;
+        }
+        if (hi > mid) {
+            if (have(hi - mid)) coarse(hi - mid); else none();
+            if (lo > hi - mid) {
+                if (lo >= mid) {
+                    if (have(lo - mid)) { coarse(lo - mid); op(fbk::EXPR_MINUS); }
+                } else if (have(lo)) {
+                    coarse(lo); op(fbk::EXPR_AND);
+                } else {
+                    none(); op(fbk::EXPR_AND);
+                }
+            } else if (lo < hi - mid) {
+                if (have(lo)) { coarse(lo); op(fbk::EXPR_OR); }
+            }
+            return;
+        }
+        if (hi < mid) {
+            if (!have(lo)) { none(); return; }
+            coarse(lo);
+            if (have(hi)) { coarse(hi); op(fbk::EXPR_MINUS); }
+            return;
+        }
+        if (have(0)) coarse(0); else none();
+        if (lo > 0 && have(lo)) { coarse(lo); op(fbk::EXPR_AND); }
+    }
+    // zona: the coarse part of a pair (src/ixzona.cpp:886-1010): bitmaps
+    // [a, b) when they hold at most half the coarse bytes, else the rows
+    // outside [lo_end, hi_begin) complemented
+    long zona_part(uint32_t a, uint32_t b) const {
+        const long all = (long)(coff.back() - coff[0]);
+        const long t = (long)(coff[b] - coff[a]);
+        return t + t <= all ? t : all - t;
+    }
+    void zona_coarse(uint32_t a, uint32_t b, bool lo_cond, uint32_t lo_end, bool hi_cond, uint32_t hi_begin,
+                     uint32_t ncoarse) {
+        const long all = (long)(coff.back() - coff[0]);
+        const long t = (long)(coff[b] - coff[a]);
+        if (t + t <= all) coarse_ranges(a, b, 0, 0, false);
+        else coarse_ranges(0, lo_cond ? lo_end : 0, hi_cond ? hi_begin : 0, hi_cond ? ncoarse : 0, true);
+    }
+};
+
+// fuzz::evaluate / zona::evaluate (src/ixfuzz.cpp:872-1095, src/ixzona.cpp:
+// 713-1017) as a program: the option, its coarse part and its fine edges.
+void two_level_program(const fbh_index* ix, double lo, int lop, double hi, int rop, TLProgram& P) {
+    TLBuilder B(ix, P);
+    P = TLProgram();
+    const uint32_t nobs = ix->nobs;
+    Locate L;
+    locate(ix, lo, lop, hi, rop, L);
+    const uint32_t h0 = L.hit0, h1 = L.hit1;
+    const bool fuzz = ix->enc == FBH_ENC_FUZZ;
+    if (nobs == 0 || h1 <= h0 || (fuzz && h0 >= nobs)) { B.none(); return; }
+    if (h0 == 0 && h1 >= nobs) { P.option = FBH_TL_ALL; B.all(); return; }
+    if (h0 + 1 == h1) { P.option = FBH_TL_EQUAL; B.fine(h0, h1); return; }
+    const auto& cb = ix->cbounds;
+    const auto& off = ix->toff;
+    const auto& coff = ix->tcoff;
+    const uint32_t ncoarse = fuzz ? (uint32_t)cb.size() - 1 : std::min<uint32_t>(B.ncb(), (uint32_t)cb.size() - 1);
+    if (h0 + 3 >= h1 || ncoarse == 0 || coff.size() != B.ncb() + 1) {
+        P.option = FBH_TL_FINE; B.sum_bins(h0, h1); return;
+    }
+    const uint32_t c0 = tl_find(cb, h0), c1 = tl_find(cb, h1);
+    auto O = [&](uint32_t i) { return (long)off[i]; };
+    int opt;
+    if (fuzz) {
+        const long finec = B.fine_cost(h0, h1);
+        if (c0 >= c1) {
+            const long t = B.fuzz_ccost(c1 - 1, c1) + (O(h0) - O(cb[c1 - 1])) + (O(cb[c1]) - O(h1));
+            if (finec <= (long)(0.99 * t)) { P.option = FBH_TL_FINE; B.sum_bins(h0, h1); return; }
+            P.option = FBH_TL_SAME_BIN;
+            B.fuzz_coarse(c1 - 1, c1);
+            if (h0 > cb[c1 - 1]) B.minus_sum(cb[c1 - 1], h0);
+            if (cb[c1] > h1) B.minus_sum(h1, cb[c1]);
+            return;
+        }
+        // options 2-5 (src/ixfuzz.cpp:943-1011); option 4's coarse cost
+        // starts at c0-1 here, where estimateCost starts it at c0
+        opt = 2;
+        long cost = B.fuzz_ccost(c0, c1 - 1) + (O(cb[c0]) - O(h0)) + (O(h1) - O(cb[c1 - 1]));
+        long t;
+        if (c0 > 0) {
+            t = B.fuzz_ccost(c0 - 1, c1 - 1) + (O(h0) - O(cb[c0 - 1])) + (O(h1) - O(cb[c1 - 1]));
+            if (t < cost) { cost = t; opt = 3; }
+        }
+        t = B.fuzz_ccost(c0 > 0 ? c0 - 1 : 0, c1) + (O(cb[c0]) - O(h0)) + (O(cb[c1]) - O(h1));
+        if (t < cost) { cost = t; opt = 4; }
+        if (c0 > 0) {
+            t = B.fuzz_ccost(c0 - 1, c1) + (O(h0) - O(cb[c0 - 1])) + (O(cb[c1]) - O(h1));
+            if (t < cost) { cost = t; opt = 5; }
+        }
+        if (cost > (long)(0.99 * finec)) opt = 1;
+        P.option = opt;
+        switch (opt) {
+        case 2:
+            B.fuzz_coarse(c0, c1 - 1);
+            if (h0 < cb[c0]) B.add_bins(h0, cb[c0]);
+            if (cb[c1 - 1] < h1) B.add_bins(cb[c1 - 1], h1);
+            break;
+        case 3:
+            B.fuzz_coarse(c0 - 1, c1 - 1);
+            if (cb[c0 - 1] < h0) B.minus_sum(cb[c0 - 1], h0);
+            if (cb[c1 - 1] < h1) B.add_bins(cb[c1 - 1], h1);
+            break;
+        case 4:
+            B.fuzz_coarse(c0, c1);
+            if (h0 < cb[c0]) B.add_bins(h0, cb[c0]);
+            if (cb[c1] > h1) B.minus_sum(h1, cb[c1]);
+            break;
+        case 5:
+            B.fuzz_coarse(c0 - 1, c1);
+            if (h0 > cb[c0 - 1]) B.minus_sum(cb[c0 - 1], h0);
+            if (cb[c1] > h1) B.minus_sum(h1, cb[c1]);
+            break;
+        default:
+            B.sum_bins(h0, h1);
+        }
+        return;
+    }
+    if (c0 >= c1) {
+        const long fine = O(h1) - O(h0);
+        const long t = (long)(coff[c0] - coff[c0 - 1]) + (O(cb[c0]) - O(cb[c0 - 1])) - fine;
+        if ((long)(0.99 * fine) <= t) { P.option = FBH_TL_FINE; B.sum_bins(h0, h1); return; }
+        P.option = FBH_TL_SAME_BIN;
+        if (B.have(c0 - 1)) B.coarse(c0 - 1); else B.all();
+        if (h0 > cb[c0 - 1]) B.minus_sum(cb[c0 - 1], h0);
+        if (h1 < cb[c0]) B.minus_sum(h1, cb[c0]);
+        return;
+    }
+    // five pairs (src/ixzona.cpp:803-882)
+    opt = 2;
+    long cost = B.zona_part(c0, c1 - 1) + (O(cb[c0]) - O(h0)) + (O(h1) - O(cb[c1 - 1]));
+    long t;
+    if (c0 > 0) {
+        t = B.zona_part(c0 - 1, c1 - 1) + (O(h0) - O(cb[c0 - 1])) + (O(h1) - O(cb[c1 - 1]));
+        if (t < cost) { cost = t; opt = 3; }
+    }
+    t = B.zona_part(c0, c1) + (O(cb[c0]) - O(h0)) + (O(cb[c1]) - O(h1));
+    if (t < cost) { cost = t; opt = 4; }
+    if (c0 > 0) {
+        t = B.zona_part(c0 - 1, c1) + (O(h0) - O(cb[c0 - 1])) + (O(cb[c1]) - O(h1));
+        if (t < cost) { cost = t; opt = 5; }
+    }
+    if (cost > (long)(0.99 * B.fine_cost(h0, h1))) opt = 1;
+    P.option = opt;
+    switch (opt) {
+    case 2:
+        if (c0 < c1 - 1) B.zona_coarse(c0, c1 - 1, c0 > 0, c0, c1 <= ncoarse, c1 - 1, ncoarse);
+        else B.none();
+        if (h0 < cb[c0]) B.add_bins(h0, cb[c0]);
+        if (cb[c1 - 1] < h1) B.add_bins(cb[c1 - 1], h1);
+        break;
+    case 3:
+        B.zona_coarse(c0 - 1, c1 - 1, c0 > 1, c0 - 1, c1 <= ncoarse, c1 - 1, ncoarse);
+        if (cb[c0 - 1] < h0) B.minus_sum(cb[c0 - 1], h0);
+        if (cb[c1 - 1] < h1) B.add_bins(cb[c1 - 1], h1);
+        break;
+    case 4:
+        B.zona_coarse(c0, c1, c0 > 0, c0, c1 < ncoarse, c1, ncoarse);
+        if (h0 < cb[c0]) B.add_bins(h0, cb[c0]);
+        if (cb[c1] > h1) B.minus_sum(h1, cb[c1]);
+        break;
+    case 5:
+        B.zona_coarse(c0 - 1, c1, c0 > 1, c0 - 1, c1 < ncoarse, c1, ncoarse);
+        if (h0 > cb[c0 - 1]) B.minus_sum(cb[c0 - 1], h0);
+        if (cb[c1] > h1) B.minus_sum(h1, cb[c1]);
+        break;
+    default:
+        B.sum_bins(h0, h1);
+    }
+}
+
+// fuzz::estimateCost / zona::estimateCost (src/ixfuzz.cpp:767-870,
+// src/ixzona.cpp:580-711)
+double two_level_cost(const fbh_index* ix, double lo, int lop, double hi, int rop) {
+    TLProgram P;
+    TLBuilder B(ix, P);
+    const uint32_t nobs = ix->nobs;
+    if (nobs == 0 || ix->toff.size() != nobs + 1) return (double)col_esize(ix->type) * ix->nrows;
+    Locate L;
+    locate(ix, lo, lop, hi, rop, L);
+    const uint32_t h0 = L.hit0, h1 = L.hit1;
+    const auto& cb = ix->cbounds;
+    const auto& off = ix->toff;
+    const auto& coff = ix->tcoff;
+    auto O = [&](uint32_t i) { return (long)off[i]; };
+    const bool have_coff = coff.size() == B.ncb() + 1 && B.ncb() > 0;
+    if (ix->enc == FBH_ENC_FUZZ) {
+        if (h1 <= h0 || h0 >= nobs) return 0.0;
+        if (h0 == 0 && h1 >= nobs) return 0.0;
+        const uint32_t ncoarse = cb.empty() ? 0u : (uint32_t)cb.size() - 1;
+        const long fine = B.fine_cost(h0, h1);
+        if (h0 + 3 >= h1 || ncoarse == 0 || !have_coff) return (double)fine;
+        const uint32_t c0 = tl_find(cb, h0), c1 = tl_find(cb, h1);
+        if (c0 >= c1) {
+            const long t = B.fuzz_ccost(c1 - 1, c1) + (O(h0) - O(cb[c1 - 1])) + (O(cb[c1]) - O(h1));
+            return (double)(t / 100 >= fine / 99 ? fine : t);
+        }
+        long cost = B.fuzz_ccost(c0, c1 - 1) + (O(cb[c0]) - O(h0)) + (O(h1) - O(cb[c1 - 1]));
+        if (c0 > 0)
+            cost = std::min(cost, B.fuzz_ccost(c0 - 1, c1 - 1) + (O(h0) - O(cb[c0 - 1])) + (O(h1) - O(cb[c1 - 1])));
+        cost = std::min(cost, B.fuzz_ccost(c0, c1) + (O(cb[c0]) - O(h0)) + (O(cb[c1]) - O(h1)));
+        if (c0 > 0)
+            cost = std::min(cost, B.fuzz_ccost(c0 - 1, c1) + (O(h0) - O(cb[c0 - 1])) + (O(cb[c1]) - O(h1)));
+        return (double)(cost / 100 >= fine / 99 ? fine : cost);
+    }
+    if (h1 <= h0) return 0.0;
+    if (h0 == 0 && h1 >= nobs) return 0.0;
+    const long fine = B.fine_cost(h0, h1);
+    const uint32_t ncoarse = cb.empty() ? 0u : std::min<uint32_t>(B.ncb(), (uint32_t)cb.size() - 1);
+    if (h0 + 1 == h1) return (double)fine;
+    if (h0 + 3 >= h1 || !have_coff) return (double)fine;
+    const uint32_t c0 = tl_find(cb, h0), c1 = tl_find(cb, h1);
+    if (c0 >= c1) {
+        const long t = (long)(coff[c0] - coff[c0 - 1]) + (O(cb[c0]) - O(cb[c0 - 1])) - fine;
+        return (double)((long)(0.99 * fine) >= t ? t : fine);
+    }
+    if (c1 < ncoarse && cb[c1] == h1 && cb[c0] == h0) return (double)B.zona_part(c0, c1);
+    long cost = B.zona_part(c0, c1 - 1) + (O(cb[c0]) - O(h0)) + (O(h1) - O(cb[c1 - 1]));
+    if (c0 > 0)
+        cost = std::min(cost, B.zona_part(c0 - 1, c1 - 1) + (O(h0) - O(cb[c0 - 1])) + (O(h1) - O(cb[c1 - 1])));
+    cost = std::min(cost, B.zona_part(c0, c1) + (O(cb[c0]) - O(h0)) + (O(cb[c1]) - O(h1)));
+    if (c0 > 0)
+        cost = std::min(cost, B.zona_part(c0 - 1, c1) + (O(h0) - O(cb[c0 - 1])) + (O(cb[c1]) - O(h1)));
+    if (cost > (long)(0.99 * fine)) cost = fine;
+    return (double)cost;
+}
+
+// Stack depth and short-circuit table of generated programs (the rules of
+// fbh_expr_prepare: the right operand of AND / MINUS is skipped on tiles
+// where the left one is empty).
+void program_tables(const std::vector<int32_t>& prog, const std::vector<uint32_t>& poff, uint32_t nq,
+                    std::vector<int32_t>& skip, uint32_t& depth) {
+    skip.assign(prog.size(), -1);
+    depth = 0;
+    for (uint32_t q = 0; q < nq; ++q) {
+        uint32_t start[fbk::EXPR_DEPTH];
+        int sp = 0;
+        for (uint32_t p = poff[q]; p < poff[q + 1]; ++p) {
+            const int32_t o = prog[p];
+            if (o >= 0) { start[sp++] = p; depth = std::max(depth, (uint32_t)sp); }
+            else if (o != fbk::EXPR_NOT) {
+                if (o == fbk::EXPR_AND || o == fbk::EXPR_MINUS) skip[start[sp - 1]] = (int32_t)(p + 1);
+                --sp;
+            }
+        }
+    }
+}
+
+// A batch of range queries on a two-level index as one expression batch:
+// each query the program of its option (queries under the scan plan: one
+// scan term).  plans_used receives each query's plan.
+int two_level_batch(fbh_index* ix, uint32_t nq, const double* lo, const int* lop, const double* hi,
+                    const int* rop, const int* plans, int plan, fbh_conj** out, std::vector<int>* plans_used,
+                    std::vector<TLProgram>* progs) {
+    std::vector<fbk::QueryArgs> h;
+    std::vector<int32_t> prog;
+    std::vector<uint32_t> toff(nq + 1, 0), poff(nq + 1, 0);
+    std::vector<int> used(nq);
+    if (progs) progs->assign(nq, TLProgram());
+    for (uint32_t i = 0; i < nq; ++i) {
+        int p = plans ? plans[i] : plan;
+        if (p == FBH_PLAN_AUTO) p = plan_heuristic(ix, lo[i], lop[i], hi[i], rop[i]);
+        if (p != FBH_PLAN_SCAN) p = FBH_PLAN_LITERAL;
+        used[i] = p;
+        TLProgram P;
+        if (p == FBH_PLAN_SCAN) {
+            fbk::QueryArgs q;
+            term_args(fbh_term{ix, lo[i], lop[i], hi[i], rop[i]}, FBH_PLAN_SCAN, q);
+            P.terms.push_back(q);
+            P.prog.push_back(0);
+        } else {
+            two_level_program(ix, lo[i], lop[i], hi[i], rop[i], P);
+        }
+        for (int32_t o : P.prog) prog.push_back(o >= 0 ? (int32_t)(h.size() + (size_t)o) : o);
+        for (const auto& q : P.terms) h.push_back(q);
+        toff[i + 1] = (uint32_t)h.size();
+        poff[i + 1] = (uint32_t)prog.size();
+        if (progs) (*progs)[i] = std::move(P);
+    }
+    std::vector<int32_t> skip;
+    uint32_t depth = 0;
+    program_tables(prog, poff, nq, skip, depth);
+    if (plans_used) *plans_used = used;
+    fbh_conj* c = new fbh_conj();
+    c->nq = nq; c->nterms = (uint32_t)h.size(); c->ntiles = ix->ntiles; c->groups = ix->groups; c->anb = ix->anb;
+    c->depth = depth;
+    c->plans = used;                         // the caller's terms are the queries
+    c->order.resize(c->nterms);
+    for (uint32_t k = 0; k < c->nterms; ++k) c->order[k] = k;
+    c->dskip = (int32_t*)dev_alloc(4ull * prog.size());
+    int rc = c->dskip ? to_dev(c->dskip, skip.data(), prog.size()) : fail(FBH_ERR_NOMEM, "device allocation failed");
+    if (!rc) rc = conj_upload(c, h, toff.data(), prog.data(), poff.data(), (uint32_t)prog.size());
+    if (rc) { delete c; return rc; }
+    *out = c;
+    return FBH_OK;
+}
+
+
+// The coarse section of a fuzz / zona index file, after the relic part
+// ending at byte `pos` (src/ixfuzz.cpp:46-143, src/ixzona.cpp:40-100): nc,
+// cbounds[nc+1], the coarse bitmaps' positions (offset width w), the
+// bitmaps.  Checked on the host like the fine level.
+int tl_read(fbh_index* ix, int enc, const uint8_t* file, size_t n, size_t pos, size_t w) {
+    auto u32 = [&](size_t at) { uint32_t v; memcpy(&v, file + at, 4); return v; };
+    if (pos + 4 > n) return fail(FBH_ERR_ARG, "truncated coarse level");
+    const uint32_t nc = u32(pos);
+    if (nc == 0 || nc > ix->nobs) return fail(FBH_ERR_ARG, "bad coarse bin count");
+    const uint32_t ncb = enc == FBH_ENC_FUZZ ? nc - (nc + 1) / 2 + 1 : nc;
+    const size_t cbeg = pos + 4, pbeg = cbeg + 4ull * (nc + 1), pend = pbeg + w * (ncb + 1ull);
+    if (pend > n) return fail(FBH_ERR_ARG, "truncated coarse level");
+    std::vector<uint32_t> cb(nc + 1);
+    for (uint32_t i = 0; i <= nc; ++i) cb[i] = u32(cbeg + 4ull * i);
+    if (cb[0] != 0 || cb[nc] != ix->nobs) return fail(FBH_ERR_ARG, "coarse bins do not cover the bitmaps");
+    for (uint32_t i = 0; i < nc; ++i)
+        if (cb[i + 1] <= cb[i]) return fail(FBH_ERR_ARG, "coarse bins are not ascending");
+    std::vector<int64_t> cpos(ncb + 1), woff(ncb + 1);
+    for (uint32_t i = 0; i <= ncb; ++i) {
+        if (w == 8) memcpy(&cpos[i], file + pbeg + 8ull * i, 8);
+        else { int32_t v; memcpy(&v, file + pbeg + 4ull * i, 4); cpos[i] = v; }
+    }
+    if (cpos[0] < (int64_t)pend || cpos[0] % 4 != 0 || (uint64_t)cpos[ncb] > n)
+        return fail(FBH_ERR_ARG, "coarse bitmap positions outside the file");
+    for (uint32_t i = 0; i <= ncb; ++i) {
+        if (cpos[i] < cpos[0] || (cpos[i] - cpos[0]) % 4 != 0 || (i && cpos[i] < cpos[i - 1]))
+            return fail(FBH_ERR_ARG, "bad coarse bitmap position");
+        woff[i] = (cpos[i] - cpos[0]) / 4;
+    }
+    int rc = tl_attach(ix, enc, std::move(cb), cpos, woff, (const uint32_t*)(file + cpos[0]));
+    if (rc) return rc;
+    ix->toff = tl_fine_offsets(ix);
+    return FBH_OK;
+}
+
+// Bitmap words a program reads (each term's OR set on its level) plus the
+// literal result.
+double tl_program_bytes(const fbh_index* ix, const TLProgram& P) {
+    double w = 4.0 * (ix->groups + 1);
+    for (const fbk::QueryArgs& q : P.terms) {
+        const fbh_index* lvl = (ix->coarse && q.tw == ix->coarse->tw) ? ix->coarse : ix;
+        auto words = [&](uint32_t b0, uint32_t b1) {
+            b1 = std::min(b1, lvl->nobs);
+            return b1 > b0 ? lvl->cumser[b1] - lvl->cumser[b0] : 0.0;
+        };
+        w += words(q.r0, q.r1) + words(q.r2, q.r3);
+    }
+    return w;
+}
+
+double tl_query_bytes(const fbh_index* ix, double lo, int lop, double hi, int rop) {
+    TLProgram P;
+    two_level_program(ix, lo, lop, hi, rop, P);
+    return tl_program_bytes(ix, P);
+}
+
+// One range on a two-level index through a one-query expression batch:
+// the count, the result bitvector and / or the first `cap` row ids.
+int tl_evaluate(fbh_index* ix, double lo, int lop, double hi, int rop, uint64_t* hits, fbh_bitvector** result,
+                uint32_t* rids, uint64_t cap) {
+    fbh_conj* c = nullptr;
+    const int plan = FBH_PLAN_LITERAL;
+    int rc = two_level_batch(ix, 1, &lo, &lop, &hi, &rop, nullptr, plan, &c, nullptr, nullptr);
+    if (rc) return rc;
+    uint64_t off[2] = {0, 0};
+    rc = fbh_conj_rids(c, off, rids, rids ? cap : 0);
+    if (!rc && hits) *hits = off[1];
+    if (!rc && result) rc = fbh_conj_result(c, 0, result);
+    delete c;
+    return rc;
+}
+
+int tl_batch_counts(fbh_index* ix, uint32_t nq, const double* lo, const int* lop, const double* hi,
+                    const int* rop, const int* plans, int plan, uint64_t* hits, int* plans_used, float* gpu_ms) {
+    fbh_conj* c = nullptr;
+    std::vector<int> used;
+    int rc = two_level_batch(ix, nq, lo, lop, hi, rop, plans, plan, &c, &used, nullptr);
+    if (rc) return rc;
+    rc = fbh_conj_run(c, hits, gpu_ms);
+    if (!rc && plans_used) for (uint32_t i = 0; i < nq; ++i) plans_used[i] = used[i];
+    delete c;
+    return rc;
+}
+
+int tl_batch_prepare(fbh_index* ix, uint32_t nq, const double* lo, const int* lop, const double* hi,
+                     const int* rop, const int* plans, int plan, fbh_conj** out) {
+    return two_level_batch(ix, nq, lo, lop, hi, rop, plans, plan, out, nullptr, nullptr);
 }
 
 } // namespace

@@ -261,6 +261,9 @@ RELIC, BIN = 1, 2
 COL_INT32, COL_UINT32, COL_UINT16, COL_FLOAT, COL_DOUBLE = 0, 1, 2, 3, 4
 OP_NONE, OP_LT, OP_LE = 0, 1, 2
 PLAN_AUTO, PLAN_LITERAL, PLAN_COMPRESSED, PLAN_SCAN = 0, 1, 2, 3
+# two-level encodings and what an evaluation on one did (fastbit_hip.h)
+ENC_FUZZ, ENC_ZONA = 1, 2
+TL_EMPTY, TL_FINE, TL_ALL, TL_EQUAL, TL_SAME_BIN = 0, 1, 6, 7, 8
 PLAN_NAMES = {0: "auto", 1: "literal", 2: "compressed", 3: "scan"}
 COST_IAPI, COST_PARTITION = 0, 1
 _COLT = {np.dtype(np.int32): COL_INT32, np.dtype(np.uint32): COL_UINT32,
@@ -304,6 +307,11 @@ def _index_sigs(L):
         "fbh_index_attach": [vp, C.c_int, C.c_uint32, _dp, C.c_uint64, _i64p, C.c_uint64, _u32p,
                              C.c_uint64, C.POINTER(vp)],
         "fbh_index_read_file": [C.c_char_p, vp, C.c_int, C.c_uint32, C.POINTER(vp)],
+        "fbh_index_build_fuzz": [vp, C.c_int, C.c_uint32, C.c_uint32, C.POINTER(vp)],
+        "fbh_index_build_zona": [vp, C.c_int, C.c_uint32, C.POINTER(vp)],
+        "fbh_index_coarse": [vp, _ip, _u32p, _u32p, C.c_uint32, _i64p, C.c_uint32, _u32p],
+        "fbh_index_coarse_bitmap": [vp, C.c_uint32, C.POINTER(vp)],
+        "fbh_index_two_level_option": [vp, C.c_double, C.c_int, C.c_double, C.c_int, _ip, _u64p],
         "fbh_index_sum_bins": [vp, C.c_uint32, C.c_uint32, _u64p],
         "fbh_index_plan_features": [vp, C.c_uint32, _dp, _ip, _dp, _ip, _fp],
         "fbh_index_time_plans": [vp, C.c_uint32, _dp, _ip, _dp, _ip, C.c_float, _fp, _u64p],
@@ -399,6 +407,52 @@ class Index:
                                  k.size, o.ctypes.data_as(_i64p), o.size, b.ctypes.data_as(_u32p), b.size,
                                  C.byref(h)), "fbh_index_attach")
         return cls(h)
+
+    @classmethod
+    def fuzz(cls, col, ncoarse=0):
+        """ibis::fuzz over a column: the relic and an interval-encoded coarse
+        level (ncoarse: the index spec's "ncoarse=", 0 for the default)."""
+        L = cls._lib()
+        a = np.ascontiguousarray(col)
+        h = C.c_void_p()
+        check(L.fbh_index_build_fuzz(a.ctypes.data_as(C.c_void_p), _COLT[a.dtype], a.size, int(ncoarse),
+                                     C.byref(h)), "fbh_index_build_fuzz")
+        return cls(h)
+
+    @classmethod
+    def zona(cls, col):
+        """ibis::zona over a column: the relic and 11 equality-encoded coarse
+        bins."""
+        L = cls._lib()
+        a = np.ascontiguousarray(col)
+        h = C.c_void_p()
+        check(L.fbh_index_build_zona(a.ctypes.data_as(C.c_void_p), _COLT[a.dtype], a.size, C.byref(h)),
+              "fbh_index_build_zona")
+        return cls(h)
+
+    def coarse(self):
+        """(encoding, cbounds, coarse byte offsets, number of coarse bitmaps)."""
+        L = self._lib()
+        enc, nc, nb = C.c_int(), C.c_uint32(), C.c_uint32()
+        check(L.fbh_index_coarse(self._h, C.byref(enc), C.byref(nc), None, 0, None, 0, C.byref(nb)),
+              "fbh_index_coarse")
+        cb = np.zeros(nc.value + 1, np.uint32)
+        co = np.zeros(nb.value + 1, np.int64)
+        check(L.fbh_index_coarse(self._h, None, None, cb.ctypes.data_as(_u32p), cb.size if nc.value else 0,
+                                 co.ctypes.data_as(_i64p), co.size if nb.value else 0, None), "fbh_index_coarse")
+        return enc.value, (cb if nc.value else cb[:0]), (co if nb.value else co[:0]), nb.value
+
+    def coarse_bitmap(self, i):
+        h = C.c_void_p()
+        check(self._lib().fbh_index_coarse_bitmap(self._h, int(i), C.byref(h)), "fbh_index_coarse_bitmap")
+        return Bitvector(h)
+
+    def two_level_option(self, lo, lop, hi, rop):
+        """(option, mask of the coarse bitmaps read) of a fuzz / zona range."""
+        opt, ld = C.c_int(), C.c_uint64()
+        check(self._lib().fbh_index_two_level_option(self._h, lo, lop, hi, rop, C.byref(opt), C.byref(ld)),
+              "fbh_index_two_level_option")
+        return opt.value, ld.value
 
     @classmethod
     def read_file(cls, path, col):

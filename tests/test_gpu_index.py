@@ -73,6 +73,26 @@ def test_relic_values(L, dtype):
             assert np.array_equal(bv.rids(), r)
 
 
+def test_relic_float_signed_zeros(L):
+    """-0.0 and +0.0 are one value of index::mapValues' std::map, keyed by
+    the zero that comes first (both orders)."""
+    rng = np.random.default_rng(11)
+    for first in (-0.0, 0.0):
+        col = (np.round(rng.standard_normal(31 * 500 + 7) * 4.0) / 2.0).astype(np.float32)
+        col[col == 0] = np.float32(0.0)
+        z = np.nonzero(col == 0)[0]
+        col[z[::2]] = np.float32(-0.0)
+        col[z[0]] = np.float32(first)
+        gi = L.Index.relic(col)
+        oi = O.OracleIndex.relic_values(col)
+        _same_words(gi, oi)
+        keys = gi.deconstruct()[0]
+        k0 = keys[keys == 0]
+        assert k0.size == 1 and np.signbit(k0[0]) == np.signbit(first)
+        c, r = oi.evaluate(-0.5, 2, 0.0, 2)
+        assert gi.evaluate(-0.5, 2, 0.0, 2, 1)[0] == c == int(((col >= -0.5) & (col <= 0)).sum())
+
+
 @pytest.mark.parametrize("n", [1000, 31 * 1000 + 5, 300001])
 def test_bin_float(L, n):
     rng = np.random.default_rng(n)
