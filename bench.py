@@ -239,6 +239,55 @@ def conj_section(L, log2n, nq, steps, learned=None, cpu=None):
     return out, rel
 
 
+def two_level_section(L, log2n, nq, steps):
+    """The same ranges on a relic and on its two-level forms: an int32
+    column of 2^log2n rows over 10,000 values (uniform), nq ranges
+    lo <= x < hi with lo uniform and width log-uniform in [1, 10000].  The
+    relic batch runs the range kernel (each query ORs its bitmap range or
+    the complement); the fuzz and zona batches run each query's option
+    program (one or two coarse bitmaps and the fine bitmaps at the edges)
+    in the expression kernel.  All three are autotuned graph replays with
+    the same counts, checked against the scan plan."""
+    card = 10000
+    rng = np.random.default_rng(9101)
+    col = rng.integers(0, card, size=1 << log2n).astype(np.int32)
+    lo, hi = relic_ranges(9102, nq, card)
+    res_bytes = 4.0 * ((1 << log2n) // 31 + 1)
+    out = dict(rows=1 << log2n, values=card, queries=nq)
+    ref = None
+    for name, make in (("relic", lambda: L.Index.relic(col)), ("fuzz", lambda: L.Index.fuzz(col)),
+                       ("zona", lambda: L.Index.zona(col))):
+        t = time.time()
+        ix = make()
+        L.lib().fbh_sync()
+        build_s = time.time() - t
+        b = L.RangeBatch(ix, lo, 2, hi, 1, plan=1)
+        cfg = b.tune()
+        b.run()
+        ms = []
+        for _ in range(steps):
+            h, m = b.run()
+            ms.append(m)
+        if ref is None:
+            ref, _ = L.RangeBatch(ix, lo, 2, hi, 1, plan=3).run()
+        if not np.array_equal(h, ref):
+            raise RuntimeError(f"{name} counts differ from the scan plan's")
+        qb = sum(ix.query_bytes(float(a), 2, float(c), 1, 1) for a, c in zip(lo, hi)) - res_bytes * nq
+        gm = float(np.median(ms))
+        r = dict(qps=round(nq / (gm * 1e-3), 1), ms_per_batch=round(gm, 3), build_s=round(build_s, 2),
+                 algo_GB=round(qb / 1e9, 3), GBs=round(qb / (gm * 1e-3) / 1e9, 1), tuned=cfg)
+        if name != "relic":
+            enc, cb, coff, nb = ix.coarse()
+            opts = [ix.two_level_option(float(a), 2, float(c), 1)[0] for a, c in zip(lo, hi)]
+            r.update(coarse_bins=int(cb.size - 1), coarse_bitmaps=int(nb),
+                     options={str(k): int(v) for k, v in zip(*np.unique(opts, return_counts=True))})
+        out[name] = r
+        log("two-level", name, r)
+        del b, ix
+    out["hits_total"] = int(ref.sum())
+    return out
+
+
 def learned_section(L, ix, train, test, epochs):
     """The learned plan choice on the same index: every query of a training
     set and of a test set timed under the three plans by the engine, the
@@ -485,6 +534,9 @@ def main():
     ap.add_argument("--conj-only", action="store_true",
                     help="run only the conjunction / expression batches (no CPU baseline) and print them as JSON")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-two-level", action="store_true")
+    ap.add_argument("--two-level-only", action="store_true",
+                    help="only the relic / fuzz / zona comparison at 2^26 rows")
     ap.add_argument("--train-queries", type=int, default=50000)
     ap.add_argument("--test-queries", type=int, default=8192)
     ap.add_argument("--epochs", type=int, default=40)
@@ -499,6 +551,9 @@ def main():
     L.lib()
     if args.wah_only:
         print(json.dumps({"wah_ops_2p28": wah_cells(L, reps=args.wah_only)}))
+        return
+    if args.two_level_only:
+        print(json.dumps({"two_level_2p26": two_level_section(L, 26, 1024, args.steps)}))
         return
     if args.conj_only:
         conj, _ = conj_section(L, 26, 4096, args.steps)
@@ -594,6 +649,7 @@ def main():
     conj, relic_learned = (None, None) if args.no_conj else conj_section(
         L, 26, 4096, 5, None if args.no_learned else (args.train_queries, args.test_queries, args.epochs),
         cpu=None if args.no_cpu else (32, workers))
+    two_level = None if args.no_two_level else two_level_section(L, 26, 1024, 5)
     cpu = None if args.no_cpu else cpu_baseline(ix, col, lo, hi, hits, args.cpu_sample, workers)
     cpu_wah = None if (args.no_cpu or args.no_wah) else cpu_baseline_wah(
         [(0.5, 1), (0.5, 64), (0.01, 1)])
@@ -627,6 +683,7 @@ def main():
         "learned_plan_relic_2p26": relic_learned,
         "wah_ops_2p28": wah,
         "conjunctions_4x2p26": conj,
+        "two_level_2p26": two_level,
         "cpu_baseline": cpu,
         "cpu_baseline_wah": cpu_wah,
     }

@@ -135,6 +135,17 @@ public:
         check(fbh_index_build_bin_nbins(col, type, nrows, nbins, &ix.h_), "fbh_index_build_bin_nbins");
         return ix;
     }
+    /* ibis::fuzz / ibis::zona: the relic plus a coarse level */
+    static index fuzz(const void* col, int type, uint32_t nrows, uint32_t ncoarse = 0) {
+        index ix;
+        check(fbh_index_build_fuzz(col, type, nrows, ncoarse, &ix.h_), "fbh_index_build_fuzz");
+        return ix;
+    }
+    static index zona(const void* col, int type, uint32_t nrows) {
+        index ix;
+        check(fbh_index_build_zona(col, type, nrows, &ix.h_), "fbh_index_build_zona");
+        return ix;
+    }
     /* fastbit_iapi_attach_full_index */
     static index attach(const void* col, int type, uint32_t nrows, const index_arrays& a) {
         index ix;

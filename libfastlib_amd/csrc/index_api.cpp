@@ -1639,6 +1639,7 @@ struct fbh_conj {
     int nt = 256;                        // threads per block
     int single_type = -1;                // >= 0: single-term queries on one column type
     int has_scan = 0;                    // 0 no term runs the scan plan, 1 some, 2 all
+    bool lit_only = false;               // literal terms only, no edge bitmaps
     uint32_t blocks = 0;                 // persistent grid
     uint32_t tpi = 1;                    // tiles per work item
     hipGraph_t graph = nullptr;
@@ -1704,6 +1705,10 @@ int conj_upload(fbh_conj* c, const std::vector<fbk::QueryArgs>& h, const uint32_
     if (!rc && prog) rc = to_dev(c->dprog, prog, nprog);
     if (!rc && prog) rc = to_dev(c->dpoff, poff, nq + 1);
     if (!rc) rc = stream_sync();
+    c->lit_only = true;
+    for (uint32_t k = 0; k < c->nterms; ++k)
+        c->lit_only = c->lit_only && h[k].plan == fbk::PLAN_LIT_D && h[k].cand0 == fbk::NOKEY &&
+                      h[k].cand1 == fbk::NOKEY;
     // single-term queries on one column type run the persistent range
     // kernel: 256 threads, 2 tiles per work item, 9 blocks per CU (the
     // tuner's choice on 2^27 rows with 10,000 bins; all resident by LDS)
@@ -1734,6 +1739,7 @@ fbk::ConjArgs conj_args(const fbh_conj* c, uint32_t q0, uint32_t nq) {
     a.out = nullptr;
     a.prog = c->dprog; a.poff = c->dpoff ? c->dpoff + q0 : nullptr; a.depth = c->depth; a.skip = c->dskip;
     a.rids = nullptr; a.rbase = nullptr; a.roff = nullptr;
+    a.lit_only = c->lit_only ? 1u : 0u;
     return a;
 }
 

@@ -81,6 +81,9 @@ struct ConjArgs {
     uint32_t* rids;
     const uint64_t* rbase;
     const uint32_t* roff;
+    // every term runs the literal plan with no edge bitmaps (relic and
+    // two-level terms): the expression kernel takes its lighter form
+    uint32_t lit_only;
 };
 
 constexpr int EXPR_DEPTH = 4;

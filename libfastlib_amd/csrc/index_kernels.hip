@@ -814,17 +814,24 @@ __device__ __forceinline__ void stream_words(const QueryArgs& q, uint32_t ws, ui
     }
 }
 
+// A single bitmap whose segment holds one word per group (literal words
+// and count-1 fills: a dense bitmap, such as a coarse bitmap of a fuzz
+// index) is copied by the whole block, each thread owning groups
+// k = tid (mod NT), with no atomics and no coverage pass; the streaming
+// path would give it to one wave.
+
 // The OR set of the tile: stream 1 = the segments of bitmaps [r0, r1),
 // stream 2 = those of [r2, r3).  The block reads the starts of up to CT_MAX
 // chunks of consecutive bitmaps into ctab; wave w streams the chunks whose
 // start lies in the w-th quarter of the words.  Called by every thread of
-// the block.
-template <class T, bool POW2, int NT, int U>
-__device__ void tile_ranges(const QueryArgs& q, uint32_t tile, uint32_t g0, uint32_t ng, word_t* R,
+// the block; returns true when the dense form above did the work (no
+// 1-fill went to cov).
+template <class T, bool POW2, int NT, int U, bool DENSE>
+__device__ bool tile_ranges(const QueryArgs& q, uint32_t tile, uint32_t g0, uint32_t ng, word_t* R,
                             uint32_t* cov, const T* col, uint32_t* ctab) {
     constexpr int NW = NT / WAVE;
     const uint32_t len1 = q.r1 - q.r0, len2 = q.r3 - q.r2;
-    if (len1 + len2 == 0) return;
+    if (len1 + len2 == 0) return true;
     const uint32_t cb = max(16u, (len1 + len2 + CT_MAX - 1) / CT_MAX);   // bitmaps per chunk
     const uint32_t n1 = (len1 + cb - 1) / cb, n2 = (len2 + cb - 1) / cb;
     const uint32_t nc = n1 + n2;                                        // <= CT_MAX + 2
@@ -841,6 +848,14 @@ __device__ void tile_ranges(const QueryArgs& q, uint32_t tile, uint32_t g0, uint
     __syncthreads();
     const uint32_t s1 = st[0], e1 = st[1], s2 = st[2], e2 = st[3];
     const uint32_t W1 = e1 - s1, W = W1 + (e2 - s2);
+    if (DENSE && len1 + len2 == 1 && W == ng) {
+        const uint32_t s = len1 ? s1 : s2;
+        for (uint32_t i = threadIdx.x; i < ng; i += NT) {
+            const word_t w = q.tw[s + i];
+            R[i] = w < HEADER0 ? w : (w >= HEADER1 ? ALLONES : 0u);
+        }
+        return true;
+    }
     // word offset of chunk k in the concatenated streams
     auto cstart = [&](uint32_t k) -> uint32_t {
         if (k >= nc) return W;
@@ -858,7 +873,7 @@ __device__ void tile_ranges(const QueryArgs& q, uint32_t tile, uint32_t g0, uint
     const uint32_t w = threadIdx.x / WAVE;
     const uint32_t ka = w == 0 ? 0u : first_ge((uint32_t)(((uint64_t)W * w) / NW));
     const uint32_t kb = w + 1 == NW ? nc : first_ge((uint32_t)(((uint64_t)W * (w + 1)) / NW));
-    if (ka >= kb) return;
+    if (ka >= kb) return false;
     // the piece in stream 1, then the piece in stream 2
     if (ka < n1) {
         const uint32_t we = kb >= n1 ? e1 : ctab[kb];
@@ -869,23 +884,28 @@ __device__ void tile_ranges(const QueryArgs& q, uint32_t tile, uint32_t g0, uint
         const uint32_t we = kb >= nc ? e2 : ctab[kb];
         stream_words<POW2, U>(q, ws, we, ng, R, cov);
     }
+    return false;
 }
 
 // Literal plan of one term over tile t into R[0, ng), by a block of NT
 // threads.  R and cov are zero on entry; the call ends with a
-// barrier.  red needs NT/64 + 1 words.
-template <class T, int NT, int U>
+// barrier.  red needs NT/64 + 1 words.  DENSE enables the block-wide form
+// for small dense OR sets (off in the range-batch kernel, where its
+// registers would cost a wave per SIMD).
+template <class T, int NT, int U, bool DENSE = true, bool EDGES = true>
 __device__ __forceinline__ void lit_tile(const QueryArgs& q, uint32_t tile, uint32_t g0, uint32_t ng, bool last, word_t* R,
                          uint32_t* cov, uint32_t* red, uint32_t* ctab) {
     const T* col = (const T*)q.col;
     const int w = threadIdx.x / WAVE;
+    bool dense = true;
     if (ng) {
-        if (last) tile_ranges<T, false, NT, U>(q, tile, g0, ng, R, cov, col, ctab);
-        else tile_ranges<T, true, NT, U>(q, tile, g0, ng, R, cov, col, ctab);
+        if (last) dense = tile_ranges<T, false, NT, U, DENSE>(q, tile, g0, ng, R, cov, col, ctab);
+        else dense = tile_ranges<T, true, NT, U, DENSE>(q, tile, g0, ng, R, cov, col, ctab);
     }
     __syncthreads();
-    // coverage prefix sums: groups under some 1-fill are all ones
-    {
+    // coverage prefix sums: groups under some 1-fill are all ones (none
+    // when the dense form ran: it is the same for every thread)
+    if (!DENSE || !dense) {
         constexpr int PER = (QT + NT - 1) / NT;
         const uint32_t k0 = threadIdx.x * PER;
         uint32_t c[PER], sum = 0;
@@ -908,7 +928,7 @@ __device__ __forceinline__ void lit_tile(const QueryArgs& q, uint32_t tile, uint
         }
         __syncthreads();
     }
-    if (w < 2 && ng) {
+    if (EDGES && w < 2 && ng) {
         const uint32_t cb = w == 0 ? q.cand0 : q.cand1;
         if (cb != NOKEY) {
             const uint32_t* TWt = q.TW + (size_t)tile * q.nobs;
@@ -1097,9 +1117,16 @@ __global__ __launch_bounds__(NT) void k_query_wah(QueryArgs q, WahOut o) {
 // every further term into R2 and R &= R2, and stops early once the tile is
 // empty.  Each term runs the plan chosen for it on the host (literal
 // through the tiled copy of its index, or the raw scan of its column).
-template <int NT>
+template <int NT, bool LIT = false>
 __device__ __forceinline__ void term_tile(const QueryArgs& q, uint32_t tile, uint32_t g0, uint32_t ng, bool last, word_t* R,
                           uint32_t* cov, uint32_t* red, uint64_t (*bal)[32], uint32_t* ctab) {
+    if (LIT) {
+        // literal plan, no edge bitmaps: the column type does not matter
+        for (uint32_t k = threadIdx.x; k <= QT; k += NT) { R[k] = 0u; cov[k] = 0u; }
+        __syncthreads();
+        lit_tile<uint32_t, NT, CJ_U, true, false>(q, tile, g0, ng, last, R, cov, red, ctab);
+        return;
+    }
     if (q.plan == PLAN_SCAN_D) {
         switch (q.type) {
         case COL_I32: scan_tile<int32_t, NT>(q, g0, ng, R, bal); break;
@@ -1280,7 +1307,7 @@ __global__ __launch_bounds__(NT) void k_range(ConjArgs a) {
             } else {
                 for (uint32_t k = threadIdx.x; k <= QT; k += NT) { R[k] = 0u; cov[k] = 0u; }
                 __syncthreads();
-                lit_tile<T, NT, QB_U>(q, tile, g0, ng, last, R, cov, red, ctab);
+                lit_tile<T, NT, QB_U, false>(q, tile, g0, ng, last, R, cov, red, ctab);
             }
             tile_epilogue<NT>(a, qi, tile, g0, ng, R, red);
         }
@@ -1293,12 +1320,12 @@ __global__ __launch_bounds__(NT) void k_range(ConjArgs a) {
 // LDS.  NOT complements within the rows (an active group keeps only its
 // nbits), NAND / NOR are NOT of AND / OR as fastbit_selection_combine
 // builds them (src/iapi.cpp:1313-1326), MINUS is AND NOT.
-template <int NT, int D>
+template <int NT, int D, bool LIT = false>
 __global__ __launch_bounds__(NT) void k_expr(ConjArgs a) {
     __shared__ word_t S[D][QT + 1];      // D >= a.depth (the launcher's choice)
     __shared__ uint32_t cov[QT + 1];
     __shared__ uint32_t red[NT / WAVE + 1];
-    __shared__ uint64_t bal[NT / WAVE][32];
+    __shared__ uint64_t bal[LIT ? 1 : NT / WAVE][32];
     __shared__ uint32_t ctab[CT_WORDS];
     const uint32_t nchunk = (a.ntiles + a.tpi - 1) / a.tpi;
     const uint32_t nitems = a.nq * nchunk;
@@ -1325,7 +1352,7 @@ __global__ __launch_bounds__(NT) void k_expr(ConjArgs a) {
                 const int32_t op = a.prog[p];
                 if (op >= 0) {
                     const QueryArgs q = a.terms[op];       // in registers (see k_range)
-                    term_tile<NT>(q, tile, g0, ng, last, S[sp], cov, red, bal, ctab);
+                    term_tile<NT, LIT>(q, tile, g0, ng, last, S[sp], cov, red, bal, ctab);
                     ++sp;
                     continue;
                 }
@@ -1733,6 +1760,15 @@ hipError_t launch_conj(const ConjArgs& a, int nt, uint32_t blocks, hipStream_t s
         // shallower stack leaves room for more blocks per CU
         if (a.depth == 0 || a.depth > (uint32_t)EXPR_DEPTH) return hipErrorInvalidValue;
         const bool d2 = a.depth <= 2;
+        if (a.lit_only && d2) {
+            switch (nt) {
+            case 128: hipLaunchKernelGGL((k_expr<128, 2, true>), dim3(blocks), dim3(128), 0, s, a); break;
+            case 256: hipLaunchKernelGGL((k_expr<256, 2, true>), dim3(blocks), dim3(256), 0, s, a); break;
+            case 512: hipLaunchKernelGGL((k_expr<512, 2, true>), dim3(blocks), dim3(512), 0, s, a); break;
+            default: return hipErrorInvalidValue;
+            }
+            return hipGetLastError();
+        }
         switch (nt) {
         case 128:
             if (d2) hipLaunchKernelGGL((k_expr<128, 2>), dim3(blocks), dim3(128), 0, s, a);

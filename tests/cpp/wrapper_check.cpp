@@ -110,6 +110,17 @@ void check_index() {
     }
     EXPECT(ix.sumBins(0, 50) == n);
     EXPECT(ix.sumBins(10, 20) == count(0, 10));
+    // two-level forms over the same column (ibis::fuzz, ibis::zona)
+    {
+        fbh::index fz = fbh::index::fuzz(col.data(), FBH_COL_INT32, n, 6);
+        fbh::index zn = fbh::index::zona(col.data(), FBH_COL_INT32, n);
+        for (int q = 0; q < 8; ++q) {
+            const double lo = (double)((int)(next() % 60) - 15), hi = lo + (double)(next() % 40);
+            const uint64_t want = count(lo, hi);
+            EXPECT(fz.evaluate(lo, FBH_OP_LE, hi, FBH_OP_LT, FBH_PLAN_LITERAL) == want);
+            EXPECT(zn.evaluate(lo, FBH_OP_LE, hi, FBH_OP_LT, FBH_PLAN_AUTO) == want);
+        }
+    }
     uint32_t c0, h0, h1, c1;
     ix.locate(0, FBH_OP_LE, 10, FBH_OP_LT, c0, h0, h1, c1);
     EXPECT(h0 == 10 && h1 == 20);
