@@ -748,9 +748,10 @@ __device__ __forceinline__ void load_round4(const uint4* tw4, uint32_t base4, ui
 // One sub-round: lane l applies the four words of 16-byte group g4 + l at
 // carry + their prefix sum over the wave (mod ng); returns the new carry.
 // FULL: every word of the sub-round lies in [ws, we) (no checks).
-template <bool POW2, bool FULL>
+template <bool POW2, bool FULL, bool TRACK>
 __device__ __forceinline__ uint32_t apply_group4(const uint4& x, uint32_t g4, uint32_t ws, uint32_t we,
-                                                 uint32_t carry, uint32_t ng, word_t* R, uint32_t* cov) {
+                                                 uint32_t carry, uint32_t ng, word_t* R, uint32_t* cov,
+                                                 uint32_t& fl) {
     const uint32_t j = (g4 + lane_id()) << 2;          // word index of x.x
     const bool v0 = FULL || (j >= ws && j < we), v1 = FULL || (j + 1 >= ws && j + 1 < we);
     const bool v2 = FULL || (j + 2 >= ws && j + 2 < we), v3 = FULL || (j + 3 >= ws && j + 3 < we);
@@ -770,6 +771,8 @@ __device__ __forceinline__ uint32_t apply_group4(const uint4& x, uint32_t g4, ui
     apply_one(x.y, v1, p1, l1, R, cov);
     apply_one(x.z, v2, p2, l2, R, cov);
     apply_one(x.w, v3, p3, l3, R, cov);
+    if (TRACK)
+        fl |= (v0 && x.x >= HEADER1) | (v1 && x.y >= HEADER1) | (v2 && x.z >= HEADER1) | (v3 && x.w >= HEADER1);
     return POW2 ? ((carry + tot) & (uint32_t)(QT - 1)) : ((carry + tot) % ng);
 }
 
@@ -778,25 +781,25 @@ __device__ __forceinline__ uint32_t apply_group4(const uint4& x, uint32_t g4, ui
 // count as absent), so one prefix sum over the wave places 256 words; a
 // round of U sub-rounds of loads is in flight while the previous round is
 // applied.  The tiled copy is padded so that whole groups can be read.
-template <bool POW2, int U>
+template <bool POW2, int U, bool TRACK>
 __device__ __forceinline__ uint32_t apply_round4(const uint4 (&x)[U], uint32_t b4, uint32_t end4, uint32_t full0,
                                                  uint32_t full1, uint32_t ws, uint32_t we, uint32_t carry,
-                                                 uint32_t ng, word_t* R, uint32_t* cov) {
+                                                 uint32_t ng, word_t* R, uint32_t* cov, uint32_t& fl) {
     #pragma unroll
     for (int u = 0; u < U; ++u) {
         const uint32_t g4 = b4 + (uint32_t)u * WAVE;
         if (g4 >= end4) break;
         if (g4 >= full0 && g4 + WAVE <= full1)
-            carry = apply_group4<POW2, true>(x[u], g4, ws, we, carry, ng, R, cov);
+            carry = apply_group4<POW2, true, TRACK>(x[u], g4, ws, we, carry, ng, R, cov, fl);
         else
-            carry = apply_group4<POW2, false>(x[u], g4, ws, we, carry, ng, R, cov);
+            carry = apply_group4<POW2, false, TRACK>(x[u], g4, ws, we, carry, ng, R, cov, fl);
     }
     return carry;
 }
 
-template <bool POW2, int U>
+template <bool POW2, int U, bool TRACK>
 __device__ __forceinline__ void stream_words(const QueryArgs& q, uint32_t ws, uint32_t we, uint32_t ng,
-                                             word_t* R, uint32_t* cov) {
+                                             word_t* R, uint32_t* cov, uint32_t& fl) {
     if (ws >= we) return;
     const uint4* tw4 = reinterpret_cast<const uint4*>(q.tw);
     const uint32_t a4 = ws >> 2, end4 = (we + 3u) >> 2;
@@ -808,9 +811,9 @@ __device__ __forceinline__ void stream_words(const QueryArgs& q, uint32_t ws, ui
     load_round4<U>(tw4, a4, end4, xa);
     for (uint32_t b4 = a4; b4 < end4; b4 += 2u * U * WAVE) {
         load_round4<U>(tw4, b4 + U * WAVE, end4, xb);
-        carry = apply_round4<POW2, U>(xa, b4, end4, full0, full1, ws, we, carry, ng, R, cov);
+        carry = apply_round4<POW2, U, TRACK>(xa, b4, end4, full0, full1, ws, we, carry, ng, R, cov, fl);
         load_round4<U>(tw4, b4 + 2u * U * WAVE, end4, xa);
-        carry = apply_round4<POW2, U>(xb, b4 + U * WAVE, end4, full0, full1, ws, we, carry, ng, R, cov);
+        carry = apply_round4<POW2, U, TRACK>(xb, b4 + U * WAVE, end4, full0, full1, ws, we, carry, ng, R, cov, fl);
     }
 }
 
@@ -823,29 +826,51 @@ __device__ __forceinline__ void stream_words(const QueryArgs& q, uint32_t ws, ui
 // The OR set of the tile: stream 1 = the segments of bitmaps [r0, r1),
 // stream 2 = those of [r2, r3).  The block reads the starts of up to CT_MAX
 // chunks of consecutive bitmaps into ctab; wave w streams the chunks whose
-// start lies in the w-th quarter of the words.  Called by every thread of
-// the block; returns true when the dense form above did the work (no
-// 1-fill went to cov).
+// start lies in the w-th quarter of the words.  With DENSE, R[0, QT] is
+// cleared here while the chunk starts are in flight (else by the caller).
+// Called by every thread of the block; returns true when the dense form
+// above did the work, and sets fl in a thread that recorded a 1-fill in
+// cov.
 template <class T, bool POW2, int NT, int U, bool DENSE>
 __device__ bool tile_ranges(const QueryArgs& q, uint32_t tile, uint32_t g0, uint32_t ng, word_t* R,
-                            uint32_t* cov, const T* col, uint32_t* ctab) {
+                            uint32_t* cov, const T* col, uint32_t* ctab, uint32_t& fl) {
     constexpr int NW = NT / WAVE;
+    static_assert(2 * NT >= CT_MAX + 2, "two chunk starts per thread");
     const uint32_t len1 = q.r1 - q.r0, len2 = q.r3 - q.r2;
-    if (len1 + len2 == 0) return true;
+    if (!DENSE && len1 + len2 == 0) return true;
     const uint32_t cb = max(16u, (len1 + len2 + CT_MAX - 1) / CT_MAX);   // bitmaps per chunk
     const uint32_t n1 = (len1 + cb - 1) / cb, n2 = (len2 + cb - 1) / cb;
     const uint32_t nc = n1 + n2;                                        // <= CT_MAX + 2
     const uint32_t* TWt = q.TW + (size_t)tile * q.nobs;
     uint32_t* st = ctab + CT_MAX + 2;        // s1, e1, s2, e2
-    for (uint32_t k = threadIdx.x; k < nc; k += NT)
-        ctab[k] = TWt[k < n1 ? q.r0 + k * cb : q.r2 + (k - n1) * cb];
-    if (threadIdx.x == 0) {
-        st[0] = len1 ? TWt[q.r0] : 0u;
-        st[1] = len1 ? TWt[q.r1] : 0u;
-        st[2] = len2 ? TWt[q.r2] : 0u;
-        st[3] = len2 ? TWt[q.r3] : 0u;
+    if (DENSE) {
+        // R cleared while the chunk starts are in flight
+        const uint32_t ka0 = threadIdx.x, ka1 = threadIdx.x + NT;
+        uint32_t t0 = 0, t1 = 0;
+        if (ka0 < nc) t0 = TWt[ka0 < n1 ? q.r0 + ka0 * cb : q.r2 + (ka0 - n1) * cb];
+        if (ka1 < nc) t1 = TWt[ka1 < n1 ? q.r0 + ka1 * cb : q.r2 + (ka1 - n1) * cb];
+        if (threadIdx.x == NT - 1) {         // the last wave, whose chunk loads are the fewest
+            st[0] = len1 ? TWt[q.r0] : 0u;
+            st[1] = len1 ? TWt[q.r1] : 0u;
+            st[2] = len2 ? TWt[q.r2] : 0u;
+            st[3] = len2 ? TWt[q.r3] : 0u;
+        }
+        for (uint32_t k = threadIdx.x; k <= (uint32_t)QT; k += NT) R[k] = 0u;
+        if (ka0 < nc) ctab[ka0] = t0;
+        if (ka1 < nc) ctab[ka1] = t1;
+    } else {
+        // the caller cleared R
+        for (uint32_t k = threadIdx.x; k < nc; k += NT)
+            ctab[k] = TWt[k < n1 ? q.r0 + k * cb : q.r2 + (k - n1) * cb];
+        if (threadIdx.x == 0) {
+            st[0] = len1 ? TWt[q.r0] : 0u;
+            st[1] = len1 ? TWt[q.r1] : 0u;
+            st[2] = len2 ? TWt[q.r2] : 0u;
+            st[3] = len2 ? TWt[q.r3] : 0u;
+        }
     }
     __syncthreads();
+    if (len1 + len2 == 0) return true;
     const uint32_t s1 = st[0], e1 = st[1], s2 = st[2], e2 = st[3];
     const uint32_t W1 = e1 - s1, W = W1 + (e2 - s2);
     if (DENSE && len1 + len2 == 1 && W == ng) {
@@ -877,40 +902,49 @@ __device__ bool tile_ranges(const QueryArgs& q, uint32_t tile, uint32_t g0, uint
     // the piece in stream 1, then the piece in stream 2
     if (ka < n1) {
         const uint32_t we = kb >= n1 ? e1 : ctab[kb];
-        stream_words<POW2, U>(q, ctab[ka], we, ng, R, cov);
+        stream_words<POW2, U, DENSE>(q, ctab[ka], we, ng, R, cov, fl);
     }
     if (kb > n1) {
         const uint32_t ws = ka > n1 ? ctab[ka] : s2;
         const uint32_t we = kb >= nc ? e2 : ctab[kb];
-        stream_words<POW2, U>(q, ws, we, ng, R, cov);
+        stream_words<POW2, U, DENSE>(q, ws, we, ng, R, cov, fl);
     }
     return false;
 }
 
 // Literal plan of one term over tile t into R[0, ng), by a block of NT
-// threads.  R and cov are zero on entry; the call ends with a
-// barrier.  red needs NT/64 + 1 words.  DENSE enables the block-wide form
-// for small dense OR sets (off in the range-batch kernel, where its
+// threads.  cov[0, QT] is zero on entry.  With DENSE, R is cleared here
+// and the coverage pass clears what the term wrote to cov; without, the
+// caller clears both before each call.  The call ends with a barrier.  red needs NT/64 + 1 words.  DENSE enables the block-wide copy
+// of a dense single-bitmap term (off in the range-batch kernel, where its
 // registers would cost a wave per SIMD).
 template <class T, int NT, int U, bool DENSE = true, bool EDGES = true>
 __device__ __forceinline__ void lit_tile(const QueryArgs& q, uint32_t tile, uint32_t g0, uint32_t ng, bool last, word_t* R,
                          uint32_t* cov, uint32_t* red, uint32_t* ctab) {
     const T* col = (const T*)q.col;
     const int w = threadIdx.x / WAVE;
-    bool dense = true;
+    uint32_t fl = 0;
     if (ng) {
-        if (last) dense = tile_ranges<T, false, NT, U, DENSE>(q, tile, g0, ng, R, cov, col, ctab);
-        else dense = tile_ranges<T, true, NT, U, DENSE>(q, tile, g0, ng, R, cov, col, ctab);
+        if (last) tile_ranges<T, false, NT, U, DENSE>(q, tile, g0, ng, R, cov, col, ctab, fl);
+        else tile_ranges<T, true, NT, U, DENSE>(q, tile, g0, ng, R, cov, col, ctab, fl);
     }
-    __syncthreads();
-    // coverage prefix sums: groups under some 1-fill are all ones (none
-    // when the dense form ran: it is the same for every thread)
-    if (!DENSE || !dense) {
+    // coverage prefix sums: groups under some 1-fill are all ones; the pass
+    // leaves cov zero.  With DENSE it runs only when some 1-fill was
+    // recorded (a block-wide vote at the barrier).
+    bool any_fill = true;
+    if (DENSE) any_fill = __syncthreads_or((int)fl) != 0;
+    else __syncthreads();
+    if (any_fill) {
         constexpr int PER = (QT + NT - 1) / NT;
         const uint32_t k0 = threadIdx.x * PER;
         uint32_t c[PER], sum = 0;
         #pragma unroll
-        for (int i = 0; i < PER; ++i) { c[i] = k0 + i < (uint32_t)QT ? cov[k0 + i] : 0u; sum += c[i]; }
+        for (int i = 0; i < PER; ++i) {
+            c[i] = k0 + i < (uint32_t)QT ? cov[k0 + i] : 0u;
+            if (DENSE && k0 + i < (uint32_t)QT) cov[k0 + i] = 0u;
+            sum += c[i];
+        }
+        if (DENSE && threadIdx.x == 0) cov[QT] = 0u;   // the end mark of a fill reaching the tile's end
         uint32_t tot;
         uint32_t run = block_excl_sum<NT>(sum, red, &tot);
         #pragma unroll
@@ -1122,8 +1156,6 @@ __device__ __forceinline__ void term_tile(const QueryArgs& q, uint32_t tile, uin
                           uint32_t* cov, uint32_t* red, uint64_t (*bal)[32], uint32_t* ctab) {
     if (LIT) {
         // literal plan, no edge bitmaps: the column type does not matter
-        for (uint32_t k = threadIdx.x; k <= QT; k += NT) { R[k] = 0u; cov[k] = 0u; }
-        __syncthreads();
         lit_tile<uint32_t, NT, CJ_U, true, false>(q, tile, g0, ng, last, R, cov, red, ctab);
         return;
     }
@@ -1137,8 +1169,6 @@ __device__ __forceinline__ void term_tile(const QueryArgs& q, uint32_t tile, uin
         }
         return;
     }
-    for (uint32_t k = threadIdx.x; k <= QT; k += NT) { R[k] = 0u; cov[k] = 0u; }
-    __syncthreads();
     switch (q.type) {
     case COL_I32: lit_tile<int32_t, NT, CJ_U>(q, tile, g0, ng, last, R, cov, red, ctab); break;
     case COL_U32: lit_tile<uint32_t, NT, CJ_U>(q, tile, g0, ng, last, R, cov, red, ctab); break;
@@ -1207,6 +1237,8 @@ __global__ __launch_bounds__(NT) void k_conj(ConjArgs a) {
     __shared__ uint32_t ctab[CT_WORDS];
     const uint32_t nchunk = (a.ntiles + a.tpi - 1) / a.tpi;
     const uint32_t nitems = a.nq * nchunk;
+    for (uint32_t k = threadIdx.x; k <= QT; k += NT) cov[k] = 0u;   // the literal plan keeps it zero
+    __syncthreads();
     for (uint32_t item = blockIdx.x; item < nitems; item += gridDim.x) {
         const uint32_t qi = item / nchunk;
         const uint32_t c = item - qi * nchunk;
@@ -1320,8 +1352,8 @@ __global__ __launch_bounds__(NT) void k_range(ConjArgs a) {
 // LDS.  NOT complements within the rows (an active group keeps only its
 // nbits), NAND / NOR are NOT of AND / OR as fastbit_selection_combine
 // builds them (src/iapi.cpp:1313-1326), MINUS is AND NOT.
-template <int NT, int D, bool LIT = false>
-__global__ __launch_bounds__(NT) void k_expr(ConjArgs a) {
+template <int NT, int D, bool LIT>
+__device__ __forceinline__ void expr_body(const ConjArgs& a) {
     __shared__ word_t S[D][QT + 1];      // D >= a.depth (the launcher's choice)
     __shared__ uint32_t cov[QT + 1];
     __shared__ uint32_t red[NT / WAVE + 1];
@@ -1330,6 +1362,8 @@ __global__ __launch_bounds__(NT) void k_expr(ConjArgs a) {
     const uint32_t nchunk = (a.ntiles + a.tpi - 1) / a.tpi;
     const uint32_t nitems = a.nq * nchunk;
     const word_t amask = a.anb ? (((1u << a.anb) - 1u) << (31u - a.anb)) : 0u;
+    for (uint32_t k = threadIdx.x; k <= QT; k += NT) cov[k] = 0u;   // the literal plan keeps it zero
+    __syncthreads();
     for (uint32_t item = blockIdx.x; item < nitems; item += gridDim.x) {
         const uint32_t qi = item / nchunk;
         const uint32_t c = item - qi * nchunk;
@@ -1385,6 +1419,17 @@ __global__ __launch_bounds__(NT) void k_expr(ConjArgs a) {
         }
     }
 }
+
+template <int NT, int D>
+__global__ __launch_bounds__(NT) void k_expr(ConjArgs a) { expr_body<NT, D, false>(a); }
+
+// Programs of literal terms with no edge bitmaps (relic, fuzz, zona): the
+// scan and edge-check code left out, registers held to 6 waves per SIMD.
+template <int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_expr_lit(ConjArgs a) {
+    expr_body<NT, 2, true>(a);
+}
+
 
 // Raw scan plan: R[g] = rows 31g..31g+30 that satisfy the range.  A wave
 // covers 64 groups (1984 rows): 31 coalesced loads of 64 values, 31
@@ -1762,9 +1807,9 @@ hipError_t launch_conj(const ConjArgs& a, int nt, uint32_t blocks, hipStream_t s
         const bool d2 = a.depth <= 2;
         if (a.lit_only && d2) {
             switch (nt) {
-            case 128: hipLaunchKernelGGL((k_expr<128, 2, true>), dim3(blocks), dim3(128), 0, s, a); break;
-            case 256: hipLaunchKernelGGL((k_expr<256, 2, true>), dim3(blocks), dim3(256), 0, s, a); break;
-            case 512: hipLaunchKernelGGL((k_expr<512, 2, true>), dim3(blocks), dim3(512), 0, s, a); break;
+            case 128: hipLaunchKernelGGL((k_expr_lit<128>), dim3(blocks), dim3(128), 0, s, a); break;
+            case 256: hipLaunchKernelGGL((k_expr_lit<256>), dim3(blocks), dim3(256), 0, s, a); break;
+            case 512: hipLaunchKernelGGL((k_expr_lit<512>), dim3(blocks), dim3(512), 0, s, a); break;
             default: return hipErrorInvalidValue;
             }
             return hipGetLastError();
