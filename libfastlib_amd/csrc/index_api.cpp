@@ -171,42 +171,60 @@ uint32_t bits_for(uint32_t maxkey) {
 }
 
 // Encode bitmaps from sorted rows; fills idx words/offsets/nw (+min/max).
+// Two passes of the chunk-parallel encoder: the first counts every chunk's
+// code words (and ORs the active bits, reduces the value ranges), the host
+// lays the bitmaps out (dropping empty inner bins, deciding which bitmaps
+// are stored decompressed), the second writes the words in place.
 int encode_bitmaps(fbh_index* ix, const uint32_t* skeys_d, const uint32_t* srows_d, uint32_t nsorted,
                    bool with_minmax, bool drop_empty_bins, bool unpack_heavy) {
     const uint32_t nbm = ix->nobs;
     // bitmap starts in the sorted key order (rows without a bitmap sort last)
     std::vector<uint32_t> start(nbm + 1);
-    {
-        DBuf<uint32_t> ds;
-        int rc0 = ds.alloc(nbm + 1);
-        if (!rc0) rc0 = check(fbk::launch_starts(skeys_d, nsorted, nbm, ds.p, nullptr, stream()), "starts");
-        if (!rc0) rc0 = to_host(start.data(), ds.p, nbm + 1);
-        if (rc0) return rc0;
-    }
-    int rc;
-    std::vector<uint64_t> tmpoff(nbm + 1);
-    tmpoff[0] = 0;
-    for (uint32_t b = 0; b < nbm; ++b) tmpoff[b + 1] = tmpoff[b] + 2ull * (start[b + 1] - start[b]) + 2;
-    DBuf<uint32_t> dstart, tmp, dnw, daval;
-    DBuf<uint64_t> dtmpoff;
-    DBuf<double> dmin, dmax;
-    if ((rc = dstart.alloc(nbm + 1)) || (rc = tmp.alloc(tmpoff[nbm] + 1)) || (rc = dnw.alloc(nbm)) ||
-        (rc = daval.alloc(nbm)) || (rc = dtmpoff.alloc(nbm + 1)))
-        return rc;
-    if (with_minmax && ((rc = dmin.alloc(nbm)) || (rc = dmax.alloc(nbm)))) return rc;
-    if ((rc = to_dev(dstart.p, start.data(), nbm + 1)) || (rc = to_dev(dtmpoff.p, tmpoff.data(), nbm + 1)))
-        return rc;
-    rc = check(fbk::launch_encode(dev_type(ix->type), srows_d, dstart.p, nbm, ix->groups, ix->anb,
-                                  dtmpoff.p, tmp.p, dnw.p, daval.p, with_minmax ? ix->col : nullptr,
-                                  with_minmax ? dmin.p : nullptr, with_minmax ? dmax.p : nullptr,
-                                  stream()), "encode");
+    DBuf<uint32_t> dstart;
+    int rc = dstart.alloc(nbm + 1);
+    if (!rc) rc = check(fbk::launch_starts(skeys_d, nsorted, nbm, dstart.p, stream()), "starts");
+    if (!rc) rc = to_host(start.data(), dstart.p, nbm + 1);
     if (rc) return rc;
-    std::vector<uint32_t> nw(nbm);
-    if ((rc = to_host(nw.data(), dnw.p, nbm))) return rc;
-    std::vector<double> mn, mx;
-    if (with_minmax) {
-        mn.resize(nbm); mx.resize(nbm);
-        if ((rc = to_host(mn.data(), dmin.p, nbm)) || (rc = to_host(mx.data(), dmax.p, nbm))) return rc;
+    // chunks of at most ENC_ROWS rows, one at least per bitmap
+    std::vector<uint32_t> cbm, cstart, cfirst(nbm + 1);
+    for (uint32_t b = 0; b < nbm; ++b) {
+        cfirst[b] = (uint32_t)cbm.size();
+        const uint32_t n = start[b + 1] - start[b];
+        const uint32_t k = n ? (n + fbk::ENC_ROWS - 1) / fbk::ENC_ROWS : 1u;
+        for (uint32_t j = 0; j < k; ++j) { cbm.push_back(b); cstart.push_back(start[b] + j * fbk::ENC_ROWS); }
+    }
+    const uint32_t nchunk = (uint32_t)cbm.size();
+    cfirst[nbm] = nchunk;
+    DBuf<uint32_t> dcbm, dcstart, dccnt, daval;
+    DBuf<double> dmin, dmax;
+    if ((rc = dcbm.alloc(nchunk)) || (rc = dcstart.alloc(nchunk)) || (rc = dccnt.alloc(nchunk)) ||
+        (rc = daval.alloc(nbm ? nbm : 1)))
+        return rc;
+    if (with_minmax && ((rc = dmin.alloc(nchunk)) || (rc = dmax.alloc(nchunk)))) return rc;
+    if ((rc = to_dev(dcbm.p, cbm.data(), nchunk)) || (rc = to_dev(dcstart.p, cstart.data(), nchunk))) return rc;
+    if (nbm && (rc = check(fbk::launch_zero_u32(daval.p, nbm, stream()), "zero"))) return rc;
+    fbk::EncArgs ea;
+    memset(&ea, 0, sizeof(ea));
+    ea.rows = srows_d; ea.start = dstart.p; ea.cbm = dcbm.p; ea.cstart = dcstart.p; ea.nchunk = nchunk;
+    ea.groups = ix->groups; ea.anb = ix->anb;
+    ea.ccnt = dccnt.p; ea.aval = daval.p;
+    ea.cmin = with_minmax ? dmin.p : nullptr; ea.cmax = with_minmax ? dmax.p : nullptr;
+    rc = check(fbk::launch_enc_count(ea, dev_type(ix->type), with_minmax ? ix->col : nullptr, stream()),
+               "encode count");
+    std::vector<uint32_t> ccnt(nchunk);
+    std::vector<double> cmn, cmx;
+    if (!rc) rc = to_host(ccnt.data(), dccnt.p, nchunk);
+    if (!rc && with_minmax) {
+        cmn.resize(nchunk); cmx.resize(nchunk);
+        if (!(rc = to_host(cmn.data(), dmin.p, nchunk))) rc = to_host(cmx.data(), dmax.p, nchunk);
+    }
+    if (rc) return rc;
+    std::vector<uint32_t> nw(nbm, 0);
+    std::vector<double> mn(nbm, DBL_MAX), mx(nbm, -DBL_MAX);
+    for (uint32_t c = 0; c < nchunk; ++c) {
+        const uint32_t b = cbm[c];
+        nw[b] += ccnt[c];
+        if (with_minmax && start[b + 1] > start[b]) { mn[b] = std::min(mn[b], cmn[c]); mx[b] = std::max(mx[b], cmx[c]); }
     }
     // bins: drop empty ones except the first and the last
     // (src/ibin.cpp:3397-3428); an empty kept bin is a null bitmap
@@ -222,51 +240,54 @@ int encode_bitmaps(fbh_index* ix, const uint32_t* skeys_d, const uint32_t* srows
     }
     const uint32_t nk = (uint32_t)keep.size();
     std::vector<uint64_t> off(nk + 1);
-    std::vector<uint32_t> knw(nk), srcnw(nk);
-    std::vector<uint64_t> ktmp(nk);
-    std::vector<uint8_t> unpack(nk, 0);
+    std::vector<uint32_t> knw(nk);
     // index::optionalUnpack with no <compressing> directive
     // (src/index.cpp:8803-8854): a bitmap whose bytes() exceed
     // max(nrows/9, sizeof(bitvector)+12) is stored decompressed.
     const uint64_t bar1 = std::max<uint64_t>(ix->nrows / 9u, 40u + 12u);
+    const uint32_t trailer = ix->anb > 0 ? 2u : 1u;
+    std::vector<uint64_t> wpos(nchunk, ~0ull), boff(nbm, 0);
+    std::vector<uint32_t> bnw(nbm, 0);
+    std::vector<uint8_t> unpack(nbm, 0);
     bool any_unpack = false;
     off[0] = 0;
-    const uint32_t trailer = ix->anb > 0 ? 2u : 1u;
     for (uint32_t k = 0; k < nk; ++k) {
         const uint32_t b = keep[k];
-        srcnw[k] = nullbm[k] ? 0u : nw[b];
-        knw[k] = srcnw[k];
+        knw[k] = nullbm[k] ? 0u : nw[b];
         if (unpack_heavy && !nullbm[k] && 4ull * nw[b] + 40u > bar1 && nw[b] != ix->groups) {
-            unpack[k] = 1;
+            unpack[b] = 1;
             knw[k] = ix->groups;
             any_unpack = true;
         }
-        ktmp[k] = tmpoff[b];
         off[k + 1] = off[k] + (nullbm[k] ? 0u : knw[k] + trailer);
+        if (nullbm[k]) continue;
+        boff[b] = off[k];
+        bnw[b] = knw[k];
+        uint64_t p = off[k];
+        for (uint32_t c = cfirst[b]; c < cfirst[b + 1]; ++c) { wpos[c] = p; p += ccnt[c]; }
     }
-    // gather per kept bitmap (device arrays for the concat)
-    std::vector<uint32_t> kval_idx(keep.begin(), keep.end());
-    DBuf<uint64_t> doff_k, dtmpoff_k;
-    DBuf<uint32_t> dnw_k, daval_k;
+    DBuf<uint64_t> dwpos, dboff;
+    DBuf<uint32_t> dbnw, words;
     DBuf<uint8_t> dunpack;
-    std::vector<uint32_t> aval(nbm);
-    if ((rc = to_host(aval.data(), daval.p, nbm))) return rc;
-    std::vector<uint32_t> kav(nk);
-    for (uint32_t k = 0; k < nk; ++k) kav[k] = aval[keep[k]];
-    if ((rc = doff_k.alloc(nk + 1)) || (rc = dtmpoff_k.alloc(nk)) || (rc = dnw_k.alloc(nk)) ||
-        (rc = daval_k.alloc(nk)))
+    if ((rc = dwpos.alloc(nchunk)) || (rc = dboff.alloc(nbm ? nbm : 1)) || (rc = dbnw.alloc(nbm ? nbm : 1)) ||
+        (rc = words.alloc(off[nk] + 1)))
         return rc;
-    if ((rc = to_dev(doff_k.p, off.data(), nk + 1)) || (rc = to_dev(dtmpoff_k.p, ktmp.data(), nk)) ||
-        (rc = to_dev(dnw_k.p, srcnw.data(), nk)) || (rc = to_dev(daval_k.p, kav.data(), nk)))
+    if ((rc = to_dev(dwpos.p, wpos.data(), nchunk)) || (nbm && (rc = to_dev(dboff.p, boff.data(), nbm))) ||
+        (nbm && (rc = to_dev(dbnw.p, bnw.data(), nbm))))
         return rc;
-    if (any_unpack && ((rc = dunpack.alloc(nk)) || (rc = to_dev(dunpack.p, unpack.data(), nk)))) return rc;
-    DBuf<uint32_t> words;
-    if ((rc = words.alloc(off[nk] + 1))) return rc;
-    rc = check(fbk::launch_concat(tmp.p, dtmpoff_k.p, dnw_k.p, daval_k.p, doff_k.p,
-                                  any_unpack ? dunpack.p : nullptr, nk, ix->groups, ix->anb, words.p,
-                                  stream()), "concat");
+    if (any_unpack) {
+        if ((rc = dunpack.alloc(nbm)) || (rc = to_dev(dunpack.p, unpack.data(), nbm))) return rc;
+        // the decompressed bitmaps get a literal per group: zero them first
+        for (uint32_t b = 0; b < nbm; ++b)
+            if (unpack[b] && (rc = check(fbk::launch_zero_u32(words.p + boff[b], ix->groups, stream()), "zero")))
+                return rc;
+    }
+    rc = check(fbk::launch_zero_u32(words.p + off[nk], 1, stream()), "zero");
+    ea.out = words.p; ea.wpos = dwpos.p; ea.boff = dboff.p; ea.bnw = dbnw.p;
+    ea.unpack = any_unpack ? dunpack.p : nullptr;
+    if (!rc) rc = check(fbk::launch_enc_write(ea, stream()), "encode write");
+    if (!rc) rc = stream_sync();
     if (rc) return rc;
-    if ((rc = stream_sync())) return rc;
     // install
     ix->cumrows.assign(nk + 1, 0.0);
     for (uint32_t k = 0; k < nk; ++k)
@@ -326,8 +347,32 @@ int finish_index(fbh_index* ix) {
         return rc;
     ix->TW = (uint32_t*)dev_alloc(4ull * (nseg + 1));
     if (!ix->TW) return fail(FBH_ERR_NOMEM, "device allocation failed");
-    rc = check(fbk::launch_toff(ix->words, ix->doff, ix->dnw, ix->nobs, ix->ntiles, (uint32_t)fbk::QT,
-                                ix->groups, toff.p, stream()), "tile offsets");
+    {
+        // tile offsets over chunks of code words: their groups, a per-bitmap
+        // scan here, then the rows
+        std::vector<uint32_t> tb, tw0;
+        for (uint32_t b = 0; b < ix->nobs; ++b)
+            for (uint32_t w = 0; w < ix->nw[b]; w += fbk::TOFF_WORDS) { tb.push_back(b); tw0.push_back(w); }
+        const uint32_t nch = (uint32_t)tb.size();
+        DBuf<uint32_t> dtb, dtw0, dclen, dcg0, dtot;
+        if ((rc = dtb.alloc(nch ? nch : 1)) || (rc = dtw0.alloc(nch ? nch : 1)) || (rc = dclen.alloc(nch ? nch : 1)) ||
+            (rc = dcg0.alloc(nch ? nch : 1)) || (rc = dtot.alloc(ix->nobs ? ix->nobs : 1)))
+            return rc;
+        if (nch && ((rc = to_dev(dtb.p, tb.data(), nch)) || (rc = to_dev(dtw0.p, tw0.data(), nch)))) return rc;
+        rc = check(fbk::launch_toff_len(ix->words, ix->doff, ix->dnw, dtb.p, dtw0.p, nch, dclen.p, stream()),
+                   "tile offset lengths");
+        std::vector<uint32_t> clen(nch), cg0(nch), tot(ix->nobs, 0);
+        if (!rc && nch) rc = to_host(clen.data(), dclen.p, nch);
+        if (rc) return rc;
+        for (uint32_t c = 0; c < nch; ++c) { cg0[c] = tot[tb[c]]; tot[tb[c]] += clen[c]; }
+        if (nch && (rc = to_dev(dcg0.p, cg0.data(), nch))) return rc;
+        if (ix->nobs && (rc = to_dev(dtot.p, tot.data(), ix->nobs))) return rc;
+        rc = check(fbk::launch_toff_rows(ix->words, ix->doff, ix->dnw, dtb.p, dtw0.p, dcg0.p, nch, ix->nobs,
+                                         ix->ntiles, (uint32_t)fbk::QT, toff.p, stream()), "tile offsets");
+        if (!rc) rc = check(fbk::launch_toff_tail(ix->doff, ix->dnw, dtot.p, ix->nobs, ix->ntiles, (uint32_t)fbk::QT,
+                                                  ix->groups, toff.p, stream()), "tile offset tails");
+        if (!rc) rc = stream_sync();        // the chunk tables are freed on return
+    }
     if (!rc) rc = check(fbk::launch_tseg_len(toff.p, ix->nobs, ix->ntiles, ix->groups, ix->anb, len.p,
                                              stream()), "segment lengths");
     if (!rc) rc = check(fbk::launch_scan_excl(len.p, ix->TW, (uint32_t)nseg + 1, tmp.p, stream()), "segment scan");

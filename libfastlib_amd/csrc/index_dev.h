@@ -102,26 +102,50 @@ hipError_t launch_unique(const uint32_t* k, uint32_t n, uint32_t* out, uint32_t*
 // (flag = tmp, pos = tmp + n of that call)
 hipError_t launch_unique_first(const uint32_t* flag, const uint32_t* pos, uint32_t n, uint32_t* first,
                                hipStream_t s);
-hipError_t launch_starts(const uint32_t* k, uint32_t n, uint32_t nbm, uint32_t* start, uint32_t* room,
-                         hipStream_t s);
+hipError_t launch_starts(const uint32_t* k, uint32_t n, uint32_t nbm, uint32_t* start, hipStream_t s);
 uint32_t rs_blocks(uint32_t n);
 hipError_t launch_rs_pass(const uint32_t* ki, const uint32_t* vi, uint32_t n, uint32_t shift,
                           uint32_t* ko, uint32_t* vo, uint32_t* tmp, hipStream_t s);
-hipError_t launch_encode(int type, const uint32_t* rows, const uint32_t* start, uint32_t nbm,
-                         uint32_t groups, uint32_t anb, const uint64_t* tmpoff, uint32_t* tmp,
-                         uint32_t* nw, uint32_t* aval, const void* col, double* vmin, double* vmax,
-                         hipStream_t s);
-// unpack[b] != 0 (unpack may be null): bitmap b is stored in literal form,
-// `groups` words.
-hipError_t launch_concat(const uint32_t* tmp, const uint64_t* tmpoff, const uint32_t* nw,
-                         const uint32_t* aval, const uint64_t* off, const uint8_t* unpack,
-                         uint32_t nbm, uint32_t groups, uint32_t anb, uint32_t* out, hipStream_t s);
+// Chunk-parallel bitmap encoding (k_enc_chunks): chunk c is rows
+// [cstart[c], min(cstart[c] + ENC_CR, start[b+1])) of bitmap cbm[c].
+constexpr uint32_t ENC_ROWS = 2048;          // rows per chunk (ENC_CR)
+struct EncArgs {
+    const uint32_t* rows;       // row ids sorted by bitmap, ascending within one
+    const uint32_t* start;      // nbm + 1 bitmap starts
+    const uint32_t* cbm;        // chunk -> bitmap
+    const uint32_t* cstart;     // chunk -> first sorted position
+    uint32_t nchunk;
+    uint32_t groups, anb;
+    // count pass
+    uint32_t* ccnt;             // code words per chunk
+    uint32_t* aval;             // active bits per bitmap (zeroed before)
+    double* cmin;               // value range per chunk (bins)
+    double* cmax;
+    // write pass
+    uint32_t* out;              // the serialized bitmaps
+    const uint64_t* wpos;       // chunk's first word in out (~0: skip the chunk)
+    const uint64_t* boff;       // bitmap's first word in out
+    const uint32_t* bnw;        // bitmap's words before its trailer
+    const uint8_t* unpack;      // bitmap stored decompressed (null: none)
+};
+hipError_t launch_enc_count(const EncArgs& a, int type, const void* col, hipStream_t s);
+hipError_t launch_enc_write(const EncArgs& a, hipStream_t s);
 // Tile offset table (ntiles + 1 rows of nbm entries): row t < ntiles holds,
 // per bitmap, {absolute index of the first code word whose span ends after
 // group t*tg, its start group} ({end, groups} past the last word); row
 // ntiles holds {end of the code words, 1 when the bitmap is not null}.
-hipError_t launch_toff(const uint32_t* words, const uint64_t* off, const uint32_t* nw, uint32_t nbm,
-                       uint32_t ntiles, uint32_t tg, uint32_t groups, uint2* toff, hipStream_t s);
+// Built over chunks of TOFF_WORDS code words (chunk c: words
+// [tw0[c], tw0[c] + TOFF_WORDS) of bitmap tb[c]): pass 1 gives each
+// chunk's groups, the host scans them per bitmap into cg0 (and total[b]),
+// pass 2 writes the rows, the tail pass the rest.
+constexpr uint32_t TOFF_WORDS = 4096;
+hipError_t launch_toff_len(const uint32_t* words, const uint64_t* off, const uint32_t* nw, const uint32_t* tb,
+                           const uint32_t* tw0, uint32_t nch, uint32_t* clen, hipStream_t s);
+hipError_t launch_toff_rows(const uint32_t* words, const uint64_t* off, const uint32_t* nw, const uint32_t* tb,
+                            const uint32_t* tw0, const uint32_t* cg0, uint32_t nch, uint32_t nbm,
+                            uint32_t ntiles, uint32_t tg, uint2* toff, hipStream_t s);
+hipError_t launch_toff_tail(const uint64_t* off, const uint32_t* nw, const uint32_t* total, uint32_t nbm,
+                            uint32_t ntiles, uint32_t tg, uint32_t groups, uint2* toff, hipStream_t s);
 // Tiled copy: len[t*nobs + b] = words of segment (t, b) (ntiles*nobs + 1
 // entries, the last 0; its exclusive scan is TW), then the copy itself.
 hipError_t launch_tseg_len(const uint2* toff, uint32_t nobs, uint32_t ntiles, uint32_t groups,

@@ -165,104 +165,29 @@ __global__ __launch_bounds__(RS_TPB) void k_rs_scatter(const uint32_t* __restric
 }
 
 // ---------------------------------------------------------------------------
-// Per-bitmap encoding.  Bitmaps of more than ENC_HEAVY rows are encoded by a
-// whole block each (k_encode_heavy); the others by one thread each:
-constexpr uint32_t ENC_HEAVY = 1u << 15;
-// Bitmap b owns sorted rows [start[b], start[b+1]);
-// one thread walks them and writes the canonical WAH code words of the rows
-// below groups*31 into tmp[toff_b ..] (room for 2*count+2 words), the active
-// bits (rows >= groups*31) into aval[b], the code word count into nw[b] and,
-// for bin indexes, the min / max of the bin's values.
-template <class T>
-__global__ __launch_bounds__(IX_TPB) void k_encode(const uint32_t* __restrict__ rows,
-                                                   const uint32_t* __restrict__ start, uint32_t nbm,
-                                                   uint32_t groups, uint32_t anb,
-                                                   const uint64_t* __restrict__ tmpoff,
-                                                   uint32_t* __restrict__ tmp,
-                                                   uint32_t* __restrict__ nw,
-                                                   uint32_t* __restrict__ aval,
-                                                   const T* __restrict__ col,
-                                                   double* __restrict__ vmin,
-                                                   double* __restrict__ vmax) {
-    const uint32_t b = blockIdx.x * IX_TPB + threadIdx.x;
-    if (b >= nbm) return;
-    if (start[b + 1] - start[b] > ENC_HEAVY) return;        // k_encode_heavy's
-    uint32_t* out = tmp + tmpoff[b];
-    uint32_t k = 0;               // words written
-    int64_t last = -1;            // last group emitted (covered)
-    uint32_t run1 = 0;            // length of the open 1-run (consecutive full groups)
-    uint32_t run1slot = 0;
-    uint32_t av = 0;
-    double mn = DBL_MAX_D, mx = -DBL_MAX_D;
-    const uint32_t s = start[b], e = start[b + 1];
-    uint32_t i = s;
-    while (i < e) {
-        const uint32_t r = rows[i];
-        if (col) {
-            const double x = (double)col[r];
-            mn = x < mn ? x : mn;
-            mx = x > mx ? x : mx;
-        }
-        const uint32_t g = r / 31u;
-        if (g >= groups) {
-            av |= 1u << (anb - 1u - (r - groups * 31u));
-            ++i;
-            continue;
-        }
-        // literal of group g: all following rows in the same group
-        word_t v = 1u << (30u - (r - g * 31u));
-        uint32_t j = i + 1;
-        while (j < e) {
-            const uint32_t r2 = rows[j];
-            if (r2 / 31u != g) break;
-            if (col) {
-                const double x = (double)col[r2];
-                mn = x < mn ? x : mn;
-                mx = x > mx ? x : mx;
-            }
-            v |= 1u << (30u - (r2 - g * 31u));
-            ++j;
-        }
-        const int64_t gap = (int64_t)g - last - 1;
-        if (gap > 0) {
-            run1 = 0;
-            out[k++] = gap >= 2 ? (HEADER0 | (uint32_t)gap) : 0u;
-        }
-        if (v == ALLONES) {
-            if (run1 > 0 && gap == 0) {
-                ++run1;
-                out[run1slot] = HEADER1 | run1;
-            } else {
-                run1 = 1;
-                run1slot = k;
-                out[k++] = ALLONES;
-            }
-        } else {
-            run1 = 0;
-            out[k++] = v;
-        }
-        last = g;
-        i = j;
-    }
-    const int64_t gap = (int64_t)groups - last - 1;
-    if (gap > 0) out[k++] = gap >= 2 ? (HEADER0 | (uint32_t)gap) : 0u;
-    nw[b] = k;
-    aval[b] = av;
-    if (vmin) { vmin[b] = mn; vmax[b] = mx; }
-}
-
-// A bitmap of more than ENC_HEAVY rows, by the block of blockIdx.x (other
-// blocks return): the same words as k_encode, chunk by chunk of
-// IX_TPB*ENC_K consecutive sorted rows.  In a chunk every thread takes
-// ENC_K consecutive rows; the first row of each group heads a unit (the
-// group's literal is the OR of its rows, at most 31); a unit writes a 0-gap
-// word before it when its group does not follow the previous unit's, and a
-// word of its own unless it continues a run of full groups.  The previous
-// unit of a thread's first unit comes from a block max-scan of the packed
-// (group, full) of the last units (groups increase along the rows), the
-// previous chunk's from LDS.  The head of a run of full groups counts the
-// run by checking 31 rows per group ahead.
+// Per-bitmap encoding, chunk-parallel.  The sorted rows of bitmap b are
+// [start[b], start[b+1]), ascending.  Work item c is a chunk of at most
+// ENC_CR consecutive rows of one bitmap (cbm[c], cstart[c]; a bitmap without
+// rows has one empty chunk), taken by one block.  Thread t takes ENC_K
+// consecutive rows; the first row of each group heads a unit whose literal
+// is the OR of the group's rows (at most 31, read ahead across chunk ends).
+// A unit writes a 0-gap word before it when its group does not follow the
+// previous unit's, and a word of its own unless it continues a run of full
+// groups (the head of a run writes the run's fill, counting full groups
+// ahead).  The previous unit of a thread's first unit comes from a block
+// max-scan of the packed (group + 1, full) of each thread's last unit; the
+// chunk's carry-in is the unit of the last row below groups*31 before the
+// chunk, its fullness from the at most 31 rows of its group.  The chunk
+// that ends the bitmap adds the 0-gap up to the last group.
+// Pass 1 (WRITE false) counts each chunk's words, ORs the active bits
+// (rows >= groups*31) into aval[b] and, for bins, each chunk's value range;
+// pass 2 writes the words from wpos[c], or, for a bitmap stored
+// decompressed, each unit's literal at boff[b] + group (the bitmap's words
+// zeroed before), and the bitmap's last chunk its trailer.
 constexpr int ENC_K = 8;
+constexpr uint32_t ENC_CR = IX_TPB * ENC_K;
+static_assert(ENC_CR == ENC_ROWS, "the host's chunk size");
+constexpr uint64_t ENC_SKIP = ~0ull;
 
 // Exclusive / inclusive block-wide max of 32-bit values (>= 0).
 template <int NT>
@@ -304,48 +229,53 @@ __device__ __forceinline__ uint32_t lower_row(const uint32_t* rows, uint32_t s, 
     return lo;
 }
 
-
-template <class T>
-__global__ __launch_bounds__(IX_TPB) void k_encode_heavy(const uint32_t* __restrict__ rows,
-                                                         const uint32_t* __restrict__ start, uint32_t groups,
-                                                         uint32_t anb, const uint64_t* __restrict__ tmpoff,
-                                                         uint32_t* __restrict__ tmp, uint32_t* __restrict__ nw,
-                                                         uint32_t* __restrict__ aval,
-                                                         const T* __restrict__ col,
-                                                         double* __restrict__ vmin, double* __restrict__ vmax) {
-    const uint32_t b = blockIdx.x;
-    const uint32_t s = start[b], e = start[b + 1];
-    if (e - s <= ENC_HEAVY) return;
+template <class T, bool WRITE>
+__global__ __launch_bounds__(IX_TPB) void k_enc_chunks(EncArgs a, const T* __restrict__ col) {
     __shared__ uint32_t sh[IX_TPB / WAVE + 1];
-    __shared__ uint64_t s_last;        // packed (group + 1) << 1 | full of the last unit so far, 0 = none
-    __shared__ uint32_t s_k;           // words written so far
-    __shared__ uint32_t s_av;
+    __shared__ uint32_t s_carry;
     __shared__ double s_mn[IX_TPB / WAVE], s_mx[IX_TPB / WAVE];
-    uint32_t* out = tmp + tmpoff[b];
-    const uint32_t lim = groups * 31u;
-    if (threadIdx.x == 0) { s_last = 0; s_k = 0; s_av = 0; }
-    double mn = DBL_MAX_D, mx = -DBL_MAX_D;
-    uint32_t av = 0;
-    __syncthreads();
-    for (uint32_t base = s; base < e; base += IX_TPB * ENC_K) {
-        const uint32_t i0 = base + threadIdx.x * ENC_K;
-        // units of this thread: heads among its rows
+    const uint32_t lim = a.groups * 31u;
+    const uint32_t* rows = a.rows;
+    for (uint32_t c = blockIdx.x; c < a.nchunk; c += gridDim.x) {
+        if (WRITE && a.wpos[c] == ENC_SKIP) continue;        // the same for the whole block
+        const uint32_t b = a.cbm[c];
+        const uint32_t s = a.start[b], e = a.start[b + 1];
+        const uint32_t c0 = a.cstart[c], c1 = min(c0 + ENC_CR, e);
+        const bool lit = WRITE && a.unpack != nullptr && a.unpack[b];
+        if (threadIdx.x == 0) {
+            // the unit of the last row below lim before the chunk
+            uint32_t carry = 0;
+            const uint32_t p = min(c0, lower_row(rows, s, e, lim));
+            if (p > s) {
+                const uint32_t g = rows[p - 1] / 31u;
+                uint32_t lo = p - 1, hi = p;
+                while (lo > s && rows[lo - 1] / 31u == g) --lo;
+                while (hi < e && rows[hi] < lim && rows[hi] / 31u == g) ++hi;
+                carry = ((g + 1u) << 1) | (hi - lo == 31u ? 1u : 0u);
+            }
+            s_carry = carry;
+        }
+        __syncthreads();
+        const uint32_t carry = s_carry;
+        const uint32_t i0 = c0 + threadIdx.x * (uint32_t)ENC_K;
         uint32_t ug[ENC_K];
         word_t uv[ENC_K];
         int nu = 0;
+        uint32_t av = 0;
+        double mn = DBL_MAX_D, mx = -DBL_MAX_D;
         uint32_t prevg = 0xFFFFFFFFu;
         if (i0 > s && i0 < e) { const uint32_t r = rows[i0 - 1]; prevg = r < lim ? r / 31u : 0xFFFFFFFEu; }
         #pragma unroll
         for (int k = 0; k < ENC_K; ++k) {
             const uint32_t i = i0 + (uint32_t)k;
-            if (i >= e) break;
+            if (i >= c1) break;
             const uint32_t r = rows[i];
-            if (col) {
+            if (!WRITE && col) {
                 const double x = (double)col[r];
                 mn = x < mn ? x : mn;
                 mx = x > mx ? x : mx;
             }
-            if (r >= lim) { av |= 1u << (anb - 1u - (r - lim)); prevg = 0xFFFFFFFEu; continue; }
+            if (r >= lim) { av |= 1u << (a.anb - 1u - (r - lim)); prevg = 0xFFFFFFFEu; continue; }
             const uint32_t g = r / 31u;
             if (g == prevg) continue;
             prevg = g;
@@ -357,195 +287,178 @@ __global__ __launch_bounds__(IX_TPB) void k_encode_heavy(const uint32_t* __restr
             }
             ug[nu] = g; uv[nu] = v; ++nu;
         }
-        // previous unit of this thread's first unit
-        const uint64_t mine = nu ? ((uint64_t)(ug[nu - 1] + 1u) << 1) | (uv[nu - 1] == ALLONES ? 1u : 0u) : 0ull;
-        uint64_t before;
-        {
-            // block exclusive max of 64-bit packed values (two passes of 32 bits suffice: the
-            // group part decides, groups < 2^27)
-            const uint32_t hi = (uint32_t)(mine >> 1), lo = (uint32_t)(mine & 1u);
-            const uint32_t key = hi * 2u + lo;      // fits: (groups + 1) * 2 < 2^32
-            uint32_t ex = block_excl_max_u32<IX_TPB>(key, sh);
-            const uint64_t carry = s_last;
-            before = ex ? (uint64_t)ex : carry;
-        }
-        // words per unit
-        uint32_t cnt = 0;
-        uint64_t prev = before;
-        #pragma unroll
-        for (int u = 0; u < ENC_K; ++u) {
-            if (u >= nu) break;
-            const int64_t pg = prev ? (int64_t)(prev >> 1) - 1 : -1;
-            const bool pfull = prev & 1u;
-            const int64_t gap = (int64_t)ug[u] - pg - 1;
-            const bool merge = uv[u] == ALLONES && pfull && gap == 0;
-            cnt += (gap > 0 ? 1u : 0u) + (merge ? 0u : 1u);
-            prev = ((uint64_t)(ug[u] + 1u) << 1) | (uv[u] == ALLONES ? 1u : 0u);
-        }
-        uint32_t tot;
-        uint32_t pos = block_excl_sum<IX_TPB>(cnt, sh, &tot) + s_k;
-        prev = before;
-        #pragma unroll
-        for (int u = 0; u < ENC_K; ++u) {
-            if (u >= nu) break;
-            const int64_t pg = prev ? (int64_t)(prev >> 1) - 1 : -1;
-            const bool pfull = prev & 1u;
-            const int64_t gap = (int64_t)ug[u] - pg - 1;
-            const bool merge = uv[u] == ALLONES && pfull && gap == 0;
-            if (gap > 0) out[pos++] = gap >= 2 ? (HEADER0 | (uint32_t)gap) : 0u;
-            if (!merge) {
-                if (uv[u] == ALLONES) {
-                    // a run of full groups: count the following full groups
-                    uint32_t L = 1;
-                    uint32_t j = lower_row(rows, s, e, (ug[u] + 1u) * 31u);
-                    for (;;) {
-                        const uint32_t g2 = ug[u] + L;
-                        if (g2 >= groups || j + 30u >= e) break;
-                        if (rows[j] != g2 * 31u || rows[j + 30u] != g2 * 31u + 30u) break;
-                        ++L; j += 31u;
+        const uint32_t mine = nu ? ((ug[nu - 1] + 1u) << 1) | (uv[nu - 1] == ALLONES ? 1u : 0u) : 0u;
+        const uint32_t ex = block_excl_max_u32<IX_TPB>(mine, sh);
+        const uint32_t before = ex ? ex : carry;
+        const bool ends = c1 == e;                           // the chunk ends the bitmap
+        if (lit) {
+            // decompressed form: every unit's literal at its group
+            word_t* dst = a.out + a.boff[b];
+            #pragma unroll
+            for (int u = 0; u < ENC_K; ++u) {
+                if (u >= nu) break;
+                dst[ug[u]] = uv[u];
+            }
+        } else {
+            uint32_t cnt = 0;
+            uint32_t prev = before;
+            #pragma unroll
+            for (int u = 0; u < ENC_K; ++u) {
+                if (u >= nu) break;
+                const int64_t pg = prev ? (int64_t)(prev >> 1) - 1 : -1;
+                const bool pfull = prev & 1u;
+                const int64_t gap = (int64_t)ug[u] - pg - 1;
+                const bool merge = uv[u] == ALLONES && pfull && gap == 0;
+                cnt += (gap > 0 ? 1u : 0u) + (merge ? 0u : 1u);
+                prev = ((ug[u] + 1u) << 1) | (uv[u] == ALLONES ? 1u : 0u);
+            }
+            uint32_t tot;
+            uint32_t pos = block_excl_sum<IX_TPB>(cnt, sh, &tot);
+            // the 0-gap after the bitmap's last unit
+            const uint32_t lastkey = ends ? block_max_u32<IX_TPB>(mine > carry ? mine : carry, sh) : 0u;
+            const int64_t lg = lastkey ? (int64_t)(lastkey >> 1) - 1 : -1;
+            const int64_t tgap = (int64_t)a.groups - lg - 1;
+            if (!WRITE) {
+                if (threadIdx.x == 0) a.ccnt[c] = tot + (ends && tgap > 0 ? 1u : 0u);
+            } else {
+                word_t* dst = a.out + a.wpos[c];
+                prev = before;
+                #pragma unroll
+                for (int u = 0; u < ENC_K; ++u) {
+                    if (u >= nu) break;
+                    const int64_t pg = prev ? (int64_t)(prev >> 1) - 1 : -1;
+                    const bool pfull = prev & 1u;
+                    const int64_t gap = (int64_t)ug[u] - pg - 1;
+                    const bool merge = uv[u] == ALLONES && pfull && gap == 0;
+                    if (gap > 0) dst[pos++] = gap >= 2 ? (HEADER0 | (uint32_t)gap) : 0u;
+                    if (!merge) {
+                        if (uv[u] == ALLONES) {
+                            // a run of full groups: count the following full groups
+                            uint32_t L = 1;
+                            uint32_t j = lower_row(rows, s, e, (ug[u] + 1u) * 31u);
+                            for (;;) {
+                                const uint32_t g2 = ug[u] + L;
+                                if (g2 >= a.groups || j + 30u >= e) break;
+                                if (rows[j] != g2 * 31u || rows[j + 30u] != g2 * 31u + 30u) break;
+                                ++L; j += 31u;
+                            }
+                            dst[pos++] = L >= 2 ? (HEADER1 | L) : ALLONES;
+                        } else {
+                            dst[pos++] = uv[u];
+                        }
                     }
-                    out[pos++] = L >= 2 ? (HEADER1 | L) : ALLONES;
-                } else {
-                    out[pos++] = uv[u];
+                    prev = ((ug[u] + 1u) << 1) | (uv[u] == ALLONES ? 1u : 0u);
+                }
+                if (ends && threadIdx.x == 0 && tgap > 0) dst[tot] = tgap >= 2 ? (HEADER0 | (uint32_t)tgap) : 0u;
+            }
+        }
+        if (!WRITE) {
+            // active bits and the chunk's value range
+            for (int d = 32; d >= 1; d >>= 1) av |= (uint32_t)__shfl_xor((int)av, d, 64);
+            if (lane_id() == 0 && av) atomicOr(&a.aval[b], av);
+            if (col) {
+                for (int d = 32; d >= 1; d >>= 1) {
+                    const double o1 = __shfl_xor(mn, d, 64), o2 = __shfl_xor(mx, d, 64);
+                    mn = o1 < mn ? o1 : mn;
+                    mx = o2 > mx ? o2 : mx;
+                }
+                if (lane_id() == 0) { s_mn[threadIdx.x / WAVE] = mn; s_mx[threadIdx.x / WAVE] = mx; }
+                __syncthreads();
+                if (threadIdx.x == 0) {
+                    double lo = DBL_MAX_D, hi = -DBL_MAX_D;
+                    for (int w = 0; w < IX_TPB / WAVE; ++w) { lo = s_mn[w] < lo ? s_mn[w] : lo; hi = s_mx[w] > hi ? s_mx[w] : hi; }
+                    a.cmin[c] = lo; a.cmax[c] = hi;
                 }
             }
-            prev = ((uint64_t)(ug[u] + 1u) << 1) | (uv[u] == ALLONES ? 1u : 0u);
+        } else if (ends && threadIdx.x == 0) {
+            // the trailer: active value and bit count (or the 0 of no active bits)
+            word_t* t = a.out + a.boff[b] + a.bnw[b];
+            if (a.anb) { t[0] = a.aval[b]; t[1] = a.anb; }
+            else t[0] = 0u;
         }
         __syncthreads();
-        // carry: the last unit of the chunk and the words so far
-        if (threadIdx.x == IX_TPB - 1) {
-            s_k += tot;
-        }
-        const uint32_t lastkey = block_max_u32<IX_TPB>((uint32_t)mine, sh);
-        if (threadIdx.x == 0 && lastkey) s_last = lastkey;
-        __syncthreads();
-    }
-    // active bits, min / max
-    {
-        uint32_t a = av;
-        for (int d = 32; d >= 1; d >>= 1) a |= (uint32_t)__shfl_xor((int)a, d, 64);
-        if (lane_id() == 0 && a) atomicOr(&s_av, a);
-        if (col) {
-            for (int d = 32; d >= 1; d >>= 1) {
-                const double o1 = __shfl_xor(mn, d, 64), o2 = __shfl_xor(mx, d, 64);
-                mn = o1 < mn ? o1 : mn;
-                mx = o2 > mx ? o2 : mx;
-            }
-            if (lane_id() == 0) { s_mn[threadIdx.x / WAVE] = mn; s_mx[threadIdx.x / WAVE] = mx; }
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t k = s_k;
-        const uint64_t last = s_last;
-        const int64_t lg = last ? (int64_t)(last >> 1) - 1 : -1;
-        const int64_t gap = (int64_t)groups - lg - 1;
-        if (gap > 0) out[k++] = gap >= 2 ? (HEADER0 | (uint32_t)gap) : 0u;
-        nw[b] = k;
-        aval[b] = s_av;
-        if (col && vmin) {
-            double a = DBL_MAX_D, z = -DBL_MAX_D;
-            for (int w = 0; w < IX_TPB / WAVE; ++w) { a = s_mn[w] < a ? s_mn[w] : a; z = s_mx[w] > z ? s_mx[w] : z; }
-            vmin[b] = a; vmax[b] = z;
-        }
     }
 }
 
-// One block writes the literal form of a canonical bitmap (the decompress of
-// index::optionalUnpack, src/bitvector.cpp:474-512): code words are taken
-// IX_TPB at a time, their group spans scanned, and every output group is
-// written by a thread that finds its code word in LDS.
-__device__ void block_unpack(const uint32_t* __restrict__ src, uint32_t n, uint32_t* __restrict__ dst) {
-    __shared__ uint32_t sw[IX_TPB];
-    __shared__ uint32_t sp[IX_TPB];
+// Tile offset table, chunk-parallel.  Chunk c is code words
+// [tw0[c], min(tw0[c] + TOFF_CH, nw[b])) of bitmap b = tb[c]; word indices
+// in the table are absolute (into the serialized array).  Pass 1 sums each
+// chunk's word lengths; with the chunks' start groups (a per-bitmap scan on
+// the host) pass 2 gives every word its span, and the word whose span holds
+// group t*tg writes row t of its bitmap.  The tail pass writes the rows of
+// tiles past a bitmap's last group ({end, groups}) and the closing row
+// ntiles ({end, 1 when the bitmap has words and a trailer}).
+constexpr uint32_t TOFF_PER = 16;
+constexpr uint32_t TOFF_CH = IX_TPB * TOFF_PER;
+static_assert(TOFF_CH == TOFF_WORDS, "the host's chunk size");
+
+__global__ __launch_bounds__(IX_TPB) void k_toff_len(const uint32_t* __restrict__ words,
+                                                     const uint64_t* __restrict__ off,
+                                                     const uint32_t* __restrict__ nw,
+                                                     const uint32_t* __restrict__ tb,
+                                                     const uint32_t* __restrict__ tw0, uint32_t nch,
+                                                     uint32_t* __restrict__ clen) {
     __shared__ uint32_t sh[IX_TPB / WAVE + 1];
-    uint64_t pos = 0;
-    for (uint32_t base = 0; base < n; base += IX_TPB) {
-        const uint32_t i = base + threadIdx.x;
-        const uint32_t w = i < n ? src[i] : 0u;
-        const uint32_t l = i < n ? word_len(w) : 0u;
+    for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
+        const uint32_t b = tb[c];
+        const uint32_t* w = words + off[b];
+        const uint32_t w0 = tw0[c], w1 = min(w0 + TOFF_CH, nw[b]);
+        uint32_t l = 0;
+        for (uint32_t i = w0 + threadIdx.x; i < w1; i += IX_TPB) l += word_len(w[i]);
         uint32_t tot;
-        const uint32_t ex = block_excl_sum<IX_TPB>(l, sh, &tot);
-        sw[threadIdx.x] = w;
-        sp[threadIdx.x] = i < n ? ex : 0xFFFFFFFFu;
+        (void)block_excl_sum<IX_TPB>(l, sh, &tot);
+        if (threadIdx.x == 0) clen[c] = tot;
         __syncthreads();
-        for (uint32_t g = threadIdx.x; g < tot; g += IX_TPB) {
-            // last k with sp[k] <= g (sp[0] == 0, strictly increasing)
-            uint32_t lo = 0, hi = IX_TPB;
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (sp[mid] <= g) lo = mid; else hi = mid;
+    }
+}
+
+__global__ __launch_bounds__(IX_TPB) void k_toff_rows(const uint32_t* __restrict__ words,
+                                                      const uint64_t* __restrict__ off,
+                                                      const uint32_t* __restrict__ nw,
+                                                      const uint32_t* __restrict__ tb,
+                                                      const uint32_t* __restrict__ tw0,
+                                                      const uint32_t* __restrict__ cg0, uint32_t nch,
+                                                      uint32_t nbm, uint32_t ntiles, uint32_t tg,
+                                                      uint2* __restrict__ toff) {
+    __shared__ uint32_t sh[IX_TPB / WAVE + 1];
+    for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
+        const uint32_t b = tb[c];
+        const uint32_t base = (uint32_t)off[b];
+        const uint32_t w0 = tw0[c], w1 = min(w0 + TOFF_CH, nw[b]);
+        const uint32_t i0 = w0 + threadIdx.x * TOFF_PER;
+        uint32_t len[TOFF_PER];
+        uint32_t sum = 0;
+        #pragma unroll
+        for (uint32_t k = 0; k < TOFF_PER; ++k) {
+            len[k] = i0 + k < w1 ? word_len(words[base + i0 + k]) : 0u;
+            sum += len[k];
+        }
+        uint32_t tot;
+        uint32_t pos = cg0[c] + block_excl_sum<IX_TPB>(sum, sh, &tot);
+        #pragma unroll
+        for (uint32_t k = 0; k < TOFF_PER; ++k) {
+            if (len[k]) {
+                const uint32_t end = pos + len[k];
+                for (uint32_t t = (pos + tg - 1) / tg; t < ntiles && (uint64_t)t * tg < end; ++t)
+                    toff[(size_t)t * nbm + b] = make_uint2(base + i0 + k, pos);
             }
-            const uint32_t x = sw[lo];
-            dst[pos + g] = x > ALLONES ? ((x & FILLBIT) ? ALLONES : 0u) : x;
+            pos += len[k];
         }
-        pos += tot;
         __syncthreads();
     }
 }
 
-// Concatenation: bitmap b's code words (or, with unpack[b], its literal
-// form of `groups` words) + trailer at off[b].
-__global__ __launch_bounds__(IX_TPB) void k_concat(const uint32_t* __restrict__ tmp,
-                                                   const uint64_t* __restrict__ tmpoff,
-                                                   const uint32_t* __restrict__ nw,
-                                                   const uint32_t* __restrict__ aval,
-                                                   const uint64_t* __restrict__ off,
-                                                   const uint8_t* __restrict__ unpack, uint32_t nbm,
-                                                   uint32_t groups, uint32_t anb,
-                                                   uint32_t* __restrict__ out) {
-    // one block per bitmap
-    const uint32_t b = blockIdx.x;
+__global__ __launch_bounds__(IX_TPB) void k_toff_tail(const uint64_t* __restrict__ off,
+                                                      const uint32_t* __restrict__ nw,
+                                                      const uint32_t* __restrict__ total, uint32_t nbm,
+                                                      uint32_t ntiles, uint32_t tg, uint32_t groups,
+                                                      uint2* __restrict__ toff) {
+    const uint32_t b = blockIdx.x * IX_TPB + threadIdx.x;
     if (b >= nbm) return;
-    if (off[b + 1] == off[b]) return;         // null bitmap: no words at all
-    const uint32_t* src = tmp + tmpoff[b];
-    uint32_t* dst = out + off[b];
-    const uint32_t n = nw[b];
-    uint32_t m = n;
-    if (unpack != nullptr && unpack[b]) {
-        block_unpack(src, n, dst);
-        m = groups;
-    } else {
-        for (uint32_t i = threadIdx.x; i < n; i += IX_TPB) dst[i] = src[i];
-    }
-    if (threadIdx.x == 0) {
-        if (anb > 0) { dst[m] = aval[b]; dst[m + 1] = anb; }
-        else dst[m] = 0u;
-    }
-}
-
-// Tile offset table: one wave per bitmap walks its code words 64 at a time
-// (lengths, DPP prefix sums); the lane whose word covers group t*QT writes
-// row t.  Rows past the bitmap's last group hold {end, groups}; the closing
-// row ntiles holds {end, 1 when the bitmap has words and a trailer}.
-// Word indices are absolute (into the serialized array).
-__global__ __launch_bounds__(IX_TPB) void k_toff(const uint32_t* __restrict__ words,
-                                                 const uint64_t* __restrict__ off,
-                                                 const uint32_t* __restrict__ nw, uint32_t nbm,
-                                                 uint32_t ntiles, uint32_t tg, uint32_t groups,
-                                                 uint2* __restrict__ toff) {
-    const uint32_t b = blockIdx.x * (IX_TPB / WAVE) + threadIdx.x / WAVE;
-    if (b >= nbm) return;
-    const int lane = (int)lane_id();
-    const uint32_t base = (uint32_t)off[b];
-    const uint32_t n = nw[b];
-    const bool null_bm = off[b + 1] == off[b];
-    if (lane == 0) toff[(size_t)ntiles * nbm + b] = make_uint2(base + n, null_bm ? 0u : 1u);
-    uint32_t carry = 0;
-    for (uint32_t i0 = 0; i0 < n; i0 += WAVE) {
-        const uint32_t i = i0 + (uint32_t)lane;
-        const uint32_t l = i < n ? word_len(words[base + i]) : 0u;
-        const uint32_t incl = wave_incl_sum(l);
-        const uint32_t start = carry + incl - l, end = carry + incl;
-        if (l) {
-            for (uint32_t t = (start + tg - 1) / tg; t < ntiles && (uint64_t)t * tg < end; ++t)
-                toff[(size_t)t * nbm + b] = make_uint2(base + i, start);
-        }
-        carry += (uint32_t)__shfl((int)incl, WAVE - 1, WAVE);
-    }
-    // tiles starting at or after the last group covered by a word
-    for (uint32_t t = (carry + tg - 1) / tg + (uint32_t)lane; t < ntiles; t += WAVE)
-        toff[(size_t)t * nbm + b] = make_uint2(base + n, groups);
+    const uint32_t end = (uint32_t)off[b] + nw[b];
+    toff[(size_t)ntiles * nbm + b] = make_uint2(end, off[b + 1] == off[b] ? 0u : 1u);
+    for (uint32_t t = (total[b] + tg - 1) / tg; t < ntiles; ++t)
+        toff[(size_t)t * nbm + b] = make_uint2(end, groups);
 }
 
 // ---------------------------------------------------------------------------
@@ -1564,14 +1477,6 @@ __global__ __launch_bounds__(IX_TPB) void k_starts(const uint32_t* __restrict__ 
     start[b] = lo;
 }
 
-// tmp offsets of the encoder: 2*count+2 words per bitmap (exclusive scan
-// input), computed from the starts
-__global__ __launch_bounds__(IX_TPB) void k_enc_room(const uint32_t* __restrict__ start, uint32_t nbm,
-                                                     uint32_t* __restrict__ room) {
-    const uint32_t b = blockIdx.x * IX_TPB + threadIdx.x;
-    if (b < nbm) room[b] = 2u * (start[b + 1] - start[b]) + 2u;
-}
-
 // ---------------------------------------------------------------------------
 // Host launchers.
 static inline uint32_t cdivu(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
@@ -1643,12 +1548,8 @@ hipError_t launch_unique_first(const uint32_t* flag, const uint32_t* pos, uint32
     return hipGetLastError();
 }
 
-hipError_t launch_starts(const uint32_t* k, uint32_t n, uint32_t nbm, uint32_t* start, uint32_t* room,
-                         hipStream_t s) {
+hipError_t launch_starts(const uint32_t* k, uint32_t n, uint32_t nbm, uint32_t* start, hipStream_t s) {
     hipLaunchKernelGGL(k_starts, dim3(cdivu(nbm + 1, IX_TPB)), dim3(IX_TPB), 0, s, k, n, nbm, start);
-    if (room && nbm)
-        hipLaunchKernelGGL(k_enc_room, dim3(cdivu(nbm, IX_TPB)), dim3(IX_TPB), 0, s,
-                           (const uint32_t*)start, nbm, room);
     return hipGetLastError();
 }
 
@@ -1669,59 +1570,54 @@ hipError_t launch_rs_pass(const uint32_t* ki, const uint32_t* vi, uint32_t n, ui
     return hipGetLastError();
 }
 
-hipError_t launch_encode(int type, const uint32_t* rows, const uint32_t* start, uint32_t nbm,
-                         uint32_t groups, uint32_t anb, const uint64_t* tmpoff, uint32_t* tmp,
-                         uint32_t* nw, uint32_t* aval, const void* col, double* vmin, double* vmax,
-                         hipStream_t s) {
-    dim3 g(cdivu(nbm, IX_TPB)), b(IX_TPB), gh(nbm);
-    if (!nbm) return hipSuccess;
+template <bool WRITE>
+static hipError_t launch_enc_t(const EncArgs& a, int type, const void* col, hipStream_t s) {
+    if (!a.nchunk) return hipSuccess;
+    dim3 g(min(a.nchunk, 1u << 16)), b(IX_TPB);
     if (!col) {
-        hipLaunchKernelGGL(k_encode<float>, g, b, 0, s, rows, start, nbm, groups, anb, tmpoff, tmp, nw,
-                           aval, (const float*)nullptr, (double*)nullptr, (double*)nullptr);
-        hipLaunchKernelGGL(k_encode_heavy<float>, gh, b, 0, s, rows, start, groups, anb, tmpoff, tmp, nw,
-                           aval, (const float*)nullptr, (double*)nullptr, (double*)nullptr);
+        hipLaunchKernelGGL((k_enc_chunks<float, WRITE>), g, b, 0, s, a, (const float*)nullptr);
         return hipGetLastError();
     }
     switch (type) {
-    case COL_I32:
-        hipLaunchKernelGGL(k_encode<int32_t>, g, b, 0, s, rows, start, nbm, groups, anb, tmpoff, tmp, nw, aval, (const int32_t*)col, vmin, vmax);
-        hipLaunchKernelGGL(k_encode_heavy<int32_t>, gh, b, 0, s, rows, start, groups, anb, tmpoff, tmp, nw, aval, (const int32_t*)col, vmin, vmax);
-        break;
-    case COL_U32:
-        hipLaunchKernelGGL(k_encode<uint32_t>, g, b, 0, s, rows, start, nbm, groups, anb, tmpoff, tmp, nw, aval, (const uint32_t*)col, vmin, vmax);
-        hipLaunchKernelGGL(k_encode_heavy<uint32_t>, gh, b, 0, s, rows, start, groups, anb, tmpoff, tmp, nw, aval, (const uint32_t*)col, vmin, vmax);
-        break;
-    case COL_U16:
-        hipLaunchKernelGGL(k_encode<uint16_t>, g, b, 0, s, rows, start, nbm, groups, anb, tmpoff, tmp, nw, aval, (const uint16_t*)col, vmin, vmax);
-        hipLaunchKernelGGL(k_encode_heavy<uint16_t>, gh, b, 0, s, rows, start, groups, anb, tmpoff, tmp, nw, aval, (const uint16_t*)col, vmin, vmax);
-        break;
-    case COL_F32:
-        hipLaunchKernelGGL(k_encode<float>, g, b, 0, s, rows, start, nbm, groups, anb, tmpoff, tmp, nw, aval, (const float*)col, vmin, vmax);
-        hipLaunchKernelGGL(k_encode_heavy<float>, gh, b, 0, s, rows, start, groups, anb, tmpoff, tmp, nw, aval, (const float*)col, vmin, vmax);
-        break;
-    case COL_F64:
-        hipLaunchKernelGGL(k_encode<double>, g, b, 0, s, rows, start, nbm, groups, anb, tmpoff, tmp, nw, aval, (const double*)col, vmin, vmax);
-        hipLaunchKernelGGL(k_encode_heavy<double>, gh, b, 0, s, rows, start, groups, anb, tmpoff, tmp, nw, aval, (const double*)col, vmin, vmax);
-        break;
+    case COL_I32: hipLaunchKernelGGL((k_enc_chunks<int32_t, WRITE>), g, b, 0, s, a, (const int32_t*)col); break;
+    case COL_U32: hipLaunchKernelGGL((k_enc_chunks<uint32_t, WRITE>), g, b, 0, s, a, (const uint32_t*)col); break;
+    case COL_U16: hipLaunchKernelGGL((k_enc_chunks<uint16_t, WRITE>), g, b, 0, s, a, (const uint16_t*)col); break;
+    case COL_F32: hipLaunchKernelGGL((k_enc_chunks<float, WRITE>), g, b, 0, s, a, (const float*)col); break;
+    case COL_F64: hipLaunchKernelGGL((k_enc_chunks<double, WRITE>), g, b, 0, s, a, (const double*)col); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_concat(const uint32_t* tmp, const uint64_t* tmpoff, const uint32_t* nw,
-                         const uint32_t* aval, const uint64_t* off, const uint8_t* unpack,
-                         uint32_t nbm, uint32_t groups, uint32_t anb, uint32_t* out, hipStream_t s) {
-    if (!nbm) return hipSuccess;
-    hipLaunchKernelGGL(k_concat, dim3(nbm), dim3(IX_TPB), 0, s, tmp, tmpoff, nw, aval, off, unpack, nbm,
-                       groups, anb, out);
+hipError_t launch_enc_count(const EncArgs& a, int type, const void* col, hipStream_t s) {
+    return launch_enc_t<false>(a, type, col, s);
+}
+
+hipError_t launch_enc_write(const EncArgs& a, hipStream_t s) {
+    return launch_enc_t<true>(a, COL_F32, nullptr, s);
+}
+
+hipError_t launch_toff_len(const uint32_t* words, const uint64_t* off, const uint32_t* nw, const uint32_t* tb,
+                           const uint32_t* tw0, uint32_t nch, uint32_t* clen, hipStream_t s) {
+    if (!nch) return hipSuccess;
+    hipLaunchKernelGGL(k_toff_len, dim3(min(nch, 1u << 16)), dim3(IX_TPB), 0, s, words, off, nw, tb, tw0, nch, clen);
     return hipGetLastError();
 }
 
-hipError_t launch_toff(const uint32_t* words, const uint64_t* off, const uint32_t* nw, uint32_t nbm,
-                       uint32_t ntiles, uint32_t tg, uint32_t groups, uint2* toff, hipStream_t s) {
+hipError_t launch_toff_rows(const uint32_t* words, const uint64_t* off, const uint32_t* nw, const uint32_t* tb,
+                            const uint32_t* tw0, const uint32_t* cg0, uint32_t nch, uint32_t nbm,
+                            uint32_t ntiles, uint32_t tg, uint2* toff, hipStream_t s) {
+    if (!nch) return hipSuccess;
+    hipLaunchKernelGGL(k_toff_rows, dim3(min(nch, 1u << 16)), dim3(IX_TPB), 0, s, words, off, nw, tb, tw0, cg0,
+                       nch, nbm, ntiles, tg, toff);
+    return hipGetLastError();
+}
+
+hipError_t launch_toff_tail(const uint64_t* off, const uint32_t* nw, const uint32_t* total, uint32_t nbm,
+                            uint32_t ntiles, uint32_t tg, uint32_t groups, uint2* toff, hipStream_t s) {
     if (!nbm) return hipSuccess;
-    hipLaunchKernelGGL(k_toff, dim3(cdivu(nbm, IX_TPB / WAVE)), dim3(IX_TPB), 0, s, words, off, nw, nbm,
-                       ntiles, tg, groups, toff);
+    hipLaunchKernelGGL(k_toff_tail, dim3(cdivu(nbm, IX_TPB)), dim3(IX_TPB), 0, s, off, nw, total, nbm, ntiles,
+                       tg, groups, toff);
     return hipGetLastError();
 }
 
