@@ -1142,7 +1142,7 @@ __device__ __forceinline__ void tile_epilogue(const ConjArgs& a, uint32_t qi, ui
 }
 
 template <int NT>
-__global__ __launch_bounds__(NT) void k_conj(ConjArgs a) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_conj(ConjArgs a) {
     __shared__ word_t R[QT + 1], R2[QT + 1];
     __shared__ uint32_t cov[QT + 1];
     __shared__ uint32_t red[NT / WAVE + 1];
@@ -1334,7 +1334,7 @@ __device__ __forceinline__ void expr_body(const ConjArgs& a) {
 }
 
 template <int NT, int D>
-__global__ __launch_bounds__(NT) void k_expr(ConjArgs a) { expr_body<NT, D, false>(a); }
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_expr(ConjArgs a) { expr_body<NT, D, false>(a); }
 
 // Programs of literal terms with no edge bitmaps (relic, fuzz, zona): the
 // scan and edge-check code left out, registers held to 6 waves per SIMD.
