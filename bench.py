@@ -542,8 +542,8 @@ def main():
     ap.add_argument("--epochs", type=int, default=40)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this benchmark runs on one GPU; multi-GPU sharding is a design "
-                                   "note only (see DESIGN.md)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this benchmark runs on one GPU; sharding over several GPUs "
+                                   "is designed but not built", "n_gpus": args.gpus}))
         sys.exit(2)
 
     import libfastlib_amd as L

@@ -1,4 +1,5 @@
-"""The gpu_bitvector shim that INTEGRATION.md shows a FastBit maintainer
+"""The gpu_bitvector shim of the integration notes (the first C++ block
+under "Bitvector operators", which a FastBit maintainer would add)
 compiles against the reference's own headers (ibis::bitvector,
 ibis::array_t) and links against the reference library and
 libfastbit_hip.so.  CPU only; needs the reference sources, so it is
