@@ -1160,6 +1160,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
             const bool last = tile + 1 == a.ntiles;
             const uint32_t g0 = tile * (uint32_t)QT;
             const uint32_t ng = last ? (a.groups - g0) + (a.anb ? 1u : 0u) : (uint32_t)QT;
+            bool empty = false;
             for (uint32_t k = k0; k < k1; ++k) {
                 const QueryArgs q = a.terms[k];        // in registers (see k_range)
                 term_tile<NT>(q, tile, g0, ng, last, k == k0 ? R : R2, cov, red, bal, ctab);
@@ -1174,11 +1175,17 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
                     for (uint32_t i = threadIdx.x; i < ng; i += NT) any |= R[i] != 0u;
                 }
                 if (!__syncthreads_or(any)) {            // the tile is empty: skip the rest
-                    for (uint32_t i = threadIdx.x; i < ng; i += NT) R[i] = 0u;
-                    __syncthreads();
+                    empty = true;
                     break;
                 }
             }
+            if (empty && !a.out) {
+                // no ones: the count is zero and the row-id pass has nothing
+                // to write (the vote's barrier ended every read of R)
+                if (!a.rids && threadIdx.x == 0) a.part[(size_t)qi * a.ntiles + tile] = 0u;
+                continue;
+            }
+            // (an empty tile written out: R[0, ng) is all zero, as the vote found)
             tile_epilogue<NT>(a, qi, tile, g0, ng, R, red);
         }
     }
@@ -1286,12 +1293,17 @@ __device__ __forceinline__ void expr_body(const ConjArgs& a) {
             const uint32_t g0 = tile * (uint32_t)QT;
             const uint32_t ng = last ? (a.groups - g0) + (a.anb ? 1u : 0u) : (uint32_t)QT;
             int sp = 0;
+            bool empty = false;
             for (uint32_t p = p0; p < p1; ++p) {
                 const int32_t sk = a.skip[p];
                 if (sk >= 0) {
                     int any = 0;
                     for (uint32_t i = threadIdx.x; i < ng; i += NT) any |= S[sp - 1][i] != 0u;
                     if (!__syncthreads_or(any)) {
+                        if ((uint32_t)sk == p1 && sp == 1) {   // the empty operand is the result
+                            empty = true;
+                            break;
+                        }
                         p = (uint32_t)sk - 1u;          // the loop's ++p resumes at sk
                         continue;
                     }
@@ -1327,6 +1339,12 @@ __device__ __forceinline__ void expr_body(const ConjArgs& a) {
                     --sp;
                 }
                 __syncthreads();
+            }
+            if (empty && !a.out) {
+                // no ones: the count is zero and the row-id pass has nothing
+                // to write (the vote's barrier ended every read of S)
+                if (!a.rids && threadIdx.x == 0) a.part[(size_t)qi * a.ntiles + tile] = 0u;
+                continue;
             }
             tile_epilogue<NT>(a, qi, tile, g0, ng, S[0], red);
         }
