@@ -1160,25 +1160,47 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void k_
             const bool last = tile + 1 == a.ntiles;
             const uint32_t g0 = tile * (uint32_t)QT;
             const uint32_t ng = last ? (a.groups - g0) + (a.anb ? 1u : 0u) : (uint32_t)QT;
-            bool empty = false;
+            bool empty = false, counted = false;
             for (uint32_t k = k0; k < k1; ++k) {
                 const QueryArgs q = a.terms[k];        // in registers (see k_range)
                 term_tile<NT>(q, tile, g0, ng, last, k == k0 ? R : R2, cov, red, bal, ctab);
+                // the last term of a count pass counts the ones as it ANDs
+                const bool fuse = k + 1 == k1 && !a.rids && !a.out;
                 int any = 0;
+                uint32_t ones = 0;
                 if (k != k0) {
                     for (uint32_t i = threadIdx.x; i < ng; i += NT) {
                         const word_t v = R[i] & R2[i];
                         R[i] = v;
                         any |= v != 0u;
+                        ones += (uint32_t)__popc(v);
                     }
                 } else {
-                    for (uint32_t i = threadIdx.x; i < ng; i += NT) any |= R[i] != 0u;
+                    for (uint32_t i = threadIdx.x; i < ng; i += NT) {
+                        any |= R[i] != 0u;
+                        ones += (uint32_t)__popc(R[i]);
+                    }
+                }
+                if (fuse) {
+                    // red is next written after a barrier that thread 0 passes
+                    // only once it has read it
+                    ones = wave_sum(ones);
+                    if (lane_id() == 0) red[threadIdx.x / WAVE] = ones;
+                    __syncthreads();
+                    if (threadIdx.x == 0) {
+                        uint32_t t = 0;
+                        for (int i = 0; i < NT / WAVE; ++i) t += red[i];
+                        a.part[(size_t)qi * a.ntiles + tile] = t;
+                    }
+                    counted = true;
+                    break;
                 }
                 if (!__syncthreads_or(any)) {            // the tile is empty: skip the rest
                     empty = true;
                     break;
                 }
             }
+            if (counted) continue;
             if (empty && !a.out) {
                 // no ones: the count is zero and the row-id pass has nothing
                 // to write (the vote's barrier ended every read of R)
