@@ -368,6 +368,68 @@ def test_conjunction_rids_and_hit_vectors(L, orc, plan):
             assert np.array_equal(batch.result(qi).write(), orc.from_bits(bits)), qi
 
 
+def _term_rows(col, lo, lop, hi, rop):
+    x = col.astype(np.float64)
+    m = np.ones(col.size, bool)
+    if lop == 1:
+        m &= lo < x
+    elif lop == 2:
+        m &= lo <= x
+    if rop == 1:
+        m &= x < hi
+    elif rop == 2:
+        m &= x <= hi
+    return m
+
+
+@pytest.mark.parametrize("plan", [0, 1, 3])
+def test_conjunction_one_term_and_empty_tiles(L, orc, plan):
+    """Queries of one term (counted in the term's own pass), queries with
+    no hit (every tile empty after some term), and tiles emptied only by
+    the last term, over a clustered column (narrow ranges touch few of the
+    five tiles): counts, row ids and hit vectors against the column, before
+    and after tuning, and the same terms as AND programs."""
+    n = 31 * 2048 * 4 + 31 * 100 + 7
+    rng = np.random.default_rng(41)
+    a = np.sort(rng.integers(0, 1000, n)).astype(np.int32)
+    b = rng.integers(0, 500, n).astype(np.int32)
+    ia, ib = L.Index.relic(a), L.Index.relic(b)
+    cols = {id(ia): a, id(ib): b}
+    qs = [
+        [(ia, 10.0, 2, 20.0, 1)],
+        [(ib, 0.0, 2, 250.0, 1)],
+        [(ia, 100.0, 2, 110.0, 1), (ia, 500.0, 2, 510.0, 1)],
+        [(ib, 0.0, 2, 500.0, 1), (ia, 300.0, 2, 330.0, 1)],
+        [(ia, 2000.0, 2, 3000.0, 1)],
+        [(ib, 10.0, 1, 20.0, 2), (ib, 15.0, 2, 400.0, 1), (ia, 0.0, 2, 999.0, 2)],
+        [(ia, 0.0, 2, 999.0, 2), (ib, 7.0, 2, 7.0, 2)],
+    ]
+    want = []
+    for q in qs:
+        m = np.ones(n, bool)
+        for (ix, lo, lop, hi, rop) in q:
+            m &= _term_rows(cols[id(ix)], lo, lop, hi, rop)
+        want.append(np.nonzero(m)[0].astype(np.uint32))
+    assert want[2].size == 0 and want[4].size == 0 and want[3].size > 0
+    print("conjunction shapes", plan, flush=True)
+    batch = L.Conjunctions(qs, plan)
+    hits, _ = batch.run()
+    assert [int(h) for h in hits] == [w.size for w in want]
+    off, rids = batch.rids()
+    for qi, w in enumerate(want):
+        assert np.array_equal(rids[off[qi]:off[qi + 1]], w), qi
+        bits = np.zeros(n, np.uint8)
+        bits[w] = 1
+        assert np.array_equal(batch.result(qi).write(), orc.from_bits(bits)), qi
+    batch.tune()
+    assert np.array_equal(batch.run()[0], hits)
+    progs = [[0] if len(q) == 1 else [0] + sum(([k, -1] for k in range(1, len(q))), []) for q in qs]
+    E = L.Expressions([(q, p) for q, p in zip(qs, progs)], plan)
+    assert np.array_equal(E.run()[0], hits)
+    eoff, erids = E.rids()
+    assert np.array_equal(eoff, off) and np.array_equal(erids, rids)
+
+
 @pytest.mark.parametrize("n", [31 * 4096, 31 * 4096 + 5, 31 * 2048 * 3 + 31 * 9 + 29, 1000])
 def test_compressed_plan_words(L, orc, n):
     """The compressed plan (OR merged per tile, canonical WAH written with
